@@ -1,0 +1,12 @@
+"""cpsz_amd: critical-point-preserving compression of 2D/3D float vector
+fields on AMD Instinct MI355X (gfx950), stream-compatible with bzhan166/cpSZ.
+
+The compute path is the HIP library ``libcpsz_hip.so`` (C ABI in
+``include/cpsz.h``); this package is its Python binding.
+"""
+from .api import (CpszError, compress_2d_dev, compress_3d_dev, critical_points_2d_dev,  # noqa: F401
+                  critical_points_3d_dev, decompress_2d_dev, decompress_3d_dev, device_count,
+                  last_stages, lib, set_timing, stream_size_2d,
+                  sz_compress_cp_preserve_2d_offline, sz_compress_cp_preserve_3d_offline_log,
+                  sz_decompress_cp_preserve_2d_offline, sz_decompress_cp_preserve_3d_online_log,
+                  sz_lossless_compress, sz_lossless_decompress)

@@ -1,0 +1,225 @@
+"""ctypes binding of libcpsz_hip.so, mirroring the reference's C++ API names.
+
+Host-buffer functions take/return numpy arrays; the ``*_dev`` functions take
+raw device pointers (e.g. ``torch.Tensor.data_ptr()``) and an optional HIP
+stream handle. Any failure raises :class:`CpszError` carrying the library's
+status code and message. There is no CPU fallback: if the HIP library is not
+built, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpsz_hip.so")
+
+_P = C.c_void_p
+_SZ = C.c_size_t
+
+# name -> argtypes (all return int)
+_SIGS = {
+    "cpsz_compress_2d_f32": [_P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_2d_f32": [_P, _SZ, _SZ, _SZ, _P, _P],
+    "cpsz_compress_3d_f32": [_P, _P, _P, _SZ, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_3d_f32": [_P, _SZ, _SZ, _SZ, _SZ, _P, _P, _P],
+    "cpsz_stream_size_2d": [_P, C.POINTER(_SZ)],
+    "cpsz_stream_size_3d": [_P, _SZ, _SZ, _SZ, C.POINTER(_SZ)],
+    "cpsz_compress_2d_f32_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_2d_f32_dev": [C.c_int, _P, _P, _SZ, _SZ, _SZ, _P, _P],
+    "cpsz_compress_3d_f32_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, C.POINTER(_P),
+                                 C.POINTER(_SZ)],
+    "cpsz_decompress_3d_f32_dev": [C.c_int, _P, _P, _SZ, _SZ, _SZ, _SZ, _P, _P, _P],
+    "cpsz_lossless_compress": [C.c_int, _P, _SZ, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_lossless_decompress": [_P, _SZ, _P, _SZ],
+    "cpsz_critical_points_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_critical_points_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ,
+                                    C.POINTER(_SZ)],
+    "cpsz_set_timing": [C.c_int, C.c_int],
+    "cpsz_last_stages": [C.c_int, C.c_char_p, _SZ, _P, C.c_int],
+    "cpsz_device_count": [],
+}
+
+
+class CpszError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cpsz error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CpszError(-5, f"{LIB_PATH} is missing; build it with __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = C.c_int
+        L.cpsz_last_error.restype = C.c_char_p
+        L.cpsz_last_error.argtypes = []
+        L.cpsz_version.restype = C.c_char_p
+        L.cpsz_free.argtypes = [_P]
+        L.cpsz_free.restype = None
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc < 0:
+        raise CpszError(rc, lib().cpsz_last_error().decode())
+    return rc
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a)
+    if a.dtype != np.float32:
+        raise TypeError("cpsz works on float32 fields")
+    return a
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+def _take(p, n):
+    out = np.frombuffer(C.string_at(p, n), dtype=np.uint8).copy()
+    lib().cpsz_free(p)
+    return out
+
+
+# ------------------------------------------------------------- host API ----
+def sz_compress_cp_preserve_2d_offline(U, V, max_pwr_eb: float = 0.1) -> np.ndarray:
+    """Compresses row-major r1 x r2 float32 fields U, V; returns the stream."""
+    U, V = _f32(U), _f32(V)
+    if U.shape != V.shape or U.ndim != 2:
+        raise ValueError("U and V must be 2D arrays of the same shape")
+    r1, r2 = U.shape
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_compress_2d_f32(_ptr(U), _ptr(V), r1, r2, max_pwr_eb, C.byref(p), C.byref(s)))
+    return _take(p, s.value)
+
+
+def sz_decompress_cp_preserve_2d_offline(stream, r1: int, r2: int):
+    stream = np.ascontiguousarray(np.frombuffer(bytes(stream), np.uint8) if isinstance(stream, (bytes, bytearray)) else stream, np.uint8)
+    U = np.empty((r1, r2), np.float32)
+    V = np.empty((r1, r2), np.float32)
+    _check(lib().cpsz_decompress_2d_f32(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)))
+    return U, V
+
+
+def sz_compress_cp_preserve_3d_offline_log(U, V, W, max_pwr_eb: float = 0.1) -> np.ndarray:
+    U, V, W = _f32(U), _f32(V), _f32(W)
+    if not (U.shape == V.shape == W.shape) or U.ndim != 3:
+        raise ValueError("U, V, W must be 3D arrays of the same shape")
+    r1, r2, r3 = U.shape
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_compress_3d_f32(_ptr(U), _ptr(V), _ptr(W), r1, r2, r3, max_pwr_eb, C.byref(p),
+                                      C.byref(s)))
+    return _take(p, s.value)
+
+
+def sz_decompress_cp_preserve_3d_online_log(stream, r1: int, r2: int, r3: int):
+    stream = np.ascontiguousarray(stream, np.uint8)
+    out = [np.empty((r1, r2, r3), np.float32) for _ in range(3)]
+    _check(lib().cpsz_decompress_3d_f32(_ptr(stream), stream.size, r1, r2, r3,
+                                        *[_ptr(o) for o in out]))
+    return tuple(out)
+
+
+def sz_lossless_compress(data, level: int = 3) -> np.ndarray:
+    data = np.ascontiguousarray(data, np.uint8)
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_lossless_compress(level, _ptr(data), data.size, C.byref(p), C.byref(s)))
+    return _take(p, s.value)
+
+
+def sz_lossless_decompress(data, size: int) -> np.ndarray:
+    data = np.ascontiguousarray(data, np.uint8)
+    out = np.empty(size, np.uint8)
+    _check(lib().cpsz_lossless_decompress(_ptr(data), data.size, _ptr(out), size))
+    return out
+
+
+def stream_size_2d(stream) -> int:
+    s = _SZ()
+    _check(lib().cpsz_stream_size_2d(_ptr(np.ascontiguousarray(stream, np.uint8)), C.byref(s)))
+    return s.value
+
+
+# ----------------------------------------------------------- device API ----
+def compress_2d_dev(dU: int, dV: int, r1: int, r2: int, max_pwr_eb: float = 0.1, device: int = 0,
+                    stream: int = 0):
+    """Returns (device pointer, size) of the stream in a library-owned buffer."""
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_compress_2d_f32_dev(device, stream or None, dU, dV, r1, r2, max_pwr_eb,
+                                          C.byref(p), C.byref(s)))
+    return p.value, s.value
+
+
+def decompress_2d_dev(d_in: int, in_size: int, r1: int, r2: int, dU: int, dV: int, device: int = 0,
+                      stream: int = 0):
+    _check(lib().cpsz_decompress_2d_f32_dev(device, stream or None, d_in, in_size, r1, r2, dU, dV))
+
+
+def compress_3d_dev(dU: int, dV: int, dW: int, r1: int, r2: int, r3: int, max_pwr_eb: float = 0.1,
+                    device: int = 0, stream: int = 0):
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_compress_3d_f32_dev(device, stream or None, dU, dV, dW, r1, r2, r3, max_pwr_eb,
+                                          C.byref(p), C.byref(s)))
+    return p.value, s.value
+
+
+def decompress_3d_dev(d_in: int, in_size: int, r1: int, r2: int, r3: int, dU: int, dV: int, dW: int,
+                      device: int = 0, stream: int = 0):
+    _check(lib().cpsz_decompress_3d_f32_dev(device, stream or None, d_in, in_size, r1, r2, r3, dU, dV,
+                                            dW))
+
+
+def critical_points_2d_dev(dU: int, dV: int, r1: int, r2: int, device: int = 0, stream: int = 0,
+                           cap: int = 1 << 20):
+    ids = np.empty(cap, np.int64)
+    types = np.empty(cap, np.int32)
+    cnt = _SZ()
+    _check(lib().cpsz_critical_points_2d_dev(device, stream or None, dU, dV, r1, r2, _ptr(ids),
+                                             _ptr(types), cap, C.byref(cnt)))
+    n = min(cnt.value, cap)
+    if cnt.value > cap:
+        raise CpszError(-3, f"{cnt.value} critical points exceed cap {cap}")
+    return dict(zip(ids[:n].tolist(), types[:n].tolist()))
+
+
+def critical_points_3d_dev(dU: int, dV: int, dW: int, r1: int, r2: int, r3: int, device: int = 0,
+                           stream: int = 0, cap: int = 1 << 22):
+    ids = np.empty(cap, np.int64)
+    types = np.empty(cap, np.int32)
+    cnt = _SZ()
+    _check(lib().cpsz_critical_points_3d_dev(device, stream or None, dU, dV, dW, r1, r2, r3,
+                                             _ptr(ids), _ptr(types), cap, C.byref(cnt)))
+    if cnt.value > cap:
+        raise CpszError(-3, f"{cnt.value} critical points exceed cap {cap}")
+    n = cnt.value
+    return dict(zip(ids[:n].tolist(), types[:n].tolist()))
+
+
+def set_timing(on: bool = True, device: int = 0):
+    _check(lib().cpsz_set_timing(device, int(on)))
+
+
+def last_stages(device: int = 0):
+    names = C.create_string_buffer(4096)
+    ms = (C.c_float * 64)()
+    n = _check(lib().cpsz_last_stages(device, names, 4096, C.cast(ms, _P), 64))
+    keys = names.value.decode().split(";") if n else []
+    return list(zip(keys, [ms[i] for i in range(n)]))
+
+
+def device_count() -> int:
+    return lib().cpsz_device_count()

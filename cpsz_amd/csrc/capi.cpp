@@ -1,0 +1,359 @@
+// extern "C" boundary of libcpsz_hip.so (declared in include/cpsz.h).
+#include <zstd.h>
+
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+
+#include "../../include/cpsz.h"
+#include "codec2d.hpp"
+#include "codec3d.hpp"
+#include "cp.hpp"
+#include "ctx.hpp"
+#include "huff_host.hpp"
+
+namespace cpsz {
+namespace {
+
+thread_local std::string g_err;
+bool g_exiting = false;
+
+struct ThreadCtxs {
+  std::map<int, Ctx*> m;
+  ~ThreadCtxs() {
+    if (g_exiting) return;  // the HIP runtime may already be gone at process exit
+    for (auto& kv : m) delete kv.second;
+  }
+};
+thread_local ThreadCtxs g_ctxs;
+
+Ctx& ctx_of(int device) {
+  static bool reg = (std::atexit([] { g_exiting = true; }), true);
+  (void)reg;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    fail(kErrHip, "no HIP device available");
+  CPSZ_REQUIRE(device >= 0 && device < count, kErrArg, "device index out of range");
+  Ctx*& c = g_ctxs.m[device];
+  if (!c) c = new Ctx(device);
+  CPSZ_HIP(hipSetDevice(device));
+  return *c;
+}
+
+int current_device() {
+  int d = 0;
+  if (hipGetDevice(&d) != hipSuccess) d = 0;
+  return d;
+}
+
+template <class F>
+int guarded(F f) {
+  try {
+    f();
+    g_err.clear();
+    return kOk;
+  } catch (const Error& e) {
+    g_err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_err = "host allocation failed";
+    return kErrAlloc;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return kErrInternal;
+  }
+}
+
+struct StreamScope {  // temporarily routes a context onto a caller stream
+  Ctx& c;
+  StreamScope(Ctx& ctx, void* s) : c(ctx) { c.set_stream((hipStream_t)s); }
+  ~StreamScope() { c.set_stream(nullptr); }
+};
+
+// Device copies of host buffers, owned for one call.
+struct DevMem {
+  void* p = nullptr;
+  explicit DevMem(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) {
+      p = nullptr;
+      fail(kErrAlloc, "device allocation failed");
+    }
+  }
+  ~DevMem() {
+    if (p) hipFree(p);
+  }
+};
+
+}  // namespace
+}  // namespace cpsz
+
+using namespace cpsz;
+
+extern "C" {
+
+const char* cpsz_last_error(void) { return g_err.c_str(); }
+const char* cpsz_version(void) { return "cpsz-mi355x 0.1 (gfx950)"; }
+int cpsz_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+void cpsz_free(void* p) { std::free(p); }
+
+int cpsz_compress_2d_f32(const float* U, const float* V, size_t r1, size_t r2, double max_pwr_eb,
+                         unsigned char** out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(U && V && out && out_size && r1 && r2, kErrArg, "null argument or empty field");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    float* d = c.buf<float>("api.in", 2 * n);
+    CPSZ_HIP(hipMemcpyAsync(d, U, n * 4, hipMemcpyHostToDevice, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(d + n, V, n * 4, hipMemcpyHostToDevice, c.stream()));
+    DeviceStream s;
+    compress_2d_device(c, d, d + n, r1, r2, max_pwr_eb, s);
+    unsigned char* h = (unsigned char*)std::malloc(s.size ? s.size : 1);
+    CPSZ_REQUIRE(h, kErrAlloc, "host allocation failed");
+    CPSZ_HIP(hipMemcpyAsync(h, s.data, s.size, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+    *out = h;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_2d_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2, float* U,
+                           float* V) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && U && V && r1 && r2 && in_size >= 36, kErrArg, "bad arguments");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    uint8_t* din = c.buf<uint8_t>("api.stream", in_size + 16);
+    CPSZ_HIP(hipMemcpyAsync(din, in, in_size, hipMemcpyHostToDevice, c.stream()));
+    float* d = c.buf<float>("api.out", 2 * n);
+    decompress_2d_device(c, din, in, in_size, r1, r2, d, d + n);
+    CPSZ_HIP(hipMemcpyAsync(U, d, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(V, d + n, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_compress_3d_f32(const float* U, const float* V, const float* W, size_t r1, size_t r2,
+                         size_t r3, double max_pwr_eb, unsigned char** out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(U && V && W && out && out_size && r1 && r2 && r3, kErrArg, "bad arguments");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2 * r3;
+    float* d = c.buf<float>("api.in", 3 * n);
+    CPSZ_HIP(hipMemcpyAsync(d, U, n * 4, hipMemcpyHostToDevice, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(d + n, V, n * 4, hipMemcpyHostToDevice, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(d + 2 * n, W, n * 4, hipMemcpyHostToDevice, c.stream()));
+    DeviceStream s;
+    compress_3d_device(c, d, d + n, d + 2 * n, r1, r2, r3, max_pwr_eb, s);
+    unsigned char* h = (unsigned char*)std::malloc(s.size ? s.size : 1);
+    CPSZ_REQUIRE(h, kErrAlloc, "host allocation failed");
+    CPSZ_HIP(hipMemcpyAsync(h, s.data, s.size, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+    *out = h;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_3d_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2,
+                           size_t r3, float* U, float* V, float* W) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && U && V && W && r1 && r2 && r3 && in_size >= 8, kErrArg, "bad arguments");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2 * r3;
+    uint8_t* din = c.buf<uint8_t>("api.stream", in_size + 16);
+    CPSZ_HIP(hipMemcpyAsync(din, in, in_size, hipMemcpyHostToDevice, c.stream()));
+    float* d = c.buf<float>("api.out", 3 * n);
+    decompress_3d_device(c, din, in, in_size, r1, r2, r3, d, d + n, d + 2 * n);
+    CPSZ_HIP(hipMemcpyAsync(U, d, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(V, d + n, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(W, d + 2 * n, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_stream_size_2d(const unsigned char* in, size_t* size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && size, kErrArg, "null argument");
+    size_t eb, u, v;
+    std::memcpy(&eb, in + 12, 8);
+    std::memcpy(&u, in + 20, 8);
+    std::memcpy(&v, in + 28, 8);
+    *size = 36 + eb + u + v;
+  });
+}
+
+int cpsz_stream_size_3d(const unsigned char* in, size_t r1, size_t r2, size_t r3, size_t* size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && size && r1 && r2 && r3, kErrArg, "bad arguments");
+    const size_t n = r1 * r2 * r3;
+    size_t off = 8 + 3 * ((n - 1) / 8 + 1);
+    size_t unp;
+    std::memcpy(&unp, in + off, 8);
+    off += 8 + 4 * unp + 8;  // unpredictable values, eb_quant_num
+    for (int s = 0; s < 2; s++) {
+      size_t nodes, nbytes;
+      uint32_t tree;
+      std::memcpy(&nodes, in + off, 8);
+      std::memcpy(&tree, in + off + 8, 4);
+      std::memcpy(&nbytes, in + off + 12 + tree, 8);
+      off += 20 + tree + nbytes;
+      if (s == 0) off += 8;  // data_quant_num
+      (void)nodes;
+    }
+    *size = off;
+  });
+}
+
+int cpsz_compress_2d_f32_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                             size_t r2, double max_pwr_eb, const unsigned char** d_out,
+                             size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_out && out_size, kErrArg, "null output argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    DeviceStream s;
+    compress_2d_device(c, dU, dV, r1, r2, max_pwr_eb, s);
+    *d_out = s.data;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_2d_f32_dev(int device, void* stream, const unsigned char* d_in, size_t in_size,
+                               size_t r1, size_t r2, float* dU, float* dV) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    decompress_2d_device(c, d_in, nullptr, in_size, r1, r2, dU, dV);
+  });
+}
+
+int cpsz_compress_3d_f32_dev(int device, void* stream, const float* dU, const float* dV,
+                             const float* dW, size_t r1, size_t r2, size_t r3, double max_pwr_eb,
+                             const unsigned char** d_out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_out && out_size, kErrArg, "null output argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    DeviceStream s;
+    compress_3d_device(c, dU, dV, dW, r1, r2, r3, max_pwr_eb, s);
+    *d_out = s.data;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_3d_f32_dev(int device, void* stream, const unsigned char* d_in, size_t in_size,
+                               size_t r1, size_t r2, size_t r3, float* dU, float* dV, float* dW) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    decompress_3d_device(c, d_in, nullptr, in_size, r1, r2, r3, dU, dV, dW);
+  });
+}
+
+int cpsz_lossless_compress(int level, const unsigned char* in, size_t n, unsigned char** out,
+                           size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && out && out_size, kErrArg, "null argument");
+    // The reference sizes its buffer as dataLength*1.2 (200 below 100 bytes),
+    // src/sz_lossless.cpp:18-23; ZSTD_compressBound is never smaller.
+    const size_t cap = ZSTD_compressBound(n);
+    unsigned char* o = (unsigned char*)std::malloc(cap);
+    CPSZ_REQUIRE(o, kErrAlloc, "host allocation failed");
+    const size_t r = ZSTD_compress(o, cap, in, n, level);
+    if (ZSTD_isError(r)) {
+      std::free(o);
+      fail(kErrInternal, std::string("ZSTD_compress: ") + ZSTD_getErrorName(r));
+    }
+    *out = o;
+    *out_size = r;
+  });
+}
+
+int cpsz_lossless_decompress(const unsigned char* in, size_t n, unsigned char* out, size_t out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && out, kErrArg, "null argument");
+    const size_t r = ZSTD_decompress(out, out_size, in, n);
+    CPSZ_REQUIRE(!ZSTD_isError(r), kErrStream, std::string("ZSTD_decompress: ") + ZSTD_getErrorName(r));
+    CPSZ_REQUIRE(r == out_size, kErrStream, "ZSTD frame size differs from the expected size");
+  });
+}
+
+int cpsz_critical_points_2d_dev(int device, void* stream, const float* dU, const float* dV,
+                                size_t r1, size_t r2, int64_t* ids, int32_t* types, size_t cap,
+                                size_t* count) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    critical_points_2d(c, dU, dV, r1, r2, ids, types, cap, count);
+  });
+}
+
+int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const float* dV,
+                                const float* dW, size_t r1, size_t r2, size_t r3, int64_t* ids,
+                                int32_t* types, size_t cap, size_t* count) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    critical_points_3d(c, dU, dV, dW, r1, r2, r3, ids, types, cap, count);
+  });
+}
+
+int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int* n3d) {
+  return guarded([&] {
+    const EbTables& t = eb_tables();
+    for (int i = 0; i < (int)t.q2d.size() && i < cap2d; i++) q2d[i] = t.q2d[i];
+    for (int i = 0; i < (int)t.q3d.size() && i < cap3d; i++) q3d[i] = t.q3d[i];
+    *n2d = (int)t.q2d.size();
+    *n3d = (int)t.q3d.size();
+  });
+}
+
+int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
+                      size_t cap, size_t* tree_size, size_t* node_count) {
+  return guarded([&] {
+    HuffEncodeTable t;
+    huff_build(freq, nbins, state_num, t);
+    CPSZ_REQUIRE(t.tree.size() <= cap, kErrAlloc, "tree buffer too small");
+    std::memcpy(tree, t.tree.data(), t.tree.size());
+    *tree_size = t.tree.size();
+    *node_count = t.node_count;
+  });
+}
+
+int cpsz_lorenzo2d_emulate(const float* x, const uint16_t* c, size_t r1, size_t r2, int wide,
+                           uint16_t* q, float* recon) {
+  return guarded([&] {
+    CPSZ_REQUIRE(x && c && q && r1 && r2, kErrArg, "bad arguments");
+    lorenzo2d_emulate(x, c, r1, r2, wide != 0, q, recon);
+  });
+}
+
+int cpsz_set_timing(int device, int on) {
+  return guarded([&] { ctx_of(device).timing = on != 0; });
+}
+
+int cpsz_last_stages(int device, char* names, size_t names_cap, float* ms, int cap) {
+  int count = 0;
+  int rc = guarded([&] {
+    Ctx& c = ctx_of(device);
+    std::string joined;
+    for (const auto& s : c.stages) {
+      if (count < cap) ms[count] = s.second;
+      if (!joined.empty()) joined += ";";
+      joined += s.first;
+      count++;
+    }
+    if (names && names_cap) {
+      std::strncpy(names, joined.c_str(), names_cap - 1);
+      names[names_cap - 1] = 0;
+    }
+  });
+  return rc ? rc : count;
+}
+
+}  // extern "C"

@@ -1,0 +1,910 @@
+// 2D critical-point-preserving codec on gfx950 (the drop-in for
+// sz_compress_cp_preserve_2d_offline / sz_decompress_cp_preserve_2d_offline).
+//
+// Device pipeline (compress), all inputs resident in HBM:
+//   k_eb2d        per-cell triangle bounds in LDS (8x64 vertex tiles + 1-cell
+//                 halo), ordered per-vertex min, |U|,|V| scaling and the base-4
+//                 eb id -> eb code per vertex and component, id histogram
+//   k_lorenzo2d_c strip-pipelined wavefront Lorenzo + linear quantization:
+//                 one wave = 64 rows (lane = row), the row above arrives through
+//                 DPP wave_shr:1; strips hand their last row to the next strip
+//                 through agent-coherent boundary rows with progress counters
+//   Huffman       histogram -> host tree (reference tie-break) -> encode
+//   unpredictable values compacted in 12x12 block-major order
+// Decompress mirrors it: parallel Huffman decode of ids and codes, block-major
+// unpredictable scatter, wavefront reconstruction.
+//
+// Bit-exactness: every float/double operation follows the reference's types
+// and order (src/sz_compress_cp_preserve_2d.cpp:17-172, quantize in
+// include/sz_compression_utils.hpp:50-65); the Lorenzo step runs in FP32, which
+// reproduces the reference's double arithmetic exactly (see DESIGN.md notes in
+// the function comments below).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "codec2d.hpp"
+#include "huff_gpu.hpp"
+#include "lorenzo2d_step.hpp"
+
+namespace cpsz {
+
+// ----------------------------------------------------------------- eb bound --
+namespace {
+
+__device__ __forceinline__ double dminf(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double dmaxm(double a, double b) { return (a > b) ? a : b; }
+
+template <typename V>
+__device__ __forceinline__ void dacc(V v, double& p, double& n) {
+  if (v >= 0) p += v;
+  else n += -v;
+}
+
+// max_eb_to_keep_sign (include/sz_cp_preserve_utils.hpp:59-90), degrees 1, 2
+__device__ __forceinline__ double dsign_bound(double p, double n, int deg) {
+  if (n == 0 || p == 0) return 1;
+  double P = p, N = n;
+  if (deg == 2) {
+    P = sqrt(p);
+    N = sqrt(n);
+  }
+  return fabs(P - N) / (P + N);
+}
+
+__device__ __forceinline__ double dbound2(double a, double b) {
+  double p = 0, n = 0;
+  dacc(a, p, n);
+  dacc(b, p, n);
+  return dsign_bound(p, n, 2);
+}
+__device__ __forceinline__ double dbound4(double a, double b, double c, double d) {
+  double p = 0, n = 0;
+  dacc(a, p, n);
+  dacc(b, p, n);
+  dacc(c, p, n);
+  dacc(d, p, n);
+  return dsign_bound(p, n, 2);
+}
+
+// max_eb_to_keep_sign_eigen_delta_2 (:75-123); rarely taken (CP triangles)
+__device__ __noinline__ double deigen(float u0, float u1, float u2, float v0, float v1, float v2,
+                                      float x0, float x1, float x2, float y0, float y1, float y2) {
+  double eb = 1;
+  float c[4];
+  {
+    const float det = (x0 - x2) * (y1 - y2) - (x1 - x2) * (y0 - y2);
+    c[0] = (y1 - y2) / det;
+    c[1] = -(y0 - y2) / det;
+    c[2] = -(x1 - x2) / det;
+    c[3] = (x0 - x2) / det;
+    double p = 0, n = 0;
+    dacc(c[0] * u0, p, n);
+    dacc(c[1] * u1, p, n);
+    dacc(-(c[0] + c[1]) * u2, p, n);
+    dacc(c[2] * v0, p, n);
+    dacc(c[3] * v1, p, n);
+    dacc(-(c[2] + c[3]) * v2, p, n);
+    eb = dsign_bound(p, n, 1);
+    double p2 = 0, n2 = 0;
+    dacc(u0 * v1, p2, n2);
+    dacc(-u0 * v2, p2, n2);
+    dacc(u1 * v2, p2, n2);
+    dacc(-u1 * v0, p2, n2);
+    dacc(u2 * v0, p2, n2);
+    dacc(-u2 * v1, p2, n2);
+    eb = dminf(eb, dsign_bound(p2, n2, 2));
+  }
+  const float m = c[1] * c[2] - c[0] * c[3];
+  const float C = (-m) * (u0 * v1 - u0 * v2 + u1 * v2 - u1 * v0 + u2 * v0 - u2 * v1);
+  if (C == 0) return 0;
+  if (C < 0) return eb;
+  const float co[6] = {c[0] * u0, c[1] * u1, -(c[1] + c[0]) * u2,
+                       c[2] * v0, c[3] * v1, -(c[3] + c[2]) * v2};
+  double p = 0, n = 0;
+  for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) dacc(co[i] * co[j], p, n);
+  dacc(-4 * m * u1 * v0, p, n);
+  dacc(4 * m * u2 * v0, p, n);
+  dacc(4 * m * u0 * v1, p, n);
+  dacc(-4 * m * u2 * v1, p, n);
+  dacc(-4 * m * u0 * v2, p, n);
+  dacc(4 * m * u1 * v2, p, n);
+  return dminf(eb, dsign_bound(p, n, 2));
+}
+
+// max_eb_to_keep_position_and_type (src/sz_compress_cp_preserve_2d.cpp:125-172)
+// for the two triangle shapes of a cell (coordinates passed "reversed", :187-206).
+__device__ double dtri(float u0, float u1, float u2, float v0, float v1, float v2, int lower) {
+  const double u0v1 = u0 * v1, u1v0 = u1 * v0, u0v2 = u0 * v2, u2v0 = u2 * v0, u1v2 = u1 * v2,
+               u2v1 = u2 * v1;
+  const double det = u0v1 - u1v0 + u1v2 - u2v1 + u2v0 - u0v2;
+  if (!(det != 0)) return 0;
+  const bool f1 = (det / (u2v0 - u0v2) >= 1);
+  const bool f2 = (det / (u1v2 - u2v1) >= 1);
+  const bool f3 = (det / (u0v1 - u1v0) >= 1);
+  if (f1 && f2 && f3) {
+    const double e1 = dminf(dbound2(u2v0, -u0v2), dbound4(u0v1, -u1v0, u1v2, -u2v1));
+    const double e2 = dminf(dbound2(u1v2, -u2v1), dbound4(u0v1, -u1v0, u2v0, -u0v2));
+    const double e3 = dminf(dbound2(u0v1, -u1v0), dbound4(u1v2, -u2v1, u2v0, -u0v2));
+    // X_upper = {{0,0},{1,0},{1,1}}, X_lower = {{0,0},{0,1},{1,1}}; x = X[.][1], y = X[.][0]
+    const double e4 = lower ? dminf(e3, deigen(u0, u1, u2, v0, v1, v2, 0.f, 1.f, 1.f, 0.f, 0.f, 1.f))
+                            : dminf(e3, deigen(u0, u1, u2, v0, v1, v2, 0.f, 0.f, 1.f, 0.f, 1.f, 1.f));
+    double eb = dminf(e1, e2);
+    return dminf(eb, e4);
+  }
+  double eb = 0;
+  if (!f1) eb = dmaxm(eb, dminf(dbound2(u2v0, -u0v2), dbound4(u0v1, -u1v0, u1v2, -u2v1)));
+  if (!f2) eb = dmaxm(eb, dminf(dbound2(u1v2, -u2v1), dbound4(u0v1, -u1v0, u2v0, -u0v2)));
+  if (!f3) eb = dmaxm(eb, dminf(dbound2(u0v1, -u1v0), dbound4(u1v2, -u2v1, u2v0, -u0v2)));
+  return eb;
+}
+
+// eb code: 0 = zero precision; c >= 1: reference id c-1 with precision
+// 4^(c-1) * DBL_EPSILON (id 0 keeps precision eps in the compressor).
+__device__ __forceinline__ uint32_t eb_code_2d(double x, const double* __restrict__ q2d, int nq) {
+  if (!(x > 2.220446049250313080847263336181640625e-16)) return 0;
+  const int e = (int)((__double_as_longlong(x) >> 52) & 0x7FF) - 1023 + 52;  // exponent of x/eps
+  int k = e >> 1;
+  if (k + 1 < nq && x >= q2d[k + 1]) k++;
+  return (uint32_t)k + 1;
+}
+
+constexpr int kTH = 8, kTW = 64;  // vertex tile
+
+__global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const float* __restrict__ V,
+                                              int r1, int r2, double max_pwr_eb,
+                                              const double* __restrict__ q2d, int nq,
+                                              uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
+                                              uint32_t* __restrict__ hist, int* __restrict__ flags) {
+  __shared__ float su[kTH + 2][kTW + 2];
+  __shared__ float sv[kTH + 2][kTW + 2];
+  __shared__ double tb[kTH + 1][kTW + 1][2];
+  __shared__ uint32_t sh_hist[kMaxEbCode];
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.y * kTH, j0 = blockIdx.x * kTW;
+  for (int k = tid; k < kMaxEbCode; k += 256) sh_hist[k] = 0;
+  for (int k = tid; k < (kTH + 2) * (kTW + 2); k += 256) {
+    const int r = k / (kTW + 2), c = k % (kTW + 2);
+    const int gi = i0 - 1 + r, gj = j0 - 1 + c;
+    float a = 0, b = 0;
+    if (gi >= 0 && gi < r1 && gj >= 0 && gj < r2) {
+      a = U[(size_t)gi * r2 + gj];
+      b = V[(size_t)gi * r2 + gj];
+    }
+    su[r][c] = a;
+    sv[r][c] = b;
+  }
+  __syncthreads();
+  for (int k = tid; k < (kTH + 1) * (kTW + 1); k += 256) {
+    const int cr = k / (kTW + 1), cc = k % (kTW + 1);
+    const int ci = i0 - 1 + cr, cj = j0 - 1 + cc;
+    double up = INFINITY, lo = INFINITY;
+    if (ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1) {
+      // upper: (i,j),(i+1,j),(i+1,j+1); lower: (i,j),(i,j+1),(i+1,j+1)
+      up = dtri(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
+                sv[cr + 1][cc + 1], 0);
+      lo = dtri(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
+                sv[cr + 1][cc + 1], 1);
+    }
+    tb[cr][cc][0] = up;
+    tb[cr][cc][1] = lo;
+  }
+  __syncthreads();
+  int bad = 0;
+  for (int k = tid; k < kTH * kTW; k += 256) {
+    const int vr = k / kTW, vc = k % kTW;
+    const int i = i0 + vr, j = j0 + vc;
+    if (i >= r1 || j >= r2) continue;
+    // incident triangles in the reference's visiting order (cells row-major, upper first)
+    double eb = max_pwr_eb;
+    eb = dminf(eb, tb[vr][vc][0]);
+    eb = dminf(eb, tb[vr][vc][1]);
+    eb = dminf(eb, tb[vr][vc + 1][0]);
+    eb = dminf(eb, tb[vr + 1][vc][1]);
+    eb = dminf(eb, tb[vr + 1][vc + 1][0]);
+    eb = dminf(eb, tb[vr + 1][vc + 1][1]);
+    const float u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
+    if (!isfinite(u) || !isfinite(v) || !(eb == eb)) bad = 1;
+    // the reference's eb/DBL_EPSILON overflows from 2^972 on (undefined id)
+    if ((double)fabsf(u) * eb >= 0x1p972 || (double)fabsf(v) * eb >= 0x1p972) bad = 1;
+    const uint32_t a = eb_code_2d((double)fabsf(u) * eb, q2d, nq);
+    const uint32_t b = eb_code_2d((double)fabsf(v) * eb, q2d, nq);
+    cu[(size_t)i * r2 + j] = (uint16_t)a;
+    cv[(size_t)i * r2 + j] = (uint16_t)b;
+    atomicAdd(&sh_hist[a ? a - 1 : 0], 1u);
+    atomicAdd(&sh_hist[b ? b - 1 : 0], 1u);
+    if (a > 82 || b > 82) atomicOr(&flags[1], 1);  // beyond the FP32 Lorenzo path
+  }
+  if (bad) atomicOr(&flags[0], 1);
+  __syncthreads();
+  for (int k = tid; k < kMaxEbCode; k += 256)
+    if (sh_hist[k]) atomicAdd(&hist[k], sh_hist[k]);
+}
+
+// ------------------------------------------------------------ wavefront ----
+constexpr int kCH = 32;  // steps per chunk (boundary publish / prefetch granularity)
+
+__device__ __forceinline__ int ld_coh(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float shr1(float src, float lane0) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(src),
+                                                    0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+}
+
+struct Strip2d {
+  const float* x[2];
+  const uint16_t* c[2];  // compress: eb codes; decompress: stream ids
+  uint16_t* q[2];        // quant codes (written by compress, read by decompress)
+  const float* raw[2];   // decompress: unpredictable value per element
+  float* out[2];         // decompress output
+  int* bnd;              // [2][nstrips][r2] boundary rows (float bits)
+  int* prog;             // [2][nstrips] published columns of the boundary row
+  int* ticket;
+  int r1, r2, nstrips;
+};
+
+template <bool COMPRESS, bool WIDE>
+__global__ void __launch_bounds__(64) k_lorenzo2d(Strip2d a) {
+  const int lane = threadIdx.x;
+  const int r1 = a.r1, r2 = a.r2;
+  const int items = 2 * a.nstrips;
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(a.ticket, 1);
+    item = __shfl(item, 0);
+    if (item >= items) return;
+    const int comp = item & 1, s = item >> 1;
+    const int row = s * 64 + lane;
+    const bool rvalid = row < r1;
+    const size_t roff = (size_t)(rvalid ? row : 0) * r2;
+    const float* X = COMPRESS ? a.x[comp] + roff : nullptr;
+    const uint16_t* C = a.c[comp] + roff;
+    uint16_t* Q = a.q[comp] + roff;
+    const float* RAW = COMPRESS ? nullptr : a.raw[comp] + roff;
+    float* OUT = COMPRESS ? nullptr : a.out[comp] + roff;
+    const bool has_up = s > 0;
+    const bool publish = s + 1 < a.nstrips;
+    int* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
+    int* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
+    int* prog_in = a.prog + comp * a.nstrips + (s > 0 ? s - 1 : 0);
+    int* prog_out = a.prog + comp * a.nstrips + s;
+    float Rprev = 0.f, Uprev = 0.f;
+    const int steps = r2 + 63;
+    for (int t0 = 0; t0 < steps; t0 += kCH) {
+      float inc = 0.f;
+      if (has_up && t0 < r2) {
+        const int need = min(t0 + kCH, r2);
+        while (ld_coh(prog_in) < need) __builtin_amdgcn_s_sleep(2);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (lane < kCH && t0 + lane < r2) inc = __int_as_float(ld_coh(bnd_in + t0 + lane));
+      }
+      float xs[kCH];
+      uint32_t cs[kCH];
+      uint32_t qs_in[kCH];
+      float raws[kCH];
+#pragma unroll
+      for (int k = 0; k < kCH; k++) {
+        const int j = t0 + k - lane;
+        const bool act = rvalid && j >= 0 && j < r2;
+        cs[k] = act ? C[j] : 0;
+        if (COMPRESS) {
+          xs[k] = act ? X[j] : 0.f;
+        } else {
+          qs_in[k] = act ? Q[j] : 0;
+          raws[k] = act ? RAW[j] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kCH; k++) {
+        const int t = t0 + k;
+        const int j = t - lane;
+        const bool act = rvalid && j >= 0 && j < r2;
+        const float lane0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inc), k));
+        const float Uv = shr1(Rprev, lane0);
+        const float pred = (Rprev + Uv) - Uprev;
+        float R;
+        if (COMPRESS) {
+          uint16_t code;
+          R = l2d_compress_step<WIDE>(pred, xs[k], cs[k], code);
+          if (act) Q[j] = code;
+        } else {
+          R = l2d_decompress_step<WIDE>(pred, qs_in[k], cs[k], raws[k]);
+          if (act) OUT[j] = R;
+        }
+        if (publish && lane == 63 && act) st_coh(bnd_out + j, __float_as_int(R));
+        Uprev = Uv;
+        Rprev = (j < 0) ? 0.f : R;
+      }
+      if (publish) {
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const int done = min(max(t0 + kCH - 63, 0), r2);
+        if (lane == 63) st_coh(prog_out, done);
+      }
+    }
+  }
+}
+
+__global__ void k_zero_i32(int* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+// ------------------------------------------------------- huffman sources ----
+struct SrcIds2d {
+  const uint16_t* cu;
+  const uint16_t* cv;
+  size_t n;
+  static constexpr int kChunk = 256;
+  template <class F>
+  __device__ void each(size_t c, F f) const {
+    const size_t b = c * kChunk, e = min(b + kChunk, 2 * n);
+    for (size_t p = b; p < e; p++) {
+      const uint32_t x = p < n ? cu[p] : cv[p - n];
+      f(x ? x - 1 : 0);
+    }
+  }
+};
+
+struct SrcCodes2d {
+  const uint16_t* q;
+  int r1, r2, nby;
+  template <class F>
+  __device__ void each(size_t b, F f) const {
+    const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+    const int i0 = bi * kBlock2d, j0 = bj * kBlock2d;
+    const int sx = min(kBlock2d, r1 - i0), sy = min(kBlock2d, r2 - j0);
+    for (int ii = 0; ii < sx; ii++) {
+      const uint16_t* row = q + (size_t)(i0 + ii) * r2 + j0;
+      for (int jj = 0; jj < sy; jj++) f((uint32_t)row[jj]);
+    }
+  }
+};
+
+struct SinkIds2d {
+  uint16_t* su;
+  uint16_t* sv;
+  size_t n;
+  __device__ void put(uint64_t p, uint32_t s) const {
+    if (p < n) su[p] = (uint16_t)s;
+    else sv[p - n] = (uint16_t)s;
+  }
+};
+
+// block-major position -> row-major element
+struct SinkCodes2d {
+  uint16_t* q;
+  int r1, r2, nby;
+  __device__ void put(uint64_t p, uint32_t s) const {
+    const uint64_t rowblk = (uint64_t)kBlock2d * r2;
+    const int bi = (int)(p / rowblk);
+    const uint64_t rem = p - (uint64_t)bi * rowblk;
+    const int sx = min(kBlock2d, r1 - bi * kBlock2d);
+    int bj = (int)(rem / ((uint64_t)sx * kBlock2d));
+    if (bj > nby - 1) bj = nby - 1;
+    const uint32_t rem2 = (uint32_t)(rem - (uint64_t)bj * sx * kBlock2d);
+    const int sy = min(kBlock2d, r2 - bj * kBlock2d);
+    const int ii = rem2 / sy, jj = rem2 - ii * sy;
+    q[(size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj] = (uint16_t)s;
+  }
+};
+
+// unpredictable values in block-major order
+__global__ void k_block_zeros(const uint16_t* __restrict__ q, int r1, int r2, int nby, size_t nb,
+                              uint32_t* __restrict__ cnt) {
+  const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  uint32_t z = 0;
+  SrcCodes2d src{q, r1, r2, nby};
+  src.each(b, [&](uint32_t s) { z += (s == 0); });
+  cnt[b] = z;
+}
+
+__global__ void k_gather_unpred(const uint16_t* __restrict__ q, const float* __restrict__ x, int r1,
+                                int r2, int nby, size_t nb, const uint64_t* __restrict__ off,
+                                float* __restrict__ out) {
+  const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+  const int i0 = bi * kBlock2d, j0 = bj * kBlock2d;
+  const int sx = min(kBlock2d, r1 - i0), sy = min(kBlock2d, r2 - j0);
+  uint64_t o = off[b];
+  for (int ii = 0; ii < sx; ii++)
+    for (int jj = 0; jj < sy; jj++) {
+      const size_t e = (size_t)(i0 + ii) * r2 + j0 + jj;
+      if (q[e] == 0) out[o++] = x[e];
+    }
+}
+
+__global__ void k_scatter_unpred(const uint16_t* __restrict__ q, const float* __restrict__ vals,
+                                 int r1, int r2, int nby, size_t nb, const uint64_t* __restrict__ off,
+                                 uint64_t count, float* __restrict__ raw, int* __restrict__ err) {
+  const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+  const int i0 = bi * kBlock2d, j0 = bj * kBlock2d;
+  const int sx = min(kBlock2d, r1 - i0), sy = min(kBlock2d, r2 - j0);
+  uint64_t o = off[b];
+  for (int ii = 0; ii < sx; ii++)
+    for (int jj = 0; jj < sy; jj++) {
+      const size_t e = (size_t)(i0 + ii) * r2 + j0 + jj;
+      if (q[e] == 0) {
+        if (o >= count) {
+          *err = 1;
+          return;
+        }
+        raw[e] = vals[o++];
+      }
+    }
+}
+
+struct U16Hist {
+  const uint16_t* p;
+  size_t n;
+  static constexpr int kChunk = 1024;
+  template <class F>
+  __device__ void each(size_t c, F f) const {
+    const size_t b = c * kChunk, e = min(b + kChunk, n);
+    for (size_t i = b; i < e; i++) f(p[i]);
+  }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------- host ----
+namespace {
+
+template <typename V>
+void put(std::vector<uint8_t>& o, const V& v) {
+  const size_t at = o.size();
+  o.resize(at + sizeof(V));
+  std::memcpy(o.data() + at, &v, sizeof(V));
+}
+
+struct Piece {  // a contiguous part of the output stream
+  size_t off;
+  const void* src;  // host (staging) or device pointer
+  size_t len;
+  bool device;
+};
+
+void launch_strips(Ctx& ctx, Strip2d& sa, bool compress, bool wide) {
+  hipStream_t st = ctx.stream();
+  const int nz = 2 * sa.nstrips + 1;
+  k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(sa.prog, nz);
+  const int items = 2 * sa.nstrips;
+  const int grid = items < 4096 ? items : 4096;
+  if (compress) {
+    if (wide) k_lorenzo2d<true, true><<<grid, 64, 0, st>>>(sa);
+    else k_lorenzo2d<true, false><<<grid, 64, 0, st>>>(sa);
+  } else {
+    if (wide) k_lorenzo2d<false, true><<<grid, 64, 0, st>>>(sa);
+    else k_lorenzo2d<false, false><<<grid, 64, 0, st>>>(sa);
+  }
+  CPSZ_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
+                        double max_pwr_eb, DeviceStream& out) {
+  CPSZ_REQUIRE(dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "compress_2d: bad arguments");
+  CPSZ_REQUIRE(r1 < (1u << 30) && r2 < (1u << 30) && r1 * r2 < (1ull << 40), kErrUnsupported,
+               "compress_2d: field too large");
+  hipStream_t st = ctx.stream();
+  const size_t n = r1 * r2;
+  ctx.stages.clear();
+  ctx.mark("start");
+  // ---- eb codes
+  uint16_t* cu = ctx.buf<uint16_t>("c2.cu", n);
+  uint16_t* cv = ctx.buf<uint16_t>("c2.cv", n);
+  uint32_t* hist = ctx.buf<uint32_t>("c2.hist", kMaxEbCode + 2 * 65536);
+  int* flags = ctx.buf<int>("c2.flags", 4);
+  CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
+  CPSZ_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
+  {
+    dim3 grid((unsigned)div_up(r2, kTW), (unsigned)div_up(r1, kTH));
+    k_eb2d<<<grid, 256, 0, st>>>(dU, dV, (int)r1, (int)r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
+                                 cu, cv, hist, flags);
+    CPSZ_HIP(hipGetLastError());
+  }
+  ctx.mark("eb");
+  int hflags[4];
+  CPSZ_HIP(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_REQUIRE(hflags[0] == 0, kErrNonFinite,
+               "compress_2d: non-finite input or bound (the reference cannot encode it)");
+  const bool wide = hflags[1] != 0;
+  // ---- Lorenzo wavefront
+  const int nstrips = (int)div_up(r1, 64);
+  uint16_t* qu = ctx.buf<uint16_t>("c2.qu", n);
+  uint16_t* qv = ctx.buf<uint16_t>("c2.qv", n);
+  Strip2d sa{};
+  sa.x[0] = dU;
+  sa.x[1] = dV;
+  sa.c[0] = cu;
+  sa.c[1] = cv;
+  sa.q[0] = qu;
+  sa.q[1] = qv;
+  sa.bnd = ctx.buf<int>("c2.bnd", (size_t)2 * nstrips * r2);
+  sa.prog = ctx.buf<int>("c2.prog", 2 * nstrips + 1);
+  sa.ticket = sa.prog + 2 * nstrips;
+  sa.r1 = (int)r1;
+  sa.r2 = (int)r2;
+  sa.nstrips = nstrips;
+  launch_strips(ctx, sa, true, wide);
+  ctx.mark("lorenzo");
+  // ---- histograms of the quant codes
+  {
+    const uint32_t win_lo = 32768 - 8192, win_n = 16384;
+    U16Hist hu{qu, n}, hv{qv, n};
+    const size_t nch = div_up(n, U16Hist::kChunk);
+    const int grid = (int)std::min<size_t>(div_up(nch, 256), 1024);
+    k_hist<<<grid, 256, win_n * 4, st>>>(hu, nch, hist + kMaxEbCode, win_lo, win_n);
+    k_hist<<<grid, 256, win_n * 4, st>>>(hv, nch, hist + kMaxEbCode + 65536, win_lo, win_n);
+    CPSZ_HIP(hipGetLastError());
+  }
+  ctx.mark("hist");
+  // ---- host trees
+  const size_t hist_n = kMaxEbCode + 2 * 65536;
+  uint32_t* hh = (uint32_t*)ctx.host_buf("c2.hist", hist_n * 4);
+  CPSZ_HIP(hipMemcpyAsync(hh, hist, hist_n * 4, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  std::vector<uint64_t> f(65536);
+  HuffEncodeTable t_eb, t_u, t_v;
+  for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
+  huff_build(f.data(), kMaxEbCode, 2 * 1024, t_eb);
+  for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + k];
+  huff_build(f.data(), 65536, 2 * kCapacity, t_u);
+  for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + 65536 + k];
+  huff_build(f.data(), 65536, 2 * kCapacity, t_v);
+  ctx.mark("tree");
+  // ---- codebooks and encode pass A (bits per chunk)
+  uint64_t* cb_code = ctx.buf<uint64_t>("c2.cbcode", 3 * 65536);
+  uint8_t* cb_len = ctx.buf<uint8_t>("c2.cblen", 3 * 65536);
+  upload_codebook(ctx, t_eb, cb_code, cb_len, "c2.eb");
+  upload_codebook(ctx, t_u, cb_code + 65536, cb_len + 65536, "c2.u");
+  upload_codebook(ctx, t_v, cb_code + 2 * 65536, cb_len + 2 * 65536, "c2.v");
+  const int nby = (int)div_up(r2, kBlock2d);
+  const size_t nb = div_up(r1, kBlock2d) * nby;
+  const size_t nch_ids = div_up(2 * n, SrcIds2d::kChunk);
+  SrcIds2d s_ids{cu, cv, n};
+  SrcCodes2d s_u{qu, (int)r1, (int)r2, nby}, s_v{qv, (int)r1, (int)r2, nby};
+  uint64_t* bits = ctx.buf<uint64_t>("c2.bits", nch_ids + 2 * nb);
+  uint64_t* offs = ctx.buf<uint64_t>("c2.offs", nch_ids + 2 * nb + 8);
+  uint32_t* zc = ctx.buf<uint32_t>("c2.zc", 2 * nb);
+  uint64_t* zoff = ctx.buf<uint64_t>("c2.zoff", 2 * nb + 4);
+  uint64_t* totals = ctx.buf<uint64_t>("c2.totals", 8);
+  k_chunk_bits<<<(int)div_up(nch_ids, 256), 256, 0, st>>>(s_ids, nch_ids, cb_len, bits);
+  k_chunk_bits<<<(int)div_up(nb, 256), 256, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
+  k_chunk_bits<<<(int)div_up(nb, 256), 256, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
+  k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, (int)r1, (int)r2, nby, nb, zc);
+  k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, (int)r1, (int)r2, nby, nb, zc + nb);
+  CPSZ_HIP(hipGetLastError());
+  scan_u64(ctx, bits, offs, nch_ids, totals + 0);
+  scan_u64(ctx, bits + nch_ids, offs + nch_ids, nb, totals + 1);
+  scan_u64(ctx, bits + nch_ids + nb, offs + nch_ids + nb, nb, totals + 2);
+  scan_u32_to_u64(ctx, zc, zoff, nb, totals + 3);
+  scan_u32_to_u64(ctx, zc + nb, zoff + nb, nb, totals + 4);
+  uint64_t* ht = (uint64_t*)ctx.host_buf("c2.totals", 8 * 8);
+  CPSZ_HIP(hipMemcpyAsync(ht, totals, 5 * 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  const uint64_t bits_eb = ht[0], bits_u = ht[1], bits_v = ht[2], unp_u = ht[3], unp_v = ht[4];
+  ctx.mark("scan");
+  // ---- encode pass B into aligned scratch
+  const size_t bytes_eb = div_up(bits_eb, 8), bytes_u = div_up(bits_u, 8), bytes_v = div_up(bits_v, 8);
+  const size_t w_eb = div_up(bytes_eb, 4) + 2, w_u = div_up(bytes_u, 4) + 2, w_v = div_up(bytes_v, 4) + 2;
+  uint32_t* words = ctx.buf<uint32_t>("c2.words", w_eb + w_u + w_v);
+  k_fill_u32<<<1024, 256, 0, st>>>(words, w_eb + w_u + w_v, 0);
+  k_encode<<<(int)div_up(nch_ids, 256), 256, 0, st>>>(s_ids, nch_ids, cb_code, cb_len, offs, words);
+  k_encode<<<(int)div_up(nb, 256), 256, 0, st>>>(s_u, nb, cb_code + 65536, cb_len + 65536,
+                                                 offs + nch_ids, words + w_eb);
+  k_encode<<<(int)div_up(nb, 256), 256, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
+                                                 offs + nch_ids + nb, words + w_eb + w_u);
+  float* unp = ctx.buf<float>("c2.unp", unp_u + unp_v + 1);
+  k_gather_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, dU, (int)r1, (int)r2, nby, nb, zoff, unp);
+  k_gather_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, dV, (int)r1, (int)r2, nby, nb, zoff + nb,
+                                                        unp + unp_u);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("encode");
+  // ---- stream layout (src/sz_compress_cp_preserve_2d.cpp:258-269, src/sz_compress_3d.cpp:219-228)
+  const size_t ind_bytes = (nb - 1) / 8 + 1;
+  auto huff_head = [](const HuffEncodeTable& t, size_t nbytes) {
+    std::vector<uint8_t> h;
+    put(h, (size_t)t.node_count);
+    put(h, (uint32_t)t.tree.size());
+    h.insert(h.end(), t.tree.begin(), t.tree.end());
+    put(h, (size_t)nbytes);
+    return h;
+  };
+  auto comp_head = [&](uint64_t unp_count) {
+    std::vector<uint8_t> h;
+    put(h, (int)kBlock2d);
+    put(h, (char)0);
+    put(h, (int)kRadius);
+    put(h, (uint64_t)0);  // meanInfo<float>{false, 0}: flag byte, 3 padding bytes, float 0
+    put(h, (size_t)0);    // reg_count
+    put(h, (size_t)unp_count);
+    return h;
+  };
+  std::vector<uint8_t> h_eb = huff_head(t_eb, bytes_eb);
+  std::vector<uint8_t> hu1 = comp_head(unp_u), hv1 = comp_head(unp_v);
+  std::vector<uint8_t> hu2(ind_bytes, 0), hv2(ind_bytes, 0);
+  {
+    std::vector<uint8_t> a = huff_head(t_u, bytes_u), b = huff_head(t_v, bytes_v);
+    hu2.insert(hu2.end(), a.begin(), a.end());
+    hv2.insert(hv2.end(), b.begin(), b.end());
+  }
+  const size_t eb_size = h_eb.size() + bytes_eb;
+  const size_t u_size = hu1.size() + 4 * unp_u + hu2.size() + bytes_u;
+  const size_t v_size = hv1.size() + 4 * unp_v + hv2.size() + bytes_v;
+  std::vector<uint8_t> hdr;
+  put(hdr, (int)4);
+  put(hdr, (double)DBL_EPSILON);
+  put(hdr, eb_size);
+  put(hdr, u_size);
+  put(hdr, v_size);
+  const size_t total = hdr.size() + eb_size + u_size + v_size;
+  uint8_t* dout = ctx.buf<uint8_t>("c2.out", total + 16);
+  // host pieces go through one pinned staging buffer
+  std::vector<const std::vector<uint8_t>*> hp = {&hdr, &h_eb, &hu1, &hu2, &hv1, &hv2};
+  size_t hbytes = 0;
+  for (auto* p : hp) hbytes += p->size();
+  uint8_t* stage = (uint8_t*)ctx.host_buf("c2.stage", hbytes);
+  std::vector<Piece> pieces;
+  size_t pos = 0, spos = 0;
+  auto host_piece = [&](const std::vector<uint8_t>& v) {
+    std::memcpy(stage + spos, v.data(), v.size());
+    pieces.push_back({pos, stage + spos, v.size(), false});
+    pos += v.size();
+    spos += v.size();
+  };
+  auto dev_piece = [&](const void* src, size_t len) {
+    pieces.push_back({pos, src, len, true});
+    pos += len;
+  };
+  host_piece(hdr);
+  host_piece(h_eb);
+  dev_piece(words, bytes_eb);
+  host_piece(hu1);
+  dev_piece(unp, 4 * unp_u);
+  host_piece(hu2);
+  dev_piece(words + w_eb, bytes_u);
+  host_piece(hv1);
+  dev_piece(unp + unp_u, 4 * unp_v);
+  host_piece(hv2);
+  dev_piece(words + w_eb + w_u, bytes_v);
+  CPSZ_REQUIRE(pos == total, kErrInternal, "stream layout mismatch");
+  for (const Piece& p : pieces) {
+    if (!p.len) continue;
+    CPSZ_HIP(hipMemcpyAsync(dout + p.off, p.src, p.len,
+                            p.device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  }
+  ctx.mark("assemble");
+  out.data = dout;
+  out.size = total;
+  CPSZ_HIP(hipStreamSynchronize(st));
+  ctx.finish_marks();
+}
+
+namespace {
+// Reads stream bytes either from a host copy or from device memory.
+struct StreamReader {
+  Ctx& ctx;
+  const uint8_t* host;  // may be null
+  const uint8_t* dev;
+  size_t size;
+  std::vector<uint8_t> tmp;
+  const uint8_t* read(size_t off, size_t len) {
+    CPSZ_REQUIRE(off <= size && len <= size - off, kErrStream, "stream truncated");
+    if (host) return host + off;
+    tmp.resize(len);
+    if (len) {
+      CPSZ_HIP(hipMemcpyAsync(tmp.data(), dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
+      CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+    }
+    return tmp.data();
+  }
+  template <typename V>
+  V get(size_t off) {
+    V v;
+    std::memcpy(&v, read(off, sizeof(V)), sizeof(V));
+    return v;
+  }
+};
+
+struct HuffSection {
+  size_t node_count;
+  uint32_t tree_size;
+  std::vector<uint8_t> tree;
+  size_t nbytes;
+  size_t bits_off;  // absolute offset of the bitstream
+  size_t end;
+};
+
+HuffSection parse_huff(StreamReader& r, size_t off) {
+  HuffSection h;
+  h.node_count = r.get<size_t>(off);
+  h.tree_size = r.get<uint32_t>(off + 8);
+  CPSZ_REQUIRE(h.node_count >= 1 && h.node_count < (1u << 24), kErrStream, "bad huffman node count");
+  const uint8_t* t = r.read(off + 12, h.tree_size);
+  h.tree.assign(t, t + h.tree_size);
+  h.nbytes = r.get<size_t>(off + 12 + h.tree_size);
+  h.bits_off = off + 20 + h.tree_size;
+  CPSZ_REQUIRE(h.nbytes <= r.size && h.bits_off + h.nbytes <= r.size, kErrStream,
+               "huffman bitstream truncated");
+  h.end = h.bits_off + h.nbytes;
+  return h;
+}
+
+}  // namespace
+
+void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
+                          size_t r1, size_t r2, float* dU, float* dV) {
+  CPSZ_REQUIRE(d_in && dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "decompress_2d: bad arguments");
+  hipStream_t st = ctx.stream();
+  const size_t n = r1 * r2;
+  ctx.stages.clear();
+  ctx.mark("start");
+  StreamReader rd{ctx, h_in, d_in, in_size, {}};
+  const int base = rd.get<int>(0);
+  const double thr = rd.get<double>(4);
+  const size_t u_size = rd.get<size_t>(20);
+  (void)u_size;
+  CPSZ_REQUIRE(base == 4 && thr == DBL_EPSILON, kErrUnsupported,
+               "decompress_2d: stream base/threshold differ from the offline 2D format");
+  HuffSection h_eb = parse_huff(rd, 36);
+  // components follow the id section directly; the u section spans u_size bytes
+  struct Comp {
+    size_t off, unp_count, unp_off;
+    HuffSection h;
+  } comp[2];
+  const int nby = (int)div_up(r2, kBlock2d);
+  const size_t nb = div_up(r1, kBlock2d) * nby;
+  size_t off = h_eb.end;
+  for (int c = 0; c < 2; c++) {
+    Comp& k = comp[c];
+    k.off = off;
+    const int bs = rd.get<int>(off);
+    const int radius = rd.get<int>(off + 5);
+    const uint8_t use_mean = rd.get<uint8_t>(off + 9);
+    const size_t reg = rd.get<size_t>(off + 17);
+    k.unp_count = rd.get<size_t>(off + 25);
+    CPSZ_REQUIRE(bs == kBlock2d && radius == kRadius && use_mean == 0 && reg == 0, kErrUnsupported,
+                 "decompress_2d: stream uses block settings the offline compressor never writes");
+    CPSZ_REQUIRE(k.unp_count <= n, kErrStream, "bad unpredictable count");
+    k.unp_off = off + 33;
+    const size_t ind = k.unp_off + 4 * k.unp_count;
+    const size_t ind_bytes = (nb - 1) / 8 + 1;
+    const uint8_t* ib = rd.read(ind, ind_bytes);
+    for (size_t b = 0; b < ind_bytes; b++)
+      CPSZ_REQUIRE(ib[b] == 0, kErrUnsupported, "regression blocks are not produced by the 2D codec");
+    k.h = parse_huff(rd, ind + ind_bytes);
+    off = c == 0 ? off + u_size : k.h.end;
+  }
+  ctx.mark("parse");
+  // ---- bitstreams into aligned scratch
+  auto stage_bits = [&](const HuffSection& h, const char* name) {
+    // slack words: a speculative decode near the end peeks up to ~110 bits past it
+    uint32_t* w = ctx.buf<uint32_t>(name, div_up(h.nbytes, 4) + 8);
+    CPSZ_HIP(hipMemsetAsync(w, 0, (div_up(h.nbytes, 4) + 8) * 4, st));
+    if (h.nbytes)
+      CPSZ_HIP(hipMemcpyAsync(w, d_in + h.bits_off, h.nbytes, hipMemcpyDeviceToDevice, st));
+    return w;
+  };
+  uint16_t* sid_u = ctx.buf<uint16_t>("d2.sidu", n);
+  uint16_t* sid_v = ctx.buf<uint16_t>("d2.sidv", n);
+  uint16_t* qu = ctx.buf<uint16_t>("d2.qu", n);
+  uint16_t* qv = ctx.buf<uint16_t>("d2.qv", n);
+  {
+    uint32_t* w = stage_bits(h_eb, "d2.w.eb");
+    DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
+    decode_stream(ctx, dd, w, h_eb.nbytes, 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb");
+  }
+  uint16_t* qs[2] = {qu, qv};
+  for (int c = 0; c < 2; c++) {
+    uint32_t* w = stage_bits(comp[c].h, c ? "d2.w.v" : "d2.w.u");
+    DeviceDecoder dd = upload_decoder(ctx, comp[c].h.tree.data(), comp[c].h.tree.size(),
+                                      comp[c].h.node_count, c ? "d2.v" : "d2.u");
+    decode_stream(ctx, dd, w, comp[c].h.nbytes, n, SinkCodes2d{qs[c], (int)r1, (int)r2, nby},
+                  c ? "d2.v" : "d2.u");
+  }
+  ctx.mark("huffman");
+  // ---- unpredictable values to per-element raw arrays (block-major order)
+  float* raw = ctx.buf<float>("d2.raw", 2 * n);
+  uint32_t* zc = ctx.buf<uint32_t>("d2.zc", 2 * nb);
+  uint64_t* zoff = ctx.buf<uint64_t>("d2.zoff", 2 * nb + 4);
+  int* err = ctx.buf<int>("d2.err", 2);
+  CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
+  for (int c = 0; c < 2; c++) {
+    float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", comp[c].unp_count + 1);
+    if (comp[c].unp_count)
+      CPSZ_HIP(hipMemcpyAsync(vals, d_in + comp[c].unp_off, 4 * comp[c].unp_count,
+                              hipMemcpyDeviceToDevice, st));
+    k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], (int)r1, (int)r2, nby, nb, zc + c * nb);
+    scan_u32_to_u64(ctx, zc + c * nb, zoff + c * (nb + 1), nb, zoff + c * (nb + 1) + nb);
+    k_scatter_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], vals, (int)r1, (int)r2, nby, nb,
+                                                           zoff + c * (nb + 1), comp[c].unp_count,
+                                                           raw + c * n, err);
+    CPSZ_HIP(hipGetLastError());
+  }
+  ctx.mark("unpred");
+  // ---- wavefront reconstruction
+  const int nstrips = (int)div_up(r1, 64);
+  Strip2d sa{};
+  sa.c[0] = sid_u;
+  sa.c[1] = sid_v;
+  sa.q[0] = qu;
+  sa.q[1] = qv;
+  sa.raw[0] = raw;
+  sa.raw[1] = raw + n;
+  sa.out[0] = dU;
+  sa.out[1] = dV;
+  sa.bnd = ctx.buf<int>("d2.bnd", (size_t)2 * nstrips * r2);
+  sa.prog = ctx.buf<int>("d2.prog", 2 * nstrips + 1);
+  sa.ticket = sa.prog + 2 * nstrips;
+  sa.r1 = (int)r1;
+  sa.r2 = (int)r2;
+  sa.nstrips = nstrips;
+  // ids above 81 need the double-precision reconstruction path
+  uint32_t max_id = 0;
+  {
+    HuffDecodeTree t;
+    CPSZ_REQUIRE(huff_parse_tree(h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, t), kErrStream,
+                 "malformed id tree");
+    for (size_t i = 0; i < t.node_count; i++)
+      if ((t.leaf[i] || (i == 0 && t.root_leaf)) && t.sym[i] > max_id) max_id = t.sym[i];
+  }
+  launch_strips(ctx, sa, false, max_id > 81);
+  ctx.mark("lorenzo");
+  int herr[2];
+  CPSZ_HIP(hipMemcpyAsync(herr, err, 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_REQUIRE(herr[0] == 0, kErrStream, "decompress_2d: more unpredictable codes than values");
+  ctx.finish_marks();
+}
+
+// Host emulation of the wavefront kernels' arithmetic (row-major order; any
+// order that respects the Lorenzo dependencies yields the same values).
+void lorenzo2d_emulate(const float* x, const uint16_t* c, size_t r1, size_t r2, bool wide,
+                       uint16_t* q, float* recon) {
+  std::vector<float> R((r1 + 1) * (r2 + 1), 0.f);
+  const size_t W = r2 + 1;
+  for (size_t i = 0; i < r1; i++)
+    for (size_t j = 0; j < r2; j++) {
+      float* rp = &R[(i + 1) * W + j + 1];
+      const float pred = (rp[-1] + rp[-(ptrdiff_t)W]) - rp[-(ptrdiff_t)W - 1];
+      uint16_t code;
+      const size_t e = i * r2 + j;
+      *rp = wide ? l2d_compress_step<true>(pred, x[e], c[e], code)
+                 : l2d_compress_step<false>(pred, x[e], c[e], code);
+      q[e] = code;
+      if (recon) recon[e] = *rp;
+    }
+}
+
+void lorenzo2d_emulate_decode(const uint16_t* q, const uint16_t* sid, const float* raw, size_t r1,
+                              size_t r2, bool wide, float* out) {
+  std::vector<float> R((r1 + 1) * (r2 + 1), 0.f);
+  const size_t W = r2 + 1;
+  for (size_t i = 0; i < r1; i++)
+    for (size_t j = 0; j < r2; j++) {
+      float* rp = &R[(i + 1) * W + j + 1];
+      const float pred = (rp[-1] + rp[-(ptrdiff_t)W]) - rp[-(ptrdiff_t)W - 1];
+      const size_t e = i * r2 + j;
+      *rp = wide ? l2d_decompress_step<true>(pred, q[e], sid[e], raw[e])
+                 : l2d_decompress_step<false>(pred, q[e], sid[e], raw[e]);
+      out[e] = *rp;
+    }
+}
+
+}  // namespace cpsz

@@ -1,0 +1,100 @@
+// Per-device context of the library: stream, grow-only device buffers, pinned
+// host staging, and the eb-id threshold tables derived from the host libm.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace cpsz {
+
+// Smallest inputs at which the reference's eb id computation reaches each id,
+// computed once per process with the host libm the reference links against:
+//  2D (include/sz_cp_preserve_utils.hpp:14-22, base 4 over DBL_EPSILON):
+//    q2d[k] = min { x > eps : (int)(log2(x/eps)/log2(4)) >= k }, k >= 1
+//  3D (src/sz_compress_cp_preserve_3d.cpp:470-476, base 2 over FLT_EPSILON on
+//    log2(1+eb)): q3d[k] = min { e > 0 : id3(e) >= k }, k >= 1
+struct EbTables {
+  std::vector<double> q2d;  // index k; q2d[0] unused; +inf past the range
+  std::vector<double> q3d;
+};
+const EbTables& eb_tables();
+
+// Exceptions of the host's log2f / exp2f against the library's own
+// correctly-rounded-in-double device evaluation (see libm_tables.cpp).
+struct LibmTables {
+  // sorted float bit patterns and the host libm results
+  std::vector<uint32_t> log2f_x, log2f_y;
+  std::vector<uint32_t> exp2f_x, exp2f_y;
+  // per input binade: max distance to a rounding midpoint among exceptions
+  std::vector<float> log2f_band, exp2f_band;
+};
+const LibmTables& libm_tables();
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t cap = 0;
+};
+
+class Ctx {
+ public:
+  explicit Ctx(int device);
+  ~Ctx();
+  Ctx(const Ctx&) = delete;
+  Ctx& operator=(const Ctx&) = delete;
+
+  int device() const { return device_; }
+  hipStream_t stream() const { return stream_; }
+  void set_stream(hipStream_t s) { stream_ = s ? s : own_stream_; }
+
+  // Named grow-only device scratch buffer (contents undefined).
+  template <typename T>
+  T* buf(const char* name, size_t count) {
+    return static_cast<T*>(raw_buf(name, count * sizeof(T)));
+  }
+  void* raw_buf(const char* name, size_t bytes);
+  // Pinned host staging buffer (grow-only).
+  void* host_buf(const char* name, size_t bytes);
+
+  // Device copies of the eb threshold tables.
+  const double* d_q2d();
+  const double* d_q3d();
+  int q2d_size() const { return (int)eb_tables().q2d.size(); }
+  int q3d_size() const { return (int)eb_tables().q3d.size(); }
+  // Device copies of the libm exception tables (uploaded on first use).
+  struct LibmDev {
+    const uint32_t *log2f_x, *log2f_y, *exp2f_x, *exp2f_y;
+    const float *log2f_band, *exp2f_band;
+    int n_log2f, n_exp2f;
+  };
+  const LibmDev& libm_dev();
+
+  // Stage timings of the last call (ms), filled when timing is enabled.
+  bool timing = false;
+  std::vector<std::pair<std::string, float>> stages;
+  void mark(const char* name);
+  void finish_marks();
+
+ private:
+  int device_;
+  hipStream_t stream_ = nullptr;
+  hipStream_t own_stream_ = nullptr;
+  std::map<std::string, DevBuf> bufs_;
+  std::map<std::string, DevBuf> hbufs_;
+  double* d_q2d_ = nullptr;
+  double* d_q3d_ = nullptr;
+  LibmDev libm_{};
+  bool libm_ready_ = false;
+  std::vector<std::pair<std::string, hipEvent_t>> marks_;
+  std::vector<hipEvent_t> event_pool_;
+};
+
+// Process-wide context for a device (created on first use).
+Ctx& ctx_for_device(int device);
+
+}  // namespace cpsz

@@ -1,0 +1,164 @@
+#include "huff_gpu.hpp"
+
+#include <cstring>
+
+namespace cpsz {
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__global__ void k_scatter_codebook(const uint32_t* syms, const uint64_t* codes, const uint8_t* lens,
+                                   int k, uint64_t* code, uint8_t* len) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  code[syms[i]] = codes[i];
+  len[syms[i]] = lens[i];
+}
+
+__global__ void k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits, uint32_t S,
+                           uint32_t nseg, const uint64_t* __restrict__ prev_exit,
+                           const uint64_t* __restrict__ prev_start, uint64_t* __restrict__ start,
+                           uint64_t* __restrict__ exitp, uint32_t* __restrict__ cnt,
+                           int* __restrict__ changed, int* __restrict__ err, int first) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  uint64_t s0;
+  if (first) {
+    s0 = (uint64_t)i * S;
+  } else {
+    s0 = i == 0 ? 0 : prev_exit[i - 1];
+    if (s0 == prev_start[i]) {  // unchanged start: keep the previous result
+      start[i] = s0;
+      exitp[i] = prev_exit[i];
+      return;
+    }
+  }
+  uint64_t end = (uint64_t)(i + 1) * S;
+  if (end > nbits) end = nbits;
+  uint64_t pos = s0;
+  uint32_t c = 0;
+  while (pos < end) {
+    if (dec_one(t, w, pos) == 0xFFFFFFFFu) {
+      *err = 1;
+      break;
+    }
+    c++;
+  }
+  start[i] = s0;
+  exitp[i] = pos;
+  cnt[i] = c;
+  (void)changed;
+}
+
+__global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t* start,
+                            int* changed) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nseg) return;
+  const uint64_t want = i == 0 ? 0 : exitp[i - 1];
+  if (start[i] != want) *changed = 1;
+}
+
+namespace {
+__global__ void k_total(const uint64_t* out, const uint64_t* in, size_t n, uint64_t* total) {
+  *total = n ? out[n - 1] + in[n - 1] : 0;
+}
+__global__ void k_total32(const uint64_t* out, const uint32_t* in, size_t n, uint64_t* total) {
+  *total = n ? out[n - 1] + in[n - 1] : 0;
+}
+struct U32ToU64 {
+  __device__ uint64_t operator()(uint32_t x) const { return x; }
+};
+}  // namespace
+
+void scan_u64(Ctx& ctx, const uint64_t* in, uint64_t* out, size_t n, uint64_t* d_total) {
+  if (n == 0) {
+    CPSZ_HIP(hipMemsetAsync(d_total, 0, 8, ctx.stream()));
+    return;
+  }
+  size_t tmp = 0;
+  CPSZ_HIP(rocprim::exclusive_scan(nullptr, tmp, in, out, (uint64_t)0, n, rocprim::plus<uint64_t>(),
+                                   ctx.stream()));
+  void* d_tmp = ctx.raw_buf("scan.tmp", tmp);
+  CPSZ_HIP(rocprim::exclusive_scan(d_tmp, tmp, in, out, (uint64_t)0, n, rocprim::plus<uint64_t>(),
+                                   ctx.stream()));
+  k_total<<<1, 1, 0, ctx.stream()>>>(out, in, n, d_total);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void scan_u32_to_u64(Ctx& ctx, const uint32_t* in, uint64_t* out, size_t n, uint64_t* d_total) {
+  if (n == 0) {
+    CPSZ_HIP(hipMemsetAsync(d_total, 0, 8, ctx.stream()));
+    return;
+  }
+  auto it = rocprim::make_transform_iterator(in, U32ToU64());
+  size_t tmp = 0;
+  CPSZ_HIP(rocprim::exclusive_scan(nullptr, tmp, it, out, (uint64_t)0, n, rocprim::plus<uint64_t>(),
+                                   ctx.stream()));
+  void* d_tmp = ctx.raw_buf("scan.tmp", tmp);
+  CPSZ_HIP(rocprim::exclusive_scan(d_tmp, tmp, it, out, (uint64_t)0, n, rocprim::plus<uint64_t>(),
+                                   ctx.stream()));
+  k_total32<<<1, 1, 0, ctx.stream()>>>(out, in, n, d_total);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void upload_codebook(Ctx& ctx, const HuffEncodeTable& t, uint64_t* d_code, uint8_t* d_len,
+                     const char* tag) {
+  const int k = (int)t.syms.size();
+  std::string s(tag);
+  const size_t bytes = k * (4 + 8 + 1);
+  uint8_t* h = (uint8_t*)ctx.host_buf((s + ".cb").c_str(), bytes);
+  std::memcpy(h, t.syms.data(), 4 * k);
+  std::memcpy(h + 4 * k, t.codes.data(), 8 * k);
+  std::memcpy(h + 12 * k, t.lens.data(), k);
+  uint8_t* d = ctx.buf<uint8_t>((s + ".cb").c_str(), bytes + 16);
+  // keep the u64 array 8-byte aligned on the device: place codes first
+  uint64_t* dc = (uint64_t*)d;
+  uint32_t* ds = (uint32_t*)(d + 8 * (size_t)k);
+  uint8_t* dl = d + 12 * (size_t)k;
+  CPSZ_HIP(hipMemcpyAsync(dc, h + 4 * k, 8 * (size_t)k, hipMemcpyHostToDevice, ctx.stream()));
+  CPSZ_HIP(hipMemcpyAsync(ds, h, 4 * (size_t)k, hipMemcpyHostToDevice, ctx.stream()));
+  CPSZ_HIP(hipMemcpyAsync(dl, h + 12 * k, (size_t)k, hipMemcpyHostToDevice, ctx.stream()));
+  k_scatter_codebook<<<(int)div_up(k, 256), 256, 0, ctx.stream()>>>(ds, dc, dl, k, d_code, d_len);
+  CPSZ_HIP(hipGetLastError());
+}
+
+DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
+                             const char* tag) {
+  HuffDecodeTree d;
+  CPSZ_REQUIRE(huff_parse_tree(tree, tree_size, node_count, d), kErrStream,
+               "malformed huffman tree in stream");
+  DeviceDecoder out;
+  if (d.root_leaf) {
+    out.single = true;
+    out.single_sym = d.root_sym;
+    return out;
+  }
+  std::vector<uint32_t> lut;
+  huff_build_lut(d, kLutBits, lut);
+  const size_t N = d.node_count;
+  std::string s(tag);
+  const size_t lut_b = lut.size() * 4, arr_b = N * 4;
+  const size_t bytes = lut_b + 3 * arr_b + N;
+  uint8_t* h = (uint8_t*)ctx.host_buf((s + ".dec").c_str(), bytes);
+  std::memcpy(h, lut.data(), lut_b);
+  std::memcpy(h + lut_b, d.left.data(), arr_b);
+  std::memcpy(h + lut_b + arr_b, d.right.data(), arr_b);
+  std::memcpy(h + lut_b + 2 * arr_b, d.sym.data(), arr_b);
+  std::memcpy(h + lut_b + 3 * arr_b, d.leaf.data(), N);
+  uint8_t* dv = ctx.buf<uint8_t>((s + ".dec").c_str(), bytes);
+  CPSZ_HIP(hipMemcpyAsync(dv, h, bytes, hipMemcpyHostToDevice, ctx.stream()));
+  out.tree.lut = (const uint32_t*)dv;
+  out.tree.left = (const uint32_t*)(dv + lut_b);
+  out.tree.right = (const uint32_t*)(dv + lut_b + arr_b);
+  out.tree.sym = (const uint32_t*)(dv + lut_b + 2 * arr_b);
+  out.tree.leaf = dv + lut_b + 3 * arr_b;
+  out.tree.nodes = (uint32_t)N;
+  // The pinned staging buffer is reused by the next decoder upload of the same
+  // tag only after the stream has consumed it (callers synchronize per call).
+  return out;
+}
+
+}  // namespace cpsz

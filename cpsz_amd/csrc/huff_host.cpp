@@ -1,0 +1,209 @@
+#include "huff_host.hpp"
+
+#include <cstring>
+#include <string>
+
+#include "common.hpp"
+
+namespace cpsz {
+
+namespace {
+
+struct Node {
+  uint64_t freq;
+  int32_t left, right;  // -1 for leaves
+  uint32_t sym;
+};
+
+// Priority queue with the reference's exact discipline: 1-based heap; insert
+// stops at a parent with freq <= new; remove moves the smaller child up along
+// one path (right child only if strictly smaller) and drops the last element
+// into the vacated leaf slot without re-sifting it (src/sz_huffman.cpp:119-144).
+class Queue {
+ public:
+  Queue(const std::vector<Node>& nodes, size_t cap) : nodes_(nodes), heap_(cap + 2) {}
+  int size() const { return end_ - 1; }
+  void push(int id) {
+    int i = end_++;
+    for (int p = i >> 1; p; p = i >> 1) {
+      if (nodes_[heap_[p]].freq <= nodes_[id].freq) break;
+      heap_[i] = heap_[p];
+      i = p;
+    }
+    heap_[i] = id;
+  }
+  int pop() {
+    const int top = heap_[1];
+    --end_;
+    int i = 1;
+    for (int c = 2; c < end_; c = i << 1) {
+      if (c + 1 < end_ && nodes_[heap_[c + 1]].freq < nodes_[heap_[c]].freq) ++c;
+      heap_[i] = heap_[c];
+      i = c;
+    }
+    heap_[i] = heap_[end_];
+    return top;
+  }
+  int top() const { return heap_[1]; }
+
+ private:
+  const std::vector<Node>& nodes_;
+  std::vector<int> heap_;
+  int end_ = 1;
+};
+
+struct Serializer {
+  const std::vector<Node>& nodes;
+  HuffEncodeTable& t;
+  size_t N, w;
+  uint8_t *L, *R, *C, *T;
+  uint32_t counter = 0;
+
+  void put_idx(uint8_t* base, size_t slot, uint32_t v) {
+    if (w == 1) base[slot] = (uint8_t)v;
+    else if (w == 2) { uint16_t x = (uint16_t)v; std::memcpy(base + 2 * slot, &x, 2); }
+    else std::memcpy(base + 4 * slot, &v, 4);
+  }
+  // Preorder: node, left subtree, right subtree (pad_tree_*, :357-415);
+  // codes: left appends 0, right appends 1 (build_code, :152-187).
+  void visit(int id, uint32_t slot, int len, uint64_t bits) {
+    CPSZ_REQUIRE(slot < N, kErrInternal, "huffman tree larger than node_count");
+    const Node& nd = nodes[id];
+    if (nd.left < 0) {
+      std::memcpy(C + 4 * (size_t)slot, &nd.sym, 4);
+      T[slot] = 1;
+      CPSZ_REQUIRE(len <= 64, kErrUnsupported, "huffman code longer than 64 bits");
+      t.syms.push_back(nd.sym);
+      t.codes.push_back(len ? (bits << (64 - len)) : 0);
+      t.lens.push_back((uint8_t)len);
+      if (len > t.max_len) t.max_len = len;
+      return;
+    }
+    const uint32_t ls = ++counter;
+    put_idx(L, slot, ls);
+    visit(nd.left, ls, len + 1, bits << 1);
+    const uint32_t rs = ++counter;
+    put_idx(R, slot, rs);
+    visit(nd.right, rs, len + 1, (bits << 1) | 1);
+  }
+};
+
+}  // namespace
+
+void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEncodeTable& t) {
+  t = HuffEncodeTable();
+  t.state_num = state_num;
+  size_t leaves = 0;
+  for (size_t s = 0; s < nbins; s++)
+    if (freq[s]) {
+      CPSZ_REQUIRE(s < 2 * (size_t)state_num, kErrInternal, "huffman symbol out of range");
+      leaves++;
+    }
+  CPSZ_REQUIRE(leaves > 0, kErrInternal, "huffman: empty input");
+  std::vector<Node> nodes;
+  nodes.reserve(2 * leaves);
+  Queue q(nodes, 2 * leaves);
+  for (size_t s = 0; s < nbins; s++) {
+    if (!freq[s]) continue;
+    nodes.push_back(Node{freq[s], -1, -1, (uint32_t)s});
+    q.push((int)nodes.size() - 1);
+  }
+  // Merge: the reference passes two qremove() calls as arguments to
+  // new_node(.., left, right) (:211); its GCC build evaluates them right to
+  // left, so the first removal becomes the right child.
+  while (q.size() > 1) {
+    const int r = q.pop();
+    const int l = q.pop();
+    nodes.push_back(Node{nodes[l].freq + nodes[r].freq, l, r, 0});
+    q.push((int)nodes.size() - 1);
+  }
+  const int root = q.top();
+  // node_count counts leaves below state_num (src/sz_compression_utils.cpp:66-68).
+  size_t counted = 0;
+  for (size_t s = 0; s < nbins && s < state_num; s++) counted += freq[s] != 0;
+  CPSZ_REQUIRE(counted == leaves, kErrUnsupported, "symbol beyond the coder's state count");
+  t.node_count = 2 * counted - 1;
+  t.single = nodes[root].left < 0;
+  const size_t N = t.node_count;
+  const size_t w = N <= 256 ? 1 : (N <= 65536 ? 2 : 4);
+  t.tree.assign(1 + 2 * N * w + 5 * N, 0);
+  Serializer s{nodes, t, N, w, nullptr, nullptr, nullptr, nullptr};
+  s.L = t.tree.data() + 1;
+  s.R = s.L + N * w;
+  s.C = s.R + N * w;
+  s.T = s.C + 4 * N;
+  s.visit(root, 0, 0, 0);
+}
+
+bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecodeTree& d) {
+  d = HuffDecodeTree();
+  if (N == 0 || N > (1u << 24)) return false;
+  const size_t w = N <= 256 ? 1 : (N <= 65536 ? 2 : 4);
+  if (tree_size < 1 + 2 * N * w + 5 * N) return false;
+  const uint8_t* L = tree + 1;
+  const uint8_t* R = L + N * w;
+  const uint8_t* C = R + N * w;
+  const uint8_t* T = C + 4 * N;
+  auto idx = [&](const uint8_t* b, size_t i) -> uint32_t {
+    if (w == 1) return b[i];
+    if (w == 2) { uint16_t v; std::memcpy(&v, b + 2 * i, 2); return v; }
+    uint32_t v; std::memcpy(&v, b + 4 * i, 4); return v;
+  };
+  d.node_count = N;
+  d.left.resize(N);
+  d.right.resize(N);
+  d.sym.resize(N);
+  d.leaf.resize(N);
+  for (size_t i = 0; i < N; i++) {
+    d.left[i] = idx(L, i);
+    d.right[i] = idx(R, i);
+    std::memcpy(&d.sym[i], C + 4 * i, 4);
+    d.leaf[i] = T[i] != 0;
+  }
+  // Root: a u8-indexed tree takes its root's leaf flag from the stream; wider
+  // trees always start from an internal root
+  // (reconstruct_HuffTree_from_bytes_anyStates, src/sz_huffman.cpp:588,615,641).
+  d.root_leaf = (w == 1) && d.leaf[0];
+  d.root_sym = d.sym[0];
+  if (d.root_leaf) return true;
+  d.leaf[0] = 0;
+  // Validate reachability: children must point forward (preorder numbering)
+  // and internal nodes need both children for a well-formed code.
+  std::vector<std::pair<uint32_t, int>> st{{0u, 0}};
+  size_t visited = 0;
+  while (!st.empty()) {
+    auto [i, depth] = st.back();
+    st.pop_back();
+    if (++visited > N || depth > 64) return false;
+    if (d.leaf[i]) {
+      if (depth > d.max_len) d.max_len = depth;
+      continue;
+    }
+    const uint32_t l = d.left[i], r = d.right[i];
+    if (l == 0 || r == 0 || l >= N || r >= N || l <= i || r <= i) return false;
+    st.push_back({r, depth + 1});
+    st.push_back({l, depth + 1});
+  }
+  return true;
+}
+
+void huff_build_lut(const HuffDecodeTree& d, int bits, std::vector<uint32_t>& lut) {
+  lut.assign((size_t)1 << bits, 0);
+  for (uint32_t v = 0; v < (1u << bits); v++) {
+    uint32_t node = 0;
+    int len = 0;
+    uint32_t entry = 0;
+    for (; len < bits; len++) {
+      const int b = (v >> (bits - 1 - len)) & 1;
+      node = b ? d.right[node] : d.left[node];
+      if (d.leaf[node]) {
+        entry = (d.sym[node] << 8) | (uint32_t)(len + 1);
+        break;
+      }
+    }
+    if (len == bits) entry = node << 8;  // long code: continue from `node`
+    lut[v] = entry;
+  }
+}
+
+}  // namespace cpsz

@@ -1,0 +1,49 @@
+// Host side of the Huffman stage: tree construction from a device-computed
+// histogram, code assignment, tree serialization, and the parsed decode form.
+//
+// The stream format and the tree shape must match the reference coder
+// (src/sz_huffman.cpp): the tree is built with the reference's priority-queue
+// discipline (:119-144, :195-215), codes are assigned left=0/right=1
+// (:152-187), and the tree is serialized in preorder with u8/u16/u32 child
+// indices chosen by node count (:357-499).
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace cpsz {
+
+struct HuffEncodeTable {
+  uint32_t state_num = 0;
+  size_t node_count = 0;              // 2 * leaves - 1
+  std::vector<uint8_t> tree;          // serialized tree bytes
+  // present symbols with their codes (left-aligned in 64 bits) and lengths
+  std::vector<uint32_t> syms;
+  std::vector<uint64_t> codes;
+  std::vector<uint8_t> lens;
+  int max_len = 0;
+  bool single = false;                // one-symbol alphabet: empty bitstream
+};
+
+// Build from frequencies over symbols [0, nbins). Symbols must be < 2*state_num.
+void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEncodeTable& t);
+
+// Parsed tree for decoding: node arrays in serialized (preorder) numbering.
+struct HuffDecodeTree {
+  size_t node_count = 0;
+  std::vector<uint32_t> left, right, sym;  // child index 0 = none
+  std::vector<uint8_t> leaf;
+  bool root_leaf = false;                  // one-symbol alphabet
+  uint32_t root_sym = 0;
+  int max_len = 0;
+};
+
+// Parses `tree_size` bytes of a serialized tree for `node_count` nodes.
+// Returns false on malformed input (index out of range, cycles, bad leaves).
+bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t node_count, HuffDecodeTree& d);
+
+// Lookup table of 2^bits entries for the first `bits` stream bits:
+// entry = sym << 8 | len for codes of length <= bits; for longer codes,
+// entry = 0x80 | node (internal node reached after `bits` bits) << 8 with len 0.
+void huff_build_lut(const HuffDecodeTree& d, int bits, std::vector<uint32_t>& lut);
+
+}  // namespace cpsz

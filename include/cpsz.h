@@ -1,0 +1,143 @@
+/*
+ * cpsz — critical-point-preserving lossy compression of 2D/3D float vector
+ * fields on AMD Instinct MI355X (gfx950). C ABI of libcpsz_hip.so.
+ *
+ * Streams are byte-compatible with the reference CPU library (bzhan166/cpSZ):
+ * a stream written here decodes with the reference decoder and vice versa.
+ * ZSTD stays a separate host-side stage, as in the reference drivers.
+ *
+ * All entry points return 0 on success or a negative CPSZ_ERR_* code; the
+ * message of the calling thread's last failure is cpsz_last_error(). Calls are
+ * thread-safe: each host thread gets its own per-device context (HIP stream and
+ * scratch buffers).
+ */
+#ifndef CPSZ_H_
+#define CPSZ_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CPSZ_OK = 0,
+  CPSZ_ERR_ARG = -1,
+  CPSZ_ERR_HIP = -2,
+  CPSZ_ERR_ALLOC = -3,
+  CPSZ_ERR_STREAM = -4,
+  CPSZ_ERR_UNSUPPORTED = -5,
+  CPSZ_ERR_NONFINITE = -6,
+  CPSZ_ERR_INTERNAL = -7
+};
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* cpsz_last_error(void);
+/* Library version string. */
+const char* cpsz_version(void);
+/* Number of visible HIP devices (0 when none). */
+int cpsz_device_count(void);
+/* Releases a buffer returned by this library. */
+void cpsz_free(void* p);
+
+/* ---- host-buffer codec (what the drop-in C++ templates call) ---------- */
+
+/* Replaces sz_compress_cp_preserve_2d_offline<float>
+ * (reference include/sz_compress_cp_preserve_2d.hpp:8-10; implementation
+ * src/sz_compress_cp_preserve_2d.cpp:175-274). U, V: r1 x r2 row-major host
+ * arrays. *out is malloc'd (release with free() or cpsz_free). */
+int cpsz_compress_2d_f32(const float* U, const float* V, size_t r1, size_t r2, double max_pwr_eb,
+                         unsigned char** out, size_t* out_size);
+
+/* Replaces sz_decompress_cp_preserve_2d_offline<float>
+ * (reference include/sz_decompress_cp_preserve_2d.hpp:8-11; implementation
+ * src/sz_decompress_cp_preserve_2d.cpp:8-44). U, V: caller buffers of r1*r2. */
+int cpsz_decompress_2d_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2,
+                           float* U, float* V);
+
+/* Replaces sz_compress_cp_preserve_3d_offline_log<float>
+ * (reference include/sz_compress_cp_preserve_3d.hpp:6-8; implementation
+ * src/sz_compress_cp_preserve_3d.cpp:360-581). r1 slowest .. r3 fastest. */
+int cpsz_compress_3d_f32(const float* U, const float* V, const float* W, size_t r1, size_t r2,
+                         size_t r3, double max_pwr_eb, unsigned char** out, size_t* out_size);
+
+/* Replaces sz_decompress_cp_preserve_3d_online_log<float>
+ * (reference include/sz_decompress_cp_preserve_3d.hpp:8-11; implementation
+ * src/sz_decompress_cp_preserve_3d.cpp:7-110). */
+int cpsz_decompress_3d_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2,
+                           size_t r3, float* U, float* V, float* W);
+
+/* Total byte size of a stream from its own framing (the reference decoders
+ * take no size argument; these read the section headers to find the end). */
+int cpsz_stream_size_2d(const unsigned char* in, size_t* size);
+int cpsz_stream_size_3d(const unsigned char* in, size_t r1, size_t r2, size_t r3, size_t* size);
+
+/* ---- device-resident codec ------------------------------------------------
+ * Inputs and outputs live in HBM of `device`; `stream` is a hipStream_t (NULL:
+ * the context's own stream). The compressed stream is returned in a
+ * library-owned device buffer valid until the next compress call of this
+ * thread on this device. */
+int cpsz_compress_2d_f32_dev(int device, void* stream, const float* dU, const float* dV,
+                             size_t r1, size_t r2, double max_pwr_eb,
+                             const unsigned char** d_out, size_t* out_size);
+int cpsz_decompress_2d_f32_dev(int device, void* stream, const unsigned char* d_in,
+                               size_t in_size, size_t r1, size_t r2, float* dU, float* dV);
+int cpsz_compress_3d_f32_dev(int device, void* stream, const float* dU, const float* dV,
+                             const float* dW, size_t r1, size_t r2, size_t r3, double max_pwr_eb,
+                             const unsigned char** d_out, size_t* out_size);
+int cpsz_decompress_3d_f32_dev(int device, void* stream, const unsigned char* d_in,
+                               size_t in_size, size_t r1, size_t r2, size_t r3, float* dU,
+                               float* dV, float* dW);
+
+/* ---- host lossless stage ---------------------------------------------------
+ * Replaces sz_lossless_compress / sz_lossless_decompress with ZSTD
+ * (reference src/sz_lossless.cpp:11-45; level 3 in the drivers). */
+int cpsz_lossless_compress(int level, const unsigned char* in, size_t n, unsigned char** out,
+                           size_t* out_size);
+int cpsz_lossless_decompress(const unsigned char* in, size_t n, unsigned char* out,
+                             size_t out_size);
+
+/* ---- critical points (0-CP-error judge) ----------------------------------
+ * Critical points per simplex, decided with exact sign predicates on the GPU.
+ * 2D: interior cells (reference test/cp_extraction_2d.cpp:126-145), ids
+ * 2*(i*(r2-1)+j)(+1); 3D: all cells (test/cp_extraction_3d.cpp:174-206), ids
+ * 6*(vertex index)+tet. Writes up to `cap` (id, type) pairs sorted by id and
+ * the total count to *count. */
+int cpsz_critical_points_2d_dev(int device, void* stream, const float* dU, const float* dV,
+                                size_t r1, size_t r2, int64_t* ids, int32_t* types, size_t cap,
+                                size_t* count);
+int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const float* dV,
+                                const float* dW, size_t r1, size_t r2, size_t r3, int64_t* ids,
+                                int32_t* types, size_t cap, size_t* count);
+
+/* ---- host-side diagnostics (no GPU needed) -------------------------------- */
+/* Copies the eb-id threshold tables the device kernels use: q2d[k] is the
+ * smallest |U|*eb with 2D id >= k, q3d[k] the smallest vertex bound with 3D
+ * id >= k (both from this host's libm, as the reference computes ids). Returns
+ * the table lengths through n2d / n3d. */
+int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int* n3d);
+/* Builds the Huffman tree the encoder would emit for the given symbol
+ * frequencies and writes the serialized tree bytes and node count. */
+int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
+                      size_t cap, size_t* tree_size, size_t* node_count);
+
+/* Host emulation of the device 2D Lorenzo arithmetic (row-major order):
+ * eb codes c (0 = zero precision, else reference id + 1) -> quant codes q and
+ * the reconstruction. `wide` selects the double-precision path used for
+ * ids above 81. */
+int cpsz_lorenzo2d_emulate(const float* x, const uint16_t* c, size_t r1, size_t r2, int wide,
+                           uint16_t* q, float* recon);
+
+/* ---- instrumentation ------------------------------------------------------ */
+/* Enables per-stage HIP-event timing for this thread's context on `device`. */
+int cpsz_set_timing(int device, int on);
+/* Stage timings of the last call: names joined by ';' into `names`, times in
+ * ms into `ms`; returns the number of stages (or a negative error). */
+int cpsz_last_stages(int device, char* names, size_t names_cap, float* ms, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CPSZ_H_ */
