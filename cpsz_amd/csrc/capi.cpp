@@ -12,6 +12,7 @@
 #include "codec3d.hpp"
 #include "cp.hpp"
 #include "ctx.hpp"
+#include "eb3d_math.hpp"
 #include "huff_host.hpp"
 
 namespace cpsz {
@@ -322,6 +323,47 @@ int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, ui
     std::memcpy(tree, t.tree.data(), t.tree.size());
     *tree_size = t.tree.size();
     *node_count = t.node_count;
+  });
+}
+
+int cpsz_prepare_libm_tables(const char* path) {
+  return guarded([&] { libm_prepare(path ? std::string(path) : libm_cache_path()); });
+}
+
+int cpsz_eval_log2f(const float* x, size_t n, float* out) {
+  return guarded([&] {
+    const LibmView v = libm_tables().log2f.view();
+    for (size_t i = 0; i < n; i++) out[i] = host_log2f(v, x[i]);
+  });
+}
+
+int cpsz_eval_exp2f(const float* y, size_t n, float* out) {
+  return guarded([&] {
+    const LibmView v = libm_tables().exp2f.view();
+    for (size_t i = 0; i < n; i++) out[i] = host_exp2f(v, y[i]);
+  });
+}
+
+int cpsz_eb3d_emulate(const float* U, const float* V, const float* W, size_t r1, size_t r2,
+                      size_t r3, double max_pwr_eb, double* eb) {
+  return guarded([&] {
+    CPSZ_REQUIRE(U && V && W && eb && r1 && r2 && r3, kErrArg, "bad arguments");
+    eb3d_emulate(U, V, W, r1, r2, r3, max_pwr_eb, eb);
+  });
+}
+
+int cpsz_lorenzo3d_emulate(const float* L, const uint8_t* code, size_t r1, size_t r2, size_t r3,
+                           uint8_t* vid, uint16_t* q) {
+  return guarded([&] {
+    CPSZ_REQUIRE(L && code && vid && q && r1 && r2 && r3, kErrArg, "bad arguments");
+    lorenzo3d_emulate(L, code, r1, r2, r3, vid, q);
+  });
+}
+
+int cpsz_eb_code_3d(const double* eb, size_t n, uint8_t* code) {
+  return guarded([&] {
+    const auto& q = eb_tables().q3d;
+    for (size_t i = 0; i < n; i++) code[i] = (uint8_t)eb_code_3d(eb[i], q.data(), (int)q.size());
   });
 }
 

@@ -1,0 +1,780 @@
+// 3D offline_log codec on gfx950: the drop-in for
+// sz_compress_cp_preserve_3d_offline_log / sz_decompress_cp_preserve_3d_online_log.
+//
+// Compress (inputs in HBM):
+//   k_eb3d       per-cell 6-tet bounds for a 4x8x64 vertex tile (+1-cell halo)
+//                folded into the tile's vertex minima with LDS u64 atomic-min
+//                (non-negative doubles order like their bit patterns), then
+//                the id code from the host-libm threshold table
+//   k_signmap    three MSB-first sign bitmaps
+//   k_log        host-libm-exact log2f of |x| (-100 for zeros)
+//   k_lorenzo3d  wavefront over items (plane i, 64-row j block): lane = j, the
+//                wave sweeps k with a one-step skew per lane; j-1 values arrive
+//                through DPP wave_shr:1, plane i-1 values through agent-coherent
+//                loads of the reconstruction with progress counters
+//   Huffman      ids (512 states) and the codes of predictable vertices
+// Decompress mirrors it and finishes with host-libm-exact exp2f.
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "codec3d.hpp"
+#include "eb3d_math.hpp"
+#include "huff_gpu.hpp"
+#include "libm_math.hpp"
+
+namespace cpsz {
+
+namespace {
+
+constexpr int kTI = 4, kTJ = 8, kTK = 64;  // vertex tile of the eb kernel
+
+__global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
+                                              const float* __restrict__ W, int r1, int r2, int r3,
+                                              double max_pwr_eb, const double* __restrict__ q3d, int nq,
+                                              uint8_t* __restrict__ code, int* __restrict__ flags) {
+  constexpr int VI = kTI + 2, VJ = kTJ + 2, VK = kTK + 2;  // vertex halo box
+  __shared__ float sv[3][VI][VJ][VK];
+  __shared__ unsigned long long seb[kTI][kTJ][kTK];
+  const int tid = threadIdx.x;
+  const int k0 = blockIdx.x * kTK, j0 = blockIdx.y * kTJ, i0 = blockIdx.z * kTI;
+  const size_t d0 = (size_t)r2 * r3, d1 = r3;
+  const float* src[3] = {U, V, W};
+  for (int e = tid; e < VI * VJ * VK; e += 256) {
+    const int a = e / (VJ * VK), b = (e / VK) % VJ, c = e % VK;
+    const int gi = i0 - 1 + a, gj = j0 - 1 + b, gk = k0 - 1 + c;
+    const bool in = gi >= 0 && gi < r1 && gj >= 0 && gj < r2 && gk >= 0 && gk < r3;
+    const size_t g = in ? gi * d0 + gj * d1 + gk : 0;
+    for (int p = 0; p < 3; p++) sv[p][a][b][c] = in ? src[p][g] : 0.f;
+  }
+  const unsigned long long init = (unsigned long long)__double_as_longlong(max_pwr_eb);
+  for (int e = tid; e < kTI * kTJ * kTK; e += 256) (&seb[0][0][0])[e] = init;
+  __syncthreads();
+  int bad = 0;
+  // cells [i0-1, i0+TI) x [j0-1, j0+TJ) x [k0-1, k0+TK)
+  constexpr int CI = kTI + 1, CJ = kTJ + 1, CK = kTK + 1;
+  for (int e = tid; e < CI * CJ * CK; e += 256) {
+    const int a = e / (CJ * CK), b = (e / CK) % CJ, c = e % CK;
+    const int ci = i0 - 1 + a, cj = j0 - 1 + b, ck = k0 - 1 + c;
+    if (ci < 0 || ci >= r1 - 1 || cj < 0 || cj >= r2 - 1 || ck < 0 || ck >= r3 - 1) continue;
+    // the cell origin sits at halo-box index (a, b, c)... vertex halo box starts at i0-1
+    for (int t = 0; t < 6; t++) {
+      int vi[4], vj[4], vk[4];
+      for (int v = 0; v < 3; v++) {
+        int di, dj, dk;
+        tet_offsets(t, v, di, dj, dk);
+        vi[v] = a + di;
+        vj[v] = b + dj;
+        vk[v] = c + dk;
+      }
+      vi[3] = a;
+      vj[3] = b;
+      vk[3] = c;
+      const double eb = tet_bound(sv[0][vi[0]][vj[0]][vk[0]], sv[0][vi[1]][vj[1]][vk[1]],
+                                  sv[0][vi[2]][vj[2]][vk[2]], sv[0][vi[3]][vj[3]][vk[3]],
+                                  sv[1][vi[0]][vj[0]][vk[0]], sv[1][vi[1]][vj[1]][vk[1]],
+                                  sv[1][vi[2]][vj[2]][vk[2]], sv[1][vi[3]][vj[3]][vk[3]],
+                                  sv[2][vi[0]][vj[0]][vk[0]], sv[2][vi[1]][vj[1]][vk[1]],
+                                  sv[2][vi[2]][vj[2]][vk[2]], sv[2][vi[3]][vj[3]][vk[3]]);
+      if (!(eb == eb)) bad = 1;
+      const unsigned long long bits = (unsigned long long)__double_as_longlong(eb);
+      for (int v = 0; v < 4; v++) {
+        // halo-box index -> tile index (tile vertex (x) = halo (x+1))
+        const int ti = vi[v] - 1, tj = vj[v] - 1, tk = vk[v] - 1;
+        if (ti >= 0 && ti < kTI && tj >= 0 && tj < kTJ && tk >= 0 && tk < kTK)
+          atomicMin(&seb[ti][tj][tk], bits);
+      }
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < kTI * kTJ * kTK; e += 256) {
+    const int a = e / (kTJ * kTK), b = (e / kTK) % kTJ, c = e % kTK;
+    const int gi = i0 + a, gj = j0 + b, gk = k0 + c;
+    if (gi >= r1 || gj >= r2 || gk >= r3) continue;
+    const float x0 = sv[0][a + 1][b + 1][c + 1], x1 = sv[1][a + 1][b + 1][c + 1],
+                x2 = sv[2][a + 1][b + 1][c + 1];
+    if (!isfinite(x0) || !isfinite(x1) || !isfinite(x2)) bad = 1;
+    const double eb = __longlong_as_double((long long)seb[a][b][c]);
+    code[gi * d0 + gj * d1 + gk] = (uint8_t)eb_code_3d(eb, q3d, nq);
+  }
+  if (bad) atomicOr(flags, 1);
+}
+
+// Three MSB-first sign bitmaps, one 32-element word per thread.
+__global__ void k_signmap(const float* __restrict__ U, const float* __restrict__ V,
+                          const float* __restrict__ W, size_t n, size_t nw, uint32_t* __restrict__ out) {
+  const size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (w >= 3 * nw) return;
+  const int p = (int)(w / nw);
+  const size_t w0 = w - p * nw;
+  const float* X = p == 0 ? U : (p == 1 ? V : W);
+  uint32_t word = 0;
+  for (int b = 0; b < 32; b++) {
+    const size_t e = w0 * 32 + b;
+    if (e < n && X[e] > 0) word |= 1u << ((b & ~7) + 7 - (b & 7));
+  }
+  out[w] = word;
+}
+
+// log_transform (include/sz_cp_preserve_utils.hpp:205-221): log2f(|x|), -100 for 0.
+__global__ void k_log(const float* __restrict__ U, const float* __restrict__ V,
+                      const float* __restrict__ W, size_t n, LibmView lv, float* __restrict__ L) {
+  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (e >= 3 * n) return;
+  const int p = (int)(e / n);
+  const size_t i = e - p * n;
+  const float x = (p == 0 ? U : (p == 1 ? V : W))[i];
+  L[e] = x != 0 ? host_log2f(lv, fabsf(x)) : -100.f;
+}
+
+// ------------------------------------------------------------- wavefront --
+constexpr int kCH3 = 32;
+
+__device__ __forceinline__ int ld_coh(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float shr1(float src, float lane0) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(src),
+                                                    0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+}
+
+struct Wave3d {
+  const float* lx;       // compress: log values [3][n]; decompress: initial values of unpredictables
+  const uint8_t* code;   // compress: eb codes; decompress: stream ids
+  const uint16_t* qin;   // decompress: quant codes [3][n]
+  float* R;              // reconstruction [3][n] (log domain)
+  uint16_t* qout;        // compress: quant codes [3][n]
+  uint8_t* vid;          // compress: stream id per vertex
+  int* prog;             // [r1][nb] completed steps
+  int* ticket;
+  int r1, r2, r3, nb;
+};
+
+template <bool COMPRESS>
+__global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
+  const int lane = threadIdx.x;
+  const int r1 = a.r1, r2 = a.r2, r3 = a.r3, nb = a.nb;
+  const size_t n = (size_t)r1 * r2 * r3, d0 = (size_t)r2 * r3;
+  const int items = r1 * nb;
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(a.ticket, 1);
+    item = __shfl(item, 0);
+    if (item >= items) return;
+    const int i = item / nb, b = item - i * nb;
+    const int j = b * 64 + lane;
+    const bool jv = j < r2;
+    const size_t row = (size_t)i * d0 + (size_t)(jv ? j : 0) * r3;
+    const size_t urow = i ? row - d0 : 0;                                  // plane i-1, same j
+    const size_t brow = (size_t)i * d0 + (size_t)(b * 64 - 1) * r3;        // plane i, j0-1
+    const size_t ubrow = i ? brow - d0 : 0;                                // plane i-1, j0-1
+    const int* prog_up = a.prog + (i ? (i - 1) * nb + b : 0);
+    const int* prog_left = a.prog + i * nb + (b ? b - 1 : 0);
+    int* prog_me = a.prog + i * nb + b;
+    float Rp[3] = {0, 0, 0}, Bp[3] = {0, 0, 0}, Dp[3] = {0, 0, 0}, Fp[3] = {0, 0, 0};
+    const int steps = r3 + 63;
+    for (int t0 = 0; t0 < steps; t0 += kCH3) {
+      // wait for plane i-1 (this block) and, for lane 0's j-1 column, the block to the left
+      if (i > 0) {
+        const int need = min(t0 + kCH3, steps);
+        while (ld_coh(prog_up) < need) __builtin_amdgcn_s_sleep(2);
+      }
+      if (b > 0) {
+        const int need = min(t0 + kCH3 + 63, steps);
+        while (ld_coh(prog_left) < need) __builtin_amdgcn_s_sleep(2);
+      }
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      // lane k holds column t0 + k of the j0-1 rows (own plane, plane above)
+      float bl[3] = {0, 0, 0}, bu[3] = {0, 0, 0};
+      if (b > 0 && lane < kCH3 && t0 + lane < r3) {
+        for (int p = 0; p < 3; p++) {
+          bl[p] = __int_as_float(ld_coh((const int*)(a.R + p * n + brow + t0 + lane)));
+          if (i > 0) bu[p] = __int_as_float(ld_coh((const int*)(a.R + p * n + ubrow + t0 + lane)));
+        }
+      }
+      float xs[kCH3][3], ups[kCH3][3];
+      uint32_t cs[kCH3];
+      uint32_t qs[kCH3][3];
+#pragma unroll
+      for (int s = 0; s < kCH3; s++) {
+        const int k = t0 + s - lane;
+        const bool act = jv && k >= 0 && k < r3;
+        cs[s] = act ? a.code[row + k] : 0;
+        for (int p = 0; p < 3; p++) {
+          xs[s][p] = act ? a.lx[p * n + row + k] : 0.f;
+          ups[s][p] = (act && i > 0) ? __int_as_float(ld_coh((const int*)(a.R + p * n + urow + k))) : 0.f;
+          if (!COMPRESS) qs[s][p] = act ? a.qin[p * n + row + k] : 0;
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < kCH3; s++) {
+        const int k = t0 + s - lane;
+        const bool act = jv && k >= 0 && k < r3;
+        float res[3];
+        float dec[3];
+        bool okall = true;
+        uint16_t qc[3];
+        const uint32_t c = cs[s];
+        for (int p = 0; p < 3; p++) {
+          const float l0b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bl[p]), s));
+          const float l0u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bu[p]), s));
+          const float B = shr1(Rp[p], l0b);   // L(i, j-1, k)
+          const float F = shr1(Dp[p], l0u);   // L(i-1, j-1, k)
+          const float D = ups[s][p];          // L(i-1, j, k)
+          // pred = d0 + d3 + d5 + d6 - d1 - d2 - d4 (src/sz_compress_cp_preserve_3d.cpp:498)
+          const float pred = Fp[p] + D + B + Rp[p] - F - Dp[p] - Bp[p];
+          if (COMPRESS) {
+            bool ok;
+            dec[p] = l3d_step(pred, xs[s][p], (int)c, ok, qc[p]);
+            okall = okall && ok;
+          } else {
+            const uint32_t id = c;
+            if (id == 0 || id == 24) {
+              res[p] = xs[s][p];
+            } else {
+              res[p] = pred + ldexpf((float)(2 * ((int)qs[s][p] - 32768)), (int)id - 23);
+            }
+          }
+          Bp[p] = B;
+          Fp[p] = F;
+          Dp[p] = D;
+        }
+        if (COMPRESS) {
+          const bool pred_ok = c != 255 && c != 0 && okall;
+          for (int p = 0; p < 3; p++) res[p] = pred_ok ? dec[p] : xs[s][p];
+          if (act) {
+            a.vid[row + k] = (uint8_t)(c == 255 ? 0 : (pred_ok ? c : 24));
+            if (pred_ok)
+              for (int p = 0; p < 3; p++) a.qout[p * n + row + k] = qc[p];
+          }
+        }
+        for (int p = 0; p < 3; p++) {
+          if (act) __hip_atomic_store((int*)(a.R + p * n + row + k), __float_as_int(res[p]),
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          Rp[p] = k < 0 ? 0.f : res[p];
+        }
+      }
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      if (lane == 0) st_coh(prog_me, min(t0 + kCH3, steps));
+    }
+  }
+}
+
+__global__ void k_zero_i32_3d(int* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+// ---------------------------------------------------------- huffman ------
+constexpr int kVChunk = 256;  // vertices per chunk
+
+struct SrcIds3d {
+  const uint8_t* vid;
+  size_t n;
+  template <class F>
+  __device__ void each(size_t c, F f) const {
+    const size_t b = c * kVChunk, e = min(b + kVChunk, n);
+    for (size_t v = b; v < e; v++) f((uint32_t)vid[v]);
+  }
+};
+
+__device__ __forceinline__ bool is_pred3(uint32_t id) { return id != 0 && id != 24; }
+
+struct SrcCodes3d {
+  const uint8_t* vid;
+  const uint16_t* q;
+  size_t n;
+  template <class F>
+  __device__ void each(size_t c, F f) const {
+    const size_t b = c * kVChunk, e = min(b + kVChunk, n);
+    for (size_t v = b; v < e; v++)
+      if (is_pred3(vid[v])) {
+        f((uint32_t)q[v]);
+        f((uint32_t)q[n + v]);
+        f((uint32_t)q[2 * n + v]);
+      }
+  }
+};
+
+__global__ void k_chunk_counts3d(const uint8_t* __restrict__ vid, size_t n, size_t nch,
+                                 uint32_t* __restrict__ npred, uint32_t* __restrict__ nunp) {
+  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
+  uint32_t p = 0;
+  for (size_t v = b; v < e; v++) p += is_pred3(vid[v]);
+  npred[c] = p;
+  nunp[c] = (uint32_t)(e - b) - p;
+}
+
+__global__ void k_gather_unpred3d(const uint8_t* __restrict__ vid, const float* __restrict__ U,
+                                  const float* __restrict__ V, const float* __restrict__ W, size_t n,
+                                  size_t nch, const uint64_t* __restrict__ uoff, float* __restrict__ out) {
+  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
+  uint64_t o = 3 * uoff[c];
+  for (size_t v = b; v < e; v++)
+    if (!is_pred3(vid[v])) {
+      out[o++] = U[v];
+      out[o++] = V[v];
+      out[o++] = W[v];
+    }
+}
+
+struct SinkU8 {
+  uint8_t* p;
+  __device__ void put(uint64_t i, uint32_t s) const { p[i] = (uint8_t)s; }
+};
+struct SinkU16 {
+  uint16_t* p;
+  __device__ void put(uint64_t i, uint32_t s) const { p[i] = (uint16_t)s; }
+};
+
+// decompress: per-vertex codes and initial values of unpredictable vertices
+__global__ void k_scatter3d(const uint8_t* __restrict__ vid, const uint16_t* __restrict__ qc,
+                            const float* __restrict__ unp, size_t n, size_t nch,
+                            const uint64_t* __restrict__ poff, const uint64_t* __restrict__ uoff,
+                            uint64_t nq, uint64_t nu, LibmView lv, uint16_t* __restrict__ q,
+                            float* __restrict__ init, int* __restrict__ err) {
+  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
+  uint64_t po = 3 * poff[c], uo = 3 * uoff[c];
+  for (size_t v = b; v < e; v++) {
+    if (is_pred3(vid[v])) {
+      if (po + 3 > nq) {
+        *err = 1;
+        return;
+      }
+      q[v] = qc[po];
+      q[n + v] = qc[po + 1];
+      q[2 * n + v] = qc[po + 2];
+      po += 3;
+    } else {
+      if (uo + 3 > nu) {
+        *err = 1;
+        return;
+      }
+      for (int p = 0; p < 3; p++) {
+        const float x = unp[uo + p];
+        init[p * n + v] = (x == 0) ? -100.f : host_log2f(lv, fabsf(x));
+      }
+      uo += 3;
+    }
+  }
+}
+
+// final values (src/sz_decompress_cp_preserve_3d.cpp:89-104)
+__global__ void k_finish3d(const uint8_t* __restrict__ vid, const float* __restrict__ R,
+                           const uint8_t* __restrict__ smap, size_t smap_bytes,
+                           const float* __restrict__ unp, size_t n, size_t nch,
+                           const uint64_t* __restrict__ uoff, LibmView ev, float* __restrict__ U,
+                           float* __restrict__ V, float* __restrict__ W) {
+  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (c >= nch) return;
+  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
+  uint64_t uo = 3 * uoff[c];
+  float* out[3] = {U, V, W};
+  for (size_t v = b; v < e; v++) {
+    if (!is_pred3(vid[v])) {
+      for (int p = 0; p < 3; p++) out[p][v] = unp[uo + p];
+      uo += 3;
+    } else {
+      for (int p = 0; p < 3; p++) {
+        const float x = R[p * n + v];
+        const bool pos = (smap[p * smap_bytes + v / 8] >> (7 - (v % 8))) & 1;
+        float y;
+        if (x < -99) y = 0;
+        else {
+          const float t = host_exp2f(ev, x);
+          y = pos ? t : -t;
+        }
+        out[p][v] = y;
+      }
+    }
+  }
+}
+
+template <typename V>
+void put(std::vector<uint8_t>& o, const V& v) {
+  const size_t at = o.size();
+  o.resize(at + sizeof(V));
+  std::memcpy(o.data() + at, &v, sizeof(V));
+}
+
+void launch_wave3d(Ctx& ctx, Wave3d& w, bool compress) {
+  hipStream_t st = ctx.stream();
+  const size_t nz = (size_t)w.r1 * w.nb + 1;
+  k_zero_i32_3d<<<(int)std::min<size_t>(div_up(nz, 256), 1024), 256, 0, st>>>(w.prog, nz);
+  const int items = w.r1 * w.nb;
+  const int grid = items < 4096 ? items : 4096;
+  if (compress) k_lorenzo3d<true><<<grid, 64, 0, st>>>(w);
+  else k_lorenzo3d<false><<<grid, 64, 0, st>>>(w);
+  CPSZ_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1,
+                        size_t r2, size_t r3, double max_pwr_eb, DeviceStream& out) {
+  CPSZ_REQUIRE(dU && dV && dW && r1 && r2 && r3, kErrArg, "compress_3d: bad arguments");
+  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 34),
+               kErrUnsupported, "compress_3d: field too large");
+  hipStream_t st = ctx.stream();
+  const size_t n = r1 * r2 * r3;
+  ctx.stages.clear();
+  ctx.mark("start");
+  const Ctx::LibmDev& lm = ctx.libm_dev();
+  uint8_t* ebc = ctx.buf<uint8_t>("c3.ebc", n);
+  int* flags = ctx.buf<int>("c3.flags", 4);
+  CPSZ_HIP(hipMemsetAsync(flags, 0, 16, st));
+  {
+    dim3 grid((unsigned)div_up(r3, kTK), (unsigned)div_up(r2, kTJ), (unsigned)div_up(r1, kTI));
+    k_eb3d<<<grid, 256, 0, st>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
+                                 ctx.q3d_size(), ebc, flags);
+    CPSZ_HIP(hipGetLastError());
+  }
+  ctx.mark("eb");
+  const size_t smap = (n - 1) / 8 + 1, nw = div_up(n, 32);
+  uint32_t* sm = ctx.buf<uint32_t>("c3.smap", 3 * nw);
+  k_signmap<<<(int)div_up(3 * nw, 256), 256, 0, st>>>(dU, dV, dW, n, nw, sm);
+  float* L = ctx.buf<float>("c3.log", 3 * n);
+  k_log<<<(int)div_up(3 * n, 256), 256, 0, st>>>(dU, dV, dW, n, lm.log2f, L);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("log");
+  int hflags[4];
+  CPSZ_HIP(hipMemcpyAsync(hflags, flags, 16, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_REQUIRE(hflags[0] == 0, kErrNonFinite,
+               "compress_3d: non-finite input or bound (values too large for the reference's "
+               "float triple products)");
+  // ---- wavefront
+  const int nb = (int)div_up(r2, 64);
+  Wave3d w{};
+  w.lx = L;
+  w.code = ebc;
+  w.R = ctx.buf<float>("c3.R", 3 * n);
+  w.qout = ctx.buf<uint16_t>("c3.q", 3 * n);
+  w.vid = ctx.buf<uint8_t>("c3.vid", n);
+  w.prog = ctx.buf<int>("c3.prog", (size_t)r1 * nb + 1);
+  w.ticket = w.prog + (size_t)r1 * nb;
+  w.r1 = (int)r1;
+  w.r2 = (int)r2;
+  w.r3 = (int)r3;
+  w.nb = nb;
+  launch_wave3d(ctx, w, true);
+  ctx.mark("lorenzo");
+  // ---- histograms
+  uint32_t* hist = ctx.buf<uint32_t>("c3.hist", kMaxEbCode + 65536);
+  CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 65536) * 4, st));
+  const size_t nch = div_up(n, kVChunk);
+  SrcIds3d s_ids{w.vid, n};
+  SrcCodes3d s_codes{w.vid, w.qout, n};
+  {
+    const int grid = (int)std::min<size_t>(div_up(nch, 256), 1024);
+    k_hist<<<grid, 256, kMaxEbCode * 4, st>>>(s_ids, nch, hist, 0, kMaxEbCode);
+    k_hist<<<grid, 256, 16384 * 4, st>>>(s_codes, nch, hist + kMaxEbCode, 32768 - 8192, 16384);
+    CPSZ_HIP(hipGetLastError());
+  }
+  uint32_t* hh = (uint32_t*)ctx.host_buf("c3.hist", (kMaxEbCode + 65536) * 4);
+  CPSZ_HIP(hipMemcpyAsync(hh, hist, (kMaxEbCode + 65536) * 4, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  ctx.mark("hist");
+  std::vector<uint64_t> f(65536);
+  HuffEncodeTable t_id, t_q;
+  for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
+  huff_build(f.data(), kMaxEbCode, 2 * 256, t_id);
+  uint64_t npred_syms = 0;
+  for (int k = 0; k < 65536; k++) npred_syms += (f[k] = hh[kMaxEbCode + k]);
+  const bool have_codes = npred_syms > 0;
+  CPSZ_REQUIRE(have_codes, kErrUnsupported,
+               "compress_3d: no predictable vertex (the reference cannot encode an empty code "
+               "section)");
+  huff_build(f.data(), 65536, 2 * kCapacity, t_q);
+  ctx.mark("tree");
+  uint64_t* cb_code = ctx.buf<uint64_t>("c3.cbcode", 2 * 65536);
+  uint8_t* cb_len = ctx.buf<uint8_t>("c3.cblen", 2 * 65536);
+  upload_codebook(ctx, t_id, cb_code, cb_len, "c3.id");
+  upload_codebook(ctx, t_q, cb_code + 65536, cb_len + 65536, "c3.q");
+  uint64_t* bits = ctx.buf<uint64_t>("c3.bits", 2 * nch);
+  uint64_t* offs = ctx.buf<uint64_t>("c3.offs", 2 * nch + 4);
+  uint32_t* cnt = ctx.buf<uint32_t>("c3.cnt", 2 * nch);
+  uint64_t* coff = ctx.buf<uint64_t>("c3.coff", 2 * nch + 4);
+  uint64_t* totals = ctx.buf<uint64_t>("c3.totals", 8);
+  k_chunk_bits<<<(int)div_up(nch, 256), 256, 0, st>>>(s_ids, nch, cb_len, bits);
+  k_chunk_bits<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
+  k_chunk_counts3d<<<(int)div_up(nch, 256), 256, 0, st>>>(w.vid, n, nch, cnt, cnt + nch);
+  CPSZ_HIP(hipGetLastError());
+  scan_u64(ctx, bits, offs, nch, totals + 0);
+  scan_u64(ctx, bits + nch, offs + nch, nch, totals + 1);
+  scan_u32_to_u64(ctx, cnt, coff, nch, totals + 2);
+  scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, totals + 3);
+  uint64_t* ht = (uint64_t*)ctx.host_buf("c3.totals", 64);
+  CPSZ_HIP(hipMemcpyAsync(ht, totals, 32, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  const uint64_t bits_id = ht[0], bits_q = ht[1], npred = ht[2], nunp = ht[3];
+  CPSZ_REQUIRE(npred + nunp == n, kErrInternal, "vertex count mismatch");
+  ctx.mark("scan");
+  const size_t by_id = div_up(bits_id, 8), by_q = div_up(bits_q, 8);
+  const size_t w_id = div_up(by_id, 4) + 2, w_q = div_up(by_q, 4) + 2;
+  uint32_t* words = ctx.buf<uint32_t>("c3.words", w_id + w_q);
+  k_fill_u32<<<1024, 256, 0, st>>>(words, w_id + w_q, 0);
+  k_encode<<<(int)div_up(nch, 256), 256, 0, st>>>(s_ids, nch, cb_code, cb_len, offs, words);
+  k_encode<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536,
+                                                  offs + nch, words + w_id);
+  float* unp = ctx.buf<float>("c3.unp", 3 * nunp + 1);
+  k_gather_unpred3d<<<(int)div_up(nch, 256), 256, 0, st>>>(w.vid, dU, dV, dW, n, nch, coff + nch + 1,
+                                                           unp);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("encode");
+  // ---- stream (src/sz_compress_cp_preserve_3d.cpp:556-579)
+  auto huff_head = [](const HuffEncodeTable& t, size_t nbytes) {
+    std::vector<uint8_t> h;
+    put(h, (size_t)t.node_count);
+    put(h, (uint32_t)t.tree.size());
+    h.insert(h.end(), t.tree.begin(), t.tree.end());
+    put(h, (size_t)nbytes);
+    return h;
+  };
+  std::vector<uint8_t> h0, h1, h2;
+  put(h0, (int)2);
+  put(h0, (int)kRadius);
+  put(h1, (size_t)(3 * nunp));
+  put(h2, (size_t)n);
+  {
+    std::vector<uint8_t> a = huff_head(t_id, by_id);
+    h2.insert(h2.end(), a.begin(), a.end());
+  }
+  std::vector<uint8_t> h3;
+  put(h3, (size_t)(3 * npred));
+  {
+    std::vector<uint8_t> a = huff_head(t_q, by_q);
+    h3.insert(h3.end(), a.begin(), a.end());
+  }
+  const size_t total = h0.size() + 3 * smap + h1.size() + 12 * nunp + h2.size() + by_id + h3.size() + by_q;
+  uint8_t* dout = ctx.buf<uint8_t>("c3.out", total + 16);
+  uint8_t* stage = (uint8_t*)ctx.host_buf("c3.stage", h0.size() + h1.size() + h2.size() + h3.size());
+  size_t pos = 0, spos = 0;
+  auto host_piece = [&](const std::vector<uint8_t>& v) {
+    std::memcpy(stage + spos, v.data(), v.size());
+    CPSZ_HIP(hipMemcpyAsync(dout + pos, stage + spos, v.size(), hipMemcpyHostToDevice, st));
+    pos += v.size();
+    spos += v.size();
+  };
+  auto dev_piece = [&](const void* src, size_t len) {
+    if (len) CPSZ_HIP(hipMemcpyAsync(dout + pos, src, len, hipMemcpyDeviceToDevice, st));
+    pos += len;
+  };
+  host_piece(h0);
+  for (int p = 0; p < 3; p++) dev_piece(sm + p * nw, smap);
+  host_piece(h1);
+  dev_piece(unp, 12 * nunp);
+  host_piece(h2);
+  dev_piece(words, by_id);
+  host_piece(h3);
+  dev_piece(words + w_id, by_q);
+  CPSZ_REQUIRE(pos == total, kErrInternal, "3D stream layout mismatch");
+  ctx.mark("assemble");
+  out.data = dout;
+  out.size = total;
+  CPSZ_HIP(hipStreamSynchronize(st));
+  ctx.finish_marks();
+}
+
+namespace {
+struct Reader3 {
+  Ctx& ctx;
+  const uint8_t* host;
+  const uint8_t* dev;
+  size_t size;
+  std::vector<uint8_t> tmp;
+  const uint8_t* read(size_t off, size_t len) {
+    CPSZ_REQUIRE(off <= size && len <= size - off, kErrStream, "stream truncated");
+    if (host) return host + off;
+    tmp.resize(len);
+    if (len) {
+      CPSZ_HIP(hipMemcpyAsync(tmp.data(), dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
+      CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+    }
+    return tmp.data();
+  }
+  template <typename V>
+  V get(size_t off) {
+    V v;
+    std::memcpy(&v, read(off, sizeof(V)), sizeof(V));
+    return v;
+  }
+};
+}  // namespace
+
+void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
+                          size_t r1, size_t r2, size_t r3, float* dU, float* dV, float* dW) {
+  CPSZ_REQUIRE(d_in && dU && dV && dW && r1 && r2 && r3, kErrArg, "decompress_3d: bad arguments");
+  hipStream_t st = ctx.stream();
+  const size_t n = r1 * r2 * r3;
+  ctx.stages.clear();
+  ctx.mark("start");
+  const Ctx::LibmDev& lm = ctx.libm_dev();
+  Reader3 rd{ctx, h_in, d_in, in_size, {}};
+  const int base = rd.get<int>(0);
+  const int radius = rd.get<int>(4);
+  CPSZ_REQUIRE(base == 2 && radius == kRadius, kErrUnsupported,
+               "decompress_3d: base/radius differ from the offline_log format");
+  const size_t smap = (n - 1) / 8 + 1;
+  size_t off = 8 + 3 * smap;
+  const size_t nunp_f = rd.get<size_t>(off);
+  CPSZ_REQUIRE(nunp_f % 3 == 0 && nunp_f <= 3 * n, kErrStream, "bad unpredictable count");
+  const size_t unp_off = off + 8;
+  off = unp_off + 4 * nunp_f;
+  const size_t eb_num = rd.get<size_t>(off);
+  CPSZ_REQUIRE(eb_num == n, kErrStream, "eb id count differs from the field size");
+  auto parse = [&](size_t at, size_t& node_count, std::vector<uint8_t>& tree, size_t& nbytes,
+                   size_t& bits_off) {
+    node_count = rd.get<size_t>(at);
+    const uint32_t ts = rd.get<uint32_t>(at + 8);
+    CPSZ_REQUIRE(node_count >= 1 && node_count < (1u << 24), kErrStream, "bad huffman node count");
+    const uint8_t* t = rd.read(at + 12, ts);
+    tree.assign(t, t + ts);
+    nbytes = rd.get<size_t>(at + 12 + ts);
+    bits_off = at + 20 + ts;
+    CPSZ_REQUIRE(bits_off + nbytes <= in_size, kErrStream, "huffman bitstream truncated");
+    return bits_off + nbytes;
+  };
+  size_t nc_id, nb_id, bo_id, nc_q, nb_q, bo_q;
+  std::vector<uint8_t> tr_id, tr_q;
+  off = parse(off + 8, nc_id, tr_id, nb_id, bo_id);
+  const size_t data_num = rd.get<size_t>(off);
+  CPSZ_REQUIRE(data_num <= 3 * n, kErrStream, "bad code count");
+  parse(off + 8, nc_q, tr_q, nb_q, bo_q);
+  ctx.mark("parse");
+  auto stage_bits = [&](size_t bo, size_t nbytes, const char* name) {
+    uint32_t* w = ctx.buf<uint32_t>(name, div_up(nbytes, 4) + 8);
+    CPSZ_HIP(hipMemsetAsync(w, 0, (div_up(nbytes, 4) + 8) * 4, st));
+    if (nbytes) CPSZ_HIP(hipMemcpyAsync(w, d_in + bo, nbytes, hipMemcpyDeviceToDevice, st));
+    return w;
+  };
+  uint8_t* vid = ctx.buf<uint8_t>("d3.vid", n);
+  uint16_t* qc = ctx.buf<uint16_t>("d3.qc", data_num + 1);
+  {
+    uint32_t* w = stage_bits(bo_id, nb_id, "d3.w.id");
+    DeviceDecoder dd = upload_decoder(ctx, tr_id.data(), tr_id.size(), nc_id, "d3.id");
+    decode_stream(ctx, dd, w, nb_id, n, SinkU8{vid}, "d3.id");
+  }
+  {
+    uint32_t* w = stage_bits(bo_q, nb_q, "d3.w.q");
+    DeviceDecoder dd = upload_decoder(ctx, tr_q.data(), tr_q.size(), nc_q, "d3.q");
+    decode_stream(ctx, dd, w, nb_q, data_num, SinkU16{qc}, "d3.q");
+  }
+  ctx.mark("huffman");
+  const size_t nch = div_up(n, kVChunk);
+  uint32_t* cnt = ctx.buf<uint32_t>("d3.cnt", 2 * nch);
+  uint64_t* coff = ctx.buf<uint64_t>("d3.coff", 2 * nch + 4);
+  k_chunk_counts3d<<<(int)div_up(nch, 256), 256, 0, st>>>(vid, n, nch, cnt, cnt + nch);
+  scan_u32_to_u64(ctx, cnt, coff, nch, coff + 2 * nch + 2);
+  scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, coff + 2 * nch + 3);
+  float* unp = ctx.buf<float>("d3.unp", nunp_f + 1);
+  if (nunp_f) CPSZ_HIP(hipMemcpyAsync(unp, d_in + unp_off, 4 * nunp_f, hipMemcpyDeviceToDevice, st));
+  uint16_t* q = ctx.buf<uint16_t>("d3.q", 3 * n);
+  float* init = ctx.buf<float>("d3.init", 3 * n);
+  int* err = ctx.buf<int>("d3.err", 2);
+  CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
+  k_scatter3d<<<(int)div_up(nch, 256), 256, 0, st>>>(vid, qc, unp, n, nch, coff, coff + nch + 1,
+                                                     data_num, nunp_f, lm.log2f, q, init, err);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("scatter");
+  const int nbk = (int)div_up(r2, 64);
+  Wave3d w{};
+  w.lx = init;
+  w.code = vid;
+  w.qin = q;
+  w.R = ctx.buf<float>("d3.R", 3 * n);
+  w.prog = ctx.buf<int>("d3.prog", (size_t)r1 * nbk + 1);
+  w.ticket = w.prog + (size_t)r1 * nbk;
+  w.r1 = (int)r1;
+  w.r2 = (int)r2;
+  w.r3 = (int)r3;
+  w.nb = nbk;
+  launch_wave3d(ctx, w, false);
+  ctx.mark("lorenzo");
+  uint8_t* sm = ctx.buf<uint8_t>("d3.smap", 3 * smap + 8);
+  CPSZ_HIP(hipMemcpyAsync(sm, d_in + 8, 3 * smap, hipMemcpyDeviceToDevice, st));
+  k_finish3d<<<(int)div_up(nch, 256), 256, 0, st>>>(vid, w.R, sm, smap, unp, n, nch, coff + nch + 1,
+                                                    lm.exp2f, dU, dV, dW);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("finish");
+  int herr[2];
+  CPSZ_HIP(hipMemcpyAsync(herr, err, 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_REQUIRE(herr[0] == 0, kErrStream, "decompress_3d: code/unpredictable counts inconsistent");
+  ctx.finish_marks();
+}
+
+// ------------------------------------------------------ host emulations ----
+void eb3d_emulate(const float* U, const float* V, const float* W, size_t r1, size_t r2, size_t r3,
+                  double max_pwr_eb, double* eb) {
+  const size_t n = r1 * r2 * r3, d0 = r2 * r3, d1 = r3;
+  for (size_t v = 0; v < n; v++) eb[v] = max_pwr_eb;
+  for (size_t i = 0; i + 1 < r1; i++)
+    for (size_t j = 0; j + 1 < r2; j++)
+      for (size_t k = 0; k + 1 < r3; k++) {
+        const size_t c = i * d0 + j * d1 + k;
+        for (int t = 0; t < 6; t++) {
+          size_t idx[4];
+          for (int v = 0; v < 3; v++) {
+            int di, dj, dk;
+            tet_offsets(t, v, di, dj, dk);
+            idx[v] = c + di * d0 + dj * d1 + dk;
+          }
+          idx[3] = c;
+          const double e = tet_bound(U[idx[0]], U[idx[1]], U[idx[2]], U[idx[3]], V[idx[0]], V[idx[1]],
+                                     V[idx[2]], V[idx[3]], W[idx[0]], W[idx[1]], W[idx[2]], W[idx[3]]);
+          for (int v = 0; v < 4; v++) eb[idx[v]] = eb[idx[v]] < e ? eb[idx[v]] : e;
+        }
+      }
+}
+
+void lorenzo3d_emulate(const float* L0, const uint8_t* code, size_t r1, size_t r2, size_t r3,
+                       uint8_t* vid, uint16_t* q) {
+  const size_t n = r1 * r2 * r3, d0 = r2 * r3, d1 = r3;
+  std::vector<float> R(L0, L0 + 3 * n);
+  for (size_t i = 0; i < r1; i++)
+    for (size_t j = 0; j < r2; j++)
+      for (size_t k = 0; k < r3; k++) {
+        const size_t v = i * d0 + j * d1 + k;
+        const uint32_t c = code[v];
+        float dec[3];
+        uint16_t qc[3];
+        bool okall = true;
+        for (int p = 0; p < 3; p++) {
+          const float* X = R.data() + p * n;
+          auto at = [&](bool in, size_t e) { return in ? X[e] : 0.f; };
+          const float e0 = at(i && j && k, v - d0 - d1 - 1), e1 = at(i && j, v - d0 - d1),
+                      e2 = at(i && k, v - d0 - 1), e3 = at(i, v - d0), e4 = at(j && k, v - d1 - 1),
+                      e5 = at(j, v - d1), e6 = at(k, v - 1);
+          const float pred = e0 + e3 + e5 + e6 - e1 - e2 - e4;
+          bool ok = false;
+          if (c != 255 && c != 0) dec[p] = l3d_step(pred, X[v], (int)c, ok, qc[p]);
+          okall = okall && ok;
+        }
+        const bool pr = c != 255 && c != 0 && okall;
+        vid[v] = (uint8_t)(c == 255 ? 0 : (pr ? c : 24));
+        for (int p = 0; p < 3; p++) {
+          if (pr) {
+            R[p * n + v] = dec[p];
+            q[p * n + v] = qc[p];
+          } else {
+            q[p * n + v] = 0;
+          }
+        }
+      }
+}
+
+}  // namespace cpsz

@@ -101,6 +101,7 @@ Ctx::~Ctx() {
   for (auto& kv : hbufs_) hipHostFree(kv.second.p);
   if (d_q2d_) hipFree(d_q2d_);
   if (d_q3d_) hipFree(d_q3d_);
+  for (void* p : libm_allocs_) hipFree(p);
   for (auto e : event_pool_) hipEventDestroy(e);
   if (own_stream_) hipStreamDestroy(own_stream_);
 }
@@ -169,17 +170,23 @@ const Ctx::LibmDev& Ctx::libm_dev() {
     auto up = [&](const void* src, size_t bytes) {
       void* p = nullptr;
       CPSZ_HIP(hipMalloc(&p, bytes ? bytes : 4));
+      libm_allocs_.push_back(p);
       if (bytes) CPSZ_HIP(hipMemcpy(p, src, bytes, hipMemcpyHostToDevice));
       return p;
     };
-    libm_.log2f_x = (const uint32_t*)up(t.log2f_x.data(), t.log2f_x.size() * 4);
-    libm_.log2f_y = (const uint32_t*)up(t.log2f_y.data(), t.log2f_y.size() * 4);
-    libm_.exp2f_x = (const uint32_t*)up(t.exp2f_x.data(), t.exp2f_x.size() * 4);
-    libm_.exp2f_y = (const uint32_t*)up(t.exp2f_y.data(), t.exp2f_y.size() * 4);
-    libm_.log2f_band = (const float*)up(t.log2f_band.data(), t.log2f_band.size() * 4);
-    libm_.exp2f_band = (const float*)up(t.exp2f_band.data(), t.exp2f_band.size() * 4);
-    libm_.n_log2f = (int)t.log2f_x.size();
-    libm_.n_exp2f = (int)t.exp2f_x.size();
+    auto view = [&](const LibmTable& x) {
+      LibmView v;
+      v.band = (const float*)up(x.band.data(), x.band.size() * 4);
+      v.kind = (const uint8_t*)up(x.kind.data(), x.kind.size());
+      v.off = (const uint32_t*)up(x.off.data(), x.off.size() * 4);
+      v.cnt = (const uint32_t*)up(x.cnt.data(), x.cnt.size() * 4);
+      v.list_x = (const uint32_t*)up(x.list_x.data(), x.list_x.size() * 4);
+      v.list_y = (const uint32_t*)up(x.list_y.data(), x.list_y.size() * 4);
+      v.bitmap = (const uint32_t*)up(x.bitmap.data(), x.bitmap.size() * 4);
+      return v;
+    };
+    libm_.log2f = view(t.log2f);
+    libm_.exp2f = view(t.exp2f);
     libm_ready_ = true;
   }
   return libm_;

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "libm_tables.hpp"
 
 namespace cpsz {
 
@@ -24,17 +25,6 @@ struct EbTables {
   std::vector<double> q3d;
 };
 const EbTables& eb_tables();
-
-// Exceptions of the host's log2f / exp2f against the library's own
-// correctly-rounded-in-double device evaluation (see libm_tables.cpp).
-struct LibmTables {
-  // sorted float bit patterns and the host libm results
-  std::vector<uint32_t> log2f_x, log2f_y;
-  std::vector<uint32_t> exp2f_x, exp2f_y;
-  // per input binade: max distance to a rounding midpoint among exceptions
-  std::vector<float> log2f_band, exp2f_band;
-};
-const LibmTables& libm_tables();
 
 struct DevBuf {
   void* p = nullptr;
@@ -68,9 +58,7 @@ class Ctx {
   int q3d_size() const { return (int)eb_tables().q3d.size(); }
   // Device copies of the libm exception tables (uploaded on first use).
   struct LibmDev {
-    const uint32_t *log2f_x, *log2f_y, *exp2f_x, *exp2f_y;
-    const float *log2f_band, *exp2f_band;
-    int n_log2f, n_exp2f;
+    LibmView log2f, exp2f;
   };
   const LibmDev& libm_dev();
 
@@ -89,6 +77,7 @@ class Ctx {
   double* d_q2d_ = nullptr;
   double* d_q3d_ = nullptr;
   LibmDev libm_{};
+  std::vector<void*> libm_allocs_;
   bool libm_ready_ = false;
   std::vector<std::pair<std::string, hipEvent_t>> marks_;
   std::vector<hipEvent_t> event_pool_;

@@ -122,12 +122,33 @@ int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int
 int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
                       size_t cap, size_t* tree_size, size_t* node_count);
 
+/* Builds the host-libm exception tables that let the device reproduce the
+ * host's log2f/exp2f bit for bit, and writes them to `path` (NULL: the cache
+ * file next to the library). The library loads that cache on first use and
+ * rebuilds it when it does not match the running host libm. */
+int cpsz_prepare_libm_tables(const char* path);
+/* The device's log2f / exp2f evaluation run on the host (for testing):
+ * log2f for finite x > 0, exp2f for y in [-99, 128). */
+int cpsz_eval_log2f(const float* x, size_t n, float* out);
+int cpsz_eval_exp2f(const float* y, size_t n, float* out);
+
 /* Host emulation of the device 2D Lorenzo arithmetic (row-major order):
  * eb codes c (0 = zero precision, else reference id + 1) -> quant codes q and
  * the reconstruction. `wide` selects the double-precision path used for
  * ids above 81. */
 int cpsz_lorenzo2d_emulate(const float* x, const uint16_t* c, size_t r1, size_t r2, int wide,
                            uint16_t* q, float* recon);
+
+/* Host emulations of the device 3D arithmetic: per-vertex bounds (the min
+ * over incident tetrahedra of the reference's per-tet bound), and the Lorenzo
+ * pass from log values L[3][n] and eb codes (255 = non-positive bound, else
+ * the id) to per-vertex stream ids and per-component codes q[3][n]. */
+int cpsz_eb3d_emulate(const float* U, const float* V, const float* W, size_t r1, size_t r2,
+                      size_t r3, double max_pwr_eb, double* eb);
+int cpsz_lorenzo3d_emulate(const float* L, const uint8_t* code, size_t r1, size_t r2, size_t r3,
+                           uint8_t* vid, uint16_t* q);
+/* eb code of each bound as the device derives it (255 or the 3D id). */
+int cpsz_eb_code_3d(const double* eb, size_t n, uint8_t* code);
 
 /* ---- instrumentation ------------------------------------------------------ */
 /* Enables per-stage HIP-event timing for this thread's context on `device`. */

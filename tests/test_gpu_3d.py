@@ -1,0 +1,92 @@
+"""3D offline_log codec on the GPU against the reference decoder fixtures and
+the CPU oracle: bit-exact streams and decoded fields, edge shapes, full size."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import cpsz_amd as cz
+from _libs import load_oracle
+from cpsz_amd.synth import field_3d
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return load_oracle()
+
+
+def _same(a, b):
+    return np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "c3d_*.npz"))),
+                         ids=os.path.basename)
+def test_golden(path):
+    z = np.load(path)
+    U, V, W = z["U"], z["V"], z["W"]
+    print(f"[3d golden] {os.path.basename(path)} {U.shape}", flush=True)
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, float(z["eb"]))
+    assert np.array_equal(s, z["stream"])
+    d = cz.sz_decompress_cp_preserve_3d_online_log(z["stream"], *U.shape)
+    for a, b in zip(d, (z["dU"], z["dV"], z["dW"])):
+        assert _same(a, b)
+
+
+@pytest.mark.parametrize("shape,seed,eb", [
+    ((2, 2, 2), 1, 0.05), ((1, 5, 70), 2, 0.05), ((70, 1, 3), 3, 0.05), ((3, 130, 5), 4, 0.05),
+    ((20, 65, 33), 5, 0.05), ((33, 64, 64), 6, 0.1), ((17, 129, 40), 7, 0.01),
+])
+def test_shapes_vs_oracle(orc, shape, seed, eb):
+    print(f"[3d oracle] {shape} seed={seed} eb={eb}", flush=True)
+    U, V, W = field_3d(*shape, seed)
+    try:
+        ref = orc.compress_3d(U, V, W, eb)
+    except AssertionError:
+        # no predictable vertex: the reference cannot write an empty code section
+        with pytest.raises(cz.CpszError):
+            cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, eb)
+        return
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, eb)
+    assert np.array_equal(s, ref)
+    d = cz.sz_decompress_cp_preserve_3d_online_log(ref, *shape)
+    o = orc.decompress_3d(ref, *shape)
+    for a, b in zip(d, o):
+        assert _same(a, b)
+
+
+def test_full_size_256(orc):
+    """256^3 (the 3D headline field): bit-exact stream and decode vs the oracle."""
+    shape = (256, 256, 256)
+    print("[3d full] 256^3", flush=True)
+    U, V, W = field_3d(*shape, 0)
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
+    ref = orc.compress_3d(U, V, W, 0.05)
+    assert s.size == ref.size and np.array_equal(s, ref)
+    d = cz.sz_decompress_cp_preserve_3d_online_log(s, *shape)
+    o = orc.decompress_3d(ref, *shape)
+    for a, b in zip(d, o):
+        assert _same(a, b)
+
+
+def test_zero_and_negative_values(orc):
+    U, V, W = field_3d(10, 12, 14, 9)
+    U[2:5, 3:6, :] = 0
+    V[:, :, 4] = -V[:, :, 4]
+    W[7, 7, 7] = 0
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
+    assert np.array_equal(s, orc.compress_3d(U, V, W, 0.05))
+    for a, b in zip(cz.sz_decompress_cp_preserve_3d_online_log(s, 10, 12, 14),
+                    orc.decompress_3d(s, 10, 12, 14)):
+        assert _same(a, b)
+
+
+def test_nonfinite_rejected_3d():
+    U, V, W = field_3d(8, 8, 8, 1)
+    W[1, 2, 3] = np.inf
+    with pytest.raises(cz.CpszError) as e:
+        cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
+    assert e.value.code == -6

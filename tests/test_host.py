@@ -133,3 +133,99 @@ def test_lorenzo2d_arithmetic_extreme_bounds():
     qa, ra = _emulate(x, c, False)
     qb, rb = _emulate(x, c, True)
     assert np.array_equal(qa, qb) and np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+
+
+def _libm_eval(name, x):
+    L = cz.lib()
+    f = getattr(L, name)
+    f.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    assert f(x.ctypes.data, x.size, out.ctypes.data) == 0
+    return out
+
+
+def _glibc(name, x):
+    m = C.CDLL("libm.so.6")
+    f = getattr(m, name)
+    f.argtypes = [C.c_float]
+    f.restype = C.c_float
+    return np.array([f(float(v)) for v in x], np.float32)
+
+
+def test_log2f_matches_host_libm():
+    rng = np.random.default_rng(11)
+    # random bit patterns over every positive binade, plus a dense slice of
+    # [0.98, 1.05] where the host log2f deviates most often
+    bits = rng.integers(1, 0x7F800000, size=300000, dtype=np.uint32)
+    dense = np.arange(np.float32(0.98).view(np.uint32), np.float32(1.05).view(np.uint32), 7,
+                      dtype=np.uint32)
+    x = np.concatenate([bits, dense[:400000]]).view(np.float32)
+    got = _libm_eval("cpsz_eval_log2f", x)
+    want = _glibc("log2f", x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_exp2f_matches_host_libm():
+    rng = np.random.default_rng(12)
+    y = np.concatenate([rng.uniform(-99, 128, 300000), rng.uniform(-1, 1, 200000)]).astype(np.float32)
+    got = _libm_eval("cpsz_eval_exp2f", y)
+    want = _glibc("exp2f", y)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def _parse_3d(orc, s, n):
+    """(ids, codes) of a 3D offline_log stream."""
+    smap = (n - 1) // 8 + 1
+    off = 8 + 3 * smap
+    nunp = int(np.frombuffer(s[off:off + 8].tobytes(), np.uint64)[0])
+    off += 8 + 4 * nunp
+    off += 8  # eb_quant_num
+    ids, used = orc.huff_decode(s[off:], n)
+    off += used
+    nq = int(np.frombuffer(s[off:off + 8].tobytes(), np.uint64)[0])
+    codes, _ = orc.huff_decode(s[off + 8:], nq)
+    return ids, codes
+
+
+def _emulate_3d(U, V, W, eb):
+    L = cz.lib()
+    r1, r2, r3 = U.shape
+    n = U.size
+    L.cpsz_eb3d_emulate.argtypes = [C.c_void_p] * 3 + [C.c_size_t] * 3 + [C.c_double, C.c_void_p]
+    L.cpsz_lorenzo3d_emulate.argtypes = [C.c_void_p, C.c_void_p] + [C.c_size_t] * 3 + [C.c_void_p] * 2
+    L.cpsz_eb_code_3d.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p]
+    e = np.empty(n, np.float64)
+    assert L.cpsz_eb3d_emulate(U.ctypes.data, V.ctypes.data, W.ctypes.data, r1, r2, r3, eb, e.ctypes.data) == 0
+    code = np.empty(n, np.uint8)
+    assert L.cpsz_eb_code_3d(e.ctypes.data, n, code.ctypes.data) == 0
+    logs = []
+    for X in (U, V, W):
+        a = np.abs(X.ravel())
+        lg = _libm_eval("cpsz_eval_log2f", np.where(a == 0, 1, a))
+        logs.append(np.where(a == 0, np.float32(-100), lg))
+    Lx = np.ascontiguousarray(np.concatenate(logs), np.float32)
+    vid = np.empty(n, np.uint8)
+    q = np.empty(3 * n, np.uint16)
+    assert L.cpsz_lorenzo3d_emulate(Lx.ctypes.data, code.ctypes.data, r1, r2, r3, vid.ctypes.data,
+                                    q.ctypes.data) == 0
+    return e, code, vid, q.reshape(3, n)
+
+
+@pytest.mark.parametrize("shape,seed,eb", [((12, 13, 14), 1, 0.05), ((9, 17, 33), 2, 0.05),
+                                           ((16, 16, 16), 3, 0.1), ((7, 40, 9), 4, 0.02)])
+def test_3d_arithmetic_matches_oracle(shape, seed, eb):
+    from cpsz_amd.synth import field_3d
+    orc = load_oracle()
+    U, V, W = field_3d(*shape, seed)
+    e, code, vid, q = _emulate_3d(U, V, W, eb)
+    eo = orc.eb_3d(U, V, W, eb).ravel()
+    assert np.array_equal(e.view(np.uint64), eo.view(np.uint64))
+    ido = orc.eb_id_3d(eo)
+    pos = eo > 0
+    assert np.array_equal(code[pos].astype(np.int32), ido[pos]) and np.all(code[~pos] == 255)
+    s = orc.compress_3d(U, V, W, eb)
+    ids, codes = _parse_3d(orc, s, U.size)
+    assert np.array_equal(vid.astype(np.int32), ids)
+    pred = (vid != 0) & (vid != 24)
+    assert np.array_equal(q[:, pred].T.ravel().astype(np.int32), codes)
