@@ -79,7 +79,14 @@ int sign_det3(const float* a, const float* b, const float* c) {
   return e.sign();
 }
 
-// 2D type: eigenvalues of the Jacobian of the linear interpolant.
+// Type classification uses only +, -, * and / on doubles in a fixed order (no
+// sqrt or transcendental functions), so the device extractor, written with the
+// same operation sequence, agrees with it bit for bit; eigenvalue signs come
+// from the characteristic polynomial's coefficients rather than from roots.
+
+// 2D type: eigenvalues of the Jacobian of the linear interpolant
+// (test/cp_extraction_2d.cpp:32-39 codes). Real pair (delta >= 0): saddle when
+// det < 0, else the sign of the trace; complex pair: the sign of the trace.
 int type_2d(const double v[3][2], const double X[3][2]) {
   // J (X_p - X_2) = v_p - v_2 for p = 0, 1
   const double xa = X[0][0] - X[2][0], xb = X[1][0] - X[2][0];
@@ -94,20 +101,22 @@ int type_2d(const double v[3][2], const double X[3][2]) {
   const double dt = J[0][0] * J[1][1] - J[1][0] * J[0][1];
   const double delta = tr * tr - 4 * dt;
   if (delta >= 0) {
-    const double s = std::sqrt(delta);
-    const double e0 = (tr + s) / 2, e1 = (tr - s) / 2;
-    if (e0 * e1 < 0) return 3;
-    if (e0 < 0) return 1;
-    if (e0 > 0) return 2;
-    return 0;
+    if (dt < 0) return 3;  // roots of opposite sign
+    if (dt > 0) return tr < 0 ? 1 : 2;  // same sign as their sum (tr != 0 here)
+    return tr > 0 ? 2 : 0;  // a zero root: the other is tr
   }
-  const double re = tr / 2;
-  if (re < 0) return 4;
-  if (re > 0) return 5;
+  if (tr < 0) return 4;
+  if (tr > 0) return 5;
   return 6;
 }
 
-// 3D type from the monic characteristic cubic x^3 + b x^2 + c x + d.
+// 3D type from the monic characteristic cubic p(x) = x^3 + b x^2 + c x + d
+// (test/cp_extraction_3d.cpp:30-38 codes: 2 * #negative real parts, +2 for a
+// complex pair, +1 for three real roots). Discriminant of Cardano's reduced
+// form decides the root structure. One real root r and a pair s +- iw: r < 0
+// iff p(0) = d > 0, and b c - d = -2 s ((r + s)^2 + w^2), so s < 0 iff
+// b c - d > 0. Three real roots: Descartes' rule of signs on p(-x), exact when
+// every root is real.
 int type_3d(const double v[4][3], const int X[4][3]) {
   double Xm[3][3], Um[3][3];
   for (int r = 0; r < 3; r++)
@@ -145,31 +154,21 @@ int type_3d(const double v[4][3], const int X[4][3]) {
   const double r = (-27 * dd + b * (9 * c - 2 * b * b)) / 54;
   const double disc = q * q * q + r * r;
   if (std::fabs(disc) < std::numeric_limits<double>::epsilon()) return 0;
-  double re[3];
-  const double shift = -b / 3;
+  int neg;
   if (disc > 0) {
-    const double sq = std::sqrt(disc);
-    const double s = std::cbrt(r + sq), t = std::cbrt(r - sq);
-    re[0] = shift + s + t;
-    re[1] = re[2] = shift - (s + t) / 2;
-  } else {
-    const double rho = std::sqrt(-q * q * q);
-    double ct = r / rho;
-    ct = ct > 1 ? 1 : (ct < -1 ? -1 : ct);
-    const double th = std::acos(ct);
-    const double m = 2 * std::sqrt(-q);
-    re[0] = shift + m * std::cos(th / 3);
-    re[1] = shift + m * std::cos((th + 2 * M_PI) / 3);
-    re[2] = shift + m * std::cos((th + 4 * M_PI) / 3);
+    neg = (dd > 0) + 2 * (b * c - dd > 0);
+    return 2 * neg + 2;
   }
-  int neg = (re[0] < 0) + (re[1] < 0) + (re[2] < 0);
-  switch (neg) {
-    case 0: return disc > 0 ? 2 : 1;
-    case 1: return disc > 0 ? 4 : 3;
-    case 2: return disc > 0 ? 6 : 5;
-    case 3: return disc > 0 ? 8 : 7;
+  // coefficients of p(-x) = -x^3 + b x^2 - c x + d; zeros are skipped
+  const double co[4] = {-1.0, b, -c, dd};
+  neg = 0;
+  double last = -1.0;
+  for (int k = 1; k < 4; k++) {
+    if (co[k] == 0) continue;
+    if ((co[k] > 0) != (last > 0)) neg++;
+    last = co[k];
   }
-  return 0;
+  return 2 * neg + 1;
 }
 
 // zero inside the closed triangle of vectors a, b, c (exact)

@@ -1,0 +1,96 @@
+"""GPU critical-point extraction against the oracle's exact extractor, and the
+0-CP-error property of the codecs (critical points of the decompressed field
+equal those of the original: same simplices, same types)."""
+import numpy as np
+import pytest
+import torch
+
+import cpsz_amd as cz
+from _libs import load_oracle
+from cpsz_amd.synth import field_2d, field_3d
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return load_oracle()
+
+
+def _dev(*arrs):
+    return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrs]
+
+
+def _cp2(U, V):
+    dU, dV = _dev(U, V)
+    out = cz.critical_points_2d_dev(dU.data_ptr(), dV.data_ptr(), *U.shape)
+    torch.cuda.synchronize()
+    return out
+
+
+def _cp3(U, V, W):
+    dU, dV, dW = _dev(U, V, W)
+    out = cz.critical_points_3d_dev(dU.data_ptr(), dV.data_ptr(), dW.data_ptr(), *U.shape)
+    torch.cuda.synchronize()
+    return out
+
+
+@pytest.mark.parametrize("shape,seed", [((4, 4), 1), ((5, 70), 2), ((200, 300), 3), ((600, 900), 4)])
+def test_cp2d_vs_oracle(orc, shape, seed):
+    print(f"[cp2d] {shape} seed={seed}", flush=True)
+    U, V = field_2d(*shape, seed)
+    got = _cp2(U, V)
+    assert got == orc.cp_2d(U, V)
+
+
+def test_cp2d_degenerate_integer_field(orc):
+    """Small integers: many zero vectors, collinear vectors and zeros on edges,
+    so the exact fallback decides most simplices."""
+    rng = np.random.default_rng(7)
+    U = rng.integers(-2, 3, (120, 130)).astype(np.float32)
+    V = rng.integers(-2, 3, (120, 130)).astype(np.float32)
+    got = _cp2(U, V)
+    ref = orc.cp_2d(U, V)
+    assert len(ref) > 100 and got == ref
+
+
+def test_cp2d_near_cancellation(orc):
+    """Values whose barycentric numerators cancel to within double rounding."""
+    rng = np.random.default_rng(11)
+    base = rng.standard_normal((90, 100)).astype(np.float32)
+    U = (base * np.float32(1 << 20)).astype(np.float32)
+    V = (base * np.float32(1 << 20) + rng.integers(-1, 2, base.shape).astype(np.float32)).astype(np.float32)
+    assert _cp2(U, V) == orc.cp_2d(U, V)
+
+
+@pytest.mark.parametrize("shape,seed", [((2, 2, 2), 1), ((3, 17, 70), 2), ((40, 50, 60), 3)])
+def test_cp3d_vs_oracle(orc, shape, seed):
+    print(f"[cp3d] {shape} seed={seed}", flush=True)
+    U, V, W = field_3d(*shape, seed)
+    assert _cp3(U, V, W) == orc.cp_3d(U, V, W)
+
+
+def test_cp3d_degenerate_integer_field(orc):
+    rng = np.random.default_rng(5)
+    U, V, W = (rng.integers(-1, 2, (20, 22, 24)).astype(np.float32) for _ in range(3))
+    got = _cp3(U, V, W)
+    ref = orc.cp_3d(U, V, W)
+    assert len(ref) > 100 and got == ref
+
+
+def test_zero_cp_error_2d():
+    U, V = field_2d(480, 720, 9)
+    s = cz.sz_compress_cp_preserve_2d_offline(U, V, 0.1)
+    dU, dV = cz.sz_decompress_cp_preserve_2d_offline(s, *U.shape)
+    a, b = _cp2(U, V), _cp2(dU, dV)
+    print(f"[cp2d] {len(a)} critical points", flush=True)
+    assert len(a) > 0 and a == b
+
+
+def test_zero_cp_error_3d():
+    U, V, W = field_3d(64, 64, 64, 9)
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
+    d = cz.sz_decompress_cp_preserve_3d_online_log(s, *U.shape)
+    a, b = _cp3(U, V, W), _cp3(*d)
+    print(f"[cp3d] {len(a)} critical points", flush=True)
+    assert len(a) > 0 and a == b

@@ -233,6 +233,46 @@ def test_cp_3d_linear_fields(orc, fun, typ):
     assert sid // 6 == 3 * 49 + 2 * 7 + 3  # cell (i=3, j=2, k=3)
 
 
+def _type_from_eigs(ev, dim):
+    """Type code from eigenvalues (test/cp_extraction_2d.cpp:32-39,
+    test/cp_extraction_3d.cpp:30-38)."""
+    cplx = bool(np.any(np.abs(ev.imag) > 1e-9))
+    if dim == 2:
+        if not cplx:
+            e0, e1 = sorted(ev.real)[::-1]
+            return 3 if e0 * e1 < 0 else (1 if e0 < 0 else 2)
+        return 4 if ev.real[0] < 0 else 5
+    neg = int(np.sum(ev.real < 0))
+    return 2 * neg + (2 if cplx else 1)
+
+
+def test_cp_types_match_eigenvalues_random_linear(orc):
+    """Random linear fields A (x - c): one critical point whose type is fixed
+    by the eigenvalues of A (numpy's eigensolver); cases near a structural
+    boundary (a real part or imaginary part near 0) are skipped."""
+    rng = np.random.default_rng(3)
+    checked = {2: 0, 3: 0}
+    for trial in range(120):
+        dim = 2 if trial % 2 else 3
+        A = rng.standard_normal((dim, dim))
+        ev = np.linalg.eigvals(A)
+        if np.min(np.abs(ev.real)) < 0.05 or np.any((np.abs(ev.imag) > 0) & (np.abs(ev.imag) < 0.05)):
+            continue
+        c = rng.uniform(2.2, 3.8, dim)
+        if dim == 2:
+            U, V = _grid_2d(7, 7, lambda x, y: tuple(A[r, 0] * (x - c[0]) + A[r, 1] * (y - c[1]) for r in range(2)))
+            cps = orc.cp_2d(U, V)
+        else:
+            U, V, W = _grid_3d(7, lambda x, y, z: tuple(A[r, 0] * (x - c[0]) + A[r, 1] * (y - c[1]) +
+                                                        A[r, 2] * (z - c[2]) for r in range(3)))
+            cps = orc.cp_3d(U, V, W)
+        assert len(cps) >= 1
+        # float32 rounding of the field can perturb J slightly; the type is robust here
+        assert set(cps.values()) == {_type_from_eigs(ev, dim)}, (trial, A, ev, cps)
+        checked[dim] += 1
+    assert checked[2] > 20 and checked[3] > 20
+
+
 def test_cp_2d_matches_float_barycentric(orc):
     """On a generic random field the exact predicate agrees with a float64
     barycentric solve wherever the latter is far from a tie."""
