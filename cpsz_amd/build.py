@@ -47,9 +47,27 @@ def build(jobs: int | None = None, verbose: bool = True) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(LIB + ".tmp", LIB)
+    build_dropin_demo()
     if verbose:
         print(f"built {LIB} from {len(srcs)} sources")
     return LIB
+
+
+ROOT = os.path.dirname(HERE)
+DEMO_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_roundtrip.cpp")
+DEMO_BIN = os.path.join(ROOT, "tests", "cpp", "dropin_roundtrip")
+
+
+def build_dropin_demo() -> str:
+    """A caller of the reference's C++ API compiled against the drop-in header
+    (include/sz_cp_preserve_dropin.hpp) and linked with libcpsz_hip.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), DEMO_SRC, "-o", DEMO_BIN + ".tmp",
+           "-L" + HERE, "-lcpsz_hip", "-Wl,-rpath,$ORIGIN/../../cpsz_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"drop-in demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DEMO_BIN + ".tmp", DEMO_BIN)
+    return DEMO_BIN
 
 
 if __name__ == "__main__":

@@ -14,6 +14,14 @@
 //                loads of the reconstruction with progress counters
 //   Huffman      ids (512 states) and the codes of predictable vertices
 // Decompress mirrors it and finishes with host-libm-exact exp2f.
+//
+// Wave layout: every per-vertex array the wavefront reads or writes (log
+// values, eb codes, stream ids, quantization codes, reconstruction) is stored
+// in the wavefront's own order, [plane i][j block][step t][lane], with vertex
+// (i, j, k) at step t = k + (j mod 64). At each step a wave then touches one
+// contiguous 64-element row per array instead of 64 rows r3 elements apart.
+// The producers (eb, log, scatter) and consumers (Huffman sources, gather,
+// finish) translate from and to row-major vertex order.
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -31,10 +39,68 @@ namespace {
 
 constexpr int kTI = 4, kTJ = 8, kTK = 64;  // vertex tile of the eb kernel
 
+// Wave layout: W lanes per j block (64, or r2 when r2 < 64), T = r3 + W - 1
+// steps per (plane, block) item.
+struct Skew3 {
+  int r1, r2, r3, nb, W, T;
+  size_t per;  // elements per component
+  __host__ __device__ size_t blk(int i, int b) const { return ((size_t)i * nb + b) * (size_t)T * W; }
+  __host__ __device__ size_t at(int i, int j, int k) const {
+    return blk(i, j >> 6) + (size_t)(k + (j & 63)) * W + (j & 63);
+  }
+  // vertex index (row-major, < 2^32) -> wave-layout index
+  __device__ size_t at_v(size_t v) const {
+    const uint32_t vv = (uint32_t)v, row = vv / (uint32_t)r3, k = vv - row * (uint32_t)r3;
+    const uint32_t i = row / (uint32_t)r2, j = row - i * (uint32_t)r2;
+    return at((int)i, (int)j, (int)k);
+  }
+};
+
+Skew3 make_skew(size_t r1, size_t r2, size_t r3) {
+  Skew3 s;
+  s.r1 = (int)r1;
+  s.r2 = (int)r2;
+  s.r3 = (int)r3;
+  s.nb = (int)div_up(r2, 64);
+  s.W = r2 < 64 ? (int)r2 : 64;
+  s.T = (int)r3 + s.W - 1;
+  s.per = r1 * (size_t)s.nb * s.T * s.W;
+  CPSZ_REQUIRE(s.per < (1ull << 31), kErrUnsupported,
+               "3D field too thin for the wavefront layout (r1 * ceil(r2/64) * (r3+63) * 64 >= 2^31)");
+  return s;
+}
+
+// Row-major walk over vertices [b, e) calling f(wave-layout index).
+template <class F>
+__device__ __forceinline__ void walk3(const Skew3& s, size_t b, size_t e, F f) {
+  if (b >= e) return;
+  const size_t row = b / (size_t)s.r3;
+  int k = (int)(b - row * s.r3);
+  int i = (int)(row / (size_t)s.r2), j = (int)(row - (size_t)i * s.r2);
+  size_t idx = s.at(i, j, k);
+  for (size_t v = b; v < e; v++) {
+    f(idx);
+    if (++k == s.r3) {
+      k = 0;
+      if (++j == s.r2) {
+        j = 0;
+        ++i;
+      }
+      idx = s.at(i, j, 0);
+    } else {
+      idx += s.W;
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
                                               const float* __restrict__ W, int r1, int r2, int r3,
                                               double max_pwr_eb, const double* __restrict__ q3d, int nq,
-                                              uint8_t* __restrict__ code, int* __restrict__ flags) {
+                                              Skew3 sk, uint8_t* __restrict__ code, int* __restrict__ flags) {
   constexpr int VI = kTI + 2, VJ = kTJ + 2, VK = kTK + 2;  // vertex halo box
   __shared__ float sv[3][VI][VJ][VK];
   __shared__ unsigned long long seb[kTI][kTJ][kTK];
@@ -97,7 +163,7 @@ __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const
                 x2 = sv[2][a + 1][b + 1][c + 1];
     if (!isfinite(x0) || !isfinite(x1) || !isfinite(x2)) bad = 1;
     const double eb = __longlong_as_double((long long)seb[a][b][c]);
-    code[gi * d0 + gj * d1 + gk] = (uint8_t)eb_code_3d(eb, q3d, nq);
+    code[sk.at(gi, gj, gk)] = (uint8_t)eb_code_3d(eb, q3d, nq);
   }
   if (bad) atomicOr(flags, 1);
 }
@@ -118,15 +184,19 @@ __global__ void k_signmap(const float* __restrict__ U, const float* __restrict__
   out[w] = word;
 }
 
-// log_transform (include/sz_cp_preserve_utils.hpp:205-221): log2f(|x|), -100 for 0.
+// log_transform (include/sz_cp_preserve_utils.hpp:205-221): log2f(|x|), -100
+// for 0, written in wave layout. Block x = one (i, j) row, y = k range.
 __global__ void k_log(const float* __restrict__ U, const float* __restrict__ V,
-                      const float* __restrict__ W, size_t n, LibmView lv, float* __restrict__ L) {
-  const size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (e >= 3 * n) return;
-  const int p = (int)(e / n);
-  const size_t i = e - p * n;
-  const float x = (p == 0 ? U : (p == 1 ? V : W))[i];
-  L[e] = x != 0 ? host_log2f(lv, fabsf(x)) : -100.f;
+                      const float* __restrict__ W, Skew3 s, LibmView lv, float* __restrict__ L) {
+  const uint32_t row = blockIdx.x;
+  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  if (k >= s.r3) return;
+  const int i = (int)(row / (uint32_t)s.r2), j = (int)(row - (uint32_t)i * s.r2);
+  const size_t v = (size_t)row * s.r3 + k, o = s.at(i, j, k);
+  const float x0 = U[v], x1 = V[v], x2 = W[v];
+  L[o] = x0 != 0 ? host_log2f(lv, fabsf(x0)) : -100.f;
+  L[s.per + o] = x1 != 0 ? host_log2f(lv, fabsf(x1)) : -100.f;
+  L[2 * s.per + o] = x2 != 0 ? host_log2f(lv, fabsf(x2)) : -100.f;
 }
 
 // ------------------------------------------------------------- wavefront --
@@ -144,22 +214,23 @@ __device__ __forceinline__ float shr1(float src, float lane0) {
 }
 
 struct Wave3d {
-  const float* lx;       // compress: log values [3][n]; decompress: initial values of unpredictables
+  const float* lx;       // compress: log values [3][per]; decompress: initial values of unpredictables
   const uint8_t* code;   // compress: eb codes; decompress: stream ids
-  const uint16_t* qin;   // decompress: quant codes [3][n]
-  float* R;              // reconstruction [3][n] (log domain)
-  uint16_t* qout;        // compress: quant codes [3][n]
+  const uint16_t* qin;   // decompress: quant codes [3][per]
+  float* R;              // reconstruction [3][per] (log domain)
+  uint16_t* qout;        // compress: quant codes [3][per]
   uint8_t* vid;          // compress: stream id per vertex
   int* prog;             // [r1][nb] completed steps
   int* ticket;
-  int r1, r2, r3, nb;
+  Skew3 s;               // all arrays above are in wave layout
 };
 
 template <bool COMPRESS>
 __global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
   const int lane = threadIdx.x;
-  const int r1 = a.r1, r2 = a.r2, r3 = a.r3, nb = a.nb;
-  const size_t n = (size_t)r1 * r2 * r3, d0 = (size_t)r2 * r3;
+  const Skew3 sk = a.s;
+  const int r1 = sk.r1, r2 = sk.r2, r3 = sk.r3, nb = sk.nb, Wd = sk.W;
+  const size_t per = sk.per;
   const int items = r1 * nb;
   for (;;) {
     int item = 0;
@@ -169,15 +240,16 @@ __global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
     const int i = item / nb, b = item - i * nb;
     const int j = b * 64 + lane;
     const bool jv = j < r2;
-    const size_t row = (size_t)i * d0 + (size_t)(jv ? j : 0) * r3;
-    const size_t urow = i ? row - d0 : 0;                                  // plane i-1, same j
-    const size_t brow = (size_t)i * d0 + (size_t)(b * 64 - 1) * r3;        // plane i, j0-1
-    const size_t ubrow = i ? brow - d0 : 0;                                // plane i-1, j0-1
+    // element (step t, this lane) of item (i, b) sits at base + t * Wd
+    const size_t row = sk.blk(i, b) + (jv ? lane : 0);
+    const size_t urow = i ? sk.blk(i - 1, b) + (jv ? lane : 0) : 0;  // plane i-1, same j
+    const size_t brow = b ? sk.blk(i, b - 1) + 63 * 64 + 63 : 0;      // plane i, row j0-1 at t + 63
+    const size_t ubrow = (b && i) ? sk.blk(i - 1, b - 1) + 63 * 64 + 63 : 0;
     const int* prog_up = a.prog + (i ? (i - 1) * nb + b : 0);
     const int* prog_left = a.prog + i * nb + (b ? b - 1 : 0);
     int* prog_me = a.prog + i * nb + b;
     float Rp[3] = {0, 0, 0}, Bp[3] = {0, 0, 0}, Dp[3] = {0, 0, 0}, Fp[3] = {0, 0, 0};
-    const int steps = r3 + 63;
+    const int steps = sk.T;
     for (int t0 = 0; t0 < steps; t0 += kCH3) {
       // wait for plane i-1 (this block) and, for lane 0's j-1 column, the block to the left
       if (i > 0) {
@@ -193,8 +265,9 @@ __global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
       float bl[3] = {0, 0, 0}, bu[3] = {0, 0, 0};
       if (b > 0 && lane < kCH3 && t0 + lane < r3) {
         for (int p = 0; p < 3; p++) {
-          bl[p] = __int_as_float(ld_coh((const int*)(a.R + p * n + brow + t0 + lane)));
-          if (i > 0) bu[p] = __int_as_float(ld_coh((const int*)(a.R + p * n + ubrow + t0 + lane)));
+          const size_t o = p * per + (size_t)(t0 + lane) * 64;
+          bl[p] = __int_as_float(ld_coh((const int*)(a.R + brow + o)));
+          if (i > 0) bu[p] = __int_as_float(ld_coh((const int*)(a.R + ubrow + o)));
         }
       }
       float xs[kCH3][3], ups[kCH3][3];
@@ -204,17 +277,19 @@ __global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
       for (int s = 0; s < kCH3; s++) {
         const int k = t0 + s - lane;
         const bool act = jv && k >= 0 && k < r3;
-        cs[s] = act ? a.code[row + k] : 0;
+        const size_t ts = (size_t)(t0 + s) * Wd;
+        cs[s] = act ? a.code[row + ts] : 0;
         for (int p = 0; p < 3; p++) {
-          xs[s][p] = act ? a.lx[p * n + row + k] : 0.f;
-          ups[s][p] = (act && i > 0) ? __int_as_float(ld_coh((const int*)(a.R + p * n + urow + k))) : 0.f;
-          if (!COMPRESS) qs[s][p] = act ? a.qin[p * n + row + k] : 0;
+          xs[s][p] = act ? a.lx[p * per + row + ts] : 0.f;
+          ups[s][p] = (act && i > 0) ? __int_as_float(ld_coh((const int*)(a.R + p * per + urow + ts))) : 0.f;
+          if (!COMPRESS) qs[s][p] = act ? a.qin[p * per + row + ts] : 0;
         }
       }
 #pragma unroll
       for (int s = 0; s < kCH3; s++) {
         const int k = t0 + s - lane;
         const bool act = jv && k >= 0 && k < r3;
+        const size_t at = row + (size_t)(t0 + s) * Wd;
         float res[3];
         float dec[3];
         bool okall = true;
@@ -248,13 +323,13 @@ __global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
           const bool pred_ok = c != 255 && c != 0 && okall;
           for (int p = 0; p < 3; p++) res[p] = pred_ok ? dec[p] : xs[s][p];
           if (act) {
-            a.vid[row + k] = (uint8_t)(c == 255 ? 0 : (pred_ok ? c : 24));
+            a.vid[at] = (uint8_t)(c == 255 ? 0 : (pred_ok ? c : 24));
             if (pred_ok)
-              for (int p = 0; p < 3; p++) a.qout[p * n + row + k] = qc[p];
+              for (int p = 0; p < 3; p++) a.qout[p * per + at] = qc[p];
           }
         }
         for (int p = 0; p < 3; p++) {
-          if (act) __hip_atomic_store((int*)(a.R + p * n + row + k), __float_as_int(res[p]),
+          if (act) __hip_atomic_store((int*)(a.R + p * per + at), __float_as_int(res[p]),
                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           Rp[p] = k < 0 ? 0.f : res[p];
         }
@@ -275,58 +350,89 @@ __global__ void k_zero_i32_3d(int* p, size_t n) {
 // ---------------------------------------------------------- huffman ------
 constexpr int kVChunk = 256;  // vertices per chunk
 
+__device__ __forceinline__ bool is_pred3(uint32_t id) { return id != 0 && id != 24; }
+
+// Huffman sources: stream (row-major) order over wave-layout arrays.
 struct SrcIds3d {
   const uint8_t* vid;
+  Skew3 s;
   size_t n;
   template <class F>
   __device__ void each(size_t c, F f) const {
     const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-    for (size_t v = b; v < e; v++) f((uint32_t)vid[v]);
+    walk3(s, b, e, [&](size_t x) { f((uint32_t)vid[x]); });
   }
 };
-
-__device__ __forceinline__ bool is_pred3(uint32_t id) { return id != 0 && id != 24; }
 
 struct SrcCodes3d {
   const uint8_t* vid;
   const uint16_t* q;
+  Skew3 s;
   size_t n;
   template <class F>
   __device__ void each(size_t c, F f) const {
     const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-    for (size_t v = b; v < e; v++)
-      if (is_pred3(vid[v])) {
-        f((uint32_t)q[v]);
-        f((uint32_t)q[n + v]);
-        f((uint32_t)q[2 * n + v]);
+    walk3(s, b, e, [&](size_t x) {
+      if (is_pred3(vid[x])) {
+        f((uint32_t)q[x]);
+        f((uint32_t)q[s.per + x]);
+        f((uint32_t)q[2 * s.per + x]);
       }
+    });
   }
 };
 
-__global__ void k_chunk_counts3d(const uint8_t* __restrict__ vid, size_t n, size_t nch,
-                                 uint32_t* __restrict__ npred, uint32_t* __restrict__ nunp) {
-  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+// The chunk kernels below run one wave per 256-vertex chunk (4 waves per
+// block); lane l takes vertices c*256 + 64*r + l, r = 0..3, and ranks the
+// predictable / unpredictable ones with ballots.
+#define CHUNK_WAVE_PROLOGUE                                                 \
+  const size_t c = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);             \
+  const int lane = threadIdx.x & 63;                                         \
   if (c >= nch) return;
-  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-  uint32_t p = 0;
-  for (size_t v = b; v < e; v++) p += is_pred3(vid[v]);
-  npred[c] = p;
-  nunp[c] = (uint32_t)(e - b) - p;
+
+// per-chunk predictable / unpredictable vertex counts; vid in wave layout
+// (SKEW) or row-major
+template <bool SKEW>
+__global__ void __launch_bounds__(256) k_chunk_counts3d(const uint8_t* __restrict__ vid, Skew3 s, size_t n,
+                                                        size_t nch, uint32_t* __restrict__ npred,
+                                                        uint32_t* __restrict__ nunp) {
+  CHUNK_WAVE_PROLOGUE
+  uint32_t p = 0, u = 0;
+  for (int r = 0; r < 4; r++) {
+    const size_t v = c * kVChunk + r * 64 + lane;
+    const bool in = v < n;
+    const bool pr = in && is_pred3(vid[SKEW ? s.at_v(in ? v : 0) : (in ? v : 0)]);
+    const uint64_t mp = __ballot(pr), mu = __ballot(in && !pr);
+    p += __popcll(mp);
+    u += __popcll(mu);
+  }
+  if (lane == 0) {
+    npred[c] = p;
+    nunp[c] = u;
+  }
 }
 
-__global__ void k_gather_unpred3d(const uint8_t* __restrict__ vid, const float* __restrict__ U,
-                                  const float* __restrict__ V, const float* __restrict__ W, size_t n,
-                                  size_t nch, const uint64_t* __restrict__ uoff, float* __restrict__ out) {
-  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (c >= nch) return;
-  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-  uint64_t o = 3 * uoff[c];
-  for (size_t v = b; v < e; v++)
-    if (!is_pred3(vid[v])) {
-      out[o++] = U[v];
-      out[o++] = V[v];
-      out[o++] = W[v];
+// compress: original values of unpredictable vertices, 3 floats each, in order
+__global__ void __launch_bounds__(256) k_gather_unpred3d(const uint8_t* __restrict__ vid, Skew3 s,
+                                                         const float* __restrict__ U,
+                                                         const float* __restrict__ V,
+                                                         const float* __restrict__ W, size_t n,
+                                                         size_t nch, const uint64_t* __restrict__ uoff,
+                                                         float* __restrict__ out) {
+  CHUNK_WAVE_PROLOGUE
+  uint64_t o = uoff[c];
+  for (int r = 0; r < 4; r++) {
+    const size_t v = c * kVChunk + r * 64 + lane;
+    const bool un = v < n && !is_pred3(vid[s.at_v(v < n ? v : 0)]);
+    const uint64_t m = __ballot(un);
+    if (un) {
+      const uint64_t at = 3 * (o + lanes_below(m));
+      out[at] = U[v];
+      out[at + 1] = V[v];
+      out[at + 2] = W[v];
     }
+    o += __popcll(m);
+  }
 }
 
 struct SinkU8 {
@@ -338,70 +444,92 @@ struct SinkU16 {
   __device__ void put(uint64_t i, uint32_t s) const { p[i] = (uint16_t)s; }
 };
 
-// decompress: per-vertex codes and initial values of unpredictable vertices
-__global__ void k_scatter3d(const uint8_t* __restrict__ vid, const uint16_t* __restrict__ qc,
-                            const float* __restrict__ unp, size_t n, size_t nch,
-                            const uint64_t* __restrict__ poff, const uint64_t* __restrict__ uoff,
-                            uint64_t nq, uint64_t nu, LibmView lv, uint16_t* __restrict__ q,
-                            float* __restrict__ init, int* __restrict__ err) {
-  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (c >= nch) return;
-  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-  uint64_t po = 3 * poff[c], uo = 3 * uoff[c];
-  for (size_t v = b; v < e; v++) {
-    if (is_pred3(vid[v])) {
-      if (po + 3 > nq) {
-        *err = 1;
-        return;
+// decompress: stream ids, codes and initial (log) values of unpredictable
+// vertices, from stream order into wave layout
+__global__ void __launch_bounds__(256) k_scatter3d(const uint8_t* __restrict__ vid, Skew3 s,
+                                                   const uint16_t* __restrict__ qc,
+                                                   const float* __restrict__ unp, size_t n, size_t nch,
+                                                   const uint64_t* __restrict__ poff,
+                                                   const uint64_t* __restrict__ uoff, uint64_t nq,
+                                                   uint64_t nu, LibmView lv, uint8_t* __restrict__ vs,
+                                                   uint16_t* __restrict__ q, float* __restrict__ init,
+                                                   int* __restrict__ err) {
+  CHUNK_WAVE_PROLOGUE
+  uint64_t po = poff[c], uo = uoff[c];
+  for (int r = 0; r < 4; r++) {
+    const size_t v = c * kVChunk + r * 64 + lane;
+    const bool in = v < n;
+    const uint32_t id = in ? vid[v] : 0;
+    const bool pr = in && is_pred3(id), un = in && !pr;
+    const uint64_t mp = __ballot(pr), mu = __ballot(un);
+    if (in) {
+      const size_t x = s.at_v(v);
+      vs[x] = (uint8_t)id;
+      if (pr) {
+        const uint64_t a = 3 * (po + lanes_below(mp));
+        if (a + 3 > nq) {
+          *err = 1;
+        } else {
+          q[x] = qc[a];
+          q[s.per + x] = qc[a + 1];
+          q[2 * s.per + x] = qc[a + 2];
+        }
+      } else {
+        const uint64_t a = 3 * (uo + lanes_below(mu));
+        if (a + 3 > nu) {
+          *err = 1;
+        } else {
+          for (int p = 0; p < 3; p++) {
+            const float xv = unp[a + p];
+            init[p * s.per + x] = (xv == 0) ? -100.f : host_log2f(lv, fabsf(xv));
+          }
+        }
       }
-      q[v] = qc[po];
-      q[n + v] = qc[po + 1];
-      q[2 * n + v] = qc[po + 2];
-      po += 3;
-    } else {
-      if (uo + 3 > nu) {
-        *err = 1;
-        return;
-      }
-      for (int p = 0; p < 3; p++) {
-        const float x = unp[uo + p];
-        init[p * n + v] = (x == 0) ? -100.f : host_log2f(lv, fabsf(x));
-      }
-      uo += 3;
     }
+    po += __popcll(mp);
+    uo += __popcll(mu);
   }
 }
 
 // final values (src/sz_decompress_cp_preserve_3d.cpp:89-104)
-__global__ void k_finish3d(const uint8_t* __restrict__ vid, const float* __restrict__ R,
-                           const uint8_t* __restrict__ smap, size_t smap_bytes,
-                           const float* __restrict__ unp, size_t n, size_t nch,
-                           const uint64_t* __restrict__ uoff, LibmView ev, float* __restrict__ U,
-                           float* __restrict__ V, float* __restrict__ W) {
-  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (c >= nch) return;
-  const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-  uint64_t uo = 3 * uoff[c];
+__global__ void __launch_bounds__(256) k_finish3d(const uint8_t* __restrict__ vid, Skew3 s,
+                                                  const float* __restrict__ R,
+                                                  const uint8_t* __restrict__ smap, size_t smap_bytes,
+                                                  const float* __restrict__ unp, size_t n, size_t nch,
+                                                  const uint64_t* __restrict__ uoff, uint64_t nu,
+                                                  LibmView ev, float* __restrict__ U,
+                                                  float* __restrict__ V, float* __restrict__ W) {
+  CHUNK_WAVE_PROLOGUE
+  uint64_t uo = uoff[c];
   float* out[3] = {U, V, W};
-  for (size_t v = b; v < e; v++) {
-    if (!is_pred3(vid[v])) {
-      for (int p = 0; p < 3; p++) out[p][v] = unp[uo + p];
-      uo += 3;
-    } else {
+  for (int r = 0; r < 4; r++) {
+    const size_t v = c * kVChunk + r * 64 + lane;
+    const bool in = v < n;
+    const bool un = in && !is_pred3(vid[in ? v : 0]);
+    const uint64_t mu = __ballot(un);
+    if (un) {
+      const uint64_t a = 3 * (uo + lanes_below(mu));
+      if (a + 3 <= nu)
+        for (int p = 0; p < 3; p++) out[p][v] = unp[a + p];
+    } else if (in) {
+      const size_t x = s.at_v(v);
       for (int p = 0; p < 3; p++) {
-        const float x = R[p * n + v];
+        const float lx = R[p * s.per + x];
         const bool pos = (smap[p * smap_bytes + v / 8] >> (7 - (v % 8))) & 1;
         float y;
-        if (x < -99) y = 0;
-        else {
-          const float t = host_exp2f(ev, x);
+        if (lx < -99) {
+          y = 0;
+        } else {
+          const float t = host_exp2f(ev, lx);
           y = pos ? t : -t;
         }
         out[p][v] = y;
       }
     }
+    uo += __popcll(mu);
   }
 }
+#undef CHUNK_WAVE_PROLOGUE
 
 template <typename V>
 void put(std::vector<uint8_t>& o, const V& v) {
@@ -412,9 +540,9 @@ void put(std::vector<uint8_t>& o, const V& v) {
 
 void launch_wave3d(Ctx& ctx, Wave3d& w, bool compress) {
   hipStream_t st = ctx.stream();
-  const size_t nz = (size_t)w.r1 * w.nb + 1;
+  const size_t nz = (size_t)w.s.r1 * w.s.nb + 1;
   k_zero_i32_3d<<<(int)std::min<size_t>(div_up(nz, 256), 1024), 256, 0, st>>>(w.prog, nz);
-  const int items = w.r1 * w.nb;
+  const int items = w.s.r1 * w.s.nb;
   const int grid = items < 4096 ? items : 4096;
   if (compress) k_lorenzo3d<true><<<grid, 64, 0, st>>>(w);
   else k_lorenzo3d<false><<<grid, 64, 0, st>>>(w);
@@ -426,28 +554,33 @@ void launch_wave3d(Ctx& ctx, Wave3d& w, bool compress) {
 void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1,
                         size_t r2, size_t r3, double max_pwr_eb, DeviceStream& out) {
   CPSZ_REQUIRE(dU && dV && dW && r1 && r2 && r3, kErrArg, "compress_3d: bad arguments");
-  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 34),
+  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 32),
                kErrUnsupported, "compress_3d: field too large");
   hipStream_t st = ctx.stream();
   const size_t n = r1 * r2 * r3;
+  const Skew3 sk = make_skew(r1, r2, r3);
   ctx.stages.clear();
   ctx.mark("start");
   const Ctx::LibmDev& lm = ctx.libm_dev();
-  uint8_t* ebc = ctx.buf<uint8_t>("c3.ebc", n);
+  uint8_t* ebc = ctx.buf<uint8_t>("c3.ebc", sk.per);
   int* flags = ctx.buf<int>("c3.flags", 4);
   CPSZ_HIP(hipMemsetAsync(flags, 0, 16, st));
   {
     dim3 grid((unsigned)div_up(r3, kTK), (unsigned)div_up(r2, kTJ), (unsigned)div_up(r1, kTI));
     k_eb3d<<<grid, 256, 0, st>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
-                                 ctx.q3d_size(), ebc, flags);
+                                 ctx.q3d_size(), sk, ebc, flags);
     CPSZ_HIP(hipGetLastError());
   }
   ctx.mark("eb");
   const size_t smap = (n - 1) / 8 + 1, nw = div_up(n, 32);
   uint32_t* sm = ctx.buf<uint32_t>("c3.smap", 3 * nw);
   k_signmap<<<(int)div_up(3 * nw, 256), 256, 0, st>>>(dU, dV, dW, n, nw, sm);
-  float* L = ctx.buf<float>("c3.log", 3 * n);
-  k_log<<<(int)div_up(3 * n, 256), 256, 0, st>>>(dU, dV, dW, n, lm.log2f, L);
+  float* L = ctx.buf<float>("c3.log", 3 * sk.per);
+  {
+    const int bx = r3 >= 256 ? 256 : (int)div_up(r3, 64) * 64;
+    dim3 grid((unsigned)(r1 * r2), (unsigned)div_up(r3, bx));
+    k_log<<<grid, bx, 0, st>>>(dU, dV, dW, sk, lm.log2f, L);
+  }
   CPSZ_HIP(hipGetLastError());
   ctx.mark("log");
   int hflags[4];
@@ -457,27 +590,24 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
                "compress_3d: non-finite input or bound (values too large for the reference's "
                "float triple products)");
   // ---- wavefront
-  const int nb = (int)div_up(r2, 64);
+  const int nb = sk.nb;
   Wave3d w{};
   w.lx = L;
   w.code = ebc;
-  w.R = ctx.buf<float>("c3.R", 3 * n);
-  w.qout = ctx.buf<uint16_t>("c3.q", 3 * n);
-  w.vid = ctx.buf<uint8_t>("c3.vid", n);
+  w.R = ctx.buf<float>("c3.R", 3 * sk.per);
+  w.qout = ctx.buf<uint16_t>("c3.q", 3 * sk.per);
+  w.vid = ctx.buf<uint8_t>("c3.vid", sk.per);
   w.prog = ctx.buf<int>("c3.prog", (size_t)r1 * nb + 1);
   w.ticket = w.prog + (size_t)r1 * nb;
-  w.r1 = (int)r1;
-  w.r2 = (int)r2;
-  w.r3 = (int)r3;
-  w.nb = nb;
+  w.s = sk;
   launch_wave3d(ctx, w, true);
   ctx.mark("lorenzo");
   // ---- histograms
   uint32_t* hist = ctx.buf<uint32_t>("c3.hist", kMaxEbCode + 65536);
   CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 65536) * 4, st));
   const size_t nch = div_up(n, kVChunk);
-  SrcIds3d s_ids{w.vid, n};
-  SrcCodes3d s_codes{w.vid, w.qout, n};
+  SrcIds3d s_ids{w.vid, sk, n};
+  SrcCodes3d s_codes{w.vid, w.qout, sk, n};
   {
     const int grid = (int)std::min<size_t>(div_up(nch, 256), 1024);
     k_hist<<<grid, 256, kMaxEbCode * 4, st>>>(s_ids, nch, hist, 0, kMaxEbCode);
@@ -511,7 +641,7 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
   uint64_t* totals = ctx.buf<uint64_t>("c3.totals", 8);
   k_chunk_bits<<<(int)div_up(nch, 256), 256, 0, st>>>(s_ids, nch, cb_len, bits);
   k_chunk_bits<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
-  k_chunk_counts3d<<<(int)div_up(nch, 256), 256, 0, st>>>(w.vid, n, nch, cnt, cnt + nch);
+  k_chunk_counts3d<true><<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, n, nch, cnt, cnt + nch);
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits, offs, nch, totals + 0);
   scan_u64(ctx, bits + nch, offs + nch, nch, totals + 1);
@@ -531,8 +661,8 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
   k_encode<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536,
                                                   offs + nch, words + w_id);
   float* unp = ctx.buf<float>("c3.unp", 3 * nunp + 1);
-  k_gather_unpred3d<<<(int)div_up(nch, 256), 256, 0, st>>>(w.vid, dU, dV, dW, n, nch, coff + nch + 1,
-                                                           unp);
+  k_gather_unpred3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, dU, dV, dW, n, nch, coff + nch + 1,
+                                                         unp);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
   // ---- stream (src/sz_compress_cp_preserve_3d.cpp:556-579)
@@ -618,8 +748,11 @@ struct Reader3 {
 void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
                           size_t r1, size_t r2, size_t r3, float* dU, float* dV, float* dW) {
   CPSZ_REQUIRE(d_in && dU && dV && dW && r1 && r2 && r3, kErrArg, "decompress_3d: bad arguments");
+  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 32),
+               kErrUnsupported, "decompress_3d: field too large");
   hipStream_t st = ctx.stream();
   const size_t n = r1 * r2 * r3;
+  const Skew3 sk = make_skew(r1, r2, r3);
   ctx.stages.clear();
   ctx.mark("start");
   const Ctx::LibmDev& lm = ctx.libm_dev();
@@ -677,37 +810,35 @@ void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
   const size_t nch = div_up(n, kVChunk);
   uint32_t* cnt = ctx.buf<uint32_t>("d3.cnt", 2 * nch);
   uint64_t* coff = ctx.buf<uint64_t>("d3.coff", 2 * nch + 4);
-  k_chunk_counts3d<<<(int)div_up(nch, 256), 256, 0, st>>>(vid, n, nch, cnt, cnt + nch);
+  k_chunk_counts3d<false><<<(int)div_up(nch, 4), 256, 0, st>>>(vid, sk, n, nch, cnt, cnt + nch);
   scan_u32_to_u64(ctx, cnt, coff, nch, coff + 2 * nch + 2);
   scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, coff + 2 * nch + 3);
   float* unp = ctx.buf<float>("d3.unp", nunp_f + 1);
   if (nunp_f) CPSZ_HIP(hipMemcpyAsync(unp, d_in + unp_off, 4 * nunp_f, hipMemcpyDeviceToDevice, st));
-  uint16_t* q = ctx.buf<uint16_t>("d3.q", 3 * n);
-  float* init = ctx.buf<float>("d3.init", 3 * n);
+  uint16_t* q = ctx.buf<uint16_t>("d3.q", 3 * sk.per);
+  float* init = ctx.buf<float>("d3.init", 3 * sk.per);
+  uint8_t* vs = ctx.buf<uint8_t>("d3.vs", sk.per);
   int* err = ctx.buf<int>("d3.err", 2);
   CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
-  k_scatter3d<<<(int)div_up(nch, 256), 256, 0, st>>>(vid, qc, unp, n, nch, coff, coff + nch + 1,
-                                                     data_num, nunp_f, lm.log2f, q, init, err);
+  k_scatter3d<<<(int)div_up(nch, 4), 256, 0, st>>>(vid, sk, qc, unp, n, nch, coff, coff + nch + 1,
+                                                   data_num, nunp_f, lm.log2f, vs, q, init, err);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("scatter");
-  const int nbk = (int)div_up(r2, 64);
+  const int nbk = sk.nb;
   Wave3d w{};
   w.lx = init;
-  w.code = vid;
+  w.code = vs;
   w.qin = q;
-  w.R = ctx.buf<float>("d3.R", 3 * n);
+  w.R = ctx.buf<float>("d3.R", 3 * sk.per);
   w.prog = ctx.buf<int>("d3.prog", (size_t)r1 * nbk + 1);
   w.ticket = w.prog + (size_t)r1 * nbk;
-  w.r1 = (int)r1;
-  w.r2 = (int)r2;
-  w.r3 = (int)r3;
-  w.nb = nbk;
+  w.s = sk;
   launch_wave3d(ctx, w, false);
   ctx.mark("lorenzo");
   uint8_t* sm = ctx.buf<uint8_t>("d3.smap", 3 * smap + 8);
   CPSZ_HIP(hipMemcpyAsync(sm, d_in + 8, 3 * smap, hipMemcpyDeviceToDevice, st));
-  k_finish3d<<<(int)div_up(nch, 256), 256, 0, st>>>(vid, w.R, sm, smap, unp, n, nch, coff + nch + 1,
-                                                    lm.exp2f, dU, dV, dW);
+  k_finish3d<<<(int)div_up(nch, 4), 256, 0, st>>>(vid, sk, w.R, sm, smap, unp, n, nch, coff + nch + 1,
+                                                  nunp_f, lm.exp2f, dU, dV, dW);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("finish");
   int herr[2];

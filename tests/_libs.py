@@ -201,6 +201,12 @@ class Reference:
         self.lib.ref_free(p)
         return out
 
+    def zstd_compress(self, data, level=3):
+        data = np.ascontiguousarray(data, np.uint8)
+        p = C.c_void_p()
+        n = self.lib.ref_zstd_compress(level, _ptr(data), data.size, C.byref(p))
+        return self._take(p, n)
+
     def compress_2d(self, U, V, eb=0.1):
         r1, r2 = U.shape
         s = _SZ()

@@ -28,6 +28,32 @@ def test_version_and_no_device_error():
     assert cz.device_count() >= 0
 
 
+def test_dropin_header_caller_links_and_host_stage(tmp_path):
+    """A caller of the reference's C++ API builds against the drop-in header; on a
+    machine without a GPU the codec fails cleanly (no exit, no fallback) and says why."""
+    import subprocess
+    demo = os.path.join(ROOT, "tests", "cpp", "dropin_roundtrip")
+    assert os.path.exists(demo), "build() compiles tests/cpp/dropin_roundtrip"
+    if cz.device_count() > 0:
+        pytest.skip("a GPU is present: covered by test_gpu_dropin")
+    inp = tmp_path / "in.bin"
+    np.zeros(2 * 30 * 40, np.float32).tofile(inp)
+    r = subprocess.run([demo, "2d", str(inp), str(tmp_path / "o"), "30", "40", "0.1"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and r.stdout.startswith("fail compress") and "HIP" in r.stdout
+
+
+def test_dropin_lossless_matches_reference_zstd_level3():
+    """The ZSTD stage is the reference's (level 3 frame, src/sz_lossless.cpp:11-45)."""
+    rng = np.random.default_rng(1)
+    data = rng.integers(0, 7, 100000).astype(np.uint8)
+    z = cz.sz_lossless_compress(data, 3)
+    assert np.array_equal(cz.sz_lossless_decompress(z, data.size), data)
+    from _libs import Reference, have_reference
+    if have_reference():
+        assert np.array_equal(z, Reference().zstd_compress(data, 3))
+
+
 def _tables():
     L = cz.lib()
     L.cpsz_eb_tables.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_int,
