@@ -17,8 +17,8 @@
 // Bit-exactness: every float/double operation follows the reference's types
 // and order (src/sz_compress_cp_preserve_2d.cpp:17-172, quantize in
 // include/sz_compression_utils.hpp:50-65); the Lorenzo step runs in FP32, which
-// reproduces the reference's double arithmetic exactly (see DESIGN.md notes in
-// the function comments below).
+// reproduces the reference's double arithmetic exactly (the argument is in the
+// comments of lorenzo2d_step.hpp).
 #include <cfloat>
 #include <cmath>
 #include <cstring>

@@ -39,14 +39,20 @@ namespace {
 
 constexpr int kTI = 4, kTJ = 8, kTK = 64;  // vertex tile of the eb kernel
 
-// Wave layout: W lanes per j block (64, or r2 when r2 < 64), T = r3 + W - 1
-// steps per (plane, block) item.
+constexpr int kP3 = 4;  // planes (waves) per wavefront workgroup
+constexpr int kC3 = 8;  // steps per wavefront chunk (a multiple of 4)
+
+// Wave layout: item (plane i, 64-row block b) holds T steps (r3 + 63 rounded
+// up to whole chunks) of 64 lanes; lane l's values for steps 4u..4u+3 are
+// contiguous, so one 16-byte load gives a lane four steps of a float array.
+// Vertex (i, j, k) sits at step t = k + (j mod 64) of lane j mod 64.
 struct Skew3 {
-  int r1, r2, r3, nb, W, T;
+  int r1, r2, r3, nb, T;
   size_t per;  // elements per component
-  __host__ __device__ size_t blk(int i, int b) const { return ((size_t)i * nb + b) * (size_t)T * W; }
+  __host__ __device__ size_t blk(int i, int b) const { return ((size_t)i * nb + b) * (size_t)T * 64; }
   __host__ __device__ size_t at(int i, int j, int k) const {
-    return blk(i, j >> 6) + (size_t)(k + (j & 63)) * W + (j & 63);
+    const int l = j & 63, t = k + l;
+    return blk(i, j >> 6) + (size_t)(t >> 2) * 256 + l * 4 + (t & 3);
   }
   // vertex index (row-major, < 2^32) -> wave-layout index
   __device__ size_t at_v(size_t v) const {
@@ -62,9 +68,8 @@ Skew3 make_skew(size_t r1, size_t r2, size_t r3) {
   s.r2 = (int)r2;
   s.r3 = (int)r3;
   s.nb = (int)div_up(r2, 64);
-  s.W = r2 < 64 ? (int)r2 : 64;
-  s.T = (int)r3 + s.W - 1;
-  s.per = r1 * (size_t)s.nb * s.T * s.W;
+  s.T = (int)(div_up(r3 + 63, kC3) * kC3);
+  s.per = r1 * (size_t)s.nb * s.T * 64;
   CPSZ_REQUIRE(s.per < (1ull << 31), kErrUnsupported,
                "3D field too thin for the wavefront layout (r1 * ceil(r2/64) * (r3+63) * 64 >= 2^31)");
   return s;
@@ -78,6 +83,7 @@ __device__ __forceinline__ void walk3(const Skew3& s, size_t b, size_t e, F f) {
   int k = (int)(b - row * s.r3);
   int i = (int)(row / (size_t)s.r2), j = (int)(row - (size_t)i * s.r2);
   size_t idx = s.at(i, j, k);
+  int t = k + (j & 63);
   for (size_t v = b; v < e; v++) {
     f(idx);
     if (++k == s.r3) {
@@ -87,8 +93,10 @@ __device__ __forceinline__ void walk3(const Skew3& s, size_t b, size_t e, F f) {
         ++i;
       }
       idx = s.at(i, j, 0);
+      t = j & 63;
     } else {
-      idx += s.W;
+      idx += ((t & 3) == 3) ? 253 : 1;
+      ++t;
     }
   }
 }
@@ -200,7 +208,15 @@ __global__ void k_log(const float* __restrict__ U, const float* __restrict__ V,
 }
 
 // ------------------------------------------------------------- wavefront --
-constexpr int kCH3 = 32;
+// A workgroup of kP3 waves owns kP3 consecutive planes of one 64-row j block.
+// Wave w sweeps plane g*kP3 + w in chunks of kC3 steps and computes chunk c in
+// iteration c + w: a systolic pipeline with one barrier per iteration. Plane
+// i-1 values reach wave w > 0 through LDS from wave w-1 (written one iteration
+// earlier); wave 0 reads them from the previous group's last plane in HBM, and
+// lane 0 reads the j0-1 row from the block to the left, both behind
+// per-(plane, block) progress counters. Everything chunk c+1 needs is loaded
+// while chunk c computes; a chunk's progress is published one iteration after
+// its stores were issued.
 
 __device__ __forceinline__ int ld_coh(const int* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -208,9 +224,28 @@ __device__ __forceinline__ int ld_coh(const int* p) {
 __device__ __forceinline__ void st_coh(int* p, int v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ float ld_cohf(const float* p) { return __int_as_float(ld_coh((const int*)p)); }
+// 16-byte agent-coherent load / store as two 8-byte atomics (p 16-byte aligned)
+__device__ __forceinline__ float4 ld_coh4(const float* p) {
+  const unsigned long long* q = (const unsigned long long*)p;
+  const unsigned long long lo = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long hi = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return make_float4(__uint_as_float((uint32_t)lo), __uint_as_float((uint32_t)(lo >> 32)),
+                     __uint_as_float((uint32_t)hi), __uint_as_float((uint32_t)(hi >> 32)));
+}
+__device__ __forceinline__ void st_coh4(float* p, float4 v) {
+  unsigned long long* q = (unsigned long long*)p;
+  __hip_atomic_store(q, ((unsigned long long)__float_as_uint(v.y) << 32) | __float_as_uint(v.x),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(q + 1, ((unsigned long long)__float_as_uint(v.w) << 32) | __float_as_uint(v.z),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ float shr1(float src, float lane0) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(src),
                                                     0x138 /* wave_shr:1 */, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float comp4(const float4& v, int s) {
+  return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
 }
 
 struct Wave3d {
@@ -225,119 +260,181 @@ struct Wave3d {
   Skew3 s;               // all arrays above are in wave layout
 };
 
+constexpr int kG3 = kC3 / 4;  // 4-step groups per chunk
+
+// what one chunk of one wave reads
+struct Chunk3 {
+  float4 x[3][kG3];   // log values (compress) / initial values (decompress)
+  uint32_t cd[kG3];   // 4 eb codes (compress) / stream ids (decompress) per group
+  uint2 q[3][kG3];    // decompress: 4 quantization codes per group
+  float4 up[3][kG3];  // wave 0: plane i-1 from HBM
+  float bl[3], bu[3]; // lane s: row j0-1 at step t0 + s (own plane, plane above)
+};
+
 template <bool COMPRESS>
-__global__ void __launch_bounds__(64) k_lorenzo3d(Wave3d a) {
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(64 * kP3) k_wave3d(Wave3d a) {
+  __shared__ float4 hand[kP3 - 1][2][kG3][3][64];
+  __shared__ int s_item;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const Skew3 sk = a.s;
-  const int r1 = sk.r1, r2 = sk.r2, r3 = sk.r3, nb = sk.nb, Wd = sk.W;
+  const int r1 = sk.r1, r2 = sk.r2, r3 = sk.r3, nb = sk.nb, T = sk.T;
   const size_t per = sk.per;
-  const int items = r1 * nb;
+  const int ngroups = (r1 + kP3 - 1) / kP3, items = ngroups * nb, nch = T / kC3;
   for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(a.ticket, 1);
-    item = __shfl(item, 0);
+    if (threadIdx.x == 0) s_item = atomicAdd(a.ticket, 1);
+    __syncthreads();
+    const int item = s_item;
+    __syncthreads();
     if (item >= items) return;
-    const int i = item / nb, b = item - i * nb;
-    const int j = b * 64 + lane;
-    const bool jv = j < r2;
-    // element (step t, this lane) of item (i, b) sits at base + t * Wd
-    const size_t row = sk.blk(i, b) + (jv ? lane : 0);
-    const size_t urow = i ? sk.blk(i - 1, b) + (jv ? lane : 0) : 0;  // plane i-1, same j
-    const size_t brow = b ? sk.blk(i, b - 1) + 63 * 64 + 63 : 0;      // plane i, row j0-1 at t + 63
-    const size_t ubrow = (b && i) ? sk.blk(i - 1, b - 1) + 63 * 64 + 63 : 0;
-    const int* prog_up = a.prog + (i ? (i - 1) * nb + b : 0);
-    const int* prog_left = a.prog + i * nb + (b ? b - 1 : 0);
-    int* prog_me = a.prog + i * nb + b;
+    const int g = item / nb, b = item - g * nb;
+    const int i = g * kP3 + w;
+    const bool pv = i < r1;
+    const size_t base = pv ? sk.blk(i, b) : 0;
+    const size_t ubase = (pv && i > 0) ? sk.blk(i - 1, b) : 0;
+    const size_t lbase = (pv && b > 0) ? sk.blk(i, b - 1) : 0;
+    const size_t lubase = (pv && b > 0 && i > 0) ? sk.blk(i - 1, b - 1) : 0;
+    const int* prog_up = a.prog + (pv && i > 0 ? (size_t)(i - 1) * nb + b : 0);
+    const int* prog_left = a.prog + (pv && b > 0 ? (size_t)i * nb + b - 1 : 0);
+    int* prog_me = a.prog + (pv ? (size_t)i * nb + b : 0);
+    int seen_up = 0, seen_left = 0;
     float Rp[3] = {0, 0, 0}, Bp[3] = {0, 0, 0}, Dp[3] = {0, 0, 0}, Fp[3] = {0, 0, 0};
-    const int steps = sk.T;
-    for (int t0 = 0; t0 < steps; t0 += kCH3) {
-      // wait for plane i-1 (this block) and, for lane 0's j-1 column, the block to the left
-      if (i > 0) {
-        const int need = min(t0 + kCH3, steps);
-        while (ld_coh(prog_up) < need) __builtin_amdgcn_s_sleep(2);
+
+    auto fetch = [&](Chunk3& X, int c) {
+      const int t0 = c * kC3;
+      if (w == 0 && i > 0) {
+        const int need = min(t0 + kC3, T);
+        while (seen_up < need) {
+          seen_up = __builtin_amdgcn_readfirstlane(ld_coh(prog_up));
+          if (seen_up < need) __builtin_amdgcn_s_sleep(1);
+        }
       }
       if (b > 0) {
-        const int need = min(t0 + kCH3 + 63, steps);
-        while (ld_coh(prog_left) < need) __builtin_amdgcn_s_sleep(2);
+        const int need = min(t0 + kC3 + 63, T);
+        while (seen_left < need) {
+          seen_left = __builtin_amdgcn_readfirstlane(ld_coh(prog_left));
+          if (seen_left < need) __builtin_amdgcn_s_sleep(1);
+        }
       }
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      // lane k holds column t0 + k of the j0-1 rows (own plane, plane above)
-      float bl[3] = {0, 0, 0}, bu[3] = {0, 0, 0};
-      if (b > 0 && lane < kCH3 && t0 + lane < r3) {
-        for (int p = 0; p < 3; p++) {
-          const size_t o = p * per + (size_t)(t0 + lane) * 64;
-          bl[p] = __int_as_float(ld_coh((const int*)(a.R + brow + o)));
-          if (i > 0) bu[p] = __int_as_float(ld_coh((const int*)(a.R + ubrow + o)));
-        }
-      }
-      float xs[kCH3][3], ups[kCH3][3];
-      uint32_t cs[kCH3];
-      uint32_t qs[kCH3][3];
 #pragma unroll
-      for (int s = 0; s < kCH3; s++) {
-        const int k = t0 + s - lane;
-        const bool act = jv && k >= 0 && k < r3;
-        const size_t ts = (size_t)(t0 + s) * Wd;
-        cs[s] = act ? a.code[row + ts] : 0;
+      for (int u = 0; u < kG3; u++) {
+        const size_t o = (size_t)(t0 / 4 + u) * 256 + lane * 4;
+        X.cd[u] = *(const uint32_t*)(a.code + base + o);
+#pragma unroll
         for (int p = 0; p < 3; p++) {
-          xs[s][p] = act ? a.lx[p * per + row + ts] : 0.f;
-          ups[s][p] = (act && i > 0) ? __int_as_float(ld_coh((const int*)(a.R + p * per + urow + ts))) : 0.f;
-          if (!COMPRESS) qs[s][p] = act ? a.qin[p * per + row + ts] : 0;
+          X.x[p][u] = *(const float4*)(a.lx + p * per + base + o);
+          if (!COMPRESS) X.q[p][u] = *(const uint2*)(a.qin + p * per + base + o);
+          if (w == 0) X.up[p][u] = i > 0 ? ld_coh4(a.R + p * per + ubase + o) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
       }
 #pragma unroll
-      for (int s = 0; s < kCH3; s++) {
-        const int k = t0 + s - lane;
-        const bool act = jv && k >= 0 && k < r3;
-        const size_t at = row + (size_t)(t0 + s) * Wd;
-        float res[3];
-        float dec[3];
-        bool okall = true;
-        uint16_t qc[3];
-        const uint32_t c = cs[s];
+      for (int p = 0; p < 3; p++) X.bl[p] = X.bu[p] = 0.f;
+      if (b > 0 && lane < kC3 && t0 + lane < r3) {
+        const int t = t0 + lane + 63;
+        const size_t o = (size_t)(t >> 2) * 256 + 63 * 4 + (t & 3);
+#pragma unroll
         for (int p = 0; p < 3; p++) {
-          const float l0b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bl[p]), s));
-          const float l0u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(bu[p]), s));
-          const float B = shr1(Rp[p], l0b);   // L(i, j-1, k)
-          const float F = shr1(Dp[p], l0u);   // L(i-1, j-1, k)
-          const float D = ups[s][p];          // L(i-1, j, k)
-          // pred = d0 + d3 + d5 + d6 - d1 - d2 - d4 (src/sz_compress_cp_preserve_3d.cpp:498)
-          const float pred = Fp[p] + D + B + Rp[p] - F - Dp[p] - Bp[p];
-          if (COMPRESS) {
-            bool ok;
-            dec[p] = l3d_step(pred, xs[s][p], (int)c, ok, qc[p]);
-            okall = okall && ok;
-          } else {
-            const uint32_t id = c;
-            if (id == 0 || id == 24) {
-              res[p] = xs[s][p];
+          X.bl[p] = ld_cohf(a.R + p * per + lbase + o);
+          if (i > 0) X.bu[p] = ld_cohf(a.R + p * per + lubase + o);
+        }
+      }
+    };
+
+    auto compute = [&](const Chunk3& X, int c) {
+      const int t0 = c * kC3;
+      float4 up[3][kG3];
+#pragma unroll
+      for (int u = 0; u < kG3; u++)
+#pragma unroll
+        for (int p = 0; p < 3; p++) up[p][u] = w == 0 ? X.up[p][u] : hand[w > 0 ? w - 1 : 0][c & 1][u][p][lane];
+#pragma unroll
+      for (int u = 0; u < kG3; u++) {
+        float rr[3][4];
+        uint32_t vv = 0, qq[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+#pragma unroll
+        for (int s4 = 0; s4 < 4; s4++) {
+          const int s = u * 4 + s4;
+          const int k = t0 + s - lane;
+          const uint32_t code = (X.cd[u] >> (8 * s4)) & 0xFFu;
+          float res[3], dec[3];
+          bool okall = true;
+          uint16_t qc[3];
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            const float l0b = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(X.bl[p]), s));
+            const float l0u = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(X.bu[p]), s));
+            const float Bv = shr1(Rp[p], l0b);   // L(i, j-1, k)
+            const float Fv = shr1(Dp[p], l0u);   // L(i-1, j-1, k)
+            const float Dv = comp4(up[p][u], s4);  // L(i-1, j, k)
+            // pred = d0 + d3 + d5 + d6 - d1 - d2 - d4 (src/sz_compress_cp_preserve_3d.cpp:498)
+            const float pred = Fp[p] + Dv + Bv + Rp[p] - Fv - Dp[p] - Bp[p];
+            const float xv = comp4(X.x[p][u], s4);
+            if (COMPRESS) {
+              bool ok;
+              dec[p] = l3d_step(pred, xv, (int)code, ok, qc[p]);
+              okall = okall & ok;
             } else {
-              res[p] = pred + ldexpf((float)(2 * ((int)qs[s][p] - 32768)), (int)id - 23);
+              const uint32_t qw = (s4 < 2) ? X.q[p][u].x : X.q[p][u].y;
+              const int qv = (int)((qw >> (16 * (s4 & 1))) & 0xFFFFu);
+              res[p] = (code == 0 || code == 24) ? xv
+                                                 : pred + ldexpf((float)(2 * (qv - 32768)), (int)code - 23);
             }
+            Bp[p] = Bv;
+            Fp[p] = Fv;
+            Dp[p] = Dv;
           }
-          Bp[p] = B;
-          Fp[p] = F;
-          Dp[p] = D;
-        }
-        if (COMPRESS) {
-          const bool pred_ok = c != 255 && c != 0 && okall;
-          for (int p = 0; p < 3; p++) res[p] = pred_ok ? dec[p] : xs[s][p];
-          if (act) {
-            a.vid[at] = (uint8_t)(c == 255 ? 0 : (pred_ok ? c : 24));
-            if (pred_ok)
-              for (int p = 0; p < 3; p++) a.qout[p * per + at] = qc[p];
+          if (COMPRESS) {
+            const bool pred_ok = (code != 255) & (code != 0) & okall;
+#pragma unroll
+            for (int p = 0; p < 3; p++) {
+              res[p] = pred_ok ? dec[p] : comp4(X.x[p][u], s4);
+              qq[p][s4 >> 1] |= (uint32_t)qc[p] << (16 * (s4 & 1));
+            }
+            vv |= (code == 255 ? 0u : (pred_ok ? code : 24u)) << (8 * s4);
+          }
+#pragma unroll
+          for (int p = 0; p < 3; p++) {
+            Rp[p] = k < 0 ? 0.f : res[p];
+            rr[p][s4] = Rp[p];
           }
         }
+        const size_t o = (size_t)(t0 / 4 + u) * 256 + lane * 4;
+#pragma unroll
         for (int p = 0; p < 3; p++) {
-          if (act) __hip_atomic_store((int*)(a.R + p * per + at), __float_as_int(res[p]),
-                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          Rp[p] = k < 0 ? 0.f : res[p];
+          const float4 v = make_float4(rr[p][0], rr[p][1], rr[p][2], rr[p][3]);
+          if (w < kP3 - 1) hand[w][c & 1][u][p][lane] = v;
+          if (!COMPRESS || w == kP3 - 1 || lane == 63) st_coh4(a.R + p * per + base + o, v);
+          if (COMPRESS) *(uint2*)(a.qout + p * per + base + o) = make_uint2(qq[p][0], qq[p][1]);
         }
+        if (COMPRESS) *(uint32_t*)(a.vid + base + o) = vv;
       }
+    };
+
+    auto iter = [&](Chunk3& cur, Chunk3& nxt, int it) {
+      const int c = it - w;
+      if (pv) {
+        if (c >= 1 && c <= nch) {  // chunk c-1's stores were issued last iteration
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          __builtin_amdgcn_s_waitcnt(0);
+          __atomic_signal_fence(__ATOMIC_SEQ_CST);
+          if (lane == 0) st_coh(prog_me, min(c * kC3, T));
+        }
+        if (c + 1 >= 0 && c + 1 < nch) fetch(nxt, c + 1);
+        if (c >= 0 && c < nch) compute(cur, c);
+      }
+      __syncthreads();
+    };
+
+    Chunk3 A, B;
+    for (int it = -1; it < nch + kP3 - 1; it += 2) {
+      iter(A, B, it);
+      iter(B, A, it + 1);
+    }
+    if (pv) {
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
       __builtin_amdgcn_s_waitcnt(0);
       __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      if (lane == 0) st_coh(prog_me, min(t0 + kCH3, steps));
+      if (lane == 0) st_coh(prog_me, T);
     }
   }
 }
@@ -542,10 +639,10 @@ void launch_wave3d(Ctx& ctx, Wave3d& w, bool compress) {
   hipStream_t st = ctx.stream();
   const size_t nz = (size_t)w.s.r1 * w.s.nb + 1;
   k_zero_i32_3d<<<(int)std::min<size_t>(div_up(nz, 256), 1024), 256, 0, st>>>(w.prog, nz);
-  const int items = w.s.r1 * w.s.nb;
-  const int grid = items < 4096 ? items : 4096;
-  if (compress) k_lorenzo3d<true><<<grid, 64, 0, st>>>(w);
-  else k_lorenzo3d<false><<<grid, 64, 0, st>>>(w);
+  const int items = (int)div_up(w.s.r1, kP3) * w.s.nb;
+  const int grid = items < 1024 ? items : 1024;
+  if (compress) k_wave3d<true><<<grid, 64 * kP3, 0, st>>>(w);
+  else k_wave3d<false><<<grid, 64 * kP3, 0, st>>>(w);
   CPSZ_HIP(hipGetLastError());
 }
 

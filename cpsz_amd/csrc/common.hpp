@@ -16,7 +16,7 @@ enum Status : int {
   kErrHip = -2,          // HIP runtime error
   kErrAlloc = -3,        // allocation failed / capacity exceeded
   kErrStream = -4,       // malformed or truncated compressed stream
-  kErrUnsupported = -5,  // valid input outside what this build supports
+  kErrUnsupported = -5,  // valid input outside what this library supports
   kErrNonFinite = -6,    // NaN/Inf input (the reference crashes on these)
   kErrInternal = -7,
 };

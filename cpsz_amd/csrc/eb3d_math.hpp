@@ -139,7 +139,7 @@ __host__ __device__ inline float l3d_step(float pred, float cur, int id, bool& o
   const float qf = floorf(fmaf(floorf(a), 0.5f, 0.5f));
   const float qs = copysignf(qf, diff);
   const float dec = pred + ldexpf(qs, id - 22);
-  ok = (a < 65535.0f) && (fabsf(dec - cur) < ldexpf(1.0f, id - 23));
+  ok = (a < 65535.0f) & (fabsf(dec - cur) < ldexpf(1.0f, id - 23));
   code = (uint16_t)((int)qs + 32768);
   return dec;
 }

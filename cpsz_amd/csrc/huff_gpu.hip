@@ -18,37 +18,47 @@ __global__ void k_scatter_codebook(const uint32_t* syms, const uint64_t* codes, 
   len[syms[i]] = lens[i];
 }
 
-__global__ void k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits, uint32_t S,
-                           uint32_t nseg, const uint64_t* __restrict__ prev_exit,
-                           const uint64_t* __restrict__ prev_start, uint64_t* __restrict__ start,
-                           uint64_t* __restrict__ exitp, uint32_t* __restrict__ cnt,
-                           int* __restrict__ changed, int* __restrict__ err, int first) {
+__global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                  uint32_t S, uint32_t nseg,
+                                                  const uint64_t* __restrict__ prev_exit,
+                                                  const uint64_t* __restrict__ prev_start,
+                                                  uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
+                                                  uint32_t* __restrict__ cnt, int* __restrict__ changed,
+                                                  int* __restrict__ err, int first) {
+  __shared__ uint32_t slut[1 << kLutBits];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg) return;
-  uint64_t s0;
-  if (first) {
-    s0 = (uint64_t)i * S;
-  } else {
-    s0 = i == 0 ? 0 : prev_exit[i - 1];
-    if (s0 == prev_start[i]) {  // unchanged start: keep the previous result
-      start[i] = s0;
-      exitp[i] = prev_exit[i];
-      return;
+  bool work = i < nseg;
+  uint64_t s0 = 0;
+  if (work) {
+    if (first) {
+      s0 = (uint64_t)i * S;
+    } else {
+      s0 = i == 0 ? 0 : prev_exit[i - 1];
+      if (s0 == prev_start[i]) {  // unchanged start: keep the previous result (and count)
+        start[i] = s0;
+        exitp[i] = prev_exit[i];
+        work = false;
+      }
     }
   }
+  if (!__syncthreads_or(work)) return;
+  for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
+  __syncthreads();
+  if (!work) return;
   uint64_t end = (uint64_t)(i + 1) * S;
   if (end > nbits) end = nbits;
-  uint64_t pos = s0;
+  BitReader br;
+  br.init(w, s0);
   uint32_t c = 0;
-  while (pos < end) {
-    if (dec_one(t, w, pos) == 0xFFFFFFFFu) {
+  while (br.pos < end) {
+    if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
       *err = 1;
       break;
     }
     c++;
   }
   start[i] = s0;
-  exitp[i] = pos;
+  exitp[i] = br.pos;
   cnt[i] = c;
   (void)changed;
 }

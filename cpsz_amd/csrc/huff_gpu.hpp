@@ -112,30 +112,54 @@ struct DecTree {
   uint32_t nodes;
 };
 
-__device__ __forceinline__ uint32_t peek32(const uint32_t* __restrict__ w, uint64_t pos) {
-  const uint64_t i = pos >> 5;
-  const int s = (int)(pos & 31);
-  const uint32_t hi = bswap32(w[i]);
-  if (!s) return hi;
-  const uint32_t lo = bswap32(w[i + 1]);
-  return (hi << s) | (lo >> (32 - s));
-}
+// MSB-first reader over big-endian 32-bit words; the word array needs at
+// least two zero words past the last bit.
+struct BitReader {
+  const uint32_t* w;
+  uint64_t buf;   // upcoming bits, MSB-aligned
+  int nb;         // valid bits in buf (> 32 after every refill)
+  uint64_t widx;  // next word to load
+  uint64_t pos;   // bit position of buf's MSB
+  __device__ __forceinline__ void init(const uint32_t* __restrict__ words, uint64_t p) {
+    w = words;
+    pos = p;
+    widx = p >> 5;
+    const int s = (int)(p & 31);
+    buf = (((uint64_t)bswap32(w[widx]) << 32) | bswap32(w[widx + 1])) << s;
+    nb = 64 - s;
+    widx += 2;
+    refill();
+  }
+  __device__ __forceinline__ void refill() {
+    if (nb <= 32) {
+      buf |= (uint64_t)bswap32(w[widx++]) << (32 - nb);
+      nb += 32;
+    }
+  }
+  __device__ __forceinline__ uint32_t peek() const { return (uint32_t)(buf >> 32); }
+  __device__ __forceinline__ void skip(int L) {
+    buf <<= L;
+    nb -= L;
+    pos += (uint64_t)L;
+    refill();
+  }
+};
 
-// Decodes one symbol at *pos; returns the symbol or 0xFFFFFFFF on a bad code.
-__device__ __forceinline__ uint32_t dec_one(const DecTree& t, const uint32_t* __restrict__ w,
-                                            uint64_t& pos) {
-  const uint32_t v = peek32(w, pos);
-  const uint32_t e = t.lut[v >> (32 - kLutBits)];
+// Decodes one symbol; returns it or 0xFFFFFFFF on a bad code. `lut` is the
+// 2^kLutBits-entry table (entry = sym << 8 | length, or node << 8 for codes
+// longer than kLutBits, which continue bit by bit down the tree).
+__device__ __forceinline__ uint32_t dec_one(const DecTree& t, const uint32_t* lut, BitReader& br) {
+  const uint32_t e = lut[br.peek() >> (32 - kLutBits)];
   const uint32_t L = e & 0xFF;
   if (L) {
-    pos += L;
+    br.skip((int)L);
     return e >> 8;
   }
   uint32_t node = e >> 8;
-  pos += kLutBits;
+  br.skip(kLutBits);
   for (int d = 0; d < 64; d++) {
-    const uint32_t b = peek32(w, pos) >> 31;
-    pos++;
+    const uint32_t b = br.peek() >> 31;
+    br.skip(1);
     node = b ? t.right[node] : t.left[node];
     if (node == 0 || node >= t.nodes) return 0xFFFFFFFFu;
     if (t.leaf[node]) return t.sym[node];
@@ -144,8 +168,8 @@ __device__ __forceinline__ uint32_t dec_one(const DecTree& t, const uint32_t* __
 }
 
 // Phase 1 / fix-up: decode from start to the first boundary >= seg_end.
-__global__ void k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits, uint32_t S,
-                           uint32_t nseg, const uint64_t* __restrict__ prev_exit,
+__global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                  uint32_t S, uint32_t nseg, const uint64_t* __restrict__ prev_exit,
                            const uint64_t* __restrict__ prev_start, uint64_t* __restrict__ start,
                            uint64_t* __restrict__ exitp, uint32_t* __restrict__ cnt,
                            int* __restrict__ changed, int* __restrict__ err, int first);
@@ -159,15 +183,16 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
                                                    const uint64_t* __restrict__ exitp,
                                                    const uint64_t* __restrict__ off, uint64_t n,
                                                    Sink sink) {
+  __shared__ uint32_t slut[1 << kLutBits];
+  for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
+  __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nseg) return;
-  uint64_t pos = start[i];
+  BitReader br;
+  br.init(w, start[i]);
   const uint64_t end = exitp[i];
   uint64_t o = off[i];
-  while (pos < end && o < n) {
-    const uint32_t s = dec_one(t, w, pos);
-    sink.put(o++, s);
-  }
+  while (br.pos < end && o < n) sink.put(o++, dec_one(t, slut, br));
 }
 
 template <class Sink>
@@ -214,7 +239,7 @@ void decode_stream(Ctx& ctx, const DeviceDecoder& dd, const uint32_t* d_words, u
   const uint64_t nbits = nbytes * 8;
   CPSZ_REQUIRE(nbits > 0, kErrStream, "empty huffman bitstream for a multi-symbol tree");
   uint32_t S = 256;
-  while (div_up(nbits, S) > (1u << 18)) S *= 2;
+  while (div_up(nbits, S) > (1u << 21)) S *= 2;
   const uint32_t nseg = (uint32_t)div_up(nbits, S);
   std::string t(tag);
   uint64_t* start[2] = {ctx.buf<uint64_t>((t + ".st0").c_str(), nseg), ctx.buf<uint64_t>((t + ".st1").c_str(), nseg)};
