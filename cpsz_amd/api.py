@@ -40,6 +40,20 @@ _SIGS = {
     "cpsz_set_timing": [C.c_int, C.c_int],
     "cpsz_last_stages": [C.c_int, C.c_char_p, _SZ, _P, C.c_int],
     "cpsz_device_count": [],
+    "cpsz_learn_init_params": [C.c_uint64, _P],
+    "cpsz_learn_analytic_eb_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, _P],
+    "cpsz_learn_features_2d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P],
+    "cpsz_learn_labels_2d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P],
+    "cpsz_gemm_bf16_dev": [C.c_int, _P, C.c_int, C.c_int, C.c_int, _P, C.c_int64, C.c_int64, _P,
+                           C.c_int64, C.c_int64, _P, C.c_int, C.c_int],
+    "cpsz_mlp_forward_dev": [C.c_int, _P, _P, _P, _SZ, _P],
+    "cpsz_mlp_grad_dev": [C.c_int, _P, _P, _P, _P, _SZ, _P, _P],
+    "cpsz_learn_train_2d_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, C.c_double, _SZ, C.c_int, C.c_int,
+                                C.c_float, C.c_uint64, _P, _P],
+    "cpsz_compress_2d_learned_f32_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, _P, C.POINTER(_P),
+                                         C.POINTER(_SZ), C.POINTER(C.c_int), C.POINTER(_SZ),
+                                         C.POINTER(_SZ)],
+    "cpsz_compress_2d_learned_f32": [_P, _P, _SZ, _SZ, C.c_double, _P, C.POINTER(_P), C.POINTER(_SZ)],
 }
 
 
@@ -68,6 +82,8 @@ def lib():
         L.cpsz_version.restype = C.c_char_p
         L.cpsz_free.argtypes = [_P]
         L.cpsz_free.restype = None
+        L.cpsz_learn_param_count.restype = _SZ
+        L.cpsz_learn_param_count.argtypes = []
         _lib = L
     return _lib
 
@@ -223,3 +239,85 @@ def last_stages(device: int = 0):
 
 def device_count() -> int:
     return lib().cpsz_device_count()
+
+
+# ---------------------------------------------------------------- learned eb --
+def learn_param_count() -> int:
+    return int(lib().cpsz_learn_param_count())
+
+
+def learn_init_params(seed: int = 0) -> np.ndarray:
+    p = np.zeros(learn_param_count(), np.float32)
+    _check(lib().cpsz_learn_init_params(seed, _ptr(p)))
+    return p
+
+
+def learn_analytic_eb_2d_dev(dU: int, dV: int, r1: int, r2: int, eb: float, dE: int, device: int = 0,
+                             stream: int = 0):
+    _check(lib().cpsz_learn_analytic_eb_2d_dev(device, stream or None, dU, dV, r1, r2, eb, dE))
+
+
+def learn_features_2d_dev(dU, dV, dE, r1, r2, eb, d_idx, m, dX, device=0, stream=0):
+    _check(lib().cpsz_learn_features_2d_dev(device, stream or None, dU, dV, dE, r1, r2, eb, d_idx or None,
+                                            m, dX))
+
+
+def learn_labels_2d_dev(dU, dV, dE, r1, r2, eb, d_idx, m, dL, device=0, stream=0):
+    _check(lib().cpsz_learn_labels_2d_dev(device, stream or None, dU, dV, dE, r1, r2, eb, d_idx or None,
+                                          m, dL))
+
+
+def gemm_bf16_dev(M, N, K, A, sam, sak, B, sbk, sbn, Cm, ldc, split=1, device=0, stream=0):
+    _check(lib().cpsz_gemm_bf16_dev(device, stream or None, M, N, K, A, sam, sak, B, sbk, sbn, Cm, ldc,
+                                    split))
+
+
+def mlp_forward_dev(params: np.ndarray, dX: int, m: int, dY: int, device: int = 0, stream: int = 0):
+    params = np.ascontiguousarray(params, np.float32)
+    _check(lib().cpsz_mlp_forward_dev(device, stream or None, _ptr(params), dX, m, dY))
+
+
+def mlp_grad_dev(params: np.ndarray, dX: int, dT: int, m: int, device: int = 0, stream: int = 0):
+    params = np.ascontiguousarray(params, np.float32)
+    g = np.zeros_like(params)
+    loss = np.zeros(1, np.float32)
+    _check(lib().cpsz_mlp_grad_dev(device, stream or None, _ptr(params), dX, dT, m, _ptr(g), _ptr(loss)))
+    return g, float(loss[0])
+
+
+def learn_train_2d_dev(fields, eb: float, params: np.ndarray, samples_per_field: int = 16384,
+                       epochs: int = 4, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
+                       device: int = 0, stream: int = 0):
+    """fields: list of (dU, dV, r1, r2) with device pointers. Returns (params, loss per epoch)."""
+    n = len(fields)
+    du = (C.c_void_p * n)(*[f[0] for f in fields])
+    dv = (C.c_void_p * n)(*[f[1] for f in fields])
+    r1 = (C.c_size_t * n)(*[f[2] for f in fields])
+    r2 = (C.c_size_t * n)(*[f[3] for f in fields])
+    p = np.ascontiguousarray(params, np.float32).copy()
+    losses = np.zeros(epochs, np.float32)
+    _check(lib().cpsz_learn_train_2d_dev(device, stream or None, n, du, dv, r1, r2, eb, samples_per_field,
+                                         epochs, batch, lr, seed, _ptr(p), _ptr(losses)))
+    return p, losses
+
+
+def compress_2d_learned_dev(dU: int, dV: int, r1: int, r2: int, eb: float, params: np.ndarray,
+                            device: int = 0, stream: int = 0):
+    """Returns (device pointer, size, verification rounds, boosted vertices, fallback vertices)."""
+    params = np.ascontiguousarray(params, np.float32)
+    p, s, it, bo, fb = _P(), _SZ(), C.c_int(), _SZ(), _SZ()
+    _check(lib().cpsz_compress_2d_learned_f32_dev(device, stream or None, dU, dV, r1, r2, eb, _ptr(params),
+                                                  C.byref(p), C.byref(s), C.byref(it), C.byref(bo),
+                                                  C.byref(fb)))
+    return p.value, s.value, it.value, bo.value, fb.value
+
+
+def sz_compress_cp_preserve_2d_learned(U, V, eb: float, params: np.ndarray) -> np.ndarray:
+    """Learned-bound 2D compression of host arrays (reference stream layout)."""
+    U = np.ascontiguousarray(U, np.float32)
+    V = np.ascontiguousarray(V, np.float32)
+    params = np.ascontiguousarray(params, np.float32)
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_compress_2d_learned_f32(_ptr(U), _ptr(V), U.shape[0], U.shape[1], eb, _ptr(params),
+                                              C.byref(p), C.byref(s)))
+    return _take(p, s.value)

@@ -14,6 +14,7 @@
 #include "ctx.hpp"
 #include "eb3d_math.hpp"
 #include "huff_host.hpp"
+#include "learn.hpp"
 
 namespace cpsz {
 namespace {
@@ -396,6 +397,140 @@ int cpsz_last_stages(int device, char* names, size_t names_cap, float* ms, int c
     }
   });
   return rc ? rc : count;
+}
+
+size_t cpsz_learn_param_count(void) { return kNumParams; }
+
+int cpsz_learn_init_params(uint64_t seed, float* params) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params, kErrArg, "null params");
+    mlp_init_params(seed, params);
+  });
+}
+
+int cpsz_learn_analytic_eb_2d_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                  size_t r2, double max_pwr_eb, double* dE) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    analytic_eb_2d(c, dU, dV, r1, r2, max_pwr_eb, dE);
+  });
+}
+
+int cpsz_learn_features_2d_dev(int device, void* stream, const float* dU, const float* dV,
+                               const double* dE, size_t r1, size_t r2, double max_pwr_eb,
+                               const uint32_t* d_idx, size_t m, uint16_t* dX) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    features_2d(c, dU, dV, dE, r1, r2, max_pwr_eb, d_idx, m, dX);
+  });
+}
+
+int cpsz_learn_labels_2d_dev(int device, void* stream, const float* dU, const float* dV,
+                             const double* dE, size_t r1, size_t r2, double max_pwr_eb,
+                             const uint32_t* d_idx, size_t m, uint8_t* dL) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    labels_2d(c, dU, dV, dE, r1, r2, max_pwr_eb, d_idx, m, dL);
+  });
+}
+
+int cpsz_gemm_bf16_dev(int device, void* stream, int M, int N, int K, const uint16_t* A, int64_t sam,
+                       int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int ldc,
+                       int split) {
+  return guarded([&] {
+    CPSZ_REQUIRE(A && B && C, kErrArg, "null operand");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    if (split > 1) CPSZ_HIP(hipMemsetAsync(C, 0, (size_t)M * ldc * sizeof(float), c.stream()));
+    gemm_bf16(c, M, N, K, A, (long)sam, (long)sak, B, (long)sbk, (long)sbn, C, ldc, split);
+  });
+}
+
+int cpsz_mlp_forward_dev(int device, void* stream, const float* params, const uint16_t* dX, size_t m,
+                         float* dY) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params && dX && dY, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    float* P = c.buf<float>("api.p", kNumParams);
+    CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, c.stream()));
+    mlp_forward(c, P, dX, m, dY);
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_mlp_grad_dev(int device, void* stream, const float* params, const uint16_t* dX,
+                      const float* dT, size_t m, float* grad, float* loss) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params && dX && dT && grad && loss, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    float* P = c.buf<float>("api.p", kNumParams);
+    float* G = c.buf<float>("api.g", kNumParams + 1);
+    CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, c.stream()));
+    mlp_grad(c, P, dX, dT, m, G, G + kNumParams);
+    CPSZ_HIP(hipMemcpyAsync(grad, G, kNumParams * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(loss, G + kNumParams, 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_learn_train_2d_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                            const float* const* dV, const size_t* r1, const size_t* r2,
+                            double max_pwr_eb, size_t samples_per_field, int epochs, int batch,
+                            float lr, uint64_t seed, float* params, float* loss_per_epoch) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && dU && dV && r1 && r2 && params, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<TrainField> f(nfields);
+    for (size_t k = 0; k < nfields; k++) f[k] = TrainField{dU[k], dV[k], r1[k], r2[k]};
+    train_2d(c, f, max_pwr_eb, samples_per_field, epochs, batch, lr, seed, params, loss_per_epoch);
+  });
+}
+
+int cpsz_compress_2d_learned_f32_dev(int device, void* stream, const float* dU, const float* dV,
+                                     size_t r1, size_t r2, double max_pwr_eb, const float* params,
+                                     const unsigned char** d_out, size_t* out_size, int* iterations,
+                                     size_t* boosted, size_t* fallback) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_out && out_size, kErrArg, "null output argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    DeviceStream s;
+    LearnedStats st;
+    compress_2d_learned(c, dU, dV, r1, r2, max_pwr_eb, params, s, st);
+    *d_out = s.data;
+    *out_size = s.size;
+    if (iterations) *iterations = st.iterations;
+    if (boosted) *boosted = st.boosted_vertices;
+    if (fallback) *fallback = st.fallback_vertices;
+  });
+}
+
+int cpsz_compress_2d_learned_f32(const float* U, const float* V, size_t r1, size_t r2,
+                                 double max_pwr_eb, const float* params, unsigned char** out,
+                                 size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(U && V && params && out && out_size && r1 && r2, kErrArg, "null argument or empty field");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    float* d = c.buf<float>("api.lin", 2 * n);
+    CPSZ_HIP(hipMemcpyAsync(d, U, n * 4, hipMemcpyHostToDevice, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(d + n, V, n * 4, hipMemcpyHostToDevice, c.stream()));
+    DeviceStream s;
+    LearnedStats st;
+    compress_2d_learned(c, d, d + n, r1, r2, max_pwr_eb, params, s, st);
+    unsigned char* h = (unsigned char*)std::malloc(s.size ? s.size : 1);
+    CPSZ_REQUIRE(h, kErrAlloc, "host allocation failed");
+    CPSZ_HIP(hipMemcpyAsync(h, s.data, s.size, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+    *out = h;
+    *out_size = s.size;
+  });
 }
 
 }  // extern "C"

@@ -158,7 +158,8 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
                                               int r1, int r2, double max_pwr_eb,
                                               const double* __restrict__ q2d, int nq,
                                               uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
-                                              uint32_t* __restrict__ hist, int* __restrict__ flags) {
+                                              uint32_t* __restrict__ hist, int* __restrict__ flags,
+                                              EbOverride ov) {
   __shared__ float su[kTH + 2][kTW + 2];
   __shared__ float sv[kTH + 2][kTW + 2];
   __shared__ double tb[kTH + 1][kTW + 1][2];
@@ -206,6 +207,10 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
     eb = dminf(eb, tb[vr + 1][vc][1]);
     eb = dminf(eb, tb[vr + 1][vc + 1][0]);
     eb = dminf(eb, tb[vr + 1][vc + 1][1]);
+    const size_t vix = (size_t)i * r2 + j;
+    if (ov.eb_out) ov.eb_out[vix] = eb;
+    if (ov.gain && !(ov.fallback && ov.fallback[vix]) && ov.gain[vix])
+      eb = dminf(max_pwr_eb, ldexp(eb, (int)ov.gain[vix]));
     const float u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     if (!isfinite(u) || !isfinite(v) || !(eb == eb)) bad = 1;
     // the reference's eb/DBL_EPSILON overflows from 2^972 on (undefined id)
@@ -495,8 +500,27 @@ void launch_strips(Ctx& ctx, Strip2d& sa, bool compress, bool wide) {
 
 }  // namespace
 
+void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
+                  double* dE) {
+  CPSZ_REQUIRE(dU && dV && dE && r1 >= 1 && r2 >= 1, kErrArg, "eb_2d: bad arguments");
+  hipStream_t st = ctx.stream();
+  const size_t n = r1 * r2;
+  uint16_t* cu = ctx.buf<uint16_t>("c2.cu", n);
+  uint16_t* cv = ctx.buf<uint16_t>("c2.cv", n);
+  uint32_t* hist = ctx.buf<uint32_t>("c2.hist", kMaxEbCode + 2 * 65536);
+  int* flags = ctx.buf<int>("c2.flags", 4);
+  CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
+  CPSZ_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
+  EbOverride ov;
+  ov.eb_out = dE;
+  dim3 grid((unsigned)div_up(r2, kTW), (unsigned)div_up(r1, kTH));
+  k_eb2d<<<grid, 256, 0, st>>>(dU, dV, (int)r1, (int)r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(), cu, cv,
+                               hist, flags, ov);
+  CPSZ_HIP(hipGetLastError());
+}
+
 void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
-                        double max_pwr_eb, DeviceStream& out) {
+                        double max_pwr_eb, DeviceStream& out, const EbOverride& ov) {
   CPSZ_REQUIRE(dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "compress_2d: bad arguments");
   CPSZ_REQUIRE(r1 < (1u << 30) && r2 < (1u << 30) && r1 * r2 < (1ull << 40), kErrUnsupported,
                "compress_2d: field too large");
@@ -514,7 +538,7 @@ void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, s
   {
     dim3 grid((unsigned)div_up(r2, kTW), (unsigned)div_up(r1, kTH));
     k_eb2d<<<grid, 256, 0, st>>>(dU, dV, (int)r1, (int)r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
-                                 cu, cv, hist, flags);
+                                 cu, cv, hist, flags, ov);
     CPSZ_HIP(hipGetLastError());
   }
   ctx.mark("eb");

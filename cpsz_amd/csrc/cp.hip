@@ -21,44 +21,13 @@
 #include <vector>
 
 #include "cp.hpp"
+#include "cp_math.hpp"
 
 namespace cpsz {
 
 namespace {
 
-struct Exp {
-  double t[56];
-  int n = 0;
-  __device__ void add(double x) {
-    double q = x;
-    int m = 0;
-    for (int i = 0; i < n; i++) {
-      const double s = q + t[i];
-      const double bv = s - q;
-      const double av = s - bv;
-      const double e = (q - av) + (t[i] - bv);
-      if (e != 0) t[m++] = e;
-      q = s;
-    }
-    if (q != 0) t[m++] = q;
-    n = m;
-  }
-  __device__ int sign() const {
-    for (int i = n - 1; i >= 0; i--)
-      if (t[i] != 0) return t[i] > 0 ? 1 : -1;
-    return 0;
-  }
-};
-
-__device__ __forceinline__ int sgn_d(double x) { return (x > 0) - (x < 0); }
-
-// sign(a*b - c*d), products of floats exact in double
-__device__ __forceinline__ int sign_cross(float a, float b, float c, float d) {
-  const double x = (double)a * b, y = (double)c * d;
-  return (x > y) - (x < y);
-}
-
-__device__ __forceinline__ void add_triple(Exp& e, double s, float a, float b, float c) {
+__device__ __forceinline__ void add_triple(Expansion& e, double s, float a, float b, float c) {
   const double ab = (double)a * b;
   const double p = ab * (double)c;
   const double q = fma(ab, (double)c, -p);
@@ -83,32 +52,9 @@ __device__ int sign_triples(const Triple* t, int m) {
   const double bound = (double)(m + 4) * 2.3e-16 * mag;
   if (sum > bound) return 1;
   if (sum < -bound) return -1;
-  Exp e;
+  Expansion e;
   for (int i = 0; i < m; i++) add_triple(e, t[i].s, t[i].a, t[i].b, t[i].c);
   return e.sign();
-}
-
-__device__ int type_2d(const double v[3][2], int lower) {
-  // X1 = {{0,0},{0,1},{1,1}} (t=0), X2 = {{0,0},{1,0},{1,1}} (t=1); x = column, y = row
-  const double X[2][3][2] = {{{0, 0}, {0, 1}, {1, 1}}, {{0, 0}, {1, 0}, {1, 1}}};
-  const double(*x)[2] = X[lower];
-  const double xa = x[0][0] - x[2][0], xb = x[1][0] - x[2][0];
-  const double ya = x[0][1] - x[2][1], yb = x[1][1] - x[2][1];
-  const double det = xa * yb - xb * ya;
-  const double ix[2][2] = {{yb / det, -xb / det}, {-ya / det, xa / det}};
-  const double F[2][2] = {{v[0][0] - v[2][0], v[1][0] - v[2][0]}, {v[0][1] - v[2][1], v[1][1] - v[2][1]}};
-  double J[2][2];
-  for (int r = 0; r < 2; r++)
-    for (int c = 0; c < 2; c++) J[r][c] = F[r][0] * ix[0][c] + F[r][1] * ix[1][c];
-  const double tr = J[0][0] + J[1][1];
-  const double dt = J[0][0] * J[1][1] - J[1][0] * J[0][1];
-  const double delta = tr * tr - 4 * dt;
-  if (delta >= 0) {
-    if (dt < 0) return 3;               // roots of opposite sign
-    if (dt > 0) return tr < 0 ? 1 : 2;  // same sign as their sum
-    return tr > 0 ? 2 : 0;              // a zero root: the other is tr
-  }
-  return tr < 0 ? 4 : (tr > 0 ? 5 : 6);
 }
 
 __device__ int type_3d(const double v[4][3], const int X[4][3]) {
@@ -187,28 +133,12 @@ __global__ void k_cp2d(const float* __restrict__ U, const float* __restrict__ V,
   const long long sid = 2ll * ((long long)i * (r2 - 1) + j);
   for (int t = 0; t < 2; t++) {
     float f[3][2];
-    bool zero = false;
     for (int p = 0; p < 3; p++) {
       f[p][0] = U[idx[t][p]];
       f[p][1] = V[idx[t][p]];
-      zero = zero || (f[p][0] == 0 && f[p][1] == 0);
     }
-    if (zero) continue;
-    const int s0 = sign_cross(f[1][0], f[2][1], f[2][0], f[1][1]);
-    const int s1 = sign_cross(f[2][0], f[0][1], f[0][0], f[2][1]);
-    const int s2 = sign_cross(f[0][0], f[1][1], f[1][0], f[0][1]);
-    if ((s0 > 0 && s1 < 0) || (s0 < 0 && s1 > 0) || (s0 > 0 && s2 < 0) || (s0 < 0 && s2 > 0) ||
-        (s1 > 0 && s2 < 0) || (s1 < 0 && s2 > 0))
-      continue;  // mixed signs: no zero inside
-    Exp e;
-    const double pr[6] = {(double)f[1][0] * f[2][1], -(double)f[2][0] * f[1][1], (double)f[2][0] * f[0][1],
-                          -(double)f[0][0] * f[2][1], (double)f[0][0] * f[1][1], -(double)f[1][0] * f[0][1]};
-    for (int k = 0; k < 6; k++) e.add(pr[k]);
-    const int s = e.sign();
-    if (s == 0 || s0 * s < 0 || s1 * s < 0 || s2 * s < 0) continue;
-    double v[3][2];
-    for (int p = 0; p < 3; p++) v[p][0] = f[p][0], v[p][1] = f[p][1];
-    out.push(sid + t, type_2d(v, t));
+    const int type = tri_cp(f, t);
+    if (type >= 0) out.push(sid + t, type);
   }
 }
 

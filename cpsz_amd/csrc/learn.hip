@@ -1,0 +1,580 @@
+// Learned error-bound proposer for the 2D codec: features, bisection labels,
+// BF16 MFMA MLP (forward, backward, Adam) and the verified learned compressor.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cp.hpp"
+#include "cp_math.hpp"
+#include "learn.hpp"
+
+namespace cpsz {
+
+namespace {
+
+__host__ __device__ inline uint16_t f2bf(float x) {
+  uint32_t b;
+  __builtin_memcpy(&b, &x, 4);
+  b += 0x7FFFu + ((b >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(b >> 16);
+}
+__device__ inline float bf2f(uint16_t h) {
+  const uint32_t b = (uint32_t)h << 16;
+  float x;
+  __builtin_memcpy(&x, &b, 4);
+  return x;
+}
+
+// ------------------------------------------------------------- features --
+// 5x5 patch of u then v (row-major, zero outside the field) divided by the
+// patch's largest magnitude, then log2(analytic / max_pwr_eb) / 16 (-4 for a
+// zero bound), then zeros up to kFeatDim.
+__global__ void k_feat2d(const float* __restrict__ U, const float* __restrict__ V,
+                         const double* __restrict__ E, int r1, int r2, double max_pwr_eb,
+                         const uint32_t* __restrict__ idx, size_t m, uint16_t* __restrict__ X) {
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t v = idx ? idx[q] : q;
+  const int i = (int)(v / (size_t)r2), j = (int)(v - (size_t)i * r2);
+  float pu[25], pv[25];
+  float s = 0.f;
+  for (int a = 0; a < 5; a++)
+    for (int b = 0; b < 5; b++) {
+      const int ii = i + a - 2, jj = j + b - 2;
+      const bool in = ii >= 0 && ii < r1 && jj >= 0 && jj < r2;
+      const size_t o = in ? (size_t)ii * r2 + jj : 0;
+      pu[a * 5 + b] = in ? U[o] : 0.f;
+      pv[a * 5 + b] = in ? V[o] : 0.f;
+      s = fmaxf(s, fmaxf(fabsf(pu[a * 5 + b]), fabsf(pv[a * 5 + b])));
+    }
+  if (!(s > 0.f)) s = 1.f;
+  uint16_t* x = X + q * kFeatDim;
+  for (int k = 0; k < 25; k++) {
+    x[k] = f2bf(pu[k] / s);
+    x[25 + k] = f2bf(pv[k] / s);
+  }
+  const double e = E[v];
+  const float le = e > 0 ? fmaxf((float)log2(e / max_pwr_eb), -64.f) / 16.f : -4.f;
+  x[50] = f2bf(le);
+  for (int k = kFeatUsed; k < kFeatDim; k++) x[k] = 0;
+}
+
+// --------------------------------------------------------------- labels --
+// Incident triangles of vertex (i, j): cell (di, dj) offset, kind t (0 upper
+// (i,j),(i+1,j),(i+1,j+1); 1 lower (i,j),(i,j+1),(i+1,j+1)) and the vertex's
+// position in the triangle.
+__constant__ int kInc[6][4] = {{-1, -1, 0, 2}, {-1, -1, 1, 2}, {-1, 0, 0, 1},
+                               {0, -1, 1, 1},  {0, 0, 0, 0},   {0, 0, 1, 0}};
+
+__device__ inline void tri_vertex(int t, int p, int& di, int& dj) {
+  // upper: (0,0),(1,0),(1,1); lower: (0,0),(0,1),(1,1)
+  di = p == 0 ? 0 : (t == 0 ? 1 : (p == 1 ? 0 : 1));
+  dj = p == 0 ? 0 : (t == 0 ? (p == 1 ? 0 : 1) : 1);
+}
+
+__global__ void k_labels2d(const float* __restrict__ U, const float* __restrict__ V,
+                           const double* __restrict__ E, int r1, int r2, double max_pwr_eb,
+                           const uint32_t* __restrict__ idx, size_t m, uint8_t* __restrict__ L) {
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t v = idx ? idx[q] : q;
+  const int i = (int)(v / (size_t)r2), j = (int)(v - (size_t)i * r2);
+  const double e = E[v];
+  if (!(e > 0)) {
+    L[q] = 0;
+    return;
+  }
+  float f[6][3][2];
+  int base[6], kind[6], pos[6];
+  bool valid[6];
+  for (int k = 0; k < 6; k++) {
+    const int ci = i + kInc[k][0], cj = j + kInc[k][1];
+    kind[k] = kInc[k][2];
+    pos[k] = kInc[k][3];
+    valid[k] = ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1;
+    base[k] = -1;
+    for (int p = 0; p < 3; p++) {
+      int di, dj;
+      tri_vertex(kind[k], p, di, dj);
+      const size_t o = valid[k] ? (size_t)(ci + di) * r2 + (cj + dj) : 0;
+      f[k][p][0] = valid[k] ? U[o] : 0.f;
+      f[k][p][1] = valid[k] ? V[o] : 0.f;
+    }
+    if (valid[k]) base[k] = tri_cp(f[k], kind[k]);
+  }
+  const float u = U[v], w = V[v];
+  int gmax = 0;
+  while (gmax < kMaxGain && ldexp(e, gmax) < max_pwr_eb) gmax++;
+  auto ok = [&](int g) {
+    const double eb = fmin(max_pwr_eb, ldexp(e, g));
+    const double du = fabs((double)u) * eb, dv = fabs((double)w) * eb;
+    for (int c = 0; c < 4; c++) {
+      const float cu = (float)((double)u + ((c & 1) ? du : -du));
+      const float cv = (float)((double)w + ((c & 2) ? dv : -dv));
+      for (int k = 0; k < 6; k++) {
+        if (!valid[k]) continue;
+        float g3[3][2];
+        for (int p = 0; p < 3; p++) {
+          g3[p][0] = p == pos[k] ? cu : f[k][p][0];
+          g3[p][1] = p == pos[k] ? cv : f[k][p][1];
+        }
+        if (tri_cp(g3, kind[k]) != base[k]) return false;
+      }
+    }
+    return true;
+  };
+  int lo = 0, hi = gmax;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (ok(mid)) lo = mid;
+    else hi = mid - 1;
+  }
+  L[q] = (uint8_t)lo;
+}
+
+// ----------------------------------------------------------------- GEMM --
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// One wave per 32x32 tile of C, four waves per block (64x64). Operand layout
+// of v_mfma_f32_32x32x8_bf16: lane t supplies A(t%32, 4*(t/32)+q) and
+// B(4*(t/32)+q, t%32), q = 0..3; it receives D(8*(r/4) + 4*(t/32) + r%4, t%32)
+// in accumulator r.
+__global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const uint16_t* __restrict__ A,
+                                                   long sam, long sak, const uint16_t* __restrict__ B,
+                                                   long sbk, long sbn, float* __restrict__ C, int ldc,
+                                                   int kchunk, int atomic) {
+  const int w = threadIdx.x >> 6, t = threadIdx.x & 63;
+  const int m0 = blockIdx.y * 64 + (w >> 1) * 32, n0 = blockIdx.x * 64 + (w & 1) * 32;
+  if (m0 >= M || n0 >= N) return;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
+  const int row = m0 + (t & 31), col = n0 + (t & 31), kq = (t >> 5) * 4;
+  const bool rv = row < M, cvld = col < N;
+  floatx16 acc;
+  for (int r = 0; r < 16; r++) acc[r] = 0.f;
+  for (int k0 = kb; k0 < ke; k0 += 8) {
+    bf16x4 a, b;
+    for (int q = 0; q < 4; q++) {
+      const int k = k0 + kq + q;
+      const bool kv = k < ke;
+      a[q] = (short)((rv && kv) ? A[(long)row * sam + (long)k * sak] : 0);
+      b[q] = (short)((cvld && kv) ? B[(long)k * sbk + (long)col * sbn] : 0);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+  }
+  for (int r = 0; r < 16; r++) {
+    const int i = m0 + 8 * (r >> 2) + 4 * (t >> 5) + (r & 3);
+    if (i < M && cvld) {
+      float* c = C + (long)i * ldc + col;
+      if (atomic) atomicAdd(c, acc[r]);
+      else *c = acc[r];
+    }
+  }
+}
+
+// ----------------------------------------------------------- elementwise --
+__global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+
+// Z[r][c] += b[c]; H = relu(Z) as BF16 (and FP32 when Hf is set)
+__global__ void k_bias_relu(float* __restrict__ Z, const float* __restrict__ b, size_t m,
+                            uint16_t* __restrict__ Hb, float* __restrict__ Hf) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= m * kHidden) return;
+  const float z = Z[i] + b[i % kHidden];
+  Z[i] = z;
+  const float h = z > 0.f ? z : 0.f;
+  Hb[i] = f2bf(h);
+  if (Hf) Hf[i] = h;
+}
+
+// y[r] = b3 + sum_c H2[r][c] w3[c] (sequential order)
+__global__ void k_out(const float* __restrict__ H2, const float* __restrict__ w3, const float* __restrict__ b3,
+                      size_t m, float* __restrict__ y) {
+  const size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  float s = 0.f;
+  const float* h = H2 + r * kHidden;
+  for (int c = 0; c < kHidden; c++) s = fmaf(h[c], w3[c], s);
+  y[r] = s + b3[0];
+}
+
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, size_t m,
+                       float* __restrict__ dy, float* __restrict__ loss) {
+  const size_t r = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (r < m) {
+    const float d = y[r] - t[r];
+    dy[r] = 2.f * d / (float)m;
+    l = d * d / (float)m;
+  }
+  for (int o = 32; o > 0; o >>= 1) l += __shfl_xor(l, o);
+  if ((threadIdx.x & 63) == 0 && l != 0.f) atomicAdd(loss, l);
+}
+
+constexpr int kRowsPerBlock = 64;
+
+// Output layer backward: gW3 += H2^T dy, gb3 += sum dy, dZ2 = dy w3 * (Z2 > 0)
+// (BF16), gb2 += column sums of dZ2. Block: 128 threads (one per column) x 64 rows.
+__global__ void k_back_out(const float* __restrict__ H2, const float* __restrict__ Z2,
+                           const float* __restrict__ dy, const float* __restrict__ w3, size_t m,
+                           float* __restrict__ gW3, float* __restrict__ gb3, float* __restrict__ gb2,
+                           uint16_t* __restrict__ dZ2) {
+  const int c = threadIdx.x;
+  const size_t r0 = blockIdx.x * (size_t)kRowsPerBlock;
+  float sw = 0.f, sb = 0.f, sy = 0.f;
+  for (int k = 0; k < kRowsPerBlock; k++) {
+    const size_t r = r0 + k;
+    if (r >= m) break;
+    const size_t o = r * kHidden + c;
+    const float g = dy[r];
+    sw += H2[o] * g;
+    const float dz = Z2[o] > 0.f ? g * w3[c] : 0.f;
+    dZ2[o] = f2bf(dz);
+    sb += dz;
+    sy += g;
+  }
+  atomicAdd(&gW3[c], sw);
+  atomicAdd(&gb2[c], sb);
+  if (c == 0) atomicAdd(gb3, sy);
+}
+
+// dZ1 = dH1 * (Z1 > 0) (BF16), gb1 += column sums
+__global__ void k_back_relu(const float* __restrict__ dH, const float* __restrict__ Z, size_t m,
+                            float* __restrict__ gb, uint16_t* __restrict__ dZ) {
+  const int c = threadIdx.x;
+  const size_t r0 = blockIdx.x * (size_t)kRowsPerBlock;
+  float sb = 0.f;
+  for (int k = 0; k < kRowsPerBlock; k++) {
+    const size_t r = r0 + k;
+    if (r >= m) break;
+    const size_t o = r * kHidden + c;
+    const float dz = Z[o] > 0.f ? dH[o] : 0.f;
+    dZ[o] = f2bf(dz);
+    sb += dz;
+  }
+  atomicAdd(&gb[c], sb);
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, size_t n, float lr, float b1, float b2, float eps, float c1,
+                       float c2) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float a = b1 * m1[i] + (1.f - b1) * gi;
+  const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+  m1[i] = a;
+  m2[i] = b;
+  p[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+__global__ void k_gather_rows(const uint16_t* __restrict__ X, const float* __restrict__ T,
+                              const uint32_t* __restrict__ perm, size_t m, uint16_t* __restrict__ Xb,
+                              float* __restrict__ Tb) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= m * kFeatDim) return;
+  const size_t r = i / kFeatDim, c = i - r * kFeatDim;
+  const size_t s = perm[r];
+  Xb[i] = X[s * kFeatDim + c];
+  if (c == 0) Tb[r] = T[s];
+}
+
+__global__ void k_u8_to_f32(const uint8_t* __restrict__ a, size_t n, float* __restrict__ b) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = (float)a[i];
+}
+
+// proposal: gain = clamp(floor(y), 0, kMaxGain)
+__global__ void k_gain(const float* __restrict__ y, size_t n, uint8_t* __restrict__ g) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float f = floorf(y[i]);
+  g[i] = (uint8_t)(f > 0.f ? (f < (float)kMaxGain ? f : (float)kMaxGain) : 0.f);
+}
+
+int blocks_for(size_t n, int bs = 256) { return (int)div_up(n, (size_t)bs); }
+
+uint64_t splitmix(uint64_t& s) {
+  uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Network activations for up to `cap` rows.
+struct Acts {
+  float *Z1, *Z2, *H2, *y;
+  uint16_t *H1b, *W1b, *W2b;
+};
+
+Acts acts(Ctx& ctx, size_t cap) {
+  Acts a;
+  a.Z1 = ctx.buf<float>("nn.z1", cap * kHidden);
+  a.Z2 = ctx.buf<float>("nn.z2", cap * kHidden);
+  a.H2 = ctx.buf<float>("nn.h2", cap * kHidden);
+  a.y = ctx.buf<float>("nn.y", cap);
+  a.H1b = ctx.buf<uint16_t>("nn.h1b", cap * kHidden);
+  a.W1b = ctx.buf<uint16_t>("nn.w1b", (size_t)kFeatDim * kHidden);
+  a.W2b = ctx.buf<uint16_t>("nn.w2b", (size_t)kHidden * kHidden);
+  return a;
+}
+
+void weights_bf16(Ctx& ctx, const float* P, Acts& a) {
+  hipStream_t st = ctx.stream();
+  k_to_bf16<<<blocks_for((size_t)kFeatDim * kHidden), 256, 0, st>>>(P + kOffW1, (size_t)kFeatDim * kHidden, a.W1b);
+  k_to_bf16<<<blocks_for((size_t)kHidden * kHidden), 256, 0, st>>>(P + kOffW2, (size_t)kHidden * kHidden, a.W2b);
+}
+
+// forward of rows [0, m) of X (m <= cap)
+void forward_rows(Ctx& ctx, const float* P, Acts& a, const uint16_t* X, size_t m) {
+  hipStream_t st = ctx.stream();
+  gemm_bf16(ctx, (int)m, kHidden, kFeatDim, X, kFeatDim, 1, a.W1b, kHidden, 1, a.Z1, kHidden, 1);
+  k_bias_relu<<<blocks_for(m * kHidden), 256, 0, st>>>(a.Z1, P + kOffB1, m, a.H1b, nullptr);
+  gemm_bf16(ctx, (int)m, kHidden, kHidden, a.H1b, kHidden, 1, a.W2b, kHidden, 1, a.Z2, kHidden, 1);
+  uint16_t* h2b = ctx.buf<uint16_t>("nn.h2b", m * kHidden);
+  k_bias_relu<<<blocks_for(m * kHidden), 256, 0, st>>>(a.Z2, P + kOffB2, m, h2b, a.H2);
+  k_out<<<blocks_for(m), 256, 0, st>>>(a.H2, P + kOffW3, P + kOffB3, m, a.y);
+  CPSZ_HIP(hipGetLastError());
+}
+
+}  // namespace
+
+void gemm_bf16(Ctx& ctx, int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B,
+               long sbk, long sbn, float* C, int ldc, int split) {
+  CPSZ_REQUIRE(M >= 0 && N >= 0 && K >= 0 && split >= 1, kErrArg, "gemm_bf16: bad sizes");
+  if (M == 0 || N == 0) return;
+  int kchunk = (int)div_up((size_t)K, (size_t)split);
+  kchunk = (int)div_up((size_t)kchunk, 8) * 8;
+  if (kchunk == 0) kchunk = 8;
+  const int nz = (int)div_up((size_t)(K ? K : 1), (size_t)kchunk);
+  dim3 grid((unsigned)div_up((size_t)N, 64), (unsigned)div_up((size_t)M, 64), (unsigned)nz);
+  k_gemm_bf16<<<grid, 256, 0, ctx.stream()>>>(M, N, K, A, sam, sak, B, sbk, sbn, C, ldc, kchunk, nz > 1);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void mlp_init_params(uint64_t seed, float* p) {
+  uint64_t s = seed ^ 0x6c6561726e656462ull;
+  auto uni = [&]() { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); };
+  auto fill = [&](size_t off, size_t n, double scale) {
+    for (size_t i = 0; i < n; i++) p[off + i] = (float)((2 * uni() - 1) * scale);
+  };
+  std::memset(p, 0, kNumParams * sizeof(float));
+  fill(kOffW1, (size_t)kFeatDim * kHidden, std::sqrt(6.0 / kFeatUsed));
+  fill(kOffW2, (size_t)kHidden * kHidden, std::sqrt(6.0 / kHidden));
+  fill(kOffW3, kHidden, std::sqrt(3.0 / kHidden));
+  for (size_t r = kFeatUsed; r < (size_t)kFeatDim; r++)
+    for (int c = 0; c < kHidden; c++) p[kOffW1 + r * kHidden + c] = 0.f;  // padding rows
+}
+
+void analytic_eb_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
+                    double* dE) {
+  eb_2d_device(ctx, dU, dV, r1, r2, max_pwr_eb, dE);
+}
+
+void features_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, size_t r1, size_t r2,
+                 double max_pwr_eb, const uint32_t* d_idx, size_t m, uint16_t* dX) {
+  CPSZ_REQUIRE(dU && dV && dE && dX && r1 * r2 < (1ull << 32), kErrArg, "features_2d: bad arguments");
+  if (!m) return;
+  k_feat2d<<<blocks_for(m), 256, 0, ctx.stream()>>>(dU, dV, dE, (int)r1, (int)r2, max_pwr_eb, d_idx, m, dX);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void labels_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, size_t r1, size_t r2,
+               double max_pwr_eb, const uint32_t* d_idx, size_t m, uint8_t* dL) {
+  CPSZ_REQUIRE(dU && dV && dE && dL && r1 * r2 < (1ull << 32), kErrArg, "labels_2d: bad arguments");
+  if (!m) return;
+  k_labels2d<<<blocks_for(m, 128), 128, 0, ctx.stream()>>>(dU, dV, dE, (int)r1, (int)r2, max_pwr_eb, d_idx,
+                                                           m, dL);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void mlp_forward(Ctx& ctx, const float* P, const uint16_t* dX, size_t m, float* dY) {
+  const size_t cap = std::min<size_t>(m, 1u << 18);
+  Acts a = acts(ctx, cap);
+  weights_bf16(ctx, P, a);
+  for (size_t r0 = 0; r0 < m; r0 += cap) {
+    const size_t mm = std::min(cap, m - r0);
+    forward_rows(ctx, P, a, dX + r0 * kFeatDim, mm);
+    CPSZ_HIP(hipMemcpyAsync(dY + r0, a.y, mm * sizeof(float), hipMemcpyDeviceToDevice, ctx.stream()));
+  }
+}
+
+void mlp_grad(Ctx& ctx, const float* P, const uint16_t* X, const float* T, size_t m, float* G, float* loss) {
+  CPSZ_REQUIRE(m > 0 && m <= (1u << 20), kErrArg, "mlp_grad: batch size out of range");
+  hipStream_t st = ctx.stream();
+  Acts a = acts(ctx, m);
+  weights_bf16(ctx, P, a);
+  forward_rows(ctx, P, a, X, m);
+  CPSZ_HIP(hipMemsetAsync(G, 0, kNumParams * sizeof(float), st));
+  CPSZ_HIP(hipMemsetAsync(loss, 0, sizeof(float), st));
+  float* dy = ctx.buf<float>("nn.dy", m);
+  k_loss<<<blocks_for(m), 256, 0, st>>>(a.y, T, m, dy, loss);
+  uint16_t* dZ2 = ctx.buf<uint16_t>("nn.dz2", m * kHidden);
+  k_back_out<<<blocks_for(m, kRowsPerBlock), kHidden, 0, st>>>(a.H2, a.Z2, dy, P + kOffW3, m, G + kOffW3,
+                                                               G + kOffB3, G + kOffB2, dZ2);
+  // gW2 = H1^T dZ2 (split over the batch)
+  const int split = (int)std::max<size_t>(1, std::min<size_t>(64, m / 512));
+  gemm_bf16(ctx, kHidden, kHidden, (int)m, a.H1b, 1, kHidden, dZ2, kHidden, 1, G + kOffW2, kHidden, split);
+  // dH1 = dZ2 W2^T
+  float* dH1 = ctx.buf<float>("nn.dh1", m * kHidden);
+  gemm_bf16(ctx, (int)m, kHidden, kHidden, dZ2, kHidden, 1, a.W2b, 1, kHidden, dH1, kHidden, 1);
+  uint16_t* dZ1 = ctx.buf<uint16_t>("nn.dz1", m * kHidden);
+  k_back_relu<<<blocks_for(m, kRowsPerBlock), kHidden, 0, st>>>(dH1, a.Z1, m, G + kOffB1, dZ1);
+  // gW1 = X^T dZ1
+  gemm_bf16(ctx, kFeatDim, kHidden, (int)m, X, 1, kFeatDim, dZ1, kHidden, 1, G + kOffW1, kHidden, split);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void train_2d(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb, size_t spf, int epochs,
+              int batch, float lr, uint64_t seed, float* params, float* loss_per_epoch) {
+  CPSZ_REQUIRE(!fields.empty() && spf > 0 && epochs > 0 && batch >= 32 && batch <= (1 << 20), kErrArg,
+               "train_2d: bad arguments");
+  hipStream_t st = ctx.stream();
+  const size_t S = fields.size() * spf;
+  uint16_t* Xall = ctx.buf<uint16_t>("tr.x", S * kFeatDim);
+  float* Tall = ctx.buf<float>("tr.t", S);
+  uint8_t* L8 = ctx.buf<uint8_t>("tr.l8", spf);
+  uint32_t* idx = ctx.buf<uint32_t>("tr.idx", std::max(spf, (size_t)batch));
+  uint32_t* hidx = (uint32_t*)ctx.host_buf("tr.idx", std::max(spf, (size_t)batch) * 4);
+  uint64_t rs = seed;
+  for (size_t f = 0; f < fields.size(); f++) {
+    const TrainField& F = fields[f];
+    const size_t n = F.r1 * F.r2;
+    CPSZ_REQUIRE(F.dU && F.dV && n > 0 && n < (1ull << 32), kErrArg, "train_2d: bad field");
+    double* E = ctx.buf<double>("tr.e", n);
+    analytic_eb_2d(ctx, F.dU, F.dV, F.r1, F.r2, max_pwr_eb, E);
+    CPSZ_HIP(hipStreamSynchronize(st));  // host index buffer reuse
+    for (size_t k = 0; k < spf; k++) hidx[k] = (uint32_t)(splitmix(rs) % n);
+    CPSZ_HIP(hipMemcpyAsync(idx, hidx, spf * 4, hipMemcpyHostToDevice, st));
+    features_2d(ctx, F.dU, F.dV, E, F.r1, F.r2, max_pwr_eb, idx, spf, Xall + f * spf * kFeatDim);
+    labels_2d(ctx, F.dU, F.dV, E, F.r1, F.r2, max_pwr_eb, idx, spf, L8);
+    k_u8_to_f32<<<blocks_for(spf), 256, 0, st>>>(L8, spf, Tall + f * spf);
+    CPSZ_HIP(hipGetLastError());
+  }
+  float* P = ctx.buf<float>("tr.p", kNumParams);
+  float* G = ctx.buf<float>("tr.g", kNumParams);
+  float* M1 = ctx.buf<float>("tr.m1", kNumParams);
+  float* M2 = ctx.buf<float>("tr.m2", kNumParams);
+  float* loss = ctx.buf<float>("tr.loss", 1);
+  CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, st));
+  CPSZ_HIP(hipMemsetAsync(M1, 0, kNumParams * 4, st));
+  CPSZ_HIP(hipMemsetAsync(M2, 0, kNumParams * 4, st));
+  uint16_t* Xb = ctx.buf<uint16_t>("tr.xb", (size_t)batch * kFeatDim);
+  float* Tb = ctx.buf<float>("tr.tb", batch);
+  uint32_t* perm = ctx.buf<uint32_t>("tr.perm", S);
+  std::vector<uint32_t> hp(S);
+  const size_t steps = S / batch;
+  CPSZ_REQUIRE(steps > 0, kErrArg, "train_2d: fewer samples than one batch");
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int t = 0;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (size_t k = 0; k < S; k++) hp[k] = (uint32_t)k;
+    for (size_t k = S - 1; k > 0; k--) std::swap(hp[k], hp[splitmix(rs) % (k + 1)]);
+    CPSZ_HIP(hipMemcpyAsync(perm, hp.data(), S * 4, hipMemcpyHostToDevice, st));
+    for (size_t b = 0; b < steps; b++) {
+      k_gather_rows<<<blocks_for((size_t)batch * kFeatDim), 256, 0, st>>>(Xall, Tall, perm + b * batch, batch,
+                                                                          Xb, Tb);
+      mlp_grad(ctx, P, Xb, Tb, batch, G, loss);
+      ++t;
+      const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+      k_adam<<<blocks_for(kNumParams), 256, 0, st>>>(P, G, M1, M2, kNumParams, lr, b1, b2, eps, c1, c2);
+      CPSZ_HIP(hipGetLastError());
+      float hl = 0;
+      if (b + 1 == steps) {  // the epoch's last batch loss
+        CPSZ_HIP(hipMemcpyAsync(&hl, loss, 4, hipMemcpyDeviceToHost, st));
+        CPSZ_HIP(hipStreamSynchronize(st));
+        if (loss_per_epoch) loss_per_epoch[ep] = hl;
+      }
+    }
+    CPSZ_HIP(hipStreamSynchronize(st));  // the permutation buffer is rewritten next epoch
+  }
+  CPSZ_HIP(hipMemcpyAsync(params, P, kNumParams * 4, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+}
+
+namespace {
+void cp_list(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, std::vector<int64_t>& ids,
+             std::vector<int32_t>& types) {
+  size_t cap = 1 << 16, count = 0;
+  for (;;) {
+    ids.resize(cap);
+    types.resize(cap);
+    critical_points_2d(ctx, dU, dV, r1, r2, ids.data(), types.data(), cap, &count);
+    if (count <= cap) break;
+    cap = count;
+  }
+  ids.resize(count);
+  types.resize(count);
+}
+}  // namespace
+
+void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
+                         double max_pwr_eb, const float* params, DeviceStream& out, LearnedStats& stats) {
+  CPSZ_REQUIRE(dU && dV && params && r1 * r2 < (1ull << 32), kErrArg, "compress_2d_learned: bad arguments");
+  hipStream_t st = ctx.stream();
+  const size_t n = r1 * r2;
+  stats = LearnedStats();
+  double* E = ctx.buf<double>("ln.e", n);
+  analytic_eb_2d(ctx, dU, dV, r1, r2, max_pwr_eb, E);
+  uint16_t* X = ctx.buf<uint16_t>("ln.x", n * kFeatDim);
+  features_2d(ctx, dU, dV, E, r1, r2, max_pwr_eb, nullptr, n, X);
+  float* P = ctx.buf<float>("ln.p", kNumParams);
+  CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, st));
+  float* y = ctx.buf<float>("ln.y", n);
+  mlp_forward(ctx, P, X, n, y);
+  uint8_t* gain = ctx.buf<uint8_t>("ln.gain", n);
+  k_gain<<<blocks_for(n), 256, 0, st>>>(y, n, gain);
+  CPSZ_HIP(hipGetLastError());
+  std::vector<uint8_t> hg(n), hf(n, 0);
+  CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, n, hipMemcpyDeviceToHost, st));
+  std::vector<int64_t> id0, id1;
+  std::vector<int32_t> ty0, ty1;
+  cp_list(ctx, dU, dV, r1, r2, id0, ty0);  // synchronizes
+  for (size_t v = 0; v < n; v++) stats.boosted_vertices += hg[v] != 0;
+  uint8_t* fb = ctx.buf<uint8_t>("ln.fb", n);
+  float* dU2 = ctx.buf<float>("ln.u2", n);
+  float* dV2 = ctx.buf<float>("ln.v2", n);
+  EbOverride ov;
+  ov.gain = gain;
+  ov.fallback = fb;
+  const int kMaxIter = 16;
+  for (int it = 0; it <= kMaxIter; it++) {
+    if (it == kMaxIter) std::fill(hf.begin(), hf.end(), 1);  // the plain method
+    CPSZ_HIP(hipMemcpyAsync(fb, hf.data(), n, hipMemcpyHostToDevice, st));
+    compress_2d_device(ctx, dU, dV, r1, r2, max_pwr_eb, out, ov);
+    decompress_2d_device(ctx, out.data, nullptr, out.size, r1, r2, dU2, dV2);
+    cp_list(ctx, dU2, dV2, r1, r2, id1, ty1);
+    stats.iterations = it + 1;
+    // simplices whose critical point appeared, vanished or changed type
+    std::vector<int64_t> bad;
+    size_t a = 0, b = 0;
+    while (a < id0.size() || b < id1.size()) {
+      if (b == id1.size() || (a < id0.size() && id0[a] < id1[b])) bad.push_back(id0[a++]);
+      else if (a == id0.size() || id1[b] < id0[a]) bad.push_back(id1[b++]);
+      else {
+        if (ty0[a] != ty1[b]) bad.push_back(id0[a]);
+        a++;
+        b++;
+      }
+    }
+    if (bad.empty()) break;
+    CPSZ_REQUIRE(it < kMaxIter, kErrInternal, "learned compression: the plain bound changed a critical point");
+    for (int64_t sid : bad) {
+      const int64_t cell = sid / 2;
+      const int t = (int)(sid % 2);
+      const size_t ci = (size_t)(cell / (int64_t)(r2 - 1)), cj = (size_t)(cell % (int64_t)(r2 - 1));
+      const size_t v0 = ci * r2 + cj;
+      const size_t v1 = t == 0 ? (ci + 1) * r2 + cj : ci * r2 + cj + 1;
+      const size_t v2 = (ci + 1) * r2 + cj + 1;
+      hf[v0] = hf[v1] = hf[v2] = 1;
+    }
+  }
+  for (size_t v = 0; v < n; v++) stats.fallback_vertices += hf[v] && hg[v];
+}
+
+}  // namespace cpsz

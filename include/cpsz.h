@@ -150,6 +150,63 @@ int cpsz_lorenzo3d_emulate(const float* L, const uint8_t* code, size_t r1, size_
 /* eb code of each bound as the device derives it (255 or the 3D id). */
 int cpsz_eb_code_3d(const double* eb, size_t n, uint8_t* code);
 
+/* ---- learned error-bound proposer (2D) -----------------------------------
+ * No counterpart in the reference (its closest precedent is the
+ * verify-and-halve loop of sz_compress_cp_preserve_2d_online_abs_relax_FN,
+ * src/sz_compress_cp_preserve_2d.cpp:1439-1502). A 51-128-128-1 MLP (BF16 on
+ * MFMA, FP32 accumulation) proposes per-vertex bound gains; every proposal is
+ * verified on the decompressed field by the exact critical-point judge and
+ * falls back to the analytic bound where it fails. Streams keep the reference
+ * layout. Parameters are a host float array of cpsz_learn_param_count():
+ * W1[64][128], b1[128], W2[128][128], b2[128], w3[128], b3. */
+size_t cpsz_learn_param_count(void);
+int cpsz_learn_init_params(uint64_t seed, float* params);
+/* The plain method's per-vertex bound before quantization (dE: r1*r2 doubles). */
+int cpsz_learn_analytic_eb_2d_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                  size_t r2, double max_pwr_eb, double* dE);
+/* BF16 features (64 per vertex, 51 used) of vertices d_idx[0..m) (all, row
+ * major, when d_idx is NULL). */
+int cpsz_learn_features_2d_dev(int device, void* stream, const float* dU, const float* dV,
+                               const double* dE, size_t r1, size_t r2, double max_pwr_eb,
+                               const uint32_t* d_idx, size_t m, uint16_t* dX);
+/* Bisection labels: the largest gain (0..24) that keeps every incident
+ * triangle's critical point when the vertex alone moves to any corner of its
+ * bound box. */
+int cpsz_learn_labels_2d_dev(int device, void* stream, const float* dU, const float* dV,
+                             const double* dE, size_t r1, size_t r2, double max_pwr_eb,
+                             const uint32_t* d_idx, size_t m, uint8_t* dL);
+/* C[M][N] = sum_k A(m,k) B(k,n), BF16 operands with element strides, FP32 C
+ * (row-major, ldc); split > 1 accumulates K slices atomically into C. */
+int cpsz_gemm_bf16_dev(int device, void* stream, int M, int N, int K, const uint16_t* A, int64_t sam,
+                       int64_t sak, const uint16_t* B, int64_t sbk, int64_t sbn, float* C, int ldc,
+                       int split);
+/* Network output for m feature rows; gradient and mean squared error of a
+ * batch against targets dT (params, grad and loss in host memory). */
+int cpsz_mlp_forward_dev(int device, void* stream, const float* params, const uint16_t* dX, size_t m,
+                         float* dY);
+int cpsz_mlp_grad_dev(int device, void* stream, const float* params, const uint16_t* dX,
+                      const float* dT, size_t m, float* grad, float* loss);
+/* Trains on samples_per_field random vertices of each field (features and
+ * bisection labels built on the GPU) with Adam; params is the start point and
+ * receives the result; loss_per_epoch (may be NULL) gets each epoch's last
+ * batch loss. */
+int cpsz_learn_train_2d_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                            const float* const* dV, const size_t* r1, const size_t* r2,
+                            double max_pwr_eb, size_t samples_per_field, int epochs, int batch,
+                            float lr, uint64_t seed, float* params, float* loss_per_epoch);
+/* Learned-bound compression; the stream is decodable by
+ * cpsz_decompress_2d_f32 and the reference decoder, and the decoded field has
+ * the original's critical points. iterations: verification rounds; boosted:
+ * vertices given a gain; fallback: boosted vertices sent back to the
+ * analytic bound. */
+int cpsz_compress_2d_learned_f32_dev(int device, void* stream, const float* dU, const float* dV,
+                                     size_t r1, size_t r2, double max_pwr_eb, const float* params,
+                                     const unsigned char** d_out, size_t* out_size, int* iterations,
+                                     size_t* boosted, size_t* fallback);
+int cpsz_compress_2d_learned_f32(const float* U, const float* V, size_t r1, size_t r2,
+                                 double max_pwr_eb, const float* params, unsigned char** out,
+                                 size_t* out_size);
+
 /* ---- instrumentation ------------------------------------------------------ */
 /* Enables per-stage HIP-event timing for this thread's context on `device`. */
 int cpsz_set_timing(int device, int on);

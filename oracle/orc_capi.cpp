@@ -138,6 +138,18 @@ int orc_tet_bound_3d(const float* uvw12, size_t n, double* out) {
   return 0;
 }
 // Critical points: writes up to cap (simplex, type) pairs, returns the count.
+int orc_learn_features_2d(const float* U, const float* V, const double* E, size_t r1, size_t r2,
+                          double max_pwr_eb, const uint32_t* idx, size_t m, uint16_t* X) {
+  learn_features_2d(U, V, E, r1, r2, max_pwr_eb, idx, m, X);
+  return 0;
+}
+
+int orc_learn_labels_2d(const float* U, const float* V, const double* E, size_t r1, size_t r2,
+                        double max_pwr_eb, const uint32_t* idx, size_t m, uint8_t* L) {
+  learn_labels_2d(U, V, E, r1, r2, max_pwr_eb, idx, m, L);
+  return 0;
+}
+
 long orc_cp_2d(const float* U, const float* V, size_t r1, size_t r2, int64_t* simplex, int* type,
                size_t cap) {
   auto cps = cp_extract_2d(U, V, r1, r2);

@@ -219,6 +219,17 @@ bool zero_in_tet(const float v[4][3]) {
 
 }  // namespace
 
+int tri_cp_2d(const float f[3][2], int lower) {
+  const double X1[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  const double X2[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+  for (int p = 0; p < 3; p++)
+    if (f[p][0] == 0 && f[p][1] == 0) return -1;
+  if (!zero_in_triangle(f[0], f[1], f[2])) return -1;
+  double v[3][2];
+  for (int p = 0; p < 3; p++) v[p][0] = f[p][0], v[p][1] = f[p][1];
+  return type_2d(v, lower ? X2 : X1);
+}
+
 std::vector<CP> cp_extract_2d(const float* U, const float* V, size_t r1, size_t r2) {
   std::vector<CP> out;
   const double X1[3][2] = {{0, 0}, {0, 1}, {1, 1}};

@@ -43,6 +43,8 @@ def load_oracle():
         f.argtypes = args
         f.restype = C.c_int
     lib.orc_cp_2d.argtypes = [_P, _P, _SZ, _SZ, _P, _P, _SZ]
+    lib.orc_learn_features_2d.argtypes = [_P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P]
+    lib.orc_learn_labels_2d.argtypes = [_P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P]
     lib.orc_cp_2d.restype = C.c_long
     lib.orc_cp_3d.argtypes = [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ]
     lib.orc_cp_3d.restype = C.c_long
@@ -148,6 +150,26 @@ class Oracle:
         out = np.empty(uvw12.shape[0], np.float64)
         self.lib.orc_tet_bound_3d(_ptr(uvw12), uvw12.shape[0], _ptr(out))
         return out
+
+    def learn_features_2d(self, U, V, E, eb, idx=None):
+        U, V = (np.ascontiguousarray(x, np.float32) for x in (U, V))
+        E = np.ascontiguousarray(E, np.float64)
+        m = U.size if idx is None else len(idx)
+        idx_a = None if idx is None else np.ascontiguousarray(idx, np.uint32)
+        X = np.zeros((m, 64), np.uint16)
+        self.lib.orc_learn_features_2d(_ptr(U), _ptr(V), _ptr(E), U.shape[0], U.shape[1], eb,
+                                       None if idx_a is None else _ptr(idx_a), m, _ptr(X))
+        return X
+
+    def learn_labels_2d(self, U, V, E, eb, idx=None):
+        U, V = (np.ascontiguousarray(x, np.float32) for x in (U, V))
+        E = np.ascontiguousarray(E, np.float64)
+        m = U.size if idx is None else len(idx)
+        idx_a = None if idx is None else np.ascontiguousarray(idx, np.uint32)
+        L = np.zeros(m, np.uint8)
+        self.lib.orc_learn_labels_2d(_ptr(U), _ptr(V), _ptr(E), U.shape[0], U.shape[1], eb,
+                                     None if idx_a is None else _ptr(idx_a), m, _ptr(L))
+        return L
 
     def cp_2d(self, U, V):
         r1, r2 = U.shape
