@@ -38,6 +38,9 @@ def parse():
     p.add_argument("--workload", choices=["3d", "2d"], default="3d")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-cp", action="store_true", help="skip the critical-point check")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="only run a plain 256 MiB device copy (the rocprofv3 FETCH_SIZE / WRITE_SIZE "
+                        "calibration: 256 MiB read and 256 MiB written per dispatch)")
     return p.parse_args()
 
 
@@ -98,6 +101,15 @@ def main():
         sys.exit(2)
     import torch
 
+    if a.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
+                          "dispatches": 5}))
+        return
     import cpsz_amd as cz
     from cpsz_amd.synth import field_2d, field_3d
 

@@ -8,10 +8,11 @@
 //                the id code from the host-libm threshold table
 //   k_signmap    three MSB-first sign bitmaps
 //   k_log        host-libm-exact log2f of |x| (-100 for zeros)
-//   k_lorenzo3d  wavefront over items (plane i, 64-row j block): lane = j, the
-//                wave sweeps k with a one-step skew per lane; j-1 values arrive
-//                through DPP wave_shr:1, plane i-1 values through agent-coherent
-//                loads of the reconstruction with progress counters
+//   k_wave3d     wavefront: a workgroup owns 4 consecutive planes of a 64-row
+//                j block (one wave per plane, lane = j); waves sweep k with a
+//                one-step skew per lane; j-1 values arrive through DPP
+//                wave_shr:1, plane i-1 values through LDS inside the workgroup
+//                and agent-coherent loads behind progress counters across them
 //   Huffman      ids (512 states) and the codes of predictable vertices
 // Decompress mirrors it and finishes with host-libm-exact exp2f.
 //
@@ -176,20 +177,20 @@ __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const
   if (bad) atomicOr(flags, 1);
 }
 
-// Three MSB-first sign bitmaps, one 32-element word per thread.
-__global__ void k_signmap(const float* __restrict__ U, const float* __restrict__ V,
-                          const float* __restrict__ W, size_t n, size_t nw, uint32_t* __restrict__ out) {
-  const size_t w = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (w >= 3 * nw) return;
-  const int p = (int)(w / nw);
-  const size_t w0 = w - p * nw;
+// Three MSB-first sign bitmaps (1 = positive): each wave ballots 64 consecutive
+// values; reversing the 64 ballot bits and then the byte order puts value 8k+j
+// at bit 7-j of byte k.
+__global__ void __launch_bounds__(256) k_signmap(const float* __restrict__ U, const float* __restrict__ V,
+                                                 const float* __restrict__ W, size_t n, size_t nq,
+                                                 uint64_t* __restrict__ out) {
+  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;  // value index over 3 * nq * 64
+  const size_t q = g >> 6;                                           // 64-value group
+  if (q >= 3 * nq) return;  // whole waves leave together (3 * nq * 64 is a multiple of 64)
+  const int p = (int)(q / nq);
+  const size_t e = (q - p * nq) * 64 + (threadIdx.x & 63);
   const float* X = p == 0 ? U : (p == 1 ? V : W);
-  uint32_t word = 0;
-  for (int b = 0; b < 32; b++) {
-    const size_t e = w0 * 32 + b;
-    if (e < n && X[e] > 0) word |= 1u << ((b & ~7) + 7 - (b & 7));
-  }
-  out[w] = word;
+  const uint64_t m = __ballot(e < n && X[e < n ? e : 0] > 0);
+  if ((threadIdx.x & 63) == 0) out[q] = __builtin_bswap64(__builtin_bitreverse64(m));
 }
 
 // log_transform (include/sz_cp_preserve_utils.hpp:205-221): log2f(|x|), -100
@@ -669,9 +670,9 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
     CPSZ_HIP(hipGetLastError());
   }
   ctx.mark("eb");
-  const size_t smap = (n - 1) / 8 + 1, nw = div_up(n, 32);
-  uint32_t* sm = ctx.buf<uint32_t>("c3.smap", 3 * nw);
-  k_signmap<<<(int)div_up(3 * nw, 256), 256, 0, st>>>(dU, dV, dW, n, nw, sm);
+  const size_t smap = (n - 1) / 8 + 1, nq = div_up(n, 64);
+  uint64_t* sm = ctx.buf<uint64_t>("c3.smap", 3 * nq);
+  k_signmap<<<(int)div_up(3 * nq * 64, 256), 256, 0, st>>>(dU, dV, dW, n, nq, sm);
   float* L = ctx.buf<float>("c3.log", 3 * sk.per);
   {
     const int bx = r3 >= 256 ? 256 : (int)div_up(r3, 64) * 64;
@@ -801,7 +802,7 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
     pos += len;
   };
   host_piece(h0);
-  for (int p = 0; p < 3; p++) dev_piece(sm + p * nw, smap);
+  for (int p = 0; p < 3; p++) dev_piece(sm + p * nq, smap);
   host_piece(h1);
   dev_piece(unp, 12 * nunp);
   host_piece(h2);

@@ -65,10 +65,15 @@ __host__ __device__ inline double tet_bound(float u0, float u1, float u2, float 
   const double M3 = c_4;
   const double M = M0 + M1 + M2 + M3;
   if (M == 0) return 0;
-  const bool f0 = (M0 == 0) || (M / M0 > 1);
-  const bool f1 = (M1 == 0) || (M / M1 > 1);
-  const bool f2 = (M2 == 0) || (M / M2 > 1);
-  const bool f3 = (M3 == 0) || (M / M3 > 1);
+  // The reference tests M / Mk > 1 with a correctly rounded division; for
+  // Mk != 0 that equals M > Mk (Mk > 0) or M < Mk (Mk < 0): the smallest
+  // quotient above 1 that two doubles can form exceeds 1 + 2^-53, so rounding
+  // never moves it to 1, and NaN / infinity compare the same way in both forms.
+  auto above = [M](double mk) { return mk == 0 || (mk > 0 ? M > mk : M < mk); };
+  const bool f0 = above(M0);
+  const bool f1 = above(M1);
+  const bool f2 = above(M2);
+  const bool f3 = above(M3);
   if (f0 && f1 && f2 && f3) {
     if (same_dir(u0, u1, u2, u3) || same_dir(v0, v1, v2, v3) || same_dir(w0, w1, w2, w3)) return 1;
     return 0;

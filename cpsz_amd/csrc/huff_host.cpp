@@ -19,36 +19,40 @@ struct Node {
 // stops at a parent with freq <= new; remove moves the smaller child up along
 // one path (right child only if strictly smaller) and drops the last element
 // into the vacated leaf slot without re-sifting it (src/sz_huffman.cpp:119-144).
+// Entries pack (frequency << 24 | node id) so the sift loops touch one
+// 8-byte word per level; comparisons use the frequency bits only, keeping the
+// reference's tie behaviour.
 class Queue {
  public:
-  Queue(const std::vector<Node>& nodes, size_t cap) : nodes_(nodes), heap_(cap + 2) {}
+  explicit Queue(size_t cap) : heap_(cap + 2) {}
   int size() const { return end_ - 1; }
-  void push(int id) {
+  void push(int id, uint64_t freq) {
+    const uint64_t e = (freq << 24) | (uint64_t)id;
     int i = end_++;
     for (int p = i >> 1; p; p = i >> 1) {
-      if (nodes_[heap_[p]].freq <= nodes_[id].freq) break;
+      if ((heap_[p] >> 24) <= freq) break;
       heap_[i] = heap_[p];
       i = p;
     }
-    heap_[i] = id;
+    heap_[i] = e;
   }
   int pop() {
-    const int top = heap_[1];
+    const int top = (int)(heap_[1] & 0xFFFFFF);
     --end_;
     int i = 1;
     for (int c = 2; c < end_; c = i << 1) {
-      if (c + 1 < end_ && nodes_[heap_[c + 1]].freq < nodes_[heap_[c]].freq) ++c;
+      // branch-free child choice (heap_ has a spare slot past end_)
+      c += (int)((c + 1 < end_) & ((heap_[c + 1] >> 24) < (heap_[c] >> 24)));
       heap_[i] = heap_[c];
       i = c;
     }
     heap_[i] = heap_[end_];
     return top;
   }
-  int top() const { return heap_[1]; }
+  int top() const { return (int)(heap_[1] & 0xFFFFFF); }
 
  private:
-  const std::vector<Node>& nodes_;
-  std::vector<int> heap_;
+  std::vector<uint64_t> heap_;
   int end_ = 1;
 };
 
@@ -100,13 +104,17 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
       leaves++;
     }
   CPSZ_REQUIRE(leaves > 0, kErrInternal, "huffman: empty input");
+  CPSZ_REQUIRE(2 * leaves < (1u << 24), kErrUnsupported, "huffman alphabet too large");
+  uint64_t total = 0;
+  for (size_t s = 0; s < nbins; s++) total += freq[s];
+  CPSZ_REQUIRE(total < (1ull << 40), kErrUnsupported, "huffman symbol count too large");
   std::vector<Node> nodes;
   nodes.reserve(2 * leaves);
-  Queue q(nodes, 2 * leaves);
+  Queue q(2 * leaves);
   for (size_t s = 0; s < nbins; s++) {
     if (!freq[s]) continue;
     nodes.push_back(Node{freq[s], -1, -1, (uint32_t)s});
-    q.push((int)nodes.size() - 1);
+    q.push((int)nodes.size() - 1, freq[s]);
   }
   // Merge: the reference passes two qremove() calls as arguments to
   // new_node(.., left, right) (:211); its GCC build evaluates them right to
@@ -115,7 +123,7 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
     const int r = q.pop();
     const int l = q.pop();
     nodes.push_back(Node{nodes[l].freq + nodes[r].freq, l, r, 0});
-    q.push((int)nodes.size() - 1);
+    q.push((int)nodes.size() - 1, nodes.back().freq);
   }
   const int root = q.top();
   // node_count counts leaves below state_num (src/sz_compression_utils.cpp:66-68).
@@ -127,6 +135,9 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
   const size_t N = t.node_count;
   const size_t w = N <= 256 ? 1 : (N <= 65536 ? 2 : 4);
   t.tree.assign(1 + 2 * N * w + 5 * N, 0);
+  t.syms.reserve(leaves);
+  t.codes.reserve(leaves);
+  t.lens.reserve(leaves);
   Serializer s{nodes, t, N, w, nullptr, nullptr, nullptr, nullptr};
   s.L = t.tree.data() + 1;
   s.R = s.L + N * w;

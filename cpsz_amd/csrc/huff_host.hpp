@@ -7,6 +7,7 @@
 // (:152-187), and the tree is serialized in preorder with u8/u16/u32 child
 // indices chosen by node count (:357-499).
 #pragma once
+#include <cstddef>
 #include <cstdint>
 #include <vector>
 
