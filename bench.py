@@ -215,8 +215,17 @@ def main():
     name, k_ms, per_v = max(cand, key=lambda x: x[1] if x[1] == x[1] else -1)
     alg = nv * per_v
     achieved = alg / (k_ms / 1e3) if k_ms == k_ms and k_ms > 0 else None
+    # HBM bytes per dispatch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE on this
+    # workload, scaled by the copy-kernel calibration (profiles/traffic_3d.json)
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tj = json.load(fh)
+        if tj.get("workload") == workload:
+            traffic = tj["bytes_per_dispatch"].get(name.split(" ")[0])
     roof = {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": traffic,
             "kernel": name, "kernel_ms": k_ms, "alg_bytes_per_launch": alg,
             "other_wavefront": {c[0]: {"ms": c[1], "GB/s": nv * c[2] / (c[1] / 1e3) / 1e9}
                                 for c in cand if c[0] != name and c[1] == c[1] and c[1] > 0}}
