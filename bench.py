@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=["3d", "2d"], default="3d")
+    p.add_argument("--workload", choices=["3d", "2d", "batch2d"], default="3d")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-cp", action="store_true", help="skip the critical-point check")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -94,6 +94,108 @@ def cpu_baseline(workload, fields, eb):
             "sample": sample, "compress_s": t1 - t0, "decompress_s": t2 - t1}
 
 
+def run_batch2d(a, torch):
+    """96 fields, h and w log-uniform in [256, 4096] (seeded), rel eb 0.1; one
+    batched compress + one batched decompress per step."""
+    import cpsz_amd as cz
+    from cpsz_amd.synth import batch_2d_shapes, field_2d_device
+
+    shapes = batch_2d_shapes(96, seed=0)
+    eb = 0.1
+    dev = [field_2d_device(r1, r2, seed=1000 + k) for k, (r1, r2) in enumerate(shapes)]
+    outs = [(torch.empty_like(u), torch.empty_like(v)) for u, v in dev]
+    nv = sum(r1 * r2 for r1, r2 in shapes)
+    field_bytes = 8 * nv
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    cz.set_timing(True, 0)
+    fl = [(u.data_ptr(), v.data_ptr(), r1, r2) for (u, v), (r1, r2) in zip(dev, shapes)]
+    tuned = {"done": False}
+
+    def step():
+        base, off, size = cz.compress_2d_batch_dev(fl, eb, tune=not tuned["done"], stream=sh)
+        tuned["done"] = True
+        st_c = cz.last_stages(0)
+        cz.decompress_2d_batch_dev([(base + int(o), int(n)) for o, n in zip(off, size)], shapes,
+                                   [(x.data_ptr(), y.data_ptr()) for x, y in outs], stream=sh)
+        st_d = cz.last_stages(0)
+        return base, off, size, st_c, st_d
+
+    for _ in range(max(a.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    stages_c, stages_d = {}, {}
+    ev0.record(stream)
+    for _ in range(a.steps):
+        base, off, size, st_c, st_d = step()
+        for k, v in st_c:
+            stages_c[k] = stages_c.get(k, 0.0) + v / a.steps
+        for k, v in st_d:
+            stages_d[k] = stages_d.get(k, 0.0) + v / a.steps
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_step = ev0.elapsed_time(ev1) / 1e3 / a.steps
+    # outside the timed region: ratio (ZSTD-3 per stream), CP errors per field
+    total = int(off[-1] + size[-1])
+    allb = np.empty(total, np.uint8)
+    C.CDLL("libamdhip64.so").hipMemcpy(C.c_void_p(allb.ctypes.data), C.c_void_p(base), C.c_size_t(total), 2)
+    zbytes = sum(cz.sz_lossless_compress(allb[o:o + n], 3).size for o, n in zip(off, size))
+    cp_err = None
+    if not a.no_cp:
+        fp = fn = ft = tot = 0
+        for (u, v), (x, y), (r1, r2) in zip(dev, outs, shapes):
+            c0 = cz.critical_points_2d_dev(u.data_ptr(), v.data_ptr(), r1, r2)
+            c1 = cz.critical_points_2d_dev(x.data_ptr(), y.data_ptr(), r1, r2)
+            tot += len(c0)
+            fp += sum(1 for k in c1 if k not in c0)
+            fn += sum(1 for k in c0 if k not in c1)
+            ft += sum(1 for k in c0 if k in c1 and c0[k] != c1[k])
+        cp_err = {"cp_total": tot, "fp": fp, "fn": fn, "ft": ft}
+    # dominant kernel: the batched wavefront launch (16 algorithmic B per vertex)
+    cand = [("k_lorenzo2d_batch<true> (batched 2D compress wavefront)", stages_c.get("lorenzo", float("nan"))),
+            ("k_lorenzo2d_batch<false> (batched 2D decompress wavefront)", stages_d.get("lorenzo", float("nan")))]
+    name, k_ms = max(cand, key=lambda x: x[1] if x[1] == x[1] else -1)
+    achieved = 16 * nv / (k_ms / 1e3) if k_ms == k_ms and k_ms > 0 else None
+    cpu = None
+    if not a.no_cpu:
+        # bounded sample: the reference (or the oracle) on the first fields up to ~3M vertices
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import _libs
+        pick, acc = [], 0
+        for k, (r1, r2) in enumerate(shapes):
+            if acc + r1 * r2 > 3_000_000 and pick:
+                break
+            pick.append(k)
+            acc += r1 * r2
+        ref = _libs.Reference() if _libs.have_reference() else _libs.load_oracle()
+        t0 = time.perf_counter()
+        for k in pick:
+            U, V = dev[k][0].cpu().numpy(), dev[k][1].cpu().numpy()
+            ref.decompress_2d(ref.compress_2d(U, V, eb), *shapes[k])
+        dt = time.perf_counter() - t0
+        cpu = {"value": 8 * acc / dt / 1e9, "unit": "GB/s", "cores": 1,
+               "kind": "reference" if _libs.have_reference() else "port",
+               "sample": f"{len(pick)} of the 96 fields ({acc} vertices), compress+decompress, one at a time"}
+    w, g = cz.batch_2d_config(shapes)
+    out = {
+        "metric": "compress+decompress GB/s and ratio at 0 CP errors: plain vs learned-eb, 1 GPU",
+        "value": field_bytes / t_step / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (generated on the GPU, seeded)",
+        "config": {"workload": "batched 2D plain cpSZ: 96 u,v float32 fields, h,w log-uniform in [256, 4096], "
+                               "rel eb 0.1", "method": "plain", "fields": 96, "vertices": nv,
+                   "field_bytes": field_bytes, "stream_bytes": int(np.sum(size)), "zstd_bytes": zbytes,
+                   "ratio": field_bytes / zbytes, "cp_errors": cp_err, "autotuned": {"workers": w, "grid_cap": g},
+                   "stages_compress_ms": stages_c, "stages_decompress_ms": stages_d},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
+                     "kernel": name, "kernel_ms": k_ms, "alg_bytes_per_launch": 16 * nv},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -109,6 +211,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4,
                           "dispatches": 5}))
+        return
+    if a.workload == "batch2d":
+        torch.cuda.set_device(0)
+        run_batch2d(a, torch)
         return
     import cpsz_amd as cz
     from cpsz_amd.synth import field_2d, field_3d

@@ -40,6 +40,10 @@ _SIGS = {
     "cpsz_set_timing": [C.c_int, C.c_int],
     "cpsz_last_stages": [C.c_int, C.c_char_p, _SZ, _P, C.c_int],
     "cpsz_device_count": [],
+    "cpsz_compress_2d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, C.c_double, C.c_int, C.POINTER(_P), _P,
+                                   _P],
+    "cpsz_decompress_2d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P],
+    "cpsz_batch_2d_config": [C.c_int, _SZ, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "cpsz_learn_init_params": [C.c_uint64, _P],
     "cpsz_learn_analytic_eb_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, _P],
     "cpsz_learn_features_2d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P],
@@ -321,3 +325,43 @@ def sz_compress_cp_preserve_2d_learned(U, V, eb: float, params: np.ndarray) -> n
     _check(lib().cpsz_compress_2d_learned_f32(_ptr(U), _ptr(V), U.shape[0], U.shape[1], eb, _ptr(params),
                                               C.byref(p), C.byref(s)))
     return _take(p, s.value)
+
+
+# --------------------------------------------------------------- 2D batches --
+def _arr(ctype, vals):
+    return (ctype * len(vals))(*vals)
+
+
+def compress_2d_batch_dev(fields, eb: float, tune: bool = False, device: int = 0, stream: int = 0):
+    """fields: list of (dU, dV, r1, r2) device pointers. Returns (base pointer of the
+    library-owned stream buffer, offsets, sizes)."""
+    n = len(fields)
+    base = _P()
+    off = np.zeros(n, np.uint64)
+    size = np.zeros(n, np.uint64)
+    _check(lib().cpsz_compress_2d_batch_dev(device, stream or None, n, _arr(C.c_void_p, [f[0] for f in fields]),
+                                            _arr(C.c_void_p, [f[1] for f in fields]),
+                                            _arr(C.c_size_t, [f[2] for f in fields]),
+                                            _arr(C.c_size_t, [f[3] for f in fields]), eb, int(tune),
+                                            C.byref(base), _ptr(off), _ptr(size)))
+    return base.value, off.astype(np.int64), size.astype(np.int64)
+
+
+def decompress_2d_batch_dev(streams, shapes, outs, device: int = 0, stream: int = 0):
+    """streams: list of (device pointer, size); shapes: (r1, r2); outs: (dU, dV) pointers."""
+    n = len(streams)
+    _check(lib().cpsz_decompress_2d_batch_dev(device, stream or None, n, _arr(C.c_void_p, [s[0] for s in streams]),
+                                              _arr(C.c_size_t, [s[1] for s in streams]),
+                                              _arr(C.c_size_t, [s[0] for s in shapes]),
+                                              _arr(C.c_size_t, [s[1] for s in shapes]),
+                                              _arr(C.c_void_p, [o[0] for o in outs]),
+                                              _arr(C.c_void_p, [o[1] for o in outs])))
+
+
+def batch_2d_config(shapes, device: int = 0):
+    """(workers, grid cap) the batch calls use for batches of these shapes."""
+    n = len(shapes)
+    w, g = C.c_int(), C.c_int()
+    _check(lib().cpsz_batch_2d_config(device, n, _arr(C.c_size_t, [s[0] for s in shapes]),
+                                      _arr(C.c_size_t, [s[1] for s in shapes]), C.byref(w), C.byref(g)))
+    return w.value, g.value

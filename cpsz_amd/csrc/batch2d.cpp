@@ -1,0 +1,263 @@
+// Batched compression / decompression of many 2D fields of uneven sizes.
+//
+// Compress: the eb kernel runs per field on the caller's stream, then ONE
+// persistent wavefront launch covers every field's strips (a single field
+// cannot fill the GPU: its wavefront has at most 2 * ceil(r1/64) waves), then
+// the entropy stage (histograms, host Huffman trees, encode, stream assembly)
+// of different fields runs concurrently on a pool of worker threads, each with
+// its own HIP stream and context, so host tree building and small kernels of
+// one field overlap with those of others. Decompress mirrors it: per-field
+// parsing and Huffman decoding on the workers, then one persistent wavefront
+// launch. Every field's stream is byte-identical to the single-field codec's.
+//
+// The autotuner times candidate configurations (worker count, persistent grid
+// size) on the caller's batch itself (outputs do not depend on them) and
+// caches the fastest per device and batch signature.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "batch2d.hpp"
+
+namespace cpsz {
+
+namespace {
+
+// Worker threads with one context (HIP stream + scratch) each; jobs run with
+// at most `limit` workers in flight.
+class Pool {
+ public:
+  Pool(int device, int n) : device_(device) {
+    for (int k = 0; k < n; k++) threads_.emplace_back([this] { loop(); });
+  }
+  ~Pool() {
+    {
+      std::lock_guard<std::mutex> g(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : threads_) t.join();
+  }
+  int size() const { return (int)threads_.size(); }
+  // Runs every job; rethrows the first failure after all have finished.
+  void run(std::vector<std::function<void(Ctx&)>>& jobs, int limit) {
+    std::unique_lock<std::mutex> g(m_);
+    limit_ = std::max(1, std::min(limit, size()));
+    for (auto& j : jobs) q_.push_back(&j);
+    pending_ = jobs.size();
+    first_error_ = nullptr;
+    cv_.notify_all();
+    done_.wait(g, [&] { return pending_ == 0; });
+    if (first_error_) std::rethrow_exception(first_error_);
+  }
+
+ private:
+  void loop() {
+    CPSZ_HIP(hipSetDevice(device_));
+    std::unique_ptr<Ctx> ctx;
+    for (;;) {
+      std::function<void(Ctx&)>* job = nullptr;
+      {
+        std::unique_lock<std::mutex> g(m_);
+        cv_.wait(g, [&] { return stop_ || (!q_.empty() && active_ < limit_); });
+        if (stop_) return;
+        job = q_.front();
+        q_.pop_front();
+        active_++;
+      }
+      std::exception_ptr err;
+      try {
+        if (!ctx) ctx.reset(new Ctx(device_));
+        (*job)(*ctx);
+      } catch (...) {
+        err = std::current_exception();
+      }
+      {
+        std::lock_guard<std::mutex> g(m_);
+        active_--;
+        if (err && !first_error_) first_error_ = err;
+        if (--pending_ == 0) done_.notify_all();
+      }
+      cv_.notify_all();
+    }
+  }
+
+  int device_;
+  std::vector<std::thread> threads_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  std::deque<std::function<void(Ctx&)>*> q_;
+  size_t pending_ = 0;
+  int active_ = 0, limit_ = 1;
+  bool stop_ = false;
+  std::exception_ptr first_error_;
+};
+
+constexpr int kMaxWorkers = 16;
+
+// Pools live until the process ends and are never destroyed: their threads
+// and contexts must not be torn down after the HIP runtime at exit.
+Pool& pool_for(int device) {
+  static std::mutex m;
+  static std::map<int, Pool*>* pools = new std::map<int, Pool*>();
+  std::lock_guard<std::mutex> g(m);
+  Pool*& p = (*pools)[device];
+  if (!p) p = new Pool(device, kMaxWorkers);
+  return *p;
+}
+
+std::string signature(const std::vector<Field2d>& f) {
+  size_t n = 0, rmax = 0;
+  for (const auto& x : f) {
+    n += x.r1 * x.r2;
+    rmax = std::max(rmax, x.r1);
+  }
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return std::to_string(dev) + ":" + std::to_string(f.size()) + ":" + std::to_string(n >> 20) + ":" +
+         std::to_string(rmax >> 6);
+}
+
+std::mutex g_tune_m;
+std::map<std::string, Batch2dConfig> g_tuned;
+
+size_t stream_bound(size_t n) { return 12 * n + (1u << 20); }
+
+}  // namespace
+
+void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_pwr_eb,
+                       const Batch2dConfig& cfg, BatchOut& out) {
+  const size_t nf = fields.size();
+  CPSZ_REQUIRE(nf > 0, kErrArg, "compress_2d_batch: empty batch");
+  hipStream_t st = ctx.stream();
+  ctx.stages.clear();
+  ctx.mark("start");
+  std::vector<size_t> soff(nf + 1, 0);
+  out.off.assign(nf, 0);
+  out.size.assign(nf, 0);
+  size_t obytes = 0;
+  for (size_t k = 0; k < nf; k++) {
+    CPSZ_REQUIRE(fields[k].dU && fields[k].dV && fields[k].r1 && fields[k].r2, kErrArg,
+                 "compress_2d_batch: bad field");
+    soff[k + 1] = soff[k] + c2_scratch_bytes(fields[k].r1, fields[k].r2);
+    out.off[k] = obytes;
+    obytes += (stream_bound(fields[k].r1 * fields[k].r2) + 255) & ~(size_t)255;
+  }
+  uint8_t* scratch = ctx.buf<uint8_t>("b2c.scratch", soff[nf]);
+  uint8_t* dout = ctx.buf<uint8_t>("b2c.out", obytes);
+  std::vector<C2Work> w(nf);
+  std::vector<C2Work*> wp(nf);
+  for (size_t k = 0; k < nf; k++) {
+    w[k].dU = fields[k].dU;
+    w[k].dV = fields[k].dV;
+    w[k].r1 = fields[k].r1;
+    w[k].r2 = fields[k].r2;
+    c2_bind(w[k], scratch + soff[k]);
+    wp[k] = &w[k];
+    c2_eb(ctx, w[k], max_pwr_eb, EbOverride());
+  }
+  ctx.mark("eb");
+  c2_check(ctx, wp.data(), nf);
+  c2_wavefront(ctx, wp.data(), nf, cfg.grid_cap);
+  CPSZ_HIP(hipStreamSynchronize(st));
+  ctx.mark("lorenzo");
+  std::vector<std::function<void(Ctx&)>> jobs(nf);
+  for (size_t k = 0; k < nf; k++) {
+    jobs[k] = [&, k](Ctx& wc) {
+      DeviceStream s;
+      c2_entropy(wc, w[k], s);
+      CPSZ_REQUIRE(s.size <= stream_bound(w[k].r1 * w[k].r2), kErrInternal, "batch stream bound exceeded");
+      CPSZ_HIP(hipMemcpyAsync(dout + out.off[k], s.data, s.size, hipMemcpyDeviceToDevice, wc.stream()));
+      CPSZ_HIP(hipStreamSynchronize(wc.stream()));
+      out.size[k] = s.size;
+    };
+  }
+  pool_for(ctx.device()).run(jobs, cfg.workers);
+  ctx.mark("entropy");
+  out.base = dout;
+  ctx.finish_marks();
+}
+
+void decompress_2d_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<size_t>& in_size,
+                         const std::vector<Field2dOut>& fields, const Batch2dConfig& cfg) {
+  const size_t nf = fields.size();
+  CPSZ_REQUIRE(nf > 0 && d_in.size() == nf && in_size.size() == nf, kErrArg, "decompress_2d_batch: bad batch");
+  hipStream_t st = ctx.stream();
+  ctx.stages.clear();
+  ctx.mark("start");
+  std::vector<size_t> soff(nf + 1, 0);
+  for (size_t k = 0; k < nf; k++) {
+    CPSZ_REQUIRE(d_in[k] && fields[k].dU && fields[k].dV && fields[k].r1 && fields[k].r2, kErrArg,
+                 "decompress_2d_batch: bad field");
+    soff[k + 1] = soff[k] + d2_scratch_bytes(fields[k].r1, fields[k].r2);
+  }
+  uint8_t* scratch = ctx.buf<uint8_t>("b2d.scratch", soff[nf]);
+  std::vector<D2Work> w(nf);
+  std::vector<D2Work*> wp(nf);
+  for (size_t k = 0; k < nf; k++) {
+    w[k].r1 = fields[k].r1;
+    w[k].r2 = fields[k].r2;
+    w[k].dU = fields[k].dU;
+    w[k].dV = fields[k].dV;
+    d2_bind(w[k], scratch + soff[k]);
+    wp[k] = &w[k];
+  }
+  CPSZ_HIP(hipStreamSynchronize(st));  // inputs written on the caller's stream
+  std::vector<std::function<void(Ctx&)>> jobs(nf);
+  for (size_t k = 0; k < nf; k++) {
+    jobs[k] = [&, k](Ctx& wc) {
+      d2_entropy(wc, d_in[k], nullptr, in_size[k], w[k]);
+      CPSZ_HIP(hipStreamSynchronize(wc.stream()));
+    };
+  }
+  pool_for(ctx.device()).run(jobs, cfg.workers);
+  ctx.mark("entropy");
+  d2_wavefront(ctx, wp.data(), nf, cfg.grid_cap);
+  d2_check(ctx, wp.data(), nf);
+  ctx.mark("lorenzo");
+  ctx.finish_marks();
+}
+
+Batch2dConfig batch_2d_config(Ctx& ctx, const std::vector<Field2d>& fields, double max_pwr_eb, bool tune) {
+  const std::string key = signature(fields);
+  {
+    std::lock_guard<std::mutex> g(g_tune_m);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end() || !tune) return it != g_tuned.end() ? it->second : Batch2dConfig();
+  }
+  const int workers[] = {4, 8, 16};
+  const int grids[] = {1024, 4096};
+  Batch2dConfig best;
+  double best_t = 1e30;
+  BatchOut o;
+  compress_2d_batch(ctx, fields, max_pwr_eb, best, o);  // warm-up: buffers, pools, tables
+  for (int wk : workers)
+    for (int gr : grids) {
+      Batch2dConfig c;
+      c.workers = wk;
+      c.grid_cap = gr;
+      CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+      const auto t0 = std::chrono::steady_clock::now();
+      compress_2d_batch(ctx, fields, max_pwr_eb, c, o);
+      CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+      const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (t < best_t) {
+        best_t = t;
+        best = c;
+      }
+    }
+  std::lock_guard<std::mutex> g(g_tune_m);
+  g_tuned[key] = best;
+  return best;
+}
+
+}  // namespace cpsz

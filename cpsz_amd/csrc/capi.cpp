@@ -8,6 +8,7 @@
 #include <string>
 
 #include "../../include/cpsz.h"
+#include "batch2d.hpp"
 #include "codec2d.hpp"
 #include "codec3d.hpp"
 #include "cp.hpp"
@@ -397,6 +398,62 @@ int cpsz_last_stages(int device, char* names, size_t names_cap, float* ms, int c
     }
   });
   return rc ? rc : count;
+}
+
+int cpsz_compress_2d_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                               const float* const* dV, const size_t* r1, const size_t* r2,
+                               double max_pwr_eb, int tune, const unsigned char** d_base,
+                               size_t* offsets, size_t* sizes) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && dU && dV && r1 && r2 && d_base && offsets && sizes, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<Field2d> f(nfields);
+    for (size_t k = 0; k < nfields; k++) f[k] = Field2d{dU[k], dV[k], r1[k], r2[k]};
+    const Batch2dConfig cfg = batch_2d_config(c, f, max_pwr_eb, tune != 0);
+    BatchOut o;
+    compress_2d_batch(c, f, max_pwr_eb, cfg, o);
+    *d_base = o.base;
+    for (size_t k = 0; k < nfields; k++) {
+      offsets[k] = o.off[k];
+      sizes[k] = o.size[k];
+    }
+  });
+}
+
+int cpsz_decompress_2d_batch_dev(int device, void* stream, size_t nfields,
+                                 const unsigned char* const* d_in, const size_t* in_size,
+                                 const size_t* r1, const size_t* r2, float* const* dU,
+                                 float* const* dV) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && d_in && in_size && r1 && r2 && dU && dV, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<Field2d> sig(nfields);
+    std::vector<Field2dOut> f(nfields);
+    std::vector<const uint8_t*> in(nfields);
+    std::vector<size_t> sz(nfields);
+    for (size_t k = 0; k < nfields; k++) {
+      sig[k] = Field2d{nullptr, nullptr, r1[k], r2[k]};
+      f[k] = Field2dOut{dU[k], dV[k], r1[k], r2[k]};
+      in[k] = d_in[k];
+      sz[k] = in_size[k];
+    }
+    decompress_2d_batch(c, in, sz, f, batch_2d_config(c, sig, 0.0, false));
+  });
+}
+
+int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const size_t* r2, int* workers,
+                         int* grid_cap) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && r1 && r2 && workers && grid_cap, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    std::vector<Field2d> sig(nfields);
+    for (size_t k = 0; k < nfields; k++) sig[k] = Field2d{nullptr, nullptr, r1[k], r2[k]};
+    const Batch2dConfig cfg = batch_2d_config(c, sig, 0.0, false);
+    *workers = cfg.workers;
+    *grid_cap = cfg.grid_cap;
+  });
 }
 
 size_t cpsz_learn_param_count(void) { return kNumParams; }

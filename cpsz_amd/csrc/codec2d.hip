@@ -254,88 +254,126 @@ struct Strip2d {
   int* prog;             // [2][nstrips] published columns of the boundary row
   int* ticket;
   int r1, r2, nstrips;
+  int wide;  // ids above 81: double-precision step
 };
+
+// One item (component, 64-row strip) of a field's wavefront.
+template <bool COMPRESS, bool WIDE>
+__device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane) {
+  const int r1 = a.r1, r2 = a.r2;
+  const int comp = item & 1, s = item >> 1;
+  const int row = s * 64 + lane;
+  const bool rvalid = row < r1;
+  const size_t roff = (size_t)(rvalid ? row : 0) * r2;
+  const float* X = COMPRESS ? a.x[comp] + roff : nullptr;
+  const uint16_t* C = a.c[comp] + roff;
+  uint16_t* Q = a.q[comp] + roff;
+  const float* RAW = COMPRESS ? nullptr : a.raw[comp] + roff;
+  float* OUT = COMPRESS ? nullptr : a.out[comp] + roff;
+  const bool has_up = s > 0;
+  const bool publish = s + 1 < a.nstrips;
+  int* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
+  int* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
+  int* prog_in = a.prog + comp * a.nstrips + (s > 0 ? s - 1 : 0);
+  int* prog_out = a.prog + comp * a.nstrips + s;
+  float Rprev = 0.f, Uprev = 0.f;
+  const int steps = r2 + 63;
+  for (int t0 = 0; t0 < steps; t0 += kCH) {
+    float inc = 0.f;
+    if (has_up && t0 < r2) {
+      const int need = min(t0 + kCH, r2);
+      while (ld_coh(prog_in) < need) __builtin_amdgcn_s_sleep(2);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      if (lane < kCH && t0 + lane < r2) inc = __int_as_float(ld_coh(bnd_in + t0 + lane));
+    }
+    float xs[kCH];
+    uint32_t cs[kCH];
+    uint32_t qs_in[kCH];
+    float raws[kCH];
+#pragma unroll
+    for (int k = 0; k < kCH; k++) {
+      const int j = t0 + k - lane;
+      const bool act = rvalid && j >= 0 && j < r2;
+      cs[k] = act ? C[j] : 0;
+      if (COMPRESS) {
+        xs[k] = act ? X[j] : 0.f;
+      } else {
+        qs_in[k] = act ? Q[j] : 0;
+        raws[k] = act ? RAW[j] : 0.f;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kCH; k++) {
+      const int t = t0 + k;
+      const int j = t - lane;
+      const bool act = rvalid && j >= 0 && j < r2;
+      const float lane0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inc), k));
+      const float Uv = shr1(Rprev, lane0);
+      const float pred = (Rprev + Uv) - Uprev;
+      float R;
+      if (COMPRESS) {
+        uint16_t code;
+        R = l2d_compress_step<WIDE>(pred, xs[k], cs[k], code);
+        if (act) Q[j] = code;
+      } else {
+        R = l2d_decompress_step<WIDE>(pred, qs_in[k], cs[k], raws[k]);
+        if (act) OUT[j] = R;
+      }
+      if (publish && lane == 63 && act) st_coh(bnd_out + j, __float_as_int(R));
+      Uprev = Uv;
+      Rprev = (j < 0) ? 0.f : R;
+    }
+    if (publish) {
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      __builtin_amdgcn_s_waitcnt(0);
+      __atomic_signal_fence(__ATOMIC_SEQ_CST);
+      const int done = min(max(t0 + kCH - 63, 0), r2);
+      if (lane == 63) st_coh(prog_out, done);
+    }
+  }
+}
 
 template <bool COMPRESS, bool WIDE>
 __global__ void __launch_bounds__(64) k_lorenzo2d(Strip2d a) {
   const int lane = threadIdx.x;
-  const int r1 = a.r1, r2 = a.r2;
   const int items = 2 * a.nstrips;
   for (;;) {
     int item = 0;
     if (lane == 0) item = atomicAdd(a.ticket, 1);
     item = __shfl(item, 0);
     if (item >= items) return;
-    const int comp = item & 1, s = item >> 1;
-    const int row = s * 64 + lane;
-    const bool rvalid = row < r1;
-    const size_t roff = (size_t)(rvalid ? row : 0) * r2;
-    const float* X = COMPRESS ? a.x[comp] + roff : nullptr;
-    const uint16_t* C = a.c[comp] + roff;
-    uint16_t* Q = a.q[comp] + roff;
-    const float* RAW = COMPRESS ? nullptr : a.raw[comp] + roff;
-    float* OUT = COMPRESS ? nullptr : a.out[comp] + roff;
-    const bool has_up = s > 0;
-    const bool publish = s + 1 < a.nstrips;
-    int* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
-    int* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
-    int* prog_in = a.prog + comp * a.nstrips + (s > 0 ? s - 1 : 0);
-    int* prog_out = a.prog + comp * a.nstrips + s;
-    float Rprev = 0.f, Uprev = 0.f;
-    const int steps = r2 + 63;
-    for (int t0 = 0; t0 < steps; t0 += kCH) {
-      float inc = 0.f;
-      if (has_up && t0 < r2) {
-        const int need = min(t0 + kCH, r2);
-        while (ld_coh(prog_in) < need) __builtin_amdgcn_s_sleep(2);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        if (lane < kCH && t0 + lane < r2) inc = __int_as_float(ld_coh(bnd_in + t0 + lane));
-      }
-      float xs[kCH];
-      uint32_t cs[kCH];
-      uint32_t qs_in[kCH];
-      float raws[kCH];
-#pragma unroll
-      for (int k = 0; k < kCH; k++) {
-        const int j = t0 + k - lane;
-        const bool act = rvalid && j >= 0 && j < r2;
-        cs[k] = act ? C[j] : 0;
-        if (COMPRESS) {
-          xs[k] = act ? X[j] : 0.f;
-        } else {
-          qs_in[k] = act ? Q[j] : 0;
-          raws[k] = act ? RAW[j] : 0.f;
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < kCH; k++) {
-        const int t = t0 + k;
-        const int j = t - lane;
-        const bool act = rvalid && j >= 0 && j < r2;
-        const float lane0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inc), k));
-        const float Uv = shr1(Rprev, lane0);
-        const float pred = (Rprev + Uv) - Uprev;
-        float R;
-        if (COMPRESS) {
-          uint16_t code;
-          R = l2d_compress_step<WIDE>(pred, xs[k], cs[k], code);
-          if (act) Q[j] = code;
-        } else {
-          R = l2d_decompress_step<WIDE>(pred, qs_in[k], cs[k], raws[k]);
-          if (act) OUT[j] = R;
-        }
-        if (publish && lane == 63 && act) st_coh(bnd_out + j, __float_as_int(R));
-        Uprev = Uv;
-        Rprev = (j < 0) ? 0.f : R;
-      }
-      if (publish) {
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        __builtin_amdgcn_s_waitcnt(0);
-        __atomic_signal_fence(__ATOMIC_SEQ_CST);
-        const int done = min(max(t0 + kCH - 63, 0), r2);
-        if (lane == 63) st_coh(prog_out, done);
-      }
+    strip_item<COMPRESS, WIDE>(a, item, lane);
+  }
+}
+
+// All fields of a batch in one persistent launch: items are numbered field by
+// field (start[f] = first item of field f) and taken from one ticket counter,
+// so every wait (strip s on strip s-1 of the same field) targets an earlier
+// ticket. Each field keeps its own boundary rows and progress counters.
+struct StripBatch {
+  const Strip2d* f;
+  const int* start;
+  int nf, items;
+  int* ticket;
+};
+
+template <bool COMPRESS>
+__global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
+  const int lane = threadIdx.x;
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(b.ticket, 1);
+    item = __shfl(item, 0);
+    if (item >= b.items) return;
+    int lo = 0, hi = b.nf - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (b.start[mid] <= item) lo = mid;
+      else hi = mid - 1;
     }
+    const Strip2d a = b.f[lo];
+    if (a.wide) strip_item<COMPRESS, true>(a, item - b.start[lo], lane);
+    else strip_item<COMPRESS, false>(a, item - b.start[lo], lane);
   }
 }
 
@@ -482,91 +520,127 @@ struct Piece {  // a contiguous part of the output stream
   bool device;
 };
 
-void launch_strips(Ctx& ctx, Strip2d& sa, bool compress, bool wide) {
+// Lorenzo wavefront for fields f[0..nf): one launch; a single field keeps the
+// plain kernel, several share one persistent launch.
+template <bool COMPRESS>
+void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
   hipStream_t st = ctx.stream();
-  const int nz = 2 * sa.nstrips + 1;
-  k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(sa.prog, nz);
-  const int items = 2 * sa.nstrips;
-  const int grid = items < 4096 ? items : 4096;
-  if (compress) {
-    if (wide) k_lorenzo2d<true, true><<<grid, 64, 0, st>>>(sa);
-    else k_lorenzo2d<true, false><<<grid, 64, 0, st>>>(sa);
-  } else {
-    if (wide) k_lorenzo2d<false, true><<<grid, 64, 0, st>>>(sa);
-    else k_lorenzo2d<false, false><<<grid, 64, 0, st>>>(sa);
+  if (f.size() == 1) {
+    Strip2d& sa = f[0];
+    const int nz = 2 * sa.nstrips + 1;
+    k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(sa.prog, nz);
+    const int items = 2 * sa.nstrips;
+    const int grid = std::min(items, grid_cap);
+    if (sa.wide) k_lorenzo2d<COMPRESS, true><<<grid, 64, 0, st>>>(sa);
+    else k_lorenzo2d<COMPRESS, false><<<grid, 64, 0, st>>>(sa);
+    CPSZ_HIP(hipGetLastError());
+    return;
   }
+  const size_t nf = f.size();
+  std::vector<int> start(nf + 1, 0);
+  for (size_t k = 0; k < nf; k++) {
+    const int nz = 2 * f[k].nstrips + 1;
+    k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(f[k].prog, nz);
+    start[k + 1] = start[k] + 2 * f[k].nstrips;
+  }
+  const size_t tbytes = nf * sizeof(Strip2d), sbytes = (nf + 1) * sizeof(int);
+  uint8_t* h = (uint8_t*)ctx.host_buf("w2b.tab", tbytes + sbytes);
+  std::memcpy(h, f.data(), tbytes);
+  std::memcpy(h + tbytes, start.data(), sbytes);
+  uint8_t* d = ctx.buf<uint8_t>("w2b.tab", tbytes + sbytes + 64);
+  int* ticket = (int*)(d + tbytes + sbytes + 16 - ((tbytes + sbytes) & 15));
+  CPSZ_HIP(hipMemcpyAsync(d, h, tbytes + sbytes, hipMemcpyHostToDevice, st));
+  k_zero_i32<<<1, 64, 0, st>>>(ticket, 1);
+  StripBatch b{(const Strip2d*)d, (const int*)(d + tbytes), (int)nf, start[nf], ticket};
+  const int grid = std::min(start[nf], grid_cap);
+  k_lorenzo2d_batch<COMPRESS><<<grid, 64, 0, st>>>(b);
   CPSZ_HIP(hipGetLastError());
+  // the host table is reused by the next batch only after this stream synchronizes
+  CPSZ_HIP(hipStreamSynchronize(st));
 }
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 
 }  // namespace
 
-void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
-                  double* dE) {
-  CPSZ_REQUIRE(dU && dV && dE && r1 >= 1 && r2 >= 1, kErrArg, "eb_2d: bad arguments");
+// ------------------------------------------------------------- staged form --
+size_t c2_scratch_bytes(size_t r1, size_t r2) {
+  const size_t n = r1 * r2, ns = div_up(r1, 64);
+  return 4 * al256(2 * n) + al256((kMaxEbCode + 2 * 65536) * 4) + al256(16) + al256(2 * ns * r2 * 4) +
+         al256((2 * ns + 1) * 4);
+}
+
+void c2_bind(C2Work& w, uint8_t* p) {
+  const size_t n = w.r1 * w.r2, ns = div_up(w.r1, 64);
+  auto take = [&](size_t bytes) {
+    uint8_t* q = p;
+    p += al256(bytes);
+    return q;
+  };
+  w.cu = (uint16_t*)take(2 * n);
+  w.cv = (uint16_t*)take(2 * n);
+  w.qu = (uint16_t*)take(2 * n);
+  w.qv = (uint16_t*)take(2 * n);
+  w.hist = (uint32_t*)take((kMaxEbCode + 2 * 65536) * 4);
+  w.flags = (int*)take(16);
+  w.bnd = (int*)take(2 * ns * w.r2 * 4);
+  w.prog = (int*)take((2 * ns + 1) * 4);
+}
+
+void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
+  CPSZ_REQUIRE(w.dU && w.dV && w.r1 >= 1 && w.r2 >= 1, kErrArg, "compress_2d: bad arguments");
+  CPSZ_REQUIRE(w.r1 < (1u << 30) && w.r2 < (1u << 30) && w.r1 * w.r2 < (1ull << 40), kErrUnsupported,
+               "compress_2d: field too large");
   hipStream_t st = ctx.stream();
-  const size_t n = r1 * r2;
-  uint16_t* cu = ctx.buf<uint16_t>("c2.cu", n);
-  uint16_t* cv = ctx.buf<uint16_t>("c2.cv", n);
-  uint32_t* hist = ctx.buf<uint32_t>("c2.hist", kMaxEbCode + 2 * 65536);
-  int* flags = ctx.buf<int>("c2.flags", 4);
-  CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
-  CPSZ_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
-  EbOverride ov;
-  ov.eb_out = dE;
-  dim3 grid((unsigned)div_up(r2, kTW), (unsigned)div_up(r1, kTH));
-  k_eb2d<<<grid, 256, 0, st>>>(dU, dV, (int)r1, (int)r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(), cu, cv,
-                               hist, flags, ov);
+  CPSZ_HIP(hipMemsetAsync(w.hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
+  CPSZ_HIP(hipMemsetAsync(w.flags, 0, 4 * sizeof(int), st));
+  dim3 grid((unsigned)div_up(w.r2, kTW), (unsigned)div_up(w.r1, kTH));
+  k_eb2d<<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
+                               w.cu, w.cv, w.hist, w.flags, ov);
   CPSZ_HIP(hipGetLastError());
 }
 
-void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
-                        double max_pwr_eb, DeviceStream& out, const EbOverride& ov) {
-  CPSZ_REQUIRE(dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "compress_2d: bad arguments");
-  CPSZ_REQUIRE(r1 < (1u << 30) && r2 < (1u << 30) && r1 * r2 < (1ull << 40), kErrUnsupported,
-               "compress_2d: field too large");
+void c2_check(Ctx& ctx, C2Work* const* w, size_t nf) {
   hipStream_t st = ctx.stream();
-  const size_t n = r1 * r2;
-  ctx.stages.clear();
-  ctx.mark("start");
-  // ---- eb codes
-  uint16_t* cu = ctx.buf<uint16_t>("c2.cu", n);
-  uint16_t* cv = ctx.buf<uint16_t>("c2.cv", n);
-  uint32_t* hist = ctx.buf<uint32_t>("c2.hist", kMaxEbCode + 2 * 65536);
-  int* flags = ctx.buf<int>("c2.flags", 4);
-  CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
-  CPSZ_HIP(hipMemsetAsync(flags, 0, 4 * sizeof(int), st));
-  {
-    dim3 grid((unsigned)div_up(r2, kTW), (unsigned)div_up(r1, kTH));
-    k_eb2d<<<grid, 256, 0, st>>>(dU, dV, (int)r1, (int)r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
-                                 cu, cv, hist, flags, ov);
-    CPSZ_HIP(hipGetLastError());
-  }
-  ctx.mark("eb");
-  int hflags[4];
-  CPSZ_HIP(hipMemcpyAsync(hflags, flags, sizeof(hflags), hipMemcpyDeviceToHost, st));
+  int* hf = (int*)ctx.host_buf("c2.hflags", std::max<size_t>(nf, 1) * 16);
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_HIP(hipMemcpyAsync(hf + 4 * k, w[k]->flags, 16, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
-  CPSZ_REQUIRE(hflags[0] == 0, kErrNonFinite,
-               "compress_2d: non-finite input or bound (the reference cannot encode it)");
-  const bool wide = hflags[1] != 0;
-  // ---- Lorenzo wavefront
-  const int nstrips = (int)div_up(r1, 64);
-  uint16_t* qu = ctx.buf<uint16_t>("c2.qu", n);
-  uint16_t* qv = ctx.buf<uint16_t>("c2.qv", n);
-  Strip2d sa{};
-  sa.x[0] = dU;
-  sa.x[1] = dV;
-  sa.c[0] = cu;
-  sa.c[1] = cv;
-  sa.q[0] = qu;
-  sa.q[1] = qv;
-  sa.bnd = ctx.buf<int>("c2.bnd", (size_t)2 * nstrips * r2);
-  sa.prog = ctx.buf<int>("c2.prog", 2 * nstrips + 1);
-  sa.ticket = sa.prog + 2 * nstrips;
-  sa.r1 = (int)r1;
-  sa.r2 = (int)r2;
-  sa.nstrips = nstrips;
-  launch_strips(ctx, sa, true, wide);
-  ctx.mark("lorenzo");
+  for (size_t k = 0; k < nf; k++) {
+    CPSZ_REQUIRE(hf[4 * k] == 0, kErrNonFinite,
+                 "compress_2d: non-finite input or bound (the reference cannot encode it)");
+    w[k]->wide = hf[4 * k + 1] != 0;
+  }
+}
+
+void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap) {
+  std::vector<Strip2d> f(nf);
+  for (size_t k = 0; k < nf; k++) {
+    Strip2d& sa = f[k];
+    sa = Strip2d{};
+    sa.x[0] = w[k]->dU;
+    sa.x[1] = w[k]->dV;
+    sa.c[0] = w[k]->cu;
+    sa.c[1] = w[k]->cv;
+    sa.q[0] = w[k]->qu;
+    sa.q[1] = w[k]->qv;
+    sa.nstrips = (int)div_up(w[k]->r1, 64);
+    sa.bnd = w[k]->bnd;
+    sa.prog = w[k]->prog;
+    sa.ticket = w[k]->prog + 2 * sa.nstrips;
+    sa.r1 = (int)w[k]->r1;
+    sa.r2 = (int)w[k]->r2;
+    sa.wide = w[k]->wide;
+  }
+  launch_wavefront<true>(ctx, f, grid_cap);
+}
+
+void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
+  hipStream_t st = ctx.stream();
+  const size_t r1 = w.r1, r2 = w.r2, n = r1 * r2;
+  const float *dU = w.dU, *dV = w.dV;
+  const uint16_t *cu = w.cu, *cv = w.cv, *qu = w.qu, *qv = w.qv;
+  uint32_t* hist = w.hist;
   // ---- histograms of the quant codes
   {
     const uint32_t win_lo = 32768 - 8192, win_n = 16384;
@@ -717,6 +791,40 @@ void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, s
   out.data = dout;
   out.size = total;
   CPSZ_HIP(hipStreamSynchronize(st));
+}
+
+void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
+                  double* dE) {
+  CPSZ_REQUIRE(dU && dV && dE && r1 >= 1 && r2 >= 1, kErrArg, "eb_2d: bad arguments");
+  C2Work w;
+  w.dU = dU;
+  w.dV = dV;
+  w.r1 = r1;
+  w.r2 = r2;
+  c2_bind(w, ctx.buf<uint8_t>("c2.scratch", c2_scratch_bytes(r1, r2)));
+  EbOverride ov;
+  ov.eb_out = dE;
+  c2_eb(ctx, w, max_pwr_eb, ov);
+}
+
+void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
+                        double max_pwr_eb, DeviceStream& out, const EbOverride& ov) {
+  ctx.stages.clear();
+  ctx.mark("start");
+  C2Work w;
+  w.dU = dU;
+  w.dV = dV;
+  w.r1 = r1;
+  w.r2 = r2;
+  CPSZ_REQUIRE(r1 >= 1 && r2 >= 1 && r1 < (1u << 30) && r2 < (1u << 30), kErrArg, "compress_2d: bad shape");
+  c2_bind(w, ctx.buf<uint8_t>("c2.scratch", c2_scratch_bytes(r1, r2)));
+  c2_eb(ctx, w, max_pwr_eb, ov);
+  ctx.mark("eb");
+  C2Work* wp = &w;
+  c2_check(ctx, &wp, 1);
+  c2_wavefront(ctx, &wp, 1, 4096);
+  ctx.mark("lorenzo");
+  c2_entropy(ctx, w, out);
   ctx.finish_marks();
 }
 
@@ -772,13 +880,35 @@ HuffSection parse_huff(StreamReader& r, size_t off) {
 
 }  // namespace
 
-void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
-                          size_t r1, size_t r2, float* dU, float* dV) {
-  CPSZ_REQUIRE(d_in && dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "decompress_2d: bad arguments");
+size_t d2_scratch_bytes(size_t r1, size_t r2) {
+  const size_t n = r1 * r2, ns = div_up(r1, 64);
+  return 4 * al256(2 * n) + al256(8 * n) + al256(2 * ns * r2 * 4) + al256((2 * ns + 1) * 4) + al256(16);
+}
+
+void d2_bind(D2Work& w, uint8_t* p) {
+  const size_t n = w.r1 * w.r2, ns = div_up(w.r1, 64);
+  auto take = [&](size_t bytes) {
+    uint8_t* q = p;
+    p += al256(bytes);
+    return q;
+  };
+  w.sid_u = (uint16_t*)take(2 * n);
+  w.sid_v = (uint16_t*)take(2 * n);
+  w.qu = (uint16_t*)take(2 * n);
+  w.qv = (uint16_t*)take(2 * n);
+  w.raw = (float*)take(8 * n);
+  w.bnd = (int*)take(2 * ns * w.r2 * 4);
+  w.prog = (int*)take((2 * ns + 1) * 4);
+  w.err = (int*)take(16);
+}
+
+void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, D2Work& w) {
+  CPSZ_REQUIRE(d_in && w.r1 >= 1 && w.r2 >= 1, kErrArg, "decompress_2d: bad arguments");
   hipStream_t st = ctx.stream();
-  const size_t n = r1 * r2;
-  ctx.stages.clear();
-  ctx.mark("start");
+  const size_t r1 = w.r1, r2 = w.r2, n = r1 * r2;
+  uint16_t *sid_u = w.sid_u, *sid_v = w.sid_v, *qu = w.qu, *qv = w.qv;
+  float* raw = w.raw;
+  int* err = w.err;
   StreamReader rd{ctx, h_in, d_in, in_size, {}};
   const int base = rd.get<int>(0);
   const double thr = rd.get<double>(4);
@@ -825,10 +955,6 @@ void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
       CPSZ_HIP(hipMemcpyAsync(w, d_in + h.bits_off, h.nbytes, hipMemcpyDeviceToDevice, st));
     return w;
   };
-  uint16_t* sid_u = ctx.buf<uint16_t>("d2.sidu", n);
-  uint16_t* sid_v = ctx.buf<uint16_t>("d2.sidv", n);
-  uint16_t* qu = ctx.buf<uint16_t>("d2.qu", n);
-  uint16_t* qv = ctx.buf<uint16_t>("d2.qv", n);
   {
     uint32_t* w = stage_bits(h_eb, "d2.w.eb");
     DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
@@ -844,10 +970,8 @@ void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
   }
   ctx.mark("huffman");
   // ---- unpredictable values to per-element raw arrays (block-major order)
-  float* raw = ctx.buf<float>("d2.raw", 2 * n);
   uint32_t* zc = ctx.buf<uint32_t>("d2.zc", 2 * nb);
   uint64_t* zoff = ctx.buf<uint64_t>("d2.zoff", 2 * nb + 4);
-  int* err = ctx.buf<int>("d2.err", 2);
   CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
   for (int c = 0; c < 2; c++) {
     float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", comp[c].unp_count + 1);
@@ -861,24 +985,6 @@ void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
                                                            raw + c * n, err);
     CPSZ_HIP(hipGetLastError());
   }
-  ctx.mark("unpred");
-  // ---- wavefront reconstruction
-  const int nstrips = (int)div_up(r1, 64);
-  Strip2d sa{};
-  sa.c[0] = sid_u;
-  sa.c[1] = sid_v;
-  sa.q[0] = qu;
-  sa.q[1] = qv;
-  sa.raw[0] = raw;
-  sa.raw[1] = raw + n;
-  sa.out[0] = dU;
-  sa.out[1] = dV;
-  sa.bnd = ctx.buf<int>("d2.bnd", (size_t)2 * nstrips * r2);
-  sa.prog = ctx.buf<int>("d2.prog", 2 * nstrips + 1);
-  sa.ticket = sa.prog + 2 * nstrips;
-  sa.r1 = (int)r1;
-  sa.r2 = (int)r2;
-  sa.nstrips = nstrips;
   // ids above 81 need the double-precision reconstruction path
   uint32_t max_id = 0;
   {
@@ -888,12 +994,61 @@ void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
     for (size_t i = 0; i < t.node_count; i++)
       if ((t.leaf[i] || (i == 0 && t.root_leaf)) && t.sym[i] > max_id) max_id = t.sym[i];
   }
-  launch_strips(ctx, sa, false, max_id > 81);
-  ctx.mark("lorenzo");
-  int herr[2];
-  CPSZ_HIP(hipMemcpyAsync(herr, err, 8, hipMemcpyDeviceToHost, st));
+  w.wide = max_id > 81;
+  ctx.mark("unpred");
+}
+
+void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap) {
+  std::vector<Strip2d> f(nf);
+  for (size_t k = 0; k < nf; k++) {
+    Strip2d& sa = f[k];
+    sa = Strip2d{};
+    const size_t n = w[k]->r1 * w[k]->r2;
+    sa.c[0] = w[k]->sid_u;
+    sa.c[1] = w[k]->sid_v;
+    sa.q[0] = w[k]->qu;
+    sa.q[1] = w[k]->qv;
+    sa.raw[0] = w[k]->raw;
+    sa.raw[1] = w[k]->raw + n;
+    sa.out[0] = w[k]->dU;
+    sa.out[1] = w[k]->dV;
+    sa.nstrips = (int)div_up(w[k]->r1, 64);
+    sa.bnd = w[k]->bnd;
+    sa.prog = w[k]->prog;
+    sa.ticket = w[k]->prog + 2 * sa.nstrips;
+    sa.r1 = (int)w[k]->r1;
+    sa.r2 = (int)w[k]->r2;
+    sa.wide = w[k]->wide;
+  }
+  launch_wavefront<false>(ctx, f, grid_cap);
+}
+
+void d2_check(Ctx& ctx, D2Work* const* w, size_t nf) {
+  hipStream_t st = ctx.stream();
+  int* he = (int*)ctx.host_buf("d2.herr", std::max<size_t>(nf, 1) * 16);
+  for (size_t k = 0; k < nf; k++) CPSZ_HIP(hipMemcpyAsync(he + 4 * k, w[k]->err, 8, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
-  CPSZ_REQUIRE(herr[0] == 0, kErrStream, "decompress_2d: more unpredictable codes than values");
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(he[4 * k] == 0, kErrStream, "decompress_2d: more unpredictable codes than values");
+}
+
+void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
+                          size_t r1, size_t r2, float* dU, float* dV) {
+  CPSZ_REQUIRE(d_in && dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "decompress_2d: bad arguments");
+  ctx.stages.clear();
+  ctx.mark("start");
+  D2Work w;
+  w.r1 = r1;
+  w.r2 = r2;
+  w.dU = dU;
+  w.dV = dV;
+  d2_bind(w, ctx.buf<uint8_t>("d2.scratch", d2_scratch_bytes(r1, r2)));
+  CPSZ_HIP(hipMemsetAsync(w.err, 0, 16, ctx.stream()));
+  d2_entropy(ctx, d_in, h_in, in_size, w);
+  D2Work* wp = &w;
+  d2_wavefront(ctx, &wp, 1, 4096);
+  ctx.mark("lorenzo");
+  d2_check(ctx, &wp, 1);
   ctx.finish_marks();
 }
 

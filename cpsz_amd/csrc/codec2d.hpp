@@ -22,6 +22,42 @@ struct EbOverride {
 // U, V: device pointers to r1 x r2 row-major float fields.
 void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
                         double max_pwr_eb, DeviceStream& out, const EbOverride& ov = EbOverride());
+// ---- staged form: the single-field calls run the stages in order; the batch
+// driver runs eb per field, one wavefront launch for all fields, and the
+// entropy stage of different fields concurrently on worker streams.
+struct C2Work {
+  const float* dU = nullptr;
+  const float* dV = nullptr;
+  size_t r1 = 0, r2 = 0;
+  uint16_t *cu = nullptr, *cv = nullptr;  // eb codes (0 = zero precision, else id + 1)
+  uint16_t *qu = nullptr, *qv = nullptr;  // quantization codes, row-major
+  uint32_t* hist = nullptr;               // kMaxEbCode + 2 * 65536 bins
+  int* flags = nullptr;                   // [0] non-finite, [1] ids above 81
+  int *bnd = nullptr, *prog = nullptr;    // wavefront boundary rows / progress counters
+  bool wide = false;
+};
+size_t c2_scratch_bytes(size_t r1, size_t r2);
+void c2_bind(C2Work& w, uint8_t* scratch);  // carves c2_scratch_bytes of device memory
+void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov);
+void c2_check(Ctx& ctx, C2Work* const* w, size_t nf);  // synchronizes; sets wide
+void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap);
+void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out);
+
+struct D2Work {
+  size_t r1 = 0, r2 = 0;
+  float *dU = nullptr, *dV = nullptr;         // outputs
+  uint16_t *sid_u = nullptr, *sid_v = nullptr;  // stream ids
+  uint16_t *qu = nullptr, *qv = nullptr;      // quantization codes
+  float* raw = nullptr;                       // unpredictable values per element (2n)
+  int *bnd = nullptr, *prog = nullptr, *err = nullptr;
+  bool wide = false;
+};
+size_t d2_scratch_bytes(size_t r1, size_t r2);
+void d2_bind(D2Work& w, uint8_t* scratch);
+void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, D2Work& w);
+void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap);
+void d2_check(Ctx& ctx, D2Work* const* w, size_t nf);
+
 // Analytic per-vertex bound of the plain method (before quantization).
 void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
                   double* dE);

@@ -48,6 +48,48 @@ def field_2d(r1: int, r2: int, seed: int = 0, n_vortex: int = 64, n_modes: int =
     return u.astype(np.float32), v.astype(np.float32)
 
 
+def field_2d_device(r1: int, r2: int, seed: int = 0, device="cuda", n_vortex: int = 64, n_modes: int = 8):
+    """field_2d's recipe evaluated on the GPU with torch in float64 (parameters
+    drawn on the host from the same PCG64 stream, noise from a seeded torch
+    generator), for batches too large to generate on the host. Returns two
+    float32 device tensors. Deterministic per (shape, seed, device type), not
+    byte-identical to field_2d (different noise generator)."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y = torch.arange(r1, dtype=torch.float64, device=device)[:, None]
+    x = torch.arange(r2, dtype=torch.float64, device=device)[None, :]
+    u = torch.zeros((r1, r2), dtype=torch.float64, device=device)
+    v = torch.zeros_like(u)
+    for _ in range(n_vortex):
+        cx, cy = rng.uniform(0, r2), rng.uniform(0, r1)
+        R, G = rng.uniform(20, 200), rng.uniform(-1, 1)
+        dx, dy = x - cx, y - cy
+        e = G * torch.exp(-(dx * dx + dy * dy) / (2 * R * R)) / (R * R)
+        u -= dy * e
+        v += dx * e
+    for comp in (u, v):
+        for _ in range(n_modes):
+            while True:
+                kx, ky = rng.integers(0, 5, size=2)
+                if kx or ky:
+                    break
+            ph = rng.uniform(0, 2 * math.pi)
+            comp += 0.1 * torch.cos(2 * math.pi * (float(kx) * x / r2 + float(ky) * y / r1) + ph)
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    u += 1e-3 * torch.randn((r1, r2), dtype=torch.float64, device=device, generator=g)
+    v += 1e-3 * torch.randn((r1, r2), dtype=torch.float64, device=device, generator=g)
+    return u.float().contiguous(), v.float().contiguous()
+
+
+def batch_2d_shapes(count: int = 96, seed: int = 0, lo: int = 256, hi: int = 4096):
+    """Heights and widths drawn log-uniform in [lo, hi] (the batched 2D workload)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    hw = np.round(np.exp(rng.uniform(math.log(lo), math.log(hi), size=(count, 2)))).astype(int)
+    return [(int(h), int(w)) for h, w in hw]
+
+
 def field_3d(r1: int, r2: int, r3: int, seed: int = 0, n_tubes: int = 32, step: int = 0):
     """3D u,v,w (r1 slowest .. r3 fastest): ABC flow + vortex tubes + noise.
 

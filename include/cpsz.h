@@ -150,6 +150,27 @@ int cpsz_lorenzo3d_emulate(const float* L, const uint8_t* code, size_t r1, size_
 /* eb code of each bound as the device derives it (255 or the 3D id). */
 int cpsz_eb_code_3d(const double* eb, size_t n, uint8_t* code);
 
+/* ---- batched 2D codec ----------------------------------------------------
+ * Many fields of uneven sizes (no counterpart in the reference, whose drivers
+ * compress one field per call). Each field's stream is byte-identical to
+ * cpsz_compress_2d_f32_dev's. One persistent wavefront launch covers every
+ * field; the entropy stages of different fields run concurrently on worker
+ * streams. tune != 0 first times the candidate configurations on this batch
+ * (outputs do not depend on them) and caches the fastest for batches of the
+ * same signature. Streams are returned in one library-owned device buffer
+ * (valid until the next batch call of this thread) at offsets[k], sizes[k]. */
+int cpsz_compress_2d_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                               const float* const* dV, const size_t* r1, const size_t* r2,
+                               double max_pwr_eb, int tune, const unsigned char** d_base,
+                               size_t* offsets, size_t* sizes);
+int cpsz_decompress_2d_batch_dev(int device, void* stream, size_t nfields,
+                                 const unsigned char* const* d_in, const size_t* in_size,
+                                 const size_t* r1, const size_t* r2, float* const* dU,
+                                 float* const* dV);
+/* Configuration the batch calls use for this batch signature. */
+int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const size_t* r2,
+                         int* workers, int* grid_cap);
+
 /* ---- learned error-bound proposer (2D) -----------------------------------
  * No counterpart in the reference (its closest precedent is the
  * verify-and-halve loop of sz_compress_cp_preserve_2d_online_abs_relax_FN,
