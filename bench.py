@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=["3d", "2d", "batch2d"], default="3d")
+    p.add_argument("--workload", choices=["3d", "2d", "batch2d", "learned2d"], default="3d")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-cp", action="store_true", help="skip the critical-point check")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -196,6 +196,70 @@ def run_batch2d(a, torch):
     print(json.dumps(out))
 
 
+def run_learned2d(a, torch):
+    """Learned eb proposer on the 2400x3600 field: trained on the GPU on 64 seeded
+    1024x1024 fields (bisection labels), then learned compress (with its exact
+    verification rounds) + decompress per step; ratio learned vs plain."""
+    import cpsz_amd as cz
+    from cpsz_amd.synth import field_2d, field_2d_device
+
+    eb = 0.1
+    t0 = time.perf_counter()
+    train = [field_2d_device(1024, 1024, seed=5000 + k) for k in range(64)]
+    params, losses = cz.learn_train_2d_dev([(u.data_ptr(), v.data_ptr(), 1024, 1024) for u, v in train], eb,
+                                           cz.learn_init_params(1), samples_per_field=16384, epochs=8,
+                                           batch=4096, lr=2e-3, seed=11)
+    train_s = time.perf_counter() - t0
+    del train
+    U, V = field_2d(2400, 3600, seed=0)
+    du, dv = torch.from_numpy(U).cuda(), torch.from_numpy(V).cuda()
+    ou, ov = torch.empty_like(du), torch.empty_like(dv)
+    stream = torch.cuda.current_stream()
+    sh = stream.cuda_stream
+    sbuf = torch.empty(64 << 20, dtype=torch.uint8, device="cuda")
+    hip = C.CDLL("libamdhip64.so")
+
+    def step():
+        p, n, it, boosted, fb = cz.compress_2d_learned_dev(du.data_ptr(), dv.data_ptr(), 2400, 3600, eb, params,
+                                                           stream=sh)
+        hip.hipMemcpyAsync(C.c_void_p(sbuf.data_ptr()), C.c_void_p(p), C.c_size_t(n), 3, C.c_void_p(sh))
+        cz.decompress_2d_dev(sbuf.data_ptr(), n, 2400, 3600, ou.data_ptr(), ov.data_ptr(), 0, sh)
+        return n, it, boosted, fb
+
+    for _ in range(max(a.warmup, 1)):
+        step()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(a.steps):
+        n, it, boosted, fb = step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    t_step = ev0.elapsed_time(ev1) / 1e3 / a.steps
+    field_bytes = U.nbytes + V.nbytes
+    learned = np.empty(n, np.uint8)
+    hip.hipMemcpy(C.c_void_p(learned.ctypes.data), C.c_void_p(sbuf.data_ptr()), C.c_size_t(n), 2)
+    plain = cz.sz_compress_cp_preserve_2d_offline(U, V, eb)
+    zl, zp = cz.sz_lossless_compress(learned, 3).size, cz.sz_lossless_compress(plain, 3).size
+    c0 = cz.critical_points_2d_dev(du.data_ptr(), dv.data_ptr(), 2400, 3600)
+    c1 = cz.critical_points_2d_dev(ou.data_ptr(), ov.data_ptr(), 2400, 3600)
+    cp_err = {"cp_total": len(c0), "fp": sum(1 for k in c1 if k not in c0), "fn": sum(1 for k in c0 if k not in c1),
+              "ft": sum(1 for k in c0 if k in c1 and c0[k] != c1[k])}
+    out = {
+        "metric": "compress+decompress GB/s and ratio at 0 CP errors: plain vs learned-eb, 1 GPU",
+        "value": field_bytes / t_step / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 codec, bf16 MLP (fp32 accumulate)", "data": "synthetic",
+        "config": {"workload": "learned eb 2D: u,v float32 2400x3600, rel eb 0.1; MLP 51-128-128-1 trained on "
+                               "64 seeded 1024x1024 fields", "method": "learned",
+                   "ratio_learned": field_bytes / zl, "ratio_plain": field_bytes / zp, "cp_errors": cp_err,
+                   "verification_rounds": it, "boosted_vertices": boosted, "fallback_vertices": fb,
+                   "train_seconds": train_s, "train_loss_per_epoch": [float(x) for x in losses]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -215,6 +279,10 @@ def main():
     if a.workload == "batch2d":
         torch.cuda.set_device(0)
         run_batch2d(a, torch)
+        return
+    if a.workload == "learned2d":
+        torch.cuda.set_device(0)
+        run_learned2d(a, torch)
         return
     import cpsz_amd as cz
     from cpsz_amd.synth import field_2d, field_3d
