@@ -44,6 +44,10 @@ _SIGS = {
                                    _P],
     "cpsz_decompress_2d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P],
     "cpsz_batch_2d_config": [C.c_int, _SZ, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "cpsz_compress_3d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int,
+                                   C.POINTER(_P), _P, _P],
+    "cpsz_decompress_3d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P],
+    "cpsz_batch_3d_config": [C.c_int, _SZ, _P, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "cpsz_learn_init_params": [C.c_uint64, _P],
     "cpsz_learn_analytic_eb_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, _P],
     "cpsz_learn_features_2d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P],
@@ -58,6 +62,18 @@ _SIGS = {
                                          C.POINTER(_SZ), C.POINTER(C.c_int), C.POINTER(_SZ),
                                          C.POINTER(_SZ)],
     "cpsz_compress_2d_learned_f32": [_P, _P, _SZ, _SZ, C.c_double, _P, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_learn_init_params_3d": [C.c_uint64, _P],
+    "cpsz_learn_analytic_eb_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, _P],
+    "cpsz_learn_features_3d_dev": [C.c_int, _P, _P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, _P, _SZ, _P],
+    "cpsz_learn_labels_3d_dev": [C.c_int, _P, _P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, _P, _SZ, _P],
+    "cpsz_mlp_forward_3d_dev": [C.c_int, _P, _P, _P, _SZ, _P],
+    "cpsz_mlp_grad_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _P, _P],
+    "cpsz_learn_train_3d_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, _SZ, C.c_int, C.c_int,
+                                C.c_float, C.c_uint64, _P, _P],
+    "cpsz_compress_3d_learned_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, _P, C.c_int,
+                                           C.POINTER(_P), _P, _P, C.POINTER(C.c_int), C.POINTER(_SZ),
+                                           C.POINTER(_SZ)],
+    "cpsz_tet_cp_emulate": [_P, C.c_int, C.POINTER(C.c_int)],
 }
 
 
@@ -88,6 +104,8 @@ def lib():
         L.cpsz_free.restype = None
         L.cpsz_learn_param_count.restype = _SZ
         L.cpsz_learn_param_count.argtypes = []
+        L.cpsz_learn_param_count_3d.restype = _SZ
+        L.cpsz_learn_param_count_3d.argtypes = []
         _lib = L
     return _lib
 
@@ -365,3 +383,114 @@ def batch_2d_config(shapes, device: int = 0):
     _check(lib().cpsz_batch_2d_config(device, n, _arr(C.c_size_t, [s[0] for s in shapes]),
                                       _arr(C.c_size_t, [s[1] for s in shapes]), C.byref(w), C.byref(g)))
     return w.value, g.value
+
+
+# --------------------------------------------------------------- 3D batches --
+def compress_3d_batch_dev(fields, eb: float, tune: bool = False, device: int = 0, stream: int = 0):
+    """fields: list of (dU, dV, dW, r1, r2, r3) device pointers. Returns (base pointer of the
+    library-owned stream buffer, offsets, sizes)."""
+    n = len(fields)
+    base = _P()
+    off = np.zeros(n, np.uint64)
+    size = np.zeros(n, np.uint64)
+    _check(lib().cpsz_compress_3d_batch_dev(device, stream or None, n,
+                                            *[_arr(C.c_void_p, [f[k] for f in fields]) for k in range(3)],
+                                            *[_arr(C.c_size_t, [f[k] for f in fields]) for k in range(3, 6)],
+                                            eb, int(tune), C.byref(base), _ptr(off), _ptr(size)))
+    return base.value, off.astype(np.int64), size.astype(np.int64)
+
+
+def decompress_3d_batch_dev(streams, shapes, outs, device: int = 0, stream: int = 0):
+    """streams: list of (device pointer, size); shapes: (r1, r2, r3); outs: (dU, dV, dW) pointers."""
+    n = len(streams)
+    _check(lib().cpsz_decompress_3d_batch_dev(device, stream or None, n, _arr(C.c_void_p, [s[0] for s in streams]),
+                                              _arr(C.c_size_t, [s[1] for s in streams]),
+                                              *[_arr(C.c_size_t, [s[k] for s in shapes]) for k in range(3)],
+                                              *[_arr(C.c_void_p, [o[k] for o in outs]) for k in range(3)]))
+
+
+def batch_3d_config(shapes, device: int = 0):
+    """(workers, grid cap) the 3D batch calls use for batches of these shapes."""
+    n = len(shapes)
+    w, g = C.c_int(), C.c_int()
+    _check(lib().cpsz_batch_3d_config(device, n, *[_arr(C.c_size_t, [s[k] for s in shapes]) for k in range(3)],
+                                      C.byref(w), C.byref(g)))
+    return w.value, g.value
+
+
+# ------------------------------------------------------------- learned eb 3D --
+def learn_param_count_3d() -> int:
+    return int(lib().cpsz_learn_param_count_3d())
+
+
+def learn_init_params_3d(seed: int = 0) -> np.ndarray:
+    p = np.zeros(learn_param_count_3d(), np.float32)
+    _check(lib().cpsz_learn_init_params_3d(seed, _ptr(p)))
+    return p
+
+
+def learn_analytic_eb_3d_dev(dU, dV, dW, r1, r2, r3, eb, dE, device=0, stream=0):
+    _check(lib().cpsz_learn_analytic_eb_3d_dev(device, stream or None, dU, dV, dW, r1, r2, r3, eb, dE))
+
+
+def learn_features_3d_dev(dU, dV, dW, dE, r1, r2, r3, eb, d_idx, m, dX, device=0, stream=0):
+    _check(lib().cpsz_learn_features_3d_dev(device, stream or None, dU, dV, dW, dE, r1, r2, r3, eb,
+                                            d_idx or None, m, dX))
+
+
+def learn_labels_3d_dev(dU, dV, dW, dE, r1, r2, r3, eb, d_idx, m, dL, device=0, stream=0):
+    _check(lib().cpsz_learn_labels_3d_dev(device, stream or None, dU, dV, dW, dE, r1, r2, r3, eb,
+                                          d_idx or None, m, dL))
+
+
+def mlp_forward_3d_dev(params: np.ndarray, dX: int, m: int, dY: int, device: int = 0, stream: int = 0):
+    params = np.ascontiguousarray(params, np.float32)
+    _check(lib().cpsz_mlp_forward_3d_dev(device, stream or None, _ptr(params), dX, m, dY))
+
+
+def mlp_grad_3d_dev(params: np.ndarray, dX: int, dT: int, m: int, device: int = 0, stream: int = 0):
+    params = np.ascontiguousarray(params, np.float32)
+    g = np.zeros_like(params)
+    loss = np.zeros(1, np.float32)
+    _check(lib().cpsz_mlp_grad_3d_dev(device, stream or None, _ptr(params), dX, dT, m, _ptr(g), _ptr(loss)))
+    return g, float(loss[0])
+
+
+def learn_train_3d_dev(fields, eb: float, params: np.ndarray, samples_per_field: int = 16384,
+                       epochs: int = 4, batch: int = 4096, lr: float = 1e-3, seed: int = 0,
+                       device: int = 0, stream: int = 0):
+    """fields: list of (dU, dV, dW, r1, r2, r3) with device pointers. Returns (params, loss per epoch)."""
+    n = len(fields)
+    p = np.ascontiguousarray(params, np.float32).copy()
+    losses = np.zeros(epochs, np.float32)
+    _check(lib().cpsz_learn_train_3d_dev(device, stream or None, n,
+                                         *[_arr(C.c_void_p, [f[k] for f in fields]) for k in range(3)],
+                                         *[_arr(C.c_size_t, [f[k] for f in fields]) for k in range(3, 6)],
+                                         eb, samples_per_field, epochs, batch, lr, seed, _ptr(p), _ptr(losses)))
+    return p, losses
+
+
+def compress_3d_learned_batch_dev(fields, eb: float, params: np.ndarray, tune: bool = False, device: int = 0,
+                                  stream: int = 0):
+    """fields: list of (dU, dV, dW, r1, r2, r3) device pointers. Returns (base pointer, offsets, sizes,
+    verification rounds, boosted vertices, fallback vertices)."""
+    n = len(fields)
+    params = np.ascontiguousarray(params, np.float32)
+    base = _P()
+    off = np.zeros(n, np.uint64)
+    size = np.zeros(n, np.uint64)
+    it, bo, fb = C.c_int(), _SZ(), _SZ()
+    _check(lib().cpsz_compress_3d_learned_batch_dev(device, stream or None, n,
+                                                    *[_arr(C.c_void_p, [f[k] for f in fields]) for k in range(3)],
+                                                    *[_arr(C.c_size_t, [f[k] for f in fields]) for k in range(3, 6)],
+                                                    eb, _ptr(params), int(tune), C.byref(base), _ptr(off), _ptr(size),
+                                                    C.byref(it), C.byref(bo), C.byref(fb)))
+    return base.value, off.astype(np.int64), size.astype(np.int64), it.value, bo.value, fb.value
+
+
+def tet_cp_emulate(f, t: int) -> int:
+    """Host evaluation of the device judge's tetrahedron predicate (-1 or the type code)."""
+    f = np.ascontiguousarray(f, np.float32).reshape(12)
+    ty = C.c_int()
+    _check(lib().cpsz_tet_cp_emulate(_ptr(f), t, C.byref(ty)))
+    return ty.value

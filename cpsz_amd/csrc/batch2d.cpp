@@ -15,105 +15,18 @@
 // caches the fastest per device and batch signature.
 #include <algorithm>
 #include <chrono>
-#include <condition_variable>
-#include <deque>
-#include <exception>
 #include <functional>
 #include <map>
-#include <memory>
 #include <mutex>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "batch2d.hpp"
+#include "pool.hpp"
 
 namespace cpsz {
 
 namespace {
-
-// Worker threads with one context (HIP stream + scratch) each; jobs run with
-// at most `limit` workers in flight.
-class Pool {
- public:
-  Pool(int device, int n) : device_(device) {
-    for (int k = 0; k < n; k++) threads_.emplace_back([this] { loop(); });
-  }
-  ~Pool() {
-    {
-      std::lock_guard<std::mutex> g(m_);
-      stop_ = true;
-    }
-    cv_.notify_all();
-    for (auto& t : threads_) t.join();
-  }
-  int size() const { return (int)threads_.size(); }
-  // Runs every job; rethrows the first failure after all have finished.
-  void run(std::vector<std::function<void(Ctx&)>>& jobs, int limit) {
-    std::unique_lock<std::mutex> g(m_);
-    limit_ = std::max(1, std::min(limit, size()));
-    for (auto& j : jobs) q_.push_back(&j);
-    pending_ = jobs.size();
-    first_error_ = nullptr;
-    cv_.notify_all();
-    done_.wait(g, [&] { return pending_ == 0; });
-    if (first_error_) std::rethrow_exception(first_error_);
-  }
-
- private:
-  void loop() {
-    CPSZ_HIP(hipSetDevice(device_));
-    std::unique_ptr<Ctx> ctx;
-    for (;;) {
-      std::function<void(Ctx&)>* job = nullptr;
-      {
-        std::unique_lock<std::mutex> g(m_);
-        cv_.wait(g, [&] { return stop_ || (!q_.empty() && active_ < limit_); });
-        if (stop_) return;
-        job = q_.front();
-        q_.pop_front();
-        active_++;
-      }
-      std::exception_ptr err;
-      try {
-        if (!ctx) ctx.reset(new Ctx(device_));
-        (*job)(*ctx);
-      } catch (...) {
-        err = std::current_exception();
-      }
-      {
-        std::lock_guard<std::mutex> g(m_);
-        active_--;
-        if (err && !first_error_) first_error_ = err;
-        if (--pending_ == 0) done_.notify_all();
-      }
-      cv_.notify_all();
-    }
-  }
-
-  int device_;
-  std::vector<std::thread> threads_;
-  std::mutex m_;
-  std::condition_variable cv_, done_;
-  std::deque<std::function<void(Ctx&)>*> q_;
-  size_t pending_ = 0;
-  int active_ = 0, limit_ = 1;
-  bool stop_ = false;
-  std::exception_ptr first_error_;
-};
-
-constexpr int kMaxWorkers = 16;
-
-// Pools live until the process ends and are never destroyed: their threads
-// and contexts must not be torn down after the HIP runtime at exit.
-Pool& pool_for(int device) {
-  static std::mutex m;
-  static std::map<int, Pool*>* pools = new std::map<int, Pool*>();
-  std::lock_guard<std::mutex> g(m);
-  Pool*& p = (*pools)[device];
-  if (!p) p = new Pool(device, kMaxWorkers);
-  return *p;
-}
 
 std::string signature(const std::vector<Field2d>& f) {
   size_t n = 0, rmax = 0;
@@ -181,7 +94,7 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
       out.size[k] = s.size;
     };
   }
-  pool_for(ctx.device()).run(jobs, cfg.workers);
+  pool_run(pool_for(ctx.device()), jobs, cfg.workers);
   ctx.mark("entropy");
   out.base = dout;
   ctx.finish_marks();
@@ -219,7 +132,7 @@ void decompress_2d_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, cons
       CPSZ_HIP(hipStreamSynchronize(wc.stream()));
     };
   }
-  pool_for(ctx.device()).run(jobs, cfg.workers);
+  pool_run(pool_for(ctx.device()), jobs, cfg.workers);
   ctx.mark("entropy");
   d2_wavefront(ctx, wp.data(), nf, cfg.grid_cap);
   d2_check(ctx, wp.data(), nf);

@@ -9,9 +9,11 @@
 
 #include "../../include/cpsz.h"
 #include "batch2d.hpp"
+#include "batch3d.hpp"
 #include "codec2d.hpp"
 #include "codec3d.hpp"
 #include "cp.hpp"
+#include "cp_math.hpp"
 #include "ctx.hpp"
 #include "eb3d_math.hpp"
 #include "huff_host.hpp"
@@ -456,6 +458,63 @@ int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const siz
   });
 }
 
+int cpsz_compress_3d_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                               const float* const* dV, const float* const* dW, const size_t* r1,
+                               const size_t* r2, const size_t* r3, double max_pwr_eb, int tune,
+                               const unsigned char** d_base, size_t* offsets, size_t* sizes) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && dU && dV && dW && r1 && r2 && r3 && d_base && offsets && sizes, kErrArg,
+                 "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<Field3d> f(nfields);
+    for (size_t k = 0; k < nfields; k++) f[k] = Field3d{dU[k], dV[k], dW[k], r1[k], r2[k], r3[k]};
+    const Batch2dConfig cfg = batch_3d_config(c, f, max_pwr_eb, tune != 0);
+    BatchOut o;
+    compress_3d_batch(c, f, max_pwr_eb, cfg, o);
+    *d_base = o.base;
+    for (size_t k = 0; k < nfields; k++) {
+      offsets[k] = o.off[k];
+      sizes[k] = o.size[k];
+    }
+  });
+}
+
+int cpsz_decompress_3d_batch_dev(int device, void* stream, size_t nfields,
+                                 const unsigned char* const* d_in, const size_t* in_size,
+                                 const size_t* r1, const size_t* r2, const size_t* r3, float* const* dU,
+                                 float* const* dV, float* const* dW) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && d_in && in_size && r1 && r2 && r3 && dU && dV && dW, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<Field3d> sig(nfields);
+    std::vector<Field3dOut> f(nfields);
+    std::vector<const uint8_t*> in(nfields);
+    std::vector<size_t> sz(nfields);
+    for (size_t k = 0; k < nfields; k++) {
+      sig[k] = Field3d{nullptr, nullptr, nullptr, r1[k], r2[k], r3[k]};
+      f[k] = Field3dOut{dU[k], dV[k], dW[k], r1[k], r2[k], r3[k]};
+      in[k] = d_in[k];
+      sz[k] = in_size[k];
+    }
+    decompress_3d_batch(c, in, sz, f, batch_3d_config(c, sig, 0.0, false));
+  });
+}
+
+int cpsz_batch_3d_config(int device, size_t nfields, const size_t* r1, const size_t* r2, const size_t* r3,
+                         int* workers, int* grid_cap) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && r1 && r2 && r3 && workers && grid_cap, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    std::vector<Field3d> sig(nfields);
+    for (size_t k = 0; k < nfields; k++) sig[k] = Field3d{nullptr, nullptr, nullptr, r1[k], r2[k], r3[k]};
+    const Batch2dConfig cfg = batch_3d_config(c, sig, 0.0, false);
+    *workers = cfg.workers;
+    *grid_cap = cfg.grid_cap;
+  });
+}
+
 size_t cpsz_learn_param_count(void) { return kNumParams; }
 
 int cpsz_learn_init_params(uint64_t seed, float* params) {
@@ -587,6 +646,131 @@ int cpsz_compress_2d_learned_f32(const float* U, const float* V, size_t r1, size
     CPSZ_HIP(hipStreamSynchronize(c.stream()));
     *out = h;
     *out_size = s.size;
+  });
+}
+
+// ---- learned proposer, 3D
+size_t cpsz_learn_param_count_3d(void) { return kNumParams3; }
+
+int cpsz_learn_init_params_3d(uint64_t seed, float* params) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params, kErrArg, "null params");
+    mlp_init_params(seed, params, kFeatDim3, kFeatUsed3);
+  });
+}
+
+int cpsz_learn_analytic_eb_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                                  size_t r1, size_t r2, size_t r3, double max_pwr_eb, double* dE) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    analytic_eb_3d(c, dU, dV, dW, r1, r2, r3, max_pwr_eb, dE);
+  });
+}
+
+int cpsz_learn_features_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                               const double* dE, size_t r1, size_t r2, size_t r3, double max_pwr_eb,
+                               const uint32_t* d_idx, size_t m, uint16_t* dX) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    features_3d(c, dU, dV, dW, dE, r1, r2, r3, max_pwr_eb, d_idx, m, dX);
+  });
+}
+
+int cpsz_learn_labels_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                             const double* dE, size_t r1, size_t r2, size_t r3, double max_pwr_eb,
+                             const uint32_t* d_idx, size_t m, uint8_t* dL) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    labels_3d(c, dU, dV, dW, dE, r1, r2, r3, max_pwr_eb, d_idx, m, dL);
+  });
+}
+
+int cpsz_mlp_forward_3d_dev(int device, void* stream, const float* params, const uint16_t* dX, size_t m,
+                            float* dY) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params && dX && dY, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    float* P = c.buf<float>("api.p3", kNumParams3);
+    CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams3 * 4, hipMemcpyHostToDevice, c.stream()));
+    mlp_forward(c, P, dX, m, dY, kFeatDim3);
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_mlp_grad_3d_dev(int device, void* stream, const float* params, const uint16_t* dX, const float* dT,
+                         size_t m, float* grad, float* loss) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params && dX && dT && grad && loss, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    float* P = c.buf<float>("api.p3", kNumParams3);
+    float* G = c.buf<float>("api.g3", kNumParams3 + 1);
+    CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams3 * 4, hipMemcpyHostToDevice, c.stream()));
+    mlp_grad(c, P, dX, dT, m, G, G + kNumParams3, kFeatDim3);
+    CPSZ_HIP(hipMemcpyAsync(grad, G, kNumParams3 * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(loss, G + kNumParams3, 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_learn_train_3d_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                            const float* const* dV, const float* const* dW, const size_t* r1, const size_t* r2,
+                            const size_t* r3, double max_pwr_eb, size_t samples_per_field, int epochs, int batch,
+                            float lr, uint64_t seed, float* params, float* loss_per_epoch) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && dU && dV && dW && r1 && r2 && r3 && params, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<TrainField3> f(nfields);
+    for (size_t k = 0; k < nfields; k++) f[k] = TrainField3{dU[k], dV[k], dW[k], r1[k], r2[k], r3[k]};
+    train_3d(c, f, max_pwr_eb, samples_per_field, epochs, batch, lr, seed, params, loss_per_epoch);
+  });
+}
+
+int cpsz_compress_3d_learned_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                                       const float* const* dV, const float* const* dW, const size_t* r1,
+                                       const size_t* r2, const size_t* r3, double max_pwr_eb,
+                                       const float* params, int tune, const unsigned char** d_base,
+                                       size_t* offsets, size_t* sizes, int* iterations, size_t* boosted,
+                                       size_t* fallback) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && dU && dV && dW && r1 && r2 && r3 && params && d_base && offsets && sizes, kErrArg,
+                 "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<TrainField3> f(nfields);
+    std::vector<Field3d> sig(nfields);
+    for (size_t k = 0; k < nfields; k++) {
+      f[k] = TrainField3{dU[k], dV[k], dW[k], r1[k], r2[k], r3[k]};
+      sig[k] = Field3d{dU[k], dV[k], dW[k], r1[k], r2[k], r3[k]};
+    }
+    const Batch2dConfig cfg = batch_3d_config(c, sig, max_pwr_eb, tune != 0);
+    const uint8_t* base = nullptr;
+    std::vector<size_t> off, size;
+    LearnedStats st;
+    compress_3d_learned_batch(c, f, max_pwr_eb, params, cfg.workers, cfg.grid_cap, base, off, size, st);
+    *d_base = base;
+    for (size_t k = 0; k < nfields; k++) {
+      offsets[k] = off[k];
+      sizes[k] = size[k];
+    }
+    if (iterations) *iterations = st.iterations;
+    if (boosted) *boosted = st.boosted_vertices;
+    if (fallback) *fallback = st.fallback_vertices;
+  });
+}
+
+int cpsz_tet_cp_emulate(const float* f, int t, int* type) {
+  return guarded([&] {
+    CPSZ_REQUIRE(f && type && t >= 0 && t < 6, kErrArg, "bad argument");
+    float g[4][3];
+    for (int p = 0; p < 4; p++)
+      for (int q = 0; q < 3; q++) g[p][q] = f[3 * p + q];
+    *type = tet_cp(g, judge_tet_a(t), judge_tet_b(t));
   });
 }
 

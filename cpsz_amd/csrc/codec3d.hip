@@ -109,7 +109,9 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
                                               const float* __restrict__ W, int r1, int r2, int r3,
                                               double max_pwr_eb, const double* __restrict__ q3d, int nq,
-                                              Skew3 sk, uint8_t* __restrict__ code, int* __restrict__ flags) {
+                                              Skew3 sk, uint8_t* __restrict__ code, int* __restrict__ flags,
+                                              const uint8_t* __restrict__ gain, const uint8_t* __restrict__ fb,
+                                              double* __restrict__ eb_out) {
   constexpr int VI = kTI + 2, VJ = kTJ + 2, VK = kTK + 2;  // vertex halo box
   __shared__ float sv[3][VI][VJ][VK];
   __shared__ unsigned long long seb[kTI][kTJ][kTK];
@@ -171,7 +173,15 @@ __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const
     const float x0 = sv[0][a + 1][b + 1][c + 1], x1 = sv[1][a + 1][b + 1][c + 1],
                 x2 = sv[2][a + 1][b + 1][c + 1];
     if (!isfinite(x0) || !isfinite(x1) || !isfinite(x2)) bad = 1;
-    const double eb = __longlong_as_double((long long)seb[a][b][c]);
+    double eb = __longlong_as_double((long long)seb[a][b][c]);
+    if (eb_out || gain) {
+      const size_t v = gi * d0 + gj * d1 + gk;
+      if (eb_out) eb_out[v] = eb;
+      if (gain) {
+        const int g = gain[v];
+        if (g && !(fb && fb[v])) eb = fmin(max_pwr_eb, ldexp(eb, g));
+      }
+    }
     code[sk.at(gi, gj, gk)] = (uint8_t)eb_code_3d(eb, q3d, nq);
   }
   if (bad) atomicOr(flags, 1);
@@ -249,6 +259,14 @@ __device__ __forceinline__ float comp4(const float4& v, int s) {
   return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
 }
 
+// global address space: pointers read from memory (the batch table) are
+// generic; accessed through these the loads and stores stay global (not flat)
+// accesses, which LDS waits do not wait for
+#define GLB __attribute__((address_space(1)))
+// constant address space: the batch table is not written while the kernel
+// runs, so uniform reads of it become scalar loads
+#define CST __attribute__((address_space(4)))
+
 struct Wave3d {
   const float* lx;       // compress: log values [3][per]; decompress: initial values of unpredictables
   const uint8_t* code;   // compress: eb codes; decompress: stream ids
@@ -257,7 +275,6 @@ struct Wave3d {
   uint16_t* qout;        // compress: quant codes [3][per]
   uint8_t* vid;          // compress: stream id per vertex
   int* prog;             // [r1][nb] completed steps
-  int* ticket;
   Skew3 s;               // all arrays above are in wave layout
 };
 
@@ -272,21 +289,73 @@ struct Chunk3 {
   float bl[3], bu[3]; // lane s: row j0-1 at step t0 + s (own plane, plane above)
 };
 
+// Fields of one launch: a single field travels in the kernel arguments (one),
+// several through a device table f[0..nf) with start[k] = first item of field
+// k. Items are numbered field by field and taken from one ticket counter, so
+// every wait (on an earlier plane group or block of the same field) targets an
+// earlier ticket, whose workgroup is already resident.
+struct WaveBatch3d {
+  Wave3d one;
+  const Wave3d* f;  // null: the single field `one`
+  const int* start;
+  int nf, items;
+  int* ticket;
+};
+
+// items of one field: (plane group, j block) pairs
+__host__ __device__ inline int wave3d_items(const Skew3& s) { return (s.r1 + kP3 - 1) / kP3 * s.nb; }
+
 template <bool COMPRESS>
-__global__ void __launch_bounds__(64 * kP3) k_wave3d(Wave3d a) {
+__global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
   __shared__ float4 hand[kP3 - 1][2][kG3][3][64];
   __shared__ int s_item;
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const Skew3 sk = a.s;
-  const int r1 = sk.r1, r2 = sk.r2, r3 = sk.r3, nb = sk.nb, T = sk.T;
-  const size_t per = sk.per;
-  const int ngroups = (r1 + kP3 - 1) / kP3, items = ngroups * nb, nch = T / kC3;
   for (;;) {
-    if (threadIdx.x == 0) s_item = atomicAdd(a.ticket, 1);
+    if (threadIdx.x == 0) s_item = atomicAdd(bt.ticket, 1);
     __syncthreads();
-    const int item = s_item;
+    const int gitem = __builtin_amdgcn_readfirstlane(s_item);  // uniform: the fields' pointers stay scalar
     __syncthreads();
-    if (item >= items) return;
+    if (gitem >= bt.items) return;
+    const CST Wave3d* tab = (const CST Wave3d*)bt.f;
+    const CST int* start = (const CST int*)bt.start;
+    int fi = 0;
+    if (bt.f) {
+      int hi = bt.nf - 1;
+      while (fi < hi) {
+        const int mid = (fi + hi + 1) >> 1;
+        if (start[mid] <= gitem) fi = mid;
+        else hi = mid - 1;
+      }
+    }
+    Wave3d a = bt.one;
+    if (bt.f) {
+      const CST Wave3d* e = tab + fi;
+      a.lx = e->lx;
+      a.code = e->code;
+      a.qin = e->qin;
+      a.R = e->R;
+      a.qout = e->qout;
+      a.vid = e->vid;
+      a.prog = e->prog;
+      a.s.r1 = e->s.r1;
+      a.s.r2 = e->s.r2;
+      a.s.r3 = e->s.r3;
+      a.s.nb = e->s.nb;
+      a.s.T = e->s.T;
+      a.s.per = e->s.per;
+    }
+    const GLB float* lx = (const GLB float*)a.lx;
+    const GLB uint8_t* code = (const GLB uint8_t*)a.code;
+    const GLB uint16_t* qin = (const GLB uint16_t*)a.qin;
+    GLB float* R = (GLB float*)a.R;
+    GLB uint16_t* qout = (GLB uint16_t*)a.qout;
+    GLB uint8_t* vid = (GLB uint8_t*)a.vid;
+    GLB int* prog = (GLB int*)a.prog;
+    const int item = bt.f ? gitem - start[fi] : gitem;
+    const Skew3 sk = a.s;
+    const int r1 = sk.r1, r3 = sk.r3, nb = sk.nb, T = sk.T;
+    const size_t per = sk.per;
+    const int nch = T / kC3;
     const int g = item / nb, b = item - g * nb;
     const int i = g * kP3 + w;
     const bool pv = i < r1;
@@ -294,9 +363,9 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(Wave3d a) {
     const size_t ubase = (pv && i > 0) ? sk.blk(i - 1, b) : 0;
     const size_t lbase = (pv && b > 0) ? sk.blk(i, b - 1) : 0;
     const size_t lubase = (pv && b > 0 && i > 0) ? sk.blk(i - 1, b - 1) : 0;
-    const int* prog_up = a.prog + (pv && i > 0 ? (size_t)(i - 1) * nb + b : 0);
-    const int* prog_left = a.prog + (pv && b > 0 ? (size_t)i * nb + b - 1 : 0);
-    int* prog_me = a.prog + (pv ? (size_t)i * nb + b : 0);
+    const int* prog_up = (const int*)(prog + (pv && i > 0 ? (size_t)(i - 1) * nb + b : 0));
+    const int* prog_left = (const int*)(prog + (pv && b > 0 ? (size_t)i * nb + b - 1 : 0));
+    int* prog_me = (int*)(prog + (pv ? (size_t)i * nb + b : 0));
     int seen_up = 0, seen_left = 0;
     float Rp[3] = {0, 0, 0}, Bp[3] = {0, 0, 0}, Dp[3] = {0, 0, 0}, Fp[3] = {0, 0, 0};
 
@@ -320,12 +389,13 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(Wave3d a) {
 #pragma unroll
       for (int u = 0; u < kG3; u++) {
         const size_t o = (size_t)(t0 / 4 + u) * 256 + lane * 4;
-        X.cd[u] = *(const uint32_t*)(a.code + base + o);
+        X.cd[u] = *(const GLB uint32_t*)(code + base + o);
 #pragma unroll
         for (int p = 0; p < 3; p++) {
-          X.x[p][u] = *(const float4*)(a.lx + p * per + base + o);
-          if (!COMPRESS) X.q[p][u] = *(const uint2*)(a.qin + p * per + base + o);
-          if (w == 0) X.up[p][u] = i > 0 ? ld_coh4(a.R + p * per + ubase + o) : make_float4(0.f, 0.f, 0.f, 0.f);
+          X.x[p][u] = *(const float4*)(lx + p * per + base + o);
+          if (!COMPRESS) X.q[p][u] = *(const uint2*)(qin + p * per + base + o);
+          if (w == 0)
+            X.up[p][u] = i > 0 ? ld_coh4((const float*)(R + p * per + ubase + o)) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
       }
 #pragma unroll
@@ -335,8 +405,8 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(Wave3d a) {
         const size_t o = (size_t)(t >> 2) * 256 + 63 * 4 + (t & 3);
 #pragma unroll
         for (int p = 0; p < 3; p++) {
-          X.bl[p] = ld_cohf(a.R + p * per + lbase + o);
-          if (i > 0) X.bu[p] = ld_cohf(a.R + p * per + lubase + o);
+          X.bl[p] = ld_cohf((const float*)(R + p * per + lbase + o));
+          if (i > 0) X.bu[p] = ld_cohf((const float*)(R + p * per + lubase + o));
         }
       }
     };
@@ -404,10 +474,10 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(Wave3d a) {
         for (int p = 0; p < 3; p++) {
           const float4 v = make_float4(rr[p][0], rr[p][1], rr[p][2], rr[p][3]);
           if (w < kP3 - 1) hand[w][c & 1][u][p][lane] = v;
-          if (!COMPRESS || w == kP3 - 1 || lane == 63) st_coh4(a.R + p * per + base + o, v);
-          if (COMPRESS) *(uint2*)(a.qout + p * per + base + o) = make_uint2(qq[p][0], qq[p][1]);
+          if (!COMPRESS || w == kP3 - 1 || lane == 63) st_coh4((float*)(R + p * per + base + o), v);
+          if (COMPRESS) *(uint2*)(qout + p * per + base + o) = make_uint2(qq[p][0], qq[p][1]);
         }
-        if (COMPRESS) *(uint32_t*)(a.vid + base + o) = vv;
+        if (COMPRESS) *(GLB uint32_t*)(vid + base + o) = vv;
       }
     };
 
@@ -636,76 +706,146 @@ void put(std::vector<uint8_t>& o, const V& v) {
   std::memcpy(o.data() + at, &v, sizeof(V));
 }
 
-void launch_wave3d(Ctx& ctx, Wave3d& w, bool compress) {
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// Lorenzo wavefront over fields f[0..nf) in one launch (see WaveBatch3d).
+template <bool COMPRESS>
+void launch_wave3d(Ctx& ctx, const std::vector<Wave3d>& f, int grid_cap) {
   hipStream_t st = ctx.stream();
-  const size_t nz = (size_t)w.s.r1 * w.s.nb + 1;
-  k_zero_i32_3d<<<(int)std::min<size_t>(div_up(nz, 256), 1024), 256, 0, st>>>(w.prog, nz);
-  const int items = (int)div_up(w.s.r1, kP3) * w.s.nb;
-  const int grid = items < 1024 ? items : 1024;
-  if (compress) k_wave3d<true><<<grid, 64 * kP3, 0, st>>>(w);
-  else k_wave3d<false><<<grid, 64 * kP3, 0, st>>>(w);
+  const size_t nf = f.size();
+  CPSZ_REQUIRE(nf > 0 && grid_cap > 0, kErrArg, "3D wavefront: empty launch");
+  std::vector<int> start(nf + 1, 0);
+  for (size_t k = 0; k < nf; k++) {
+    const size_t nz = (size_t)f[k].s.r1 * f[k].s.nb;
+    k_zero_i32_3d<<<(int)std::min<size_t>(div_up(nz, 256), 1024), 256, 0, st>>>(f[k].prog, nz);
+    CPSZ_REQUIRE((size_t)start[k] + wave3d_items(f[k].s) < (1u << 30), kErrUnsupported,
+                 "3D wavefront: too many items in one launch");
+    start[k + 1] = start[k] + wave3d_items(f[k].s);
+  }
+  WaveBatch3d bt{};
+  bt.nf = (int)nf;
+  bt.items = start[nf];
+  if (nf == 1) {
+    bt.one = f[0];
+    bt.ticket = ctx.buf<int>("w3.ticket", 1);
+  } else {
+    const size_t tbytes = nf * sizeof(Wave3d), sbytes = (nf + 1) * sizeof(int);
+    uint8_t* h = (uint8_t*)ctx.host_buf("w3.tab", tbytes + sbytes);
+    std::memcpy(h, f.data(), tbytes);
+    std::memcpy(h + tbytes, start.data(), sbytes);
+    uint8_t* d = ctx.buf<uint8_t>("w3.tab", tbytes + sbytes + 64);
+    CPSZ_HIP(hipMemcpyAsync(d, h, tbytes + sbytes, hipMemcpyHostToDevice, st));
+    bt.f = (const Wave3d*)d;
+    bt.start = (const int*)(d + tbytes);
+    bt.ticket = (int*)(d + ((tbytes + sbytes + 15) & ~(size_t)15));
+  }
+  k_zero_i32_3d<<<1, 64, 0, st>>>(bt.ticket, 1);
+  const int grid = std::min(bt.items, grid_cap);
+  k_wave3d<COMPRESS><<<grid, 64 * kP3, 0, st>>>(bt);
   CPSZ_HIP(hipGetLastError());
+  // the pinned host table is rewritten by the next batch launch only after
+  // this copy has run
+  if (nf > 1) CPSZ_HIP(hipStreamSynchronize(st));
+}
+
+void check_dims_3d(size_t r1, size_t r2, size_t r3, const char* who) {
+  CPSZ_REQUIRE(r1 && r2 && r3, kErrArg, std::string(who) + ": bad arguments");
+  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 32),
+               kErrUnsupported, std::string(who) + ": field too large");
+}
+
+template <typename V>
+V* carve(uint8_t*& p, size_t count) {
+  V* q = (V*)p;
+  p += al256(count * sizeof(V));
+  return q;
 }
 
 }  // namespace
 
-void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1,
-                        size_t r2, size_t r3, double max_pwr_eb, DeviceStream& out) {
-  CPSZ_REQUIRE(dU && dV && dW && r1 && r2 && r3, kErrArg, "compress_3d: bad arguments");
-  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 32),
-               kErrUnsupported, "compress_3d: field too large");
-  hipStream_t st = ctx.stream();
-  const size_t n = r1 * r2 * r3;
+// ------------------------------------------------------------- compress ----
+size_t c3_scratch_bytes(size_t r1, size_t r2, size_t r3) {
   const Skew3 sk = make_skew(r1, r2, r3);
-  ctx.stages.clear();
-  ctx.mark("start");
+  const size_t n = r1 * r2 * r3, nq = div_up(n, 64);
+  return al256(sk.per) + 2 * al256(3 * sk.per * 4) + al256(3 * sk.per * 2) + al256(sk.per) + al256(3 * nq * 8) +
+         al256(r1 * sk.nb * 4) + al256(16);
+}
+
+void c3_bind(C3Work& w, uint8_t* p) {
+  const Skew3 sk = make_skew(w.r1, w.r2, w.r3);
+  const size_t n = w.r1 * w.r2 * w.r3, nq = div_up(n, 64);
+  w.ebc = carve<uint8_t>(p, sk.per);
+  w.L = carve<float>(p, 3 * sk.per);
+  w.R = carve<float>(p, 3 * sk.per);
+  w.q = carve<uint16_t>(p, 3 * sk.per);
+  w.vid = carve<uint8_t>(p, sk.per);
+  w.smap = carve<uint64_t>(p, 3 * nq);
+  w.prog = carve<int>(p, w.r1 * sk.nb);
+  w.flags = carve<int>(p, 4);
+}
+
+void c3_eb(Ctx& ctx, C3Work& w, double max_pwr_eb, const EbOverride& ov) {
+  CPSZ_REQUIRE(w.dU && w.dV && w.dW, kErrArg, "compress_3d: bad arguments");
+  check_dims_3d(w.r1, w.r2, w.r3, "compress_3d");
+  hipStream_t st = ctx.stream();
+  const size_t r1 = w.r1, r2 = w.r2, r3 = w.r3, n = r1 * r2 * r3;
+  const Skew3 sk = make_skew(r1, r2, r3);
   const Ctx::LibmDev& lm = ctx.libm_dev();
-  uint8_t* ebc = ctx.buf<uint8_t>("c3.ebc", sk.per);
-  int* flags = ctx.buf<int>("c3.flags", 4);
-  CPSZ_HIP(hipMemsetAsync(flags, 0, 16, st));
+  CPSZ_HIP(hipMemsetAsync(w.flags, 0, 16, st));
   {
     dim3 grid((unsigned)div_up(r3, kTK), (unsigned)div_up(r2, kTJ), (unsigned)div_up(r1, kTI));
-    k_eb3d<<<grid, 256, 0, st>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
-                                 ctx.q3d_size(), sk, ebc, flags);
+    k_eb3d<<<grid, 256, 0, st>>>(w.dU, w.dV, w.dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
+                                 ctx.q3d_size(), sk, w.ebc, w.flags, ov.gain, ov.fallback, ov.eb_out);
     CPSZ_HIP(hipGetLastError());
   }
   ctx.mark("eb");
-  const size_t smap = (n - 1) / 8 + 1, nq = div_up(n, 64);
-  uint64_t* sm = ctx.buf<uint64_t>("c3.smap", 3 * nq);
-  k_signmap<<<(int)div_up(3 * nq * 64, 256), 256, 0, st>>>(dU, dV, dW, n, nq, sm);
-  float* L = ctx.buf<float>("c3.log", 3 * sk.per);
+  const size_t nq = div_up(n, 64);
+  k_signmap<<<(int)div_up(3 * nq * 64, 256), 256, 0, st>>>(w.dU, w.dV, w.dW, n, nq, w.smap);
   {
     const int bx = r3 >= 256 ? 256 : (int)div_up(r3, 64) * 64;
     dim3 grid((unsigned)(r1 * r2), (unsigned)div_up(r3, bx));
-    k_log<<<grid, bx, 0, st>>>(dU, dV, dW, sk, lm.log2f, L);
+    k_log<<<grid, bx, 0, st>>>(w.dU, w.dV, w.dW, sk, lm.log2f, w.L);
   }
   CPSZ_HIP(hipGetLastError());
   ctx.mark("log");
-  int hflags[4];
-  CPSZ_HIP(hipMemcpyAsync(hflags, flags, 16, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
-  CPSZ_REQUIRE(hflags[0] == 0, kErrNonFinite,
-               "compress_3d: non-finite input or bound (values too large for the reference's "
-               "float triple products)");
-  // ---- wavefront
-  const int nb = sk.nb;
-  Wave3d w{};
-  w.lx = L;
-  w.code = ebc;
-  w.R = ctx.buf<float>("c3.R", 3 * sk.per);
-  w.qout = ctx.buf<uint16_t>("c3.q", 3 * sk.per);
-  w.vid = ctx.buf<uint8_t>("c3.vid", sk.per);
-  w.prog = ctx.buf<int>("c3.prog", (size_t)r1 * nb + 1);
-  w.ticket = w.prog + (size_t)r1 * nb;
-  w.s = sk;
-  launch_wave3d(ctx, w, true);
-  ctx.mark("lorenzo");
+}
+
+void c3_check(Ctx& ctx, C3Work* const* w, size_t nf) {
+  int* h = (int*)ctx.host_buf("c3.hflags", 16 * nf);
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_HIP(hipMemcpyAsync(h + 4 * k, w[k]->flags, 16, hipMemcpyDeviceToHost, ctx.stream()));
+  CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(h[4 * k] == 0, kErrNonFinite,
+                 "compress_3d: non-finite input or bound (values too large for the reference's "
+                 "float triple products)");
+}
+
+void c3_wavefront(Ctx& ctx, C3Work* const* w, size_t nf, int grid_cap) {
+  std::vector<Wave3d> f(nf);
+  for (size_t k = 0; k < nf; k++) {
+    f[k] = Wave3d{};
+    f[k].lx = w[k]->L;
+    f[k].code = w[k]->ebc;
+    f[k].R = w[k]->R;
+    f[k].qout = w[k]->q;
+    f[k].vid = w[k]->vid;
+    f[k].prog = w[k]->prog;
+    f[k].s = make_skew(w[k]->r1, w[k]->r2, w[k]->r3);
+  }
+  launch_wave3d<true>(ctx, f, grid_cap);
+}
+
+void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
+  hipStream_t st = ctx.stream();
+  const size_t n = w.r1 * w.r2 * w.r3;
+  const Skew3 sk = make_skew(w.r1, w.r2, w.r3);
   // ---- histograms
   uint32_t* hist = ctx.buf<uint32_t>("c3.hist", kMaxEbCode + 65536);
   CPSZ_HIP(hipMemsetAsync(hist, 0, (kMaxEbCode + 65536) * 4, st));
   const size_t nch = div_up(n, kVChunk);
   SrcIds3d s_ids{w.vid, sk, n};
-  SrcCodes3d s_codes{w.vid, w.qout, sk, n};
+  SrcCodes3d s_codes{w.vid, w.q, sk, n};
   {
     const int grid = (int)std::min<size_t>(div_up(nch, 256), 1024);
     k_hist<<<grid, 256, kMaxEbCode * 4, st>>>(s_ids, nch, hist, 0, kMaxEbCode);
@@ -722,8 +862,7 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
   huff_build(f.data(), kMaxEbCode, 2 * 256, t_id);
   uint64_t npred_syms = 0;
   for (int k = 0; k < 65536; k++) npred_syms += (f[k] = hh[kMaxEbCode + k]);
-  const bool have_codes = npred_syms > 0;
-  CPSZ_REQUIRE(have_codes, kErrUnsupported,
+  CPSZ_REQUIRE(npred_syms > 0, kErrUnsupported,
                "compress_3d: no predictable vertex (the reference cannot encode an empty code "
                "section)");
   huff_build(f.data(), 65536, 2 * kCapacity, t_q);
@@ -759,7 +898,7 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
   k_encode<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536,
                                                   offs + nch, words + w_id);
   float* unp = ctx.buf<float>("c3.unp", 3 * nunp + 1);
-  k_gather_unpred3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, dU, dV, dW, n, nch, coff + nch + 1,
+  k_gather_unpred3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, w.dU, w.dV, w.dW, n, nch, coff + nch + 1,
                                                          unp);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
@@ -772,6 +911,7 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
     put(h, (size_t)nbytes);
     return h;
   };
+  const size_t smap = (n - 1) / 8 + 1, nq = div_up(n, 64);
   std::vector<uint8_t> h0, h1, h2;
   put(h0, (int)2);
   put(h0, (int)kRadius);
@@ -802,7 +942,7 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
     pos += len;
   };
   host_piece(h0);
-  for (int p = 0; p < 3; p++) dev_piece(sm + p * nq, smap);
+  for (int p = 0; p < 3; p++) dev_piece(w.smap + p * nq, smap);
   host_piece(h1);
   dev_piece(unp, 12 * nunp);
   host_piece(h2);
@@ -813,10 +953,52 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
   ctx.mark("assemble");
   out.data = dout;
   out.size = total;
-  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_HIP(hipStreamSynchronize(st));  // the pinned header staging is reused by the next call
+}
+
+void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1,
+                        size_t r2, size_t r3, double max_pwr_eb, DeviceStream& out, const EbOverride& ov) {
+  CPSZ_REQUIRE(dU && dV && dW, kErrArg, "compress_3d: bad arguments");
+  check_dims_3d(r1, r2, r3, "compress_3d");
+  ctx.stages.clear();
+  ctx.mark("start");
+  C3Work w;
+  w.dU = dU;
+  w.dV = dV;
+  w.dW = dW;
+  w.r1 = r1;
+  w.r2 = r2;
+  w.r3 = r3;
+  c3_bind(w, ctx.buf<uint8_t>("c3.scratch", c3_scratch_bytes(r1, r2, r3)));
+  c3_eb(ctx, w, max_pwr_eb, ov);
+  C3Work* wp = &w;
+  c3_check(ctx, &wp, 1);
+  c3_wavefront(ctx, &wp, 1, 1024);
+  ctx.mark("lorenzo");
+  c3_entropy(ctx, w, out);
   ctx.finish_marks();
 }
 
+void eb_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1, size_t r2, size_t r3,
+                  double max_pwr_eb, double* dE) {
+  CPSZ_REQUIRE(dU && dV && dW && dE, kErrArg, "eb_3d: bad arguments");
+  check_dims_3d(r1, r2, r3, "eb_3d");
+  hipStream_t st = ctx.stream();
+  const Skew3 sk = make_skew(r1, r2, r3);
+  uint8_t* code = ctx.buf<uint8_t>("e3.code", sk.per);
+  int* flags = ctx.buf<int>("e3.flags", 4);
+  CPSZ_HIP(hipMemsetAsync(flags, 0, 16, st));
+  dim3 grid((unsigned)div_up(r3, kTK), (unsigned)div_up(r2, kTJ), (unsigned)div_up(r1, kTI));
+  k_eb3d<<<grid, 256, 0, st>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(), ctx.q3d_size(), sk,
+                               code, flags, nullptr, nullptr, dE);
+  CPSZ_HIP(hipGetLastError());
+  int* h = (int*)ctx.host_buf("e3.hflags", 16);
+  CPSZ_HIP(hipMemcpyAsync(h, flags, 16, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_REQUIRE(h[0] == 0, kErrNonFinite, "eb_3d: non-finite input or bound");
+}
+
+// ----------------------------------------------------------- decompress ----
 namespace {
 struct Reader3 {
   Ctx& ctx;
@@ -843,16 +1025,37 @@ struct Reader3 {
 };
 }  // namespace
 
-void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
-                          size_t r1, size_t r2, size_t r3, float* dU, float* dV, float* dW) {
-  CPSZ_REQUIRE(d_in && dU && dV && dW && r1 && r2 && r3, kErrArg, "decompress_3d: bad arguments");
-  CPSZ_REQUIRE(r1 < (1u << 20) && r2 < (1u << 20) && r3 < (1u << 20) && r1 * r2 * r3 < (1ull << 32),
-               kErrUnsupported, "decompress_3d: field too large");
-  hipStream_t st = ctx.stream();
-  const size_t n = r1 * r2 * r3;
+size_t d3_scratch_bytes(size_t r1, size_t r2, size_t r3) {
   const Skew3 sk = make_skew(r1, r2, r3);
-  ctx.stages.clear();
-  ctx.mark("start");
+  const size_t n = r1 * r2 * r3, nch = div_up(n, kVChunk), smap = (n - 1) / 8 + 1;
+  return al256(n) + al256((3 * n + 1) * 2) + al256((3 * n + 1) * 4) + al256(3 * sk.per * 2) +
+         2 * al256(3 * sk.per * 4) + al256(sk.per) + al256(r1 * sk.nb * 4) + al256(16) + al256(3 * smap + 8) +
+         al256(2 * nch * 4) + al256((2 * nch + 4) * 8);
+}
+
+void d3_bind(D3Work& w, uint8_t* p) {
+  const Skew3 sk = make_skew(w.r1, w.r2, w.r3);
+  const size_t n = w.r1 * w.r2 * w.r3, nch = div_up(n, kVChunk), smap = (n - 1) / 8 + 1;
+  w.vid = carve<uint8_t>(p, n);
+  w.qc = carve<uint16_t>(p, 3 * n + 1);
+  w.unp = carve<float>(p, 3 * n + 1);
+  w.q = carve<uint16_t>(p, 3 * sk.per);
+  w.init = carve<float>(p, 3 * sk.per);
+  w.R = carve<float>(p, 3 * sk.per);
+  w.vs = carve<uint8_t>(p, sk.per);
+  w.prog = carve<int>(p, w.r1 * sk.nb);
+  w.err = carve<int>(p, 4);
+  w.sm = carve<uint8_t>(p, 3 * smap + 8);
+  w.cnt = carve<uint32_t>(p, 2 * nch);
+  w.coff = carve<uint64_t>(p, 2 * nch + 4);
+}
+
+void d3_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, D3Work& w) {
+  CPSZ_REQUIRE(d_in && w.dU && w.dV && w.dW, kErrArg, "decompress_3d: bad arguments");
+  check_dims_3d(w.r1, w.r2, w.r3, "decompress_3d");
+  hipStream_t st = ctx.stream();
+  const size_t n = w.r1 * w.r2 * w.r3;
+  const Skew3 sk = make_skew(w.r1, w.r2, w.r3);
   const Ctx::LibmDev& lm = ctx.libm_dev();
   Reader3 rd{ctx, h_in, d_in, in_size, {}};
   const int base = rd.get<int>(0);
@@ -887,62 +1090,89 @@ void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
   parse(off + 8, nc_q, tr_q, nb_q, bo_q);
   ctx.mark("parse");
   auto stage_bits = [&](size_t bo, size_t nbytes, const char* name) {
-    uint32_t* w = ctx.buf<uint32_t>(name, div_up(nbytes, 4) + 8);
-    CPSZ_HIP(hipMemsetAsync(w, 0, (div_up(nbytes, 4) + 8) * 4, st));
-    if (nbytes) CPSZ_HIP(hipMemcpyAsync(w, d_in + bo, nbytes, hipMemcpyDeviceToDevice, st));
-    return w;
+    uint32_t* wb = ctx.buf<uint32_t>(name, div_up(nbytes, 4) + 8);
+    CPSZ_HIP(hipMemsetAsync(wb, 0, (div_up(nbytes, 4) + 8) * 4, st));
+    if (nbytes) CPSZ_HIP(hipMemcpyAsync(wb, d_in + bo, nbytes, hipMemcpyDeviceToDevice, st));
+    return wb;
   };
-  uint8_t* vid = ctx.buf<uint8_t>("d3.vid", n);
-  uint16_t* qc = ctx.buf<uint16_t>("d3.qc", data_num + 1);
   {
-    uint32_t* w = stage_bits(bo_id, nb_id, "d3.w.id");
+    uint32_t* wb = stage_bits(bo_id, nb_id, "d3.w.id");
     DeviceDecoder dd = upload_decoder(ctx, tr_id.data(), tr_id.size(), nc_id, "d3.id");
-    decode_stream(ctx, dd, w, nb_id, n, SinkU8{vid}, "d3.id");
+    decode_stream(ctx, dd, wb, nb_id, n, SinkU8{w.vid}, "d3.id");
   }
   {
-    uint32_t* w = stage_bits(bo_q, nb_q, "d3.w.q");
+    uint32_t* wb = stage_bits(bo_q, nb_q, "d3.w.q");
     DeviceDecoder dd = upload_decoder(ctx, tr_q.data(), tr_q.size(), nc_q, "d3.q");
-    decode_stream(ctx, dd, w, nb_q, data_num, SinkU16{qc}, "d3.q");
+    decode_stream(ctx, dd, wb, nb_q, data_num, SinkU16{w.qc}, "d3.q");
   }
   ctx.mark("huffman");
   const size_t nch = div_up(n, kVChunk);
-  uint32_t* cnt = ctx.buf<uint32_t>("d3.cnt", 2 * nch);
-  uint64_t* coff = ctx.buf<uint64_t>("d3.coff", 2 * nch + 4);
-  k_chunk_counts3d<false><<<(int)div_up(nch, 4), 256, 0, st>>>(vid, sk, n, nch, cnt, cnt + nch);
-  scan_u32_to_u64(ctx, cnt, coff, nch, coff + 2 * nch + 2);
-  scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, coff + 2 * nch + 3);
-  float* unp = ctx.buf<float>("d3.unp", nunp_f + 1);
-  if (nunp_f) CPSZ_HIP(hipMemcpyAsync(unp, d_in + unp_off, 4 * nunp_f, hipMemcpyDeviceToDevice, st));
-  uint16_t* q = ctx.buf<uint16_t>("d3.q", 3 * sk.per);
-  float* init = ctx.buf<float>("d3.init", 3 * sk.per);
-  uint8_t* vs = ctx.buf<uint8_t>("d3.vs", sk.per);
-  int* err = ctx.buf<int>("d3.err", 2);
-  CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
-  k_scatter3d<<<(int)div_up(nch, 4), 256, 0, st>>>(vid, sk, qc, unp, n, nch, coff, coff + nch + 1,
-                                                   data_num, nunp_f, lm.log2f, vs, q, init, err);
+  k_chunk_counts3d<false><<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, n, nch, w.cnt, w.cnt + nch);
+  scan_u32_to_u64(ctx, w.cnt, w.coff, nch, w.coff + 2 * nch + 2);
+  scan_u32_to_u64(ctx, w.cnt + nch, w.coff + nch + 1, nch, w.coff + 2 * nch + 3);
+  if (nunp_f) CPSZ_HIP(hipMemcpyAsync(w.unp, d_in + unp_off, 4 * nunp_f, hipMemcpyDeviceToDevice, st));
+  CPSZ_HIP(hipMemsetAsync(w.err, 0, 16, st));
+  k_scatter3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, w.qc, w.unp, n, nch, w.coff, w.coff + nch + 1,
+                                                   data_num, nunp_f, lm.log2f, w.vs, w.q, w.init, w.err);
   CPSZ_HIP(hipGetLastError());
+  CPSZ_HIP(hipMemcpyAsync(w.sm, d_in + 8, 3 * smap, hipMemcpyDeviceToDevice, st));
+  w.nunp = nunp_f;
   ctx.mark("scatter");
-  const int nbk = sk.nb;
-  Wave3d w{};
-  w.lx = init;
-  w.code = vs;
-  w.qin = q;
-  w.R = ctx.buf<float>("d3.R", 3 * sk.per);
-  w.prog = ctx.buf<int>("d3.prog", (size_t)r1 * nbk + 1);
-  w.ticket = w.prog + (size_t)r1 * nbk;
-  w.s = sk;
-  launch_wave3d(ctx, w, false);
-  ctx.mark("lorenzo");
-  uint8_t* sm = ctx.buf<uint8_t>("d3.smap", 3 * smap + 8);
-  CPSZ_HIP(hipMemcpyAsync(sm, d_in + 8, 3 * smap, hipMemcpyDeviceToDevice, st));
-  k_finish3d<<<(int)div_up(nch, 4), 256, 0, st>>>(vid, sk, w.R, sm, smap, unp, n, nch, coff + nch + 1,
-                                                  nunp_f, lm.exp2f, dU, dV, dW);
+}
+
+void d3_wavefront(Ctx& ctx, D3Work* const* w, size_t nf, int grid_cap) {
+  std::vector<Wave3d> f(nf);
+  for (size_t k = 0; k < nf; k++) {
+    f[k] = Wave3d{};
+    f[k].lx = w[k]->init;
+    f[k].code = w[k]->vs;
+    f[k].qin = w[k]->q;
+    f[k].R = w[k]->R;
+    f[k].prog = w[k]->prog;
+    f[k].s = make_skew(w[k]->r1, w[k]->r2, w[k]->r3);
+  }
+  launch_wave3d<false>(ctx, f, grid_cap);
+}
+
+void d3_finish(Ctx& ctx, const D3Work& w) {
+  const size_t n = w.r1 * w.r2 * w.r3, nch = div_up(n, kVChunk), smap = (n - 1) / 8 + 1;
+  const Skew3 sk = make_skew(w.r1, w.r2, w.r3);
+  k_finish3d<<<(int)div_up(nch, 4), 256, 0, ctx.stream()>>>(w.vid, sk, w.R, w.sm, smap, w.unp, n, nch,
+                                                            w.coff + nch + 1, w.nunp, ctx.libm_dev().exp2f,
+                                                            w.dU, w.dV, w.dW);
   CPSZ_HIP(hipGetLastError());
+}
+
+void d3_check(Ctx& ctx, D3Work* const* w, size_t nf) {
+  int* h = (int*)ctx.host_buf("d3.herr", 16 * nf);
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_HIP(hipMemcpyAsync(h + 4 * k, w[k]->err, 16, hipMemcpyDeviceToHost, ctx.stream()));
+  CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(h[4 * k] == 0, kErrStream, "decompress_3d: code/unpredictable counts inconsistent");
+}
+
+void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
+                          size_t r1, size_t r2, size_t r3, float* dU, float* dV, float* dW) {
+  CPSZ_REQUIRE(d_in && dU && dV && dW, kErrArg, "decompress_3d: bad arguments");
+  check_dims_3d(r1, r2, r3, "decompress_3d");
+  ctx.stages.clear();
+  ctx.mark("start");
+  D3Work w;
+  w.r1 = r1;
+  w.r2 = r2;
+  w.r3 = r3;
+  w.dU = dU;
+  w.dV = dV;
+  w.dW = dW;
+  d3_bind(w, ctx.buf<uint8_t>("d3.scratch", d3_scratch_bytes(r1, r2, r3)));
+  d3_entropy(ctx, d_in, h_in, in_size, w);
+  D3Work* wp = &w;
+  d3_wavefront(ctx, &wp, 1, 1024);
+  ctx.mark("lorenzo");
+  d3_finish(ctx, w);
   ctx.mark("finish");
-  int herr[2];
-  CPSZ_HIP(hipMemcpyAsync(herr, err, 8, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
-  CPSZ_REQUIRE(herr[0] == 0, kErrStream, "decompress_3d: code/unpredictable counts inconsistent");
+  d3_check(ctx, &wp, 1);
   ctx.finish_marks();
 }
 

@@ -27,88 +27,6 @@ namespace cpsz {
 
 namespace {
 
-__device__ __forceinline__ void add_triple(Expansion& e, double s, float a, float b, float c) {
-  const double ab = (double)a * b;
-  const double p = ab * (double)c;
-  const double q = fma(ab, (double)c, -p);
-  e.add(s * p);
-  if (q != 0) e.add(s * q);
-}
-
-// sign of a sum of signed triple products: filtered, exact fallback
-struct Triple {
-  float a, b, c;
-  double s;
-};
-
-__device__ int sign_triples(const Triple* t, int m) {
-  double sum = 0, mag = 0;
-  for (int i = 0; i < m; i++) {
-    const double p = t[i].s * ((double)t[i].a * t[i].b) * t[i].c;
-    sum += p;
-    mag += fabs(p);
-  }
-  // each product carries <= 1 rounding (2 ulp), the sum <= m roundings
-  const double bound = (double)(m + 4) * 2.3e-16 * mag;
-  if (sum > bound) return 1;
-  if (sum < -bound) return -1;
-  Expansion e;
-  for (int i = 0; i < m; i++) add_triple(e, t[i].s, t[i].a, t[i].b, t[i].c);
-  return e.sign();
-}
-
-__device__ int type_3d(const double v[4][3], const int X[4][3]) {
-  double Xm[3][3], Um[3][3];
-  for (int r = 0; r < 3; r++)
-    for (int c = 0; c < 3; c++) {
-      Xm[r][c] = X[c][r] - X[3][r];
-      Um[r][c] = v[c][r] - v[3][r];
-    }
-  const double d = Xm[0][0] * (Xm[1][1] * Xm[2][2] - Xm[1][2] * Xm[2][1]) -
-                   Xm[0][1] * (Xm[1][0] * Xm[2][2] - Xm[1][2] * Xm[2][0]) +
-                   Xm[0][2] * (Xm[1][0] * Xm[2][1] - Xm[1][1] * Xm[2][0]);
-  double inv[3][3];
-  inv[0][0] = (Xm[1][1] * Xm[2][2] - Xm[1][2] * Xm[2][1]) / d;
-  inv[0][1] = (Xm[0][2] * Xm[2][1] - Xm[0][1] * Xm[2][2]) / d;
-  inv[0][2] = (Xm[0][1] * Xm[1][2] - Xm[0][2] * Xm[1][1]) / d;
-  inv[1][0] = (Xm[1][2] * Xm[2][0] - Xm[1][0] * Xm[2][2]) / d;
-  inv[1][1] = (Xm[0][0] * Xm[2][2] - Xm[0][2] * Xm[2][0]) / d;
-  inv[1][2] = (Xm[0][2] * Xm[1][0] - Xm[0][0] * Xm[1][2]) / d;
-  inv[2][0] = (Xm[1][0] * Xm[2][1] - Xm[1][1] * Xm[2][0]) / d;
-  inv[2][1] = (Xm[0][1] * Xm[2][0] - Xm[0][0] * Xm[2][1]) / d;
-  inv[2][2] = (Xm[0][0] * Xm[1][1] - Xm[0][1] * Xm[1][0]) / d;
-  double J[3][3];
-  for (int r = 0; r < 3; r++)
-    for (int c = 0; c < 3; c++) J[r][c] = inv[r][0] * Um[0][c] + inv[r][1] * Um[1][c] + inv[r][2] * Um[2][c];
-  const double tr = J[0][0] + J[1][1] + J[2][2];
-  const double m2 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) + (J[0][0] * J[2][2] - J[0][2] * J[2][0]) +
-                    (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
-  const double dj = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
-                    J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                    J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-  const double b = -tr, c = m2, dd = -dj;
-  const double q = (3 * c - b * b) / 9;
-  const double r = (-27 * dd + b * (9 * c - 2 * b * b)) / 54;
-  const double disc = q * q * q + r * r;
-  if (fabs(disc) < 2.220446049250313e-16) return 0;
-  if (disc > 0) {
-    // one real root r0 (< 0 iff p(0) = dd > 0) and a pair s +- iw with
-    // b c - dd = -2 s ((r0 + s)^2 + w^2)
-    const int neg = (dd > 0) + 2 * (b * c - dd > 0);
-    return 2 * neg + 2;
-  }
-  // three real roots: sign changes of p(-x) = -x^3 + b x^2 - c x + dd
-  const double co[4] = {-1.0, b, -c, dd};
-  int neg = 0;
-  double last = -1.0;
-  for (int k = 1; k < 4; k++) {
-    if (co[k] == 0) continue;
-    if ((co[k] > 0) != (last > 0)) neg++;
-    last = co[k];
-  }
-  return 2 * neg + 1;
-}
-
 struct CpOut {
   unsigned long long* count;
   long long* ids;
@@ -150,50 +68,18 @@ __global__ void k_cp3d(const float* __restrict__ U, const float* __restrict__ V,
   const int k = (int)(cell % nk), j = (int)((cell / nk) % nj), i = (int)(cell / (nk * nj));
   const size_t d0 = (size_t)r2 * r3, d1 = r3;
   const size_t base = i * d0 + j * d1 + k;
-  // (id offset, vertex 1 and 2 as (di, dj, dk)); vertex 0 = (i,j,k), vertex 3 = (i+1,j+1,k+1)
-  const int defs[6][7] = {{0, 1, 0, 0, 1, 1, 0}, {2, 0, 1, 0, 1, 1, 0}, {1, 1, 0, 0, 1, 0, 1},
-                          {4, 0, 0, 1, 1, 0, 1}, {3, 0, 1, 0, 0, 1, 1}, {5, 0, 0, 1, 0, 1, 1}};
+  auto at = [&](int c) { return base + (c >> 2) * d0 + ((c >> 1) & 1) * d1 + (c & 1); };
   for (int t = 0; t < 6; t++) {
-    const int* D = defs[t];
-    const size_t vi[4] = {base, base + D[1] * d0 + D[2] * d1 + D[3], base + D[4] * d0 + D[5] * d1 + D[6],
-                          base + d0 + d1 + 1};
+    const int ca = judge_tet_a(t), cb = judge_tet_b(t);
+    const size_t vi[4] = {base, at(ca), at(cb), base + d0 + d1 + 1};
     float f[4][3];
-    bool zero = false;
     for (int p = 0; p < 4; p++) {
       f[p][0] = U[vi[p]];
       f[p][1] = V[vi[p]];
       f[p][2] = W[vi[p]];
-      zero = zero || (f[p][0] == 0 && f[p][1] == 0 && f[p][2] == 0);
     }
-    if (zero) continue;
-    // numerators: D_p = (-1)^(p+1) det(other three vectors), D = sum
-    const int rows[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
-    int sp[4];
-    Triple all[24];
-    for (int p = 0; p < 4; p++) {
-      const float* a = f[rows[p][0]];
-      const float* b = f[rows[p][1]];
-      const float* c = f[rows[p][2]];
-      const double sg = (p & 1) ? 1.0 : -1.0;
-      Triple tr[6] = {{a[0], b[1], c[2], sg}, {a[0], b[2], c[1], -sg}, {a[1], b[0], c[2], -sg},
-                      {a[1], b[2], c[0], sg},  {a[2], b[0], c[1], sg},  {a[2], b[1], c[0], -sg}};
-      sp[p] = sign_triples(tr, 6);
-      for (int q = 0; q < 6; q++) all[6 * p + q] = tr[q];
-    }
-    bool mixed = false;
-    for (int p = 0; p < 4; p++)
-      for (int q = p + 1; q < 4; q++) mixed = mixed || (sp[p] * sp[q] < 0);
-    if (mixed) continue;
-    const int sd = sign_triples(all, 24);
-    if (sd == 0) continue;
-    bool in = true;
-    for (int p = 0; p < 4; p++) in = in && (sp[p] * sd >= 0);
-    if (!in) continue;
-    const int X[4][3] = {{0, 0, 0}, {D[3], D[2], D[1]}, {D[6], D[5], D[4]}, {1, 1, 1}};
-    double v[4][3];
-    for (int p = 0; p < 4; p++)
-      for (int q = 0; q < 3; q++) v[p][q] = f[p][q];
-    out.push(6ll * (long long)base + D[0], type_3d(v, X));
+    const int type = tet_cp(f, ca, cb);
+    if (type >= 0) out.push(6ll * (long long)base + judge_tet_sid(t), type);
   }
 }
 

@@ -1,5 +1,5 @@
-// Exact 2D critical-point predicate and type, shared by the GPU judge, the
-// learned-bound labeler and their host emulations.
+// Exact 2D and 3D critical-point predicates and types, shared by the GPU
+// judge, the learned-bound labelers and their host emulations.
 //
 // A triangle of the reference's triangulation (test/cp_extraction_2d.cpp:62-147)
 // holds a critical point when the zero vector lies in the closed triangle of
@@ -70,6 +70,137 @@ __host__ __device__ inline int cp_type_2d(const double v[3][2], int lower) {
     return tr > 0 ? 2 : 0;              // a zero root: the other is tr
   }
   return tr < 0 ? 4 : (tr > 0 ? 5 : 6);
+}
+
+// ------------------------------------------------------------------- 3D --
+// sum_i s_i * a_i * b_i * c_i (floats, s_i = +-1): its sign, filtered by a
+// forward error bound, else exact (FMA-based TwoProduct into an expansion).
+struct Triple {
+  float a, b, c;
+  double s;
+};
+
+__host__ __device__ inline void add_triple(Expansion& e, double s, float a, float b, float c) {
+  const double ab = (double)a * b;  // exact
+  const double p = ab * (double)c;
+  const double q = fma(ab, (double)c, -p);
+  e.add(s * p);
+  if (q != 0) e.add(s * q);
+}
+
+__host__ __device__ inline int sign_triples(const Triple* t, int m) {
+  double sum = 0, mag = 0;
+  for (int i = 0; i < m; i++) {
+    const double p = t[i].s * ((double)t[i].a * t[i].b) * t[i].c;
+    sum += p;
+    mag += fabs(p);
+  }
+  // each product carries <= 1 rounding (2 ulp), the sum <= m roundings
+  const double bound = (double)(m + 4) * 2.3e-16 * mag;
+  if (sum > bound) return 1;
+  if (sum < -bound) return -1;
+  Expansion e;
+  for (int i = 0; i < m; i++) add_triple(e, t[i].s, t[i].a, t[i].b, t[i].c);
+  return e.sign();
+}
+
+// Type code (test/cp_extraction_3d.cpp:30-38) of the critical point of a
+// tetrahedron with vertex vectors v[p] at coordinates X[p].
+__host__ __device__ inline int type_3d(const double v[4][3], const int X[4][3]) {
+  double Xm[3][3], Um[3][3];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) {
+      Xm[r][c] = X[c][r] - X[3][r];
+      Um[r][c] = v[c][r] - v[3][r];
+    }
+  const double d = Xm[0][0] * (Xm[1][1] * Xm[2][2] - Xm[1][2] * Xm[2][1]) -
+                   Xm[0][1] * (Xm[1][0] * Xm[2][2] - Xm[1][2] * Xm[2][0]) +
+                   Xm[0][2] * (Xm[1][0] * Xm[2][1] - Xm[1][1] * Xm[2][0]);
+  double inv[3][3];
+  inv[0][0] = (Xm[1][1] * Xm[2][2] - Xm[1][2] * Xm[2][1]) / d;
+  inv[0][1] = (Xm[0][2] * Xm[2][1] - Xm[0][1] * Xm[2][2]) / d;
+  inv[0][2] = (Xm[0][1] * Xm[1][2] - Xm[0][2] * Xm[1][1]) / d;
+  inv[1][0] = (Xm[1][2] * Xm[2][0] - Xm[1][0] * Xm[2][2]) / d;
+  inv[1][1] = (Xm[0][0] * Xm[2][2] - Xm[0][2] * Xm[2][0]) / d;
+  inv[1][2] = (Xm[0][2] * Xm[1][0] - Xm[0][0] * Xm[1][2]) / d;
+  inv[2][0] = (Xm[1][0] * Xm[2][1] - Xm[1][1] * Xm[2][0]) / d;
+  inv[2][1] = (Xm[0][1] * Xm[2][0] - Xm[0][0] * Xm[2][1]) / d;
+  inv[2][2] = (Xm[0][0] * Xm[1][1] - Xm[0][1] * Xm[1][0]) / d;
+  double J[3][3];
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) J[r][c] = inv[r][0] * Um[0][c] + inv[r][1] * Um[1][c] + inv[r][2] * Um[2][c];
+  const double tr = J[0][0] + J[1][1] + J[2][2];
+  const double m2 = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) + (J[0][0] * J[2][2] - J[0][2] * J[2][0]) +
+                    (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+  const double dj = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                    J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                    J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+  const double b = -tr, c = m2, dd = -dj;
+  const double q = (3 * c - b * b) / 9;
+  const double r = (-27 * dd + b * (9 * c - 2 * b * b)) / 54;
+  const double disc = q * q * q + r * r;
+  if (fabs(disc) < 2.220446049250313e-16) return 0;
+  if (disc > 0) {
+    // one real root r0 (< 0 iff p(0) = dd > 0) and a pair s +- iw with
+    // b c - dd = -2 s ((r0 + s)^2 + w^2)
+    const int neg = (dd > 0) + 2 * (b * c - dd > 0);
+    return 2 * neg + 2;
+  }
+  // three real roots: sign changes of p(-x) = -x^3 + b x^2 - c x + dd
+  const double co[4] = {-1.0, b, -c, dd};
+  int neg = 0;
+  double last = -1.0;
+  for (int k = 1; k < 4; k++) {
+    if (co[k] == 0) continue;
+    if ((co[k] > 0) != (last > 0)) neg++;
+    last = co[k];
+  }
+  return 2 * neg + 1;
+}
+
+// The six tetrahedra of a cell in the judge's order
+// (test/cp_extraction_3d.cpp:127-209): vertex 0 is the cell origin, vertex 3
+// the opposite corner (1,1,1), vertices 1 and 2 the corners ca(t), cb(t),
+// coded di << 2 | dj << 1 | dk; the simplex id is 6 * origin + sid(t).
+__host__ __device__ inline int judge_tet_a(int t) { return (int)((0x121424u >> (4 * t)) & 7u); }
+__host__ __device__ inline int judge_tet_b(int t) { return (int)((0x335566u >> (4 * t)) & 7u); }
+__host__ __device__ inline int judge_tet_sid(int t) { return (int)((0x534120u >> (4 * t)) & 7u); }
+
+// -1 when the tetrahedron (vertex vectors f[p] = (u, v, w), vertices 1 and 2
+// at corners ca, cb) holds no critical point, else its type code.
+__host__ __device__ inline int tet_cp(const float f[4][3], int ca, int cb) {
+  for (int p = 0; p < 4; p++)
+    if (f[p][0] == 0 && f[p][1] == 0 && f[p][2] == 0) return -1;
+  // numerators: D_p = (-1)^(p+1) det(other three vectors), D = sum
+  int sp[4];
+  Triple all[24];
+  for (int p = 0; p < 4; p++) {
+    const float* a = f[p == 0 ? 1 : 0];
+    const float* b = f[p <= 1 ? 2 : 1];
+    const float* c = f[p <= 2 ? 3 : 2];
+    const double sg = (p & 1) ? 1.0 : -1.0;
+    Triple* tr = all + 6 * p;
+    tr[0] = Triple{a[0], b[1], c[2], sg};
+    tr[1] = Triple{a[0], b[2], c[1], -sg};
+    tr[2] = Triple{a[1], b[0], c[2], -sg};
+    tr[3] = Triple{a[1], b[2], c[0], sg};
+    tr[4] = Triple{a[2], b[0], c[1], sg};
+    tr[5] = Triple{a[2], b[1], c[0], -sg};
+    sp[p] = sign_triples(tr, 6);
+  }
+  for (int p = 0; p < 4; p++)
+    for (int q = p + 1; q < 4; q++)
+      if (sp[p] * sp[q] < 0) return -1;
+  const int sd = sign_triples(all, 24);
+  if (sd == 0) return -1;
+  for (int p = 0; p < 4; p++)
+    if (sp[p] * sd < 0) return -1;
+  // coordinates (x, y, z) = (dk, dj, di)
+  const int X[4][3] = {{0, 0, 0}, {ca & 1, (ca >> 1) & 1, ca >> 2}, {cb & 1, (cb >> 1) & 1, cb >> 2}, {1, 1, 1}};
+  double v[4][3];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 3; q++) v[p][q] = f[p][q];
+  return type_3d(v, X);
 }
 
 // -1 when the triangle (vertex vectors f[p] = (u, v)) holds no critical point,

@@ -1,11 +1,14 @@
-// Learned error-bound proposer for the 2D codec: features, bisection labels,
-// BF16 MFMA MLP (forward, backward, Adam) and the verified learned compressor.
+// Learned error-bound proposer for the 2D and 3D codecs: features, bisection
+// labels, BF16 MFMA MLP (forward, backward, Adam) and the verified learned
+// compressors (single 2D field; batches of 3D fields).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
 
+#include "batch3d.hpp"
+#include "codec3d.hpp"
 #include "cp.hpp"
 #include "cp_math.hpp"
 #include "learn.hpp"
@@ -274,13 +277,13 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 }
 
 __global__ void k_gather_rows(const uint16_t* __restrict__ X, const float* __restrict__ T,
-                              const uint32_t* __restrict__ perm, size_t m, uint16_t* __restrict__ Xb,
+                              const uint32_t* __restrict__ perm, size_t m, int fdim, uint16_t* __restrict__ Xb,
                               float* __restrict__ Tb) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (i >= m * kFeatDim) return;
-  const size_t r = i / kFeatDim, c = i - r * kFeatDim;
+  if (i >= m * fdim) return;
+  const size_t r = i / fdim, c = i - r * fdim;
   const size_t s = perm[r];
-  Xb[i] = X[s * kFeatDim + c];
+  Xb[i] = X[s * fdim + c];
   if (c == 0) Tb[r] = T[s];
 }
 
@@ -312,34 +315,39 @@ struct Acts {
   uint16_t *H1b, *W1b, *W2b;
 };
 
-Acts acts(Ctx& ctx, size_t cap) {
+Acts acts(Ctx& ctx, size_t cap, int fdim) {
   Acts a;
   a.Z1 = ctx.buf<float>("nn.z1", cap * kHidden);
   a.Z2 = ctx.buf<float>("nn.z2", cap * kHidden);
   a.H2 = ctx.buf<float>("nn.h2", cap * kHidden);
   a.y = ctx.buf<float>("nn.y", cap);
   a.H1b = ctx.buf<uint16_t>("nn.h1b", cap * kHidden);
-  a.W1b = ctx.buf<uint16_t>("nn.w1b", (size_t)kFeatDim * kHidden);
+  a.W1b = ctx.buf<uint16_t>("nn.w1b", (size_t)fdim * kHidden);
   a.W2b = ctx.buf<uint16_t>("nn.w2b", (size_t)kHidden * kHidden);
   return a;
 }
 
-void weights_bf16(Ctx& ctx, const float* P, Acts& a) {
+void weights_bf16(Ctx& ctx, const float* P, Acts& a, const NetLayout& L) {
   hipStream_t st = ctx.stream();
-  k_to_bf16<<<blocks_for((size_t)kFeatDim * kHidden), 256, 0, st>>>(P + kOffW1, (size_t)kFeatDim * kHidden, a.W1b);
-  k_to_bf16<<<blocks_for((size_t)kHidden * kHidden), 256, 0, st>>>(P + kOffW2, (size_t)kHidden * kHidden, a.W2b);
+  const size_t n1 = (size_t)L.fdim * kHidden, n2 = (size_t)kHidden * kHidden;
+  k_to_bf16<<<blocks_for(n1), 256, 0, st>>>(P + L.w1, n1, a.W1b);
+  k_to_bf16<<<blocks_for(n2), 256, 0, st>>>(P + L.w2, n2, a.W2b);
 }
 
 // forward of rows [0, m) of X (m <= cap)
-void forward_rows(Ctx& ctx, const float* P, Acts& a, const uint16_t* X, size_t m) {
+void forward_rows(Ctx& ctx, const float* P, Acts& a, const uint16_t* X, size_t m, const NetLayout& L) {
   hipStream_t st = ctx.stream();
-  gemm_bf16(ctx, (int)m, kHidden, kFeatDim, X, kFeatDim, 1, a.W1b, kHidden, 1, a.Z1, kHidden, 1);
-  k_bias_relu<<<blocks_for(m * kHidden), 256, 0, st>>>(a.Z1, P + kOffB1, m, a.H1b, nullptr);
+  gemm_bf16(ctx, (int)m, kHidden, L.fdim, X, L.fdim, 1, a.W1b, kHidden, 1, a.Z1, kHidden, 1);
+  k_bias_relu<<<blocks_for(m * kHidden), 256, 0, st>>>(a.Z1, P + L.b1, m, a.H1b, nullptr);
   gemm_bf16(ctx, (int)m, kHidden, kHidden, a.H1b, kHidden, 1, a.W2b, kHidden, 1, a.Z2, kHidden, 1);
   uint16_t* h2b = ctx.buf<uint16_t>("nn.h2b", m * kHidden);
-  k_bias_relu<<<blocks_for(m * kHidden), 256, 0, st>>>(a.Z2, P + kOffB2, m, h2b, a.H2);
-  k_out<<<blocks_for(m), 256, 0, st>>>(a.H2, P + kOffW3, P + kOffB3, m, a.y);
+  k_bias_relu<<<blocks_for(m * kHidden), 256, 0, st>>>(a.Z2, P + L.b2, m, h2b, a.H2);
+  k_out<<<blocks_for(m), 256, 0, st>>>(a.H2, P + L.w3, P + L.b3, m, a.y);
   CPSZ_HIP(hipGetLastError());
+}
+
+void check_fdim(int fdim) {
+  CPSZ_REQUIRE(fdim == kFeatDim || fdim == kFeatDim3, kErrArg, "MLP: unsupported feature width");
 }
 
 }  // namespace
@@ -357,18 +365,21 @@ void gemm_bf16(Ctx& ctx, int M, int N, int K, const uint16_t* A, long sam, long 
   CPSZ_HIP(hipGetLastError());
 }
 
-void mlp_init_params(uint64_t seed, float* p) {
+void mlp_init_params(uint64_t seed, float* p, int fdim, int fused) {
+  check_fdim(fdim);
+  CPSZ_REQUIRE(fused > 0 && fused <= fdim, kErrArg, "mlp_init_params: bad feature count");
+  const NetLayout L = net_layout(fdim);
   uint64_t s = seed ^ 0x6c6561726e656462ull;
   auto uni = [&]() { return (double)(splitmix(s) >> 11) * (1.0 / 9007199254740992.0); };
   auto fill = [&](size_t off, size_t n, double scale) {
     for (size_t i = 0; i < n; i++) p[off + i] = (float)((2 * uni() - 1) * scale);
   };
-  std::memset(p, 0, kNumParams * sizeof(float));
-  fill(kOffW1, (size_t)kFeatDim * kHidden, std::sqrt(6.0 / kFeatUsed));
-  fill(kOffW2, (size_t)kHidden * kHidden, std::sqrt(6.0 / kHidden));
-  fill(kOffW3, kHidden, std::sqrt(3.0 / kHidden));
-  for (size_t r = kFeatUsed; r < (size_t)kFeatDim; r++)
-    for (int c = 0; c < kHidden; c++) p[kOffW1 + r * kHidden + c] = 0.f;  // padding rows
+  std::memset(p, 0, L.count * sizeof(float));
+  fill(L.w1, (size_t)fdim * kHidden, std::sqrt(6.0 / fused));
+  fill(L.w2, (size_t)kHidden * kHidden, std::sqrt(6.0 / kHidden));
+  fill(L.w3, kHidden, std::sqrt(3.0 / kHidden));
+  for (size_t r = fused; r < (size_t)fdim; r++)
+    for (int c = 0; c < kHidden; c++) p[L.w1 + r * kHidden + c] = 0.f;  // padding rows
 }
 
 void analytic_eb_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
@@ -393,41 +404,95 @@ void labels_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, siz
   CPSZ_HIP(hipGetLastError());
 }
 
-void mlp_forward(Ctx& ctx, const float* P, const uint16_t* dX, size_t m, float* dY) {
+void mlp_forward(Ctx& ctx, const float* P, const uint16_t* dX, size_t m, float* dY, int fdim) {
+  check_fdim(fdim);
+  if (!m) return;
+  const NetLayout L = net_layout(fdim);
   const size_t cap = std::min<size_t>(m, 1u << 18);
-  Acts a = acts(ctx, cap);
-  weights_bf16(ctx, P, a);
+  Acts a = acts(ctx, cap, fdim);
+  weights_bf16(ctx, P, a, L);
   for (size_t r0 = 0; r0 < m; r0 += cap) {
     const size_t mm = std::min(cap, m - r0);
-    forward_rows(ctx, P, a, dX + r0 * kFeatDim, mm);
+    forward_rows(ctx, P, a, dX + r0 * fdim, mm, L);
     CPSZ_HIP(hipMemcpyAsync(dY + r0, a.y, mm * sizeof(float), hipMemcpyDeviceToDevice, ctx.stream()));
   }
 }
 
-void mlp_grad(Ctx& ctx, const float* P, const uint16_t* X, const float* T, size_t m, float* G, float* loss) {
+void mlp_grad(Ctx& ctx, const float* P, const uint16_t* X, const float* T, size_t m, float* G, float* loss,
+              int fdim) {
   CPSZ_REQUIRE(m > 0 && m <= (1u << 20), kErrArg, "mlp_grad: batch size out of range");
+  check_fdim(fdim);
+  const NetLayout L = net_layout(fdim);
   hipStream_t st = ctx.stream();
-  Acts a = acts(ctx, m);
-  weights_bf16(ctx, P, a);
-  forward_rows(ctx, P, a, X, m);
-  CPSZ_HIP(hipMemsetAsync(G, 0, kNumParams * sizeof(float), st));
+  Acts a = acts(ctx, m, fdim);
+  weights_bf16(ctx, P, a, L);
+  forward_rows(ctx, P, a, X, m, L);
+  CPSZ_HIP(hipMemsetAsync(G, 0, L.count * sizeof(float), st));
   CPSZ_HIP(hipMemsetAsync(loss, 0, sizeof(float), st));
   float* dy = ctx.buf<float>("nn.dy", m);
   k_loss<<<blocks_for(m), 256, 0, st>>>(a.y, T, m, dy, loss);
   uint16_t* dZ2 = ctx.buf<uint16_t>("nn.dz2", m * kHidden);
-  k_back_out<<<blocks_for(m, kRowsPerBlock), kHidden, 0, st>>>(a.H2, a.Z2, dy, P + kOffW3, m, G + kOffW3,
-                                                               G + kOffB3, G + kOffB2, dZ2);
+  k_back_out<<<blocks_for(m, kRowsPerBlock), kHidden, 0, st>>>(a.H2, a.Z2, dy, P + L.w3, m, G + L.w3,
+                                                               G + L.b3, G + L.b2, dZ2);
   // gW2 = H1^T dZ2 (split over the batch)
   const int split = (int)std::max<size_t>(1, std::min<size_t>(64, m / 512));
-  gemm_bf16(ctx, kHidden, kHidden, (int)m, a.H1b, 1, kHidden, dZ2, kHidden, 1, G + kOffW2, kHidden, split);
+  gemm_bf16(ctx, kHidden, kHidden, (int)m, a.H1b, 1, kHidden, dZ2, kHidden, 1, G + L.w2, kHidden, split);
   // dH1 = dZ2 W2^T
   float* dH1 = ctx.buf<float>("nn.dh1", m * kHidden);
   gemm_bf16(ctx, (int)m, kHidden, kHidden, dZ2, kHidden, 1, a.W2b, 1, kHidden, dH1, kHidden, 1);
   uint16_t* dZ1 = ctx.buf<uint16_t>("nn.dz1", m * kHidden);
-  k_back_relu<<<blocks_for(m, kRowsPerBlock), kHidden, 0, st>>>(dH1, a.Z1, m, G + kOffB1, dZ1);
+  k_back_relu<<<blocks_for(m, kRowsPerBlock), kHidden, 0, st>>>(dH1, a.Z1, m, G + L.b1, dZ1);
   // gW1 = X^T dZ1
-  gemm_bf16(ctx, kFeatDim, kHidden, (int)m, X, 1, kFeatDim, dZ1, kHidden, 1, G + kOffW1, kHidden, split);
+  gemm_bf16(ctx, fdim, kHidden, (int)m, X, 1, fdim, dZ1, kHidden, 1, G + L.w1, kHidden, split);
   CPSZ_HIP(hipGetLastError());
+}
+
+void train_samples(Ctx& ctx, const uint16_t* Xall, const float* Tall, size_t S, int fdim, int epochs, int batch,
+                   float lr, uint64_t& rs, float* params, float* loss_per_epoch) {
+  check_fdim(fdim);
+  CPSZ_REQUIRE(epochs > 0 && batch >= 32 && batch <= (1 << 20), kErrArg, "train: bad arguments");
+  const size_t steps = S / batch;
+  CPSZ_REQUIRE(steps > 0, kErrArg, "train: fewer samples than one batch");
+  const NetLayout L = net_layout(fdim);
+  const size_t np = L.count;
+  hipStream_t st = ctx.stream();
+  float* P = ctx.buf<float>("tr.p", np);
+  float* G = ctx.buf<float>("tr.g", np);
+  float* M1 = ctx.buf<float>("tr.m1", np);
+  float* M2 = ctx.buf<float>("tr.m2", np);
+  float* loss = ctx.buf<float>("tr.loss", 1);
+  CPSZ_HIP(hipMemcpyAsync(P, params, np * 4, hipMemcpyHostToDevice, st));
+  CPSZ_HIP(hipMemsetAsync(M1, 0, np * 4, st));
+  CPSZ_HIP(hipMemsetAsync(M2, 0, np * 4, st));
+  uint16_t* Xb = ctx.buf<uint16_t>("tr.xb", (size_t)batch * fdim);
+  float* Tb = ctx.buf<float>("tr.tb", batch);
+  uint32_t* perm = ctx.buf<uint32_t>("tr.perm", S);
+  std::vector<uint32_t> hp(S);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  int t = 0;
+  for (int ep = 0; ep < epochs; ep++) {
+    for (size_t k = 0; k < S; k++) hp[k] = (uint32_t)k;
+    for (size_t k = S - 1; k > 0; k--) std::swap(hp[k], hp[splitmix(rs) % (k + 1)]);
+    CPSZ_HIP(hipMemcpyAsync(perm, hp.data(), S * 4, hipMemcpyHostToDevice, st));
+    for (size_t b = 0; b < steps; b++) {
+      k_gather_rows<<<blocks_for((size_t)batch * fdim), 256, 0, st>>>(Xall, Tall, perm + b * batch, batch, fdim,
+                                                                       Xb, Tb);
+      mlp_grad(ctx, P, Xb, Tb, batch, G, loss, fdim);
+      ++t;
+      const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
+      k_adam<<<blocks_for(np), 256, 0, st>>>(P, G, M1, M2, np, lr, b1, b2, eps, c1, c2);
+      CPSZ_HIP(hipGetLastError());
+      float hl = 0;
+      if (b + 1 == steps) {  // the epoch's last batch loss
+        CPSZ_HIP(hipMemcpyAsync(&hl, loss, 4, hipMemcpyDeviceToHost, st));
+        CPSZ_HIP(hipStreamSynchronize(st));
+        if (loss_per_epoch) loss_per_epoch[ep] = hl;
+      }
+    }
+    CPSZ_HIP(hipStreamSynchronize(st));  // the permutation buffer is rewritten next epoch
+  }
+  CPSZ_HIP(hipMemcpyAsync(params, P, np * 4, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
 }
 
 void train_2d(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb, size_t spf, int epochs,
@@ -439,8 +504,8 @@ void train_2d(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb
   uint16_t* Xall = ctx.buf<uint16_t>("tr.x", S * kFeatDim);
   float* Tall = ctx.buf<float>("tr.t", S);
   uint8_t* L8 = ctx.buf<uint8_t>("tr.l8", spf);
-  uint32_t* idx = ctx.buf<uint32_t>("tr.idx", std::max(spf, (size_t)batch));
-  uint32_t* hidx = (uint32_t*)ctx.host_buf("tr.idx", std::max(spf, (size_t)batch) * 4);
+  uint32_t* idx = ctx.buf<uint32_t>("tr.idx", spf);
+  uint32_t* hidx = (uint32_t*)ctx.host_buf("tr.idx", spf * 4);
   uint64_t rs = seed;
   for (size_t f = 0; f < fields.size(); f++) {
     const TrainField& F = fields[f];
@@ -456,45 +521,7 @@ void train_2d(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb
     k_u8_to_f32<<<blocks_for(spf), 256, 0, st>>>(L8, spf, Tall + f * spf);
     CPSZ_HIP(hipGetLastError());
   }
-  float* P = ctx.buf<float>("tr.p", kNumParams);
-  float* G = ctx.buf<float>("tr.g", kNumParams);
-  float* M1 = ctx.buf<float>("tr.m1", kNumParams);
-  float* M2 = ctx.buf<float>("tr.m2", kNumParams);
-  float* loss = ctx.buf<float>("tr.loss", 1);
-  CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, st));
-  CPSZ_HIP(hipMemsetAsync(M1, 0, kNumParams * 4, st));
-  CPSZ_HIP(hipMemsetAsync(M2, 0, kNumParams * 4, st));
-  uint16_t* Xb = ctx.buf<uint16_t>("tr.xb", (size_t)batch * kFeatDim);
-  float* Tb = ctx.buf<float>("tr.tb", batch);
-  uint32_t* perm = ctx.buf<uint32_t>("tr.perm", S);
-  std::vector<uint32_t> hp(S);
-  const size_t steps = S / batch;
-  CPSZ_REQUIRE(steps > 0, kErrArg, "train_2d: fewer samples than one batch");
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  int t = 0;
-  for (int ep = 0; ep < epochs; ep++) {
-    for (size_t k = 0; k < S; k++) hp[k] = (uint32_t)k;
-    for (size_t k = S - 1; k > 0; k--) std::swap(hp[k], hp[splitmix(rs) % (k + 1)]);
-    CPSZ_HIP(hipMemcpyAsync(perm, hp.data(), S * 4, hipMemcpyHostToDevice, st));
-    for (size_t b = 0; b < steps; b++) {
-      k_gather_rows<<<blocks_for((size_t)batch * kFeatDim), 256, 0, st>>>(Xall, Tall, perm + b * batch, batch,
-                                                                          Xb, Tb);
-      mlp_grad(ctx, P, Xb, Tb, batch, G, loss);
-      ++t;
-      const float c1 = 1.f - std::pow(b1, (float)t), c2 = 1.f - std::pow(b2, (float)t);
-      k_adam<<<blocks_for(kNumParams), 256, 0, st>>>(P, G, M1, M2, kNumParams, lr, b1, b2, eps, c1, c2);
-      CPSZ_HIP(hipGetLastError());
-      float hl = 0;
-      if (b + 1 == steps) {  // the epoch's last batch loss
-        CPSZ_HIP(hipMemcpyAsync(&hl, loss, 4, hipMemcpyDeviceToHost, st));
-        CPSZ_HIP(hipStreamSynchronize(st));
-        if (loss_per_epoch) loss_per_epoch[ep] = hl;
-      }
-    }
-    CPSZ_HIP(hipStreamSynchronize(st));  // the permutation buffer is rewritten next epoch
-  }
-  CPSZ_HIP(hipMemcpyAsync(params, P, kNumParams * 4, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
+  train_samples(ctx, Xall, Tall, S, kFeatDim, epochs, batch, lr, rs, params, loss_per_epoch);
 }
 
 namespace {
@@ -575,6 +602,364 @@ void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, 
     }
   }
   for (size_t v = 0; v < n; v++) stats.fallback_vertices += hf[v] && hg[v];
+}
+
+// =================================================================== 3D ====
+namespace {
+
+// 3x3x3 patch of u, then v, then w (di slowest, dk fastest; zero outside)
+// divided by the patch's largest magnitude, then log2(analytic / max) / 16
+// (-4 for a zero bound), zeros up to kFeatDim3. Vertex idx[q], or v0 + q.
+__global__ void k_feat3d(const float* __restrict__ U, const float* __restrict__ V,
+                         const float* __restrict__ W, const double* __restrict__ E, int r1, int r2, int r3,
+                         double max_pwr_eb, const uint32_t* __restrict__ idx, size_t v0, size_t m,
+                         uint16_t* __restrict__ X) {
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t v = idx ? idx[q] : v0 + q;
+  const size_t d0 = (size_t)r2 * r3;
+  const int i = (int)(v / d0);
+  const size_t rem = v - (size_t)i * d0;
+  const int j = (int)(rem / (size_t)r3), k = (int)(rem - (size_t)j * r3);
+  const float* src[3] = {U, V, W};
+  float pt[81];
+  float s = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; a++)
+#pragma unroll
+    for (int b = 0; b < 3; b++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int ii = i + a - 1, jj = j + b - 1, kk = k + c - 1;
+        const bool in = ii >= 0 && ii < r1 && jj >= 0 && jj < r2 && kk >= 0 && kk < r3;
+        const size_t o = in ? (size_t)ii * d0 + (size_t)jj * r3 + kk : 0;
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+          const float x = in ? src[p][o] : 0.f;
+          pt[p * 27 + a * 9 + b * 3 + c] = x;
+          s = fmaxf(s, fabsf(x));
+        }
+      }
+  if (!(s > 0.f)) s = 1.f;
+  uint16_t* x = X + q * kFeatDim3;
+#pragma unroll
+  for (int t = 0; t < 81; t++) x[t] = f2bf(pt[t] / s);
+  const double e = E[v];
+  const float le = e > 0 ? fmaxf((float)log2(e / max_pwr_eb), -64.f) / 16.f : -4.f;
+  x[81] = f2bf(le);
+  for (int t = kFeatUsed3; t < kFeatDim3; t++) x[t] = 0;
+}
+
+// Bisection labels over the incident tetrahedra. Slot s = 6 c + t: tetrahedron
+// t (judge order) of the cell whose corner c (di << 2 | dj << 1 | dk) is the
+// vertex; incident when c is one of its corners {0, a(t), b(t), 7} and the
+// cell lies inside the grid. Base types are kept as type + 1 in 4-bit fields.
+__global__ void __launch_bounds__(128) k_labels3d(const float* __restrict__ U, const float* __restrict__ V,
+                                                  const float* __restrict__ W, const double* __restrict__ E,
+                                                  int r1, int r2, int r3, double max_pwr_eb,
+                                                  const uint32_t* __restrict__ idx, size_t m,
+                                                  uint8_t* __restrict__ L) {
+  const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (q >= m) return;
+  const size_t v = idx ? idx[q] : q;
+  const double e = E[v];
+  if (!(e > 0)) {
+    L[q] = 0;
+    return;
+  }
+  const size_t d0 = (size_t)r2 * r3, d1 = r3;
+  const int i = (int)(v / d0);
+  const size_t rem = v - (size_t)i * d0;
+  const int j = (int)(rem / d1), k = (int)(rem - (size_t)j * d1);
+  // value of tetrahedron slot s, with the vertex itself replaced by xv when set
+  auto eval = [&](int sl, const float* xv) {
+    const int c = sl / 6, t = sl - 6 * (sl / 6);
+    const int ca = judge_tet_a(t), cb = judge_tet_b(t);
+    const size_t org = v - (size_t)(c >> 2) * d0 - (size_t)((c >> 1) & 1) * d1 - (size_t)(c & 1);
+    const int cs[4] = {0, ca, cb, 7};
+    float f[4][3];
+    for (int p = 0; p < 4; p++) {
+      const int cp = cs[p];
+      if (xv && cp == c) {
+        f[p][0] = xv[0];
+        f[p][1] = xv[1];
+        f[p][2] = xv[2];
+      } else {
+        const size_t o = org + (size_t)(cp >> 2) * d0 + (size_t)((cp >> 1) & 1) * d1 + (size_t)(cp & 1);
+        f[p][0] = U[o];
+        f[p][1] = V[o];
+        f[p][2] = W[o];
+      }
+    }
+    return tet_cp(f, ca, cb);
+  };
+  uint64_t inc = 0, bt0 = 0, bt1 = 0, bt2 = 0;
+  for (int sl = 0; sl < 48; sl++) {
+    const int c = sl / 6, t = sl - 6 * (sl / 6);
+    const int ci = i - (c >> 2), cj = j - ((c >> 1) & 1), ck = k - (c & 1);
+    if (ci < 0 || ci >= r1 - 1 || cj < 0 || cj >= r2 - 1 || ck < 0 || ck >= r3 - 1) continue;
+    if (!(c == 0 || c == 7 || c == judge_tet_a(t) || c == judge_tet_b(t))) continue;
+    inc |= 1ull << sl;
+    const uint64_t ty = (uint64_t)(eval(sl, nullptr) + 1) << (4 * (sl & 15));
+    if (sl < 16) bt0 |= ty;
+    else if (sl < 32) bt1 |= ty;
+    else bt2 |= ty;
+  }
+  const float x0[3] = {U[v], V[v], W[v]};
+  int gmax = 0;
+  while (gmax < kMaxGain && ldexp(e, gmax) < max_pwr_eb) gmax++;
+  auto ok = [&](int g) {
+    const double up = 1.0 + fmin(max_pwr_eb, ldexp(e, g));
+    for (int corner = 0; corner < 8; corner++) {
+      float xv[3];
+      for (int p = 0; p < 3; p++)
+        xv[p] = ((corner >> p) & 1) ? (float)((double)x0[p] * up) : (float)((double)x0[p] / up);
+      for (uint64_t mm = inc; mm; mm &= mm - 1) {
+        const int sl = __builtin_ctzll(mm);
+        const uint64_t wd = sl < 16 ? bt0 : (sl < 32 ? bt1 : bt2);
+        const int base = (int)((wd >> (4 * (sl & 15))) & 15u);
+        if (eval(sl, xv) + 1 != base) return false;
+      }
+    }
+    return true;
+  };
+  int lo = 0, hi = gmax;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (ok(mid)) lo = mid;
+    else hi = mid - 1;
+  }
+  L[q] = (uint8_t)lo;
+}
+
+void check_field_3d(const float* dU, const float* dV, const float* dW, size_t r1, size_t r2, size_t r3,
+                    const char* who) {
+  CPSZ_REQUIRE(dU && dV && dW && r1 && r2 && r3 && r1 * r2 * r3 < (1ull << 32) && r1 < (1u << 20) &&
+                   r2 < (1u << 20) && r3 < (1u << 20),
+               kErrArg, std::string(who) + ": bad arguments");
+}
+
+void features_3d_range(Ctx& ctx, const float* dU, const float* dV, const float* dW, const double* dE, size_t r1,
+                       size_t r2, size_t r3, double max_pwr_eb, const uint32_t* d_idx, size_t v0, size_t m,
+                       uint16_t* dX) {
+  if (!m) return;
+  k_feat3d<<<blocks_for(m), 256, 0, ctx.stream()>>>(dU, dV, dW, dE, (int)r1, (int)r2, (int)r3, max_pwr_eb, d_idx,
+                                                    v0, m, dX);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void cp_list_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1, size_t r2, size_t r3,
+                std::vector<int64_t>& ids, std::vector<int32_t>& types) {
+  size_t cap = 1 << 16, count = 0;
+  for (;;) {
+    ids.resize(cap);
+    types.resize(cap);
+    critical_points_3d(ctx, dU, dV, dW, r1, r2, r3, ids.data(), types.data(), cap, &count);
+    if (count <= cap) break;
+    cap = count;
+  }
+  ids.resize(count);
+  types.resize(count);
+}
+
+// simplices whose critical point appeared, vanished or changed type
+std::vector<int64_t> cp_diff(const std::vector<int64_t>& id0, const std::vector<int32_t>& ty0,
+                             const std::vector<int64_t>& id1, const std::vector<int32_t>& ty1) {
+  std::vector<int64_t> bad;
+  size_t a = 0, b = 0;
+  while (a < id0.size() || b < id1.size()) {
+    if (b == id1.size() || (a < id0.size() && id0[a] < id1[b])) bad.push_back(id0[a++]);
+    else if (a == id0.size() || id1[b] < id0[a]) bad.push_back(id1[b++]);
+    else {
+      if (ty0[a] != ty1[b]) bad.push_back(id0[a]);
+      a++;
+      b++;
+    }
+  }
+  return bad;
+}
+
+}  // namespace
+
+void analytic_eb_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1, size_t r2,
+                    size_t r3, double max_pwr_eb, double* dE) {
+  eb_3d_device(ctx, dU, dV, dW, r1, r2, r3, max_pwr_eb, dE);
+}
+
+void features_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, const double* dE, size_t r1,
+                 size_t r2, size_t r3, double max_pwr_eb, const uint32_t* d_idx, size_t m, uint16_t* dX) {
+  check_field_3d(dU, dV, dW, r1, r2, r3, "features_3d");
+  CPSZ_REQUIRE(dE && dX, kErrArg, "features_3d: bad arguments");
+  CPSZ_REQUIRE(d_idx || m <= r1 * r2 * r3, kErrArg, "features_3d: more rows than vertices");
+  features_3d_range(ctx, dU, dV, dW, dE, r1, r2, r3, max_pwr_eb, d_idx, 0, m, dX);
+}
+
+void labels_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, const double* dE, size_t r1,
+               size_t r2, size_t r3, double max_pwr_eb, const uint32_t* d_idx, size_t m, uint8_t* dL) {
+  check_field_3d(dU, dV, dW, r1, r2, r3, "labels_3d");
+  CPSZ_REQUIRE(dE && dL, kErrArg, "labels_3d: bad arguments");
+  CPSZ_REQUIRE(d_idx || m <= r1 * r2 * r3, kErrArg, "labels_3d: more rows than vertices");
+  if (!m) return;
+  k_labels3d<<<blocks_for(m, 128), 128, 0, ctx.stream()>>>(dU, dV, dW, dE, (int)r1, (int)r2, (int)r3, max_pwr_eb,
+                                                           d_idx, m, dL);
+  CPSZ_HIP(hipGetLastError());
+}
+
+void train_3d(Ctx& ctx, const std::vector<TrainField3>& fields, double max_pwr_eb, size_t spf, int epochs,
+              int batch, float lr, uint64_t seed, float* params, float* loss_per_epoch) {
+  CPSZ_REQUIRE(!fields.empty() && spf > 0 && epochs > 0 && batch >= 32 && batch <= (1 << 20), kErrArg,
+               "train_3d: bad arguments");
+  hipStream_t st = ctx.stream();
+  const size_t S = fields.size() * spf;
+  uint16_t* Xall = ctx.buf<uint16_t>("tr3.x", S * kFeatDim3);
+  float* Tall = ctx.buf<float>("tr3.t", S);
+  uint8_t* L8 = ctx.buf<uint8_t>("tr3.l8", spf);
+  uint32_t* idx = ctx.buf<uint32_t>("tr3.idx", spf);
+  uint32_t* hidx = (uint32_t*)ctx.host_buf("tr3.idx", spf * 4);
+  uint64_t rs = seed;
+  for (size_t f = 0; f < fields.size(); f++) {
+    const TrainField3& F = fields[f];
+    check_field_3d(F.dU, F.dV, F.dW, F.r1, F.r2, F.r3, "train_3d");
+    const size_t n = F.r1 * F.r2 * F.r3;
+    double* E = ctx.buf<double>("tr3.e", n);
+    analytic_eb_3d(ctx, F.dU, F.dV, F.dW, F.r1, F.r2, F.r3, max_pwr_eb, E);  // synchronizes
+    for (size_t k = 0; k < spf; k++) hidx[k] = (uint32_t)(splitmix(rs) % n);
+    CPSZ_HIP(hipMemcpyAsync(idx, hidx, spf * 4, hipMemcpyHostToDevice, st));
+    features_3d_range(ctx, F.dU, F.dV, F.dW, E, F.r1, F.r2, F.r3, max_pwr_eb, idx, 0, spf,
+                      Xall + f * spf * kFeatDim3);
+    labels_3d(ctx, F.dU, F.dV, F.dW, E, F.r1, F.r2, F.r3, max_pwr_eb, idx, spf, L8);
+    k_u8_to_f32<<<blocks_for(spf), 256, 0, st>>>(L8, spf, Tall + f * spf);
+    CPSZ_HIP(hipGetLastError());
+    CPSZ_HIP(hipStreamSynchronize(st));  // host index buffer reuse
+  }
+  train_samples(ctx, Xall, Tall, S, kFeatDim3, epochs, batch, lr, rs, params, loss_per_epoch);
+}
+
+void compress_3d_learned_batch(Ctx& ctx, const std::vector<TrainField3>& fields, double max_pwr_eb,
+                               const float* params, int workers, int grid_cap, const uint8_t*& base,
+                               std::vector<size_t>& off, std::vector<size_t>& size, LearnedStats& stats) {
+  const size_t nf = fields.size();
+  CPSZ_REQUIRE(nf > 0 && params, kErrArg, "compress_3d_learned_batch: bad arguments");
+  hipStream_t st = ctx.stream();
+  stats = LearnedStats();
+  std::vector<size_t> nofs(nf + 1, 0);
+  size_t maxn = 0;
+  for (size_t k = 0; k < nf; k++) {
+    const TrainField3& F = fields[k];
+    check_field_3d(F.dU, F.dV, F.dW, F.r1, F.r2, F.r3, "compress_3d_learned_batch");
+    const size_t n = F.r1 * F.r2 * F.r3;
+    nofs[k + 1] = nofs[k] + n;
+    maxn = std::max(maxn, n);
+  }
+  const size_t ntot = nofs[nf];
+  // ---- proposals: gain = clamp(floor(network output), 0, 24) per vertex
+  float* P = ctx.buf<float>("l3.p", kNumParams3);
+  CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams3 * 4, hipMemcpyHostToDevice, st));
+  uint8_t* gain = ctx.buf<uint8_t>("l3.gain", ntot);
+  uint8_t* fb = ctx.buf<uint8_t>("l3.fb", ntot);
+  double* E = ctx.buf<double>("l3.e", maxn);
+  const size_t chunk = std::min<size_t>(maxn, 1u << 21);
+  uint16_t* X = ctx.buf<uint16_t>("l3.x", chunk * kFeatDim3);
+  float* y = ctx.buf<float>("l3.y", chunk);
+  for (size_t k = 0; k < nf; k++) {
+    const TrainField3& F = fields[k];
+    const size_t n = nofs[k + 1] - nofs[k];
+    analytic_eb_3d(ctx, F.dU, F.dV, F.dW, F.r1, F.r2, F.r3, max_pwr_eb, E);
+    for (size_t v0 = 0; v0 < n; v0 += chunk) {
+      const size_t mm = std::min(chunk, n - v0);
+      features_3d_range(ctx, F.dU, F.dV, F.dW, E, F.r1, F.r2, F.r3, max_pwr_eb, nullptr, v0, mm, X);
+      mlp_forward(ctx, P, X, mm, y, kFeatDim3);
+      k_gain<<<blocks_for(mm), 256, 0, st>>>(y, mm, gain + nofs[k] + v0);
+      CPSZ_HIP(hipGetLastError());
+    }
+  }
+  std::vector<uint8_t> hg(ntot), hf(ntot, 0);
+  CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, ntot, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  for (size_t v = 0; v < ntot; v++) stats.boosted_vertices += hg[v] != 0;
+  // ---- critical points of the originals
+  std::vector<std::vector<int64_t>> id0(nf);
+  std::vector<std::vector<int32_t>> ty0(nf);
+  for (size_t k = 0; k < nf; k++) {
+    const TrainField3& F = fields[k];
+    cp_list_3d(ctx, F.dU, F.dV, F.dW, F.r1, F.r2, F.r3, id0[k], ty0[k]);
+  }
+  // ---- verify and fall back until every field keeps its critical points
+  float* dec = ctx.buf<float>("l3.dec", 3 * ntot);
+  std::vector<size_t> soff(nf + 1, 0);
+  for (size_t k = 0; k < nf; k++) soff[k + 1] = soff[k] + (((nofs[k + 1] - nofs[k]) * 32 + (4u << 20) + 255) & ~(size_t)255);
+  uint8_t* out = ctx.buf<uint8_t>("l3.out", soff[nf]);
+  off.assign(soff.begin(), soff.end() - 1);
+  size.assign(nf, 0);
+  std::vector<size_t> pending(nf);
+  for (size_t k = 0; k < nf; k++) pending[k] = k;
+  Batch2dConfig cfg;
+  cfg.workers = workers;
+  cfg.grid_cap = grid_cap;
+  const int kMaxIter = 16;
+  for (int it = 0; it <= kMaxIter && !pending.empty(); it++) {
+    if (it == kMaxIter)  // the plain method for whatever is left
+      for (size_t k : pending) std::fill(hf.begin() + nofs[k], hf.begin() + nofs[k + 1], 1);
+    std::vector<Field3d> sub;
+    std::vector<EbOverride> ov;
+    for (size_t k : pending) {
+      const TrainField3& F = fields[k];
+      sub.push_back(Field3d{F.dU, F.dV, F.dW, F.r1, F.r2, F.r3});
+      EbOverride o;
+      o.gain = gain + nofs[k];
+      o.fallback = fb + nofs[k];
+      ov.push_back(o);
+      CPSZ_HIP(hipMemcpyAsync(fb + nofs[k], hf.data() + nofs[k], nofs[k + 1] - nofs[k], hipMemcpyHostToDevice, st));
+    }
+    BatchOut bo;
+    compress_3d_batch(ctx, sub, max_pwr_eb, cfg, bo, &ov);
+    std::vector<const uint8_t*> ins;
+    std::vector<size_t> insz;
+    std::vector<Field3dOut> outs;
+    for (size_t q = 0; q < pending.size(); q++) {
+      const size_t k = pending[q];
+      const TrainField3& F = fields[k];
+      ins.push_back(bo.base + bo.off[q]);
+      insz.push_back(bo.size[q]);
+      float* d = dec + 3 * nofs[k];
+      const size_t n = nofs[k + 1] - nofs[k];
+      outs.push_back(Field3dOut{d, d + n, d + 2 * n, F.r1, F.r2, F.r3});
+    }
+    decompress_3d_batch(ctx, ins, insz, outs, cfg);
+    stats.iterations = it + 1;
+    std::vector<size_t> still;
+    for (size_t q = 0; q < pending.size(); q++) {
+      const size_t k = pending[q];
+      const TrainField3& F = fields[k];
+      std::vector<int64_t> id1;
+      std::vector<int32_t> ty1;
+      cp_list_3d(ctx, outs[q].dU, outs[q].dV, outs[q].dW, F.r1, F.r2, F.r3, id1, ty1);
+      const std::vector<int64_t> bad = cp_diff(id0[k], ty0[k], id1, ty1);
+      if (bad.empty()) {
+        CPSZ_REQUIRE(bo.size[q] <= soff[k + 1] - soff[k], kErrInternal, "learned stream bound exceeded");
+        CPSZ_HIP(hipMemcpyAsync(out + soff[k], bo.base + bo.off[q], bo.size[q], hipMemcpyDeviceToDevice, st));
+        size[k] = bo.size[q];
+        continue;
+      }
+      CPSZ_REQUIRE(it < kMaxIter, kErrInternal, "learned compression: the plain bound changed a critical point");
+      const size_t d0 = F.r2 * F.r3, d1 = F.r3;
+      for (int64_t sid : bad) {
+        const size_t org = (size_t)(sid / 6);
+        const int which = (int)(sid % 6);
+        int t = 0;
+        while (judge_tet_sid(t) != which) t++;
+        const int cs[4] = {0, judge_tet_a(t), judge_tet_b(t), 7};
+        for (int c : cs) hf[nofs[k] + org + (size_t)(c >> 2) * d0 + (size_t)((c >> 1) & 1) * d1 + (c & 1)] = 1;
+      }
+      still.push_back(k);
+    }
+    // the accepted streams leave the batch buffer before the next round rewrites it
+    CPSZ_HIP(hipStreamSynchronize(st));
+    pending.swap(still);
+  }
+  CPSZ_REQUIRE(pending.empty(), kErrInternal, "learned compression did not converge");
+  for (size_t v = 0; v < ntot; v++) stats.fallback_vertices += hf[v] && hg[v];
+  base = out;
 }
 
 }  // namespace cpsz

@@ -20,21 +20,40 @@
 
 namespace cpsz {
 
-constexpr int kFeatDim = 64;   // 51 used, zero padded
+constexpr int kFeatDim = 64;   // 2D: 51 used, zero padded
 constexpr int kFeatUsed = 51;  // 25 u + 25 v + log2(analytic / max)
+constexpr int kFeatDim3 = 96;  // 3D: 82 used, zero padded
+constexpr int kFeatUsed3 = 82; // 27 u + 27 v + 27 w + log2(analytic / max)
 constexpr int kHidden = 128;
 constexpr int kMaxGain = 24;
 
-// fp32 parameter block: W1[64][128], b1[128], W2[128][128], b2[128], w3[128], b3
-constexpr size_t kOffW1 = 0;
-constexpr size_t kOffB1 = kOffW1 + (size_t)kFeatDim * kHidden;
-constexpr size_t kOffW2 = kOffB1 + kHidden;
-constexpr size_t kOffB2 = kOffW2 + (size_t)kHidden * kHidden;
-constexpr size_t kOffW3 = kOffB2 + kHidden;
-constexpr size_t kOffB3 = kOffW3 + kHidden;
-constexpr size_t kNumParams = kOffB3 + 1;
+// fp32 parameter block for feature width F: W1[F][128], b1[128],
+// W2[128][128], b2[128], w3[128], b3
+struct NetLayout {
+  int fdim;
+  size_t w1, b1, w2, b2, w3, b3, count;
+};
+constexpr NetLayout net_layout(int fdim) {
+  return NetLayout{fdim,
+                   0,
+                   (size_t)fdim * kHidden,
+                   (size_t)fdim * kHidden + kHidden,
+                   (size_t)fdim * kHidden + kHidden + (size_t)kHidden * kHidden,
+                   (size_t)fdim * kHidden + 2 * kHidden + (size_t)kHidden * kHidden,
+                   (size_t)fdim * kHidden + 3 * kHidden + (size_t)kHidden * kHidden,
+                   (size_t)fdim * kHidden + 3 * kHidden + (size_t)kHidden * kHidden + 1};
+}
+constexpr size_t kOffW1 = net_layout(kFeatDim).w1;
+constexpr size_t kOffB1 = net_layout(kFeatDim).b1;
+constexpr size_t kOffW2 = net_layout(kFeatDim).w2;
+constexpr size_t kOffB2 = net_layout(kFeatDim).b2;
+constexpr size_t kOffW3 = net_layout(kFeatDim).w3;
+constexpr size_t kOffB3 = net_layout(kFeatDim).b3;
+constexpr size_t kNumParams = net_layout(kFeatDim).count;
+constexpr size_t kNumParams3 = net_layout(kFeatDim3).count;
 
-void mlp_init_params(uint64_t seed, float* params);
+// He-uniform weights (fused inputs of fdim), zero biases and padding rows.
+void mlp_init_params(uint64_t seed, float* params, int fdim = kFeatDim, int fused = kFeatUsed);
 
 // Analytic per-vertex bound (the plain method's, before quantization).
 void analytic_eb_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,
@@ -54,10 +73,15 @@ void labels_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, siz
 void gemm_bf16(Ctx& ctx, int M, int N, int K, const uint16_t* A, long sam, long sak, const uint16_t* B,
                long sbk, long sbn, float* C, int ldc, int split);
 
-void mlp_forward(Ctx& ctx, const float* d_params, const uint16_t* dX, size_t m, float* dY);
+// dX: m rows of fdim BF16 features; d_params in the layout of net_layout(fdim).
+void mlp_forward(Ctx& ctx, const float* d_params, const uint16_t* dX, size_t m, float* dY, int fdim = kFeatDim);
 // Mean squared error of the network output against dT and its gradient.
 void mlp_grad(Ctx& ctx, const float* d_params, const uint16_t* dX, const float* dT, size_t m, float* d_grad,
-              float* d_loss);
+              float* d_loss, int fdim = kFeatDim);
+// Adam on S samples (features Xall, targets Tall) in shuffled batches;
+// params (host) is the start point and receives the result.
+void train_samples(Ctx& ctx, const uint16_t* Xall, const float* Tall, size_t S, int fdim, int epochs, int batch,
+                   float lr, uint64_t& rng, float* params, float* loss_per_epoch);
 
 struct TrainField {
   const float* dU;
@@ -74,5 +98,29 @@ struct LearnedStats {
 };
 void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
                          double max_pwr_eb, const float* params, DeviceStream& out, LearnedStats& st);
+
+// ---- 3D (learn3d.hip): 3x3x3 patch of (u, v, w) plus the analytic bound;
+// labels keep every incident tetrahedron's critical point (24 per interior
+// vertex) when the vertex alone moves to any corner of its box
+// x * (1 + eb)^(+-1) per component (the offline_log codec keeps the sign and
+// |log2 x' - log2 x| < log2(1 + eb)).
+void analytic_eb_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1, size_t r2,
+                    size_t r3, double max_pwr_eb, double* dE);
+void features_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, const double* dE, size_t r1,
+                 size_t r2, size_t r3, double max_pwr_eb, const uint32_t* d_idx, size_t m, uint16_t* dX);
+void labels_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, const double* dE, size_t r1,
+               size_t r2, size_t r3, double max_pwr_eb, const uint32_t* d_idx, size_t m, uint8_t* dL);
+struct TrainField3 {
+  const float *dU, *dV, *dW;
+  size_t r1, r2, r3;
+};
+void train_3d(Ctx& ctx, const std::vector<TrainField3>& fields, double max_pwr_eb, size_t samples_per_field,
+              int epochs, int batch, float lr, uint64_t seed, float* params, float* loss_per_epoch);
+// Learned-bound compression of a batch of 3D fields (one persistent
+// wavefront launch per verification round); the streams are returned in a
+// library-owned device buffer like the batch codec's.
+void compress_3d_learned_batch(Ctx& ctx, const std::vector<TrainField3>& fields, double max_pwr_eb,
+                               const float* params, int workers, int grid_cap, const uint8_t*& base,
+                               std::vector<size_t>& off, std::vector<size_t>& size, LearnedStats& st);
 
 }  // namespace cpsz

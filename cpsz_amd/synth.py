@@ -135,6 +135,49 @@ def field_3d(r1: int, r2: int, r3: int, seed: int = 0, n_tubes: int = 32, step: 
     return u.astype(np.float32), v.astype(np.float32), w.astype(np.float32)
 
 
+def field_3d_device(r1: int, r2: int, r3: int, seed: int = 0, n_tubes: int = 32, step: int = 0, device="cuda"):
+    """field_3d's recipe evaluated on the GPU with torch in float64 (parameters
+    drawn on the host from the same PCG64 stream, noise from a seeded torch
+    generator), for timestep batches too large to generate on the host.
+    Returns three float32 device tensors. Deterministic per (shape, seed,
+    step, device type), not byte-identical to field_3d (different noise)."""
+    import torch
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A, B, C = rng.uniform(0.5, 1.5, size=3)
+    f64 = dict(dtype=torch.float64, device=device)
+    i = torch.arange(r1, **f64)[:, None, None]
+    j = torch.arange(r2, **f64)[None, :, None]
+    k = torch.arange(r3, **f64)[None, None, :]
+    X = 4 * math.pi * k / r3
+    Y = 4 * math.pi * j / r2
+    Z = 4 * math.pi * i / r1
+    shape = (r1, r2, r3)
+    u = (A * torch.sin(Z) + C * torch.cos(Y)).expand(shape).clone()
+    v = (B * torch.sin(X) + A * torch.cos(Z)).expand(shape).clone()
+    w = (C * torch.sin(Y) + B * torch.cos(X)).expand(shape).clone()
+    drift = rng.normal(size=3)
+    drift /= np.linalg.norm(drift)
+    for _ in range(n_tubes):
+        a = rng.normal(size=3)
+        a /= np.linalg.norm(a)
+        c = rng.uniform(0, 1, size=3) * np.array([r3, r2, r1]) + 2.0 * step * drift
+        R = rng.uniform(4, 24)
+        G = rng.uniform(-1, 1)
+        px, py, pz = k - float(c[0]), j - float(c[1]), i - float(c[2])
+        dot = px * float(a[0]) + py * float(a[1]) + pz * float(a[2])
+        d2 = px * px + py * py + pz * pz - dot * dot
+        g = G * torch.exp(-d2 / (2 * R * R)) / R
+        u += g * (float(a[1]) * pz - float(a[2]) * py)
+        v += g * (float(a[2]) * px - float(a[0]) * pz)
+        w += g * (float(a[0]) * py - float(a[1]) * px)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(seed * 1000003 + step)
+    for comp in (u, v, w):
+        comp += 1e-3 * torch.randn(shape, generator=gen, **f64)
+    return u.float().contiguous(), v.float().contiguous(), w.float().contiguous()
+
+
 def batch_shapes_2d(count: int = 96, seed: int = 0, lo: int = 256, hi: int = 4096):
     """(h, w) pairs drawn log-uniform in [lo, hi] (the batched 2D workload)."""
     rng = np.random.Generator(np.random.PCG64(seed))

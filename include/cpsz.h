@@ -171,6 +171,23 @@ int cpsz_decompress_2d_batch_dev(int device, void* stream, size_t nfields,
 int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const size_t* r2,
                          int* workers, int* grid_cap);
 
+/* ---- batched 3D codec ----------------------------------------------------
+ * Many 3D fields (timesteps, ensemble members; sizes may differ) per call, no
+ * counterpart in the reference. Each field's stream is byte-identical to
+ * cpsz_compress_3d_f32_dev's. One persistent wavefront launch covers every
+ * field; the entropy stages run concurrently on worker streams. Outputs and
+ * tuning as for the batched 2D codec. */
+int cpsz_compress_3d_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                               const float* const* dV, const float* const* dW, const size_t* r1,
+                               const size_t* r2, const size_t* r3, double max_pwr_eb, int tune,
+                               const unsigned char** d_base, size_t* offsets, size_t* sizes);
+int cpsz_decompress_3d_batch_dev(int device, void* stream, size_t nfields,
+                                 const unsigned char* const* d_in, const size_t* in_size,
+                                 const size_t* r1, const size_t* r2, const size_t* r3, float* const* dU,
+                                 float* const* dV, float* const* dW);
+int cpsz_batch_3d_config(int device, size_t nfields, const size_t* r1, const size_t* r2, const size_t* r3,
+                         int* workers, int* grid_cap);
+
 /* ---- learned error-bound proposer (2D) -----------------------------------
  * No counterpart in the reference (its closest precedent is the
  * verify-and-halve loop of sz_compress_cp_preserve_2d_online_abs_relax_FN,
@@ -227,6 +244,50 @@ int cpsz_compress_2d_learned_f32_dev(int device, void* stream, const float* dU, 
 int cpsz_compress_2d_learned_f32(const float* U, const float* V, size_t r1, size_t r2,
                                  double max_pwr_eb, const float* params, unsigned char** out,
                                  size_t* out_size);
+
+/* ---- learned error-bound proposer (3D) -----------------------------------
+ * Same design as the 2D proposer, no counterpart in the reference: an
+ * 82-128-128-1 MLP over the 3x3x3 patch of (u, v, w) (divided by the patch's
+ * largest magnitude) plus log2(analytic / max_pwr_eb) / 16, BF16 on MFMA with
+ * FP32 accumulation. Labels: the largest gain (0..24) that keeps every
+ * incident tetrahedron's critical point when the vertex alone moves to any
+ * corner of its box x * (1 + eb)^(+-1). Parameters: W1[96][128] (rows 82..95
+ * zero), b1, W2[128][128], b2, w3[128], b3. */
+size_t cpsz_learn_param_count_3d(void);
+int cpsz_learn_init_params_3d(uint64_t seed, float* params);
+int cpsz_learn_analytic_eb_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                                  size_t r1, size_t r2, size_t r3, double max_pwr_eb, double* dE);
+/* 96 BF16 features per vertex (82 used) of vertices d_idx[0..m) (all when d_idx is NULL). */
+int cpsz_learn_features_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                               const double* dE, size_t r1, size_t r2, size_t r3, double max_pwr_eb,
+                               const uint32_t* d_idx, size_t m, uint16_t* dX);
+int cpsz_learn_labels_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                             const double* dE, size_t r1, size_t r2, size_t r3, double max_pwr_eb,
+                             const uint32_t* d_idx, size_t m, uint8_t* dL);
+int cpsz_mlp_forward_3d_dev(int device, void* stream, const float* params, const uint16_t* dX, size_t m,
+                            float* dY);
+int cpsz_mlp_grad_3d_dev(int device, void* stream, const float* params, const uint16_t* dX, const float* dT,
+                         size_t m, float* grad, float* loss);
+int cpsz_learn_train_3d_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                            const float* const* dV, const float* const* dW, const size_t* r1, const size_t* r2,
+                            const size_t* r3, double max_pwr_eb, size_t samples_per_field, int epochs, int batch,
+                            float lr, uint64_t seed, float* params, float* loss_per_epoch);
+/* Learned-bound compression of a batch of 3D fields: every verification
+ * round compresses the fields still pending in one batched call (one
+ * persistent wavefront launch), decodes them in one batched call and judges
+ * them exactly; the vertices of tetrahedra whose critical point changed fall
+ * back to the analytic bound. Streams (decodable by the plain 3D decoder, with
+ * the originals' critical points) are returned like the batch codec's. */
+int cpsz_compress_3d_learned_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                                       const float* const* dV, const float* const* dW, const size_t* r1,
+                                       const size_t* r2, const size_t* r3, double max_pwr_eb,
+                                       const float* params, int tune, const unsigned char** d_base,
+                                       size_t* offsets, size_t* sizes, int* iterations, size_t* boosted,
+                                       size_t* fallback);
+/* Host evaluation of the exact tetrahedron predicate the device judge uses:
+ * f = 4 vertex vectors (u, v, w) of tetrahedron t (0..5, the judge's order);
+ * *type = -1 (no critical point) or the type code. */
+int cpsz_tet_cp_emulate(const float* f, int t, int* type);
 
 /* ---- instrumentation ------------------------------------------------------ */
 /* Enables per-stage HIP-event timing for this thread's context on `device`. */

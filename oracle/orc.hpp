@@ -124,12 +124,18 @@ std::vector<CP> cp_extract_2d(const float* U, const float* V, size_t r1, size_t 
 // -1 if the triangle of vertex vectors f[p] = (u, v) holds no critical point,
 // else its type; lower selects the coordinates of the lower triangle.
 int tri_cp_2d(const float f[3][2], int lower);
+// -1 or the type of tetrahedron t (0..5, judge order) with vertex vectors f.
+int tet_cp_3d(const float f[4][3], int t);
 
 // Learned-bound proposer checkers (no reference counterpart; see orc_learn.cpp).
 void learn_features_2d(const float* U, const float* V, const double* E, size_t r1, size_t r2,
                        double max_pwr_eb, const uint32_t* idx, size_t m, uint16_t* X);
 void learn_labels_2d(const float* U, const float* V, const double* E, size_t r1, size_t r2,
                      double max_pwr_eb, const uint32_t* idx, size_t m, uint8_t* L);
+void learn_features_3d(const float* U, const float* V, const float* W, const double* E, size_t r1, size_t r2,
+                       size_t r3, double max_pwr_eb, const uint32_t* idx, size_t m, uint16_t* X);
+void learn_labels_3d(const float* U, const float* V, const float* W, const double* E, size_t r1, size_t r2,
+                     size_t r3, double max_pwr_eb, const uint32_t* idx, size_t m, uint8_t* L);
 std::vector<CP> cp_extract_3d(const float* U, const float* V, const float* W, size_t r1,
                               size_t r2, size_t r3);
 

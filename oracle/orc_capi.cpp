@@ -150,6 +150,25 @@ int orc_learn_labels_2d(const float* U, const float* V, const double* E, size_t 
   return 0;
 }
 
+int orc_learn_features_3d(const float* U, const float* V, const float* W, const double* E, size_t r1, size_t r2,
+                          size_t r3, double max_pwr_eb, const uint32_t* idx, size_t m, uint16_t* X) {
+  learn_features_3d(U, V, W, E, r1, r2, r3, max_pwr_eb, idx, m, X);
+  return 0;
+}
+
+int orc_learn_labels_3d(const float* U, const float* V, const float* W, const double* E, size_t r1, size_t r2,
+                        size_t r3, double max_pwr_eb, const uint32_t* idx, size_t m, uint8_t* L) {
+  learn_labels_3d(U, V, W, E, r1, r2, r3, max_pwr_eb, idx, m, L);
+  return 0;
+}
+
+int orc_tet_cp_3d(const float* f12, int t) {
+  float f[4][3];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 3; q++) f[p][q] = f12[3 * p + q];
+  return tet_cp_3d(f, t);
+}
+
 long orc_cp_2d(const float* U, const float* V, size_t r1, size_t r2, int64_t* simplex, int* type,
                size_t cap) {
   auto cps = cp_extract_2d(U, V, r1, r2);

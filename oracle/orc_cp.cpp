@@ -230,6 +230,21 @@ int tri_cp_2d(const float f[3][2], int lower) {
   return type_2d(v, lower ? X2 : X1);
 }
 
+int tet_cp_3d(const float f[4][3], int t) {
+  // (vertex 1, vertex 2) of tetrahedron t as (di, dj, dk), the judge's order
+  // (test/cp_extraction_3d.cpp:127-209)
+  static const int A[6][3] = {{1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 0, 1}};
+  static const int B[6][3] = {{1, 1, 0}, {1, 1, 0}, {1, 0, 1}, {1, 0, 1}, {0, 1, 1}, {0, 1, 1}};
+  for (int p = 0; p < 4; p++)
+    if (f[p][0] == 0 && f[p][1] == 0 && f[p][2] == 0) return -1;
+  if (!zero_in_tet(f)) return -1;
+  const int X[4][3] = {{0, 0, 0}, {A[t][2], A[t][1], A[t][0]}, {B[t][2], B[t][1], B[t][0]}, {1, 1, 1}};
+  double v[4][3];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 3; q++) v[p][q] = f[p][q];
+  return type_3d(v, X);
+}
+
 std::vector<CP> cp_extract_2d(const float* U, const float* V, size_t r1, size_t r2) {
   std::vector<CP> out;
   const double X1[3][2] = {{0, 0}, {0, 1}, {1, 1}};

@@ -45,6 +45,10 @@ def load_oracle():
     lib.orc_cp_2d.argtypes = [_P, _P, _SZ, _SZ, _P, _P, _SZ]
     lib.orc_learn_features_2d.argtypes = [_P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P]
     lib.orc_learn_labels_2d.argtypes = [_P, _P, _P, _SZ, _SZ, C.c_double, _P, _SZ, _P]
+    lib.orc_learn_features_3d.argtypes = [_P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, _P, _SZ, _P]
+    lib.orc_learn_labels_3d.argtypes = [_P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, _P, _SZ, _P]
+    lib.orc_tet_cp_3d.argtypes = [_P, C.c_int]
+    lib.orc_tet_cp_3d.restype = C.c_int
     lib.orc_cp_2d.restype = C.c_long
     lib.orc_cp_3d.argtypes = [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ]
     lib.orc_cp_3d.restype = C.c_long
@@ -170,6 +174,30 @@ class Oracle:
         self.lib.orc_learn_labels_2d(_ptr(U), _ptr(V), _ptr(E), U.shape[0], U.shape[1], eb,
                                      None if idx_a is None else _ptr(idx_a), m, _ptr(L))
         return L
+
+    def learn_features_3d(self, U, V, W, E, eb, idx=None):
+        U, V, W = (np.ascontiguousarray(x, np.float32) for x in (U, V, W))
+        E = np.ascontiguousarray(E, np.float64)
+        m = U.size if idx is None else len(idx)
+        idx_a = None if idx is None else np.ascontiguousarray(idx, np.uint32)
+        X = np.zeros((m, 96), np.uint16)
+        self.lib.orc_learn_features_3d(_ptr(U), _ptr(V), _ptr(W), _ptr(E), *U.shape, eb,
+                                       None if idx_a is None else _ptr(idx_a), m, _ptr(X))
+        return X
+
+    def learn_labels_3d(self, U, V, W, E, eb, idx=None):
+        U, V, W = (np.ascontiguousarray(x, np.float32) for x in (U, V, W))
+        E = np.ascontiguousarray(E, np.float64)
+        m = U.size if idx is None else len(idx)
+        idx_a = None if idx is None else np.ascontiguousarray(idx, np.uint32)
+        L = np.zeros(m, np.uint8)
+        self.lib.orc_learn_labels_3d(_ptr(U), _ptr(V), _ptr(W), _ptr(E), *U.shape, eb,
+                                     None if idx_a is None else _ptr(idx_a), m, _ptr(L))
+        return L
+
+    def tet_cp_3d(self, f, t):
+        f = np.ascontiguousarray(f, np.float32).reshape(12)
+        return self.lib.orc_tet_cp_3d(_ptr(f), t)
 
     def cp_2d(self, U, V):
         r1, r2 = U.shape

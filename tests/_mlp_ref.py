@@ -29,10 +29,24 @@ def rbf(x):
     return bf16_to_f32(bf16_bits(x)).astype(np.float64)
 
 
+F3 = 96  # the 3D network's feature width (82 used)
+
+
+def layout(n_params):
+    """Offsets (w1, b1, w2, b2, w3, b3) and feature width of a parameter block."""
+    f = (n_params - 1 - 3 * H - H * H) // H
+    assert f * H + 3 * H + H * H + 1 == n_params
+    w1, b1 = 0, f * H
+    w2 = b1 + H
+    b2 = w2 + H * H
+    w3 = b2 + H
+    return (w1, b1, w2, b2, w3, w3 + H), f
+
+
 def unpack(p):
     p = np.asarray(p, np.float64)
-    return (p[OFF_W1:OFF_B1].reshape(F, H), p[OFF_B1:OFF_W2], p[OFF_W2:OFF_B2].reshape(H, H),
-            p[OFF_B2:OFF_W3], p[OFF_W3:OFF_B3], p[OFF_B3])
+    (w1, b1, w2, b2, w3, b3), f = layout(p.size)
+    return (p[w1:b1].reshape(f, H), p[b1:w2], p[w2:b2].reshape(H, H), p[b2:w3], p[w3:b3], p[b3])
 
 
 def forward(p, Xbits, bf=True):
@@ -56,15 +70,16 @@ def grad(p, Xbits, T, bf=True):
     d = y - np.asarray(T, np.float64)
     loss = float(np.mean(d * d))
     dy = 2 * d / m
-    g = np.zeros(N_PARAMS)
-    g[OFF_W3:OFF_B3] = H2.T @ dy
-    g[OFF_B3] = dy.sum()
+    (ow1, ob1, ow2, ob2, ow3, ob3), _ = layout(np.size(p))
+    g = np.zeros(np.size(p))
+    g[ow3:ob3] = H2.T @ dy
+    g[ob3] = dy.sum()
     dZ2 = dy[:, None] * w3[None, :] * (Z2 > 0)
-    g[OFF_B2:OFF_W3] = dZ2.sum(0)
+    g[ob2:ow3] = dZ2.sum(0)
     dZ2r = r(dZ2)
-    g[OFF_W2:OFF_B2] = (H1.T @ dZ2r).ravel()
+    g[ow2:ob2] = (H1.T @ dZ2r).ravel()
     dH1 = dZ2r @ r(W2).T
     dZ1 = dH1 * (Z1 > 0)
-    g[OFF_B1:OFF_W2] = dZ1.sum(0)
-    g[OFF_W1:OFF_B1] = (X.T @ r(dZ1)).ravel()
+    g[ob1:ow2] = dZ1.sum(0)
+    g[ow1:ob1] = (X.T @ r(dZ1)).ravel()
     return g, loss
