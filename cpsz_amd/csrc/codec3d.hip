@@ -309,7 +309,8 @@ template <bool COMPRESS>
 __global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
   __shared__ float4 hand[kP3 - 1][2][kG3][3][64];
   __shared__ int s_item;
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // the wave index is uniform: tests on it become scalar branches
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (;;) {
     if (threadIdx.x == 0) s_item = atomicAdd(bt.ticket, 1);
     __syncthreads();
@@ -367,12 +368,16 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
     const int* prog_left = (const int*)(prog + (pv && b > 0 ? (size_t)i * nb + b - 1 : 0));
     int* prog_me = (int*)(prog + (pv ? (size_t)i * nb + b : 0));
     int seen_up = 0, seen_left = 0;
+    // counters read one iteration ahead (VGPR values, converted when used, so
+    // that their loads overlap the chunk's compute instead of stalling it)
+    int ahead_up = 0, ahead_left = 0;
     float Rp[3] = {0, 0, 0}, Bp[3] = {0, 0, 0}, Dp[3] = {0, 0, 0}, Fp[3] = {0, 0, 0};
 
     auto fetch = [&](Chunk3& X, int c) {
       const int t0 = c * kC3;
       if (w == 0 && i > 0) {
         const int need = min(t0 + kC3, T);
+        seen_up = max(seen_up, __builtin_amdgcn_readfirstlane(ahead_up));
         while (seen_up < need) {
           seen_up = __builtin_amdgcn_readfirstlane(ld_coh(prog_up));
           if (seen_up < need) __builtin_amdgcn_s_sleep(1);
@@ -380,6 +385,7 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
       }
       if (b > 0) {
         const int need = min(t0 + kC3 + 63, T);
+        seen_left = max(seen_left, __builtin_amdgcn_readfirstlane(ahead_left));
         while (seen_left < need) {
           seen_left = __builtin_amdgcn_readfirstlane(ld_coh(prog_left));
           if (seen_left < need) __builtin_amdgcn_s_sleep(1);
@@ -409,6 +415,8 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
           if (i > 0) X.bu[p] = ld_cohf((const float*)(R + p * per + lubase + o));
         }
       }
+      if (w == 0 && i > 0 && seen_up < T) ahead_up = ld_coh(prog_up);
+      if (b > 0 && seen_left < T) ahead_left = ld_coh(prog_left);
     };
 
     auto compute = [&](const Chunk3& X, int c) {
@@ -417,7 +425,10 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
 #pragma unroll
       for (int u = 0; u < kG3; u++)
 #pragma unroll
-        for (int p = 0; p < 3; p++) up[p][u] = w == 0 ? X.up[p][u] : hand[w > 0 ? w - 1 : 0][c & 1][u][p][lane];
+        for (int p = 0; p < 3; p++) {
+          const float4 h = hand[w > 0 ? w - 1 : 0][c & 1][u][p][lane];  // wave 0 reads and discards
+          up[p][u] = w == 0 ? X.up[p][u] : h;
+        }
 #pragma unroll
       for (int u = 0; u < kG3; u++) {
         float rr[3][4];
@@ -447,21 +458,23 @@ __global__ void __launch_bounds__(64 * kP3) k_wave3d(WaveBatch3d bt) {
             } else {
               const uint32_t qw = (s4 < 2) ? X.q[p][u].x : X.q[p][u].y;
               const int qv = (int)((qw >> (16 * (s4 & 1))) & 0xFFFFu);
-              res[p] = (code == 0 || code == 24) ? xv
-                                                 : pred + ldexpf((float)(2 * (qv - 32768)), (int)code - 23);
+              const float rec = pred + ldexpf((float)(2 * (qv - 32768)), (int)code - 23);
+              res[p] = ((code == 0u) | (code == 24u)) ? xv : rec;
             }
             Bp[p] = Bv;
             Fp[p] = Fv;
             Dp[p] = Dv;
           }
           if (COMPRESS) {
-            const bool pred_ok = (code != 255) & (code != 0) & okall;
+            // codes 1..254 are predictable candidates (255: zero bound, 0: lossless)
+            const bool pred_ok = ((code - 1u) < 254u) & okall;
 #pragma unroll
             for (int p = 0; p < 3; p++) {
               res[p] = pred_ok ? dec[p] : comp4(X.x[p][u], s4);
               qq[p][s4 >> 1] |= (uint32_t)qc[p] << (16 * (s4 & 1));
             }
-            vv |= (code == 255 ? 0u : (pred_ok ? code : 24u)) << (8 * s4);
+            const uint32_t id = pred_ok ? code : 24u;
+            vv |= (code == 255u ? 0u : id) << (8 * s4);
           }
 #pragma unroll
           for (int p = 0; p < 3; p++) {
