@@ -24,10 +24,28 @@ __host__ __device__ inline float acbrt1_f(float x0) {
   return x;
 }
 
-// max_eb_to_keep_sign_3d_offline (:186-191)
+// acbrt1_f for arguments with a clear sign bit (sums of magnitudes): the
+// integer steps on a non-negative bit pattern are exact as unsigned shifts,
+// which cost one instruction where signed division by 4, 16, 256 costs three.
+__host__ __device__ inline float acbrt1_nonneg_f(float x0) {
+  uint32_t ix;
+  __builtin_memcpy(&ix, &x0, 4);
+  ix = (ix >> 2) + (ix >> 4);
+  ix = ix + (ix >> 4);
+  ix = ix + (ix >> 8);
+  ix = 0x2a5137a0u + ix;
+  float x;
+  __builtin_memcpy(&x, &ix, 4);
+  x = 0.33333333f * (2.0f * x + x0 / (x * x));
+  x = 0.33333333f * (2.0f * x + x0 / (x * x));
+  return x;
+}
+
+// max_eb_to_keep_sign_3d_offline (:186-191); p and n are sums of magnitudes
+// (never negative, never -0: the sums start from +0)
 __host__ __device__ inline float cube_sign_bound(float p, float n) {
   if (p == 0 && n == 0) return 0;
-  const float P = acbrt1_f(p), N = acbrt1_f(n);
+  const float P = acbrt1_nonneg_f(p), N = acbrt1_nonneg_f(n);
   return fabsf(P - N) / (P + N);
 }
 
