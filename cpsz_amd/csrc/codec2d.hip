@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "codec2d.hpp"
+#include "stream_reader.hpp"
 #include "huff_gpu.hpp"
 #include "lorenzo2d_step.hpp"
 
@@ -830,30 +831,6 @@ void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, s
 
 namespace {
 // Reads stream bytes either from a host copy or from device memory.
-struct StreamReader {
-  Ctx& ctx;
-  const uint8_t* host;  // may be null
-  const uint8_t* dev;
-  size_t size;
-  std::vector<uint8_t> tmp;
-  const uint8_t* read(size_t off, size_t len) {
-    CPSZ_REQUIRE(off <= size && len <= size - off, kErrStream, "stream truncated");
-    if (host) return host + off;
-    tmp.resize(len);
-    if (len) {
-      CPSZ_HIP(hipMemcpyAsync(tmp.data(), dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
-      CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
-    }
-    return tmp.data();
-  }
-  template <typename V>
-  V get(size_t off) {
-    V v;
-    std::memcpy(&v, read(off, sizeof(V)), sizeof(V));
-    return v;
-  }
-};
-
 struct HuffSection {
   size_t node_count;
   uint32_t tree_size;
@@ -909,7 +886,7 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   uint16_t *sid_u = w.sid_u, *sid_v = w.sid_v, *qu = w.qu, *qv = w.qv;
   float* raw = w.raw;
   int* err = w.err;
-  StreamReader rd{ctx, h_in, d_in, in_size, {}};
+  StreamReader rd(ctx, h_in, d_in, in_size);
   const int base = rd.get<int>(0);
   const double thr = rd.get<double>(4);
   const size_t u_size = rd.get<size_t>(20);

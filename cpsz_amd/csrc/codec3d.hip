@@ -33,6 +33,7 @@
 #include "eb3d_math.hpp"
 #include "huff_gpu.hpp"
 #include "libm_math.hpp"
+#include "stream_reader.hpp"
 
 namespace cpsz {
 
@@ -1046,31 +1047,6 @@ void eb_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, s
 }
 
 // ----------------------------------------------------------- decompress ----
-namespace {
-struct Reader3 {
-  Ctx& ctx;
-  const uint8_t* host;
-  const uint8_t* dev;
-  size_t size;
-  std::vector<uint8_t> tmp;
-  const uint8_t* read(size_t off, size_t len) {
-    CPSZ_REQUIRE(off <= size && len <= size - off, kErrStream, "stream truncated");
-    if (host) return host + off;
-    tmp.resize(len);
-    if (len) {
-      CPSZ_HIP(hipMemcpyAsync(tmp.data(), dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
-      CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
-    }
-    return tmp.data();
-  }
-  template <typename V>
-  V get(size_t off) {
-    V v;
-    std::memcpy(&v, read(off, sizeof(V)), sizeof(V));
-    return v;
-  }
-};
-}  // namespace
 
 size_t d3_scratch_bytes(size_t r1, size_t r2, size_t r3) {
   const Skew3 sk = make_skew(r1, r2, r3);
@@ -1104,7 +1080,7 @@ void d3_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   const size_t n = w.r1 * w.r2 * w.r3;
   const Skew3 sk = make_skew(w.r1, w.r2, w.r3);
   const Ctx::LibmDev& lm = ctx.libm_dev();
-  Reader3 rd{ctx, h_in, d_in, in_size, {}};
+  StreamReader rd(ctx, h_in, d_in, in_size);
   const int base = rd.get<int>(0);
   const int radius = rd.get<int>(4);
   CPSZ_REQUIRE(base == 2 && radius == kRadius, kErrUnsupported,

@@ -1,0 +1,54 @@
+// Reader of a compressed stream held in device memory (or in host memory when
+// the caller has a host copy). Header fields sit close together, so reads go
+// through a cached 64 KiB window in pinned host memory: one device-to-host
+// copy serves every field inside it instead of one copy (and one stream
+// synchronization) per field. A returned pointer is valid until the next read.
+#pragma once
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "ctx.hpp"
+
+namespace cpsz {
+
+struct StreamReader {
+  static constexpr size_t kWindow = 64u << 10;
+  Ctx& ctx;
+  const uint8_t* host;  // may be null
+  const uint8_t* dev;
+  size_t size;
+  std::vector<uint8_t> tmp;
+  uint8_t* win = nullptr;
+  size_t win_off = 0, win_len = 0;
+
+  StreamReader(Ctx& c, const uint8_t* h, const uint8_t* d, size_t n) : ctx(c), host(h), dev(d), size(n) {}
+
+  const uint8_t* read(size_t off, size_t len) {
+    CPSZ_REQUIRE(off <= size && len <= size - off, kErrStream, "stream truncated");
+    if (host) return host + off;
+    if (win && off >= win_off && off + len <= win_off + win_len) return win + (off - win_off);
+    if (len <= kWindow) {
+      if (!win) win = (uint8_t*)ctx.host_buf("rd.window", kWindow);
+      win_off = off;
+      win_len = std::min(kWindow, size - off);
+      if (win_len) {
+        CPSZ_HIP(hipMemcpyAsync(win, dev + off, win_len, hipMemcpyDeviceToHost, ctx.stream()));
+        CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+      }
+      return win;
+    }
+    tmp.resize(len);
+    CPSZ_HIP(hipMemcpyAsync(tmp.data(), dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
+    CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+    return tmp.data();
+  }
+  template <typename V>
+  V get(size_t off) {
+    V v;
+    std::memcpy(&v, read(off, sizeof(V)), sizeof(V));
+    return v;
+  }
+};
+
+}  // namespace cpsz
