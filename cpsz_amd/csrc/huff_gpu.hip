@@ -26,6 +26,7 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
                                                   uint32_t* __restrict__ cnt, int* __restrict__ changed,
                                                   int* __restrict__ err, int first) {
   __shared__ uint32_t slut[1 << kLutBits];
+  __shared__ uint32_t sw[kStageWords];
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   bool work = i < nseg;
   uint64_t s0 = 0;
@@ -43,22 +44,60 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
   }
   if (!__syncthreads_or(work)) return;
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
+  uint64_t lbase;
+  const uint32_t lcount = stage_words(sw, w, nbits, S, blockIdx.x * blockDim.x, lbase);
   __syncthreads();
   if (!work) return;
   uint64_t end = (uint64_t)(i + 1) * S;
   if (end > nbits) end = nbits;
   BitReader br;
-  br.init(w, s0);
+  br.init(w, s0, sw, lbase, lcount);
   uint32_t c = 0;
-  while (br.pos < end) {
-    if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
-      *err = 1;
+  if (first) {
+    while (br.pos < end) {
+      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
+        *err = 1;
+        break;
+      }
+      c++;
+    }
+    start[i] = s0;
+    exitp[i] = br.pos;
+    cnt[i] = c;
+    return;
+  }
+  // New start: decode the new path and the previous pass's path of this
+  // segment in lockstep (always advancing the one behind) until they reach
+  // the same code boundary; from there both paths coincide, so the previous
+  // pass's exit and the rest of its count carry over. Prefix codes resync
+  // within a few code words, so a fix-up costs a few symbols, not a segment.
+  BitReader old;
+  old.init(w, prev_start[i], sw, lbase, lcount);
+  const uint64_t old_exit = prev_exit[i];
+  const uint32_t old_cnt = cnt[i];
+  uint32_t c_old = 0;
+  uint64_t ex = 0;
+  for (;;) {
+    if (br.pos >= end) {
+      ex = br.pos;
       break;
     }
-    c++;
+    if (br.pos == old.pos) {
+      ex = old_exit;
+      c += old_cnt - c_old;
+      break;
+    }
+    const bool adv_new = br.pos < old.pos;
+    if (dec_one(t, slut, adv_new ? br : old) == 0xFFFFFFFFu) {
+      *err = 1;
+      ex = br.pos;
+      break;
+    }
+    c += adv_new;
+    c_old += !adv_new;
   }
   start[i] = s0;
-  exitp[i] = br.pos;
+  exitp[i] = ex;
   cnt[i] = c;
   (void)changed;
 }
@@ -69,6 +108,34 @@ __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t
   if (i >= nseg) return;
   const uint64_t want = i == 0 ? 0 : exitp[i - 1];
   if (start[i] != want) *changed = 1;
+}
+
+// Level-2 decode tables: entry e of prefix p (pfx triples (node, K, off),
+// offsets ascending) walks the K bits of e - off down from node.
+__global__ void k_fill_lut2(const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
+                            const uint32_t* __restrict__ sym, const uint8_t* __restrict__ leaf,
+                            const uint32_t* __restrict__ pfx, uint32_t npfx, uint32_t n2,
+                            uint32_t* __restrict__ lut2) {
+  const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n2) return;
+  uint32_t lo = 0, hi = npfx - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (pfx[3 * mid + 2] <= e) lo = mid;
+    else hi = mid - 1;
+  }
+  uint32_t node = pfx[3 * lo];
+  const int K = (int)pfx[3 * lo + 1];
+  const uint32_t u = e - pfx[3 * lo + 2];
+  uint32_t out = 0;
+  for (int len = 0; len < K; len++) {
+    node = ((u >> (K - 1 - len)) & 1) ? right[node] : left[node];
+    if (leaf[node]) {
+      out = (sym[node] << 8) | (uint32_t)(len + 1);
+      break;
+    }
+  }
+  lut2[e] = out ? out : node << 8;
 }
 
 namespace {
@@ -146,25 +213,37 @@ DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, si
     out.single_sym = d.root_sym;
     return out;
   }
-  std::vector<uint32_t> lut;
-  huff_build_lut(d, kLutBits, lut);
+  std::vector<uint32_t> lut, pfx;
+  const size_t n2 = huff_build_lut(d, kLutBits, kLut2Bits, lut, pfx);
   const size_t N = d.node_count;
+  const uint32_t npfx = (uint32_t)(pfx.size() / 3);
   std::string s(tag);
-  const size_t lut_b = lut.size() * 4, arr_b = N * 4;
-  const size_t bytes = lut_b + 3 * arr_b + N;
+  const size_t lut_b = lut.size() * 4, pfx_b = pfx.size() * 4, arr_b = N * 4;
+  const size_t bytes = lut_b + pfx_b + 3 * arr_b + N;
   uint8_t* h = (uint8_t*)ctx.host_buf((s + ".dec").c_str(), bytes);
   std::memcpy(h, lut.data(), lut_b);
-  std::memcpy(h + lut_b, d.left.data(), arr_b);
-  std::memcpy(h + lut_b + arr_b, d.right.data(), arr_b);
-  std::memcpy(h + lut_b + 2 * arr_b, d.sym.data(), arr_b);
-  std::memcpy(h + lut_b + 3 * arr_b, d.leaf.data(), N);
+  if (pfx_b) std::memcpy(h + lut_b, pfx.data(), pfx_b);
+  uint8_t* q = h + lut_b + pfx_b;
+  std::memcpy(q, d.left.data(), arr_b);
+  std::memcpy(q + arr_b, d.right.data(), arr_b);
+  std::memcpy(q + 2 * arr_b, d.sym.data(), arr_b);
+  std::memcpy(q + 3 * arr_b, d.leaf.data(), N);
   uint8_t* dv = ctx.buf<uint8_t>((s + ".dec").c_str(), bytes);
   CPSZ_HIP(hipMemcpyAsync(dv, h, bytes, hipMemcpyHostToDevice, ctx.stream()));
   out.tree.lut = (const uint32_t*)dv;
-  out.tree.left = (const uint32_t*)(dv + lut_b);
-  out.tree.right = (const uint32_t*)(dv + lut_b + arr_b);
-  out.tree.sym = (const uint32_t*)(dv + lut_b + 2 * arr_b);
-  out.tree.leaf = dv + lut_b + 3 * arr_b;
+  const uint32_t* dpfx = (const uint32_t*)(dv + lut_b);
+  const uint8_t* dq = dv + lut_b + pfx_b;
+  out.tree.left = (const uint32_t*)dq;
+  out.tree.right = (const uint32_t*)(dq + arr_b);
+  out.tree.sym = (const uint32_t*)(dq + 2 * arr_b);
+  out.tree.leaf = dq + 3 * arr_b;
+  uint32_t* lut2 = ctx.buf<uint32_t>((s + ".lut2").c_str(), n2 + 1);
+  out.tree.lut2 = lut2;
+  if (n2) {
+    k_fill_lut2<<<(int)div_up(n2, 256), 256, 0, ctx.stream()>>>(out.tree.left, out.tree.right, out.tree.sym,
+                                                                 out.tree.leaf, dpfx, npfx, (uint32_t)n2, lut2);
+    CPSZ_HIP(hipGetLastError());
+  }
   out.tree.nodes = (uint32_t)N;
   // The pinned staging buffer is reused by the next decoder upload of the same
   // tag only after the stream has consumed it (callers synchronize per call).

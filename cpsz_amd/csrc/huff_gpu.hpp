@@ -26,6 +26,7 @@
 namespace cpsz {
 
 constexpr int kLutBits = 12;
+constexpr int kLut2Bits = 12;  // width cap of the level-2 decode tables
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
 
@@ -105,6 +106,7 @@ __global__ void k_scatter_codebook(const uint32_t* syms, const uint64_t* codes, 
 // ------------------------------------------------------------------ decode --
 struct DecTree {
   const uint32_t* lut;    // 2^kLutBits entries
+  const uint32_t* lut2;   // level-2 tables (huff_build_lut)
   const uint32_t* left;   // node arrays (preorder numbering)
   const uint32_t* right;
   const uint32_t* sym;
@@ -113,26 +115,44 @@ struct DecTree {
 };
 
 // MSB-first reader over big-endian 32-bit words; the word array needs at
-// least two zero words past the last bit.
+// least two zero words past the last bit. Words [lbase, lbase + lcount) are
+// also staged in LDS (lw) and read from there: a refill is then an LDS read,
+// not a global-memory round trip on the decoder's serial dependency chain.
+typedef __attribute__((address_space(3))) const uint32_t lds_u32;  // an LDS word
+
 struct BitReader {
   const uint32_t* w;
+  lds_u32* lw;
+  uint64_t lbase;
+  uint32_t lcount;
   uint64_t buf;   // upcoming bits, MSB-aligned
   int nb;         // valid bits in buf (> 32 after every refill)
   uint64_t widx;  // next word to load
   uint64_t pos;   // bit position of buf's MSB
-  __device__ __forceinline__ void init(const uint32_t* __restrict__ words, uint64_t p) {
+  __device__ __forceinline__ uint32_t word(uint64_t k) const {
+    const uint64_t d = k - lbase;
+    uint32_t x;
+    if (d < lcount) x = lw[d];
+    else x = w[k];
+    return bswap32(x);
+  }
+  __device__ __forceinline__ void init(const uint32_t* __restrict__ words, uint64_t p,
+                                       const uint32_t* staged = nullptr, uint64_t sbase = 0, uint32_t scount = 0) {
     w = words;
+    lw = (lds_u32*)staged;
+    lbase = sbase;
+    lcount = scount;
     pos = p;
     widx = p >> 5;
     const int s = (int)(p & 31);
-    buf = (((uint64_t)bswap32(w[widx]) << 32) | bswap32(w[widx + 1])) << s;
+    buf = (((uint64_t)word(widx) << 32) | word(widx + 1)) << s;
     nb = 64 - s;
     widx += 2;
     refill();
   }
   __device__ __forceinline__ void refill() {
     if (nb <= 32) {
-      buf |= (uint64_t)bswap32(w[widx++]) << (32 - nb);
+      buf |= (uint64_t)word(widx++) << (32 - nb);
       nb += 32;
     }
   }
@@ -145,18 +165,47 @@ struct BitReader {
   }
 };
 
+// A block's 256 segments of 256 bits span 2048 words; the few words past them
+// cover the last segment's overrun to its first boundary past the end.
+constexpr uint32_t kStageWords = 2048 + 8;
+
+// Stages the words of block segments [b0, b0 + 256) (segment size S bits) into
+// sw; returns how many were staged (0 unless S == 256). Callers synchronize.
+__device__ __forceinline__ uint32_t stage_words(uint32_t* sw, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                uint32_t S, uint32_t b0, uint64_t& lbase) {
+  lbase = (uint64_t)b0 * (S / 32);
+  if (S != 256) return 0;
+  const uint64_t avail = (nbits + 31) / 32 + 8;  // words allocated, zero padded
+  uint32_t cnt = kStageWords;
+  if (lbase + cnt > avail) cnt = avail > lbase ? (uint32_t)(avail - lbase) : 0u;
+  for (uint32_t k = threadIdx.x; k < cnt; k += blockDim.x) sw[k] = w[lbase + k];
+  return cnt;
+}
+
 // Decodes one symbol; returns it or 0xFFFFFFFF on a bad code. `lut` is the
-// 2^kLutBits-entry table (entry = sym << 8 | length, or node << 8 for codes
-// longer than kLutBits, which continue bit by bit down the tree).
+// level-1 table (in LDS): sym << 8 | length for short codes, off << 8 | 0x80 |
+// K for prefixes continued by a level-2 table (t.lut2, global memory) over the
+// next K bits; codes longer still continue bit by bit down the tree.
 __device__ __forceinline__ uint32_t dec_one(const DecTree& t, const uint32_t* lut, BitReader& br) {
   const uint32_t e = lut[br.peek() >> (32 - kLutBits)];
   const uint32_t L = e & 0xFF;
-  if (L) {
+  if (L && !(L & 0x80u)) {
     br.skip((int)L);
     return e >> 8;
   }
   uint32_t node = e >> 8;
   br.skip(kLutBits);
+  if (L & 0x80u) {
+    const int K = (int)(L & 0x7F);
+    const uint32_t e2 = t.lut2[node + (br.peek() >> (32 - K))];
+    const uint32_t L2 = e2 & 0xFF;
+    if (L2) {
+      br.skip((int)L2);
+      return e2 >> 8;
+    }
+    node = e2 >> 8;
+    br.skip(K);
+  }
   for (int d = 0; d < 64; d++) {
     const uint32_t b = br.peek() >> 31;
     br.skip(1);
@@ -178,18 +227,21 @@ __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t
                             int* changed);
 
 template <class Sink>
-__global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __restrict__ w,
-                                                   uint32_t nseg, const uint64_t* __restrict__ start,
+__global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                   uint32_t S, uint32_t nseg, const uint64_t* __restrict__ start,
                                                    const uint64_t* __restrict__ exitp,
                                                    const uint64_t* __restrict__ off, uint64_t n,
                                                    Sink sink) {
   __shared__ uint32_t slut[1 << kLutBits];
+  __shared__ uint32_t sw[kStageWords];
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
+  uint64_t lbase;
+  const uint32_t lcount = stage_words(sw, w, nbits, S, blockIdx.x * blockDim.x, lbase);
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nseg) return;
   BitReader br;
-  br.init(w, start[i]);
+  br.init(w, start[i], sw, lbase, lcount);
   const uint64_t end = exitp[i];
   uint64_t o = off[i];
   while (br.pos < end && o < n) sink.put(o++, dec_one(t, slut, br));
@@ -273,7 +325,7 @@ void decode_stream(Ctx& ctx, const DeviceDecoder& dd, const uint32_t* d_words, u
   CPSZ_HIP(hipMemcpyAsync(&total, off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
   CPSZ_REQUIRE(total >= n, kErrStream, "huffman bitstream holds fewer symbols than expected");
-  k_dec_write<<<blocks, 256, 0, st>>>(dd.tree, d_words, nseg, start[cur], ex[cur], off, n, sink);
+  k_dec_write<<<blocks, 256, 0, st>>>(dd.tree, d_words, nbits, S, nseg, start[cur], ex[cur], off, n, sink);
   CPSZ_HIP(hipGetLastError());
 }
 

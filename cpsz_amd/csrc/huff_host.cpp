@@ -1,5 +1,6 @@
 #include "huff_host.hpp"
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -198,23 +199,38 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecode
   return true;
 }
 
-void huff_build_lut(const HuffDecodeTree& d, int bits, std::vector<uint32_t>& lut) {
+size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<uint32_t>& lut,
+                      std::vector<uint32_t>& pfx) {
+  const size_t N = d.node_count;
+  // subtree heights: children follow their parent in preorder numbering
+  std::vector<uint8_t> h(N, 0);
+  for (size_t i = N; i-- > 0;)
+    if (!d.leaf[i]) h[i] = (uint8_t)(1 + std::max(h[d.left[i]], h[d.right[i]]));
   lut.assign((size_t)1 << bits, 0);
+  pfx.clear();
+  size_t total = 0;
   for (uint32_t v = 0; v < (1u << bits); v++) {
-    uint32_t node = 0;
-    int len = 0;
-    uint32_t entry = 0;
-    for (; len < bits; len++) {
-      const int b = (v >> (bits - 1 - len)) & 1;
-      node = b ? d.right[node] : d.left[node];
+    uint32_t node = 0, e = 0;
+    for (int len = 0; len < bits; len++) {
+      node = ((v >> (bits - 1 - len)) & 1) ? d.right[node] : d.left[node];
       if (d.leaf[node]) {
-        entry = (d.sym[node] << 8) | (uint32_t)(len + 1);
+        e = (d.sym[node] << 8) | (uint32_t)(len + 1);
         break;
       }
     }
-    if (len == bits) entry = node << 8;  // long code: continue from `node`
-    lut[v] = entry;
+    if (e) {
+      lut[v] = e;
+      continue;
+    }
+    const uint32_t K = (uint32_t)std::min<int>(h[node], bits2);
+    CPSZ_REQUIRE(total < (1u << 24), kErrUnsupported, "huffman decode tables too large");
+    pfx.push_back(node);
+    pfx.push_back(K);
+    pfx.push_back((uint32_t)total);
+    lut[v] = ((uint32_t)total << 8) | 0x80u | K;
+    total += (size_t)1 << K;
   }
+  return total;
 }
 
 }  // namespace cpsz

@@ -45,6 +45,15 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t node_count, H
 // Lookup table of 2^bits entries for the first `bits` stream bits:
 // entry = sym << 8 | len for codes of length <= bits; for longer codes,
 // entry = 0x80 | node (internal node reached after `bits` bits) << 8 with len 0.
-void huff_build_lut(const HuffDecodeTree& d, int bits, std::vector<uint32_t>& lut);
+// Two-level decode tables. Level 1 (2^bits entries) maps the next `bits`
+// bits to sym << 8 | length for codes of at most `bits` bits; a prefix that
+// ends inside the tree maps to off << 8 | 0x80 | K: level-2 table lut2[off ..
+// off + 2^K) over the next K bits (K = min(height of the subtree, bits2)),
+// whose entries are sym << 8 | remaining length, or node << 8 (length 0) for
+// codes still longer, which continue bit by bit down the tree from `node`.
+// Builds level 1 and, per continued prefix, (node, K, off) into `pfx`;
+// returns the level-2 entry count (the device fills level 2 from `pfx`).
+size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<uint32_t>& lut,
+                      std::vector<uint32_t>& pfx);
 
 }  // namespace cpsz
