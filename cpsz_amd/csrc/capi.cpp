@@ -330,6 +330,14 @@ int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, ui
   });
 }
 
+int cpsz_huffman_decode_host(const uint8_t* tree, size_t tree_size, size_t node_count, const uint8_t* bits,
+                             size_t nbytes, size_t n, uint32_t* out) {
+  return guarded([&] {
+    CPSZ_REQUIRE(tree && out && (bits || !nbytes), kErrArg, "null argument");
+    decode_host(tree, tree_size, node_count, bits, nbytes, n, out);
+  });
+}
+
 int cpsz_prepare_libm_tables(const char* path) {
   return guarded([&] { libm_prepare(path ? std::string(path) : libm_cache_path()); });
 }

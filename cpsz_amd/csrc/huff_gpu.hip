@@ -110,6 +110,17 @@ __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t
   if (start[i] != want) *changed = 1;
 }
 
+// Level-2 entry for the K bits u below `node`: sym << 8 | length of the leaf
+// reached, or the node reached after K bits, node << 8.
+__host__ __device__ inline uint32_t lut2_entry(const uint32_t* left, const uint32_t* right, const uint32_t* sym,
+                                               const uint8_t* leaf, uint32_t node, int K, uint32_t u) {
+  for (int len = 0; len < K; len++) {
+    node = ((u >> (K - 1 - len)) & 1) ? right[node] : left[node];
+    if (leaf[node]) return (sym[node] << 8) | (uint32_t)(len + 1);
+  }
+  return node << 8;
+}
+
 // Level-2 decode tables: entry e of prefix p (pfx triples (node, K, off),
 // offsets ascending) walks the K bits of e - off down from node.
 __global__ void k_fill_lut2(const uint32_t* __restrict__ left, const uint32_t* __restrict__ right,
@@ -124,18 +135,63 @@ __global__ void k_fill_lut2(const uint32_t* __restrict__ left, const uint32_t* _
     if (pfx[3 * mid + 2] <= e) lo = mid;
     else hi = mid - 1;
   }
-  uint32_t node = pfx[3 * lo];
-  const int K = (int)pfx[3 * lo + 1];
-  const uint32_t u = e - pfx[3 * lo + 2];
-  uint32_t out = 0;
-  for (int len = 0; len < K; len++) {
-    node = ((u >> (K - 1 - len)) & 1) ? right[node] : left[node];
-    if (leaf[node]) {
-      out = (sym[node] << 8) | (uint32_t)(len + 1);
-      break;
-    }
+  lut2[e] = lut2_entry(left, right, sym, leaf, pfx[3 * lo], (int)pfx[3 * lo + 1], e - pfx[3 * lo + 2]);
+}
+
+void decode_host(const uint8_t* tree, size_t tree_size, size_t node_count, const uint8_t* bits, size_t nbytes,
+                 size_t n, uint32_t* out) {
+  HuffDecodeTree d;
+  CPSZ_REQUIRE(huff_parse_tree(tree, tree_size, node_count, d), kErrStream, "malformed huffman tree");
+  if (d.root_leaf) {
+    for (size_t i = 0; i < n; i++) out[i] = d.root_sym;
+    return;
   }
-  lut2[e] = out ? out : node << 8;
+  std::vector<uint32_t> lut, pfx;
+  const size_t n2 = huff_build_lut(d, kLutBits, kLut2Bits, lut, pfx);
+  std::vector<uint32_t> lut2(n2);
+  for (size_t p = 0; p < pfx.size() / 3; p++)
+    for (uint32_t u = 0; u < (1u << pfx[3 * p + 1]); u++)
+      lut2[pfx[3 * p + 2] + u] = lut2_entry(d.left.data(), d.right.data(), d.sym.data(), d.leaf.data(),
+                                            pfx[3 * p], (int)pfx[3 * p + 1], u);
+  const uint64_t nbits = 8 * (uint64_t)nbytes;
+  uint64_t pos = 0;
+  auto peek = [&](int k) -> uint32_t {  // next k <= 32 bits, zeros past the end
+    uint32_t v = 0;
+    for (int b = 0; b < k; b++) {
+      const uint64_t q = pos + b;
+      v = (v << 1) | (q < nbits ? (bits[q >> 3] >> (7 - (q & 7))) & 1u : 0u);
+    }
+    return v;
+  };
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t e = lut[peek(kLutBits)];
+    const uint32_t L = e & 0xFF;
+    if (L && !(L & 0x80u)) {
+      pos += L;
+      out[i] = e >> 8;
+      continue;
+    }
+    uint32_t node = e >> 8;
+    pos += kLutBits;
+    if (L & 0x80u) {
+      const int K = (int)(L & 0x7F);
+      const uint32_t e2 = lut2[node + peek(K)];
+      if (e2 & 0xFF) {
+        pos += e2 & 0xFF;
+        out[i] = e2 >> 8;
+        continue;
+      }
+      node = e2 >> 8;
+      pos += K;
+    }
+    for (;;) {
+      CPSZ_REQUIRE(pos < nbits, kErrStream, "huffman bitstream exhausted");
+      node = peek(1) ? d.right[node] : d.left[node];
+      pos++;
+      if (d.leaf[node]) break;
+    }
+    out[i] = d.sym[node];
+  }
 }
 
 namespace {

@@ -147,6 +147,17 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
   s.visit(root, 0, 0, 0);
 }
 
+namespace {
+template <typename I>
+void read_indices(const uint8_t* b, size_t N, uint32_t* out) {
+  for (size_t i = 0; i < N; i++) {
+    I v;
+    std::memcpy(&v, b + sizeof(I) * i, sizeof(I));
+    out[i] = v;
+  }
+}
+}  // namespace
+
 bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecodeTree& d) {
   d = HuffDecodeTree();
   if (N == 0 || N > (1u << 24)) return false;
@@ -156,22 +167,23 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecode
   const uint8_t* R = L + N * w;
   const uint8_t* C = R + N * w;
   const uint8_t* T = C + 4 * N;
-  auto idx = [&](const uint8_t* b, size_t i) -> uint32_t {
-    if (w == 1) return b[i];
-    if (w == 2) { uint16_t v; std::memcpy(&v, b + 2 * i, 2); return v; }
-    uint32_t v; std::memcpy(&v, b + 4 * i, 4); return v;
-  };
   d.node_count = N;
   d.left.resize(N);
   d.right.resize(N);
   d.sym.resize(N);
   d.leaf.resize(N);
-  for (size_t i = 0; i < N; i++) {
-    d.left[i] = idx(L, i);
-    d.right[i] = idx(R, i);
-    std::memcpy(&d.sym[i], C + 4 * i, 4);
-    d.leaf[i] = T[i] != 0;
+  if (w == 1) {
+    read_indices<uint8_t>(L, N, d.left.data());
+    read_indices<uint8_t>(R, N, d.right.data());
+  } else if (w == 2) {
+    read_indices<uint16_t>(L, N, d.left.data());
+    read_indices<uint16_t>(R, N, d.right.data());
+  } else {
+    std::memcpy(d.left.data(), L, 4 * N);
+    std::memcpy(d.right.data(), R, 4 * N);
   }
+  std::memcpy(d.sym.data(), C, 4 * N);
+  for (size_t i = 0; i < N; i++) d.leaf[i] = T[i] != 0;
   // Root: a u8-indexed tree takes its root's leaf flag from the stream; wider
   // trees always start from an internal root
   // (reconstruct_HuffTree_from_bytes_anyStates, src/sz_huffman.cpp:588,615,641).
@@ -179,22 +191,23 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecode
   d.root_sym = d.sym[0];
   if (d.root_leaf) return true;
   d.leaf[0] = 0;
-  // Validate reachability: children must point forward (preorder numbering)
-  // and internal nodes need both children for a well-formed code.
-  std::vector<std::pair<uint32_t, int>> st{{0u, 0}};
-  size_t visited = 0;
-  while (!st.empty()) {
-    auto [i, depth] = st.back();
-    st.pop_back();
-    if (++visited > N || depth > 64) return false;
+  // Validate the nodes reachable from the root in one forward pass: children
+  // point forward (preorder numbering), so a node's depth is known before its
+  // index is reached; internal nodes need two children, no node may have two
+  // parents, and codes are at most 64 bits.
+  std::vector<uint8_t> depth(N, 0);  // depth + 1 of reached nodes, 0 = unreached
+  depth[0] = 1;
+  for (size_t i = 0; i < N; i++) {
+    const int di = depth[i];
+    if (!di) continue;
     if (d.leaf[i]) {
-      if (depth > d.max_len) d.max_len = depth;
+      if (di - 1 > d.max_len) d.max_len = di - 1;
       continue;
     }
     const uint32_t l = d.left[i], r = d.right[i];
-    if (l == 0 || r == 0 || l >= N || r >= N || l <= i || r <= i) return false;
-    st.push_back({r, depth + 1});
-    st.push_back({l, depth + 1});
+    if (l >= N || r >= N || l <= i || r <= i || l == r || di > 64) return false;
+    if (depth[l] || depth[r]) return false;
+    depth[l] = depth[r] = (uint8_t)(di + 1);
   }
   return true;
 }
@@ -203,32 +216,45 @@ size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<
                       std::vector<uint32_t>& pfx) {
   const size_t N = d.node_count;
   // subtree heights: children follow their parent in preorder numbering
+  // (indices of nodes the root cannot reach are not validated: bounded only)
   std::vector<uint8_t> h(N, 0);
   for (size_t i = N; i-- > 0;)
-    if (!d.leaf[i]) h[i] = (uint8_t)(1 + std::max(h[d.left[i]], h[d.right[i]]));
+    if (!d.leaf[i] && d.left[i] < N && d.right[i] < N)
+      h[i] = (uint8_t)std::min(64, 1 + std::max<int>(h[d.left[i]], h[d.right[i]]));
   lut.assign((size_t)1 << bits, 0);
   pfx.clear();
   size_t total = 0;
-  for (uint32_t v = 0; v < (1u << bits); v++) {
-    uint32_t node = 0, e = 0;
-    for (int len = 0; len < bits; len++) {
-      node = ((v >> (bits - 1 - len)) & 1) ? d.right[node] : d.left[node];
-      if (d.leaf[node]) {
-        e = (d.sym[node] << 8) | (uint32_t)(len + 1);
-        break;
-      }
-    }
-    if (e) {
-      lut[v] = e;
+  // Depth-first over the nodes of depth <= bits, left before right, so the
+  // level-1 ranges (and the level-2 offsets) come out in increasing order: a
+  // leaf at depth dl fills the 2^(bits - dl) entries starting with its code; an
+  // internal node at depth `bits` starts a level-2 table.
+  struct Item {
+    uint32_t node, code;
+    int depth;
+  };
+  std::vector<Item> st;
+  st.push_back({0, 0, 0});
+  while (!st.empty()) {
+    const Item it = st.back();
+    st.pop_back();
+    if (it.depth > 0 && d.leaf[it.node]) {
+      const uint32_t e = (d.sym[it.node] << 8) | (uint32_t)it.depth;
+      const uint32_t lo = it.code << (bits - it.depth), n = 1u << (bits - it.depth);
+      for (uint32_t k = 0; k < n; k++) lut[lo + k] = e;
       continue;
     }
-    const uint32_t K = (uint32_t)std::min<int>(h[node], bits2);
-    CPSZ_REQUIRE(total < (1u << 24), kErrUnsupported, "huffman decode tables too large");
-    pfx.push_back(node);
-    pfx.push_back(K);
-    pfx.push_back((uint32_t)total);
-    lut[v] = ((uint32_t)total << 8) | 0x80u | K;
-    total += (size_t)1 << K;
+    if (it.depth == bits) {
+      const uint32_t K = (uint32_t)std::min<int>(h[it.node], bits2);
+      CPSZ_REQUIRE(total < (1u << 24), kErrUnsupported, "huffman decode tables too large");
+      pfx.push_back(it.node);
+      pfx.push_back(K);
+      pfx.push_back((uint32_t)total);
+      lut[it.code] = ((uint32_t)total << 8) | 0x80u | K;
+      total += (size_t)1 << K;
+      continue;
+    }
+    st.push_back({d.right[it.node], (it.code << 1) | 1u, it.depth + 1});
+    st.push_back({d.left[it.node], it.code << 1, it.depth + 1});
   }
   return total;
 }

@@ -56,4 +56,9 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t node_count, H
 size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<uint32_t>& lut,
                       std::vector<uint32_t>& pfx);
 
+// Host decode of n symbols of an MSB-first bitstream with the same two-level
+// tables the device decoder uses (huff_gpu.hip; for tests of the tables).
+void decode_host(const uint8_t* tree, size_t tree_size, size_t node_count, const uint8_t* bits, size_t nbytes,
+                 size_t n, uint32_t* out);
+
 }  // namespace cpsz

@@ -121,6 +121,11 @@ int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int
  * frequencies and writes the serialized tree bytes and node count. */
 int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
                       size_t cap, size_t* tree_size, size_t* node_count);
+/* Decodes n symbols of an MSB-first Huffman bitstream on the host with the
+ * two-level tables the GPU decoder builds from a serialized tree (a check of
+ * the table construction). */
+int cpsz_huffman_decode_host(const uint8_t* tree, size_t tree_size, size_t node_count, const uint8_t* bits,
+                             size_t nbytes, size_t n, uint32_t* out);
 
 /* Builds the host-libm exception tables that let the device reproduce the
  * host's log2f/exp2f bit for bit, and writes them to `path` (NULL: the cache

@@ -255,3 +255,32 @@ def test_3d_arithmetic_matches_oracle(shape, seed, eb):
     assert np.array_equal(vid.astype(np.int32), ids)
     pred = (vid != 0) & (vid != 24)
     assert np.array_equal(q[:, pred].T.ravel().astype(np.int32), codes)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_two_level_decode_tables_match_oracle_encoder(seed):
+    """The GPU decoder's two-level tables (12-bit level 1, level-2 tables over
+    up to 12 more bits, tree walk beyond), run on the host, decode the oracle's
+    Huffman sections exactly, including codes longer than 24 bits."""
+    orc = load_oracle()
+    rng = np.random.default_rng(100 + seed)
+    state = [131072, 2048, 512][seed % 3]
+    if seed < 3:  # broad alphabets: most codes 10-20 bits
+        k = min(int(rng.integers(2000, 30000)), state - state // 4 - 1)  # symbols stay below state
+        syms = (rng.zipf(1.3, 200000) % k + state // 4).astype(np.int32)
+    else:  # very skewed: code lengths up to ~40 bits
+        syms = (np.minimum(rng.geometric(0.45, 100000), 60) + state // 4).astype(np.int32)
+    sec = orc.huff_encode(state, syms)
+    node_count = int(np.frombuffer(sec[:8].tobytes(), np.uint64)[0])
+    tree_size = int(np.frombuffer(sec[8:12].tobytes(), np.uint32)[0])
+    tree = sec[12:12 + tree_size]
+    nbytes = int(np.frombuffer(sec[12 + tree_size:20 + tree_size].tobytes(), np.uint64)[0])
+    bits = np.ascontiguousarray(sec[20 + tree_size:20 + tree_size + nbytes])
+    out = np.empty(syms.size, np.uint32)
+    L = cz.lib()
+    L.cpsz_huffman_decode_host.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t,
+                                           C.c_size_t, C.c_void_p]
+    tree = np.ascontiguousarray(tree)
+    assert L.cpsz_huffman_decode_host(tree.ctypes.data, tree.size, node_count, bits.ctypes.data, nbytes,
+                                      syms.size, out.ctypes.data) == 0
+    assert np.array_equal(out, syms.astype(np.uint32))
