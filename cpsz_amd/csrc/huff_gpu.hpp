@@ -226,6 +226,14 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
 __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t* start,
                             int* changed);
 
+// Phase 3: decode every segment into its final position. The symbols of a
+// block's 256 consecutive segments are contiguous in the output: they are
+// gathered in LDS and written out by consecutive threads (coalesced), instead
+// of each thread storing its own run one symbol at a time (64 scattered
+// 2-byte stores per instruction, which reach HBM as ~15x the bytes). A block
+// whose run exceeds the staging buffer writes directly.
+constexpr uint32_t kDecStage = 12288;  // u16 symbols staged per block
+
 template <class Sink>
 __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
                                                    uint32_t S, uint32_t nseg, const uint64_t* __restrict__ start,
@@ -234,17 +242,31 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
                                                    Sink sink) {
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
+  __shared__ uint16_t stage[kDecStage];
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
   uint64_t lbase;
   const uint32_t lcount = stage_words(sw, w, nbits, S, blockIdx.x * blockDim.x, lbase);
   __syncthreads();
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nseg) return;
-  BitReader br;
-  br.init(w, start[i], sw, lbase, lcount);
-  const uint64_t end = exitp[i];
-  uint64_t o = off[i];
-  while (br.pos < end && o < n) sink.put(o++, dec_one(t, slut, br));
+  const uint32_t b0 = blockIdx.x * blockDim.x;
+  const uint32_t b1 = min(b0 + blockDim.x, nseg);  // off[] holds nseg + 1 entries
+  const uint64_t o_begin = min(off[b0], n), o_end = min(off[b1], n);
+  const bool staged = o_end - o_begin <= (uint64_t)kDecStage;
+  const uint32_t i = b0 + threadIdx.x;
+  if (i < nseg) {
+    BitReader br;
+    br.init(w, start[i], sw, lbase, lcount);
+    const uint64_t end = exitp[i];
+    uint64_t o = off[i];
+    if (staged) {
+      while (br.pos < end && o < o_end) stage[o++ - o_begin] = (uint16_t)dec_one(t, slut, br);
+    } else {
+      while (br.pos < end && o < n) sink.put(o++, dec_one(t, slut, br));
+    }
+  }
+  if (!staged) return;  // uniform across the block
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)(o_end - o_begin);
+  for (uint32_t x = threadIdx.x; x < cnt; x += blockDim.x) sink.put(o_begin + x, stage[x]);
 }
 
 template <class Sink>
