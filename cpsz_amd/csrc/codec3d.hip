@@ -118,16 +118,22 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
 // >= 0; non-negative floats order like their bit patterns). A plane is final
 // after the layers on both sides of it: then its ids are written (wave layout)
 // and its slot is reset for plane + 2.
+// FAST evaluates the tetrahedra with the approximate sign bounds (tet_bound_t):
+// a vertex whose id is constant over the error envelope of its approximate
+// bound gets that id; the others are appended to `fix` (count in *nfix) and
+// recomputed exactly by k_eb3d_fix. FAST never writes eb_out.
 constexpr int kMJ = 15, kMK = 63, kMS = 16;
 constexpr int kVJ = kMJ + 2, kVK = kMK + 2;  // vertex halo box of one plane
 constexpr uint32_t kInfBits = 0x7f800000u;
 
-__global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
+template <bool FAST>
+__global__ void __launch_bounds__(256, 3) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
                                               const float* __restrict__ W, int r1, int r2, int r3,
                                               double max_pwr_eb, const double* __restrict__ q3d, int nq,
                                               Skew3 sk, uint8_t* __restrict__ code, int* __restrict__ flags,
                                               const uint8_t* __restrict__ gain, const uint8_t* __restrict__ fb,
-                                              double* __restrict__ eb_out) {
+                                              double* __restrict__ eb_out, uint32_t* __restrict__ fix,
+                                              uint32_t* __restrict__ nfix) {
   __shared__ float sv[2][3][kVJ][kVK];
   __shared__ uint32_t seb[2][kMJ][kMK];
   const int tid = threadIdx.x;
@@ -182,8 +188,9 @@ __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const
             tet_offsets(t, v, di, dj, dk);
             cv[v] = di << 2 | dj << 1 | dk;
           }
-          const double eb = tet_bound(x[0][cv[0]], x[0][cv[1]], x[0][cv[2]], x[0][0], x[1][cv[0]], x[1][cv[1]],
-                                      x[1][cv[2]], x[1][0], x[2][cv[0]], x[2][cv[1]], x[2][cv[2]], x[2][0]);
+          const double eb = tet_bound_t<FAST>(x[0][cv[0]], x[0][cv[1]], x[0][cv[2]], x[0][0], x[1][cv[0]],
+                                              x[1][cv[1]], x[1][cv[2]], x[1][0], x[2][cv[0]], x[2][cv[1]],
+                                              x[2][cv[2]], x[2][0]);
           if (!(eb == eb)) bad = 1;
           const float ef = (float)eb;  // exact: the bound is a float, 0 or 1
 #pragma unroll
@@ -206,18 +213,92 @@ __global__ void __launch_bounds__(256) k_eb3d(const float* __restrict__ U, const
         if (gj >= r2 || gk >= r3) continue;
         const uint32_t bits = seb[L & 1][tj][tk];
         seb[L & 1][tj][tk] = kInfBits;
-        double eb = fmin(max_pwr_eb, (double)__uint_as_float(bits));
-        if (eb_out || gain) {
-          const size_t v = (size_t)L * d0 + (size_t)gj * d1 + gk;
-          if (eb_out) eb_out[v] = eb;
-          if (gain) {
-            const int g = gain[v];
-            if (g && !(fb && fb[v])) eb = fmin(max_pwr_eb, ldexp(eb, g));
+        const size_t v = (size_t)L * d0 + (size_t)gj * d1 + gk;
+        const int g = gain && !(fb && fb[v]) ? gain[v] : 0;
+        if constexpr (FAST) {
+          const double F = (double)__uint_as_float(bits);
+          double lo = fmin(max_pwr_eb, (F - kEbAbs) * (1 - kEbRel));
+          double hi = fmin(max_pwr_eb, (F + kEbAbs) * (1 + kEbRel));
+          if (g) {
+            lo = fmin(max_pwr_eb, ldexp(lo, g));
+            hi = fmin(max_pwr_eb, ldexp(hi, g));
           }
+          const uint32_t c = eb_code_3d(lo, q3d, nq);
+          const bool sure = lo > 0 && c == eb_code_3d(hi, q3d, nq);
+          code[sk.at(L, gj, gk)] = (uint8_t)c;
+          const uint64_t m = __ballot(!sure);
+          if (!sure) {
+            const uint32_t lead = (uint32_t)__ffsll((unsigned long long)m) - 1;
+            uint32_t base = 0;
+            if (__lane_id() == lead) base = atomicAdd(nfix, (uint32_t)__popcll(m));
+            base = __shfl(base, (int)lead);
+            fix[base + lanes_below(m)] = (uint32_t)v;
+          }
+        } else {
+          double eb = fmin(max_pwr_eb, (double)__uint_as_float(bits));
+          if (eb_out) eb_out[v] = eb;
+          if (g) eb = fmin(max_pwr_eb, ldexp(eb, g));
+          code[sk.at(L, gj, gk)] = (uint8_t)eb_code_3d(eb, q3d, nq);
         }
-        code[sk.at(L, gj, gk)] = (uint8_t)eb_code_3d(eb, q3d, nq);
       }
     }
+  }
+  if (bad) atomicOr(flags, 1);
+}
+
+// Exact bounds for the vertices k_eb3d<true> could not decide: the minimum of
+// the exact bounds of the 24 tetrahedra around the vertex (those of the eight
+// cells that hold it, as corner c of the cell whose origin is the vertex minus
+// c's offsets). The set of tetrahedra and their bounds equal the marching
+// kernel's, so the minimum does too. Grid-stride over the list.
+__global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, const float* __restrict__ V,
+                                                  const float* __restrict__ W, int r1, int r2, int r3,
+                                                  double max_pwr_eb, const double* __restrict__ q3d, int nq,
+                                                  Skew3 sk, uint8_t* __restrict__ code, int* __restrict__ flags,
+                                                  const uint8_t* __restrict__ gain,
+                                                  const uint8_t* __restrict__ fb,
+                                                  const uint32_t* __restrict__ fix,
+                                                  const uint32_t* __restrict__ nfix) {
+  const uint32_t count = *nfix;
+  const size_t d0 = (size_t)r2 * r3, d1 = r3;
+  int bad = 0;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
+    const uint32_t v = fix[e];
+    const int i = (int)(v / d0), rem = (int)(v - (size_t)i * d0), j = rem / r3, k = rem - j * r3;
+    float m = __uint_as_float(kInfBits);
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const int ci = i - (c >> 2), cj = j - ((c >> 1) & 1), ck = k - (c & 1);
+      if (ci < 0 || cj < 0 || ck < 0 || ci > r1 - 2 || cj > r2 - 2 || ck > r3 - 2) continue;
+      float x[3][8];
+#pragma unroll
+      for (int cr = 0; cr < 8; cr++) {
+        const size_t g = (size_t)(ci + (cr >> 2)) * d0 + (size_t)(cj + ((cr >> 1) & 1)) * d1 + ck + (cr & 1);
+        x[0][cr] = U[g];
+        x[1][cr] = V[g];
+        x[2][cr] = W[g];
+      }
+#pragma unroll
+      for (int t = 0; t < 6; t++) {
+        int cv[3];
+        bool holds = c == 0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+          int di, dj, dk;
+          tet_offsets(t, q, di, dj, dk);
+          cv[q] = di << 2 | dj << 1 | dk;
+          holds = holds || cv[q] == c;
+        }
+        if (!holds) continue;  // c and t are compile-time: no divergence
+        const double eb = tet_bound(x[0][cv[0]], x[0][cv[1]], x[0][cv[2]], x[0][0], x[1][cv[0]], x[1][cv[1]],
+                                    x[1][cv[2]], x[1][0], x[2][cv[0]], x[2][cv[1]], x[2][cv[2]], x[2][0]);
+        if (!(eb == eb)) bad = 1;
+        m = fminf(m, (float)eb);
+      }
+    }
+    double eb = fmin(max_pwr_eb, (double)m);
+    if (gain && !(fb && fb[v]) && gain[v]) eb = fmin(max_pwr_eb, ldexp(eb, (int)gain[v]));
+    code[sk.at(i, j, k)] = (uint8_t)eb_code_3d(eb, q3d, nq);
   }
   if (bad) atomicOr(flags, 1);
 }
@@ -842,8 +923,21 @@ void c3_eb(Ctx& ctx, C3Work& w, double max_pwr_eb, const EbOverride& ov) {
   CPSZ_HIP(hipMemsetAsync(w.flags, 0, 16, st));
   {
     dim3 grid((unsigned)div_up(r3, kMK), (unsigned)div_up(r2, kMJ), (unsigned)div_up(r1, kMS));
-    k_eb3d<<<grid, 256, 0, st>>>(w.dU, w.dV, w.dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
-                                 ctx.q3d_size(), sk, w.ebc, w.flags, ov.gain, ov.fallback, ov.eb_out);
+    if (ov.eb_out) {
+      k_eb3d<false><<<grid, 256, 0, st>>>(w.dU, w.dV, w.dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
+                                          ctx.q3d_size(), sk, w.ebc, w.flags, ov.gain, ov.fallback, ov.eb_out,
+                                          nullptr, nullptr);
+    } else {
+      // the undecided-vertex list (at most n entries) lives in R, which the
+      // wavefront fills only later
+      uint32_t* fix = (uint32_t*)w.R;
+      uint32_t* nfix = (uint32_t*)(w.flags + 1);
+      k_eb3d<true><<<grid, 256, 0, st>>>(w.dU, w.dV, w.dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
+                                         ctx.q3d_size(), sk, w.ebc, w.flags, ov.gain, ov.fallback, nullptr, fix,
+                                         nfix);
+      k_eb3d_fix<<<1024, 256, 0, st>>>(w.dU, w.dV, w.dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
+                                       ctx.q3d_size(), sk, w.ebc, w.flags, ov.gain, ov.fallback, fix, nfix);
+    }
     CPSZ_HIP(hipGetLastError());
   }
   ctx.mark("eb");
@@ -1037,8 +1131,8 @@ void eb_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, s
   int* flags = ctx.buf<int>("e3.flags", 4);
   CPSZ_HIP(hipMemsetAsync(flags, 0, 16, st));
   dim3 grid((unsigned)div_up(r3, kMK), (unsigned)div_up(r2, kMJ), (unsigned)div_up(r1, kMS));
-  k_eb3d<<<grid, 256, 0, st>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(), ctx.q3d_size(), sk,
-                               code, flags, nullptr, nullptr, dE);
+  k_eb3d<false><<<grid, 256, 0, st>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, max_pwr_eb, ctx.d_q3d(),
+                                      ctx.q3d_size(), sk, code, flags, nullptr, nullptr, dE, nullptr, nullptr);
   CPSZ_HIP(hipGetLastError());
   int* h = (int*)ctx.host_buf("e3.hflags", 16);
   CPSZ_HIP(hipMemcpyAsync(h, flags, 16, hipMemcpyDeviceToHost, st));

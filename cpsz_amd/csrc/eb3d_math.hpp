@@ -49,6 +49,60 @@ __host__ __device__ inline float cube_sign_bound(float p, float n) {
   return fabsf(P - N) / (P + N);
 }
 
+// ---- approximate bounds with a proven error envelope
+//
+// The five correctly rounded divisions of cube_sign_bound cost about ten
+// instructions each; the approximate form replaces every a / b with
+// a * rcp(b) (hardware reciprocal, 1 ulp). For arguments in [0, 2^126):
+//  * acbrt1_fast is within 2 ulp of acbrt1_nonneg_f (checked exhaustively
+//    over every float of that range, subnormals included, on the device), so
+//    its relative error is at most eps = 2^-22;
+//  * with P' = P(1+a), N' = N(1+b), |a|, |b| <= eps = 4u, u = 2^-24, every
+//    float operation rounded to nearest (relative error <= u) and the
+//    reciprocal within 2u: writing E = |P-N|/(P+N), the exact bound lies in
+//    [E(1-u)^2/(1+u), E(1+u)^2/(1-u)] and the fast one in
+//    [(E-eps)(1-u)^2(1-2u)/((1+eps)(1+u)), (E+eps)(1+u)^2(1+2u)/((1-eps)(1-u))],
+//    so |fast - exact| <= 12.01u e + 4.0001u, below e * kEbRel + kEbAbs.
+// Minima and maxima of values that each obey such an envelope obey it too
+// (the maps e -> e (1 -/+ R) -/+ A are increasing), so a vertex bound F'
+// assembled from approximate tetrahedron bounds satisfies
+// F in [(F' - A)(1 - R), (F' + A)(1 + R)]. Vertices whose id is not constant
+// over that interval are recomputed exactly.
+constexpr double kEbAbs = 0x1.2p-22;  // A = 4.5u
+constexpr double kEbRel = 0x1p-20;    // R = 16u
+// Sign-bound arguments are the eight per-cofactor sums or sums of three of
+// them: the eight below 2^124 keeps every argument below 2^126.
+constexpr uint32_t kSafeBits = 0x7D800000u;  // bit pattern of 2^124
+
+__host__ __device__ inline float rcp_approx(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_rcpf(x);
+#else
+  return 1.0f / x;
+#endif
+}
+
+__host__ __device__ inline float acbrt1_fast(float x0) {
+  uint32_t ix;
+  __builtin_memcpy(&ix, &x0, 4);
+  ix = (ix >> 2) + (ix >> 4);
+  ix = ix + (ix >> 4);
+  ix = ix + (ix >> 8);
+  ix = 0x2a5137a0u + ix;
+  float x;
+  __builtin_memcpy(&x, &ix, 4);
+  x = 0.33333333f * (2.0f * x + x0 * rcp_approx(x * x));
+  x = 0.33333333f * (2.0f * x + x0 * rcp_approx(x * x));
+  return x;
+}
+
+// cube_sign_bound within the envelope above, for p, n in [0, 2^126).
+__host__ __device__ inline float cube_sign_bound_fast(float p, float n) {
+  if (p == 0 && n == 0) return 0;
+  const float P = acbrt1_fast(p), N = acbrt1_fast(n);
+  return fabsf(P - N) * rcp_approx(P + N);
+}
+
 __host__ __device__ inline void acc3(double v, double& p, double& n) {
   if (v >= 0) p += v;
   else n += -v;
@@ -62,7 +116,11 @@ __host__ __device__ inline bool same_dir(float a, float b, float c, float d) {
 
 // max_eb_to_keep_position_and_type_3d_offline
 // (src/sz_compress_cp_preserve_3d.cpp:264-358); vertex 3 is the cell origin.
-__host__ __device__ inline double tet_bound(float u0, float u1, float u2, float u3, float v0, float v1,
+// FAST evaluates the sign bounds with cube_sign_bound_fast and returns 0 (which
+// marks every vertex of the tetrahedron for exact recomputation) when an
+// argument falls outside that function's range.
+template <bool FAST>
+__host__ __device__ inline double tet_bound_t(float u0, float u1, float u2, float u3, float v0, float v1,
                                             float v2, float v3, float w0, float w1, float w2,
                                             float w3) {
   const double u3v1w2 = u3 * v1 * w2, u3v2w1 = u3 * v2 * w1, u3v2w0 = u3 * v2 * w0,
@@ -108,27 +166,46 @@ __host__ __device__ inline double tet_bound(float u0, float u1, float u2, float 
   const float pm0 = (float)p0, pm1 = (float)p1, pm2 = (float)p2, pm3 = (float)p3;
   const float nm0 = (float)n0, nm1 = (float)n1, nm2 = (float)n2, nm3 = (float)n3;
   float eb = 0;
+  auto csb = [](float p, float n) {
+    if constexpr (FAST) return cube_sign_bound_fast(p, n);
+    else return cube_sign_bound(p, n);
+  };
   if (!f0) {
-    const float e1 = cube_sign_bound(pm0, nm0), e2 = cube_sign_bound(pm1 + pm2 + pm3, nm1 + nm2 + nm3);
+    const float e1 = csb(pm0, nm0), e2 = csb(pm1 + pm2 + pm3, nm1 + nm2 + nm3);
     const float m = e1 < e2 ? e1 : e2;
     eb = eb > m ? eb : m;
   }
   if (!f1) {
-    const float e1 = cube_sign_bound(pm1, nm1), e2 = cube_sign_bound(pm0 + pm2 + pm3, nm0 + nm2 + nm3);
+    const float e1 = csb(pm1, nm1), e2 = csb(pm0 + pm2 + pm3, nm0 + nm2 + nm3);
     const float m = e1 < e2 ? e1 : e2;
     eb = eb > m ? eb : m;
   }
   if (!f2) {
-    const float e1 = cube_sign_bound(pm2, nm2), e2 = cube_sign_bound(pm0 + pm1 + pm3, nm0 + nm1 + nm3);
+    const float e1 = csb(pm2, nm2), e2 = csb(pm0 + pm1 + pm3, nm0 + nm1 + nm3);
     const float m = e1 < e2 ? e1 : e2;
     eb = eb > m ? eb : m;
   }
   if (!f3) {
-    const float e1 = cube_sign_bound(pm3, nm3), e2 = cube_sign_bound(pm0 + pm1 + pm2, nm0 + nm1 + nm2);
+    const float e1 = csb(pm3, nm3), e2 = csb(pm0 + pm1 + pm2, nm0 + nm1 + nm2);
     const float m = e1 < e2 ? e1 : e2;
     eb = eb > m ? eb : m;
   }
+  if constexpr (FAST) {
+    // NaN and infinity fail this test too (as unsigned bit patterns)
+    uint32_t b = 0;
+    for (float x : {pm0, pm1, pm2, pm3, nm0, nm1, nm2, nm3}) {
+      uint32_t xb;
+      __builtin_memcpy(&xb, &x, 4);
+      b = b > xb ? b : xb;
+    }
+    if (b >= kSafeBits) return 0;
+  }
   return eb;
+}
+
+__host__ __device__ inline double tet_bound(float u0, float u1, float u2, float u3, float v0, float v1, float v2,
+                                            float v3, float w0, float w1, float w2, float w3) {
+  return tet_bound_t<false>(u0, u1, u2, u3, v0, v1, v2, v3, w0, w1, w2, w3);
 }
 
 // Tetrahedra of a cell as (k, j, i) offsets of vertices 0..2; vertex 3 is the

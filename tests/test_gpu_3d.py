@@ -90,3 +90,27 @@ def test_nonfinite_rejected_3d():
     with pytest.raises(cz.CpszError) as e:
         cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
     assert e.value.code == -6
+
+
+@pytest.mark.parametrize("kind", ["tiny", "small", "large", "integers"])
+def test_bound_fast_path_extremes(orc, kind):
+    """Fields whose vertex bounds sit near id thresholds, are exactly 0, or
+    come from sums beyond 2^124: the approximate bound pass must leave those
+    vertices to the exact recomputation, so the stream stays identical."""
+    shape = (20, 33, 40)
+    U, V, W = field_3d(*shape, 11)
+    if kind == "integers":
+        rng = np.random.default_rng(11)
+        U, V, W = (rng.integers(-3, 4, size=shape).astype(np.float32) for _ in range(3))
+    else:
+        scale = {"tiny": 1e-13, "small": 1e-6, "large": 1e12}[kind]
+        U, V, W = (np.float32(scale) * x for x in (U, V, W))
+    print(f"[3d fast-path] {kind} {shape}", flush=True)
+    try:
+        ref = orc.compress_3d(U, V, W, 0.05)
+    except AssertionError:
+        with pytest.raises(cz.CpszError):
+            cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
+        return
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
+    assert np.array_equal(s, ref)
