@@ -389,19 +389,26 @@ struct SrcIds2d {
   const uint16_t* cv;
   size_t n;
   static constexpr int kChunk = 256;
-  template <class F>
-  __device__ void each(size_t c, F f) const {
-    const size_t b = c * kChunk, e = min(b + kChunk, 2 * n);
-    for (size_t p = b; p < e; p++) {
-      const uint32_t x = p < n ? cu[p] : cv[p - n];
-      f(x ? x - 1 : 0);
-    }
+  static constexpr int kMaxItems = kChunk, kPer = 1;
+  __device__ int get(size_t c, int idx, uint32_t* sym) const {
+    const size_t p0 = c * kChunk + idx, p = p0 < 2 * n ? p0 : 2 * n - 1;
+    const uint32_t x = p < n ? cu[p] : cv[p - n];
+    sym[0] = x ? x - 1 : 0;
+    return p0 < 2 * n;
   }
 };
 
 struct SrcCodes2d {
   const uint16_t* q;
   int r1, r2, nby;
+  static constexpr int kMaxItems = kBlock2d * kBlock2d, kPer = 1;
+  __device__ int get(size_t b, int idx, uint32_t* sym) const {
+    const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+    const int sx = min(kBlock2d, r1 - bi * kBlock2d), sy = min(kBlock2d, r2 - bj * kBlock2d);
+    const int k = min(idx, sx * sy - 1), ii = k / sy, jj = k - ii * sy;
+    sym[0] = q[(size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj];
+    return idx < sx * sy;
+  }
   template <class F>
   __device__ void each(size_t b, F f) const {
     const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
@@ -415,6 +422,7 @@ struct SrcCodes2d {
 };
 
 struct SinkIds2d {
+  static constexpr int kBytes = 2;
   uint16_t* su;
   uint16_t* sv;
   size_t n;
@@ -426,6 +434,7 @@ struct SinkIds2d {
 
 // block-major position -> row-major element
 struct SinkCodes2d {
+  static constexpr int kBytes = 2;
   uint16_t* q;
   int r1, r2, nby;
   __device__ void put(uint64_t p, uint32_t s) const {
@@ -495,10 +504,11 @@ struct U16Hist {
   const uint16_t* p;
   size_t n;
   static constexpr int kChunk = 1024;
-  template <class F>
-  __device__ void each(size_t c, F f) const {
-    const size_t b = c * kChunk, e = min(b + kChunk, n);
-    for (size_t i = b; i < e; i++) f(p[i]);
+  static constexpr int kMaxItems = kChunk, kPer = 1;
+  __device__ int get(size_t c, int idx, uint32_t* sym) const {
+    const size_t i = c * kChunk + idx;
+    sym[0] = p[i < n ? i : n - 1];
+    return i < n;
   }
 };
 
@@ -647,7 +657,7 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
     const uint32_t win_lo = 32768 - 8192, win_n = 16384;
     U16Hist hu{qu, n}, hv{qv, n};
     const size_t nch = div_up(n, U16Hist::kChunk);
-    const int grid = (int)std::min<size_t>(div_up(nch, 256), 1024);
+    const int grid = (int)std::min<size_t>(encode_blocks(nch), 512);
     k_hist<<<grid, 256, win_n * 4, st>>>(hu, nch, hist + kMaxEbCode, win_lo, win_n);
     k_hist<<<grid, 256, win_n * 4, st>>>(hv, nch, hist + kMaxEbCode + 65536, win_lo, win_n);
     CPSZ_HIP(hipGetLastError());
@@ -683,9 +693,9 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   uint32_t* zc = ctx.buf<uint32_t>("c2.zc", 2 * nb);
   uint64_t* zoff = ctx.buf<uint64_t>("c2.zoff", 2 * nb + 4);
   uint64_t* totals = ctx.buf<uint64_t>("c2.totals", 8);
-  k_chunk_bits<<<(int)div_up(nch_ids, 256), 256, 0, st>>>(s_ids, nch_ids, cb_len, bits);
-  k_chunk_bits<<<(int)div_up(nb, 256), 256, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
-  k_chunk_bits<<<(int)div_up(nb, 256), 256, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
+  k_chunk_bits<<<(int)encode_blocks(nch_ids), 256, 0, st>>>(s_ids, nch_ids, cb_len, bits);
+  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
+  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
   k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, (int)r1, (int)r2, nby, nb, zc);
   k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, (int)r1, (int)r2, nby, nb, zc + nb);
   CPSZ_HIP(hipGetLastError());
@@ -704,11 +714,11 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   const size_t w_eb = div_up(bytes_eb, 4) + 2, w_u = div_up(bytes_u, 4) + 2, w_v = div_up(bytes_v, 4) + 2;
   uint32_t* words = ctx.buf<uint32_t>("c2.words", w_eb + w_u + w_v);
   k_fill_u32<<<1024, 256, 0, st>>>(words, w_eb + w_u + w_v, 0);
-  k_encode<<<(int)div_up(nch_ids, 256), 256, 0, st>>>(s_ids, nch_ids, cb_code, cb_len, offs, words);
-  k_encode<<<(int)div_up(nb, 256), 256, 0, st>>>(s_u, nb, cb_code + 65536, cb_len + 65536,
-                                                 offs + nch_ids, words + w_eb);
-  k_encode<<<(int)div_up(nb, 256), 256, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
-                                                 offs + nch_ids + nb, words + w_eb + w_u);
+  k_encode<<<(int)encode_blocks(nch_ids), 256, 0, st>>>(s_ids, nch_ids, cb_code, cb_len, offs, bits, words);
+  k_encode<<<(int)encode_blocks(nb), 256, 0, st>>>(s_u, nb, cb_code + 65536, cb_len + 65536, offs + nch_ids,
+                                                   bits + nch_ids, words + w_eb);
+  k_encode<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
+                                                   offs + nch_ids + nb, bits + nch_ids + nb, words + w_eb + w_u);
   float* unp = ctx.buf<float>("c2.unp", unp_u + unp_v + 1);
   k_gather_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, dU, (int)r1, (int)r2, nby, nb, zoff, unp);
   k_gather_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, dV, (int)r1, (int)r2, nby, nb, zoff + nb,

@@ -76,32 +76,6 @@ Skew3 make_skew(size_t r1, size_t r2, size_t r3) {
   return s;
 }
 
-// Row-major walk over vertices [b, e) calling f(wave-layout index).
-template <class F>
-__device__ __forceinline__ void walk3(const Skew3& s, size_t b, size_t e, F f) {
-  if (b >= e) return;
-  const size_t row = b / (size_t)s.r3;
-  int k = (int)(b - row * s.r3);
-  int i = (int)(row / (size_t)s.r2), j = (int)(row - (size_t)i * s.r2);
-  size_t idx = s.at(i, j, k);
-  int t = k + (j & 63);
-  for (size_t v = b; v < e; v++) {
-    f(idx);
-    if (++k == s.r3) {
-      k = 0;
-      if (++j == s.r2) {
-        j = 0;
-        ++i;
-      }
-      idx = s.at(i, j, 0);
-      t = j & 63;
-    } else {
-      idx += ((t & 3) == 3) ? 253 : 1;
-      ++t;
-    }
-  }
-}
-
 __device__ __forceinline__ uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
@@ -323,8 +297,9 @@ __global__ void __launch_bounds__(256) k_signmap(const float* __restrict__ U, co
 // for 0, written in wave layout. Block x = one (i, j) row, y = k range.
 __global__ void k_log(const float* __restrict__ U, const float* __restrict__ V,
                       const float* __restrict__ W, Skew3 s, LibmView lv, float* __restrict__ L) {
-  const uint32_t row = blockIdx.x;
-  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  const uint32_t l = xcd_block(blockIdx.x + blockIdx.y * gridDim.x, gridDim.x * gridDim.y);
+  const uint32_t row = l % gridDim.x;
+  const int k = (int)(l / gridDim.x) * blockDim.x + threadIdx.x;
   if (k >= s.r3) return;
   const int i = (int)(row / (uint32_t)s.r2), j = (int)(row - (uint32_t)i * s.r2);
   const size_t v = (size_t)row * s.r3 + k, o = s.at(i, j, k);
@@ -654,10 +629,11 @@ struct SrcIds3d {
   const uint8_t* vid;
   Skew3 s;
   size_t n;
-  template <class F>
-  __device__ void each(size_t c, F f) const {
-    const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-    walk3(s, b, e, [&](size_t x) { f((uint32_t)vid[x]); });
+  static constexpr int kMaxItems = kVChunk, kPer = 1;
+  __device__ int get(size_t c, int idx, uint32_t* sym) const {
+    const size_t v = c * kVChunk + idx;
+    sym[0] = vid[s.at_v(v < n ? v : n - 1)];
+    return v < n;
   }
 };
 
@@ -666,16 +642,13 @@ struct SrcCodes3d {
   const uint16_t* q;
   Skew3 s;
   size_t n;
-  template <class F>
-  __device__ void each(size_t c, F f) const {
-    const size_t b = c * kVChunk, e = min(b + kVChunk, n);
-    walk3(s, b, e, [&](size_t x) {
-      if (is_pred3(vid[x])) {
-        f((uint32_t)q[x]);
-        f((uint32_t)q[s.per + x]);
-        f((uint32_t)q[2 * s.per + x]);
-      }
-    });
+  static constexpr int kMaxItems = kVChunk, kPer = 3;
+  __device__ int get(size_t c, int idx, uint32_t* sym) const {
+    const size_t v = c * kVChunk + idx, x = s.at_v(v < n ? v : n - 1);
+    sym[0] = q[x];  // loads are unconditional so that a wave issues them all at once
+    sym[1] = q[s.per + x];
+    sym[2] = q[2 * s.per + x];
+    return v < n && is_pred3(vid[x]) ? 3 : 0;
   }
 };
 
@@ -683,7 +656,7 @@ struct SrcCodes3d {
 // block); lane l takes vertices c*256 + 64*r + l, r = 0..3, and ranks the
 // predictable / unpredictable ones with ballots.
 #define CHUNK_WAVE_PROLOGUE                                                 \
-  const size_t c = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);             \
+  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6); \
   const int lane = threadIdx.x & 63;                                         \
   if (c >= nch) return;
 
@@ -733,10 +706,12 @@ __global__ void __launch_bounds__(256) k_gather_unpred3d(const uint8_t* __restri
 }
 
 struct SinkU8 {
+  static constexpr int kBytes = 1;
   uint8_t* p;
   __device__ void put(uint64_t i, uint32_t s) const { p[i] = (uint8_t)s; }
 };
 struct SinkU16 {
+  static constexpr int kBytes = 2;
   uint16_t* p;
   __device__ void put(uint64_t i, uint32_t s) const { p[i] = (uint16_t)s; }
 };
@@ -989,7 +964,7 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   SrcIds3d s_ids{w.vid, sk, n};
   SrcCodes3d s_codes{w.vid, w.q, sk, n};
   {
-    const int grid = (int)std::min<size_t>(div_up(nch, 256), 1024);
+    const int grid = (int)std::min<size_t>(encode_blocks(nch), 512);
     k_hist<<<grid, 256, kMaxEbCode * 4, st>>>(s_ids, nch, hist, 0, kMaxEbCode);
     k_hist<<<grid, 256, 16384 * 4, st>>>(s_codes, nch, hist + kMaxEbCode, 32768 - 8192, 16384);
     CPSZ_HIP(hipGetLastError());
@@ -1018,8 +993,8 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   uint32_t* cnt = ctx.buf<uint32_t>("c3.cnt", 2 * nch);
   uint64_t* coff = ctx.buf<uint64_t>("c3.coff", 2 * nch + 4);
   uint64_t* totals = ctx.buf<uint64_t>("c3.totals", 8);
-  k_chunk_bits<<<(int)div_up(nch, 256), 256, 0, st>>>(s_ids, nch, cb_len, bits);
-  k_chunk_bits<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
+  k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_len, bits);
+  k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
   k_chunk_counts3d<true><<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, n, nch, cnt, cnt + nch);
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits, offs, nch, totals + 0);
@@ -1036,9 +1011,9 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   const size_t w_id = div_up(by_id, 4) + 2, w_q = div_up(by_q, 4) + 2;
   uint32_t* words = ctx.buf<uint32_t>("c3.words", w_id + w_q);
   k_fill_u32<<<1024, 256, 0, st>>>(words, w_id + w_q, 0);
-  k_encode<<<(int)div_up(nch, 256), 256, 0, st>>>(s_ids, nch, cb_code, cb_len, offs, words);
-  k_encode<<<(int)div_up(nch, 256), 256, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536,
-                                                  offs + nch, words + w_id);
+  k_encode<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_code, cb_len, offs, bits, words);
+  k_encode<<<(int)encode_blocks(nch), 256, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536, offs + nch,
+                                                    bits + nch, words + w_id);
   float* unp = ctx.buf<float>("c3.unp", 3 * nunp + 1);
   k_gather_unpred3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, w.dU, w.dV, w.dW, n, nch, coff + nch + 1,
                                                          unp);

@@ -47,4 +47,16 @@ constexpr int kMaxEbCode = 1024;      // histogram bins reserved for eb id codes
 
 inline size_t div_up(size_t a, size_t b) { return (a + b - 1) / b; }
 
+// Workgroups are dispatched round-robin over the 8 XCDs, each with its own
+// L2: block b runs on XCD b % 8. xcd_block renumbers the blocks so that each
+// XCD receives a contiguous range of logical blocks; kernels whose
+// neighbouring blocks read or write the same cache lines (stream-order passes
+// over the wavefront layout, block rows of 2D tiles) then find those lines in
+// one L2 instead of fetching them once per XCD.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t per = nb / kXcds, rem = nb % kXcds, x = b % kXcds, k = b / kXcds;
+  return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+
 }  // namespace cpsz

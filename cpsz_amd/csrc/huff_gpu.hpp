@@ -1,14 +1,16 @@
 // GPU Huffman stage shared by the 2D and 3D codecs.
 //
-// Symbols are produced by a "source" functor that visits the symbols of one
-// chunk in stream order (`each(chunk, f)`), so the block-major (2D) or
-// predictable-vertex (3D) orders of the reference stream are walked in place
-// without materializing a permuted copy.
+// Symbols are produced by a "source" that exposes one chunk as items in
+// stream order (`get(chunk, item, sym[])`), so the block-major
+// (2D) or predictable-vertex (3D) orders of the reference stream are walked in
+// place without materializing a permuted copy. One wave takes one chunk, its
+// lanes consecutive items.
 //
 //   histogram : LDS-privatized window + global atomics outside the window
-//   encode    : per-chunk bit counts -> exclusive scan -> each chunk packs its
-//               codes MSB-first into 32-bit big-endian words; only the first
-//               and last (shared) word of a chunk use atomicOr
+//   encode    : per-chunk bit counts -> exclusive scan -> each wave packs its
+//               chunk's codes MSB-first in LDS (lane offsets from a wave
+//               prefix sum) and stores 32-bit big-endian words coalesced; only
+//               the first and last (shared) word of a chunk use atomicOr
 //   decode    : self-synchronizing parallel decode of the single unchunked
 //               bitstream (speculative segment decode, fix-up rounds until every
 //               segment starts on its predecessor's exit, scan, final decode)
@@ -17,6 +19,7 @@
 
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -31,72 +34,180 @@ constexpr int kLut2Bits = 12;  // width cap of the level-2 decode tables
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
 
 // --------------------------------------------------------------- histogram --
+// One wave per chunk (sources as for the encode kernels below), grid-stride
+// over chunks; symbols in [win_lo, win_lo + win_n) count in LDS, the rest in
+// global memory directly.
 template <class Src>
 __global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t* __restrict__ hist,
-                                              uint32_t win_lo, uint32_t win_n) {
-  extern __shared__ uint32_t sh[];
-  for (uint32_t i = threadIdx.x; i < win_n; i += blockDim.x) sh[i] = 0;
-  __syncthreads();
-  for (size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < nchunks;
-       c += (size_t)gridDim.x * blockDim.x) {
-    src.each(c, [&](uint32_t s) {
-      const uint32_t o = s - win_lo;
-      if (o < win_n) atomicAdd(&sh[o], 1u);
-      else atomicAdd(&hist[s], 1u);
-    });
-  }
-  __syncthreads();
-  for (uint32_t i = threadIdx.x; i < win_n; i += blockDim.x)
-    if (sh[i]) atomicAdd(&hist[win_lo + i], sh[i]);
-}
+                                              uint32_t win_lo, uint32_t win_n);
 
 // ------------------------------------------------------------------ encode --
+// The encode-side kernels run one wave per chunk (4 chunks per 256-thread
+// block). A source exposes its chunk as items taken by lanes in stream order:
+//   Src::kMaxItems            items per chunk at most
+//   Src::kPer                 symbols per item at most
+//   int get(c, idx, sym[])    the symbols of item idx, in stream order, and
+//                             their count; idx in [0, kMaxItems), 0 symbols
+//                             past the chunk's end (branch-free, so that a
+//                             wave issues every round's loads together)
+constexpr int kEncWaves = 4;
+
+inline size_t encode_blocks(size_t nchunks) { return (nchunks + kEncWaves - 1) / kEncWaves; }
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  return x;
+}
+
 template <class Src>
 __global__ void __launch_bounds__(256) k_chunk_bits(Src src, size_t nchunks,
                                                     const uint8_t* __restrict__ len,
                                                     uint64_t* __restrict__ bits) {
-  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (c >= nchunks) return;
-  uint64_t b = 0;
-  src.each(c, [&](uint32_t s) { b += len[s]; });
-  bits[c] = b;
+  constexpr int kRounds = (Src::kMaxItems + 63) / 64;
+  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (c >= nchunks) return;  // whole wave
+  // every round's symbols are fetched before any is used (overlapped latency)
+  uint32_t sym[kRounds][Src::kPer];
+  int ns[kRounds];
+#pragma unroll
+  for (int r = 0; r < kRounds; r++) ns[r] = src.get(c, r * 64 + lane, sym[r]);
+  uint32_t b = 0;
+#pragma unroll
+  for (int r = 0; r < kRounds; r++)
+#pragma unroll
+    for (int q = 0; q < Src::kPer; q++)
+      if (q < ns[r]) b += len[sym[r][q]];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
+  if (lane == 0) bits[c] = b;
 }
 
+template <class Src>
+__global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t* __restrict__ hist,
+                                              uint32_t win_lo, uint32_t win_n) {
+  constexpr int kRounds = (Src::kMaxItems + 63) / 64;
+  extern __shared__ uint32_t sh[];
+  for (uint32_t i = threadIdx.x; i < win_n; i += blockDim.x) sh[i] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  // symbol 0 (the unpredictable marker of the 2D code streams, often frequent)
+  // counts in a register when it lies outside the window: one global atomic
+  // per wave instead of one per occurrence on a single address
+  const bool zero_out = win_lo > 0;
+  uint32_t zeros = 0;
+  for (size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + (threadIdx.x >> 6); c < nchunks;
+       c += (size_t)gridDim.x * kEncWaves) {
+    uint32_t sym[kRounds][Src::kPer];
+    int ns[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) ns[r] = src.get(c, r * 64 + lane, sym[r]);
+#pragma unroll
+    for (int r = 0; r < kRounds; r++)
+#pragma unroll
+      for (int q = 0; q < Src::kPer; q++) {
+        if (q < ns[r]) {
+          const uint32_t o = sym[r][q] - win_lo;
+          if (o < win_n) atomicAdd(&sh[o], 1u);
+          else if (zero_out && sym[r][q] == 0) zeros++;
+          else atomicAdd(&hist[sym[r][q]], 1u);
+        }
+      }
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) zeros += __shfl_xor(zeros, o);
+  if (lane == 0 && zeros) atomicAdd(&hist[0], zeros);
+  __syncthreads();
+  // blocks start their flush at different bins so that their global atomics
+  // on the popular bins do not all arrive together
+  const uint32_t rot = (uint32_t)(((uint64_t)blockIdx.x * 2654435761u) % win_n);
+  for (uint32_t i = threadIdx.x; i < win_n; i += blockDim.x) {
+    const uint32_t b = i + rot < win_n ? i + rot : i + rot - win_n;
+    if (sh[b]) atomicAdd(&hist[win_lo + b], sh[b]);
+  }
+}
+
+// Writes the MSB-aligned code cw (L bits) at bit p of an MSB-first word array.
+__device__ __forceinline__ void put_bits(uint32_t* st, uint32_t p, uint64_t cw, uint32_t L) {
+  const uint32_t wi = p >> 5, o = p & 31;
+  atomicOr(&st[wi], (uint32_t)(cw >> (32 + o)));
+  if (L > 32 - o) atomicOr(&st[wi + 1], (uint32_t)((cw << (32 - o)) >> 32));
+  if (L > 64 - o) atomicOr(&st[wi + 2], (uint32_t)((cw << (64 - o)) >> 32));  // o > 0 here
+}
+
+// Each wave assembles its chunk's bits in LDS (lanes place consecutive
+// symbols at offsets from a wave prefix sum) and then stores the chunk's
+// words with consecutive lanes on consecutive words. The words a chunk
+// shares with its neighbours (a partial first or last word) are merged with
+// atomicOr into the zero-filled output; the others are plain stores.
 template <class Src>
 __global__ void __launch_bounds__(256) k_encode(Src src, size_t nchunks,
                                                 const uint64_t* __restrict__ code,
                                                 const uint8_t* __restrict__ len,
                                                 const uint64_t* __restrict__ off,
+                                                const uint64_t* __restrict__ bits,
                                                 uint32_t* __restrict__ words) {
-  const size_t c = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (c >= nchunks) return;
-  const uint64_t pos0 = off[c];
-  size_t w = pos0 >> 5;
-  int nb = (int)(pos0 & 31);  // bits held in acc (leading ones belong to the previous chunk)
-  uint64_t acc = 0;           // MSB-aligned pending bits
-  bool shared = nb != 0;      // the current word is shared with the previous chunk
-  src.each(c, [&](uint32_t s) {
-    const int L = len[s];
-    uint64_t cw = code[s];
-    int rem = L;
-    while (rem > 0) {
-      const int take = rem < (64 - nb) ? rem : (64 - nb);
-      acc |= cw >> nb;  // bits of cw past `take` are zero or shifted out
-      cw <<= take;
-      nb += take;
-      rem -= take;
-      while (nb >= 32) {
-        const uint32_t word = bswap32((uint32_t)(acc >> 32));
-        if (shared) atomicOr(&words[w], word);
-        else words[w] = word;
-        shared = false;
-        acc <<= 32;
-        nb -= 32;
-        w++;
+  constexpr int kStage = Src::kMaxItems * Src::kPer * 2 + 2;  // 64-bit codes at most
+  __shared__ uint32_t stage[kEncWaves][kStage];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + wv;
+  const bool live = c < nchunks;  // uniform per wave
+  uint32_t* st = stage[wv];
+  uint64_t b0 = 0, nbits = 0;
+  if (live) {
+    b0 = off[c];
+    nbits = bits[c];
+  }
+  const uint32_t sh = (uint32_t)(b0 & 31), nw = (uint32_t)((sh + nbits + 31) >> 5);
+  for (uint32_t k = lane; k < nw; k += 64) st[k] = 0;
+  __syncthreads();
+  if (live) {
+    constexpr int kRounds = (Src::kMaxItems + 63) / 64;
+    // every round's symbols, lengths and codes are fetched before any is used
+    uint32_t sym[kRounds][Src::kPer], L[kRounds][Src::kPer];
+    uint64_t cw[kRounds][Src::kPer];
+    int ns[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) ns[r] = src.get(c, r * 64 + lane, sym[r]);
+#pragma unroll
+    for (int r = 0; r < kRounds; r++)
+#pragma unroll
+      for (int q = 0; q < Src::kPer; q++) {
+        const uint32_t l = len[sym[r][q]];  // symbols are always valid table indices
+        L[r][q] = q < ns[r] ? l : 0u;
+        cw[r][q] = code[sym[r][q]];
       }
+    uint32_t pos = sh;
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+      uint32_t b = 0;
+#pragma unroll
+      for (int q = 0; q < Src::kPer; q++) b += L[r][q];
+      const uint32_t incl = wave_incl_scan(b);
+      uint32_t p = pos + incl - b;
+#pragma unroll
+      for (int q = 0; q < Src::kPer; q++) {
+        if (L[r][q]) put_bits(st, p, cw[r][q], L[r][q]);
+        p += L[r][q];
+      }
+      pos += __shfl(incl, 63);
     }
-  });
-  if (nb > 0) atomicOr(&words[w], bswap32((uint32_t)(acc >> 32)));
+  }
+  __syncthreads();
+  if (live) {
+    const size_t w0 = b0 >> 5;
+    const bool tail_shared = ((b0 + nbits) & 31) != 0;
+    for (uint32_t k = lane; k < nw; k += 64) {
+      const uint32_t word = bswap32(st[k]);
+      if ((k == 0 && sh != 0) || (k == nw - 1 && tail_shared)) atomicOr(&words[w0 + k], word);
+      else words[w0 + k] = word;
+    }
+  }
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v);
@@ -231,8 +342,13 @@ __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t
 // gathered in LDS and written out by consecutive threads (coalesced), instead
 // of each thread storing its own run one symbol at a time (64 scattered
 // 2-byte stores per instruction, which reach HBM as ~15x the bytes). A block
-// whose run exceeds the staging buffer writes directly.
-constexpr uint32_t kDecStage = 12288;  // u16 symbols staged per block
+// whose run exceeds the staging buffer writes directly. Sink::kBytes (1 or 2)
+// is the width of a staged symbol.
+template <int kBytes>
+struct DecStage {
+  using T = typename std::conditional<kBytes == 1, uint8_t, uint16_t>::type;
+  static constexpr uint32_t kCap = kBytes == 1 ? 32768 : 12288;  // symbols per block
+};
 
 template <class Sink>
 __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
@@ -242,7 +358,8 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
                                                    Sink sink) {
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
-  __shared__ uint16_t stage[kDecStage];
+  using Stage = DecStage<Sink::kBytes>;
+  __shared__ typename Stage::T stage[Stage::kCap];
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
   uint64_t lbase;
   const uint32_t lcount = stage_words(sw, w, nbits, S, blockIdx.x * blockDim.x, lbase);
@@ -250,7 +367,7 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
   const uint32_t b0 = blockIdx.x * blockDim.x;
   const uint32_t b1 = min(b0 + blockDim.x, nseg);  // off[] holds nseg + 1 entries
   const uint64_t o_begin = min(off[b0], n), o_end = min(off[b1], n);
-  const bool staged = o_end - o_begin <= (uint64_t)kDecStage;
+  const bool staged = o_end - o_begin <= (uint64_t)Stage::kCap;
   const uint32_t i = b0 + threadIdx.x;
   if (i < nseg) {
     BitReader br;
@@ -258,7 +375,7 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
     const uint64_t end = exitp[i];
     uint64_t o = off[i];
     if (staged) {
-      while (br.pos < end && o < o_end) stage[o++ - o_begin] = (uint16_t)dec_one(t, slut, br);
+      while (br.pos < end && o < o_end) stage[o++ - o_begin] = (typename Stage::T)dec_one(t, slut, br);
     } else {
       while (br.pos < end && o < n) sink.put(o++, dec_one(t, slut, br));
     }
