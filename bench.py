@@ -489,36 +489,54 @@ def main():
             cp_err = {"cp_total": len(c0), "fp": fp, "fn": fn, "ft": ft}
         except Exception as e:  # report a failed CP check instead of hiding the bench line
             cp_err = {"error": str(e)}
-    # roofline of the dominant kernel: the slower of the two wavefronts.
+    # roofline of the dominant kernel (largest share of the step's GPU time).
     # Algorithmic bytes per vertex (each input read once, each output written once):
+    #   3D bounds     k_eb3d<true> + k_eb3d_fix: 12 B field in, 1 B eb code out                = 13
     #   3D compress  k_wave3d<true>:   12 B log values + 1 B eb code in, 6 B codes + 1 B id out = 20
     #   3D decompress k_wave3d<false>: 1 B id + 6 B codes + 12 B initial values in, 12 B out     = 31
+    #   2D bounds     k_eb2d:            8 B field in, 4 B eb codes out                          = 12
     #   2D compress  k_lorenzo2d<true>:  8 B field + 4 B eb codes in, 4 B codes out             = 16
     #   2D decompress k_lorenzo2d<false>: 4 B ids + 4 B codes in, 8 B field out                  = 16
+    # Kernel times are the library's per-stage HIP-event marks on its stream
+    # (each stage named here is that kernel and nothing else of weight).
     nv = int(np.prod(shape))
     if a.workload == "2d":
-        cand = [("k_lorenzo2d<true> (2D compress wavefront)", stages_c.get("lorenzo", float("nan")), 16),
+        cand = [("k_eb2d (2D bounds)", stages_c.get("eb", float("nan")), 12),
+                ("k_lorenzo2d<true> (2D compress wavefront)", stages_c.get("lorenzo", float("nan")), 16),
                 ("k_lorenzo2d<false> (2D decompress wavefront)", stages_d.get("lorenzo", float("nan")), 16)]
     else:
-        cand = [("k_wave3d<true> (3D compress wavefront)", stages_c.get("lorenzo", float("nan")), 20),
+        cand = [("k_eb3d<true> + k_eb3d_fix (3D bounds)", stages_c.get("eb", float("nan")), 13),
+                ("k_wave3d<true> (3D compress wavefront)", stages_c.get("lorenzo", float("nan")), 20),
                 ("k_wave3d<false> (3D decompress wavefront)", stages_d.get("lorenzo", float("nan")), 31)]
     name, k_ms, per_v = max(cand, key=lambda x: x[1] if x[1] == x[1] else -1)
     alg = nv * per_v
     achieved = alg / (k_ms / 1e3) if k_ms == k_ms and k_ms > 0 else None
+    kernels = [k.strip() for k in name.split(" (")[0].split("+")]
     # HBM bytes per dispatch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE on this
-    # workload, scaled by the copy-kernel calibration (profiles/traffic_3d.json)
-    traffic = None
+    # workload, scaled by the copy-kernel calibration (profiles/traffic_3d.json);
+    # VALU instructions per dispatch from rocprofv3 SQ_INSTS_VALU (profiles/sq_3d.json)
+    traffic, valu = None, None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{a.workload}.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             tj = json.load(fh)
-        if tj.get("workload") == workload:
-            traffic = tj["bytes_per_dispatch"].get(name.split(" ")[0])
+        if tj.get("workload") == workload and all(k in tj["bytes_per_dispatch"] for k in kernels):
+            traffic = sum(tj["bytes_per_dispatch"][k] for k in kernels)
+    sfile = os.path.join(ROOT, "profiles", f"sq_{a.workload}.json")
+    if os.path.exists(sfile) and achieved:
+        with open(sfile) as fh:
+            sj = json.load(fh)
+        per = sj["valu_instructions_per_dispatch"]
+        if sj.get("workload") == workload and all(k in per for k in kernels):
+            instr = sum(per[k] for k in kernels)
+            rate = instr / (k_ms / 1e3)
+            valu = {"instructions_per_launch": instr, "achieved": rate / 1e9, "peak": sj["valu_peak_per_s"] / 1e9,
+                    "unit": "G wave64 VALU instr/s", "frac": rate / sj["valu_peak_per_s"]}
     roof = {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9,
             "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": traffic,
-            "kernel": name, "kernel_ms": k_ms, "alg_bytes_per_launch": alg,
-            "other_wavefront": {c[0]: {"ms": c[1], "GB/s": nv * c[2] / (c[1] / 1e3) / 1e9}
-                                for c in cand if c[0] != name and c[1] == c[1] and c[1] > 0}}
+            "kernel": name, "kernel_ms": k_ms, "alg_bytes_per_launch": alg, "valu_issue": valu,
+            "other_kernels": {c[0]: {"ms": c[1], "GB/s": nv * c[2] / (c[1] / 1e3) / 1e9}
+                              for c in cand if c[0] != name and c[1] == c[1] and c[1] > 0}}
     cpu = None if a.no_cpu else cpu_baseline(a.workload, fields, eb)
     out = {
         "metric": "compress+decompress GB/s and ratio at 0 CP errors: plain vs learned-eb, 1 GPU",

@@ -239,6 +239,18 @@ __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, c
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < count; e += gridDim.x * blockDim.x) {
     const uint32_t v = fix[e];
     const int i = (int)(v / d0), rem = (int)(v - (size_t)i * d0), j = rem / r3, k = rem - j * r3;
+    // the 3 x 3 x 3 vertex neighbourhood once (clamped at the borders, where
+    // the cells that would use the clamped values are skipped)
+    float nb[3][27];
+#pragma unroll
+    for (int o = 0; o < 27; o++) {
+      const int gi = min(max(i + o / 9 - 1, 0), r1 - 1), gj = min(max(j + (o / 3) % 3 - 1, 0), r2 - 1),
+                gk = min(max(k + o % 3 - 1, 0), r3 - 1);
+      const size_t g = (size_t)gi * d0 + (size_t)gj * d1 + gk;
+      nb[0][o] = U[g];
+      nb[1][o] = V[g];
+      nb[2][o] = W[g];
+    }
     float m = __uint_as_float(kInfBits);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -247,10 +259,11 @@ __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, c
       float x[3][8];
 #pragma unroll
       for (int cr = 0; cr < 8; cr++) {
-        const size_t g = (size_t)(ci + (cr >> 2)) * d0 + (size_t)(cj + ((cr >> 1) & 1)) * d1 + ck + (cr & 1);
-        x[0][cr] = U[g];
-        x[1][cr] = V[g];
-        x[2][cr] = W[g];
+        const int o = (1 - (c >> 2) + (cr >> 2)) * 9 + (1 - ((c >> 1) & 1) + ((cr >> 1) & 1)) * 3 +
+                      (1 - (c & 1) + (cr & 1));
+        x[0][cr] = nb[0][o];
+        x[1][cr] = nb[1][o];
+        x[2][cr] = nb[2][o];
       }
 #pragma unroll
       for (int t = 0; t < 6; t++) {
@@ -277,20 +290,28 @@ __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, c
   if (bad) atomicOr(flags, 1);
 }
 
-// Three MSB-first sign bitmaps (1 = positive): each wave ballots 64 consecutive
-// values; reversing the 64 ballot bits and then the byte order puts value 8k+j
-// at bit 7-j of byte k.
+// Three MSB-first sign bitmaps (1 = positive): a wave ballots 64 consecutive
+// values per group; reversing the 64 ballot bits and then the byte order puts
+// value 8k+j at bit 7-j of byte k. Block y = component; each wave takes 16
+// consecutive groups, loading all of them before its ballots.
+constexpr int kSignGroups = 16;
 __global__ void __launch_bounds__(256) k_signmap(const float* __restrict__ U, const float* __restrict__ V,
                                                  const float* __restrict__ W, size_t n, size_t nq,
                                                  uint64_t* __restrict__ out) {
-  const size_t g = blockIdx.x * (size_t)blockDim.x + threadIdx.x;  // value index over 3 * nq * 64
-  const size_t q = g >> 6;                                           // 64-value group
-  if (q >= 3 * nq) return;  // whole waves leave together (3 * nq * 64 is a multiple of 64)
-  const int p = (int)(q / nq);
-  const size_t e = (q - p * nq) * 64 + (threadIdx.x & 63);
+  const int p = blockIdx.y, lane = threadIdx.x & 63;
   const float* X = p == 0 ? U : (p == 1 ? V : W);
-  const uint64_t m = __ballot(e < n && X[e < n ? e : 0] > 0);
-  if ((threadIdx.x & 63) == 0) out[q] = __builtin_bswap64(__builtin_bitreverse64(m));
+  const size_t q0 = ((size_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * kSignGroups;
+  float v[kSignGroups];
+#pragma unroll
+  for (int r = 0; r < kSignGroups; r++) {
+    const size_t e = (q0 + r) * 64 + lane;
+    v[r] = e < n ? X[e] : 0.f;
+  }
+#pragma unroll
+  for (int r = 0; r < kSignGroups; r++) {
+    const uint64_t m = __ballot(v[r] > 0);
+    if (lane == 0 && q0 + r < nq) out[p * nq + q0 + r] = __builtin_bswap64(__builtin_bitreverse64(m));
+  }
 }
 
 // log_transform (include/sz_cp_preserve_utils.hpp:205-221): log2f(|x|), -100
@@ -917,7 +938,7 @@ void c3_eb(Ctx& ctx, C3Work& w, double max_pwr_eb, const EbOverride& ov) {
   }
   ctx.mark("eb");
   const size_t nq = div_up(n, 64);
-  k_signmap<<<(int)div_up(3 * nq * 64, 256), 256, 0, st>>>(w.dU, w.dV, w.dW, n, nq, w.smap);
+  k_signmap<<<dim3((unsigned)div_up(nq, 4 * kSignGroups), 3), 256, 0, st>>>(w.dU, w.dV, w.dW, n, nq, w.smap);
   {
     const int bx = r3 >= 256 ? 256 : (int)div_up(r3, 64) * 64;
     dim3 grid((unsigned)(r1 * r2), (unsigned)div_up(r3, bx));

@@ -119,7 +119,11 @@ __host__ __device__ inline float midpoint_distance(double y, float f) {
   const double mid = (fd + nd) * 0.5;
   const double ulp = fabs(nd - fd);
   if (!(ulp > 0 && ulp <= 1.0e308)) return 0.5f;
-  return (float)(fabs(y - mid) / ulp);
+  // ulp is the spacing of adjacent floats, a power of two in [2^-149, 2^104]:
+  // multiplying by its reciprocal (built from the exponent) equals the division
+  // exactly, both being exact scalings by a power of two
+  const double inv = dfrom((uint64_t)(2046 - (int)(dbits(ulp) >> 52)) << 52);
+  return (float)(fabs(y - mid) * inv);
 }
 
 // Exception tables for one function, laid out per input binade ("slot").
