@@ -260,7 +260,9 @@ void upload_codebook(Ctx& ctx, const HuffEncodeTable& t, uint64_t* d_code, uint8
 
 DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
                              const char* tag) {
-  HuffDecodeTree d;
+  // per-thread working objects keep their allocations from call to call
+  thread_local HuffDecodeTree d;
+  thread_local std::vector<uint32_t> lut, pfx;
   CPSZ_REQUIRE(huff_parse_tree(tree, tree_size, node_count, d), kErrStream,
                "malformed huffman tree in stream");
   DeviceDecoder out;
@@ -269,7 +271,6 @@ DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, si
     out.single_sym = d.root_sym;
     return out;
   }
-  std::vector<uint32_t> lut, pfx;
   const size_t n2 = huff_build_lut(d, kLutBits, kLut2Bits, lut, pfx);
   const size_t N = d.node_count;
   const uint32_t npfx = (uint32_t)(pfx.size() / 3);

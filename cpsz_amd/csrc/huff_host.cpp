@@ -159,7 +159,12 @@ void read_indices(const uint8_t* b, size_t N, uint32_t* out) {
 }  // namespace
 
 bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecodeTree& d) {
-  d = HuffDecodeTree();
+  // fields are overwritten, not reassigned: a reused object keeps its
+  // allocations (fresh large vectors cost page faults on every call)
+  d.node_count = 0;
+  d.root_leaf = false;
+  d.root_sym = 0;
+  d.max_len = 0;
   if (N == 0 || N > (1u << 24)) return false;
   const size_t w = N <= 256 ? 1 : (N <= 65536 ? 2 : 4);
   if (tree_size < 1 + 2 * N * w + 5 * N) return false;
@@ -195,7 +200,8 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecode
   // point forward (preorder numbering), so a node's depth is known before its
   // index is reached; internal nodes need two children, no node may have two
   // parents, and codes are at most 64 bits.
-  std::vector<uint8_t> depth(N, 0);  // depth + 1 of reached nodes, 0 = unreached
+  std::vector<uint8_t>& depth = d.scratch;  // depth + 1 of reached nodes, 0 = unreached
+  depth.assign(N, 0);
   depth[0] = 1;
   for (size_t i = 0; i < N; i++) {
     const int di = depth[i];
@@ -217,7 +223,8 @@ size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<
   const size_t N = d.node_count;
   // subtree heights: children follow their parent in preorder numbering
   // (indices of nodes the root cannot reach are not validated: bounded only)
-  std::vector<uint8_t> h(N, 0);
+  std::vector<uint8_t>& h = d.scratch;
+  h.assign(N, 0);
   for (size_t i = N; i-- > 0;)
     if (!d.leaf[i] && d.left[i] < N && d.right[i] < N)
       h[i] = (uint8_t)std::min(64, 1 + std::max<int>(h[d.left[i]], h[d.right[i]]));

@@ -36,6 +36,7 @@ struct HuffDecodeTree {
   bool root_leaf = false;                  // one-symbol alphabet
   uint32_t root_sym = 0;
   int max_len = 0;
+  mutable std::vector<uint8_t> scratch;  // working storage of the parse and table build
 };
 
 // Parses `tree_size` bytes of a serialized tree for `node_count` nodes.
