@@ -51,9 +51,21 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
   uint64_t end = (uint64_t)(i + 1) * S;
   if (end > nbits) end = nbits;
   BitReader br;
-  br.init(w, s0, sw, lbase, lcount);
   uint32_t c = 0;
   if (first) {
+    // Warm-up: start kDecWarm bits before the segment and take as its start
+    // the first code boundary at or after i * S. A prefix-code path usually
+    // resyncs within a few code words, so this path is often already the true
+    // one there: its start then equals the previous segment's exit, and the
+    // fix-up passes skip the segment.
+    br.init(w, s0 - (s0 < kDecWarm ? s0 : kDecWarm), sw, lbase, lcount);
+    while (br.pos < s0) {
+      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
+        *err = 1;
+        break;
+      }
+    }
+    s0 = br.pos;
     while (br.pos < end) {
       if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
         *err = 1;
@@ -66,6 +78,7 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
     cnt[i] = c;
     return;
   }
+  br.init(w, s0, sw, lbase, lcount);
   // New start: decode the new path and the previous pass's path of this
   // segment in lockstep (always advancing the one behind) until they reach
   // the same code boundary; from there both paths coincide, so the previous
