@@ -222,9 +222,8 @@ __global__ void __launch_bounds__(256, 3) k_eb3d(const float* __restrict__ U, co
 
 // Exact bounds for the vertices k_eb3d<true> could not decide: the minimum of
 // the exact bounds of the 24 tetrahedra around the vertex (those of the eight
-// cells that hold it, as corner c of the cell whose origin is the vertex minus
-// c's offsets). The set of tetrahedra and their bounds equal the marching
-// kernel's, so the minimum does too. Grid-stride over the list.
+// cells that hold it). The set of tetrahedra and their bounds equal the
+// marching kernel's, so the minimum does too. Grid-stride over the list.
 __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, const float* __restrict__ V,
                                                   const float* __restrict__ W, int r1, int r2, int r3,
                                                   double max_pwr_eb, const double* __restrict__ q3d, int nq,
@@ -251,6 +250,8 @@ __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, c
       nb[1][o] = V[g];
       nb[2][o] = W[g];
     }
+    // cell c has its origin at the vertex minus c's offsets; the vertex is
+    // that cell's corner c
     float m = __uint_as_float(kInfBits);
 #pragma unroll
     for (int c = 0; c < 8; c++) {
@@ -276,7 +277,7 @@ __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, c
           cv[q] = di << 2 | dj << 1 | dk;
           holds = holds || cv[q] == c;
         }
-        if (!holds) continue;  // c and t are compile-time: no divergence
+        if (!holds) continue;  // c and t are the same in every lane: no divergence
         const double eb = tet_bound(x[0][cv[0]], x[0][cv[1]], x[0][cv[2]], x[0][0], x[1][cv[0]], x[1][cv[1]],
                                     x[1][cv[2]], x[1][0], x[2][cv[0]], x[2][cv[1]], x[2][cv[2]], x[2][0]);
         if (!(eb == eb)) bad = 1;

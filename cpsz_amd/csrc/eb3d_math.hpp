@@ -154,15 +154,41 @@ __host__ __device__ inline double tet_bound_t(float u0, float u1, float u2, floa
     if (same_dir(u0, u1, u2, u3) || same_dir(v0, v1, v2, v3) || same_dir(w0, w1, w2, w3)) return 1;
     return 0;
   }
+  // FAST: a NaN product (an overflowed float product times 0) makes M NaN;
+  // such tetrahedra go to the exact recomputation. The sign split then runs
+  // without selects: p accumulates v + |v| (2v or +0) and n accumulates
+  // |v| - v (0 or -2v), both exact, and the sums are halved at the end. Adding
+  // +0 changes nothing, and doubling every term doubles every rounded partial
+  // sum exactly (the terms are float products, far from the double range's
+  // ends), so the halved sums are the reference's bit for bit.
+  if (FAST && M != M) return 0;
   double p0 = 0, n0 = 0, p1 = 0, n1 = 0, p2 = 0, n2 = 0, p3 = 0, n3 = 0;
-  acc3(-u3v1w2, p0, n0); acc3(u3v2w1, p0, n0); acc3(u1v3w2, p0, n0);
-  acc3(-u2v3w1, p0, n0); acc3(-u1v2w3, p0, n0); acc3(u2v1w3, p0, n0);
-  acc3(-u3v2w0, p1, n1); acc3(u3v0w2, p1, n1); acc3(u2v3w0, p1, n1);
-  acc3(-u0v3w2, p1, n1); acc3(u0v2w3, p1, n1); acc3(-u2v0w3, p1, n1);
-  acc3(-u3v0w1, p2, n2); acc3(u3v1w0, p2, n2); acc3(u0v3w1, p2, n2);
-  acc3(-u1v3w0, p2, n2); acc3(-u0v1w3, p2, n2); acc3(u1v0w3, p2, n2);
-  acc3(u0v1w2, p3, n3); acc3(-u0v2w1, p3, n3); acc3(u1v2w0, p3, n3);
-  acc3(-u1v0w2, p3, n3); acc3(u2v0w1, p3, n3); acc3(-u2v1w0, p3, n3);
+  auto acc = [](double v, double& p, double& n) {
+    if constexpr (FAST) {
+      p += v + fabs(v);
+      n += fabs(v) - v;
+    } else {
+      acc3(v, p, n);
+    }
+  };
+  acc(-u3v1w2, p0, n0); acc(u3v2w1, p0, n0); acc(u1v3w2, p0, n0);
+  acc(-u2v3w1, p0, n0); acc(-u1v2w3, p0, n0); acc(u2v1w3, p0, n0);
+  acc(-u3v2w0, p1, n1); acc(u3v0w2, p1, n1); acc(u2v3w0, p1, n1);
+  acc(-u0v3w2, p1, n1); acc(u0v2w3, p1, n1); acc(-u2v0w3, p1, n1);
+  acc(-u3v0w1, p2, n2); acc(u3v1w0, p2, n2); acc(u0v3w1, p2, n2);
+  acc(-u1v3w0, p2, n2); acc(-u0v1w3, p2, n2); acc(u1v0w3, p2, n2);
+  acc(u0v1w2, p3, n3); acc(-u0v2w1, p3, n3); acc(u1v2w0, p3, n3);
+  acc(-u1v0w2, p3, n3); acc(u2v0w1, p3, n3); acc(-u2v1w0, p3, n3);
+  if constexpr (FAST) {
+    p0 *= 0.5;
+    p1 *= 0.5;
+    p2 *= 0.5;
+    p3 *= 0.5;
+    n0 *= 0.5;
+    n1 *= 0.5;
+    n2 *= 0.5;
+    n3 *= 0.5;
+  }
   const float pm0 = (float)p0, pm1 = (float)p1, pm2 = (float)p2, pm3 = (float)p3;
   const float nm0 = (float)n0, nm1 = (float)n1, nm2 = (float)n2, nm3 = (float)n3;
   float eb = 0;
