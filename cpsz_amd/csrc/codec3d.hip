@@ -101,7 +101,7 @@ constexpr int kVJ = kMJ + 2, kVK = kMK + 2;  // vertex halo box of one plane
 constexpr uint32_t kInfBits = 0x7f800000u;
 
 template <bool FAST>
-__global__ void __launch_bounds__(256, 3) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
+__global__ void __launch_bounds__(256, 4) k_eb3d(const float* __restrict__ U, const float* __restrict__ V,
                                               const float* __restrict__ W, int r1, int r2, int r3,
                                               double max_pwr_eb, const double* __restrict__ q3d, int nq,
                                               Skew3 sk, uint8_t* __restrict__ code, int* __restrict__ flags,
