@@ -40,8 +40,8 @@ namespace cpsz {
 namespace {
 
 
-constexpr int kP3 = 4;  // planes (waves) per wavefront workgroup
-constexpr int kC3 = 8;  // steps per wavefront chunk (a multiple of 4)
+constexpr int kP3 = 8;  // planes (waves) per wavefront workgroup
+constexpr int kC3 = 4;  // steps per wavefront chunk (a multiple of 4)
 
 // Wave layout: item (plane i, 64-row block b) holds T steps (r3 + 63 rounded
 // up to whole chunks) of 64 lanes; lane l's values for steps 4u..4u+3 are
