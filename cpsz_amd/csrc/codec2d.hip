@@ -22,6 +22,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -942,16 +943,26 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
       CPSZ_HIP(hipMemcpyAsync(w, d_in + h.bits_off, h.nbytes, hipMemcpyDeviceToDevice, st));
     return w;
   };
-  {
+  // the two code trees are parsed and tabulated on another host thread while
+  // the device decodes the id stream
+  HostDecoder hc[2];
+  std::future<void> codes_ready = std::async(std::launch::async, [&] {
+    for (int c = 0; c < 2; c++)
+      prepare_decoder(comp[c].h.tree.data(), comp[c].h.tree.size(), comp[c].h.node_count, hc[c]);
+  });
+  try {
     uint32_t* w = stage_bits(h_eb, "d2.w.eb");
     DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
     decode_stream(ctx, dd, w, h_eb.nbytes, 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb");
+  } catch (...) {
+    codes_ready.wait();  // the helper thread uses this frame's objects
+    throw;
   }
+  codes_ready.get();  // rethrows a malformed code tree
   uint16_t* qs[2] = {qu, qv};
   for (int c = 0; c < 2; c++) {
     uint32_t* w = stage_bits(comp[c].h, c ? "d2.w.v" : "d2.w.u");
-    DeviceDecoder dd = upload_decoder(ctx, comp[c].h.tree.data(), comp[c].h.tree.size(),
-                                      comp[c].h.node_count, c ? "d2.v" : "d2.u");
+    DeviceDecoder dd = upload_decoder(ctx, hc[c], c ? "d2.v" : "d2.u");
     decode_stream(ctx, dd, w, comp[c].h.nbytes, n, SinkCodes2d{qs[c], (int)r1, (int)r2, nby},
                   c ? "d2.v" : "d2.u");
   }

@@ -26,6 +26,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
+#include <future>
 #include <string>
 #include <vector>
 
@@ -1203,20 +1204,29 @@ void d3_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   CPSZ_REQUIRE(data_num <= 3 * n, kErrStream, "bad code count");
   parse(off + 8, nc_q, tr_q, nb_q, bo_q);
   ctx.mark("parse");
+  // the code tree (the large one) is parsed and tabulated on another host
+  // thread while the device decodes the id stream
+  HostDecoder hq;
+  std::future<void> q_ready =
+      std::async(std::launch::async, [&] { prepare_decoder(tr_q.data(), tr_q.size(), nc_q, hq); });
   auto stage_bits = [&](size_t bo, size_t nbytes, const char* name) {
     uint32_t* wb = ctx.buf<uint32_t>(name, div_up(nbytes, 4) + 8);
     CPSZ_HIP(hipMemsetAsync(wb, 0, (div_up(nbytes, 4) + 8) * 4, st));
     if (nbytes) CPSZ_HIP(hipMemcpyAsync(wb, d_in + bo, nbytes, hipMemcpyDeviceToDevice, st));
     return wb;
   };
-  {
+  try {
     uint32_t* wb = stage_bits(bo_id, nb_id, "d3.w.id");
     DeviceDecoder dd = upload_decoder(ctx, tr_id.data(), tr_id.size(), nc_id, "d3.id");
     decode_stream(ctx, dd, wb, nb_id, n, SinkU8{w.vid}, "d3.id");
+  } catch (...) {
+    q_ready.wait();  // the helper thread uses this frame's objects
+    throw;
   }
+  q_ready.get();  // rethrows a malformed code tree
   {
     uint32_t* wb = stage_bits(bo_q, nb_q, "d3.w.q");
-    DeviceDecoder dd = upload_decoder(ctx, tr_q.data(), tr_q.size(), nc_q, "d3.q");
+    DeviceDecoder dd = upload_decoder(ctx, hq, "d3.q");
     decode_stream(ctx, dd, wb, nb_q, data_num, SinkU16{w.qc}, "d3.q");
   }
   ctx.mark("huffman");

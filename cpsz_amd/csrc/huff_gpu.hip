@@ -271,20 +271,23 @@ void upload_codebook(Ctx& ctx, const HuffEncodeTable& t, uint64_t* d_code, uint8
   CPSZ_HIP(hipGetLastError());
 }
 
-DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
-                             const char* tag) {
-  // per-thread working objects keep their allocations from call to call
-  thread_local HuffDecodeTree d;
-  thread_local std::vector<uint32_t> lut, pfx;
-  CPSZ_REQUIRE(huff_parse_tree(tree, tree_size, node_count, d), kErrStream,
+void prepare_decoder(const uint8_t* tree, size_t tree_size, size_t node_count, HostDecoder& h) {
+  CPSZ_REQUIRE(huff_parse_tree(tree, tree_size, node_count, h.d), kErrStream,
                "malformed huffman tree in stream");
+  h.n2 = h.d.root_leaf ? 0 : huff_build_lut(h.d, kLutBits, kLut2Bits, h.lut, h.pfx);
+}
+
+DeviceDecoder upload_decoder(Ctx& ctx, const HostDecoder& hd, const char* tag) {
+  const HuffDecodeTree& d = hd.d;
   DeviceDecoder out;
   if (d.root_leaf) {
     out.single = true;
     out.single_sym = d.root_sym;
     return out;
   }
-  const size_t n2 = huff_build_lut(d, kLutBits, kLut2Bits, lut, pfx);
+  const std::vector<uint32_t>& lut = hd.lut;
+  const std::vector<uint32_t>& pfx = hd.pfx;
+  const size_t n2 = hd.n2;
   const size_t N = d.node_count;
   const uint32_t npfx = (uint32_t)(pfx.size() / 3);
   std::string s(tag);
@@ -318,6 +321,13 @@ DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, si
   // The pinned staging buffer is reused by the next decoder upload of the same
   // tag only after the stream has consumed it (callers synchronize per call).
   return out;
+}
+
+DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
+                             const char* tag) {
+  thread_local HostDecoder hd;  // keeps its allocations from call to call
+  prepare_decoder(tree, tree_size, node_count, hd);
+  return upload_decoder(ctx, hd, tag);
 }
 
 }  // namespace cpsz

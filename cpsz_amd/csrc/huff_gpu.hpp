@@ -409,6 +409,16 @@ struct DeviceDecoder {
   uint32_t single_sym = 0;
 };
 // Parses a serialized tree and uploads the decode tables (tag names the buffers).
+// The host half of a decoder: the parsed tree and the level-1 / level-2
+// tables. prepare_decoder makes no HIP call, so it can run on another thread
+// while the device works.
+struct HostDecoder {
+  HuffDecodeTree d;
+  std::vector<uint32_t> lut, pfx;
+  size_t n2 = 0;  // level-2 entries
+};
+void prepare_decoder(const uint8_t* tree, size_t tree_size, size_t node_count, HostDecoder& h);
+DeviceDecoder upload_decoder(Ctx& ctx, const HostDecoder& h, const char* tag);
 DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
                              const char* tag);
 
