@@ -76,7 +76,17 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
     w[k].r2 = fields[k].r2;
     c2_bind(w[k], scratch + soff[k]);
     wp[k] = &w[k];
-    c2_eb(ctx, w[k], max_pwr_eb, EbOverride());
+  }
+  // bounds passes on the workers' streams: small fields fill only part of the
+  // device, so several run at once
+  {
+    std::vector<std::function<void(Ctx&)>> eb_jobs(nf);
+    for (size_t k = 0; k < nf; k++)
+      eb_jobs[k] = [&, k](Ctx& wc) {
+        c2_eb(wc, w[k], max_pwr_eb, EbOverride());
+        CPSZ_HIP(hipStreamSynchronize(wc.stream()));
+      };
+    pool_run(pool_for(ctx.device()), eb_jobs, cfg.workers);
   }
   ctx.mark("eb");
   c2_check(ctx, wp.data(), nf);

@@ -77,7 +77,17 @@ void compress_3d_batch(Ctx& ctx, const std::vector<Field3d>& fields, double max_
     w[k].r3 = fields[k].r3;
     c3_bind(w[k], scratch + soff[k]);
     wp[k] = &w[k];
-    c3_eb(ctx, w[k], max_pwr_eb, ov ? (*ov)[k] : EbOverride());
+  }
+  // bounds passes on the workers' streams: one field's grid fills only part of
+  // the device, so several fields run at once
+  {
+    std::vector<std::function<void(Ctx&)>> eb_jobs(nf);
+    for (size_t k = 0; k < nf; k++)
+      eb_jobs[k] = [&, k](Ctx& wc) {
+        c3_eb(wc, w[k], max_pwr_eb, ov ? (*ov)[k] : EbOverride());
+        CPSZ_HIP(hipStreamSynchronize(wc.stream()));
+      };
+    pool_run(pool_for(ctx.device()), eb_jobs, cfg.workers);
   }
   c3_check(ctx, wp.data(), nf);
   ctx.mark("eb");
