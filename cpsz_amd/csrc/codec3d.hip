@@ -994,7 +994,19 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   }
   uint32_t* hh = (uint32_t*)ctx.host_buf("c3.hist", (kMaxEbCode + 65536) * 4);
   CPSZ_HIP(hipMemcpyAsync(hh, hist, (kMaxEbCode + 65536) * 4, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
+  const hipEvent_t hist_done = ctx.record_point();
+  // per-chunk vertex counts and their scans need no tree: they run while the
+  // host builds the trees
+  uint64_t* bits = ctx.buf<uint64_t>("c3.bits", 2 * nch);
+  uint64_t* offs = ctx.buf<uint64_t>("c3.offs", 2 * nch + 4);
+  uint32_t* cnt = ctx.buf<uint32_t>("c3.cnt", 2 * nch);
+  uint64_t* coff = ctx.buf<uint64_t>("c3.coff", 2 * nch + 4);
+  uint64_t* totals = ctx.buf<uint64_t>("c3.totals", 8);
+  k_chunk_counts3d<true><<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, n, nch, cnt, cnt + nch);
+  CPSZ_HIP(hipGetLastError());
+  scan_u32_to_u64(ctx, cnt, coff, nch, totals + 2);
+  scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, totals + 3);
+  CPSZ_HIP(hipEventSynchronize(hist_done));
   ctx.mark("hist");
   std::vector<uint64_t> f(65536);
   HuffEncodeTable t_id, t_q;
@@ -1011,19 +1023,11 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   uint8_t* cb_len = ctx.buf<uint8_t>("c3.cblen", 2 * 65536);
   upload_codebook(ctx, t_id, cb_code, cb_len, "c3.id");
   upload_codebook(ctx, t_q, cb_code + 65536, cb_len + 65536, "c3.q");
-  uint64_t* bits = ctx.buf<uint64_t>("c3.bits", 2 * nch);
-  uint64_t* offs = ctx.buf<uint64_t>("c3.offs", 2 * nch + 4);
-  uint32_t* cnt = ctx.buf<uint32_t>("c3.cnt", 2 * nch);
-  uint64_t* coff = ctx.buf<uint64_t>("c3.coff", 2 * nch + 4);
-  uint64_t* totals = ctx.buf<uint64_t>("c3.totals", 8);
   k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_len, bits);
   k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
-  k_chunk_counts3d<true><<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, n, nch, cnt, cnt + nch);
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits, offs, nch, totals + 0);
   scan_u64(ctx, bits + nch, offs + nch, nch, totals + 1);
-  scan_u32_to_u64(ctx, cnt, coff, nch, totals + 2);
-  scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, totals + 3);
   uint64_t* ht = (uint64_t*)ctx.host_buf("c3.totals", 64);
   CPSZ_HIP(hipMemcpyAsync(ht, totals, 32, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));

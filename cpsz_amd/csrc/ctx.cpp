@@ -103,7 +103,14 @@ Ctx::~Ctx() {
   if (d_q3d_) hipFree(d_q3d_);
   for (void* p : libm_allocs_) hipFree(p);
   for (auto e : event_pool_) hipEventDestroy(e);
+  if (point_) hipEventDestroy(point_);
   if (own_stream_) hipStreamDestroy(own_stream_);
+}
+
+hipEvent_t Ctx::record_point() {
+  if (!point_) CPSZ_HIP(hipEventCreateWithFlags(&point_, hipEventDisableTiming));
+  CPSZ_HIP(hipEventRecord(point_, stream_));
+  return point_;
 }
 
 void* Ctx::raw_buf(const char* name, size_t bytes) {

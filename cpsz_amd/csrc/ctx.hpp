@@ -62,6 +62,10 @@ class Ctx {
   };
   const LibmDev& libm_dev();
 
+  // Records an event on the stream and returns it: a host wait on it lets
+  // later work queued on the stream overlap the host's own work.
+  hipEvent_t record_point();
+
   // Stage timings of the last call (ms), filled when timing is enabled.
   bool timing = false;
   std::vector<std::pair<std::string, float>> stages;
@@ -81,6 +85,7 @@ class Ctx {
   bool libm_ready_ = false;
   std::vector<std::pair<std::string, hipEvent_t>> marks_;
   std::vector<hipEvent_t> event_pool_;
+  hipEvent_t point_ = nullptr;
 };
 
 // Process-wide context for a device (created on first use).
