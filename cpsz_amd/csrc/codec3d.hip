@@ -1086,7 +1086,9 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
     pos += len;
   };
   host_piece(h0);
-  for (int p = 0; p < 3; p++) dev_piece(w.smap + p * nq, smap);
+  if (nq * 8 == smap) dev_piece(w.smap, 3 * smap);  // the three bitmaps are already contiguous
+  else
+    for (int p = 0; p < 3; p++) dev_piece(w.smap + p * nq, smap);
   host_piece(h1);
   dev_piece(unp, 12 * nunp);
   host_piece(h2);

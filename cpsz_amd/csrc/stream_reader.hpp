@@ -6,7 +6,6 @@
 #pragma once
 #include <algorithm>
 #include <cstring>
-#include <vector>
 
 #include "ctx.hpp"
 
@@ -18,7 +17,6 @@ struct StreamReader {
   const uint8_t* host;  // may be null
   const uint8_t* dev;
   size_t size;
-  std::vector<uint8_t> tmp;
   uint8_t* win = nullptr;
   size_t win_off = 0, win_len = 0;
 
@@ -38,10 +36,12 @@ struct StreamReader {
       }
       return win;
     }
-    tmp.resize(len);
-    CPSZ_HIP(hipMemcpyAsync(tmp.data(), dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
+    // longer reads (a large Huffman tree) go through their own pinned buffer:
+    // a pageable destination would be staged by the runtime in pieces
+    uint8_t* big = (uint8_t*)ctx.host_buf("rd.big", len);
+    CPSZ_HIP(hipMemcpyAsync(big, dev + off, len, hipMemcpyDeviceToHost, ctx.stream()));
     CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
-    return tmp.data();
+    return big;
   }
   template <typename V>
   V get(size_t off) {
