@@ -18,3 +18,4 @@ from .api import (compress_2d_learned_dev, gemm_bf16_dev, learn_analytic_eb_2d_d
 from .api import (compress_3d_learned_batch_dev, learn_analytic_eb_3d_dev, learn_features_3d_dev,  # noqa: F401
                   learn_init_params_3d, learn_labels_3d_dev, learn_param_count_3d, learn_train_3d_dev,
                   mlp_forward_3d_dev, mlp_grad_3d_dev, tet_cp_emulate)
+from .api import check_fast_bounds  # noqa: F401

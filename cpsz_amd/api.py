@@ -74,6 +74,7 @@ _SIGS = {
                                            C.POINTER(_P), _P, _P, C.POINTER(C.c_int), C.POINTER(_SZ),
                                            C.POINTER(_SZ)],
     "cpsz_tet_cp_emulate": [_P, C.c_int, C.POINTER(C.c_int)],
+    "cpsz_check_fast_bounds": [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
 }
 
 
@@ -486,6 +487,15 @@ def compress_3d_learned_batch_dev(fields, eb: float, params: np.ndarray, tune: b
                                                     eb, _ptr(params), int(tune), C.byref(base), _ptr(off), _ptr(size),
                                                     C.byref(it), C.byref(bo), C.byref(fb)))
     return base.value, off.astype(np.int64), size.astype(np.int64), it.value, bo.value, fb.value
+
+
+def check_fast_bounds(device: int = 0):
+    """Device check of the approximate 3D bound pass's premises: (largest ulp
+    distance of the fast cube root over [0, 2^126), largest error-to-envelope
+    ratio of the fast sign bound over hashed argument pairs)."""
+    u, r = C.c_uint32(), C.c_double()
+    _check(lib().cpsz_check_fast_bounds(device, C.byref(u), C.byref(r)))
+    return u.value, r.value
 
 
 def tet_cp_emulate(f, t: int) -> int:

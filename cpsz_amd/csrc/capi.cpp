@@ -667,6 +667,14 @@ int cpsz_learn_init_params_3d(uint64_t seed, float* params) {
   });
 }
 
+int cpsz_check_fast_bounds(int device, uint32_t* max_ulp, double* max_ratio) {
+  return guarded([&] {
+    CPSZ_REQUIRE(max_ulp && max_ratio, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    check_fast_bounds(c, max_ulp, max_ratio);
+  });
+}
+
 int cpsz_learn_analytic_eb_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
                                   size_t r1, size_t r2, size_t r3, double max_pwr_eb, double* dE) {
   return guarded([&] {

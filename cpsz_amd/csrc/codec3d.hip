@@ -292,6 +292,54 @@ __global__ void __launch_bounds__(256) k_eb3d_fix(const float* __restrict__ U, c
   if (bad) atomicOr(flags, 1);
 }
 
+// Self-check of the premises of the approximate bounds (cube_sign_bound_fast):
+// the largest ulp distance between acbrt1_fast and acbrt1_nonneg_f over the
+// bit patterns [lo, hi), and, over hashed argument pairs in [0, 2^126) with
+// ratios from far apart to nearly equal, the largest |fast - exact| / (e R + A).
+__global__ void __launch_bounds__(256) k_check_cbrt(uint32_t lo, uint32_t hi, unsigned* __restrict__ out) {
+  unsigned worst = 0;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint64_t b = (uint64_t)lo + blockIdx.x * blockDim.x + threadIdx.x; b < hi; b += stride) {
+    const float x = __uint_as_float((uint32_t)b);
+    const uint32_t a = __float_as_uint(acbrt1_nonneg_f(x)), f = __float_as_uint(acbrt1_fast(x));
+    const unsigned d = a > f ? a - f : f - a;
+    worst = d > worst ? d : worst;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned t = __shfl_xor(worst, o);
+    worst = t > worst ? t : worst;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, worst);
+}
+
+__device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+__global__ void __launch_bounds__(256) k_check_csb(uint32_t iters, uint32_t seed, unsigned* __restrict__ out) {
+  float worst = 0.f;
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t it = 0; it < iters; it++) {
+    const uint32_t h1 = mix32(g * 2654435761u + it * 40503u + seed), h2 = mix32(h1 ^ 0x9e3779b9u),
+                   h3 = mix32(h2 + 12345u);
+    const float p = ldexpf(1.0f + (h1 & 0xFFFFFF) * 0x1p-24f, (int)(h2 % 250) - 125);
+    const float t = ldexpf((h3 & 0xFFFF) * 0x1p-16f, -(int)((h3 >> 16) % 30));
+    float n = (h2 & 0x100) ? p * (1.0f + t) : p / (1.0f + t);
+    if ((h3 & 0x7) == 0) n = 0;
+    if (!(n < 0x1p125f)) continue;
+    const float e = cube_sign_bound(p, n), f = cube_sign_bound_fast(p, n);
+    const float r = (float)(fabs((double)f - (double)e) / ((double)e * kEbRel + kEbAbs));
+    worst = fmaxf(worst, r);
+  }
+  for (int o = 32; o > 0; o >>= 1) worst = fmaxf(worst, __shfl_xor(worst, o));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(worst));  // non-negative: bits order as values
+}
+
 // Three MSB-first sign bitmaps (1 = positive): a wave ballots 64 consecutive
 // values per group; reversing the 64 ballot bits and then the byte order puts
 // value 8k+j at bit 7-j of byte k. Block y = component; each wave takes 16
@@ -1142,6 +1190,22 @@ void eb_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, s
   CPSZ_HIP(hipMemcpyAsync(h, flags, 16, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
   CPSZ_REQUIRE(h[0] == 0, kErrNonFinite, "eb_3d: non-finite input or bound");
+}
+
+void check_fast_bounds(Ctx& ctx, uint32_t* max_ulp, double* max_ratio) {
+  hipStream_t st = ctx.stream();
+  unsigned* d = ctx.buf<unsigned>("chk.fast", 2);
+  CPSZ_HIP(hipMemsetAsync(d, 0, 8, st));
+  k_check_cbrt<<<4096, 256, 0, st>>>(0u, 0x7E800000u, d);  // [0, 2^126)
+  k_check_csb<<<1024, 256, 0, st>>>(256u, 0x5eedu, d + 1);
+  CPSZ_HIP(hipGetLastError());
+  unsigned h[2];
+  CPSZ_HIP(hipMemcpyAsync(h, d, 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  *max_ulp = h[0];
+  float r;
+  std::memcpy(&r, &h[1], 4);
+  *max_ratio = r;
 }
 
 // ----------------------------------------------------------- decompress ----

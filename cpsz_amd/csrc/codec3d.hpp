@@ -12,6 +12,11 @@ void compress_3d_device(Ctx& ctx, const float* dU, const float* dV, const float*
 void decompress_3d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size,
                           size_t r1, size_t r2, size_t r3, float* dU, float* dV, float* dW);
 
+// Device check of the approximate bound's premises: the largest ulp distance
+// of the fast cube root over [0, 2^126), and the largest ratio of the fast sign
+// bound's error to its envelope over hashed argument pairs (<= 1 when sound).
+void check_fast_bounds(Ctx& ctx, uint32_t* max_ulp, double* max_ratio);
+
 // Analytic per-vertex bound of the plain method (before quantization), row-major.
 void eb_3d_device(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1, size_t r2, size_t r3,
                   double max_pwr_eb, double* dE);

@@ -289,6 +289,12 @@ int cpsz_compress_3d_learned_batch_dev(int device, void* stream, size_t nfields,
                                        const float* params, int tune, const unsigned char** d_base,
                                        size_t* offsets, size_t* sizes, int* iterations, size_t* boosted,
                                        size_t* fallback);
+/* Device check of the premises behind the 3D codec's approximate bound pass
+ * (no reference counterpart; for tests): *max_ulp = largest ulp distance
+ * between the fast and the exact cube root over every float in [0, 2^126)
+ * (the pass assumes <= 2); *max_ratio = largest |fast - exact| sign bound
+ * divided by its error envelope over hashed argument pairs (<= 1 when sound). */
+int cpsz_check_fast_bounds(int device, uint32_t* max_ulp, double* max_ratio);
 /* Host evaluation of the exact tetrahedron predicate the device judge uses:
  * f = 4 vertex vectors (u, v, w) of tetrahedron t (0..5, the judge's order);
  * *type = -1 (no critical point) or the type code. */

@@ -114,3 +114,15 @@ def test_bound_fast_path_extremes(orc, kind):
         return
     s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05)
     assert np.array_equal(s, ref)
+
+
+def test_fast_bound_premises():
+    """The approximate bound pass assumes the fast cube root is within 2 ulp of
+    the reference's over [0, 2^126) (checked here exhaustively) and that the
+    fast sign bound stays inside its error envelope (checked on 2.6e8 hashed
+    argument pairs, from far apart to nearly equal)."""
+    print("[3d fast-bound premises]", flush=True)
+    ulp, ratio = cz.check_fast_bounds(0)
+    assert ulp <= 2
+    assert ratio <= 1.0
+    print(f"max ulp {ulp}, max error/envelope {ratio:.3f}", flush=True)
