@@ -8,7 +8,7 @@
 //   k_lorenzo2d_c strip-pipelined wavefront Lorenzo + linear quantization:
 //                 one wave = 64 rows (lane = row), the row above arrives through
 //                 DPP wave_shr:1; strips hand their last row to the next strip
-//                 through agent-coherent boundary rows with progress counters
+//                 as tagged agent-coherent values the consumer spins on
 //   Huffman       histogram -> host tree (reference tie-break) -> encode
 //   unpredictable values compacted in 12x12 block-major order
 // Decompress mirrors it: parallel Huffman decode of ids and codes, block-major
@@ -234,13 +234,6 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
 // ------------------------------------------------------------ wavefront ----
 constexpr int kCH = 32;  // steps per chunk (boundary publish / prefetch granularity)
 
-__device__ __forceinline__ int ld_coh(const int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_coh(int* p, int v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 __device__ __forceinline__ float shr1(float src, float lane0) {
   return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(src),
                                                     0x138 /* wave_shr:1 */, 0xF, 0xF, false));
@@ -252,14 +245,35 @@ struct Strip2d {
   uint16_t* q[2];        // quant codes (written by compress, read by decompress)
   const float* raw[2];   // decompress: unpredictable value per element
   float* out[2];         // decompress output
-  int* bnd;              // [2][nstrips][r2] boundary rows (float bits)
-  int* prog;             // [2][nstrips] published columns of the boundary row
+  uint64_t* bnd;         // [2][nstrips][r2] boundary rows (kBndTag | float bits, 0 = not yet)
+  int* prog;             // [2][nstrips] unused; [2 * nstrips] the item ticket
   int* ticket;
   int r1, r2, nstrips;
   int wide;  // ids above 81: double-precision step
 };
 
-// One item (component, 64-row strip) of a field's wavefront.
+// One item (component, 64-row strip) of a field's wavefront. The strip's last
+// row is handed to the next strip value by value: lane 63 stores each value
+// with a tag in the high word (kBndTag | float bits; the rows are zeroed before
+// the launch), and the consumer spins until the values it needs carry the tag.
+// No progress counter and no wait for store completion sit on the producer's
+// path. The strip's own rows for chunk c+1 are loaded while chunk c computes.
+constexpr uint64_t kBndTag = 1ull << 32;
+
+__device__ __forceinline__ uint64_t ld_coh64(const uint64_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_coh64(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+struct StripChunk {
+  float xs[kCH];
+  uint32_t cs[kCH];
+  uint32_t qs[kCH];
+  float raws[kCH];
+};
+
 template <bool COMPRESS, bool WIDE>
 __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane) {
   const int r1 = a.r1, r2 = a.r2;
@@ -274,36 +288,36 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
   float* OUT = COMPRESS ? nullptr : a.out[comp] + roff;
   const bool has_up = s > 0;
   const bool publish = s + 1 < a.nstrips;
-  int* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
-  int* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
-  int* prog_in = a.prog + comp * a.nstrips + (s > 0 ? s - 1 : 0);
-  int* prog_out = a.prog + comp * a.nstrips + s;
+  const uint64_t* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
+  uint64_t* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
   float Rprev = 0.f, Uprev = 0.f;
   const int steps = r2 + 63;
-  for (int t0 = 0; t0 < steps; t0 += kCH) {
-    float inc = 0.f;
-    if (has_up && t0 < r2) {
-      const int need = min(t0 + kCH, r2);
-      while (ld_coh(prog_in) < need) __builtin_amdgcn_s_sleep(2);
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      if (lane < kCH && t0 + lane < r2) inc = __int_as_float(ld_coh(bnd_in + t0 + lane));
-    }
-    float xs[kCH];
-    uint32_t cs[kCH];
-    uint32_t qs_in[kCH];
-    float raws[kCH];
+  auto load = [&](StripChunk& B, int t0) {
 #pragma unroll
     for (int k = 0; k < kCH; k++) {
       const int j = t0 + k - lane;
       const bool act = rvalid && j >= 0 && j < r2;
-      cs[k] = act ? C[j] : 0;
+      B.cs[k] = act ? C[j] : 0;
       if (COMPRESS) {
-        xs[k] = act ? X[j] : 0.f;
+        B.xs[k] = act ? X[j] : 0.f;
       } else {
-        qs_in[k] = act ? Q[j] : 0;
-        raws[k] = act ? RAW[j] : 0.f;
+        B.qs[k] = act ? Q[j] : 0;
+        B.raws[k] = act ? RAW[j] : 0.f;
       }
     }
+  };
+  auto chunk = [&](const StripChunk& B, StripChunk& N, int t0) {
+    float inc = 0.f;
+    if (has_up && t0 < r2) {
+      const bool want = lane < kCH && t0 + lane < r2;
+      uint64_t v = want ? ld_coh64(bnd_in + t0 + lane) : kBndTag;
+      while (!__all((v >> 32) != 0)) {
+        __builtin_amdgcn_s_sleep(1);
+        if ((v >> 32) == 0) v = ld_coh64(bnd_in + t0 + lane);
+      }
+      inc = __uint_as_float((uint32_t)v);
+    }
+    if (t0 + kCH < steps) load(N, t0 + kCH);
 #pragma unroll
     for (int k = 0; k < kCH; k++) {
       const int t = t0 + k;
@@ -315,23 +329,22 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       float R;
       if (COMPRESS) {
         uint16_t code;
-        R = l2d_compress_step<WIDE>(pred, xs[k], cs[k], code);
+        R = l2d_compress_step<WIDE>(pred, B.xs[k], B.cs[k], code);
         if (act) Q[j] = code;
       } else {
-        R = l2d_decompress_step<WIDE>(pred, qs_in[k], cs[k], raws[k]);
+        R = l2d_decompress_step<WIDE>(pred, B.qs[k], B.cs[k], B.raws[k]);
         if (act) OUT[j] = R;
       }
-      if (publish && lane == 63 && act) st_coh(bnd_out + j, __float_as_int(R));
+      if (publish && lane == 63 && act) st_coh64(bnd_out + j, kBndTag | __float_as_uint(R));
       Uprev = Uv;
       Rprev = (j < 0) ? 0.f : R;
     }
-    if (publish) {
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      __builtin_amdgcn_s_waitcnt(0);
-      __atomic_signal_fence(__ATOMIC_SEQ_CST);
-      const int done = min(max(t0 + kCH - 63, 0), r2);
-      if (lane == 63) st_coh(prog_out, done);
-    }
+  };
+  StripChunk A, B;
+  load(A, 0);
+  for (int t0 = 0; t0 < steps; t0 += 2 * kCH) {
+    chunk(A, B, t0);
+    if (t0 + kCH < steps) chunk(B, A, t0 + kCH);
   }
 }
 
@@ -541,6 +554,8 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
     Strip2d& sa = f[0];
     const int nz = 2 * sa.nstrips + 1;
     k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(sa.prog, nz);
+    const size_t nb = (size_t)2 * sa.nstrips * sa.r2 * 2;  // tagged boundary rows as 32-bit words
+    k_zero_i32<<<(int)div_up(nb, 256), 256, 0, st>>>((int*)sa.bnd, (int)nb);
     const int items = 2 * sa.nstrips;
     const int grid = std::min(items, grid_cap);
     if (sa.wide) k_lorenzo2d<COMPRESS, true><<<grid, 64, 0, st>>>(sa);
@@ -553,6 +568,8 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
   for (size_t k = 0; k < nf; k++) {
     const int nz = 2 * f[k].nstrips + 1;
     k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(f[k].prog, nz);
+    const size_t nb = (size_t)2 * f[k].nstrips * f[k].r2 * 2;
+    k_zero_i32<<<(int)div_up(nb, 256), 256, 0, st>>>((int*)f[k].bnd, (int)nb);
     start[k + 1] = start[k] + 2 * f[k].nstrips;
   }
   const size_t tbytes = nf * sizeof(Strip2d), sbytes = (nf + 1) * sizeof(int);
@@ -578,7 +595,7 @@ size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
 // ------------------------------------------------------------- staged form --
 size_t c2_scratch_bytes(size_t r1, size_t r2) {
   const size_t n = r1 * r2, ns = div_up(r1, 64);
-  return 4 * al256(2 * n) + al256((kMaxEbCode + 2 * 65536) * 4) + al256(16) + al256(2 * ns * r2 * 4) +
+  return 4 * al256(2 * n) + al256((kMaxEbCode + 2 * 65536) * 4) + al256(16) + al256(2 * ns * r2 * 8) +
          al256((2 * ns + 1) * 4);
 }
 
@@ -595,7 +612,7 @@ void c2_bind(C2Work& w, uint8_t* p) {
   w.qv = (uint16_t*)take(2 * n);
   w.hist = (uint32_t*)take((kMaxEbCode + 2 * 65536) * 4);
   w.flags = (int*)take(16);
-  w.bnd = (int*)take(2 * ns * w.r2 * 4);
+  w.bnd = (uint64_t*)take(2 * ns * w.r2 * 8);
   w.prog = (int*)take((2 * ns + 1) * 4);
 }
 
@@ -870,7 +887,7 @@ HuffSection parse_huff(StreamReader& r, size_t off) {
 
 size_t d2_scratch_bytes(size_t r1, size_t r2) {
   const size_t n = r1 * r2, ns = div_up(r1, 64);
-  return 4 * al256(2 * n) + al256(8 * n) + al256(2 * ns * r2 * 4) + al256((2 * ns + 1) * 4) + al256(16);
+  return 4 * al256(2 * n) + al256(8 * n) + al256(2 * ns * r2 * 8) + al256((2 * ns + 1) * 4) + al256(16);
 }
 
 void d2_bind(D2Work& w, uint8_t* p) {
@@ -885,7 +902,7 @@ void d2_bind(D2Work& w, uint8_t* p) {
   w.qu = (uint16_t*)take(2 * n);
   w.qv = (uint16_t*)take(2 * n);
   w.raw = (float*)take(8 * n);
-  w.bnd = (int*)take(2 * ns * w.r2 * 4);
+  w.bnd = (uint64_t*)take(2 * ns * w.r2 * 8);
   w.prog = (int*)take((2 * ns + 1) * 4);
   w.err = (int*)take(16);
 }

@@ -33,7 +33,8 @@ struct C2Work {
   uint16_t *qu = nullptr, *qv = nullptr;  // quantization codes, row-major
   uint32_t* hist = nullptr;               // kMaxEbCode + 2 * 65536 bins
   int* flags = nullptr;                   // [0] non-finite, [1] ids above 81
-  int *bnd = nullptr, *prog = nullptr;    // wavefront boundary rows / progress counters
+  uint64_t* bnd = nullptr;  // wavefront boundary rows (tagged values)
+  int* prog = nullptr;      // wavefront item ticket
   bool wide = false;
 };
 size_t c2_scratch_bytes(size_t r1, size_t r2);
@@ -49,7 +50,8 @@ struct D2Work {
   uint16_t *sid_u = nullptr, *sid_v = nullptr;  // stream ids
   uint16_t *qu = nullptr, *qv = nullptr;      // quantization codes
   float* raw = nullptr;                       // unpredictable values per element (2n)
-  int *bnd = nullptr, *prog = nullptr, *err = nullptr;
+  uint64_t* bnd = nullptr;  // wavefront boundary rows (tagged values)
+  int *prog = nullptr, *err = nullptr;
   bool wide = false;
 };
 size_t d2_scratch_bytes(size_t r1, size_t r2);
