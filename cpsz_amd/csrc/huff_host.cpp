@@ -37,20 +37,56 @@ class Queue {
     }
     heap_[i] = e;
   }
-  int pop() {
-    const int top = (int)(heap_[1] & 0xFFFFFF);
-    --end_;
-    int i = 1;
-    for (int c = 2; c < end_; c = i << 1) {
-      // branch-free child choice (heap_ has a spare slot past end_)
-      c += (int)((c + 1 < end_) & ((heap_[c + 1] >> 24) < (heap_[c] >> 24)));
-      heap_[i] = heap_[c];
-      i = c;
-    }
-    heap_[i] = heap_[end_];
-    return top;
-  }
   int top() const { return (int)(heap_[1] & 0xFFFFFF); }
+  // Removes the two smallest entries as two consecutive qremove() calls would
+  // (child choice branch-free: heap_ has a spare slot past the end). The
+  // second pop's descent trails the first's by one level, so their two chains
+  // of dependent loads overlap. Every node the second pop reads at a level was
+  // last written by the first pop one level further down, earlier in program
+  // order; the second pop fills its hole only after the first has filled its
+  // own (the last slots can be shared).
+  void pop2(int& a, int& b) {
+    uint64_t* h = heap_.data();
+    a = (int)(h[1] & 0xFFFFFF);
+    const int e1 = --end_;
+    int i1 = 1;
+    bool done1 = false;
+    auto step1 = [&] {
+      const int c = 2 * i1;
+      if (c < e1) {
+        const int cc = c + (int)((c + 1 < e1) & ((h[c + 1] >> 24) < (h[c] >> 24)));
+        h[i1] = h[cc];
+        i1 = cc;
+      } else {
+        h[i1] = h[e1];
+        done1 = true;
+      }
+    };
+    step1();
+    b = (int)(h[1] & 0xFFFFFF);
+    const int e2 = --end_;
+    int i2 = 1;
+    bool walk2 = true;
+    while (!done1) {
+      step1();
+      if (walk2) {
+        const int c = 2 * i2;
+        if (c < e2) {
+          const int cc = c + (int)((c + 1 < e2) & ((h[c + 1] >> 24) < (h[c] >> 24)));
+          h[i2] = h[cc];
+          i2 = cc;
+        } else {
+          walk2 = false;
+        }
+      }
+    }
+    for (int c = 2 * i2; walk2 && c < e2; c = 2 * i2) {
+      const int cc = c + (int)((c + 1 < e2) & ((h[c + 1] >> 24) < (h[c] >> 24)));
+      h[i2] = h[cc];
+      i2 = cc;
+    }
+    h[i2] = h[e2];
+  }
 
  private:
   std::vector<uint64_t> heap_;
@@ -121,8 +157,8 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
   // new_node(.., left, right) (:211); its GCC build evaluates them right to
   // left, so the first removal becomes the right child.
   while (q.size() > 1) {
-    const int r = q.pop();
-    const int l = q.pop();
+    int r, l;
+    q.pop2(r, l);
     nodes.push_back(Node{nodes[l].freq + nodes[r].freq, l, r, 0});
     q.push((int)nodes.size() - 1, nodes.back().freq);
   }
