@@ -22,11 +22,11 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
-#include <future>
 #include <string>
 #include <vector>
 
 #include "codec2d.hpp"
+#include "pool.hpp"
 #include "stream_reader.hpp"
 #include "huff_gpu.hpp"
 #include "lorenzo2d_step.hpp"
@@ -686,14 +686,25 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   uint32_t* hh = (uint32_t*)ctx.host_buf("c2.hist", hist_n * 4);
   CPSZ_HIP(hipMemcpyAsync(hh, hist, hist_n * 4, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
-  std::vector<uint64_t> f(65536);
-  HuffEncodeTable t_eb, t_u, t_v;
-  for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
-  huff_build(f.data(), kMaxEbCode, 2 * 1024, t_eb);
-  for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + k];
-  huff_build(f.data(), 65536, 2 * kCapacity, t_u);
-  for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + 65536 + k];
-  huff_build(f.data(), 65536, 2 * kCapacity, t_v);
+  // the v tree is built on the helper thread beside the eb and u trees; the
+  // tables are kept per thread so their storage is reused from call to call
+  // (the helper reaches the calling thread's objects through references: a
+  // thread_local named in the job would be the helper thread's own)
+  thread_local HuffEncodeTable t_eb, t_u, t_v_keep;
+  thread_local std::vector<uint64_t> f, fv_keep;
+  HuffEncodeTable& t_v = t_v_keep;
+  std::vector<uint64_t>& fv = fv_keep;
+  f.resize(65536);
+  fv.resize(65536);
+  for (int k = 0; k < 65536; k++) fv[k] = hh[kMaxEbCode + 65536 + k];
+  {
+    HelperJob v_tree([&] { huff_build(fv.data(), 65536, 2 * kCapacity, t_v); }, !ctx.batch_worker);
+    for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
+    huff_build(f.data(), kMaxEbCode, 2 * 1024, t_eb);
+    for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + k];
+    huff_build(f.data(), 65536, 2 * kCapacity, t_u);
+    v_tree.join();
+  }
   ctx.mark("tree");
   // ---- codebooks and encode pass A (bits per chunk)
   uint64_t* cb_code = ctx.buf<uint64_t>("c2.cbcode", 3 * 65536);
@@ -962,20 +973,21 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   };
   // the two code trees are parsed and tabulated on another host thread while
   // the device decodes the id stream
-  HostDecoder hc[2];
-  std::future<void> codes_ready = std::async(std::launch::async, [&] {
+  // kept per calling thread (their storage is reused); the helper fills them
+  // through the pointer (a thread_local named in the job would be the helper
+  // thread's own)
+  thread_local HostDecoder hc_keep[2];
+  HostDecoder* hc = hc_keep;
+  HelperJob codes_ready([&] {
     for (int c = 0; c < 2; c++)
       prepare_decoder(comp[c].h.tree.data(), comp[c].h.tree.size(), comp[c].h.node_count, hc[c]);
-  });
-  try {
+  }, !ctx.batch_worker);
+  {
     uint32_t* w = stage_bits(h_eb, "d2.w.eb");
     DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
     decode_stream(ctx, dd, w, h_eb.nbytes, 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb");
-  } catch (...) {
-    codes_ready.wait();  // the helper thread uses this frame's objects
-    throw;
   }
-  codes_ready.get();  // rethrows a malformed code tree
+  codes_ready.join();  // rethrows a malformed code tree
   uint16_t* qs[2] = {qu, qv};
   for (int c = 0; c < 2; c++) {
     uint32_t* w = stage_bits(comp[c].h, c ? "d2.w.v" : "d2.w.u");

@@ -26,7 +26,6 @@
 #include <cfloat>
 #include <cmath>
 #include <cstring>
-#include <future>
 #include <string>
 #include <vector>
 
@@ -34,6 +33,7 @@
 #include "eb3d_math.hpp"
 #include "huff_gpu.hpp"
 #include "libm_math.hpp"
+#include "pool.hpp"
 #include "stream_reader.hpp"
 
 namespace cpsz {
@@ -1056,8 +1056,9 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   scan_u32_to_u64(ctx, cnt + nch, coff + nch + 1, nch, totals + 3);
   CPSZ_HIP(hipEventSynchronize(hist_done));
   ctx.mark("hist");
-  std::vector<uint64_t> f(65536);
-  HuffEncodeTable t_id, t_q;
+  thread_local std::vector<uint64_t> f;
+  f.resize(65536);
+  thread_local HuffEncodeTable t_id, t_q;  // kept per thread: their storage is reused
   for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
   huff_build(f.data(), kMaxEbCode, 2 * 256, t_id);
   uint64_t npred_syms = 0;
@@ -1276,24 +1277,24 @@ void d3_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   ctx.mark("parse");
   // the code tree (the large one) is parsed and tabulated on another host
   // thread while the device decodes the id stream
-  HostDecoder hq;
-  std::future<void> q_ready =
-      std::async(std::launch::async, [&] { prepare_decoder(tr_q.data(), tr_q.size(), nc_q, hq); });
+  // kept per calling thread (its storage is reused); the helper fills it
+  // through the reference (a thread_local named in the job would be the
+  // helper thread's own)
+  thread_local HostDecoder hq_keep;
+  HostDecoder& hq = hq_keep;
+  HelperJob q_ready([&] { prepare_decoder(tr_q.data(), tr_q.size(), nc_q, hq); }, !ctx.batch_worker);
   auto stage_bits = [&](size_t bo, size_t nbytes, const char* name) {
     uint32_t* wb = ctx.buf<uint32_t>(name, div_up(nbytes, 4) + 8);
     CPSZ_HIP(hipMemsetAsync(wb, 0, (div_up(nbytes, 4) + 8) * 4, st));
     if (nbytes) CPSZ_HIP(hipMemcpyAsync(wb, d_in + bo, nbytes, hipMemcpyDeviceToDevice, st));
     return wb;
   };
-  try {
+  {
     uint32_t* wb = stage_bits(bo_id, nb_id, "d3.w.id");
     DeviceDecoder dd = upload_decoder(ctx, tr_id.data(), tr_id.size(), nc_id, "d3.id");
     decode_stream(ctx, dd, wb, nb_id, n, SinkU8{w.vid}, "d3.id");
-  } catch (...) {
-    q_ready.wait();  // the helper thread uses this frame's objects
-    throw;
   }
-  q_ready.get();  // rethrows a malformed code tree
+  q_ready.join();  // rethrows a malformed code tree
   {
     uint32_t* wb = stage_bits(bo_q, nb_q, "d3.w.q");
     DeviceDecoder dd = upload_decoder(ctx, hq, "d3.q");

@@ -41,6 +41,9 @@ class Ctx {
   int device() const { return device_; }
   hipStream_t stream() const { return stream_; }
   void set_stream(hipStream_t s) { stream_ = s ? s : own_stream_; }
+  // Set on the contexts of batch workers: many fields already keep the host
+  // cores busy, so a codec runs its host work inline instead of on a helper.
+  bool batch_worker = false;
 
   // Named grow-only device scratch buffer (contents undefined).
   template <typename T>

@@ -10,44 +10,43 @@ namespace cpsz {
 
 namespace {
 
-struct Node {
-  uint64_t freq;
-  int32_t left, right;  // -1 for leaves
-  uint32_t sym;
-};
-
 // Priority queue with the reference's exact discipline: 1-based heap; insert
 // stops at a parent with freq <= new; remove moves the smaller child up along
 // one path (right child only if strictly smaller) and drops the last element
 // into the vacated leaf slot without re-sifting it (src/sz_huffman.cpp:119-144).
 // Entries pack (frequency << 24 | node id) so the sift loops touch one
-// 8-byte word per level; comparisons use the frequency bits only, keeping the
-// reference's tie behaviour.
+// 8-byte word per level and a removal hands back its frequency with its id;
+// comparisons use the frequency bits only, keeping the reference's tie
+// behaviour.
 class Queue {
  public:
-  explicit Queue(size_t cap) : heap_(cap + 2) {}
+  // `h` holds at least (entries + 2) words: the spare slot past the end lets
+  // the child choice read h[c + 1] unconditionally
+  explicit Queue(uint64_t* h) : h_(h) {}
   int size() const { return end_ - 1; }
-  void push(int id, uint64_t freq) {
-    const uint64_t e = (freq << 24) | (uint64_t)id;
+  void push(uint64_t e) {
+    const uint64_t freq = e >> 24;
+    uint64_t* h = h_;
     int i = end_++;
     for (int p = i >> 1; p; p = i >> 1) {
-      if ((heap_[p] >> 24) <= freq) break;
-      heap_[i] = heap_[p];
+      if ((h[p] >> 24) <= freq) break;
+      h[i] = h[p];
       i = p;
     }
-    heap_[i] = e;
+    h[i] = e;
   }
-  int top() const { return (int)(heap_[1] & 0xFFFFFF); }
+  uint64_t top() const { return h_[1]; }
   // Removes the two smallest entries as two consecutive qremove() calls would
-  // (child choice branch-free: heap_ has a spare slot past the end). The
-  // second pop's descent trails the first's by one level, so their two chains
-  // of dependent loads overlap. Every node the second pop reads at a level was
-  // last written by the first pop one level further down, earlier in program
-  // order; the second pop fills its hole only after the first has filled its
-  // own (the last slots can be shared).
-  void pop2(int& a, int& b) {
-    uint64_t* h = heap_.data();
-    a = (int)(h[1] & 0xFFFFFF);
+  // (child choice branch-free). The second pop's descent trails the first's
+  // by one level, so their two chains of dependent loads overlap. Every node
+  // the second pop reads at a level was last written by the first pop one
+  // level further down, earlier in program order; the second pop fills its
+  // hole only after the first has filled its own (the last slots can be
+  // shared). Each step prefetches the four grandchildren of the node it moves
+  // to, the only entries the step after next can read.
+  void pop2(uint64_t& a, uint64_t& b) {
+    uint64_t* h = h_;
+    a = h[1];
     const int e1 = --end_;
     int i1 = 1;
     bool done1 = false;
@@ -55,6 +54,7 @@ class Queue {
       const int c = 2 * i1;
       if (c < e1) {
         const int cc = c + (int)((c + 1 < e1) & ((h[c + 1] >> 24) < (h[c] >> 24)));
+        __builtin_prefetch(h + 4 * (size_t)cc);
         h[i1] = h[cc];
         i1 = cc;
       } else {
@@ -63,7 +63,7 @@ class Queue {
       }
     };
     step1();
-    b = (int)(h[1] & 0xFFFFFF);
+    b = h[1];
     const int e2 = --end_;
     int i2 = 1;
     bool walk2 = true;
@@ -89,98 +89,136 @@ class Queue {
   }
 
  private:
-  std::vector<uint64_t> heap_;
+  uint64_t* h_;
   int end_ = 1;
 };
 
-struct Serializer {
-  const std::vector<Node>& nodes;
-  HuffEncodeTable& t;
-  size_t N, w;
-  uint8_t *L, *R, *C, *T;
-  uint32_t counter = 0;
-
-  void put_idx(uint8_t* base, size_t slot, uint32_t v) {
-    if (w == 1) base[slot] = (uint8_t)v;
-    else if (w == 2) { uint16_t x = (uint16_t)v; std::memcpy(base + 2 * slot, &x, 2); }
-    else std::memcpy(base + 4 * slot, &v, 4);
-  }
-  // Preorder: node, left subtree, right subtree (pad_tree_*, :357-415);
-  // codes: left appends 0, right appends 1 (build_code, :152-187).
-  void visit(int id, uint32_t slot, int len, uint64_t bits) {
-    CPSZ_REQUIRE(slot < N, kErrInternal, "huffman tree larger than node_count");
-    const Node& nd = nodes[id];
-    if (nd.left < 0) {
-      std::memcpy(C + 4 * (size_t)slot, &nd.sym, 4);
-      T[slot] = 1;
-      CPSZ_REQUIRE(len <= 64, kErrUnsupported, "huffman code longer than 64 bits");
-      t.syms.push_back(nd.sym);
-      t.codes.push_back(len ? (bits << (64 - len)) : 0);
-      t.lens.push_back((uint8_t)len);
-      if (len > t.max_len) t.max_len = len;
-      return;
-    }
-    const uint32_t ls = ++counter;
-    put_idx(L, slot, ls);
-    visit(nd.left, ls, len + 1, bits << 1);
-    const uint32_t rs = ++counter;
-    put_idx(R, slot, rs);
-    visit(nd.right, rs, len + 1, (bits << 1) | 1);
-  }
+// Working storage of a tree build, kept per thread: fresh vectors of a few
+// hundred KiB would cost a page fault per 4 KiB on every call.
+struct BuildWork {
+  std::vector<uint64_t> heap;
+  std::vector<uint32_t> sym;   // leaf id -> symbol (leaves take ids [0, leaves))
+  std::vector<int32_t> kids;   // internal node leaves + k: kids[2k] left, kids[2k + 1] right
 };
+
+struct Pending {
+  int32_t id;
+  uint32_t parent_slot;
+  int len;
+  uint64_t bits;
+};
+
+// Preorder serialization: node, left subtree, right subtree (pad_tree_*,
+// :357-415), each child taking the next slot number when it is reached;
+// codes: left appends 0, right appends 1 (build_code, :152-187). Walked with
+// an explicit stack of right children still to visit.
+template <typename I>
+void serialize(const BuildWork& wk, uint32_t leaves, int32_t root, size_t N, HuffEncodeTable& t) {
+  uint8_t* L = t.tree.data() + 1;
+  uint8_t* R = L + N * sizeof(I);
+  uint8_t* C = R + N * sizeof(I);
+  uint8_t* T = C + 4 * N;
+  uint32_t* syms = t.syms.data();
+  uint64_t* codes = t.codes.data();
+  uint8_t* lens = t.lens.data();
+  Pending stack[64];
+  int sp = 0;
+  uint32_t counter = 0, k = 0;
+  int32_t id = root;
+  uint32_t slot = 0;
+  int len = 0, max_len = 0;
+  uint64_t bits = 0;
+  for (;;) {
+    while (id >= (int32_t)leaves) {
+      // children sit at depth len + 1: a code longer than 64 bits follows
+      CPSZ_REQUIRE(len < 64, kErrUnsupported, "huffman code longer than 64 bits");
+      const int32_t* kd = &wk.kids[2 * (size_t)(id - leaves)];
+      const I ls = (I)++counter;
+      std::memcpy(L + sizeof(I) * slot, &ls, sizeof(I));
+      stack[sp++] = Pending{kd[1], slot, len + 1, (bits << 1) | 1};
+      id = kd[0];
+      slot = ls;
+      len++;
+      bits <<= 1;
+    }
+    const uint32_t s = wk.sym[id];
+    std::memcpy(C + 4 * (size_t)slot, &s, 4);
+    T[slot] = 1;
+    syms[k] = s;
+    codes[k] = len ? (bits << (64 - len)) : 0;
+    lens[k] = (uint8_t)len;
+    k++;
+    if (len > max_len) max_len = len;
+    if (!sp) break;
+    const Pending p = stack[--sp];
+    const I rs = (I)++counter;
+    std::memcpy(R + sizeof(I) * p.parent_slot, &rs, sizeof(I));
+    id = p.id;
+    slot = rs;
+    len = p.len;
+    bits = p.bits;
+  }
+  CPSZ_REQUIRE(counter + 1 == N && k == leaves, kErrInternal, "huffman tree size mismatch");
+  t.max_len = max_len;
+}
 
 }  // namespace
 
 void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEncodeTable& t) {
-  t = HuffEncodeTable();
+  // fields are overwritten, not reassigned: a reused table keeps its allocations
   t.state_num = state_num;
+  t.max_len = 0;
   size_t leaves = 0;
+  uint64_t total = 0;
   for (size_t s = 0; s < nbins; s++)
     if (freq[s]) {
       CPSZ_REQUIRE(s < 2 * (size_t)state_num, kErrInternal, "huffman symbol out of range");
       leaves++;
+      total += freq[s];
     }
   CPSZ_REQUIRE(leaves > 0, kErrInternal, "huffman: empty input");
   CPSZ_REQUIRE(2 * leaves < (1u << 24), kErrUnsupported, "huffman alphabet too large");
-  uint64_t total = 0;
-  for (size_t s = 0; s < nbins; s++) total += freq[s];
   CPSZ_REQUIRE(total < (1ull << 40), kErrUnsupported, "huffman symbol count too large");
-  std::vector<Node> nodes;
-  nodes.reserve(2 * leaves);
-  Queue q(2 * leaves);
+  thread_local BuildWork wk;
+  wk.heap.resize(2 * leaves + 2);
+  wk.sym.resize(leaves);
+  wk.kids.resize(2 * leaves);
+  Queue q(wk.heap.data());
+  uint32_t next = 0;
   for (size_t s = 0; s < nbins; s++) {
     if (!freq[s]) continue;
-    nodes.push_back(Node{freq[s], -1, -1, (uint32_t)s});
-    q.push((int)nodes.size() - 1, freq[s]);
+    wk.sym[next] = (uint32_t)s;
+    q.push(freq[s] << 24 | next);
+    next++;
   }
   // Merge: the reference passes two qremove() calls as arguments to
   // new_node(.., left, right) (:211); its GCC build evaluates them right to
   // left, so the first removal becomes the right child.
+  int32_t* kids = wk.kids.data();
   while (q.size() > 1) {
-    int r, l;
-    q.pop2(r, l);
-    nodes.push_back(Node{nodes[l].freq + nodes[r].freq, l, r, 0});
-    q.push((int)nodes.size() - 1, nodes.back().freq);
+    uint64_t er, el;
+    q.pop2(er, el);
+    kids[2 * (size_t)(next - leaves)] = (int32_t)(el & 0xFFFFFF);
+    kids[2 * (size_t)(next - leaves) + 1] = (int32_t)(er & 0xFFFFFF);
+    q.push(((er >> 24) + (el >> 24)) << 24 | next);
+    next++;
   }
-  const int root = q.top();
+  const int32_t root = (int32_t)(q.top() & 0xFFFFFF);
   // node_count counts leaves below state_num (src/sz_compression_utils.cpp:66-68).
   size_t counted = 0;
   for (size_t s = 0; s < nbins && s < state_num; s++) counted += freq[s] != 0;
   CPSZ_REQUIRE(counted == leaves, kErrUnsupported, "symbol beyond the coder's state count");
   t.node_count = 2 * counted - 1;
-  t.single = nodes[root].left < 0;
+  t.single = root < (int32_t)leaves;
   const size_t N = t.node_count;
   const size_t w = N <= 256 ? 1 : (N <= 65536 ? 2 : 4);
   t.tree.assign(1 + 2 * N * w + 5 * N, 0);
-  t.syms.reserve(leaves);
-  t.codes.reserve(leaves);
-  t.lens.reserve(leaves);
-  Serializer s{nodes, t, N, w, nullptr, nullptr, nullptr, nullptr};
-  s.L = t.tree.data() + 1;
-  s.R = s.L + N * w;
-  s.C = s.R + N * w;
-  s.T = s.C + 4 * N;
-  s.visit(root, 0, 0, 0);
+  t.syms.resize(leaves);
+  t.codes.resize(leaves);
+  t.lens.resize(leaves);
+  if (w == 1) serialize<uint8_t>(wk, (uint32_t)leaves, root, N, t);
+  else if (w == 2) serialize<uint16_t>(wk, (uint32_t)leaves, root, N, t);
+  else serialize<uint32_t>(wk, (uint32_t)leaves, root, N, t);
 }
 
 namespace {

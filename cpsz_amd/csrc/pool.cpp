@@ -44,7 +44,10 @@ class WorkerPool {
       }
       std::exception_ptr err;
       try {
-        if (!ctx) ctx.reset(new Ctx(device_));
+        if (!ctx) {
+          ctx.reset(new Ctx(device_));
+          ctx->batch_worker = true;
+        }
         (*job)(*ctx);
       } catch (...) {
         err = std::current_exception();
@@ -82,4 +85,78 @@ void pool_run(WorkerPool& pool, std::vector<std::function<void(Ctx&)>>& jobs, in
   pool.run(jobs, limit);
 }
 
+struct HostHelper::State {
+  std::mutex m;
+  std::condition_variable cv;
+  std::function<void()> job;
+  bool pending = false, busy = false, quit = false;
+  std::exception_ptr err;
+  std::thread th;
+};
+
+HostHelper::HostHelper() : s_(new State) {
+  State* s = s_;
+  s->th = std::thread([s] {
+    for (;;) {
+      std::function<void()> job;
+      {
+        std::unique_lock<std::mutex> g(s->m);
+        s->cv.wait(g, [&] { return s->pending || s->quit; });
+        if (s->quit && !s->pending) return;
+        job = std::move(s->job);
+        s->pending = false;
+      }
+      std::exception_ptr err;
+      try {
+        job();
+      } catch (...) {
+        err = std::current_exception();
+      }
+      {
+        std::lock_guard<std::mutex> g(s->m);
+        s->err = err;
+        s->busy = false;
+      }
+      s->cv.notify_all();
+    }
+  });
+}
+
+HostHelper::~HostHelper() {
+  {
+    std::lock_guard<std::mutex> g(s_->m);
+    s_->quit = true;
+  }
+  s_->cv.notify_all();
+  s_->th.join();
+  delete s_;
+}
+
+void HostHelper::start(std::function<void()> job) {
+  {
+    std::lock_guard<std::mutex> g(s_->m);
+    CPSZ_REQUIRE(!s_->busy, kErrInternal, "host helper already has a job");
+    s_->job = std::move(job);
+    s_->pending = s_->busy = true;
+    s_->err = nullptr;
+  }
+  s_->cv.notify_all();
+}
+
+void HostHelper::wait() {
+  std::unique_lock<std::mutex> g(s_->m);
+  s_->cv.wait(g, [&] { return !s_->busy; });
+  if (s_->err) {
+    std::exception_ptr e = s_->err;
+    s_->err = nullptr;
+    std::rethrow_exception(e);
+  }
+}
+
+HostHelper& host_helper() {
+  thread_local HostHelper h;
+  return h;
+}
+
 }  // namespace cpsz
+
