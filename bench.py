@@ -429,13 +429,13 @@ def main():
     # the stream is copied out of the library-owned buffer so compress and
     # decompress of a step are independent calls
     sbuf = torch.empty(64 << 20, dtype=torch.uint8, device=dev)
+    hip = C.CDLL("libamdhip64.so")
 
     def step():
         p, n = compress()
         st_c = cz.last_stages(0)
         if n > sbuf.numel():
             raise RuntimeError("stream larger than the staging buffer")
-        hip = C.CDLL("libamdhip64.so")
         hip.hipMemcpyAsync(C.c_void_p(sbuf.data_ptr()), C.c_void_p(p), C.c_size_t(n), 3, C.c_void_p(sh))
         if a.workload == "2d":
             cz.decompress_2d_dev(sbuf.data_ptr(), n, *shape, outs[0].data_ptr(), outs[1].data_ptr(), 0, sh)

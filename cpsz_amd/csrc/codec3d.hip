@@ -1294,23 +1294,26 @@ void d3_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
     DeviceDecoder dd = upload_decoder(ctx, tr_id.data(), tr_id.size(), nc_id, "d3.id");
     decode_stream(ctx, dd, wb, nb_id, n, SinkU8{w.vid}, "d3.id");
   }
-  q_ready.join();  // rethrows a malformed code tree
-  {
-    uint32_t* wb = stage_bits(bo_q, nb_q, "d3.w.q");
-    DeviceDecoder dd = upload_decoder(ctx, hq, "d3.q");
-    decode_stream(ctx, dd, wb, nb_q, data_num, SinkU16{w.qc}, "d3.q");
-  }
-  ctx.mark("huffman");
+  // what needs the ids but not the codes is queued while the helper may still
+  // be tabulating the code tree: the code bitstream's staging, per-chunk
+  // vertex counts and their scans, the unpredictable values and sign bitmaps
+  uint32_t* wq = stage_bits(bo_q, nb_q, "d3.w.q");
   const size_t nch = div_up(n, kVChunk);
   k_chunk_counts3d<false><<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, n, nch, w.cnt, w.cnt + nch);
   scan_u32_to_u64(ctx, w.cnt, w.coff, nch, w.coff + 2 * nch + 2);
   scan_u32_to_u64(ctx, w.cnt + nch, w.coff + nch + 1, nch, w.coff + 2 * nch + 3);
   if (nunp_f) CPSZ_HIP(hipMemcpyAsync(w.unp, d_in + unp_off, 4 * nunp_f, hipMemcpyDeviceToDevice, st));
   CPSZ_HIP(hipMemsetAsync(w.err, 0, 16, st));
+  CPSZ_HIP(hipMemcpyAsync(w.sm, d_in + 8, 3 * smap, hipMemcpyDeviceToDevice, st));
+  q_ready.join();  // rethrows a malformed code tree
+  {
+    DeviceDecoder dd = upload_decoder(ctx, hq, "d3.q");
+    decode_stream(ctx, dd, wq, nb_q, data_num, SinkU16{w.qc}, "d3.q");
+  }
+  ctx.mark("huffman");
   k_scatter3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, w.qc, w.unp, n, nch, w.coff, w.coff + nch + 1,
                                                    data_num, nunp_f, lm.log2f, w.vs, w.q, w.init, w.err);
   CPSZ_HIP(hipGetLastError());
-  CPSZ_HIP(hipMemcpyAsync(w.sm, d_in + 8, 3 * smap, hipMemcpyDeviceToDevice, st));
   w.nunp = nunp_f;
   ctx.mark("scatter");
 }

@@ -274,34 +274,65 @@ bool huff_parse_tree(const uint8_t* tree, size_t tree_size, size_t N, HuffDecode
   // point forward (preorder numbering), so a node's depth is known before its
   // index is reached; internal nodes need two children, no node may have two
   // parents, and codes are at most 64 bits.
-  std::vector<uint8_t>& depth = d.scratch;  // depth + 1 of reached nodes, 0 = unreached
-  depth.assign(N, 0);
+  // (Raw pointers in locals: the byte stores into the depths may alias
+  // anything, which would reload the vectors' data pointers every node.)
+  d.scratch.assign(N, 0);
+  uint8_t* const depth = d.scratch.data();  // depth + 1 of reached nodes, 0 = unreached
+  const uint32_t* const left = d.left.data();
+  const uint32_t* const right = d.right.data();
+  const uint8_t* const is_leaf = d.leaf.data();
   depth[0] = 1;
+  int max_len = 0;
   for (size_t i = 0; i < N; i++) {
     const int di = depth[i];
     if (!di) continue;
-    if (d.leaf[i]) {
-      if (di - 1 > d.max_len) d.max_len = di - 1;
+    if (is_leaf[i]) {
+      if (di - 1 > max_len) max_len = di - 1;
       continue;
     }
-    const uint32_t l = d.left[i], r = d.right[i];
+    const uint32_t l = left[i], r = right[i];
     if (l >= N || r >= N || l <= i || r <= i || l == r || di > 64) return false;
     if (depth[l] || depth[r]) return false;
     depth[l] = depth[r] = (uint8_t)(di + 1);
   }
+  d.max_len = max_len;
   return true;
 }
 
 size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<uint32_t>& lut,
                       std::vector<uint32_t>& pfx) {
   const size_t N = d.node_count;
-  // subtree heights: children follow their parent in preorder numbering
-  // (indices of nodes the root cannot reach are not validated: bounded only)
+  // Level-2 widths: K = the height of a prefix node's subtree, capped at
+  // bits2 (any K in [1, bits2] decodes correctly; K only sizes the table). In
+  // preorder a subtree is its root followed by the run of deeper nodes, so one
+  // forward pass over the depths the parse recorded (depth + 1 of each
+  // reachable node, 0 for the others) finds every prefix node's height; K is
+  // written over the root's own entry, which the pass has already read.
   std::vector<uint8_t>& h = d.scratch;
-  h.assign(N, 0);
-  for (size_t i = N; i-- > 0;)
-    if (!d.leaf[i] && d.left[i] < N && d.right[i] < N)
-      h[i] = (uint8_t)std::min(64, 1 + std::max<int>(h[d.left[i]], h[d.right[i]]));
+  CPSZ_REQUIRE(h.size() == N, kErrInternal, "huffman decode tables need a parsed tree");
+  {
+    const int droot = bits + 1;
+    size_t root = N;
+    int maxd = 0;
+    auto close = [&] {
+      if (root < N) h[root] = (uint8_t)std::max(1, std::min(bits2, maxd - droot));
+      root = N;
+    };
+    for (size_t i = 0; i < N; i++) {
+      const int di = h[i];
+      if (!di) continue;
+      if (di <= droot) {
+        close();
+        if (di == droot && !d.leaf[i]) {
+          root = i;
+          maxd = di;
+        }
+      } else if (di > maxd) {
+        maxd = di;
+      }
+    }
+    close();
+  }
   lut.assign((size_t)1 << bits, 0);
   pfx.clear();
   size_t total = 0;
@@ -325,7 +356,7 @@ size_t huff_build_lut(const HuffDecodeTree& d, int bits, int bits2, std::vector<
       continue;
     }
     if (it.depth == bits) {
-      const uint32_t K = (uint32_t)std::min<int>(h[it.node], bits2);
+      const uint32_t K = (uint32_t)std::max(1, std::min<int>(h[it.node], bits2));
       CPSZ_REQUIRE(total < (1u << 24), kErrUnsupported, "huffman decode tables too large");
       pfx.push_back(it.node);
       pfx.push_back(K);

@@ -36,7 +36,9 @@ struct HuffDecodeTree {
   bool root_leaf = false;                  // one-symbol alphabet
   uint32_t root_sym = 0;
   int max_len = 0;
-  mutable std::vector<uint8_t> scratch;  // working storage of the parse and table build
+  // depth + 1 of each node the parse reached (0 = unreached), read and then
+  // overwritten by huff_build_lut
+  mutable std::vector<uint8_t> scratch;
 };
 
 // Parses `tree_size` bytes of a serialized tree for `node_count` nodes.
