@@ -685,33 +685,9 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   const size_t hist_n = kMaxEbCode + 2 * 65536;
   uint32_t* hh = (uint32_t*)ctx.host_buf("c2.hist", hist_n * 4);
   CPSZ_HIP(hipMemcpyAsync(hh, hist, hist_n * 4, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
-  // the v tree is built on the helper thread beside the eb and u trees; the
-  // tables are kept per thread so their storage is reused from call to call
-  // (the helper reaches the calling thread's objects through references: a
-  // thread_local named in the job would be the helper thread's own)
-  thread_local HuffEncodeTable t_eb, t_u, t_v_keep;
-  thread_local std::vector<uint64_t> f, fv_keep;
-  HuffEncodeTable& t_v = t_v_keep;
-  std::vector<uint64_t>& fv = fv_keep;
-  f.resize(65536);
-  fv.resize(65536);
-  for (int k = 0; k < 65536; k++) fv[k] = hh[kMaxEbCode + 65536 + k];
-  {
-    HelperJob v_tree([&] { huff_build(fv.data(), 65536, 2 * kCapacity, t_v); }, !ctx.batch_worker);
-    for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
-    huff_build(f.data(), kMaxEbCode, 2 * 1024, t_eb);
-    for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + k];
-    huff_build(f.data(), 65536, 2 * kCapacity, t_u);
-    v_tree.join();
-  }
-  ctx.mark("tree");
-  // ---- codebooks and encode pass A (bits per chunk)
-  uint64_t* cb_code = ctx.buf<uint64_t>("c2.cbcode", 3 * 65536);
-  uint8_t* cb_len = ctx.buf<uint8_t>("c2.cblen", 3 * 65536);
-  upload_codebook(ctx, t_eb, cb_code, cb_len, "c2.eb");
-  upload_codebook(ctx, t_u, cb_code + 65536, cb_len + 65536, "c2.u");
-  upload_codebook(ctx, t_v, cb_code + 2 * 65536, cb_len + 2 * 65536, "c2.v");
+  const hipEvent_t hist_done = ctx.record_point();
+  // per-block unpredictable counts and their scans need no tree: they run
+  // while the host builds the trees
   const int nby = (int)div_up(r2, kBlock2d);
   const size_t nb = div_up(r1, kBlock2d) * nby;
   const size_t nch_ids = div_up(2 * n, SrcIds2d::kChunk);
@@ -722,17 +698,47 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   uint32_t* zc = ctx.buf<uint32_t>("c2.zc", 2 * nb);
   uint64_t* zoff = ctx.buf<uint64_t>("c2.zoff", 2 * nb + 4);
   uint64_t* totals = ctx.buf<uint64_t>("c2.totals", 8);
-  k_chunk_bits<<<(int)encode_blocks(nch_ids), 256, 0, st>>>(s_ids, nch_ids, cb_len, bits);
-  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
-  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
   k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, (int)r1, (int)r2, nby, nb, zc);
   k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, (int)r1, (int)r2, nby, nb, zc + nb);
   CPSZ_HIP(hipGetLastError());
-  scan_u64(ctx, bits, offs, nch_ids, totals + 0);
-  scan_u64(ctx, bits + nch_ids, offs + nch_ids, nb, totals + 1);
-  scan_u64(ctx, bits + nch_ids + nb, offs + nch_ids + nb, nb, totals + 2);
   scan_u32_to_u64(ctx, zc, zoff, nb, totals + 3);
   scan_u32_to_u64(ctx, zc + nb, zoff + nb, nb, totals + 4);
+  CPSZ_HIP(hipEventSynchronize(hist_done));
+  // the v tree is built on the helper thread beside the eb and u trees, and
+  // the id stream's bit counts run on the device meanwhile; the tables are
+  // kept per thread so their storage is reused from call to call (the helper
+  // reaches the calling thread's objects through references: a thread_local
+  // named in the job would be the helper thread's own)
+  thread_local HuffEncodeTable t_eb, t_u, t_v_keep;
+  thread_local std::vector<uint64_t> f, fv_keep;
+  HuffEncodeTable& t_v = t_v_keep;
+  std::vector<uint64_t>& fv = fv_keep;
+  f.resize(65536);
+  fv.resize(65536);
+  uint64_t* cb_code = ctx.buf<uint64_t>("c2.cbcode", 3 * 65536);
+  uint8_t* cb_len = ctx.buf<uint8_t>("c2.cblen", 3 * 65536);
+  for (int k = 0; k < 65536; k++) fv[k] = hh[kMaxEbCode + 65536 + k];
+  {
+    HelperJob v_tree([&] { huff_build(fv.data(), 65536, 2 * kCapacity, t_v); }, !ctx.batch_worker);
+    for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
+    huff_build(f.data(), kMaxEbCode, 2 * 1024, t_eb);
+    upload_codebook(ctx, t_eb, cb_code, cb_len, "c2.eb");
+    k_chunk_bits<<<(int)encode_blocks(nch_ids), 256, 0, st>>>(s_ids, nch_ids, cb_len, bits);
+    CPSZ_HIP(hipGetLastError());
+    scan_u64(ctx, bits, offs, nch_ids, totals + 0);
+    for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + k];
+    huff_build(f.data(), 65536, 2 * kCapacity, t_u);
+    v_tree.join();
+  }
+  ctx.mark("tree");
+  // ---- codebooks and encode pass A (bits per chunk)
+  upload_codebook(ctx, t_u, cb_code + 65536, cb_len + 65536, "c2.u");
+  upload_codebook(ctx, t_v, cb_code + 2 * 65536, cb_len + 2 * 65536, "c2.v");
+  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
+  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
+  CPSZ_HIP(hipGetLastError());
+  scan_u64(ctx, bits + nch_ids, offs + nch_ids, nb, totals + 1);
+  scan_u64(ctx, bits + nch_ids + nb, offs + nch_ids + nb, nb, totals + 2);
   uint64_t* ht = (uint64_t*)ctx.host_buf("c2.totals", 8 * 8);
   CPSZ_HIP(hipMemcpyAsync(ht, totals, 5 * 8, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));

@@ -1059,8 +1059,16 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   thread_local std::vector<uint64_t> f;
   f.resize(65536);
   thread_local HuffEncodeTable t_id, t_q;  // kept per thread: their storage is reused
+  uint64_t* cb_code = ctx.buf<uint64_t>("c3.cbcode", 2 * 65536);
+  uint8_t* cb_len = ctx.buf<uint8_t>("c3.cblen", 2 * 65536);
   for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
   huff_build(f.data(), kMaxEbCode, 2 * 256, t_id);
+  // the id stream's bit counts run on the device while the host builds the
+  // code tree
+  upload_codebook(ctx, t_id, cb_code, cb_len, "c3.id");
+  k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_len, bits);
+  CPSZ_HIP(hipGetLastError());
+  scan_u64(ctx, bits, offs, nch, totals + 0);
   uint64_t npred_syms = 0;
   for (int k = 0; k < 65536; k++) npred_syms += (f[k] = hh[kMaxEbCode + k]);
   CPSZ_REQUIRE(npred_syms > 0, kErrUnsupported,
@@ -1068,14 +1076,9 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
                "section)");
   huff_build(f.data(), 65536, 2 * kCapacity, t_q);
   ctx.mark("tree");
-  uint64_t* cb_code = ctx.buf<uint64_t>("c3.cbcode", 2 * 65536);
-  uint8_t* cb_len = ctx.buf<uint8_t>("c3.cblen", 2 * 65536);
-  upload_codebook(ctx, t_id, cb_code, cb_len, "c3.id");
   upload_codebook(ctx, t_q, cb_code + 65536, cb_len + 65536, "c3.q");
-  k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_len, bits);
   k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
   CPSZ_HIP(hipGetLastError());
-  scan_u64(ctx, bits, offs, nch, totals + 0);
   scan_u64(ctx, bits + nch, offs + nch, nch, totals + 1);
   uint64_t* ht = (uint64_t*)ctx.host_buf("c3.totals", 64);
   CPSZ_HIP(hipMemcpyAsync(ht, totals, 32, hipMemcpyDeviceToHost, st));
