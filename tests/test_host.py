@@ -110,6 +110,65 @@ def test_huffman_tree_matches_oracle(seed):
     assert np.array_equal(tree, sec[12:12 + tree_size])
 
 
+def test_huffman_tree_builds_in_sequence_match_oracle():
+    """Tree builds reuse per-thread working storage: alternating large and
+    small alphabets on one thread must each still match the oracle's tree."""
+    orc = load_oracle()
+    rng = np.random.default_rng(77)
+    for it in range(10):
+        state = [131072, 512][it % 2]
+        k = int(rng.integers(20000, 30000)) if state > 600 else int(rng.integers(1, 60))
+        syms = (rng.geometric(0.001 if state > 600 else 0.1, 60000) % k + state // 4).astype(np.int32)
+        freq = np.bincount(syms, minlength=2 * state)
+        tree, nodes = _tree(freq, state)
+        sec = orc.huff_encode(state, syms)
+        tree_size = int(np.frombuffer(sec[8:12].tobytes(), np.uint32)[0])
+        assert nodes == int(np.frombuffer(sec[:8].tobytes(), np.uint64)[0])
+        assert np.array_equal(tree, sec[12:12 + tree_size])
+
+
+def _decode_host(tree, node_count, bits, n):
+    L = cz.lib()
+    L.cpsz_huffman_decode_host.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p, C.c_size_t,
+                                           C.c_size_t, C.c_void_p]
+    tree = np.ascontiguousarray(tree)
+    bits = np.ascontiguousarray(bits)
+    out = np.empty(max(n, 1), np.uint32)
+    rc = L.cpsz_huffman_decode_host(tree.ctypes.data, tree.size, node_count, bits.ctypes.data, bits.size, n,
+                                    out.ctypes.data)
+    return rc, out[:n]
+
+
+def test_huffman_decoder_rejects_malformed_trees():
+    """Tree validation before the decode tables are built: children out of
+    range, pointing backwards, shared by two parents or equal to each other are
+    errors, never a crash; the untouched tree still decodes."""
+    orc = load_oracle()
+    rng = np.random.default_rng(5)
+    state = 2048
+    syms = (rng.geometric(0.002, 20000) % 1000 + 512).astype(np.int32)
+    sec = orc.huff_encode(state, syms)
+    nc = int(np.frombuffer(sec[:8].tobytes(), np.uint64)[0])
+    ts = int(np.frombuffer(sec[8:12].tobytes(), np.uint32)[0])
+    tree = sec[12:12 + ts].copy()
+    nbytes = int(np.frombuffer(sec[12 + ts:20 + ts].tobytes(), np.uint64)[0])
+    bits = sec[20 + ts:20 + ts + nbytes].copy()
+    rc, out = _decode_host(tree, nc, bits, syms.size)
+    assert rc == 0 and np.array_equal(out, syms.astype(np.uint32))
+    assert nc > 256 and nc <= 65536  # u16 child indices
+    left = tree[1:1 + 2 * nc].view(np.uint16)
+    right = tree[1 + 2 * nc:1 + 4 * nc].view(np.uint16)
+    internal = np.flatnonzero(left[1:] != 0) + 1
+    p = int(internal[len(internal) // 2])
+    for field, value in [("left", nc + 5), ("right", 0xFFFF), ("left", p), ("right", int(left[p])),
+                         ("right", int(right[0]))]:
+        bad = tree.copy()
+        arr = (bad[1:1 + 2 * nc] if field == "left" else bad[1 + 2 * nc:1 + 4 * nc]).view(np.uint16)
+        arr[p] = value
+        rc, _ = _decode_host(bad, nc, bits, syms.size)
+        assert rc != 0, (field, value)
+
+
 def _emulate(x, c, wide):
     L = cz.lib()
     L.cpsz_lorenzo2d_emulate.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_size_t, C.c_int,
