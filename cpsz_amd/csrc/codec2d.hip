@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "codec2d.hpp"
@@ -292,11 +293,17 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
   uint64_t* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
   float Rprev = 0.f, Uprev = 0.f;
   const int steps = r2 + 63;
-  auto load = [&](StripChunk& B, int t0) {
+  // An interior chunk has every lane's kCH columns inside the row and all 64
+  // rows present, so none of its steps needs a bounds mask; most chunks are
+  // interior and run the unmasked body.
+  const bool full_rows = s * 64 + 64 <= r1;
+  auto interior = [&](int t0) { return full_rows && t0 >= 63 && t0 + kCH <= r2; };
+  auto load = [&](StripChunk& B, int t0, auto inner) {
+    constexpr bool IN = decltype(inner)::value;
 #pragma unroll
     for (int k = 0; k < kCH; k++) {
       const int j = t0 + k - lane;
-      const bool act = rvalid && j >= 0 && j < r2;
+      const bool act = IN || (rvalid && j >= 0 && j < r2);
       B.cs[k] = act ? C[j] : 0;
       if (COMPRESS) {
         B.xs[k] = act ? X[j] : 0.f;
@@ -306,7 +313,8 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       }
     }
   };
-  auto chunk = [&](const StripChunk& B, StripChunk& N, int t0) {
+  auto chunk = [&](const StripChunk& B, StripChunk& N, int t0, auto inner) {
+    constexpr bool IN = decltype(inner)::value;
     float inc = 0.f;
     if (has_up && t0 < r2) {
       const bool want = lane < kCH && t0 + lane < r2;
@@ -317,12 +325,15 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       }
       inc = __uint_as_float((uint32_t)v);
     }
-    if (t0 + kCH < steps) load(N, t0 + kCH);
+    if (t0 + kCH < steps) {
+      if (interior(t0 + kCH)) load(N, t0 + kCH, std::true_type{});
+      else load(N, t0 + kCH, std::false_type{});
+    }
 #pragma unroll
     for (int k = 0; k < kCH; k++) {
       const int t = t0 + k;
       const int j = t - lane;
-      const bool act = rvalid && j >= 0 && j < r2;
+      const bool act = IN || (rvalid && j >= 0 && j < r2);
       const float lane0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inc), k));
       const float Uv = shr1(Rprev, lane0);
       const float pred = (Rprev + Uv) - Uprev;
@@ -337,14 +348,18 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       }
       if (publish && lane == 63 && act) st_coh64(bnd_out + j, kBndTag | __float_as_uint(R));
       Uprev = Uv;
-      Rprev = (j < 0) ? 0.f : R;
+      Rprev = (!IN && j < 0) ? 0.f : R;
     }
   };
+  auto run = [&](const StripChunk& B, StripChunk& N, int t0) {
+    if (interior(t0)) chunk(B, N, t0, std::true_type{});
+    else chunk(B, N, t0, std::false_type{});
+  };
   StripChunk A, B;
-  load(A, 0);
+  load(A, 0, std::false_type{});
   for (int t0 = 0; t0 < steps; t0 += 2 * kCH) {
-    chunk(A, B, t0);
-    if (t0 + kCH < steps) chunk(B, A, t0 + kCH);
+    run(A, B, t0);
+    if (t0 + kCH < steps) run(B, A, t0 + kCH);
   }
 }
 

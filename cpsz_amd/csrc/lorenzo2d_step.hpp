@@ -25,21 +25,23 @@ namespace cpsz {
 
 template <bool WIDE>
 __host__ __device__ inline float l2d_compress_step(float pred, float x, uint32_t c, uint16_t& code) {
-  if (c == 0) {
-    code = 0;
-    return x;
-  }
-  const int id = (int)c - 1;
   if (!WIDE) {
+    // branch-free: code 0 is computed with id 0 and then rejected
+    const int id = c ? (int)c - 1 : 0;
     const float diff = x - pred;
     const float a = ldexpf(fabsf(diff), 52 - 2 * id);
     const float qf = floorf(fmaf(floorf(a), 0.5f, 0.5f));
     const float qs = copysignf(qf, diff);
     const float rec = pred + ldexpf(qs, 2 * id - 51);
-    const bool ok = (a < 65535.0f) && (fabsf(rec - x) <= ldexpf(1.0f, 2 * id - 52));
+    const bool ok = (c != 0) & (a < 65535.0f) & (fabsf(rec - x) <= ldexpf(1.0f, 2 * id - 52));
     code = ok ? (uint16_t)((int)qs + 32768) : 0;
     return ok ? rec : x;
   } else {
+    if (c == 0) {
+      code = 0;
+      return x;
+    }
+    const int id = (int)c - 1;
     const double prec = ldexp(1.0, 2 * id - 52);
     const double diff = x - pred;
     double qd = fabs(diff) / prec + 1;
@@ -63,11 +65,11 @@ __host__ __device__ inline float l2d_compress_step(float pred, float x, uint32_t
 // (src/sz_decompress_cp_preserve_2d.cpp:30-33).
 template <bool WIDE>
 __host__ __device__ inline float l2d_decompress_step(float pred, uint32_t q, uint32_t sid, float raw) {
-  if (q == 0) return raw;
   if (!WIDE) {
     const float kE = sid ? ldexpf((float)(2 * ((int)q - 32768)), 2 * (int)sid - 52) : 0.f;
-    return pred + kE;
+    return q == 0 ? raw : pred + kE;
   }
+  if (q == 0) return raw;
   const double prec = sid ? ldexp(1.0, 2 * (int)sid - 52) : 0.0;
   return (float)((double)pred + 2 * ((int)q - 32768) * prec);
 }
