@@ -992,42 +992,40 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
       CPSZ_HIP(hipMemcpyAsync(w, d_in + h.bits_off, h.nbytes, hipMemcpyDeviceToDevice, st));
     return w;
   };
-  // the code trees are parsed and tabulated on the helper thread one step
-  // ahead of the device: the u tree while the id stream decodes, the v tree
-  // while the u stream decodes. The tables are kept per calling thread (their
-  // storage is reused); the helper fills them through the pointer (a
-  // thread_local named in the job would be the helper thread's own).
+  // The code trees are parsed and tabulated while the device decodes the id
+  // stream (the u tree on the helper thread, the v tree on this one between
+  // queuing the id decode and waiting for it); then the u and v streams decode
+  // in lockstep. The tables are kept per calling thread (their storage is
+  // reused); the helper fills them through the pointer (a thread_local named
+  // in the job would be the helper thread's own).
   thread_local HostDecoder hc_keep[2];
   HostDecoder* hc = hc_keep;
-  const bool beside = !ctx.batch_worker;
   auto prep = [&](int c) {
     prepare_decoder(comp[c].h.tree.data(), comp[c].h.tree.size(), comp[c].h.node_count, hc[c]);
   };
-  uint16_t* qs[2] = {qu, qv};
-  auto decode_codes = [&](int c) {
-    uint32_t* w = stage_bits(comp[c].h, c ? "d2.w.v" : "d2.w.u");
-    DeviceDecoder dd = upload_decoder(ctx, hc[c], c ? "d2.v" : "d2.u");
-    decode_stream(ctx, dd, w, comp[c].h.nbytes, n, SinkCodes2d{qs[c], (int)r1, (int)r2, nby},
-                  c ? "d2.v" : "d2.u");
-  };
   {
-    HelperJob u_ready([&] { prep(0); }, beside);
+    HelperJob u_ready([&] { prep(0); }, !ctx.batch_worker);
     uint32_t* w = stage_bits(h_eb, "d2.w.eb");
     DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
-    decode_stream(ctx, dd, w, h_eb.nbytes, 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb");
+    decode_stream(ctx, dd, w, h_eb.nbytes, 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb", [&] { prep(1); });
     u_ready.join();  // rethrows a malformed code tree
   }
   {
-    HelperJob v_ready([&] { prep(1); }, beside);
-    decode_codes(0);
-    v_ready.join();
+    uint32_t* wu = stage_bits(comp[0].h, "d2.w.u");
+    uint32_t* wv = stage_bits(comp[1].h, "d2.w.v");
+    const DeviceDecoder du = upload_decoder(ctx, hc[0], "d2.u");
+    const DeviceDecoder dv = upload_decoder(ctx, hc[1], "d2.v");
+    DecJob<SinkCodes2d> jobs[2] = {
+        {&du, wu, comp[0].h.nbytes, n, SinkCodes2d{qu, (int)r1, (int)r2, nby}, "d2.u"},
+        {&dv, wv, comp[1].h.nbytes, n, SinkCodes2d{qv, (int)r1, (int)r2, nby}, "d2.v"}};
+    decode_streams(ctx, jobs, 2);
   }
-  decode_codes(1);
   ctx.mark("huffman");
   // ---- unpredictable values to per-element raw arrays (block-major order)
   uint32_t* zc = ctx.buf<uint32_t>("d2.zc", 2 * nb);
   uint64_t* zoff = ctx.buf<uint64_t>("d2.zoff", 2 * nb + 4);
   CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
+  uint16_t* qs[2] = {qu, qv};
   for (int c = 0; c < 2; c++) {
     float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", comp[c].unp_count + 1);
     if (comp[c].unp_count)

@@ -19,6 +19,8 @@
 
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
+#include <functional>
+#include <string>
 #include <type_traits>
 #include <vector>
 
@@ -422,61 +424,130 @@ DeviceDecoder upload_decoder(Ctx& ctx, const HostDecoder& h, const char* tag);
 DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
                              const char* tag);
 
-// Decodes n symbols of a bitstream already copied to aligned device words
-// (padded with at least 8 zero bytes) into `sink`.
+// One bitstream to decode: n symbols of words already copied to aligned
+// device words (padded with at least 8 zero bytes) into `sink`; `tag` names
+// its buffers.
 template <class Sink>
-void decode_stream(Ctx& ctx, const DeviceDecoder& dd, const uint32_t* d_words, uint64_t nbytes,
-                   uint64_t n, Sink sink, const char* tag);
+struct DecJob {
+  const DeviceDecoder* dd;
+  const uint32_t* words;
+  uint64_t nbytes, n;
+  Sink sink;
+  const char* tag;
+};
 
+// Decodes several independent bitstreams in lockstep on the context's stream:
+// their passes are queued together, so one host check serves every stream of
+// a round, and one read of the totals serves all. `overlap` (may be empty) is
+// host work run once the first round is queued, before the first wait.
 template <class Sink>
-void decode_stream(Ctx& ctx, const DeviceDecoder& dd, const uint32_t* d_words, uint64_t nbytes,
-                   uint64_t n, Sink sink, const char* tag) {
+void decode_streams(Ctx& ctx, DecJob<Sink>* jobs, int nj, const std::function<void()>& overlap = {}) {
   hipStream_t st = ctx.stream();
-  if (n == 0) return;
-  if (dd.single) {
-    k_dec_const<<<1024, 256, 0, st>>>(n, dd.single_sym, sink);
-    CPSZ_HIP(hipGetLastError());
-    return;
-  }
-  const uint64_t nbits = nbytes * 8;
-  CPSZ_REQUIRE(nbits > 0, kErrStream, "empty huffman bitstream for a multi-symbol tree");
-  uint32_t S = 256;
-  while (div_up(nbits, S) > (1u << 21)) S *= 2;
-  const uint32_t nseg = (uint32_t)div_up(nbits, S);
-  std::string t(tag);
-  uint64_t* start[2] = {ctx.buf<uint64_t>((t + ".st0").c_str(), nseg), ctx.buf<uint64_t>((t + ".st1").c_str(), nseg)};
-  uint64_t* ex[2] = {ctx.buf<uint64_t>((t + ".ex0").c_str(), nseg), ctx.buf<uint64_t>((t + ".ex1").c_str(), nseg)};
-  uint32_t* cnt = ctx.buf<uint32_t>((t + ".cnt").c_str(), nseg);
-  uint64_t* off = ctx.buf<uint64_t>((t + ".off").c_str(), nseg + 1);
-  int* flags = ctx.buf<int>("dec.flags", 2);
-  int* hflags = (int*)ctx.host_buf("dec.flags", 2 * sizeof(int));
-  const int blocks = (int)div_up(nseg, 256);
-  CPSZ_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(int), st));
-  k_dec_spec<<<blocks, 256, 0, st>>>(dd.tree, d_words, nbits, S, nseg, nullptr, nullptr, start[0],
-                                     ex[0], cnt, flags, flags + 1, 1);
-  CPSZ_HIP(hipGetLastError());
-  int cur = 0;
-  for (int round = 0; round < 4096; round++) {
-    // two fix-up rounds per host check
-    for (int k = 0; k < 2; k++) {
-      k_dec_spec<<<blocks, 256, 0, st>>>(dd.tree, d_words, nbits, S, nseg, ex[cur], start[cur],
-                                         start[cur ^ 1], ex[cur ^ 1], cnt, flags, flags + 1, 0);
-      cur ^= 1;
+  struct State {
+    uint64_t nbits = 0;
+    uint32_t S = 256, nseg = 0;
+    int blocks = 0, cur = 0;
+    bool active = false, done = false;
+    uint64_t *start[2] = {nullptr, nullptr}, *ex[2] = {nullptr, nullptr}, *off = nullptr;
+    uint32_t* cnt = nullptr;
+    int* flags = nullptr;
+  };
+  std::vector<State> sv(nj);
+  bool overlapped = false;
+  auto run_overlap = [&] {
+    if (!overlapped && overlap) {
+      overlapped = true;
+      overlap();
     }
-    CPSZ_HIP(hipMemsetAsync(flags, 0, sizeof(int), st));
-    k_dec_check<<<blocks, 256, 0, st>>>(nseg, ex[cur], start[cur], flags);
-    CPSZ_HIP(hipMemcpyAsync(hflags, flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
-    CPSZ_HIP(hipStreamSynchronize(st));
-    CPSZ_REQUIRE(hflags[1] == 0, kErrStream, "invalid huffman code in stream");
-    if (!hflags[0]) break;
+  };
+  for (int k = 0; k < nj; k++) {
+    const DecJob<Sink>& j = jobs[k];
+    State& s = sv[k];
+    if (j.n == 0) continue;
+    if (j.dd->single) {
+      k_dec_const<<<1024, 256, 0, st>>>(j.n, j.dd->single_sym, j.sink);
+      CPSZ_HIP(hipGetLastError());
+      continue;
+    }
+    s.nbits = j.nbytes * 8;
+    CPSZ_REQUIRE(s.nbits > 0, kErrStream, "empty huffman bitstream for a multi-symbol tree");
+    while (div_up(s.nbits, s.S) > (1u << 21)) s.S *= 2;
+    s.nseg = (uint32_t)div_up(s.nbits, s.S);
+    const std::string t(j.tag);
+    s.start[0] = ctx.buf<uint64_t>((t + ".st0").c_str(), s.nseg);
+    s.start[1] = ctx.buf<uint64_t>((t + ".st1").c_str(), s.nseg);
+    s.ex[0] = ctx.buf<uint64_t>((t + ".ex0").c_str(), s.nseg);
+    s.ex[1] = ctx.buf<uint64_t>((t + ".ex1").c_str(), s.nseg);
+    s.cnt = ctx.buf<uint32_t>((t + ".cnt").c_str(), s.nseg);
+    s.off = ctx.buf<uint64_t>((t + ".off").c_str(), s.nseg + 1);
+    s.flags = ctx.buf<int>((t + ".flags").c_str(), 2);
+    s.blocks = (int)div_up(s.nseg, 256);
+    s.active = true;
+    CPSZ_HIP(hipMemsetAsync(s.flags, 0, 2 * sizeof(int), st));
+    k_dec_spec<<<s.blocks, 256, 0, st>>>(j.dd->tree, j.words, s.nbits, s.S, s.nseg, nullptr, nullptr, s.start[0],
+                                         s.ex[0], s.cnt, s.flags, s.flags + 1, 1);
+    CPSZ_HIP(hipGetLastError());
   }
-  scan_u32_to_u64(ctx, cnt, off, nseg, off + nseg);
-  uint64_t total = 0;
-  CPSZ_HIP(hipMemcpyAsync(&total, off + nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+  int* hflags = (int*)ctx.host_buf("dec.hflags", 2 * sizeof(int) * (size_t)(nj > 0 ? nj : 1));
+  for (int round = 0; round < 4096; round++) {
+    bool any = false;
+    for (int k = 0; k < nj; k++) {
+      State& s = sv[k];
+      if (!s.active || s.done) continue;
+      const DecJob<Sink>& j = jobs[k];
+      // two fix-up rounds per host check
+      for (int r = 0; r < 2; r++) {
+        k_dec_spec<<<s.blocks, 256, 0, st>>>(j.dd->tree, j.words, s.nbits, s.S, s.nseg, s.ex[s.cur],
+                                             s.start[s.cur], s.start[s.cur ^ 1], s.ex[s.cur ^ 1], s.cnt, s.flags,
+                                             s.flags + 1, 0);
+        s.cur ^= 1;
+      }
+      CPSZ_HIP(hipMemsetAsync(s.flags, 0, sizeof(int), st));
+      k_dec_check<<<s.blocks, 256, 0, st>>>(s.nseg, s.ex[s.cur], s.start[s.cur], s.flags);
+      CPSZ_HIP(hipMemcpyAsync(hflags + 2 * k, s.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+      any = true;
+    }
+    run_overlap();
+    if (!any) break;
+    CPSZ_HIP(hipStreamSynchronize(st));
+    for (int k = 0; k < nj; k++) {
+      State& s = sv[k];
+      if (!s.active || s.done) continue;
+      CPSZ_REQUIRE(hflags[2 * k + 1] == 0, kErrStream, "invalid huffman code in stream");
+      if (!hflags[2 * k]) s.done = true;
+    }
+  }
+  run_overlap();
+  uint64_t* htot = (uint64_t*)ctx.host_buf("dec.htotals", sizeof(uint64_t) * (size_t)(nj > 0 ? nj : 1));
+  bool any = false;
+  for (int k = 0; k < nj; k++) {
+    State& s = sv[k];
+    if (!s.active) continue;
+    scan_u32_to_u64(ctx, s.cnt, s.off, s.nseg, s.off + s.nseg);
+    CPSZ_HIP(hipMemcpyAsync(htot + k, s.off + s.nseg, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    any = true;
+  }
+  if (!any) return;
   CPSZ_HIP(hipStreamSynchronize(st));
-  CPSZ_REQUIRE(total >= n, kErrStream, "huffman bitstream holds fewer symbols than expected");
-  k_dec_write<<<blocks, 256, 0, st>>>(dd.tree, d_words, nbits, S, nseg, start[cur], ex[cur], off, n, sink);
-  CPSZ_HIP(hipGetLastError());
+  for (int k = 0; k < nj; k++)
+    if (sv[k].active)
+      CPSZ_REQUIRE(htot[k] >= jobs[k].n, kErrStream, "huffman bitstream holds fewer symbols than expected");
+  for (int k = 0; k < nj; k++) {
+    const State& s = sv[k];
+    if (!s.active) continue;
+    const DecJob<Sink>& j = jobs[k];
+    k_dec_write<<<s.blocks, 256, 0, st>>>(j.dd->tree, j.words, s.nbits, s.S, s.nseg, s.start[s.cur], s.ex[s.cur],
+                                          s.off, j.n, j.sink);
+    CPSZ_HIP(hipGetLastError());
+  }
+}
+
+// Decodes n symbols of one bitstream (see decode_streams).
+template <class Sink>
+void decode_stream(Ctx& ctx, const DeviceDecoder& dd, const uint32_t* d_words, uint64_t nbytes, uint64_t n,
+                   Sink sink, const char* tag, const std::function<void()>& overlap = {}) {
+  DecJob<Sink> j{&dd, d_words, nbytes, n, sink, tag};
+  decode_streams(ctx, &j, 1, overlap);
 }
 
 }  // namespace cpsz
