@@ -32,7 +32,7 @@ namespace cpsz {
 
 constexpr int kLutBits = 12;
 constexpr int kLut2Bits = 12;  // width cap of the level-2 decode tables
-constexpr uint64_t kDecWarm = 128;  // bits decoded before a segment in the speculative pass
+constexpr uint64_t kDecWarm = 384;  // bits decoded before a segment in the speculative pass
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
 
