@@ -30,8 +30,9 @@
 
 namespace cpsz {
 
-constexpr int kLutBits = 12;
+constexpr int kLutBits = 11;
 constexpr int kLut2Bits = 12;  // width cap of the level-2 decode tables
+constexpr int kFixPerCheck = 2;   // fix-up passes queued per host convergence check
 constexpr uint64_t kDecWarm = 384;  // bits decoded before a segment in the speculative pass
 
 __device__ __forceinline__ uint32_t bswap32(uint32_t x) { return __builtin_bswap32(x); }
@@ -495,8 +496,8 @@ void decode_streams(Ctx& ctx, DecJob<Sink>* jobs, int nj, const std::function<vo
       State& s = sv[k];
       if (!s.active || s.done) continue;
       const DecJob<Sink>& j = jobs[k];
-      // two fix-up rounds per host check
-      for (int r = 0; r < 2; r++) {
+      // kFixPerCheck fix-up rounds per host check
+      for (int r = 0; r < kFixPerCheck; r++) {
         k_dec_spec<<<s.blocks, 256, 0, st>>>(j.dd->tree, j.words, s.nbits, s.S, s.nseg, s.ex[s.cur],
                                              s.start[s.cur], s.start[s.cur ^ 1], s.ex[s.cur ^ 1], s.cnt, s.flags,
                                              s.flags + 1, 0);

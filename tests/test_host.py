@@ -318,7 +318,7 @@ def test_3d_arithmetic_matches_oracle(shape, seed, eb):
 
 @pytest.mark.parametrize("seed", range(6))
 def test_two_level_decode_tables_match_oracle_encoder(seed):
-    """The GPU decoder's two-level tables (12-bit level 1, level-2 tables over
+    """The GPU decoder's two-level tables (11-bit level 1, level-2 tables over
     up to 12 more bits, tree walk beyond), run on the host, decode the oracle's
     Huffman sections exactly, including codes longer than 24 bits."""
     orc = load_oracle()
