@@ -77,6 +77,9 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
     c2_bind(w[k], scratch + soff[k]);
     wp[k] = &w[k];
   }
+  // inputs written on the caller's stream must be complete before the workers'
+  // streams read them
+  CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
   // bounds passes on the workers' streams: small fields fill only part of the
   // device, so several run at once
   {

@@ -78,6 +78,9 @@ void compress_3d_batch(Ctx& ctx, const std::vector<Field3d>& fields, double max_
     c3_bind(w[k], scratch + soff[k]);
     wp[k] = &w[k];
   }
+  // inputs written on the caller's stream must be complete before the workers'
+  // streams read them
+  CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
   // bounds passes on the workers' streams: one field's grid fills only part of
   // the device, so several fields run at once
   {

@@ -1268,7 +1268,7 @@ void d3_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
     tree.assign(t, t + ts);
     nbytes = rd.get<size_t>(at + 12 + ts);
     bits_off = at + 20 + ts;
-    CPSZ_REQUIRE(bits_off + nbytes <= in_size, kErrStream, "huffman bitstream truncated");
+    CPSZ_REQUIRE(bits_off <= in_size && nbytes <= in_size - bits_off, kErrStream, "huffman bitstream truncated");
     return bits_off + nbytes;
   };
   size_t nc_id, nb_id, bo_id, nc_q, nb_q, bo_q;

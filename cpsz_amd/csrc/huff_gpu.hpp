@@ -519,6 +519,11 @@ void decode_streams(Ctx& ctx, DecJob<Sink>* jobs, int nj, const std::function<vo
     }
   }
   run_overlap();
+  // a stream whose segment starts have not settled within the round limit
+  // would decode from inconsistent starts: refuse it
+  for (int k = 0; k < nj; k++)
+    CPSZ_REQUIRE(!sv[k].active || sv[k].done, kErrStream,
+                 "huffman decode did not converge (bitstream does not resynchronize)");
   uint64_t* htot = (uint64_t*)ctx.host_buf("dec.htotals", sizeof(uint64_t) * (size_t)(nj > 0 ? nj : 1));
   bool any = false;
   for (int k = 0; k < nj; k++) {

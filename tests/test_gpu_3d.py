@@ -126,3 +126,33 @@ def test_fast_bound_premises():
     assert ulp <= 2
     assert ratio <= 1.0
     print(f"max ulp {ulp}, max error/envelope {ratio:.3f}", flush=True)
+
+
+def _id_section_nbytes_offset(s, n):
+    """Offset of the eb-id Huffman section's bitstream byte count (layout:
+    int base | int radius | 3 sign maps | size_t nunp | floats | size_t n |
+    size_t nodes | u32 tree_size | tree | size_t nbytes | bits)."""
+    smap = (n - 1) // 8 + 1
+    off = 8 + 3 * smap
+    nunp = int(np.frombuffer(s[off:off + 8].tobytes(), np.uint64)[0])
+    off += 8 + 4 * nunp + 8
+    ts = int(np.frombuffer(s[off + 8:off + 12].tobytes(), np.uint32)[0])
+    return off + 12 + ts
+
+
+@pytest.mark.parametrize("case", ["truncated", "huge_nbytes", "wrapping_nbytes"])
+def test_malformed_3d_stream_rejected(case):
+    """Truncated streams and bitstream byte counts that overflow the bounds
+    check are rejected as stream errors before any decode kernel runs."""
+    print(f"[3d] malformed stream: {case}", flush=True)
+    U, V, W = field_3d(9, 10, 11, 5)
+    s = cz.sz_compress_cp_preserve_3d_offline_log(U, V, W, 0.05).copy()
+    if case == "truncated":
+        s = s[: s.size // 2]
+    else:
+        at = _id_section_nbytes_offset(s, U.size)
+        bad = (1 << 63) if case == "huge_nbytes" else (1 << 64) - 16
+        s[at:at + 8] = np.frombuffer(np.uint64(bad).tobytes(), np.uint8)
+    with pytest.raises(cz.CpszError) as e:
+        cz.sz_decompress_cp_preserve_3d_online_log(s, 9, 10, 11)
+    assert e.value.code == -4

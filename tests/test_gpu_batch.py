@@ -65,3 +65,28 @@ def test_batch_round_trip_matches_single_decoder(batch):
         dU, dV = cz.sz_decompress_cp_preserve_2d_offline(allb[off[k]:off[k] + size[k]], *shape)
         assert np.array_equal(outs[k][0].cpu().numpy().view(np.uint32), dU.view(np.uint32)), shape
         assert np.array_equal(outs[k][1].cpu().numpy().view(np.uint32), dV.view(np.uint32)), shape
+
+
+def test_batch_compress_waits_for_inputs_on_caller_stream(batch):
+    """Inputs written by a kernel queued on the caller's stream (behind slow
+    work) are what the batch compresses: the bounds passes, which run on the
+    library's worker streams, wait for the caller's stream first."""
+    fields, _ = batch
+    print("[batch] inputs written on a side stream", flush=True)
+    side = torch.cuda.Stream()
+    src = [(torch.from_numpy(U).cuda(), torch.from_numpy(V).cuda()) for U, V in fields]
+    dst = [(torch.zeros_like(u), torch.zeros_like(v)) for u, v in src]
+    big = torch.randn(4096, 4096, device="cuda")
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        for _ in range(8):  # delays the copies below by milliseconds
+            big = torch.tanh(big @ big)
+        for (u, v), (a, b) in zip(src, dst):
+            a.copy_(u)
+            b.copy_(v)
+    base, off, size = cz.compress_2d_batch_dev([(a.data_ptr(), b.data_ptr(), *SHAPES[k]) for k, (a, b)
+                                                in enumerate(dst)], 0.1, stream=side.cuda_stream)
+    side.synchronize()
+    allb = _host(base, int(off[-1] + size[-1]))
+    for k, ((U, V), shape) in enumerate(zip(fields, SHAPES)):
+        assert np.array_equal(allb[off[k]:off[k] + size[k]], cz.sz_compress_cp_preserve_2d_offline(U, V, 0.1)), shape
