@@ -15,15 +15,20 @@
 //   2D offline eb derivation + eb ids ....... pinned (golden streams)
 //   2D Lorenzo codec (compress/decompress) .. pinned (golden streams, decoded
 //                                             fields from the compiled reference)
-//   3D offline_log compressor ............... partially pinned: stream decoded
-//                                             by the compiled reference 3D
-//                                             decoder; acbrt1 against cbrt; the
-//                                             per-tet bound by brute-force sign
-//                                             checks. The reference compressor
-//                                             itself does not build here (its
-//                                             file includes FTK), so the exact
-//                                             per-tet bound bits are parity
-//                                             unpinned beyond those checks.
+//   3D offline_log compressor ............... pinned without the reference
+//                                             compressor (its file includes FTK):
+//                                             acbrt1 (every float), the sign
+//                                             bound, same_direction, eb
+//                                             quantization and log_transform bit
+//                                             for bit against the reference
+//                                             header compiled into _ref; the
+//                                             per-tet bound against exact
+//                                             cofactors on integer tetrahedra;
+//                                             vertex bounds against the Kuhn
+//                                             split; the stream through the
+//                                             compiled reference decoder.
+//                                             Planted mistakes (ORC_MUTANT in
+//                                             orc_3d.cpp) each fail a pin.
 //   3D online_log decompressor .............. pinned (compiled reference decoder)
 //   CP extraction (2D/3D) ................... parity unpinned against FTK
 //                                             (absent); pinned by exact
@@ -107,7 +112,18 @@ void eb_3d_offline(const T* U, const T* V, const T* W, size_t r1, size_t r2, siz
 // eb id for a vertex bound as the 3D compressor derives it (:470-476):
 // 0 when required_eb <= 0, else eb_exponential_quantize(log2(1+eb), 2, 1, FLT_EPSILON).
 int eb_id_3d(double required_eb);
+// eb_exponential_quantize (include/sz_cp_preserve_utils.hpp:14-22): id, and eb
+// replaced by base^id * threshold (0 at or below the threshold).
+int eb_exp_quantize(double& eb, int base, double log_of_base, double threshold);
+// log_transform (include/sz_cp_preserve_utils.hpp:205-221): log2f|x| and the
+// sign flag (1 = positive) per value, -100 and 0 for zeros.
+template <typename T>
+void log_transform_3d(const T* x, size_t n, T* logv, uint8_t* sign);
 float acbrt1(float x);
+// max_eb_to_keep_sign_3d_offline (include/sz_cp_preserve_utils.hpp:186-191)
+float sign_bound_3d(float positive, float negative);
+// same_direction (include/sz_cp_preserve_utils.hpp:52-57)
+bool same_direction_3d(float a, float b, float c, float d);
 // max_eb_to_keep_position_and_type_3d_offline (:264-358)
 double tet_bound_3d(const float u[4], const float v[4], const float w[4]);
 // max_eb_to_keep_position_and_type (src/sz_compress_cp_preserve_2d.cpp:125-172)

@@ -5,6 +5,19 @@
 // include/sz_cp_preserve_utils.hpp:14-22 (eb quantization), :41-57
 // (same_direction), :167-191 (acbrt1 bound), :205-221 (log transform).
 // Decompressor: src/sz_decompress_cp_preserve_3d.cpp:7-110.
+//
+// ORC_MUTANT (default 0) plants one known mistake per value, so that a CPU
+// test can show each is caught by at least one parity pin:
+//   1 MIN/MAX swapped in the per-tet combination (:339-354)
+//   2 cofactor sums kept in double instead of float (:334-335)
+//   3 '>' instead of '>=' in the unpredictable test (:507)
+//   4 first tetrahedron's vertex order permuted (:9-16)
+//   5 cbrtf in place of acbrt1 in the sign bound (sz_cp_preserve_utils.hpp:188-189)
+//   6 first tetrahedron replaced by one outside the cell's Kuhn split (:9-16)
+// Builds with ORC_MUTANT != 0 exist only inside that test.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -30,6 +43,7 @@ inline void acc(V value, double& pos, double& neg) {
 template <typename T>
 inline int sgn(T v) { return (T(0) < v) - (v < T(0)); }
 
+// same_direction (sz_cp_preserve_utils.hpp:52-57)
 template <typename T>
 inline bool same_direction(T a, T b, T c, T d) {
   int s = sgn(a);
@@ -37,12 +51,21 @@ inline bool same_direction(T a, T b, T c, T d) {
   return s == sgn(b) && s == sgn(c) && s == sgn(d);
 }
 
-// max_eb_to_keep_sign_3d_offline (sz_cp_preserve_utils.hpp:186-191)
-inline float cube_sign_bound(float positive, float negative) {
-  if (positive == 0 && negative == 0) return 0;
-  float P = acbrt1(positive);
-  float N = acbrt1(negative);
-  return std::fabs(P - N) / (P + N);
+inline float cube_sign_bound(float positive, float negative) { return sign_bound_3d(positive, negative); }
+
+#if ORC_MUTANT == 2
+using CofSum = double;
+#else
+using CofSum = float;  // float p_m0 = positive_m0, ... (:334-335)
+#endif
+
+// eb = MAX(eb, MIN(eb1, eb2)) (:339-354)
+inline float combine(float eb, float e1, float e2) {
+#if ORC_MUTANT == 1
+  return minm(eb, maxm(e1, e2));
+#else
+  return maxm(eb, minm(e1, e2));
+#endif
 }
 
 template <typename T>
@@ -86,23 +109,23 @@ double tet_bound(T u0, T u1, T u2, T u3, T v0, T v1, T v2, T v3, T w0, T w1, T w
   acc(-u1v3w0, p2, n2); acc(-u0v1w3, p2, n2); acc(u1v0w3, p2, n2);
   acc(u0v1w2, p3, n3); acc(-u0v2w1, p3, n3); acc(u1v2w0, p3, n3);
   acc(-u1v0w2, p3, n3); acc(u2v0w1, p3, n3); acc(-u2v1w0, p3, n3);
-  float pm0 = p0, pm1 = p1, pm2 = p2, pm3 = p3;
-  float nm0 = n0, nm1 = n1, nm2 = n2, nm3 = n3;
+  CofSum pm0 = p0, pm1 = p1, pm2 = p2, pm3 = p3;
+  CofSum nm0 = n0, nm1 = n1, nm2 = n2, nm3 = n3;
   if (!flag[0]) {
     float e1 = cube_sign_bound(pm0, nm0), e2 = cube_sign_bound(pm1 + pm2 + pm3, nm1 + nm2 + nm3);
-    eb = maxm(eb, minm(e1, e2));
+    eb = combine(eb, e1, e2);
   }
   if (!flag[1]) {
     float e1 = cube_sign_bound(pm1, nm1), e2 = cube_sign_bound(pm0 + pm2 + pm3, nm0 + nm2 + nm3);
-    eb = maxm(eb, minm(e1, e2));
+    eb = combine(eb, e1, e2);
   }
   if (!flag[2]) {
     float e1 = cube_sign_bound(pm2, nm2), e2 = cube_sign_bound(pm0 + pm1 + pm3, nm0 + nm1 + nm3);
-    eb = maxm(eb, minm(e1, e2));
+    eb = combine(eb, e1, e2);
   }
   if (!flag[3]) {
     float e1 = cube_sign_bound(pm3, nm3), e2 = cube_sign_bound(pm0 + pm1 + pm2, nm0 + nm1 + nm2);
-    eb = maxm(eb, minm(e1, e2));
+    eb = combine(eb, e1, e2);
   }
   return eb;
 }
@@ -110,7 +133,13 @@ double tet_bound(T u0, T u1, T u2, T u3, T v0, T v1, T v2, T v3, T w0, T w1, T w
 // Vertex offsets (k, j, i) of the six tetrahedra of a cell; the cell's origin
 // vertex is the fourth vertex of each (src/sz_compress_cp_preserve_3d.cpp:9-46).
 const int kTet[6][3][3] = {
+#if ORC_MUTANT == 4
+    {{0, 1, 1}, {0, 0, 1}, {1, 1, 1}}, {{0, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+#elif ORC_MUTANT == 6
+    {{0, 0, 1}, {1, 1, 0}, {1, 1, 1}}, {{0, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+#else
     {{0, 0, 1}, {0, 1, 1}, {1, 1, 1}}, {{0, 1, 0}, {0, 1, 1}, {1, 1, 1}},
+#endif
     {{0, 0, 1}, {1, 0, 1}, {1, 1, 1}}, {{1, 0, 0}, {1, 0, 1}, {1, 1, 1}},
     {{0, 1, 0}, {1, 1, 0}, {1, 1, 1}}, {{1, 0, 0}, {1, 1, 0}, {1, 1, 1}},
 };
@@ -159,6 +188,47 @@ float acbrt1(float x0) {
   return x;
 }
 
+// max_eb_to_keep_sign_3d_offline (sz_cp_preserve_utils.hpp:186-191)
+float sign_bound_3d(float positive, float negative) {
+  if (positive == 0 && negative == 0) return 0;
+#if ORC_MUTANT == 5
+  float P = cbrtf(positive);
+  float N = cbrtf(negative);
+#else
+  float P = acbrt1(positive);
+  float N = acbrt1(negative);
+#endif
+  return std::fabs(P - N) / (P + N);
+}
+
+bool same_direction_3d(float a, float b, float c, float d) { return same_direction(a, b, c, d); }
+
+int eb_exp_quantize(double& eb, int base, double log_of_base, double threshold) {
+  // eb_exponential_quantize (sz_cp_preserve_utils.hpp:14-22)
+  if (eb <= threshold) {
+    eb = 0;
+    return 0;
+  }
+  int id = std::log2(eb / threshold) / log_of_base;
+  eb = std::pow((double)base, (double)id) * threshold;
+  return id;
+}
+
+template <typename T>
+void log_transform_3d(const T* x, size_t n, T* logv, uint8_t* sign) {
+  // log_transform (sz_cp_preserve_utils.hpp:205-221): log2f of |x| (a double
+  // input is converted to float by the log2f call), -100 for zeros
+  for (size_t i = 0; i < n; i++) {
+    if (x[i] != 0) {
+      sign[i] = (x[i] > 0);
+      logv[i] = (x[i] > 0) ? log2f(x[i]) : log2f(-x[i]);
+    } else {
+      sign[i] = 0;
+      logv[i] = -100;
+    }
+  }
+}
+
 double tet_bound_3d(const float u[4], const float v[4], const float w[4]) {
   return tet_bound<float>(u[0], u[1], u[2], u[3], v[0], v[1], v[2], v[3], w[0], w[1], w[2], w[3]);
 }
@@ -190,10 +260,8 @@ void eb_3d_offline(const T* U, const T* V, const T* W, size_t r1, size_t r2, siz
 
 int eb_id_3d(double required_eb) {
   if (!(required_eb > 0)) return 0;
-  const double threshold = std::numeric_limits<float>::epsilon();
   double abs_eb = std::log2(1 + required_eb);
-  if (abs_eb <= threshold) return 0;
-  return (int)(std::log2(abs_eb / threshold) / std::log2(2.0));
+  return eb_exp_quantize(abs_eb, 2, std::log2(2.0), std::numeric_limits<float>::epsilon());
 }
 
 template <typename T>
@@ -209,19 +277,9 @@ std::vector<uint8_t> compress_3d_offline_log(const T* U, const T* V, const T* W,
   put(out, base);
   put(out, (int)kRadius);
   for (int p = 0; p < 3; p++) {
-    // log_transform (sz_cp_preserve_utils.hpp:205-221)
     std::vector<uint8_t> sign(n);
     logv[p].resize(n);
-    for (size_t i = 0; i < n; i++) {
-      const T x = src[p][i];
-      if (x != 0) {
-        sign[i] = (x > 0);
-        logv[p][i] = (x > 0) ? log2f(x) : log2f(-x);
-      } else {
-        sign[i] = 0;
-        logv[p][i] = -100;
-      }
-    }
+    log_transform_3d(src[p], n, logv[p].data(), sign.data());
     put_bits(out, sign);
   }
   const double threshold = std::numeric_limits<float>::epsilon();
@@ -240,14 +298,7 @@ std::vector<uint8_t> compress_3d_offline_log(const T* U, const T* V, const T* W,
           bool unp = false;
           T dec[3];
           double abs_eb = std::log2(1 + required);
-          int id;
-          if (abs_eb <= threshold) {
-            abs_eb = 0;
-            id = 0;
-          } else {
-            id = std::log2(abs_eb / threshold) / log_of_base;
-            abs_eb = std::pow((double)base, (double)id) * threshold;
-          }
+          const int id = eb_exp_quantize(abs_eb, base, log_of_base, threshold);
           if (id > 0) {
             for (int p = 0; p < 3; p++) {
               const T* L = logv[p].data() + idx;
@@ -267,7 +318,11 @@ std::vector<uint8_t> compress_3d_offline_log(const T* U, const T* V, const T* W,
                 int qi = (int)(qd / 2) + kRadius;
                 codes[cpos + p] = qi;
                 dec[p] = pred + 2 * (qi - kRadius) * abs_eb;
+#if ORC_MUTANT == 3
+                if (std::fabs(dec[p] - cur) > abs_eb) {
+#else
                 if (std::fabs(dec[p] - cur) >= abs_eb) {
+#endif
                   unp = true;
                   break;
                 }
@@ -389,6 +444,8 @@ bool decompress_3d_online_log(const uint8_t* s, size_t len, size_t r1, size_t r2
   return true;
 }
 
+template void log_transform_3d<float>(const float*, size_t, float*, uint8_t*);
+template void log_transform_3d<double>(const double*, size_t, double*, uint8_t*);
 template void eb_3d_offline<float>(const float*, const float*, const float*, size_t, size_t, size_t,
                                    double, double*);
 template std::vector<uint8_t> compress_3d_offline_log<float>(const float*, const float*,

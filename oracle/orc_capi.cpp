@@ -130,6 +130,28 @@ int orc_acbrt1(const float* x, size_t n, float* out) {
   for (size_t i = 0; i < n; i++) out[i] = acbrt1(x[i]);
   return 0;
 }
+float orc_acbrt1_one(float x) { return orc::acbrt1(x); }
+int orc_sign_bound_3d(const float* pos, const float* neg, size_t n, float* out) {
+  for (size_t i = 0; i < n; i++) out[i] = orc::sign_bound_3d(pos[i], neg[i]);
+  return 0;
+}
+int orc_same_direction_3d(const float* q4, size_t n, int* out) {
+  for (size_t i = 0; i < n; i++) out[i] = orc::same_direction_3d(q4[4 * i], q4[4 * i + 1], q4[4 * i + 2], q4[4 * i + 3]);
+  return 0;
+}
+int orc_eb_exp_quantize(const double* eb, size_t n, int base, double log_of_base, double threshold, int* ids,
+                        double* eb_out) {
+  for (size_t i = 0; i < n; i++) {
+    double e = eb[i];
+    ids[i] = orc::eb_exp_quantize(e, base, log_of_base, threshold);
+    eb_out[i] = e;
+  }
+  return 0;
+}
+int orc_log_transform_3d_f32(const float* x, size_t n, float* logv, uint8_t* sign) {
+  orc::log_transform_3d(x, n, logv, sign);
+  return 0;
+}
 int orc_tet_bound_3d(const float* uvw12, size_t n, double* out) {
   for (size_t i = 0; i < n; i++) {
     const float* t = uvw12 + 12 * i;

@@ -6,9 +6,13 @@
 #include <fcntl.h>
 #include <unistd.h>
 
+#include <atomic>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
+#include <vector>
 
 #include "sz_compress_3d.hpp"
 #include "sz_compress_cp_preserve_2d.hpp"
@@ -18,6 +22,7 @@
 #include "sz_decompress_cp_preserve_3d.hpp"
 #include "sz_decompression_utils.hpp"
 #include "sz_lossless.hpp"
+#include "sz_cp_preserve_utils.hpp"
 
 namespace {
 struct Quiet {
@@ -146,6 +151,75 @@ size_t ref_zstd_compress(int level, unsigned char* data, size_t len, unsigned ch
 }
 size_t ref_zstd_decompress(unsigned char* data, size_t len, unsigned char** out, size_t target) {
   return sz_lossless_decompress(ZSTD_COMPRESSOR, data, len, out, target);
+}
+
+// ---- header-only helpers of the 3D offline_log compressor -------------------
+// (include/sz_cp_preserve_utils.hpp). The compressor's own source does not build
+// here (src/sz_compress_cp_preserve_3d.cpp includes FTK), but these helpers it
+// calls compile from the reference header alone.
+
+// acbrt1 (:173-184) for one value.
+float ref_acbrt1_one(float x) { return acbrt1(x); }
+
+// Compares acbrt1 (:173-184) with `f` on every float bit pattern in [lo, hi)
+// using `threads` threads; returns the number of inputs whose results differ
+// bit for bit and the smallest such pattern in *first (hi when none).
+uint64_t ref_acbrt1_compare(float (*f)(float), uint32_t lo, uint32_t hi, int threads, uint32_t* first) {
+  if (threads < 1) threads = 1;
+  std::atomic<uint64_t> bad{0};
+  std::atomic<uint32_t> lowest{hi};
+  std::vector<std::thread> pool;
+  const uint64_t span = (uint64_t)hi - lo;
+  for (int t = 0; t < threads; t++)
+    pool.emplace_back([&, t] {
+      const uint32_t a = (uint32_t)(lo + span * t / threads), b = (uint32_t)(lo + span * (t + 1) / threads);
+      uint64_t nb = 0;
+      uint32_t lw = hi;
+      for (uint32_t bits = a; bits < b; bits++) {
+        float x, r0, r1;
+        std::memcpy(&x, &bits, 4);
+        r0 = acbrt1(x);
+        r1 = f(x);
+        if (std::memcmp(&r0, &r1, 4) != 0) {
+          if (!nb) lw = bits;
+          nb++;
+        }
+      }
+      bad += nb;
+      uint32_t cur = lowest.load();
+      while (lw < cur && !lowest.compare_exchange_weak(cur, lw)) {
+      }
+    });
+  for (auto& th : pool) th.join();
+  *first = lowest.load();
+  return bad.load();
+}
+
+// max_eb_to_keep_sign_3d_offline (:186-191) elementwise.
+void ref_sign_bound_3d(const float* pos, const float* neg, size_t n, float* out) {
+  for (size_t i = 0; i < n; i++) out[i] = max_eb_to_keep_sign_3d_offline(pos[i], neg[i]);
+}
+
+// same_direction (:52-57) over groups of four values.
+void ref_same_direction(const float* q4, size_t n, int* out) {
+  for (size_t i = 0; i < n; i++) out[i] = same_direction(q4[4 * i], q4[4 * i + 1], q4[4 * i + 2], q4[4 * i + 3]);
+}
+
+// eb_exponential_quantize (:14-22) elementwise.
+void ref_eb_exp_quantize(const double* eb, size_t n, int base, double log_of_base, double threshold, int* ids,
+                         double* eb_out) {
+  for (size_t i = 0; i < n; i++) {
+    double e = eb[i];
+    ids[i] = eb_exponential_quantize(e, base, log_of_base, threshold);
+    eb_out[i] = e;
+  }
+}
+
+// log_transform<float> (:205-221).
+void ref_log_transform_f32(const float* x, size_t n, float* logv, unsigned char* sign) {
+  float* l = log_transform(x, sign, n);
+  std::memcpy(logv, l, n * sizeof(float));
+  std::free(l);
 }
 
 }  // extern "C"

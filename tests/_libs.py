@@ -16,8 +16,8 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def load_oracle():
-    lib = C.CDLL(ORC_PATH)
+def load_oracle(path=ORC_PATH):
+    lib = C.CDLL(path)
     pp = C.POINTER(C.c_void_p)
     for name, args in {
         "orc_compress_2d_f32": [_P, _P, _SZ, _SZ, C.c_double, pp, C.POINTER(_SZ)],
@@ -38,6 +38,10 @@ def load_oracle():
         "orc_eb_id_3d": [_P, _SZ, _P],
         "orc_acbrt1": [_P, _SZ, _P],
         "orc_tet_bound_3d": [_P, _SZ, _P],
+        "orc_sign_bound_3d": [_P, _P, _SZ, _P],
+        "orc_same_direction_3d": [_P, _SZ, _P],
+        "orc_eb_exp_quantize": [_P, _SZ, C.c_int, C.c_double, C.c_double, _P, _P],
+        "orc_log_transform_3d_f32": [_P, _SZ, _P, _P],
     }.items():
         f = getattr(lib, name)
         f.argtypes = args
@@ -53,6 +57,8 @@ def load_oracle():
     lib.orc_cp_3d.argtypes = [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ]
     lib.orc_cp_3d.restype = C.c_long
     lib.orc_free.argtypes = [_P]
+    lib.orc_acbrt1_one.argtypes = [C.c_float]
+    lib.orc_acbrt1_one.restype = C.c_float
     return Oracle(lib)
 
 
@@ -149,6 +155,32 @@ class Oracle:
         self.lib.orc_acbrt1(_ptr(x), x.size, _ptr(out))
         return out
 
+    def sign_bound_3d(self, pos, neg):
+        pos, neg = (np.ascontiguousarray(x, np.float32) for x in (pos, neg))
+        out = np.empty(pos.shape, np.float32)
+        self.lib.orc_sign_bound_3d(_ptr(pos), _ptr(neg), pos.size, _ptr(out))
+        return out
+
+    def same_direction_3d(self, q4):
+        q4 = np.ascontiguousarray(q4, np.float32)
+        out = np.empty(q4.shape[0], np.int32)
+        self.lib.orc_same_direction_3d(_ptr(q4), q4.shape[0], _ptr(out))
+        return out
+
+    def eb_exp_quantize(self, eb, base, log_of_base, threshold):
+        eb = np.ascontiguousarray(eb, np.float64)
+        ids = np.empty(eb.shape, np.int32)
+        out = np.empty(eb.shape, np.float64)
+        self.lib.orc_eb_exp_quantize(_ptr(eb), eb.size, base, log_of_base, threshold, _ptr(ids), _ptr(out))
+        return ids, out
+
+    def log_transform_3d(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        logv = np.empty(x.shape, np.float32)
+        sign = np.empty(x.shape, np.uint8)
+        self.lib.orc_log_transform_3d_f32(_ptr(x), x.size, _ptr(logv), _ptr(sign))
+        return logv, sign
+
     def tet_bound_3d(self, uvw12):
         uvw12 = np.ascontiguousarray(uvw12, np.float32)
         out = np.empty(uvw12.shape[0], np.float64)
@@ -244,7 +276,49 @@ class Reference:
         lib.ref_zstd_compress.restype = _SZ
         lib.ref_zstd_decompress.argtypes = [_P, _SZ, C.POINTER(C.c_void_p), _SZ]
         lib.ref_zstd_decompress.restype = _SZ
+        lib.ref_acbrt1_one.argtypes = [C.c_float]
+        lib.ref_acbrt1_one.restype = C.c_float
+        lib.ref_acbrt1_compare.argtypes = [_P, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint32)]
+        lib.ref_acbrt1_compare.restype = C.c_uint64
+        lib.ref_sign_bound_3d.argtypes = [_P, _P, _SZ, _P]
+        lib.ref_same_direction.argtypes = [_P, _SZ, _P]
+        lib.ref_eb_exp_quantize.argtypes = [_P, _SZ, C.c_int, C.c_double, C.c_double, _P, _P]
+        lib.ref_log_transform_f32.argtypes = [_P, _SZ, _P, _P]
         self.lib = lib
+
+    # header-only helpers of the 3D compressor (include/sz_cp_preserve_utils.hpp)
+    def acbrt1_compare(self, fn_ptr, lo, hi, threads=8):
+        """Number of float bit patterns in [lo, hi) where the C function at
+        fn_ptr differs from the reference acbrt1, and the first of them."""
+        first = C.c_uint32()
+        bad = self.lib.ref_acbrt1_compare(fn_ptr, lo, hi, threads, C.byref(first))
+        return int(bad), first.value
+
+    def sign_bound_3d(self, pos, neg):
+        pos, neg = (np.ascontiguousarray(x, np.float32) for x in (pos, neg))
+        out = np.empty(pos.shape, np.float32)
+        self.lib.ref_sign_bound_3d(_ptr(pos), _ptr(neg), pos.size, _ptr(out))
+        return out
+
+    def same_direction(self, q4):
+        q4 = np.ascontiguousarray(q4, np.float32)
+        out = np.empty(q4.shape[0], np.int32)
+        self.lib.ref_same_direction(_ptr(q4), q4.shape[0], _ptr(out))
+        return out
+
+    def eb_exp_quantize(self, eb, base, log_of_base, threshold):
+        eb = np.ascontiguousarray(eb, np.float64)
+        ids = np.empty(eb.shape, np.int32)
+        out = np.empty(eb.shape, np.float64)
+        self.lib.ref_eb_exp_quantize(_ptr(eb), eb.size, base, log_of_base, threshold, _ptr(ids), _ptr(out))
+        return ids, out
+
+    def log_transform(self, x):
+        x = np.ascontiguousarray(x, np.float32)
+        logv = np.empty(x.shape, np.float32)
+        sign = np.empty(x.shape, np.uint8)
+        self.lib.ref_log_transform_f32(_ptr(x), x.size, _ptr(logv), _ptr(sign))
+        return logv, sign
 
     def _take(self, p, n):
         out = np.frombuffer(C.string_at(p, n), dtype=np.uint8).copy()
