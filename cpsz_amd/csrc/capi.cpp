@@ -288,6 +288,34 @@ int cpsz_lossless_decompress(const unsigned char* in, size_t n, unsigned char* o
   });
 }
 
+int cpsz_lossless_is_frame(const unsigned char* in, size_t n) {
+  if (!in) return 0;
+  return ZSTD_getFrameContentSize(in, n) != ZSTD_CONTENTSIZE_ERROR ? 1 : 0;
+}
+
+int cpsz_lossless_decompress_prefix(const unsigned char* in, size_t n, unsigned char* out, size_t out_size,
+                                    size_t* produced) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && out && produced, kErrArg, "null argument");
+    ZSTD_DStream* ds = ZSTD_createDStream();
+    CPSZ_REQUIRE(ds, kErrAlloc, "ZSTD_createDStream failed");
+    ZSTD_inBuffer ib{in, n, 0};
+    ZSTD_outBuffer ob{out, out_size, 0};
+    size_t r = ZSTD_initDStream(ds);
+    // stops when the prefix is full, the frame ends or the input runs out
+    while (!ZSTD_isError(r) && ob.pos < ob.size) {
+      const size_t before_in = ib.pos, before_out = ob.pos;
+      r = ZSTD_decompressStream(ds, &ob, &ib);
+      if (r == 0 || (ib.pos == before_in && ob.pos == before_out)) break;
+    }
+    ZSTD_freeDStream(ds);
+    CPSZ_REQUIRE(!ZSTD_isError(r), kErrStream, std::string("ZSTD_decompressStream: ") + ZSTD_getErrorName(r));
+    // ZSTD may use the space past the decoded bytes as scratch: clear it
+    std::memset(out + ob.pos, 0, out_size - ob.pos);
+    *produced = ob.pos;
+  });
+}
+
 int cpsz_critical_points_2d_dev(int device, void* stream, const float* dU, const float* dV,
                                 size_t r1, size_t r2, int64_t* ids, int32_t* types, size_t cap,
                                 size_t* count) {

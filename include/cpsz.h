@@ -97,6 +97,15 @@ int cpsz_lossless_compress(int level, const unsigned char* in, size_t n, unsigne
                            size_t* out_size);
 int cpsz_lossless_decompress(const unsigned char* in, size_t n, unsigned char* out,
                              size_t out_size);
+/* Replaces is_lossless_compressed_data (reference src/sz_lossless.cpp:3-9):
+ * 1 when `in` starts with a ZSTD frame whose header parses, else 0. */
+int cpsz_lossless_is_frame(const unsigned char* in, size_t n);
+/* Replaces the decoding in sz_lossless_decompress65536bytes (reference
+ * src/sz_lossless.cpp:47-62): decodes the first out_size bytes of the frame
+ * (fewer if the frame is shorter) into `out`, zero-filling the rest of it;
+ * *produced = bytes decoded. The reference reads a stream header this way. */
+int cpsz_lossless_decompress_prefix(const unsigned char* in, size_t n, unsigned char* out,
+                                    size_t out_size, size_t* produced);
 
 /* ---- critical points (0-CP-error judge) ----------------------------------
  * Critical points per simplex, decided with exact sign predicates on the GPU.

@@ -8,7 +8,9 @@
 //   sz_decompress_cp_preserve_2d_offline   include/sz_decompress_cp_preserve_2d.hpp:8-11
 //   sz_compress_cp_preserve_3d_offline_log include/sz_compress_cp_preserve_3d.hpp:6-8
 //   sz_decompress_cp_preserve_3d_online_log include/sz_decompress_cp_preserve_3d.hpp:8-11
-//   sz_lossless_compress / _decompress     include/sz_lossless.hpp:13-14
+//   ZSTD_COMPRESSOR, is_lossless_compressed_data, sz_lossless_compress,
+//   sz_lossless_decompress, sz_lossless_decompress65536bytes
+//                                          include/sz_lossless.hpp:9-15
 // Streams are byte-compatible with the reference in both directions.
 //
 // Differences a caller can observe:
@@ -160,20 +162,32 @@ inline void sz_decompress_cp_preserve_3d_online_log<double>(const unsigned char*
   cpsz_dropin::release(W);
 }
 
-// Host ZSTD stage with the reference's signatures (src/sz_lossless.cpp:11-45):
-// returns the output length (0 on failure); *compressBytes / *oriData are
-// malloc'd. losslessCompressor is ignored (ZSTD is the only one, as in the
-// reference).
-inline unsigned long sz_lossless_compress(int, int level, unsigned char* data, unsigned long dataLength,
-                                          unsigned char** compressBytes) {
+// Host ZSTD stage with the reference's names and signatures
+// (include/sz_lossless.hpp:9-15, src/sz_lossless.cpp:3-62). The outputs are
+// malloc'd, as in the reference. An unknown compressor id returns 0 and leaves
+// the output pointer untouched, as the reference's default branch does; a
+// ZSTD failure returns 0 with a null output instead of an unchecked buffer.
+#ifndef ZSTD_COMPRESSOR
+#define ZSTD_COMPRESSOR 1
+#endif
+
+inline int is_lossless_compressed_data(unsigned char* compressedBytes, size_t cmpSize) {
+  return cpsz_lossless_is_frame(compressedBytes, cmpSize) ? ZSTD_COMPRESSOR : -1;
+}
+
+inline unsigned long sz_lossless_compress(int losslessCompressor, int level, unsigned char* data,
+                                          unsigned long dataLength, unsigned char** compressBytes) {
+  if (losslessCompressor != ZSTD_COMPRESSOR) return 0;
   size_t out = 0;
   *compressBytes = nullptr;
   if (cpsz_lossless_compress(level, data, dataLength, compressBytes, &out) != CPSZ_OK) return 0;
   return (unsigned long)out;
 }
 
-inline unsigned long sz_lossless_decompress(int, unsigned char* compressBytes, unsigned long cmpSize,
-                                            unsigned char** oriData, unsigned long targetOriSize) {
+inline unsigned long sz_lossless_decompress(int losslessCompressor, unsigned char* compressBytes,
+                                            unsigned long cmpSize, unsigned char** oriData,
+                                            unsigned long targetOriSize) {
+  if (losslessCompressor != ZSTD_COMPRESSOR) return 0;
   *oriData = static_cast<unsigned char*>(std::malloc(targetOriSize ? targetOriSize : 1));
   if (!*oriData) return 0;
   if (cpsz_lossless_decompress(compressBytes, cmpSize, *oriData, targetOriSize) != CPSZ_OK) {
@@ -182,6 +196,22 @@ inline unsigned long sz_lossless_decompress(int, unsigned char* compressBytes, u
     return 0;
   }
   return targetOriSize;
+}
+
+// The first 65536 bytes of the frame, zero-filled past its end (the reference
+// reads stream headers this way).
+inline unsigned long sz_lossless_decompress65536bytes(int losslessCompressor, unsigned char* compressBytes,
+                                                      unsigned long cmpSize, unsigned char** oriData) {
+  if (losslessCompressor != ZSTD_COMPRESSOR) return 0;
+  *oriData = static_cast<unsigned char*>(std::calloc(65536, 1));
+  if (!*oriData) return 0;
+  size_t got = 0;
+  if (cpsz_lossless_decompress_prefix(compressBytes, cmpSize, *oriData, 65536, &got) != CPSZ_OK) {
+    std::free(*oriData);
+    *oriData = nullptr;
+    return 0;
+  }
+  return 65536;
 }
 
 #endif  // SZ_CP_PRESERVE_DROPIN_HPP_

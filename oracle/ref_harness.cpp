@@ -152,6 +152,11 @@ size_t ref_zstd_compress(int level, unsigned char* data, size_t len, unsigned ch
 size_t ref_zstd_decompress(unsigned char* data, size_t len, unsigned char** out, size_t target) {
   return sz_lossless_decompress(ZSTD_COMPRESSOR, data, len, out, target);
 }
+// is_lossless_compressed_data / sz_lossless_decompress65536bytes (src/sz_lossless.cpp:3-9,47-62)
+int ref_is_lossless(unsigned char* data, size_t len) { return is_lossless_compressed_data(data, len); }
+size_t ref_zstd_decompress65536(unsigned char* data, size_t len, unsigned char** out) {
+  return sz_lossless_decompress65536bytes(ZSTD_COMPRESSOR, data, len, out);
+}
 
 // ---- header-only helpers of the 3D offline_log compressor -------------------
 // (include/sz_cp_preserve_utils.hpp). The compressor's own source does not build

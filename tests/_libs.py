@@ -276,6 +276,10 @@ class Reference:
         lib.ref_zstd_compress.restype = _SZ
         lib.ref_zstd_decompress.argtypes = [_P, _SZ, C.POINTER(C.c_void_p), _SZ]
         lib.ref_zstd_decompress.restype = _SZ
+        lib.ref_is_lossless.argtypes = [_P, _SZ]
+        lib.ref_is_lossless.restype = C.c_int
+        lib.ref_zstd_decompress65536.argtypes = [_P, _SZ, C.POINTER(C.c_void_p)]
+        lib.ref_zstd_decompress65536.restype = _SZ
         lib.ref_acbrt1_one.argtypes = [C.c_float]
         lib.ref_acbrt1_one.restype = C.c_float
         lib.ref_acbrt1_compare.argtypes = [_P, C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint32)]
@@ -329,6 +333,16 @@ class Reference:
         data = np.ascontiguousarray(data, np.uint8)
         p = C.c_void_p()
         n = self.lib.ref_zstd_compress(level, _ptr(data), data.size, C.byref(p))
+        return self._take(p, n)
+
+    def is_lossless(self, data):
+        data = np.ascontiguousarray(data, np.uint8)
+        return self.lib.ref_is_lossless(_ptr(data), data.size)
+
+    def zstd_decompress65536(self, data):
+        data = np.ascontiguousarray(data, np.uint8)
+        p = C.c_void_p()
+        n = self.lib.ref_zstd_decompress65536(_ptr(data), data.size, C.byref(p))
         return self._take(p, n)
 
     def compress_2d(self, U, V, eb=0.1):

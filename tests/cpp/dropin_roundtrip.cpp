@@ -30,10 +30,12 @@ int main(int argc, char** argv) {
     return 2;
   }
   const bool is3d = std::strcmp(argv[1], "3d") == 0;
-  const size_t r1 = std::strtoull(argv[4], nullptr, 10), r2 = std::strtoull(argv[5], nullptr, 10);
-  const size_t r3 = is3d ? std::strtoull(argv[6], nullptr, 10) : 1;
-  const double eb = std::atof(argv[is3d ? 7 : 6]);
-  const size_t n = r1 * r2 * r3, ncomp = is3d ? 3 : 2;
+  // int dims and atof, as the reference drivers parse them
+  // (test/sz_compress_cp_preserve_2d_test.cpp:14-16)
+  const int r1 = std::atoi(argv[4]), r2 = std::atoi(argv[5]);
+  const int r3 = is3d ? std::atoi(argv[6]) : 1;
+  const double max_eb = std::atof(argv[is3d ? 7 : 6]);
+  const size_t n = (size_t)r1 * r2 * r3, ncomp = is3d ? 3 : 2;
   std::vector<float> in(ncomp * n);
   FILE* f = std::fopen(argv[2], "rb");
   if (!f || std::fread(in.data(), 4, in.size(), f) != in.size()) {
@@ -45,23 +47,33 @@ int main(int argc, char** argv) {
   size_t size = 0;
   unsigned char* stream =
       is3d ? sz_compress_cp_preserve_3d_offline_log(in.data(), in.data() + n, in.data() + 2 * n, r1, r2, r3,
-                                                    size, false, eb)
-           : sz_compress_cp_preserve_2d_offline(in.data(), in.data() + n, r1, r2, size, false, eb);
+                                                    size, false, max_eb)
+           : sz_compress_cp_preserve_2d_offline(in.data(), in.data() + n, r1, r2, size, false, max_eb);
   if (!stream) {
     std::printf("fail compress: %s\n", cpsz_last_error());
     return 1;
   }
+  // test/sz_compress_cp_preserve_2d_test.cpp:29,36
   unsigned char* z = nullptr;
-  const unsigned long zs = sz_lossless_compress(1, 3, stream, size, &z);
+  const size_t zs = sz_lossless_compress(ZSTD_COMPRESSOR, 3, stream, size, &z);
   unsigned char* back = nullptr;
-  const unsigned long bs = sz_lossless_decompress(1, z, zs, &back, size);
+  const size_t bs = sz_lossless_decompress(ZSTD_COMPRESSOR, z, zs, &back, size);
   if (zs == 0 || bs != size || std::memcmp(back, stream, size) != 0) {
     std::printf("fail zstd round trip\n");
     return 1;
   }
+  // the rest of include/sz_lossless.hpp: frame check and 64 KiB header read
+  unsigned char* head = nullptr;
+  if (is_lossless_compressed_data(z, zs) != ZSTD_COMPRESSOR || is_lossless_compressed_data(stream, 4) != -1 ||
+      sz_lossless_decompress65536bytes(ZSTD_COMPRESSOR, z, zs, &head) != 65536 ||
+      std::memcmp(head, stream, size < 65536 ? size : 65536) != 0) {
+    std::printf("fail lossless header helpers\n");
+    return 1;
+  }
+  std::free(head);
   float *U = nullptr, *V = nullptr, *W = nullptr;
-  if (is3d) sz_decompress_cp_preserve_3d_online_log(back, r1, r2, r3, U, V, W);
-  else sz_decompress_cp_preserve_2d_offline(back, r1, r2, U, V);
+  if (is3d) sz_decompress_cp_preserve_3d_online_log<float>(back, r1, r2, r3, U, V, W);
+  else sz_decompress_cp_preserve_2d_offline<float>(back, r1, r2, U, V);
   if (!U || !V || (is3d && !W)) {
     std::printf("fail decompress: %s\n", cpsz_last_error());
     return 1;
@@ -82,6 +94,6 @@ int main(int argc, char** argv) {
     std::printf("fail write\n");
     return 1;
   }
-  std::printf("ok %zu %lu\n", size, zs);
+  std::printf("ok %zu %zu\n", size, zs);
   return 0;
 }

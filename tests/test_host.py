@@ -54,6 +54,35 @@ def test_dropin_lossless_matches_reference_zstd_level3():
         assert np.array_equal(z, Reference().zstd_compress(data, 3))
 
 
+def test_dropin_lossless_header_helpers():
+    """is_lossless_compressed_data and the 64 KiB header read of the drop-in's C ABI
+    (src/sz_lossless.cpp:3-9,47-62): frame detection, and the first 65536 bytes
+    of a frame, zero-filled past a shorter frame's end."""
+    L = cz.lib()
+    L.cpsz_lossless_is_frame.argtypes = [C.c_void_p, C.c_size_t]
+    L.cpsz_lossless_decompress_prefix.argtypes = [C.c_void_p, C.c_size_t, C.c_void_p, C.c_size_t,
+                                                  C.POINTER(C.c_size_t)]
+    rng = np.random.default_rng(2)
+    from _libs import Reference, have_reference
+    ref = Reference() if have_reference() else None
+    for n in (10, 1000, 65536, 200000):
+        data = rng.integers(0, 5, n).astype(np.uint8)
+        z = cz.sz_lossless_compress(data, 3)
+        assert L.cpsz_lossless_is_frame(z.ctypes.data, z.size) == 1
+        assert L.cpsz_lossless_is_frame(data.ctypes.data, min(n, 4)) == 0
+        out = np.zeros(65536, np.uint8)
+        got = C.c_size_t()
+        assert L.cpsz_lossless_decompress_prefix(z.ctypes.data, z.size, out.ctypes.data, 65536,
+                                                 C.byref(got)) == 0
+        m = min(n, 65536)
+        assert got.value == m and np.array_equal(out[:m], data[:m]) and not out[m:].any()
+        if ref is not None:
+            assert ref.is_lossless(z) == 1 and ref.is_lossless(data[:4]) == -1
+            head = ref.zstd_decompress65536(z)
+            if n <= 65536:  # a longer frame does not fit the reference's buffer at all
+                assert head.size == 65536 and np.array_equal(head[:m], out[:m])
+
+
 def _tables():
     L = cz.lib()
     L.cpsz_eb_tables.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.c_void_p, C.c_int,
