@@ -1,9 +1,13 @@
-"""Benchmark: compress+decompress throughput of the cpSZ plain codec on one MI355X.
+"""Benchmark: compress+decompress throughput of the cpSZ codec on one MI355X.
 
-Workload (default): 3D plain cpSZ on u,v,w float32 256^3 (ABC flow + 32 vortex
-tubes + noise), relative eb 0.05, offline_log compressor / online_log
-decompressor. ``--workload 2d`` runs the 2D plain codec on 2400x3600 u,v
-(64 Gaussian vortices + 8 Fourier modes + noise), relative eb 0.1.
+Workload (default, the largest single-GPU configuration): 96 u,v float32 2D
+fields with heights and widths log-uniform in [256, 4096] (seeded; 220M
+vertices, 1.76 GB), relative eb 0.1, each step one batched compress and one
+batched decompress of all fields. The plain method (the drop-in) gives
+`value`; the learned-eb method is timed on the same batch and reported beside
+it (ratio and GB/s, plain vs learned). Other workloads: ``--workload 3d`` (3D
+plain cpSZ on u,v,w 256^3, rel eb 0.05), ``2d`` (2400x3600, rel eb 0.1),
+``learned2d`` and ``learned3d`` (16 timesteps of 192^3, batched).
 
 A step = one device-resident compress (field in HBM -> pre-ZSTD stream in HBM)
 plus one device-resident decompress (stream in HBM -> field in HBM). value =
@@ -35,8 +39,9 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
-    p.add_argument("--workload", choices=["3d", "2d", "batch2d", "learned2d", "learned3d"], default="3d")
+    p.add_argument("--workload", choices=["batch2d", "3d", "2d", "learned2d", "learned3d"], default="batch2d")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--no-learned", action="store_true", help="batch2d: skip the learned method")
     p.add_argument("--no-cp", action="store_true", help="skip the critical-point check")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain 256 MiB device copy (the rocprofv3 FETCH_SIZE / WRITE_SIZE "
@@ -46,6 +51,11 @@ def parse():
 
 def cpu_baseline(workload, fields, eb):
     """Times the CPU codec on a bounded sample, single-threaded."""
+    with _QuietStdout():
+        return _cpu_baseline(workload, fields, eb)
+
+
+def _cpu_baseline(workload, fields, eb):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import _libs
 
@@ -94,9 +104,129 @@ def cpu_baseline(workload, fields, eb):
             "sample": sample, "compress_s": t1 - t0, "decompress_s": t2 - t1}
 
 
+def _cpu_info():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+class _QuietStdout:
+    """Sends file descriptor 1 to /dev/null for the duration (the reference CPU
+    library prints progress to stdout; the bench's own stdout is one JSON line)."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        fd = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(fd, 1)
+        os.close(fd)
+        return self
+
+    def __exit__(self, *exc):
+        C.CDLL(None).fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        return False
+
+
+def _ref_worker(args):
+    """One field through the CPU codec in a worker process: (seconds, vertices)."""
+    path, r1, r2, eb = args
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _libs
+    uv = np.load(path)
+    ref = _libs.Reference() if _libs.have_reference() else _libs.load_oracle()
+    with _QuietStdout():
+        t0 = time.perf_counter()
+        ref.decompress_2d(ref.compress_2d(np.ascontiguousarray(uv[0]), np.ascontiguousarray(uv[1]), eb), r1, r2)
+        dt = time.perf_counter() - t0
+    return dt, r1 * r2
+
+
+def cpu_baseline_batch2d(fields_host, shapes, eb):
+    """The compiled reference (the oracle where it is absent) on a bounded
+    sample of the batch: single-threaded, median of 5 over the first fields up
+    to ~12M vertices; then one process per core over fields up to ~48M
+    vertices, median of 3 (wall time, field bytes / second)."""
+    import multiprocessing as mp
+    import statistics
+    import tempfile
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _libs
+    have_ref = _libs.have_reference()
+    ref = _libs.Reference() if have_ref else _libs.load_oracle()
+
+    def first(limit):
+        pick, acc = [], 0
+        for k, (r1, r2) in enumerate(shapes):
+            if acc + r1 * r2 > limit and pick:
+                break
+            pick.append(k)
+            acc += r1 * r2
+        return pick, acc
+
+    pick1, nv1 = first(12_000_000)
+    runs = []
+    with _QuietStdout():
+        for _ in range(5):
+            t0 = time.perf_counter()
+            for k in pick1:
+                U, V = fields_host[k]
+                ref.decompress_2d(ref.compress_2d(U, V, eb), *shapes[k])
+            runs.append(time.perf_counter() - t0)
+    t1 = statistics.median(runs)
+    model, ncpu, avail = _cpu_info()
+    procs = max(1, min(16, avail))  # the GPU box's CPU share is 16 cores per GPU
+    pickn, nvn = first(48_000_000)
+    tmp = tempfile.mkdtemp(prefix="cpsz_bench_")
+    jobs = []
+    try:
+        for k in pickn:
+            path = os.path.join(tmp, f"f{k}.npy")
+            np.save(path, np.stack(fields_host[k]))
+            jobs.append((path, shapes[k][0], shapes[k][1], eb))
+        jobs.sort(key=lambda j: -j[1] * j[2])  # largest first balances the pool
+        walls = []
+        with mp.get_context("spawn").Pool(procs) as pool:
+            pool.map(_ref_worker, jobs[:procs])  # start-up and page-in, untimed
+            for _ in range(3):
+                t0 = time.perf_counter()
+                pool.map(_ref_worker, jobs, chunksize=1)
+                walls.append(time.perf_counter() - t0)
+        tn = statistics.median(walls)
+    finally:
+        for j in jobs:
+            try:
+                os.remove(j[0])
+            except OSError:
+                pass
+        os.rmdir(tmp)
+    kind = "reference" if have_ref else "port"
+    return {"value": 8 * nv1 / t1 / 1e9, "unit": "GB/s", "cores": 1, "kind": kind,
+            "sample": f"{len(pick1)} of the 96 fields ({nv1} vertices), compress+decompress one at a time, "
+                      f"single thread, median of 5",
+            "cpu_model": model, "nproc": ncpu, "cores_available": avail,
+            "multi_process": {"value": 8 * nvn / tn / 1e9, "unit": "GB/s", "processes": procs,
+                              "sample": f"{len(pickn)} of the 96 fields ({nvn} vertices), one field per task, "
+                                        f"median of 3 wall times"}}
+
+
 def run_batch2d(a, torch):
-    """96 fields, h and w log-uniform in [256, 4096] (seeded), rel eb 0.1; one
-    batched compress + one batched decompress per step."""
+    """96 fields, h and w log-uniform in [256, 4096] (seeded), rel eb 0.1. A
+    step is one batched compress + one batched decompress of all fields. The
+    plain method (the drop-in) gives `value`; the learned method is timed on
+    the same batch for the plain-vs-learned ratio and throughput."""
     import cpsz_amd as cz
     from cpsz_amd.synth import batch_2d_shapes, field_2d_device
 
@@ -108,41 +238,45 @@ def run_batch2d(a, torch):
     field_bytes = 8 * nv
     stream = torch.cuda.current_stream()
     sh = stream.cuda_stream
+    hip = C.CDLL("libamdhip64.so")
     cz.set_timing(True, 0)
     fl = [(u.data_ptr(), v.data_ptr(), r1, r2) for (u, v), (r1, r2) in zip(dev, shapes)]
     tuned = {"done": False}
+
+    def decode(base, off, size):
+        cz.decompress_2d_batch_dev([(base + int(o), int(n)) for o, n in zip(off, size)], shapes,
+                                   [(x.data_ptr(), y.data_ptr()) for x, y in outs], stream=sh)
 
     def step():
         base, off, size = cz.compress_2d_batch_dev(fl, eb, tune=not tuned["done"], stream=sh)
         tuned["done"] = True
         st_c = cz.last_stages(0)
-        cz.decompress_2d_batch_dev([(base + int(o), int(n)) for o, n in zip(off, size)], shapes,
-                                   [(x.data_ptr(), y.data_ptr()) for x, y in outs], stream=sh)
+        decode(base, off, size)
         st_d = cz.last_stages(0)
         return base, off, size, st_c, st_d
 
-    for _ in range(max(a.warmup, 1)):
-        step()
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    stages_c, stages_d = {}, {}
-    ev0.record(stream)
-    for _ in range(a.steps):
-        base, off, size, st_c, st_d = step()
-        for k, v in st_c:
-            stages_c[k] = stages_c.get(k, 0.0) + v / a.steps
-        for k, v in st_d:
-            stages_d[k] = stages_d.get(k, 0.0) + v / a.steps
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    t_step = ev0.elapsed_time(ev1) / 1e3 / a.steps
-    # outside the timed region: ratio (ZSTD-3 per stream), CP errors per field
-    total = int(off[-1] + size[-1])
-    allb = np.empty(total, np.uint8)
-    C.CDLL("libamdhip64.so").hipMemcpy(C.c_void_p(allb.ctypes.data), C.c_void_p(base), C.c_size_t(total), 2)
-    zbytes = sum(cz.sz_lossless_compress(allb[o:o + n], 3).size for o, n in zip(off, size))
-    cp_err = None
-    if not a.no_cp:
+    def timed(fn, steps, warmup):
+        for _ in range(max(warmup, 1)):
+            r = fn()
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        acc = []
+        ev0.record(stream)
+        for _ in range(steps):
+            r = fn()
+            acc.append(r)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        return acc, ev0.elapsed_time(ev1) / 1e3 / steps
+
+    def zbytes(base, off, size):
+        total = int(off[-1] + size[-1])
+        allb = np.empty(total, np.uint8)
+        hip.hipMemcpy(C.c_void_p(allb.ctypes.data), C.c_void_p(base), C.c_size_t(total), 2)
+        return sum(cz.sz_lossless_compress(allb[o:o + n], 3).size for o, n in zip(off, size)), \
+            int(np.sum(size))
+
+    def cp_errors():
         fp = fn = ft = tot = 0
         for (u, v), (x, y), (r1, r2) in zip(dev, outs, shapes):
             c0 = cz.critical_points_2d_dev(u.data_ptr(), v.data_ptr(), r1, r2)
@@ -151,32 +285,69 @@ def run_batch2d(a, torch):
             fp += sum(1 for k in c1 if k not in c0)
             fn += sum(1 for k in c0 if k not in c1)
             ft += sum(1 for k in c0 if k in c1 and c0[k] != c1[k])
-        cp_err = {"cp_total": tot, "fp": fp, "fn": fn, "ft": ft}
-    # dominant kernel: the batched wavefront launch (16 algorithmic B per vertex)
-    cand = [("k_lorenzo2d_batch<true> (batched 2D compress wavefront)", stages_c.get("lorenzo", float("nan"))),
-            ("k_lorenzo2d_batch<false> (batched 2D decompress wavefront)", stages_d.get("lorenzo", float("nan")))]
-    name, k_ms = max(cand, key=lambda x: x[1] if x[1] == x[1] else -1)
-    achieved = 16 * nv / (k_ms / 1e3) if k_ms == k_ms and k_ms > 0 else None
+        return {"cp_total": tot, "fp": fp, "fn": fn, "ft": ft}
+
+    # ---- plain (timed region: a.steps batched compress + decompress)
+    t_wall = time.perf_counter()
+    acc, t_step = timed(step, a.steps, a.warmup)
+    t_wall = time.perf_counter() - t_wall
+    stages_c, stages_d = {}, {}
+    for _, _, _, st_c, st_d in acc:
+        for k, v in st_c:
+            stages_c[k] = stages_c.get(k, 0.0) + v / a.steps
+        for k, v in st_d:
+            stages_d[k] = stages_d.get(k, 0.0) + v / a.steps
+    base, off, size = acc[-1][:3]
+    zb_plain, sb_plain = zbytes(base, off, size)
+    cp_plain = None if a.no_cp else cp_errors()
+    # ---- learned: trained on 64 seeded 1024x1024 fields (bisection labels), then
+    # learned batched compress (proposals, verification rounds, fallback) +
+    # batched decompress of the same 96 fields
+    learned = None
+    if not a.no_learned:
+        t0 = time.perf_counter()
+        train = [field_2d_device(1024, 1024, seed=5000 + k) for k in range(64)]
+        params, losses = cz.learn_train_2d_dev([(u.data_ptr(), v.data_ptr(), 1024, 1024) for u, v in train], eb,
+                                               cz.learn_init_params(1), samples_per_field=16384, epochs=8,
+                                               batch=4096, lr=2e-3, seed=11)
+        train_s = time.perf_counter() - t0
+        del train
+
+        def lstep():
+            base, off, size, it, boosted, fb = cz.compress_2d_learned_batch_dev(fl, eb, params, stream=sh)
+            decode(base, off, size)
+            return base, off, size, it, boosted, fb
+
+        lacc, t_learned = timed(lstep, max(1, min(a.steps, 2)), 1)
+        lb, lo, ls, it, boosted, fb = lacc[-1]
+        zb_l, sb_l = zbytes(lb, lo, ls)
+        learned = {"gbps": field_bytes / t_learned / 1e9, "ms_per_step": t_learned * 1e3,
+                   "steps": len(lacc), "ratio": field_bytes / zb_l, "stream_bytes": sb_l, "zstd_bytes": zb_l,
+                   "cp_errors": None if a.no_cp else cp_errors(), "verification_rounds": it,
+                   "boosted_vertices": boosted, "fallback_vertices": fb, "train_seconds": train_s,
+                   "train_loss_per_epoch": [float(x) for x in losses]}
+    # ---- roofline of the dominant kernel. Its duration comes from the
+    # library's HIP-event marks on its launching stream (the stage named here
+    # is that one launch). Algorithmic bytes per vertex: the wavefronts read
+    # and write 16 B (compress: 8 B field + 4 B eb codes in, 4 B codes out;
+    # decompress: 4 B ids + 4 B codes in, 8 B field out).
+    cand = [("k_lorenzo2d_batch<true> (batched 2D compress wavefront)", stages_c.get("lorenzo", float("nan")), 16),
+            ("k_lorenzo2d_batch<false> (batched 2D decompress wavefront)", stages_d.get("lorenzo", float("nan")),
+             16)]
+    name, k_ms, per_v = max(cand, key=lambda x: x[1] if x[1] == x[1] else -1)
+    achieved = per_v * nv / (k_ms / 1e3) if k_ms == k_ms and k_ms > 0 else None
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_batch2d.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            tj = json.load(fh)
+        kern = name.split(" (")[0]
+        if kern in tj.get("bytes_per_dispatch", {}):
+            traffic = tj["bytes_per_dispatch"][kern]
     cpu = None
     if not a.no_cpu:
-        # bounded sample: the reference (or the oracle) on the first fields up to ~3M vertices
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        import _libs
-        pick, acc = [], 0
-        for k, (r1, r2) in enumerate(shapes):
-            if acc + r1 * r2 > 3_000_000 and pick:
-                break
-            pick.append(k)
-            acc += r1 * r2
-        ref = _libs.Reference() if _libs.have_reference() else _libs.load_oracle()
-        t0 = time.perf_counter()
-        for k in pick:
-            U, V = dev[k][0].cpu().numpy(), dev[k][1].cpu().numpy()
-            ref.decompress_2d(ref.compress_2d(U, V, eb), *shapes[k])
-        dt = time.perf_counter() - t0
-        cpu = {"value": 8 * acc / dt / 1e9, "unit": "GB/s", "cores": 1,
-               "kind": "reference" if _libs.have_reference() else "port",
-               "sample": f"{len(pick)} of the 96 fields ({acc} vertices), compress+decompress, one at a time"}
+        host = [(u.cpu().numpy(), v.cpu().numpy()) for u, v in dev[:40]]
+        cpu = cpu_baseline_batch2d(host, shapes[:40], eb)
     w, g = cz.batch_2d_config(shapes)
     out = {
         "metric": "compress+decompress GB/s and ratio at 0 CP errors: plain vs learned-eb, 1 GPU",
@@ -184,13 +355,18 @@ def run_batch2d(a, torch):
         "ms_per_step": t_step * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic (generated on the GPU, seeded)",
         "config": {"workload": "batched 2D plain cpSZ: 96 u,v float32 fields, h,w log-uniform in [256, 4096], "
-                               "rel eb 0.1", "method": "plain", "fields": 96, "vertices": nv,
-                   "field_bytes": field_bytes, "stream_bytes": int(np.sum(size)), "zstd_bytes": zbytes,
-                   "ratio": field_bytes / zbytes, "cp_errors": cp_err, "autotuned": {"workers": w, "grid_cap": g},
-                   "stages_compress_ms": stages_c, "stages_decompress_ms": stages_d},
+                               "rel eb 0.1 (220,143,645 vertices, 1.76 GB)", "method": "plain", "fields": 96,
+                   "vertices": nv, "field_bytes": field_bytes, "stream_bytes": sb_plain, "zstd_bytes": zb_plain,
+                   "ratio": field_bytes / zb_plain, "cp_errors": cp_plain, "wall_s": t_wall,
+                   "autotuned": {"workers": w, "grid_cap": g},
+                   "stages_compress_ms": stages_c, "stages_decompress_ms": stages_d,
+                   "learned": learned,
+                   "learned_vs_plain": None if learned is None else {
+                       "ratio_learned": learned["ratio"], "ratio_plain": field_bytes / zb_plain,
+                       "gbps_learned": learned["gbps"], "gbps_plain": field_bytes / t_step / 1e9}},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9 if achieved else None, "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": None,
-                     "kernel": name, "kernel_ms": k_ms, "alg_bytes_per_launch": 16 * nv},
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK if achieved else None, "traffic": traffic,
+                     "kernel": name, "kernel_ms": k_ms, "alg_bytes_per_launch": per_v * nv},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -70,6 +70,9 @@ _SIGS = {
     "cpsz_mlp_grad_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _P, _P],
     "cpsz_learn_train_3d_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, _SZ, C.c_int, C.c_int,
                                 C.c_float, C.c_uint64, _P, _P],
+    "cpsz_compress_2d_learned_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, C.c_double, _P, C.c_int,
+                                           C.POINTER(_P), _P, _P, C.POINTER(C.c_int), C.POINTER(_SZ),
+                                           C.POINTER(_SZ)],
     "cpsz_compress_3d_learned_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, _P, C.c_int,
                                            C.POINTER(_P), _P, _P, C.POINTER(C.c_int), C.POINTER(_SZ),
                                            C.POINTER(_SZ)],
@@ -469,6 +472,24 @@ def learn_train_3d_dev(fields, eb: float, params: np.ndarray, samples_per_field:
                                          *[_arr(C.c_size_t, [f[k] for f in fields]) for k in range(3, 6)],
                                          eb, samples_per_field, epochs, batch, lr, seed, _ptr(p), _ptr(losses)))
     return p, losses
+
+
+def compress_2d_learned_batch_dev(fields, eb: float, params: np.ndarray, tune: bool = False, device: int = 0,
+                                  stream: int = 0):
+    """fields: list of (dU, dV, r1, r2) device pointers. Returns (base pointer, offsets, sizes,
+    verification rounds, boosted vertices, fallback vertices)."""
+    n = len(fields)
+    params = np.ascontiguousarray(params, np.float32)
+    base = _P()
+    off = np.zeros(n, np.uint64)
+    size = np.zeros(n, np.uint64)
+    it, bo, fb = C.c_int(), _SZ(), _SZ()
+    _check(lib().cpsz_compress_2d_learned_batch_dev(device, stream or None, n,
+                                                    *[_arr(C.c_void_p, [f[k] for f in fields]) for k in range(2)],
+                                                    *[_arr(C.c_size_t, [f[k] for f in fields]) for k in range(2, 4)],
+                                                    eb, _ptr(params), int(tune), C.byref(base), _ptr(off), _ptr(size),
+                                                    C.byref(it), C.byref(bo), C.byref(fb)))
+    return base.value, off.astype(np.int64), size.astype(np.int64), it.value, bo.value, fb.value
 
 
 def compress_3d_learned_batch_dev(fields, eb: float, params: np.ndarray, tune: bool = False, device: int = 0,

@@ -48,9 +48,10 @@ size_t stream_bound(size_t n) { return 12 * n + (1u << 20); }
 }  // namespace
 
 void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_pwr_eb,
-                       const Batch2dConfig& cfg, BatchOut& out) {
+                       const Batch2dConfig& cfg, BatchOut& out, const std::vector<EbOverride>* ov) {
   const size_t nf = fields.size();
   CPSZ_REQUIRE(nf > 0, kErrArg, "compress_2d_batch: empty batch");
+  CPSZ_REQUIRE(!ov || ov->size() == nf, kErrArg, "compress_2d_batch: one override per field");
   hipStream_t st = ctx.stream();
   ctx.stages.clear();
   ctx.mark("start");
@@ -86,7 +87,7 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
     std::vector<std::function<void(Ctx&)>> eb_jobs(nf);
     for (size_t k = 0; k < nf; k++)
       eb_jobs[k] = [&, k](Ctx& wc) {
-        c2_eb(wc, w[k], max_pwr_eb, EbOverride());
+        c2_eb(wc, w[k], max_pwr_eb, ov ? (*ov)[k] : EbOverride());
         CPSZ_HIP(hipStreamSynchronize(wc.stream()));
       };
     pool_run(pool_for(ctx.device()), eb_jobs, cfg.workers);

@@ -25,8 +25,9 @@ struct BatchOut {
   std::vector<size_t> off, size;  // per field
 };
 
+// ov: null, or one EbOverride per field (the learned proposer's hook).
 void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_pwr_eb,
-                       const Batch2dConfig& cfg, BatchOut& out);
+                       const Batch2dConfig& cfg, BatchOut& out, const std::vector<EbOverride>* ov = nullptr);
 void decompress_2d_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<size_t>& in_size,
                          const std::vector<Field2dOut>& fields, const Batch2dConfig& cfg);
 // Cached configuration for this batch signature; with tune, times the

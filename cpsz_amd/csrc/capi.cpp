@@ -685,6 +685,38 @@ int cpsz_compress_2d_learned_f32(const float* U, const float* V, size_t r1, size
   });
 }
 
+int cpsz_compress_2d_learned_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                                       const float* const* dV, const size_t* r1, const size_t* r2,
+                                       double max_pwr_eb, const float* params, int tune,
+                                       const unsigned char** d_base, size_t* offsets, size_t* sizes,
+                                       int* iterations, size_t* boosted, size_t* fallback) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && dU && dV && r1 && r2 && params && d_base && offsets && sizes, kErrArg,
+                 "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    std::vector<TrainField> f(nfields);
+    std::vector<Field2d> sig(nfields);
+    for (size_t k = 0; k < nfields; k++) {
+      f[k] = TrainField{dU[k], dV[k], r1[k], r2[k]};
+      sig[k] = Field2d{dU[k], dV[k], r1[k], r2[k]};
+    }
+    const Batch2dConfig cfg = batch_2d_config(c, sig, max_pwr_eb, tune != 0);
+    const uint8_t* base = nullptr;
+    std::vector<size_t> off, size;
+    LearnedStats st;
+    compress_2d_learned_batch(c, f, max_pwr_eb, params, cfg.workers, cfg.grid_cap, base, off, size, st);
+    *d_base = base;
+    for (size_t k = 0; k < nfields; k++) {
+      offsets[k] = off[k];
+      sizes[k] = size[k];
+    }
+    if (iterations) *iterations = st.iterations;
+    if (boosted) *boosted = st.boosted_vertices;
+    if (fallback) *fallback = st.fallback_vertices;
+  });
+}
+
 // ---- learned proposer, 3D
 size_t cpsz_learn_param_count_3d(void) { return kNumParams3; }
 

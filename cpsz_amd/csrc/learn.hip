@@ -36,10 +36,10 @@ __device__ inline float bf2f(uint16_t h) {
 // zero bound), then zeros up to kFeatDim.
 __global__ void k_feat2d(const float* __restrict__ U, const float* __restrict__ V,
                          const double* __restrict__ E, int r1, int r2, double max_pwr_eb,
-                         const uint32_t* __restrict__ idx, size_t m, uint16_t* __restrict__ X) {
+                         const uint32_t* __restrict__ idx, size_t v0, size_t m, uint16_t* __restrict__ X) {
   const size_t q = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (q >= m) return;
-  const size_t v = idx ? idx[q] : q;
+  const size_t v = idx ? idx[q] : v0 + q;
   const int i = (int)(v / (size_t)r2), j = (int)(v - (size_t)i * r2);
   float pu[25], pv[25];
   float s = 0.f;
@@ -391,7 +391,7 @@ void features_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, s
                  double max_pwr_eb, const uint32_t* d_idx, size_t m, uint16_t* dX) {
   CPSZ_REQUIRE(dU && dV && dE && dX && r1 * r2 < (1ull << 32), kErrArg, "features_2d: bad arguments");
   if (!m) return;
-  k_feat2d<<<blocks_for(m), 256, 0, ctx.stream()>>>(dU, dV, dE, (int)r1, (int)r2, max_pwr_eb, d_idx, m, dX);
+  k_feat2d<<<blocks_for(m), 256, 0, ctx.stream()>>>(dU, dV, dE, (int)r1, (int)r2, max_pwr_eb, d_idx, 0, m, dX);
   CPSZ_HIP(hipGetLastError());
 }
 
@@ -950,6 +950,137 @@ void compress_3d_learned_batch(Ctx& ctx, const std::vector<TrainField3>& fields,
         while (judge_tet_sid(t) != which) t++;
         const int cs[4] = {0, judge_tet_a(t), judge_tet_b(t), 7};
         for (int c : cs) hf[nofs[k] + org + (size_t)(c >> 2) * d0 + (size_t)((c >> 1) & 1) * d1 + (c & 1)] = 1;
+      }
+      still.push_back(k);
+    }
+    // the accepted streams leave the batch buffer before the next round rewrites it
+    CPSZ_HIP(hipStreamSynchronize(st));
+    pending.swap(still);
+  }
+  CPSZ_REQUIRE(pending.empty(), kErrInternal, "learned compression did not converge");
+  for (size_t v = 0; v < ntot; v++) stats.fallback_vertices += hf[v] && hg[v];
+  base = out;
+}
+
+// ------------------------------------------------------- 2D learned batch --
+// The single-field learned compressor over a batch: proposals per field in
+// row chunks, then rounds of one batched compress (one persistent wavefront
+// launch for all pending fields), one batched decompress and the exact judge
+// per field; the vertices of every changed triangle fall back to the analytic
+// bound until each field keeps its critical points.
+void compress_2d_learned_batch(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb,
+                               const float* params, int workers, int grid_cap, const uint8_t*& base,
+                               std::vector<size_t>& off, std::vector<size_t>& size, LearnedStats& stats) {
+  const size_t nf = fields.size();
+  CPSZ_REQUIRE(nf > 0 && params, kErrArg, "compress_2d_learned_batch: bad arguments");
+  hipStream_t st = ctx.stream();
+  stats = LearnedStats();
+  std::vector<size_t> nofs(nf + 1, 0);
+  size_t maxn = 0;
+  for (size_t k = 0; k < nf; k++) {
+    const TrainField& F = fields[k];
+    const size_t n = F.r1 * F.r2;
+    CPSZ_REQUIRE(F.dU && F.dV && n > 0 && n < (1ull << 32), kErrArg, "compress_2d_learned_batch: bad field");
+    nofs[k + 1] = nofs[k] + n;
+    maxn = std::max(maxn, n);
+  }
+  const size_t ntot = nofs[nf];
+  // ---- proposals: gain = clamp(floor(network output), 0, 24) per vertex
+  float* P = ctx.buf<float>("l2b.p", kNumParams);
+  CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, st));
+  uint8_t* gain = ctx.buf<uint8_t>("l2b.gain", ntot);
+  uint8_t* fb = ctx.buf<uint8_t>("l2b.fb", ntot);
+  double* E = ctx.buf<double>("l2b.e", maxn);
+  const size_t chunk = std::min<size_t>(maxn, 1u << 21);
+  uint16_t* X = ctx.buf<uint16_t>("l2b.x", chunk * kFeatDim);
+  float* y = ctx.buf<float>("l2b.y", chunk);
+  for (size_t k = 0; k < nf; k++) {
+    const TrainField& F = fields[k];
+    const size_t n = nofs[k + 1] - nofs[k];
+    analytic_eb_2d(ctx, F.dU, F.dV, F.r1, F.r2, max_pwr_eb, E);
+    for (size_t v0 = 0; v0 < n; v0 += chunk) {
+      const size_t mm = std::min(chunk, n - v0);
+      k_feat2d<<<blocks_for(mm), 256, 0, st>>>(F.dU, F.dV, E, (int)F.r1, (int)F.r2, max_pwr_eb, nullptr, v0, mm,
+                                               X);
+      CPSZ_HIP(hipGetLastError());
+      mlp_forward(ctx, P, X, mm, y);
+      k_gain<<<blocks_for(mm), 256, 0, st>>>(y, mm, gain + nofs[k] + v0);
+      CPSZ_HIP(hipGetLastError());
+    }
+  }
+  std::vector<uint8_t> hg(ntot), hf(ntot, 0);
+  CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, ntot, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  for (size_t v = 0; v < ntot; v++) stats.boosted_vertices += hg[v] != 0;
+  // ---- critical points of the originals
+  std::vector<std::vector<int64_t>> id0(nf);
+  std::vector<std::vector<int32_t>> ty0(nf);
+  for (size_t k = 0; k < nf; k++) cp_list(ctx, fields[k].dU, fields[k].dV, fields[k].r1, fields[k].r2, id0[k], ty0[k]);
+  // ---- verify and fall back until every field keeps its critical points
+  float* dec = ctx.buf<float>("l2b.dec", 2 * ntot);
+  std::vector<size_t> soff(nf + 1, 0);
+  for (size_t k = 0; k < nf; k++) soff[k + 1] = soff[k] + ((12 * (nofs[k + 1] - nofs[k]) + (1u << 20) + 255) & ~(size_t)255);
+  uint8_t* out = ctx.buf<uint8_t>("l2b.out", soff[nf]);
+  off.assign(soff.begin(), soff.end() - 1);
+  size.assign(nf, 0);
+  std::vector<size_t> pending(nf);
+  for (size_t k = 0; k < nf; k++) pending[k] = k;
+  Batch2dConfig cfg;
+  cfg.workers = workers;
+  cfg.grid_cap = grid_cap;
+  const int kMaxIter = 16;
+  for (int it = 0; it <= kMaxIter && !pending.empty(); it++) {
+    if (it == kMaxIter)  // the plain method for whatever is left
+      for (size_t k : pending) std::fill(hf.begin() + nofs[k], hf.begin() + nofs[k + 1], 1);
+    std::vector<Field2d> sub;
+    std::vector<EbOverride> ov;
+    for (size_t k : pending) {
+      const TrainField& F = fields[k];
+      sub.push_back(Field2d{F.dU, F.dV, F.r1, F.r2});
+      EbOverride o;
+      o.gain = gain + nofs[k];
+      o.fallback = fb + nofs[k];
+      ov.push_back(o);
+      CPSZ_HIP(hipMemcpyAsync(fb + nofs[k], hf.data() + nofs[k], nofs[k + 1] - nofs[k], hipMemcpyHostToDevice, st));
+    }
+    BatchOut bo;
+    compress_2d_batch(ctx, sub, max_pwr_eb, cfg, bo, &ov);
+    std::vector<const uint8_t*> ins;
+    std::vector<size_t> insz;
+    std::vector<Field2dOut> outs;
+    for (size_t q = 0; q < pending.size(); q++) {
+      const size_t k = pending[q];
+      const TrainField& F = fields[k];
+      ins.push_back(bo.base + bo.off[q]);
+      insz.push_back(bo.size[q]);
+      float* d = dec + 2 * nofs[k];
+      outs.push_back(Field2dOut{d, d + (nofs[k + 1] - nofs[k]), F.r1, F.r2});
+    }
+    decompress_2d_batch(ctx, ins, insz, outs, cfg);
+    stats.iterations = it + 1;
+    std::vector<size_t> still;
+    for (size_t q = 0; q < pending.size(); q++) {
+      const size_t k = pending[q];
+      const TrainField& F = fields[k];
+      std::vector<int64_t> id1;
+      std::vector<int32_t> ty1;
+      cp_list(ctx, outs[q].dU, outs[q].dV, F.r1, F.r2, id1, ty1);
+      const std::vector<int64_t> bad = cp_diff(id0[k], ty0[k], id1, ty1);
+      if (bad.empty()) {
+        CPSZ_REQUIRE(bo.size[q] <= soff[k + 1] - soff[k], kErrInternal, "learned stream bound exceeded");
+        CPSZ_HIP(hipMemcpyAsync(out + soff[k], bo.base + bo.off[q], bo.size[q], hipMemcpyDeviceToDevice, st));
+        size[k] = bo.size[q];
+        continue;
+      }
+      CPSZ_REQUIRE(it < kMaxIter, kErrInternal, "learned compression: the plain bound changed a critical point");
+      for (int64_t sid : bad) {
+        const int64_t cell = sid / 2;
+        const int t = (int)(sid % 2);
+        const size_t ci = (size_t)(cell / (int64_t)(F.r2 - 1)), cj = (size_t)(cell % (int64_t)(F.r2 - 1));
+        const size_t v0 = ci * F.r2 + cj;
+        const size_t v1 = t == 0 ? (ci + 1) * F.r2 + cj : ci * F.r2 + cj + 1;
+        const size_t v2 = (ci + 1) * F.r2 + cj + 1;
+        hf[nofs[k] + v0] = hf[nofs[k] + v1] = hf[nofs[k] + v2] = 1;
       }
       still.push_back(k);
     }

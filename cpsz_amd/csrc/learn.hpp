@@ -98,6 +98,11 @@ struct LearnedStats {
 };
 void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
                          double max_pwr_eb, const float* params, DeviceStream& out, LearnedStats& st);
+// Learned-bound compression of a batch of 2D fields (one persistent wavefront
+// launch per verification round); streams in a library-owned device buffer.
+void compress_2d_learned_batch(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb,
+                               const float* params, int workers, int grid_cap, const uint8_t*& base,
+                               std::vector<size_t>& off, std::vector<size_t>& size, LearnedStats& st);
 
 // ---- 3D (learn3d.hip): 3x3x3 patch of (u, v, w) plus the analytic bound;
 // labels keep every incident tetrahedron's critical point (24 per interior

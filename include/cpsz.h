@@ -286,6 +286,17 @@ int cpsz_learn_train_3d_dev(int device, void* stream, size_t nfields, const floa
                             const float* const* dV, const float* const* dW, const size_t* r1, const size_t* r2,
                             const size_t* r3, double max_pwr_eb, size_t samples_per_field, int epochs, int batch,
                             float lr, uint64_t seed, float* params, float* loss_per_epoch);
+/* Learned-bound compression of a batch of 2D fields (no reference
+ * counterpart): the single-field learned compressor over many fields, each
+ * verification round one batched compress and decompress of the fields still
+ * pending, judged exactly per field; the vertices of changed triangles fall
+ * back to the analytic bound. Streams (decodable by the plain 2D decoder, with
+ * the originals' critical points) are returned like the batch codec's. */
+int cpsz_compress_2d_learned_batch_dev(int device, void* stream, size_t nfields, const float* const* dU,
+                                       const float* const* dV, const size_t* r1, const size_t* r2,
+                                       double max_pwr_eb, const float* params, int tune,
+                                       const unsigned char** d_base, size_t* offsets, size_t* sizes,
+                                       int* iterations, size_t* boosted, size_t* fallback);
 /* Learned-bound compression of a batch of 3D fields: every verification
  * round compresses the fields still pending in one batched call (one
  * persistent wavefront launch), decodes them in one batched call and judges

@@ -144,3 +144,27 @@ def test_learned_ratio_gain_at_zero_cp_errors(orc, trained):
     # every vertex within its own (learned or analytic) bound: |x' - x| <= |x| * 0.1
     for a, b in ((U, dU2), (V, dV2)):
         assert np.all(np.abs(b.astype(np.float64) - a) <= np.abs(a.astype(np.float64)) * 0.1 + 1e-30)
+
+
+def test_learned_batch_equals_single_field_learned(trained):
+    """The batched learned compressor gives every field the stream the
+    single-field learned compressor gives it (same proposals, same per-field
+    verification and fallback), across uneven shapes."""
+    p, _ = trained
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    shapes = [(130, 170), (300, 90), (64, 257)]
+    fields = [field_2d(r1, r2, 700 + k) for k, (r1, r2) in enumerate(shapes)]
+    dev = [(_dev(U), _dev(V)) for U, V in fields]
+    base, off, size, it, boosted, fb = cz.compress_2d_learned_batch_dev(
+        [(u.data_ptr(), v.data_ptr(), r1, r2) for (u, v), (r1, r2) in zip(dev, shapes)], 0.1, p)
+    torch.cuda.synchronize()
+    print(f"[learned batch] sizes {list(size)}, {it} rounds, {boosted} boosted, {fb} fell back", flush=True)
+    for k, ((u, v), (r1, r2)) in enumerate(zip(dev, shapes)):
+        got = np.empty(int(size[k]), np.uint8)
+        hip.hipMemcpy(C.c_void_p(got.ctypes.data), C.c_void_p(base + int(off[k])), C.c_size_t(got.size), 2)
+        ptr, n1, _, _, _ = cz.compress_2d_learned_dev(u.data_ptr(), v.data_ptr(), r1, r2, 0.1, p)
+        torch.cuda.synchronize()
+        one = np.empty(n1, np.uint8)
+        hip.hipMemcpy(C.c_void_p(one.ctypes.data), C.c_void_p(ptr), C.c_size_t(n1), 2)
+        assert np.array_equal(got, one), k
