@@ -12,6 +12,7 @@ from .api import (CpszError, compress_2d_dev, compress_3d_dev, critical_points_2
                   sz_lossless_compress, sz_lossless_decompress)
 from .api import batch_2d_config, compress_2d_batch_dev, decompress_2d_batch_dev  # noqa: F401
 from .api import batch_3d_config, compress_3d_batch_dev, decompress_3d_batch_dev  # noqa: F401
+from .api import compress_2d_learned_batch_dev  # noqa: F401
 from .api import (compress_2d_learned_dev, gemm_bf16_dev, learn_analytic_eb_2d_dev,  # noqa: F401
                   learn_features_2d_dev, learn_init_params, learn_labels_2d_dev, learn_param_count,
                   learn_train_2d_dev, mlp_forward_dev, mlp_grad_dev, sz_compress_cp_preserve_2d_learned)

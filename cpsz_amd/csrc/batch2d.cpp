@@ -6,9 +6,11 @@
 // the entropy stage (histograms, host Huffman trees, encode, stream assembly)
 // of different fields runs concurrently on a pool of worker threads, each with
 // its own HIP stream and context, so host tree building and small kernels of
-// one field overlap with those of others. Decompress mirrors it: per-field
-// parsing and Huffman decoding on the workers, then one persistent wavefront
-// launch. Every field's stream is byte-identical to the single-field codec's.
+// one field overlap with those of others. Decompress: per-field parsing and
+// decoder tables on the workers, then every field's bitstreams decoded
+// together (one grid per decode pass for all fields), the unpredictable
+// scatter on the workers and one persistent wavefront launch. Every field's
+// stream is byte-identical to the single-field codec's.
 //
 // The autotuner times candidate configurations (worker count, persistent grid
 // size) on the caller's batch itself (outputs do not depend on them) and
@@ -139,14 +141,7 @@ void decompress_2d_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, cons
     wp[k] = &w[k];
   }
   CPSZ_HIP(hipStreamSynchronize(st));  // inputs written on the caller's stream
-  std::vector<std::function<void(Ctx&)>> jobs(nf);
-  for (size_t k = 0; k < nf; k++) {
-    jobs[k] = [&, k](Ctx& wc) {
-      d2_entropy(wc, d_in[k], nullptr, in_size[k], w[k]);
-      CPSZ_HIP(hipStreamSynchronize(wc.stream()));
-    };
-  }
-  pool_run(pool_for(ctx.device()), jobs, cfg.workers);
+  d2_entropy_batch(ctx, d_in, in_size, wp.data(), nf, cfg.workers);
   ctx.mark("entropy");
   d2_wavefront(ctx, wp.data(), nf, cfg.grid_cap);
   d2_check(ctx, wp.data(), nf);

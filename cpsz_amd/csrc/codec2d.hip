@@ -915,6 +915,80 @@ HuffSection parse_huff(StreamReader& r, size_t off) {
   return h;
 }
 
+// The header fields and section positions of a 2D stream.
+struct D2Plan {
+  HuffSection h_eb;
+  struct Comp {
+    size_t off = 0, unp_count = 0, unp_off = 0;
+    HuffSection h;
+  } comp[2];
+  int nby = 0;
+  size_t nb = 0;
+};
+
+void d2_parse(StreamReader& rd, size_t r1, size_t r2, D2Plan& p) {
+  const size_t n = r1 * r2;
+  const int base = rd.get<int>(0);
+  const double thr = rd.get<double>(4);
+  const size_t u_size = rd.get<size_t>(20);
+  CPSZ_REQUIRE(base == 4 && thr == DBL_EPSILON, kErrUnsupported,
+               "decompress_2d: stream base/threshold differ from the offline 2D format");
+  p.h_eb = parse_huff(rd, 36);
+  // components follow the id section directly; the u section spans u_size bytes
+  p.nby = (int)div_up(r2, kBlock2d);
+  p.nb = div_up(r1, kBlock2d) * p.nby;
+  size_t off = p.h_eb.end;
+  for (int c = 0; c < 2; c++) {
+    D2Plan::Comp& k = p.comp[c];
+    k.off = off;
+    const int bs = rd.get<int>(off);
+    const int radius = rd.get<int>(off + 5);
+    const uint8_t use_mean = rd.get<uint8_t>(off + 9);
+    const size_t reg = rd.get<size_t>(off + 17);
+    k.unp_count = rd.get<size_t>(off + 25);
+    CPSZ_REQUIRE(bs == kBlock2d && radius == kRadius && use_mean == 0 && reg == 0, kErrUnsupported,
+                 "decompress_2d: stream uses block settings the offline compressor never writes");
+    CPSZ_REQUIRE(k.unp_count <= n, kErrStream, "bad unpredictable count");
+    k.unp_off = off + 33;
+    const size_t ind = k.unp_off + 4 * k.unp_count;
+    const size_t ind_bytes = (p.nb - 1) / 8 + 1;
+    const uint8_t* ib = rd.read(ind, ind_bytes);
+    for (size_t b = 0; b < ind_bytes; b++)
+      CPSZ_REQUIRE(ib[b] == 0, kErrUnsupported, "regression blocks are not produced by the 2D codec");
+    k.h = parse_huff(rd, ind + ind_bytes);
+    off = c == 0 ? off + u_size : k.h.end;
+  }
+}
+
+// Largest symbol of a parsed tree (ids above 81 need the wide wavefront path).
+uint32_t max_symbol(const HuffDecodeTree& t) {
+  uint32_t m = 0;
+  for (size_t i = 0; i < t.node_count; i++)
+    if ((t.leaf[i] || (i == 0 && t.root_leaf)) && t.sym[i] > m) m = t.sym[i];
+  return m;
+}
+
+// Unpredictable values to per-element raw arrays (block-major order).
+void d2_unpred(Ctx& ctx, const uint8_t* d_in, const D2Plan& p, D2Work& w) {
+  hipStream_t st = ctx.stream();
+  const size_t r1 = w.r1, r2 = w.r2, n = r1 * r2, nb = p.nb;
+  uint32_t* zc = ctx.buf<uint32_t>("d2.zc", 2 * nb);
+  uint64_t* zoff = ctx.buf<uint64_t>("d2.zoff", 2 * nb + 4);
+  CPSZ_HIP(hipMemsetAsync(w.err, 0, 8, st));
+  uint16_t* qs[2] = {w.qu, w.qv};
+  for (int c = 0; c < 2; c++) {
+    float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", p.comp[c].unp_count + 1);
+    if (p.comp[c].unp_count)
+      CPSZ_HIP(hipMemcpyAsync(vals, d_in + p.comp[c].unp_off, 4 * p.comp[c].unp_count, hipMemcpyDeviceToDevice, st));
+    k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], (int)r1, (int)r2, p.nby, nb, zc + c * nb);
+    scan_u32_to_u64(ctx, zc + c * nb, zoff + c * (nb + 1), nb, zoff + c * (nb + 1) + nb);
+    k_scatter_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], vals, (int)r1, (int)r2, p.nby, nb,
+                                                           zoff + c * (nb + 1), p.comp[c].unp_count,
+                                                           w.raw + c * n, w.err);
+    CPSZ_HIP(hipGetLastError());
+  }
+}
+
 }  // namespace
 
 size_t d2_scratch_bytes(size_t r1, size_t r2) {
@@ -944,44 +1018,11 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   hipStream_t st = ctx.stream();
   const size_t r1 = w.r1, r2 = w.r2, n = r1 * r2;
   uint16_t *sid_u = w.sid_u, *sid_v = w.sid_v, *qu = w.qu, *qv = w.qv;
-  float* raw = w.raw;
-  int* err = w.err;
   StreamReader rd(ctx, h_in, d_in, in_size);
-  const int base = rd.get<int>(0);
-  const double thr = rd.get<double>(4);
-  const size_t u_size = rd.get<size_t>(20);
-  (void)u_size;
-  CPSZ_REQUIRE(base == 4 && thr == DBL_EPSILON, kErrUnsupported,
-               "decompress_2d: stream base/threshold differ from the offline 2D format");
-  HuffSection h_eb = parse_huff(rd, 36);
-  // components follow the id section directly; the u section spans u_size bytes
-  struct Comp {
-    size_t off, unp_count, unp_off;
-    HuffSection h;
-  } comp[2];
-  const int nby = (int)div_up(r2, kBlock2d);
-  const size_t nb = div_up(r1, kBlock2d) * nby;
-  size_t off = h_eb.end;
-  for (int c = 0; c < 2; c++) {
-    Comp& k = comp[c];
-    k.off = off;
-    const int bs = rd.get<int>(off);
-    const int radius = rd.get<int>(off + 5);
-    const uint8_t use_mean = rd.get<uint8_t>(off + 9);
-    const size_t reg = rd.get<size_t>(off + 17);
-    k.unp_count = rd.get<size_t>(off + 25);
-    CPSZ_REQUIRE(bs == kBlock2d && radius == kRadius && use_mean == 0 && reg == 0, kErrUnsupported,
-                 "decompress_2d: stream uses block settings the offline compressor never writes");
-    CPSZ_REQUIRE(k.unp_count <= n, kErrStream, "bad unpredictable count");
-    k.unp_off = off + 33;
-    const size_t ind = k.unp_off + 4 * k.unp_count;
-    const size_t ind_bytes = (nb - 1) / 8 + 1;
-    const uint8_t* ib = rd.read(ind, ind_bytes);
-    for (size_t b = 0; b < ind_bytes; b++)
-      CPSZ_REQUIRE(ib[b] == 0, kErrUnsupported, "regression blocks are not produced by the 2D codec");
-    k.h = parse_huff(rd, ind + ind_bytes);
-    off = c == 0 ? off + u_size : k.h.end;
-  }
+  D2Plan p;
+  d2_parse(rd, r1, r2, p);
+  const HuffSection& h_eb = p.h_eb;
+  const int nby = p.nby;
   ctx.mark("parse");
   // ---- bitstreams into aligned scratch
   auto stage_bits = [&](const HuffSection& h, const char* name) {
@@ -1001,7 +1042,7 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   thread_local HostDecoder hc_keep[2];
   HostDecoder* hc = hc_keep;
   auto prep = [&](int c) {
-    prepare_decoder(comp[c].h.tree.data(), comp[c].h.tree.size(), comp[c].h.node_count, hc[c]);
+    prepare_decoder(p.comp[c].h.tree.data(), p.comp[c].h.tree.size(), p.comp[c].h.node_count, hc[c]);
   };
   {
     HelperJob u_ready([&] { prep(0); }, !ctx.batch_worker);
@@ -1011,43 +1052,90 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
     u_ready.join();  // rethrows a malformed code tree
   }
   {
-    uint32_t* wu = stage_bits(comp[0].h, "d2.w.u");
-    uint32_t* wv = stage_bits(comp[1].h, "d2.w.v");
+    uint32_t* wu = stage_bits(p.comp[0].h, "d2.w.u");
+    uint32_t* wv = stage_bits(p.comp[1].h, "d2.w.v");
     const DeviceDecoder du = upload_decoder(ctx, hc[0], "d2.u");
     const DeviceDecoder dv = upload_decoder(ctx, hc[1], "d2.v");
     DecJob<SinkCodes2d> jobs[2] = {
-        {&du, wu, comp[0].h.nbytes, n, SinkCodes2d{qu, (int)r1, (int)r2, nby}, "d2.u"},
-        {&dv, wv, comp[1].h.nbytes, n, SinkCodes2d{qv, (int)r1, (int)r2, nby}, "d2.v"}};
+        {&du, wu, p.comp[0].h.nbytes, n, SinkCodes2d{qu, (int)r1, (int)r2, nby}, "d2.u"},
+        {&dv, wv, p.comp[1].h.nbytes, n, SinkCodes2d{qv, (int)r1, (int)r2, nby}, "d2.v"}};
     decode_streams(ctx, jobs, 2);
   }
   ctx.mark("huffman");
-  // ---- unpredictable values to per-element raw arrays (block-major order)
-  uint32_t* zc = ctx.buf<uint32_t>("d2.zc", 2 * nb);
-  uint64_t* zoff = ctx.buf<uint64_t>("d2.zoff", 2 * nb + 4);
-  CPSZ_HIP(hipMemsetAsync(err, 0, 8, st));
-  uint16_t* qs[2] = {qu, qv};
-  for (int c = 0; c < 2; c++) {
-    float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", comp[c].unp_count + 1);
-    if (comp[c].unp_count)
-      CPSZ_HIP(hipMemcpyAsync(vals, d_in + comp[c].unp_off, 4 * comp[c].unp_count,
-                              hipMemcpyDeviceToDevice, st));
-    k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], (int)r1, (int)r2, nby, nb, zc + c * nb);
-    scan_u32_to_u64(ctx, zc + c * nb, zoff + c * (nb + 1), nb, zoff + c * (nb + 1) + nb);
-    k_scatter_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], vals, (int)r1, (int)r2, nby, nb,
-                                                           zoff + c * (nb + 1), comp[c].unp_count,
-                                                           raw + c * n, err);
-    CPSZ_HIP(hipGetLastError());
-  }
-  // ids above 81 need the double-precision reconstruction path
-  uint32_t max_id = 0;
+  d2_unpred(ctx, d_in, p, w);
   {
     HuffDecodeTree t;
     CPSZ_REQUIRE(huff_parse_tree(h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, t), kErrStream,
                  "malformed id tree");
-    for (size_t i = 0; i < t.node_count; i++)
-      if ((t.leaf[i] || (i == 0 && t.root_leaf)) && t.sym[i] > max_id) max_id = t.sym[i];
+    w.wide = max_symbol(t) > 81;
   }
-  w.wide = max_id > 81;
+  ctx.mark("unpred");
+}
+
+// Batched form: every field's headers parsed and trees tabulated on the
+// workers; all decoders uploaded in one copy and all bitstreams staged in one
+// launch; the id streams of every field decoded together, then the u and v
+// streams of every field together (one grid per pass, one host check per
+// round); the unpredictable scatter of each field on the workers.
+void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<size_t>& in_size,
+                      D2Work* const* w, size_t nf, int workers) {
+  hipStream_t st = ctx.stream();
+  std::vector<D2Plan> plan(nf);
+  std::vector<HostDecoder> hd(3 * nf);
+  {
+    std::vector<std::function<void(Ctx&)>> jobs(nf);
+    for (size_t k = 0; k < nf; k++)
+      jobs[k] = [&, k](Ctx& wc) {
+        CPSZ_REQUIRE(d_in[k] && w[k]->r1 >= 1 && w[k]->r2 >= 1, kErrArg, "decompress_2d: bad arguments");
+        StreamReader rd(wc, nullptr, d_in[k], in_size[k]);
+        d2_parse(rd, w[k]->r1, w[k]->r2, plan[k]);
+        const D2Plan& p = plan[k];
+        prepare_decoder(p.h_eb.tree.data(), p.h_eb.tree.size(), p.h_eb.node_count, hd[3 * k]);
+        for (int c = 0; c < 2; c++)
+          prepare_decoder(p.comp[c].h.tree.data(), p.comp[c].h.tree.size(), p.comp[c].h.node_count,
+                          hd[3 * k + 1 + c]);
+        w[k]->wide = max_symbol(hd[3 * k].d) > 81;
+      };
+    pool_run(pool_for(ctx.device()), jobs, workers);
+  }
+  ctx.mark("parse");
+  std::vector<const HostDecoder*> hp(3 * nf);
+  for (size_t k = 0; k < 3 * nf; k++) hp[k] = &hd[k];
+  std::vector<DeviceDecoder> dd(3 * nf);
+  upload_decoders(ctx, hp.data(), (int)(3 * nf), dd.data(), "b2d.dec");
+  std::vector<BitsCopy> bc(3 * nf);
+  for (size_t k = 0; k < nf; k++) {
+    const D2Plan& p = plan[k];
+    bc[3 * k] = BitsCopy{d_in[k] + p.h_eb.bits_off, p.h_eb.nbytes, nullptr, 0};
+    for (int c = 0; c < 2; c++)
+      bc[3 * k + 1 + c] = BitsCopy{d_in[k] + p.comp[c].h.bits_off, p.comp[c].h.nbytes, nullptr, 0};
+  }
+  stage_bits_multi(ctx, bc.data(), (int)(3 * nf), "b2d.bits");
+  std::vector<DecJob<SinkIds2d>> ji(nf);
+  std::vector<DecJob<SinkCodes2d>> jc(2 * nf);
+  for (size_t k = 0; k < nf; k++) {
+    const size_t r1 = w[k]->r1, r2 = w[k]->r2, n = r1 * r2;
+    const D2Plan& p = plan[k];
+    ji[k] = {&dd[3 * k], bc[3 * k].dst, p.h_eb.nbytes, 2 * n, SinkIds2d{w[k]->sid_u, w[k]->sid_v, n}, "b2d"};
+    uint16_t* qs[2] = {w[k]->qu, w[k]->qv};
+    for (int c = 0; c < 2; c++)
+      jc[2 * k + c] = {&dd[3 * k + 1 + c], bc[3 * k + 1 + c].dst, p.comp[c].h.nbytes, n,
+                       SinkCodes2d{qs[c], (int)r1, (int)r2, p.nby}, "b2d"};
+  }
+  decode_streams_multi(ctx, ji.data(), (int)nf, "b2d.ids");
+  decode_streams_multi(ctx, jc.data(), (int)(2 * nf), "b2d.codes");
+  ctx.mark("huffman");
+  // the workers' streams start after the decode
+  CPSZ_HIP(hipStreamSynchronize(st));
+  {
+    std::vector<std::function<void(Ctx&)>> jobs(nf);
+    for (size_t k = 0; k < nf; k++)
+      jobs[k] = [&, k](Ctx& wc) {
+        d2_unpred(wc, d_in[k], plan[k], *w[k]);
+        CPSZ_HIP(hipStreamSynchronize(wc.stream()));
+      };
+    pool_run(pool_for(ctx.device()), jobs, workers);
+  }
   ctx.mark("unpred");
 }
 

@@ -1,5 +1,7 @@
 // 2D offline codec entry points on device buffers (see codec2d.hip).
 #pragma once
+#include <vector>
+
 #include "ctx.hpp"
 
 namespace cpsz {
@@ -57,6 +59,10 @@ struct D2Work {
 size_t d2_scratch_bytes(size_t r1, size_t r2);
 void d2_bind(D2Work& w, uint8_t* scratch);
 void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, D2Work& w);
+// The entropy stage of many fields at once (headers and trees on `workers`
+// pool threads, every field's bitstreams decoded together on ctx's stream).
+void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<size_t>& in_size,
+                      D2Work* const* w, size_t nf, int workers);
 void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap);
 void d2_check(Ctx& ctx, D2Work* const* w, size_t nf);
 

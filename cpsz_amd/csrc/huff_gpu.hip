@@ -18,16 +18,15 @@ __global__ void k_scatter_codebook(const uint32_t* syms, const uint64_t* codes, 
   len[syms[i]] = lens[i];
 }
 
-__global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
-                                                  uint32_t S, uint32_t nseg,
-                                                  const uint64_t* __restrict__ prev_exit,
-                                                  const uint64_t* __restrict__ prev_start,
-                                                  uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
-                                                  uint32_t* __restrict__ cnt, int* __restrict__ changed,
-                                                  int* __restrict__ err, int first) {
-  __shared__ uint32_t slut[1 << kLutBits];
-  __shared__ uint32_t sw[kStageWords];
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// Block `blk` of one stream's speculative / fix-up pass (k_dec_spec runs one
+// stream's blocks, k_dec_spec_multi every stream's blocks in one grid).
+__device__ __forceinline__ void dec_spec_block(const DecTree& t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                               uint32_t S, uint32_t nseg, const uint64_t* __restrict__ prev_exit,
+                                               const uint64_t* __restrict__ prev_start,
+                                               uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
+                                               uint32_t* __restrict__ cnt, int* __restrict__ err, int first,
+                                               uint32_t blk, uint32_t* slut, uint32_t* sw) {
+  const uint32_t i = blk * blockDim.x + threadIdx.x;
   bool work = i < nseg;
   uint64_t s0 = 0;
   if (work) {
@@ -45,7 +44,7 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
   if (!__syncthreads_or(work)) return;
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
   uint64_t lbase;
-  const uint32_t lcount = stage_words(sw, w, nbits, S, blockIdx.x * blockDim.x, lbase);
+  const uint32_t lcount = stage_words(sw, w, nbits, S, blk * blockDim.x, lbase);
   __syncthreads();
   if (!work) return;
   uint64_t end = (uint64_t)(i + 1) * S;
@@ -112,7 +111,121 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
   start[i] = s0;
   exitp[i] = ex;
   cnt[i] = c;
+}
+
+__global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                  uint32_t S, uint32_t nseg,
+                                                  const uint64_t* __restrict__ prev_exit,
+                                                  const uint64_t* __restrict__ prev_start,
+                                                  uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
+                                                  uint32_t* __restrict__ cnt, int* __restrict__ changed,
+                                                  int* __restrict__ err, int first) {
+  __shared__ uint32_t slut[1 << kLutBits];
+  __shared__ uint32_t sw[kStageWords];
   (void)changed;
+  dec_spec_block(t, w, nbits, S, nseg, prev_exit, prev_start, start, exitp, cnt, err, first, blockIdx.x, slut, sw);
+}
+
+// ---- multi-stream passes: block b belongs to the job j with
+// blk0[j] <= b < blk0[j + 1]; `cur` selects the ping-pong buffers.
+__device__ __forceinline__ int mdec_job(const uint32_t* __restrict__ blk0, int nj, uint32_t b) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_dec_spec_multi(const MDecJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ blk0, int nj, int cur,
+                                                        int first, int* __restrict__ err) {
+  __shared__ uint32_t slut[1 << kLutBits];
+  __shared__ uint32_t sw[kStageWords];
+  const int j = mdec_job(blk0, nj, blockIdx.x);
+  const MDecJob& J = jobs[j];
+  if (first)
+    dec_spec_block(J.t, J.w, J.nbits, J.S, J.nseg, nullptr, nullptr, J.start[0], J.ex[0], J.cnt, err + j, 1,
+                   blockIdx.x - blk0[j], slut, sw);
+  else
+    dec_spec_block(J.t, J.w, J.nbits, J.S, J.nseg, J.ex[cur], J.start[cur], J.start[cur ^ 1], J.ex[cur ^ 1],
+                   J.cnt, err + j, 0, blockIdx.x - blk0[j], slut, sw);
+}
+
+// changed[j] = 1 when a segment of job j does not start at its predecessor's exit
+__global__ void __launch_bounds__(256) k_dec_check_multi(const MDecJob* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ blk0, int nj, int cur,
+                                                         int* __restrict__ changed) {
+  const int j = mdec_job(blk0, nj, blockIdx.x);
+  const MDecJob& J = jobs[j];
+  const uint32_t i = (blockIdx.x - blk0[j]) * blockDim.x + threadIdx.x;
+  if (i >= J.nseg) return;
+  const uint64_t want = i == 0 ? 0 : J.ex[cur][i - 1];
+  if (J.start[cur][i] != want) changed[j] = 1;
+}
+
+// Exclusive scan of the per-segment counts of every job, in three passes:
+// block sums, a scan of each job's block sums (one workgroup per job, which
+// also writes the job's total), then the in-block scans.
+__global__ void __launch_bounds__(256) k_dec_bsum_multi(const MDecJob* __restrict__ jobs,
+                                                        const uint32_t* __restrict__ blk0, int nj) {
+  __shared__ uint64_t red[256];
+  const int j = mdec_job(blk0, nj, blockIdx.x);
+  const MDecJob& J = jobs[j];
+  const uint32_t b = blockIdx.x - blk0[j];
+  const uint32_t i = b * blockDim.x + threadIdx.x;
+  red[threadIdx.x] = i < J.nseg ? J.cnt[i] : 0;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) J.bsum[b] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_dec_bscan_multi(const MDecJob* __restrict__ jobs, uint64_t* __restrict__ totals) {
+  __shared__ uint64_t sc[256];
+  const MDecJob& J = jobs[blockIdx.x];
+  uint64_t carry = 0;
+  for (uint32_t base = 0; base < J.nblk; base += 256) {
+    const uint32_t k = base + threadIdx.x;
+    const uint64_t v = k < J.nblk ? J.bsum[k] : 0;
+    sc[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = 1; s < 256; s <<= 1) {  // inclusive Hillis-Steele scan
+      const uint64_t a = (int)threadIdx.x >= s ? sc[threadIdx.x - s] : 0;
+      __syncthreads();
+      sc[threadIdx.x] += a;
+      __syncthreads();
+    }
+    if (k < J.nblk) J.bsum[k] = carry + sc[threadIdx.x] - v;
+    carry += sc[255];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    J.off[J.nseg] = carry;
+    totals[blockIdx.x] = carry;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_dec_off_multi(const MDecJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ blk0, int nj) {
+  __shared__ uint64_t sc[256];
+  const int j = mdec_job(blk0, nj, blockIdx.x);
+  const MDecJob& J = jobs[j];
+  const uint32_t b = blockIdx.x - blk0[j];
+  const uint32_t i = b * blockDim.x + threadIdx.x;
+  const uint64_t v = i < J.nseg ? J.cnt[i] : 0;
+  sc[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = 1; s < 256; s <<= 1) {
+    const uint64_t a = (int)threadIdx.x >= s ? sc[threadIdx.x - s] : 0;
+    __syncthreads();
+    sc[threadIdx.x] += a;
+    __syncthreads();
+  }
+  if (i < J.nseg) J.off[i] = J.bsum[b] + sc[threadIdx.x] - v;
 }
 
 __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t* start,
@@ -321,6 +434,158 @@ DeviceDecoder upload_decoder(Ctx& ctx, const HostDecoder& hd, const char* tag) {
   // The pinned staging buffer is reused by the next decoder upload of the same
   // tag only after the stream has consumed it (callers synchronize per call).
   return out;
+}
+
+// ---- many decoders at once: one packed host-to-device copy and one level-2
+// fill launch for all of them
+namespace {
+struct Lut2Fill {
+  const uint32_t *left, *right, *sym, *pfx;
+  const uint8_t* leaf;
+  uint32_t* lut2;
+  uint32_t npfx, n2;
+  uint64_t e0;  // first entry of this decoder in the launch
+};
+
+__global__ void k_fill_lut2_multi(const Lut2Fill* __restrict__ f, int nf, uint64_t total) {
+  const uint64_t g = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (g >= total) return;
+  int lo = 0, hi = nf - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (f[mid].e0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  const Lut2Fill& d = f[lo];
+  const uint32_t e = (uint32_t)(g - d.e0);
+  uint32_t a = 0, b = d.npfx - 1;
+  while (a < b) {
+    const uint32_t mid = (a + b + 1) >> 1;
+    if (d.pfx[3 * mid + 2] <= e) a = mid;
+    else b = mid - 1;
+  }
+  d.lut2[e] = lut2_entry(d.left, d.right, d.sym, d.leaf, d.pfx[3 * a], (int)d.pfx[3 * a + 1], e - d.pfx[3 * a + 2]);
+}
+
+__global__ void k_stage_bits_multi(const BitsCopy* __restrict__ c, const uint32_t* __restrict__ blk0, int nc) {
+  int lo = 0, hi = nc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const BitsCopy& j = c[lo];
+  const uint64_t k0 = (uint64_t)(blockIdx.x - blk0[lo]) * kStageBlockWords;
+  for (uint32_t t = threadIdx.x; t < kStageBlockWords; t += blockDim.x) {
+    const uint64_t k = k0 + t;
+    if (k >= j.nwords) break;
+    const uint64_t b = 4 * k;
+    uint32_t v = 0;
+    if (b + 4 <= j.nbytes) {
+      v = (uint32_t)j.src[b] | (uint32_t)j.src[b + 1] << 8 | (uint32_t)j.src[b + 2] << 16 |
+          (uint32_t)j.src[b + 3] << 24;
+    } else {
+      for (int q = 0; q < 4; q++)
+        if (b + q < j.nbytes) v |= (uint32_t)j.src[b + q] << (8 * q);
+    }
+    j.dst[k] = v;
+  }
+}
+}  // namespace
+
+void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecoder* out, const char* tag) {
+  hipStream_t st = ctx.stream();
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  std::vector<size_t> base(n + 1, 0), l2base(n + 1, 0);
+  for (int k = 0; k < n; k++) {
+    const HostDecoder& h = *hd[k];
+    const size_t N = h.d.root_leaf ? 0 : h.d.node_count;
+    base[k + 1] = base[k] + (h.d.root_leaf ? 0 : al(h.lut.size() * 4 + h.pfx.size() * 4 + 13 * N));
+    l2base[k + 1] = l2base[k] + (h.d.root_leaf ? 0 : al(4 * (h.n2 + 1)));
+  }
+  const std::string t(tag);
+  const size_t nfill = sizeof(Lut2Fill) * (size_t)(n ? n : 1);
+  uint8_t* hbuf = (uint8_t*)ctx.host_buf((t + ".h").c_str(), base[n] + nfill);
+  uint8_t* dbuf = ctx.buf<uint8_t>((t + ".d").c_str(), base[n] + nfill);
+  uint32_t* dl2 = ctx.buf<uint32_t>((t + ".lut2").c_str(), l2base[n] / 4 + 1);
+  Lut2Fill* hf = (Lut2Fill*)(hbuf + base[n]);
+  int nf = 0;
+  uint64_t total2 = 0;
+  for (int k = 0; k < n; k++) {
+    const HostDecoder& h = *hd[k];
+    DeviceDecoder& o = out[k];
+    o = DeviceDecoder();
+    if (h.d.root_leaf) {
+      o.single = true;
+      o.single_sym = h.d.root_sym;
+      continue;
+    }
+    const size_t N = h.d.node_count, lut_b = h.lut.size() * 4, pfx_b = h.pfx.size() * 4, arr_b = N * 4;
+    uint8_t* q = hbuf + base[k];
+    std::memcpy(q, h.lut.data(), lut_b);
+    if (pfx_b) std::memcpy(q + lut_b, h.pfx.data(), pfx_b);
+    uint8_t* a = q + lut_b + pfx_b;
+    std::memcpy(a, h.d.left.data(), arr_b);
+    std::memcpy(a + arr_b, h.d.right.data(), arr_b);
+    std::memcpy(a + 2 * arr_b, h.d.sym.data(), arr_b);
+    std::memcpy(a + 3 * arr_b, h.d.leaf.data(), N);
+    const uint8_t* dq = dbuf + base[k];
+    const uint8_t* da = dq + lut_b + pfx_b;
+    o.tree.lut = (const uint32_t*)dq;
+    o.tree.left = (const uint32_t*)da;
+    o.tree.right = (const uint32_t*)(da + arr_b);
+    o.tree.sym = (const uint32_t*)(da + 2 * arr_b);
+    o.tree.leaf = da + 3 * arr_b;
+    o.tree.lut2 = dl2 + l2base[k] / 4;
+    o.tree.nodes = (uint32_t)N;
+    if (h.n2) {
+      Lut2Fill& f = hf[nf++];
+      f.left = o.tree.left;
+      f.right = o.tree.right;
+      f.sym = o.tree.sym;
+      f.leaf = o.tree.leaf;
+      f.pfx = (const uint32_t*)(dq + lut_b);
+      f.lut2 = dl2 + l2base[k] / 4;
+      f.npfx = (uint32_t)(h.pfx.size() / 3);
+      f.n2 = (uint32_t)h.n2;
+      f.e0 = total2;
+      total2 += h.n2;
+    }
+  }
+  CPSZ_HIP(hipMemcpyAsync(dbuf, hbuf, base[n] + sizeof(Lut2Fill) * nf, hipMemcpyHostToDevice, st));
+  if (total2) {
+    k_fill_lut2_multi<<<(unsigned)div_up(total2, (uint64_t)256), 256, 0, st>>>((const Lut2Fill*)(dbuf + base[n]),
+                                                                                nf, total2);
+    CPSZ_HIP(hipGetLastError());
+  }
+}
+
+void stage_bits_multi(Ctx& ctx, BitsCopy* c, int n, const char* tag) {
+  hipStream_t st = ctx.stream();
+  std::vector<uint32_t> blk0(n + 1, 0);
+  size_t words = 0;
+  for (int k = 0; k < n; k++) {
+    c[k].nwords = div_up(c[k].nbytes, (uint64_t)4) + 8;  // zero slack past the last bit
+    words += (c[k].nwords + 63) & ~(uint64_t)63;
+  }
+  const std::string t(tag);
+  uint32_t* dw = ctx.buf<uint32_t>((t + ".words").c_str(), words + 1);
+  size_t w0 = 0;
+  for (int k = 0; k < n; k++) {
+    c[k].dst = dw + w0;
+    w0 += (c[k].nwords + 63) & ~(uint64_t)63;
+    blk0[k + 1] = blk0[k] + (uint32_t)div_up(c[k].nwords, (uint64_t)kStageBlockWords);
+  }
+  const size_t tb = sizeof(BitsCopy) * n, bb = 4 * (size_t)(n + 1);
+  uint8_t* h = (uint8_t*)ctx.host_buf((t + ".tab").c_str(), tb + bb);
+  std::memcpy(h, c, tb);
+  std::memcpy(h + tb, blk0.data(), bb);
+  uint8_t* d = ctx.buf<uint8_t>((t + ".tab").c_str(), tb + bb);
+  CPSZ_HIP(hipMemcpyAsync(d, h, tb + bb, hipMemcpyHostToDevice, st));
+  if (blk0[n]) {
+    k_stage_bits_multi<<<blk0[n], 256, 0, st>>>((const BitsCopy*)d, (const uint32_t*)(d + tb), n);
+    CPSZ_HIP(hipGetLastError());
+  }
 }
 
 DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,

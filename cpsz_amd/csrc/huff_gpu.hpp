@@ -19,6 +19,7 @@
 
 #include <rocprim/device/device_scan.hpp>
 #include <rocprim/iterator/transform_iterator.hpp>
+#include <cstring>
 #include <functional>
 #include <string>
 #include <type_traits>
@@ -341,6 +342,32 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
 __global__ void k_dec_check(uint32_t nseg, const uint64_t* exitp, const uint64_t* start,
                             int* changed);
 
+// One bitstream of a multi-stream decode (every stream's segments in one grid):
+// its grid blocks are [blk0, blk0 + nblk); start/ex are the ping-pong buffers of
+// the passes, cnt the per-segment counts, off their exclusive scan (nseg + 1
+// entries) and bsum the per-block sums, then block offsets.
+struct MDecJob {
+  DecTree t;
+  const uint32_t* w;
+  uint64_t nbits, n;
+  uint32_t S, nseg, blk0, nblk;
+  uint64_t* start[2];
+  uint64_t* ex[2];
+  uint32_t* cnt;
+  uint64_t* off;
+  uint64_t* bsum;
+};
+template <class Sink>
+struct MDecJobS {
+  MDecJob j;
+  Sink sink;
+};
+__global__ void k_dec_spec_multi(const MDecJob* jobs, const uint32_t* blk0, int nj, int cur, int first, int* err);
+__global__ void k_dec_check_multi(const MDecJob* jobs, const uint32_t* blk0, int nj, int cur, int* changed);
+__global__ void k_dec_bsum_multi(const MDecJob* jobs, const uint32_t* blk0, int nj);
+__global__ void k_dec_bscan_multi(const MDecJob* jobs, uint64_t* totals);
+__global__ void k_dec_off_multi(const MDecJob* jobs, const uint32_t* blk0, int nj);
+
 // Phase 3: decode every segment into its final position. The symbols of a
 // block's 256 consecutive segments are contiguous in the output: they are
 // gathered in LDS and written out by consecutive threads (coalesced), instead
@@ -355,20 +382,18 @@ struct DecStage {
 };
 
 template <class Sink>
-__global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
-                                                   uint32_t S, uint32_t nseg, const uint64_t* __restrict__ start,
-                                                   const uint64_t* __restrict__ exitp,
-                                                   const uint64_t* __restrict__ off, uint64_t n,
-                                                   Sink sink) {
-  __shared__ uint32_t slut[1 << kLutBits];
-  __shared__ uint32_t sw[kStageWords];
+__device__ __forceinline__ void dec_write_block(const DecTree& t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                uint32_t S, uint32_t nseg, const uint64_t* __restrict__ start,
+                                                const uint64_t* __restrict__ exitp,
+                                                const uint64_t* __restrict__ off, uint64_t n, const Sink& sink,
+                                                uint32_t blk, uint32_t* slut, uint32_t* sw,
+                                                typename DecStage<Sink::kBytes>::T* stage) {
   using Stage = DecStage<Sink::kBytes>;
-  __shared__ typename Stage::T stage[Stage::kCap];
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
   uint64_t lbase;
-  const uint32_t lcount = stage_words(sw, w, nbits, S, blockIdx.x * blockDim.x, lbase);
+  const uint32_t lcount = stage_words(sw, w, nbits, S, blk * blockDim.x, lbase);
   __syncthreads();
-  const uint32_t b0 = blockIdx.x * blockDim.x;
+  const uint32_t b0 = blk * blockDim.x;
   const uint32_t b1 = min(b0 + blockDim.x, nseg);  // off[] holds nseg + 1 entries
   const uint64_t o_begin = min(off[b0], n), o_end = min(off[b1], n);
   const bool staged = o_end - o_begin <= (uint64_t)Stage::kCap;
@@ -388,6 +413,35 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
   __syncthreads();
   const uint32_t cnt = (uint32_t)(o_end - o_begin);
   for (uint32_t x = threadIdx.x; x < cnt; x += blockDim.x) sink.put(o_begin + x, stage[x]);
+}
+
+template <class Sink>
+__global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                                   uint32_t S, uint32_t nseg, const uint64_t* __restrict__ start,
+                                                   const uint64_t* __restrict__ exitp,
+                                                   const uint64_t* __restrict__ off, uint64_t n,
+                                                   Sink sink) {
+  __shared__ uint32_t slut[1 << kLutBits];
+  __shared__ uint32_t sw[kStageWords];
+  __shared__ typename DecStage<Sink::kBytes>::T stage[DecStage<Sink::kBytes>::kCap];
+  dec_write_block(t, w, nbits, S, nseg, start, exitp, off, n, sink, blockIdx.x, slut, sw, stage);
+}
+
+template <class Sink>
+__global__ void __launch_bounds__(256) k_dec_write_multi(const MDecJobS<Sink>* __restrict__ jobs,
+                                                         const uint32_t* __restrict__ blk0, int nj, int cur) {
+  __shared__ uint32_t slut[1 << kLutBits];
+  __shared__ uint32_t sw[kStageWords];
+  __shared__ typename DecStage<Sink::kBytes>::T stage[DecStage<Sink::kBytes>::kCap];
+  int lo = 0, hi = nj - 1;  // the job owning this block
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const MDecJob& J = jobs[lo].j;
+  dec_write_block(J.t, J.w, J.nbits, J.S, J.nseg, J.start[cur], J.ex[cur], J.off, J.n, jobs[lo].sink,
+                  blockIdx.x - J.blk0, slut, sw, stage);
 }
 
 template <class Sink>
@@ -424,6 +478,21 @@ void prepare_decoder(const uint8_t* tree, size_t tree_size, size_t node_count, H
 DeviceDecoder upload_decoder(Ctx& ctx, const HostDecoder& h, const char* tag);
 DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, size_t node_count,
                              const char* tag);
+
+// Many decoders in one packed copy and one level-2 fill launch (out[k] for
+// hd[k]); `tag` names the buffers.
+void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecoder* out, const char* tag);
+
+// Bitstreams at arbitrary byte offsets copied into aligned word arrays with
+// zero slack (the decoder's input form), all in one launch. Fills dst/nwords.
+struct BitsCopy {
+  const uint8_t* src;
+  uint64_t nbytes;
+  uint32_t* dst;
+  uint64_t nwords;
+};
+constexpr uint32_t kStageBlockWords = 2048;
+void stage_bits_multi(Ctx& ctx, BitsCopy* c, int n, const char* tag);
 
 // One bitstream to decode: n symbols of words already copied to aligned
 // device words (padded with at least 8 zero bytes) into `sink`; `tag` names
@@ -546,6 +615,127 @@ void decode_streams(Ctx& ctx, DecJob<Sink>* jobs, int nj, const std::function<vo
                                           s.off, j.n, j.sink);
     CPSZ_HIP(hipGetLastError());
   }
+}
+
+// Decodes many independent bitstreams with every stream's segments in one grid
+// per pass: one launch per pass for all streams, one host check per round
+// (the streams whose segments have settled pass through unchanged), a
+// three-launch scan of all streams' counts and one write launch. `tag` names
+// the buffers (one arena for all streams).
+template <class Sink>
+void decode_streams_multi(Ctx& ctx, const DecJob<Sink>* jobs, int nj, const char* tag) {
+  hipStream_t st = ctx.stream();
+  std::vector<int> act;
+  for (int k = 0; k < nj; k++) {
+    const DecJob<Sink>& j = jobs[k];
+    if (j.n == 0) continue;
+    if (j.dd->single) {
+      k_dec_const<<<1024, 256, 0, st>>>(j.n, j.dd->single_sym, j.sink);
+      CPSZ_HIP(hipGetLastError());
+      continue;
+    }
+    CPSZ_REQUIRE(j.nbytes > 0, kErrStream, "empty huffman bitstream for a multi-symbol tree");
+    act.push_back(k);
+  }
+  const int na = (int)act.size();
+  if (!na) return;
+  std::vector<MDecJobS<Sink>> h(na);
+  std::vector<uint32_t> blk0(na + 1, 0);
+  size_t arena = 0;
+  auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  for (int a = 0; a < na; a++) {
+    const DecJob<Sink>& j = jobs[act[a]];
+    MDecJob& m = h[a].j;
+    m = MDecJob{};
+    m.t = j.dd->tree;
+    m.w = j.words;
+    m.nbits = j.nbytes * 8;
+    m.n = j.n;
+    m.S = 256;
+    while (div_up(m.nbits, (uint64_t)m.S) > (1u << 21)) m.S *= 2;
+    m.nseg = (uint32_t)div_up(m.nbits, (uint64_t)m.S);
+    m.nblk = (uint32_t)div_up(m.nseg, 256u);
+    m.blk0 = blk0[a];
+    CPSZ_REQUIRE((uint64_t)blk0[a] + m.nblk < (1ull << 31), kErrUnsupported, "huffman decode: too many segments");
+    blk0[a + 1] = blk0[a] + m.nblk;
+    h[a].sink = j.sink;
+    arena += 4 * al(8 * (size_t)m.nseg) + al(4 * (size_t)m.nseg) + al(8 * ((size_t)m.nseg + 1)) +
+             al(8 * (size_t)m.nblk);
+  }
+  const uint32_t nblocks = blk0[na];
+  uint8_t* d = ctx.buf<uint8_t>((std::string(tag) + ".arena").c_str(), arena);
+  for (int a = 0; a < na; a++) {
+    MDecJob& m = h[a].j;
+    auto take = [&](size_t bytes) {
+      uint8_t* q = d;
+      d += al(bytes);
+      return q;
+    };
+    m.start[0] = (uint64_t*)take(8 * (size_t)m.nseg);
+    m.start[1] = (uint64_t*)take(8 * (size_t)m.nseg);
+    m.ex[0] = (uint64_t*)take(8 * (size_t)m.nseg);
+    m.ex[1] = (uint64_t*)take(8 * (size_t)m.nseg);
+    m.cnt = (uint32_t*)take(4 * (size_t)m.nseg);
+    m.off = (uint64_t*)take(8 * ((size_t)m.nseg + 1));
+    m.bsum = (uint64_t*)take(8 * (size_t)m.nblk);
+  }
+  // job table, block starts and flags travel in one copy each way
+  const size_t tb = al(na * sizeof(MDecJobS<Sink>)), bb = al((na + 1) * 4), fb = al(2 * na * 4), ob = na * 8;
+  const std::string t(tag);
+  uint8_t* hh = (uint8_t*)ctx.host_buf((t + ".tab").c_str(), tb + bb + fb + ob);
+  std::memcpy(hh, h.data(), na * sizeof(MDecJobS<Sink>));
+  std::memcpy(hh + tb, blk0.data(), (na + 1) * 4);
+  uint8_t* dt = ctx.buf<uint8_t>((t + ".tab").c_str(), tb + bb + fb + ob);
+  CPSZ_HIP(hipMemcpyAsync(dt, hh, tb + bb, hipMemcpyHostToDevice, st));
+  const MDecJobS<Sink>* djs = (const MDecJobS<Sink>*)dt;
+  // MDecJobS<Sink> starts with its MDecJob, so a stride-aware view is needed
+  // for the Sink-agnostic kernels: they take an MDecJob table of their own
+  std::vector<MDecJob> hj(na);
+  for (int a = 0; a < na; a++) hj[a] = h[a].j;
+  MDecJob* dj = ctx.buf<MDecJob>((t + ".jobs").c_str(), na);
+  uint8_t* hjb = (uint8_t*)ctx.host_buf((t + ".jobs").c_str(), na * sizeof(MDecJob));
+  std::memcpy(hjb, hj.data(), na * sizeof(MDecJob));
+  CPSZ_HIP(hipMemcpyAsync(dj, hjb, na * sizeof(MDecJob), hipMemcpyHostToDevice, st));
+  const uint32_t* dblk0 = (const uint32_t*)(dt + tb);
+  int* changed = (int*)(dt + tb + bb);
+  int* err = changed + na;
+  uint64_t* totals = (uint64_t*)(dt + tb + bb + fb);
+  int* hflags = (int*)(hh + tb + bb);
+  uint64_t* htot = (uint64_t*)(hh + tb + bb + fb);
+  CPSZ_HIP(hipMemsetAsync(err, 0, na * 4, st));
+  k_dec_spec_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na, 0, 1, err);
+  CPSZ_HIP(hipGetLastError());
+  int cur = 0;
+  bool settled = false;
+  for (int round = 0; round < 4096 && !settled; round++) {
+    for (int r = 0; r < kFixPerCheck; r++) {
+      k_dec_spec_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na, cur, 0, err);
+      cur ^= 1;
+    }
+    CPSZ_HIP(hipMemsetAsync(changed, 0, na * 4, st));
+    k_dec_check_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na, cur, changed);
+    CPSZ_HIP(hipGetLastError());
+    CPSZ_HIP(hipMemcpyAsync(hflags, changed, 2 * na * 4, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipStreamSynchronize(st));
+    settled = true;
+    for (int a = 0; a < na; a++) {
+      CPSZ_REQUIRE(hflags[na + a] == 0, kErrStream, "invalid huffman code in stream");
+      if (hflags[a]) settled = false;
+    }
+  }
+  // a stream whose segment starts have not settled within the round limit
+  // would decode from inconsistent starts: refuse it
+  CPSZ_REQUIRE(settled, kErrStream, "huffman decode did not converge (bitstream does not resynchronize)");
+  k_dec_bsum_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na);
+  k_dec_bscan_multi<<<na, 256, 0, st>>>(dj, totals);
+  k_dec_off_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na);
+  CPSZ_HIP(hipGetLastError());
+  CPSZ_HIP(hipMemcpyAsync(htot, totals, na * 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  for (int a = 0; a < na; a++)
+    CPSZ_REQUIRE(htot[a] >= h[a].j.n, kErrStream, "huffman bitstream holds fewer symbols than expected");
+  k_dec_write_multi<Sink><<<nblocks, 256, 0, st>>>(djs, dblk0, na, cur);
+  CPSZ_HIP(hipGetLastError());
 }
 
 // Decodes n symbols of one bitstream (see decode_streams).
