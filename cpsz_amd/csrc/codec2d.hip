@@ -480,53 +480,85 @@ struct SinkCodes2d {
   }
 };
 
-// unpredictable values in block-major order
-__global__ void k_block_zeros(const uint16_t* __restrict__ q, int r1, int r2, int nby, size_t nb,
-                              uint32_t* __restrict__ cnt) {
-  const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  if (b >= nb) return;
+// Unpredictable values in block-major order. One wave per 12x12 block (4
+// blocks per 256-thread workgroup); its lanes take the block's elements in
+// row-major order, 64 at a time, so a row's 12 codes are read by consecutive
+// lanes, and the ranks of the zero codes come from a ballot.
+constexpr int kUnpWaves = 4;
+inline int unp_grid(size_t nb) { return (int)div_up(nb, (size_t)kUnpWaves); }
+
+struct BlockView {
+  int i0, j0, sy, m;
+  __device__ BlockView(size_t b, int r1, int r2, int nby) {
+    const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+    i0 = bi * kBlock2d;
+    j0 = bj * kBlock2d;
+    const int sx = min(kBlock2d, r1 - i0);
+    sy = min(kBlock2d, r2 - j0);
+    m = sx * sy;
+  }
+  __device__ size_t elem(int k, int r2) const {
+    const int ii = k / sy, jj = k - ii * sy;
+    return (size_t)(i0 + ii) * r2 + j0 + jj;
+  }
+};
+
+__global__ void __launch_bounds__(256) k_block_zeros(const uint16_t* __restrict__ q, int r1, int r2, int nby,
+                                                     size_t nb, uint32_t* __restrict__ cnt) {
+  const size_t b = (size_t)blockIdx.x * kUnpWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= nb) return;  // uniform across the wave
+  const BlockView v(b, r1, r2, nby);
   uint32_t z = 0;
-  SrcCodes2d src{q, r1, r2, nby};
-  src.each(b, [&](uint32_t s) { z += (s == 0); });
-  cnt[b] = z;
+  for (int k0 = 0; k0 < v.m; k0 += 64) {
+    const int k = k0 + lane;
+    const bool zero = k < v.m && q[v.elem(k, r2)] == 0;
+    z += (uint32_t)__popcll(__ballot(zero));
+  }
+  if (lane == 0) cnt[b] = z;
 }
 
-__global__ void k_gather_unpred(const uint16_t* __restrict__ q, const float* __restrict__ x, int r1,
-                                int r2, int nby, size_t nb, const uint64_t* __restrict__ off,
-                                float* __restrict__ out) {
-  const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_gather_unpred(const uint16_t* __restrict__ q, const float* __restrict__ x,
+                                                       int r1, int r2, int nby, size_t nb,
+                                                       const uint64_t* __restrict__ off, float* __restrict__ out) {
+  const size_t b = (size_t)blockIdx.x * kUnpWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= nb) return;
-  const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
-  const int i0 = bi * kBlock2d, j0 = bj * kBlock2d;
-  const int sx = min(kBlock2d, r1 - i0), sy = min(kBlock2d, r2 - j0);
+  const BlockView v(b, r1, r2, nby);
   uint64_t o = off[b];
-  for (int ii = 0; ii < sx; ii++)
-    for (int jj = 0; jj < sy; jj++) {
-      const size_t e = (size_t)(i0 + ii) * r2 + j0 + jj;
-      if (q[e] == 0) out[o++] = x[e];
-    }
+  const uint64_t below = (1ull << lane) - 1;
+  for (int k0 = 0; k0 < v.m; k0 += 64) {
+    const int k = k0 + lane;
+    const size_t e = k < v.m ? v.elem(k, r2) : 0;
+    const bool zero = k < v.m && q[e] == 0;
+    const uint64_t mask = __ballot(zero);
+    if (zero) out[o + __popcll(mask & below)] = x[e];
+    o += __popcll(mask);
+  }
 }
 
-__global__ void k_scatter_unpred(const uint16_t* __restrict__ q, const float* __restrict__ vals,
-                                 int r1, int r2, int nby, size_t nb, const uint64_t* __restrict__ off,
-                                 uint64_t count, float* __restrict__ raw, int* __restrict__ err) {
-  const size_t b = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_scatter_unpred(const uint16_t* __restrict__ q, const float* __restrict__ vals,
+                                                        int r1, int r2, int nby, size_t nb,
+                                                        const uint64_t* __restrict__ off, uint64_t count,
+                                                        float* __restrict__ raw, int* __restrict__ err) {
+  const size_t b = (size_t)blockIdx.x * kUnpWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
   if (b >= nb) return;
-  const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
-  const int i0 = bi * kBlock2d, j0 = bj * kBlock2d;
-  const int sx = min(kBlock2d, r1 - i0), sy = min(kBlock2d, r2 - j0);
+  const BlockView v(b, r1, r2, nby);
   uint64_t o = off[b];
-  for (int ii = 0; ii < sx; ii++)
-    for (int jj = 0; jj < sy; jj++) {
-      const size_t e = (size_t)(i0 + ii) * r2 + j0 + jj;
-      if (q[e] == 0) {
-        if (o >= count) {
-          *err = 1;
-          return;
-        }
-        raw[e] = vals[o++];
-      }
+  const uint64_t below = (1ull << lane) - 1;
+  for (int k0 = 0; k0 < v.m; k0 += 64) {
+    const int k = k0 + lane;
+    const size_t e = k < v.m ? v.elem(k, r2) : 0;
+    const bool zero = k < v.m && q[e] == 0;
+    const uint64_t mask = __ballot(zero);
+    if (zero) {
+      const uint64_t r = o + __popcll(mask & below);
+      if (r < count) raw[e] = vals[r];
+      else *err = 1;
     }
+    o += __popcll(mask);
+  }
 }
 
 struct U16Hist {
@@ -713,8 +745,8 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   uint32_t* zc = ctx.buf<uint32_t>("c2.zc", 2 * nb);
   uint64_t* zoff = ctx.buf<uint64_t>("c2.zoff", 2 * nb + 4);
   uint64_t* totals = ctx.buf<uint64_t>("c2.totals", 8);
-  k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, (int)r1, (int)r2, nby, nb, zc);
-  k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, (int)r1, (int)r2, nby, nb, zc + nb);
+  k_block_zeros<<<unp_grid(nb), 256, 0, st>>>(qu, (int)r1, (int)r2, nby, nb, zc);
+  k_block_zeros<<<unp_grid(nb), 256, 0, st>>>(qv, (int)r1, (int)r2, nby, nb, zc + nb);
   CPSZ_HIP(hipGetLastError());
   scan_u32_to_u64(ctx, zc, zoff, nb, totals + 3);
   scan_u32_to_u64(ctx, zc + nb, zoff + nb, nb, totals + 4);
@@ -770,8 +802,8 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   k_encode<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
                                                    offs + nch_ids + nb, bits + nch_ids + nb, words + w_eb + w_u);
   float* unp = ctx.buf<float>("c2.unp", unp_u + unp_v + 1);
-  k_gather_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qu, dU, (int)r1, (int)r2, nby, nb, zoff, unp);
-  k_gather_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qv, dV, (int)r1, (int)r2, nby, nb, zoff + nb,
+  k_gather_unpred<<<unp_grid(nb), 256, 0, st>>>(qu, dU, (int)r1, (int)r2, nby, nb, zoff, unp);
+  k_gather_unpred<<<unp_grid(nb), 256, 0, st>>>(qv, dV, (int)r1, (int)r2, nby, nb, zoff + nb,
                                                         unp + unp_u);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
@@ -980,9 +1012,9 @@ void d2_unpred(Ctx& ctx, const uint8_t* d_in, const D2Plan& p, D2Work& w) {
     float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", p.comp[c].unp_count + 1);
     if (p.comp[c].unp_count)
       CPSZ_HIP(hipMemcpyAsync(vals, d_in + p.comp[c].unp_off, 4 * p.comp[c].unp_count, hipMemcpyDeviceToDevice, st));
-    k_block_zeros<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], (int)r1, (int)r2, p.nby, nb, zc + c * nb);
+    k_block_zeros<<<unp_grid(nb), 256, 0, st>>>(qs[c], (int)r1, (int)r2, p.nby, nb, zc + c * nb);
     scan_u32_to_u64(ctx, zc + c * nb, zoff + c * (nb + 1), nb, zoff + c * (nb + 1) + nb);
-    k_scatter_unpred<<<(int)div_up(nb, 256), 256, 0, st>>>(qs[c], vals, (int)r1, (int)r2, p.nby, nb,
+    k_scatter_unpred<<<unp_grid(nb), 256, 0, st>>>(qs[c], vals, (int)r1, (int)r2, p.nby, nb,
                                                            zoff + c * (nb + 1), p.comp[c].unp_count,
                                                            w.raw + c * n, w.err);
     CPSZ_HIP(hipGetLastError());
@@ -1103,6 +1135,7 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
   for (size_t k = 0; k < 3 * nf; k++) hp[k] = &hd[k];
   std::vector<DeviceDecoder> dd(3 * nf);
   upload_decoders(ctx, hp.data(), (int)(3 * nf), dd.data(), "b2d.dec");
+  ctx.mark("upload");
   std::vector<BitsCopy> bc(3 * nf);
   for (size_t k = 0; k < nf; k++) {
     const D2Plan& p = plan[k];
@@ -1111,6 +1144,7 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
       bc[3 * k + 1 + c] = BitsCopy{d_in[k] + p.comp[c].h.bits_off, p.comp[c].h.nbytes, nullptr, 0};
   }
   stage_bits_multi(ctx, bc.data(), (int)(3 * nf), "b2d.bits");
+  ctx.mark("stage");
   std::vector<DecJob<SinkIds2d>> ji(nf);
   std::vector<DecJob<SinkCodes2d>> jc(2 * nf);
   for (size_t k = 0; k < nf; k++) {

@@ -126,6 +126,152 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
   dec_spec_block(t, w, nbits, S, nseg, prev_exit, prev_start, start, exitp, cnt, err, first, blockIdx.x, slut, sw);
 }
 
+// ---- block-settled passes (the multi-stream decoder). A pass leaves every
+// block internally consistent: each segment but the block's first starts at
+// its predecessor's exit. So only the links between blocks need host rounds.
+
+// Decodes from s0 along the new path and along the old path of the segment
+// (start os, exit oe, count oc) in lockstep, always advancing the one behind,
+// until the two meet at a code boundary (from there they coincide, so the old
+// exit and the rest of its count carry over) or the new path reaches `end`.
+__device__ __forceinline__ void dec_resync(const DecTree& t, const uint32_t* slut, const uint32_t* __restrict__ w,
+                                           const uint32_t* sw, uint64_t lbase, uint32_t lcount, uint64_t end,
+                                           uint64_t s0, uint64_t os, uint64_t oe, uint32_t oc, uint64_t& ex,
+                                           uint32_t& c, int* __restrict__ err) {
+  BitReader br, old;
+  br.init(w, s0, sw, lbase, lcount);
+  old.init(w, os, sw, lbase, lcount);
+  uint32_t cn = 0, co = 0;
+  for (;;) {
+    if (br.pos >= end) {
+      ex = br.pos;
+      break;
+    }
+    if (br.pos == old.pos) {
+      ex = oe;
+      cn += oc - co;
+      break;
+    }
+    const bool adv_new = br.pos < old.pos;
+    if (dec_one(t, slut, adv_new ? br : old) == 0xFFFFFFFFu) {
+      *err = 1;
+      ex = br.pos;
+      break;
+    }
+    cn += adv_new;
+    co += !adv_new;
+  }
+  c = cn;
+}
+
+// Settles a block: while some segment does not start at its predecessor's
+// exit, every such segment re-decodes from that exit (in parallel: a serial
+// fix of one segment at a time would leave all but one lane of the wave idle).
+// After iteration k the first k + 1 segments are final (the first one is
+// given), so blockDim iterations suffice; a segment that adopts a still-wrong
+// predecessor exit is corrected in a later iteration.
+// lex: blockDim LDS words.
+__device__ __forceinline__ void dec_settle(const DecTree& t, const uint32_t* slut, const uint32_t* __restrict__ w,
+                                           const uint32_t* sw, uint64_t lbase, uint32_t lcount, uint64_t end,
+                                           bool valid, uint64_t& st, uint64_t& ex, uint32_t& c, uint64_t* lex,
+                                           int* __restrict__ err) {
+  for (uint32_t it = 0; it < blockDim.x; it++) {
+    __syncthreads();
+    lex[threadIdx.x] = ex;
+    __syncthreads();
+    bool redo = false;
+    uint64_t want = 0;
+    if (valid && threadIdx.x > 0) {
+      want = lex[threadIdx.x - 1];
+      redo = want != st;
+    }
+    if (!__syncthreads_or(redo)) break;
+    if (redo) {
+      uint64_t e2;
+      uint32_t c2;
+      dec_resync(t, slut, w, sw, lbase, lcount, end, want, st, ex, c, e2, c2, err);
+      st = want;
+      ex = e2;
+      c = c2;
+    }
+  }
+}
+
+__device__ __forceinline__ void dec_pass_block(const DecTree& t, const uint32_t* __restrict__ w, uint64_t nbits,
+                                               uint32_t S, uint32_t nseg, const uint64_t* __restrict__ prev_exit,
+                                               const uint64_t* __restrict__ prev_start,
+                                               uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
+                                               uint32_t* __restrict__ cnt, int* __restrict__ err, int first,
+                                               uint32_t blk, uint32_t* slut, uint32_t* sw, uint64_t* lex) {
+  const uint32_t i = blk * blockDim.x + threadIdx.x;
+  const bool valid = i < nseg;
+  uint64_t st = 0, ex = 0, s0 = 0;
+  uint32_t c = 0;
+  bool work;
+  if (first) {
+    work = valid;
+  } else {
+    // only a block whose first segment gets a new start changes
+    if (valid) {
+      st = prev_start[i];
+      ex = prev_exit[i];
+      c = cnt[i];
+    }
+    if (threadIdx.x == 0 && valid) s0 = i == 0 ? 0 : prev_exit[i - 1];
+    work = threadIdx.x == 0 && valid && s0 != st;
+    if (!__syncthreads_or(work)) {
+      if (valid) {
+        start[i] = st;
+        exitp[i] = ex;
+      }
+      return;
+    }
+  }
+  for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
+  uint64_t lbase;
+  const uint32_t lcount = stage_words(sw, w, nbits, S, blk * blockDim.x, lbase);
+  __syncthreads();
+  uint64_t end = (uint64_t)(i + 1) * S;
+  if (end > nbits) end = nbits;
+  if (first && valid) {
+    // Warm-up: start kDecWarm bits before the segment and take as its start
+    // the first code boundary at or after i * S. A prefix-code path usually
+    // resyncs within a few code words, so this path is often already the true
+    // one there.
+    const uint64_t a = (uint64_t)i * S;
+    BitReader br;
+    br.init(w, a - (a < kDecWarm ? a : kDecWarm), sw, lbase, lcount);
+    while (br.pos < a) {
+      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
+        *err = 1;
+        break;
+      }
+    }
+    st = br.pos;
+    while (br.pos < end) {
+      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
+        *err = 1;
+        break;
+      }
+      c++;
+    }
+    ex = br.pos;
+  } else if (work) {
+    uint64_t e2;
+    uint32_t c2;
+    dec_resync(t, slut, w, sw, lbase, lcount, end, s0, st, ex, c, e2, c2, err);
+    st = s0;
+    ex = e2;
+    c = c2;
+  }
+  dec_settle(t, slut, w, sw, lbase, lcount, end, valid, st, ex, c, lex, err);
+  if (valid) {
+    start[i] = st;
+    exitp[i] = ex;
+    cnt[i] = c;
+  }
+}
+
 // ---- multi-stream passes: block b belongs to the job j with
 // blk0[j] <= b < blk0[j + 1]; `cur` selects the ping-pong buffers.
 __device__ __forceinline__ int mdec_job(const uint32_t* __restrict__ blk0, int nj, uint32_t b) {
@@ -143,14 +289,15 @@ __global__ void __launch_bounds__(256) k_dec_spec_multi(const MDecJob* __restric
                                                         int first, int* __restrict__ err) {
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
+  __shared__ uint64_t lex[256];
   const int j = mdec_job(blk0, nj, blockIdx.x);
   const MDecJob& J = jobs[j];
   if (first)
-    dec_spec_block(J.t, J.w, J.nbits, J.S, J.nseg, nullptr, nullptr, J.start[0], J.ex[0], J.cnt, err + j, 1,
-                   blockIdx.x - blk0[j], slut, sw);
+    dec_pass_block(J.t, J.w, J.nbits, J.S, J.nseg, nullptr, nullptr, J.start[0], J.ex[0], J.cnt, err + j, 1,
+                   blockIdx.x - blk0[j], slut, sw, lex);
   else
-    dec_spec_block(J.t, J.w, J.nbits, J.S, J.nseg, J.ex[cur], J.start[cur], J.start[cur ^ 1], J.ex[cur ^ 1],
-                   J.cnt, err + j, 0, blockIdx.x - blk0[j], slut, sw);
+    dec_pass_block(J.t, J.w, J.nbits, J.S, J.nseg, J.ex[cur], J.start[cur], J.start[cur ^ 1], J.ex[cur ^ 1],
+                   J.cnt, err + j, 0, blockIdx.x - blk0[j], slut, sw, lex);
 }
 
 // changed[j] = 1 when a segment of job j does not start at its predecessor's exit

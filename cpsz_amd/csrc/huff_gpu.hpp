@@ -705,9 +705,11 @@ void decode_streams_multi(Ctx& ctx, const DecJob<Sink>* jobs, int nj, const char
   CPSZ_HIP(hipMemsetAsync(err, 0, na * 4, st));
   k_dec_spec_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na, 0, 1, err);
   CPSZ_HIP(hipGetLastError());
+  ctx.mark((t + ".spec").c_str());
   int cur = 0;
   bool settled = false;
-  for (int round = 0; round < 4096 && !settled; round++) {
+  int rounds = 0;
+  for (int round = 0; round < 4096 && !settled; round++, rounds++) {
     for (int r = 0; r < kFixPerCheck; r++) {
       k_dec_spec_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na, cur, 0, err);
       cur ^= 1;
@@ -726,6 +728,7 @@ void decode_streams_multi(Ctx& ctx, const DecJob<Sink>* jobs, int nj, const char
   // a stream whose segment starts have not settled within the round limit
   // would decode from inconsistent starts: refuse it
   CPSZ_REQUIRE(settled, kErrStream, "huffman decode did not converge (bitstream does not resynchronize)");
+  ctx.mark((t + ".fix" + std::to_string(rounds)).c_str());
   k_dec_bsum_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na);
   k_dec_bscan_multi<<<na, 256, 0, st>>>(dj, totals);
   k_dec_off_multi<<<nblocks, 256, 0, st>>>(dj, dblk0, na);
@@ -734,8 +737,10 @@ void decode_streams_multi(Ctx& ctx, const DecJob<Sink>* jobs, int nj, const char
   CPSZ_HIP(hipStreamSynchronize(st));
   for (int a = 0; a < na; a++)
     CPSZ_REQUIRE(htot[a] >= h[a].j.n, kErrStream, "huffman bitstream holds fewer symbols than expected");
+  ctx.mark((t + ".scan").c_str());
   k_dec_write_multi<Sink><<<nblocks, 256, 0, st>>>(djs, dblk0, na, cur);
   CPSZ_HIP(hipGetLastError());
+  ctx.mark((t + ".write").c_str());
 }
 
 // Decodes n symbols of one bitstream (see decode_streams).
