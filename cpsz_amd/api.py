@@ -35,6 +35,7 @@ _SIGS = {
     "cpsz_lossless_compress": [C.c_int, _P, _SZ, C.POINTER(_P), C.POINTER(_SZ)],
     "cpsz_lossless_decompress": [_P, _SZ, _P, _SZ],
     "cpsz_critical_points_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_critical_points_2d_noalloc_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
     "cpsz_critical_points_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ,
                                     C.POINTER(_SZ)],
     "cpsz_set_timing": [C.c_int, C.c_int],

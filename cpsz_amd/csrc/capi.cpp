@@ -326,6 +326,13 @@ int cpsz_critical_points_2d_dev(int device, void* stream, const float* dU, const
   });
 }
 
+int cpsz_critical_points_2d_noalloc_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                        size_t r2, int64_t* ids, int32_t* types, size_t cap, size_t* count) {
+  return guarded([&] {
+    critical_points_2d_noalloc(device, (hipStream_t)stream, dU, dV, r1, r2, ids, types, cap, count);
+  });
+}
+
 int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const float* dV,
                                 const float* dW, size_t r1, size_t r2, size_t r3, int64_t* ids,
                                 int32_t* types, size_t cap, size_t* count) {

@@ -17,6 +17,7 @@
 // root plus a complex pair, Descartes' rule of signs for three real roots.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <numeric>
 #include <vector>
 
@@ -118,7 +119,73 @@ CpOut make_out(Ctx& ctx, size_t cap_dev, unsigned long long*& dcount) {
   return o;
 }
 
+// ---- the no-allocation form of the 2D judge ---------------------------------
+// Its scratch (the counter and the unsorted (id, type) list) is this
+// file-scope device array: every GPU the library runs on holds its own copy
+// (12 MiB + 256 B) until the process ends. Host code reaches a device's copy
+// only through the address hipGetSymbolAddress returns on that device.
+__device__ __align__(256) uint8_t cp_scratch[256 + 12 * (size_t)kNoAllocMaxCp];
+
+__global__ void k_zero_count(unsigned long long* c) { *c = 0; }
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_m;
+uint8_t* g_scratch[kMaxDevices] = {};
+
+// Address of this device's scratch (looked up once per device, with it current).
+uint8_t* scratch_of(int device) {
+  CPSZ_REQUIRE(device >= 0 && device < kMaxDevices, kErrArg, "device index out of range");
+  std::lock_guard<std::mutex> g(g_scratch_m);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    CPSZ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(cp_scratch)));
+    CPSZ_REQUIRE(p, kErrHip, "no device address for the critical-point scratch");
+    g_scratch[device] = (uint8_t*)p;
+  }
+  return g_scratch[device];
+}
+
 }  // namespace
+
+void critical_points_2d_noalloc(int device, hipStream_t st, const float* dU, const float* dV, size_t r1, size_t r2,
+                                int64_t* ids, int32_t* types, size_t cap, size_t* count) {
+  CPSZ_REQUIRE(dU && dV && ids && types && count, kErrArg, "critical_points_2d_noalloc: bad arguments");
+  CPSZ_REQUIRE(r1 < (1u << 28) && r2 < (1u << 28), kErrUnsupported,
+               "critical_points_2d_noalloc: field dimensions must stay below 2^28");
+  int ndev = 0;
+  CPSZ_HIP(hipGetDeviceCount(&ndev));
+  CPSZ_REQUIRE(device >= 0 && device < ndev, kErrArg, "device index out of range");
+  CPSZ_HIP(hipSetDevice(device));
+  uint8_t* base = scratch_of(device);
+  CpOut o;
+  o.count = (unsigned long long*)base;
+  o.ids = (long long*)(base + 256);
+  o.types = (int*)(base + 256 + 8 * (size_t)kNoAllocMaxCp);
+  o.cap = kNoAllocMaxCp;
+  k_zero_count<<<1, 1, 0, st>>>(o.count);
+  const size_t cells = (r1 > 3 && r2 > 3) ? (r1 - 3) * (r2 - 3) : 0;
+  if (cells) k_cp2d<<<(int)div_up(cells, 256), 256, 0, st>>>(dU, dV, (int)r1, (int)r2, o);
+  CPSZ_HIP(hipGetLastError());
+  unsigned long long n = 0;
+  CPSZ_HIP(hipMemcpyAsync(&n, o.count, 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  *count = (size_t)n;
+  CPSZ_REQUIRE(n <= (unsigned long long)kNoAllocMaxCp, kErrUnsupported,
+               "critical_points_2d_noalloc: more critical points than its scratch holds (2^20)");
+  CPSZ_REQUIRE(n <= cap, kErrArg, "critical_points_2d_noalloc: output arrays hold fewer entries than *count");
+  if (!n) return;
+  // into the caller's arrays, then sorted there by id: the type rides in the
+  // low 4 bits of the id word, so one in-place sort of one array suffices
+  CPSZ_HIP(hipMemcpyAsync(ids, o.ids, 8 * n, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipMemcpyAsync(types, o.types, 4 * n, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  for (size_t k = 0; k < n; k++) ids[k] = ids[k] << 4 | (int64_t)(types[k] & 15);
+  std::sort(ids, ids + n);
+  for (size_t k = 0; k < n; k++) {
+    types[k] = (int32_t)(ids[k] & 15);
+    ids[k] >>= 4;
+  }
+}
 
 void critical_points_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
                         int64_t* ids, int32_t* types, size_t cap, size_t* count) {

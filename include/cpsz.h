@@ -116,6 +116,19 @@ int cpsz_lossless_decompress_prefix(const unsigned char* in, size_t n, unsigned 
 int cpsz_critical_points_2d_dev(int device, void* stream, const float* dU, const float* dV,
                                 size_t r1, size_t r2, int64_t* ids, int32_t* types, size_t cap,
                                 size_t* count);
+/* The 2D critical-point judge without allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same critical points as cpsz_critical_points_2d_dev, but it calls no
+ * hipMalloc/hipFree and creates no context: its scratch is a fixed device
+ * array of 12 MiB + 256 B per GPU (held until the process ends), shared by
+ * every call on that GPU, so calls that overlap on one GPU race. `stream` may
+ * be NULL (the null stream). Limits: at most 2^20 critical points (else
+ * CPSZ_ERR_UNSUPPORTED), r1, r2 < 2^28; ids/types are host arrays of `cap`
+ * entries and cap must be >= the count (else CPSZ_ERR_ARG, *count set). */
+int cpsz_critical_points_2d_noalloc_dev(int device, void* stream, const float* dU, const float* dV,
+                                        size_t r1, size_t r2, int64_t* ids, int32_t* types, size_t cap,
+                                        size_t* count);
 int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const float* dV,
                                 const float* dW, size_t r1, size_t r2, size_t r3, int64_t* ids,
                                 int32_t* types, size_t cap, size_t* count);
