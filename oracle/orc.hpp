@@ -84,6 +84,13 @@ bool decompress_2d_offline(const uint8_t* s, size_t len, size_t r1, size_t r2,
                            std::vector<T>& U, std::vector<T>& V);
 
 // Stage functions exposed for tests.
+// 2D offline_log pair (src/sz_compress_cp_preserve_2d.cpp:285-374,
+// src/sz_decompress_cp_preserve_2d.cpp:54-99).
+template <typename T>
+std::vector<uint8_t> compress_2d_offline_log(const T* U, const T* V, size_t r1, size_t r2, double max_pwr_eb);
+template <typename T>
+bool decompress_2d_offline_log(const uint8_t* s, size_t len, size_t r1, size_t r2, std::vector<T>& U,
+                               std::vector<T>& V);
 template <typename T>
 void eb_2d_offline(const T* U, const T* V, size_t r1, size_t r2, double max_pwr_eb,
                    double* eb);

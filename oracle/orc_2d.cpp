@@ -413,6 +413,97 @@ bool decompress_2d_offline(const uint8_t* s, size_t len, size_t r1, size_t r2, s
   return true;
 }
 
+// ---- offline_log: bounds in the log domain -------------------------------
+// sz_compress_cp_preserve_2d_offline_log (src/sz_compress_cp_preserve_2d.cpp:285-374):
+// the offline vertex bound eb, then eb := log2(1 + eb) quantized base 2 over
+// DBL_EPSILON (one id per vertex, shared by u and v), sign maps of u and v,
+// and the Lorenzo codec run on log2f|x| with that bound.
+template <typename T>
+std::vector<uint8_t> compress_2d_offline_log(const T* U, const T* V, size_t r1, size_t r2, double max_pwr_eb) {
+  const size_t n = r1 * r2;
+  std::vector<double> eb(n);
+  eb_2d_offline(U, V, r1, r2, max_pwr_eb, eb.data());
+  const size_t smap = (n - 1) / 8 + 1;
+  std::vector<uint8_t> signs(2 * smap, 0);
+  std::vector<T> logu(n), logv(n);
+  {
+    std::vector<uint8_t> sign(n);
+    const T* src[2] = {U, V};
+    T* dst[2] = {logu.data(), logv.data()};
+    for (int c = 0; c < 2; c++) {
+      log_transform_3d(src[c], n, dst[c], sign.data());  // log_transform (sz_cp_preserve_utils.hpp:205-221)
+      // convertIntArray2ByteArray_fast_1b_to_result_sz (src/sz_compression_utils.cpp:31-52)
+      for (size_t i = 0; i < n; i++)
+        if (sign[i] == 1) signs[c * smap + i / 8] |= (uint8_t)(1u << (7 - (i % 8)));
+    }
+  }
+  const double threshold = std::numeric_limits<double>::epsilon();
+  std::vector<int> ids(n);
+  for (size_t i = 0; i < n; i++) {
+    eb[i] = std::log2(1 + eb[i]);
+    ids[i] = eb_exp_quantize(eb[i], 2, std::log2(2.0), threshold);
+  }
+  std::vector<uint8_t> sec_eb;
+  huff_encode_section(2 * 1024, ids.data(), n, sec_eb);
+  std::vector<uint8_t> sec_u = lorenzo_2d_with_eb<T>(logu.data(), eb.data(), r1, r2, nullptr, nullptr);
+  std::vector<uint8_t> sec_v = lorenzo_2d_with_eb<T>(logv.data(), eb.data(), r1, r2, nullptr, nullptr);
+  std::vector<uint8_t> out;
+  put(out, (int)2);
+  put(out, threshold);
+  put(out, (size_t)sec_eb.size());
+  put(out, (size_t)sec_u.size());
+  put(out, (size_t)sec_v.size());
+  out.insert(out.end(), sec_eb.begin(), sec_eb.end());
+  out.insert(out.end(), signs.begin(), signs.end());
+  out.insert(out.end(), sec_u.begin(), sec_u.end());
+  out.insert(out.end(), sec_v.begin(), sec_v.end());
+  return out;
+}
+
+// sz_decompress_cp_preserve_2d_offline_log (src/sz_decompress_cp_preserve_2d.cpp:54-99):
+// id 0 -> bound 0, else base^id * threshold; the log values decode with that
+// bound; x = 0 below -99, else +-exp2 by the sign map.
+template <typename T>
+bool decompress_2d_offline_log(const uint8_t* s, size_t len, size_t r1, size_t r2, std::vector<T>& U,
+                               std::vector<T>& V) {
+  const uint8_t* p = s;
+  const uint8_t* end = s + len;
+  int base;
+  double threshold;
+  size_t eb_size, u_size, v_size;
+  if (!get(p, end, base) || !get(p, end, threshold) || !get(p, end, eb_size) || !get(p, end, u_size) ||
+      !get(p, end, v_size))
+    return false;
+  const size_t n = r1 * r2;
+  std::vector<int> ids;
+  if (!huff_decode_section(p, end, n, ids)) return false;
+  std::vector<double> eb(n);
+  for (size_t i = 0; i < n; i++) eb[i] = ids[i] == 0 ? 0 : std::pow((double)base, (double)ids[i]) * threshold;
+  const size_t smap = (n - 1) / 8 + 1;
+  if ((size_t)(end - p) < 2 * smap) return false;
+  const uint8_t* signs = p;
+  p += 2 * smap;
+  if (!decode_component<T>(p, end, eb, r1, r2, U)) return false;
+  p += u_size;
+  if (!decode_component<T>(p, end, eb, r1, r2, V)) return false;
+  std::vector<T>* out[2] = {&U, &V};
+  for (int c = 0; c < 2; c++)
+    for (size_t i = 0; i < n; i++) {
+      T& x = (*out[c])[i];
+      const bool pos = (signs[c * smap + i / 8] >> (7 - (i % 8))) & 1;
+      if (x < -99) x = 0;
+      else x = pos ? std::exp2(x) : -std::exp2(x);
+    }
+  return true;
+}
+
+template std::vector<uint8_t> compress_2d_offline_log<float>(const float*, const float*, size_t, size_t, double);
+template std::vector<uint8_t> compress_2d_offline_log<double>(const double*, const double*, size_t, size_t,
+                                                              double);
+template bool decompress_2d_offline_log<float>(const uint8_t*, size_t, size_t, size_t, std::vector<float>&,
+                                               std::vector<float>&);
+template bool decompress_2d_offline_log<double>(const uint8_t*, size_t, size_t, size_t, std::vector<double>&,
+                                                std::vector<double>&);
 template void eb_2d_offline<float>(const float*, const float*, size_t, size_t, double, double*);
 template void eb_2d_offline<double>(const double*, const double*, size_t, size_t, double, double*);
 template std::vector<uint8_t> lorenzo_2d_with_eb<float>(const float*, const double*, size_t, size_t,

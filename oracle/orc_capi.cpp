@@ -44,6 +44,18 @@ int orc_decompress_2d_f64(const uint8_t* s, size_t len, size_t r1, size_t r2, do
   std::memcpy(V, v.data(), v.size() * 8);
   return 0;
 }
+int orc_compress_2d_log_f32(const float* U, const float* V, size_t r1, size_t r2, double eb, uint8_t** out,
+                            size_t* size) {
+  try { return emit(compress_2d_offline_log<float>(U, V, r1, r2, eb), out, size); } catch (...) { return -1; }
+}
+int orc_decompress_2d_log_f32(const uint8_t* s, size_t len, size_t r1, size_t r2, float* U, float* V) {
+  std::vector<float> u, v;
+  if (!decompress_2d_offline_log<float>(s, len, r1, r2, u, v)) return -1;
+  std::memcpy(U, u.data(), 4 * r1 * r2);
+  std::memcpy(V, v.data(), 4 * r1 * r2);
+  return 0;
+}
+
 int orc_eb_2d_f32(const float* U, const float* V, size_t r1, size_t r2, double eb, double* out) {
   eb_2d_offline<float>(U, V, r1, r2, eb, out);
   return 0;

@@ -145,6 +145,26 @@ unsigned char* ref_lorenzo_2d_f32(const float* data, const double* prec, size_t 
   *size = s;
   return out;
 }
+// sz_compress/decompress_cp_preserve_2d_offline_log<float>
+// (src/sz_compress_cp_preserve_2d.cpp:285-374, src/sz_decompress_cp_preserve_2d.cpp:54-99)
+unsigned char* ref_compress_2d_log_f32(const float* U, const float* V, size_t r1, size_t r2, double eb,
+                                       size_t* size) {
+  size_t s = 0;
+  unsigned char* out = sz_compress_cp_preserve_2d_offline_log<float>(U, V, r1, r2, s, false, eb);
+  *size = s;
+  return out;
+}
+int ref_decompress_2d_log_f32(const unsigned char* s, size_t r1, size_t r2, float* U, float* V) {
+  float *u = nullptr, *v = nullptr;
+  sz_decompress_cp_preserve_2d_offline_log<float>(s, r1, r2, u, v);
+  if (!u || !v) return -1;
+  std::memcpy(U, u, 4 * r1 * r2);
+  std::memcpy(V, v, 4 * r1 * r2);
+  std::free(u);
+  std::free(v);
+  return 0;
+}
+
 // sz_lossless_compress / _decompress (src/sz_lossless.cpp:11-45), ZSTD.
 size_t ref_zstd_compress(int level, unsigned char* data, size_t len, unsigned char** out) {
   return sz_lossless_compress(ZSTD_COMPRESSOR, level, data, len, out);

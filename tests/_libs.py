@@ -24,6 +24,8 @@ def load_oracle(path=ORC_PATH):
         "orc_compress_2d_f64": [_P, _P, _SZ, _SZ, C.c_double, pp, C.POINTER(_SZ)],
         "orc_decompress_2d_f32": [_P, _SZ, _SZ, _SZ, _P, _P],
         "orc_decompress_2d_f64": [_P, _SZ, _SZ, _SZ, _P, _P],
+        "orc_compress_2d_log_f32": [_P, _P, _SZ, _SZ, C.c_double, pp, C.POINTER(_SZ)],
+        "orc_decompress_2d_log_f32": [_P, _SZ, _SZ, _SZ, _P, _P],
         "orc_eb_2d_f32": [_P, _P, _SZ, _SZ, C.c_double, _P],
         "orc_eb_quantize_2d": [_P, _SZ, _P, _P],
         "orc_tri_bound_2d": [_P, _SZ, C.c_int, _P],
@@ -84,6 +86,18 @@ class Oracle:
         V = np.empty((r1, r2), dtype)
         f = self.lib.orc_decompress_2d_f32 if dtype == np.float32 else self.lib.orc_decompress_2d_f64
         assert f(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)) == 0
+        return U, V
+
+    def compress_2d_log(self, U, V, eb=0.1):
+        r1, r2 = U.shape
+        p, n = C.c_void_p(), _SZ()
+        assert self.lib.orc_compress_2d_log_f32(_ptr(U), _ptr(V), r1, r2, eb, C.byref(p), C.byref(n)) == 0
+        return _take(self.lib, p, n.value)
+
+    def decompress_2d_log(self, stream, r1, r2):
+        U = np.empty((r1, r2), np.float32)
+        V = np.empty((r1, r2), np.float32)
+        assert self.lib.orc_decompress_2d_log_f32(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)) == 0
         return U, V
 
     def eb_2d(self, U, V, eb=0.1):
@@ -276,6 +290,9 @@ class Reference:
         lib.ref_zstd_compress.restype = _SZ
         lib.ref_zstd_decompress.argtypes = [_P, _SZ, C.POINTER(C.c_void_p), _SZ]
         lib.ref_zstd_decompress.restype = _SZ
+        lib.ref_compress_2d_log_f32.argtypes = [_P, _P, _SZ, _SZ, C.c_double, C.POINTER(_SZ)]
+        lib.ref_compress_2d_log_f32.restype = C.c_void_p
+        lib.ref_decompress_2d_log_f32.argtypes = [_P, _SZ, _SZ, _P, _P]
         lib.ref_is_lossless.argtypes = [_P, _SZ]
         lib.ref_is_lossless.restype = C.c_int
         lib.ref_zstd_decompress65536.argtypes = [_P, _SZ, C.POINTER(C.c_void_p)]
@@ -335,6 +352,18 @@ class Reference:
         n = self.lib.ref_zstd_compress(level, _ptr(data), data.size, C.byref(p))
         return self._take(p, n)
 
+    def compress_2d_log(self, U, V, eb=0.1):
+        r1, r2 = U.shape
+        s = _SZ()
+        p = self.lib.ref_compress_2d_log_f32(_ptr(U), _ptr(V), r1, r2, eb, C.byref(s))
+        return self._take(p, s.value)
+
+    def decompress_2d_log(self, stream, r1, r2):
+        U = np.empty((r1, r2), np.float32)
+        V = np.empty((r1, r2), np.float32)
+        self.lib.ref_decompress_2d_log_f32(_ptr(stream), r1, r2, _ptr(U), _ptr(V))
+        return U, V
+
     def is_lossless(self, data):
         data = np.ascontiguousarray(data, np.uint8)
         return self.lib.ref_is_lossless(_ptr(data), data.size)
@@ -382,6 +411,18 @@ class Reference:
         s = _SZ()
         p = self.lib.ref_lorenzo_2d_f32(_ptr(data), _ptr(prec), r1, r2, C.byref(s))
         return self._take(p, s.value)
+
+
+def mask_2d_log_padding(stream, r1, r2, itemsize=4):
+    """mask_2d_padding for the 2D offline_log layout: the two sign maps of
+    ceil(n/8) bytes sit between the id section and the u section."""
+    s = stream.copy()
+    hdr = 4 + 8 + 8 * 3
+    eb_size, u_size, v_size = np.frombuffer(s[12:36].tobytes(), dtype=np.uint64)
+    smap = (r1 * r2 - 1) // 8 + 1
+    for off in (hdr + int(eb_size) + 2 * smap, hdr + int(eb_size) + 2 * smap + int(u_size)):
+        s[off + 10: off + 9 + itemsize] = 0
+    return s
 
 
 def mask_2d_padding(stream, r1, r2, itemsize=4):

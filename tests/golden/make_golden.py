@@ -15,11 +15,28 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
-from _libs import Reference, load_oracle, mask_2d_padding  # noqa: E402
+from _libs import Reference, load_oracle, mask_2d_log_padding, mask_2d_padding  # noqa: E402
 from cpsz_amd.synth import field_2d, field_3d  # noqa: E402
 
 
+def offline_log():
+    """2D offline_log streams and decoded fields from the compiled reference
+    (src/sz_compress_cp_preserve_2d.cpp:285-374, src/sz_decompress_cp_preserve_2d.cpp:54-99),
+    including exact zeros (log value -100, sign 0)."""
+    ref = Reference()
+    for r1, r2, seed, eb, zeros in [(40, 50, 1, 0.1, False), (77, 63, 2, 0.05, True), (19, 130, 3, 0.2, True)]:
+        U, V = field_2d(r1, r2, seed)
+        if zeros:
+            U[::7, ::5] = 0
+            V[3::11, :] = 0
+        s = mask_2d_log_padding(ref.compress_2d_log(U, V, eb), r1, r2)
+        dU, dV = ref.decompress_2d_log(s, r1, r2)
+        np.savez_compressed(os.path.join(HERE, f"c2l_{r1}x{r2}_s{seed}.npz"), U=U, V=V, eb=np.float64(eb),
+                            stream=s, dU=dU, dV=dV)
+
+
 def main():
+    offline_log()
     ref = Reference()
     orc = load_oracle()
     out = {}
@@ -88,4 +105,7 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "log":
+        offline_log()
+    else:
+        main()

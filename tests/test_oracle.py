@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from _libs import Reference, have_reference, load_oracle, mask_2d_padding
+from _libs import Reference, have_reference, load_oracle, mask_2d_log_padding, mask_2d_padding
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLD = os.path.join(HERE, "golden")
@@ -33,6 +33,20 @@ def test_2d_golden(orc, path):
     s = mask_2d_padding(orc.compress_2d(U, V, float(z["eb"])), r1, r2, item)
     assert np.array_equal(s, z["stream"])
     dU, dV = orc.decompress_2d(z["stream"], r1, r2, U.dtype)
+    assert np.array_equal(_bits(dU), _bits(z["dU"]))
+    assert np.array_equal(_bits(dV), _bits(z["dV"]))
+
+
+@pytest.mark.parametrize("path", sorted(glob.glob(os.path.join(GOLD, "c2l_*.npz"))),
+                         ids=lambda p: os.path.basename(p))
+def test_2d_log_golden(orc, path):
+    """offline_log pair against the compiled reference's fixtures (exact zeros included)."""
+    z = np.load(path)
+    U, V = z["U"], z["V"]
+    r1, r2 = U.shape
+    s = mask_2d_log_padding(orc.compress_2d_log(U, V, float(z["eb"])), r1, r2)
+    assert np.array_equal(s, z["stream"])
+    dU, dV = orc.decompress_2d_log(z["stream"], r1, r2)
     assert np.array_equal(_bits(dU), _bits(z["dU"]))
     assert np.array_equal(_bits(dV), _bits(z["dV"]))
 
@@ -83,6 +97,31 @@ def test_2d_live_reference(orc, seed):
     a = mask_2d_padding(orc.compress_2d(U, V, eb), r1, r2)
     b = mask_2d_padding(ref.compress_2d(U, V, eb), r1, r2)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.skipif(not have_reference(), reason="reference build absent (oracle/_ref)")
+@pytest.mark.parametrize("seed", range(6))
+def test_2d_log_live_reference(orc, seed):
+    """offline_log pair against the live reference build on random shapes, bounds
+    and zero patterns (streams byte-equal, decoded fields bit-equal)."""
+    from cpsz_amd.synth import field_2d
+    rng = np.random.default_rng(50 + seed)
+    # The reference itself faults on some small inputs (fields two columns wide;
+    # narrow fields with many zeros, whose unpredictable values outgrow its
+    # output buffer estimate), so narrow shapes are pinned through the
+    # fixtures and the GPU edge-shape tests.
+    r1, r2 = int(rng.integers(8, 90)), int(rng.integers(8, 90))
+    eb = float(rng.choice([0.1, 0.01, 0.3]))
+    U, V = field_2d(r1, r2, 200 + seed)
+    if seed % 2 and min(r1, r2) >= 32:
+        U[rng.random(U.shape) < 0.05] = 0
+        V[rng.random(V.shape) < 0.05] = 0
+    ref = Reference()
+    a = mask_2d_log_padding(orc.compress_2d_log(U, V, eb), r1, r2)
+    b = mask_2d_log_padding(ref.compress_2d_log(U, V, eb), r1, r2)
+    assert np.array_equal(a, b)
+    for x, y in zip(orc.decompress_2d_log(b, r1, r2), ref.decompress_2d_log(b, r1, r2)):
+        assert np.array_equal(_bits(x), _bits(y))
 
 
 @pytest.mark.skipif(not have_reference(), reason="reference build absent (oracle/_ref)")
