@@ -7,8 +7,9 @@ The compute path is the HIP library ``libcpsz_hip.so`` (C ABI in
 from .api import (CpszError, compress_2d_dev, compress_3d_dev, critical_points_2d_dev,  # noqa: F401
                   critical_points_3d_dev, decompress_2d_dev, decompress_3d_dev, device_count,
                   last_stages, lib, set_timing, stream_size_2d,
-                  sz_compress_cp_preserve_2d_offline, sz_compress_cp_preserve_3d_offline_log,
-                  sz_decompress_cp_preserve_2d_offline, sz_decompress_cp_preserve_3d_online_log,
+                  sz_compress_cp_preserve_2d_offline, sz_compress_cp_preserve_2d_offline_log,
+                  sz_compress_cp_preserve_3d_offline_log, sz_decompress_cp_preserve_2d_offline,
+                  sz_decompress_cp_preserve_2d_offline_log, sz_decompress_cp_preserve_3d_online_log,
                   sz_lossless_compress, sz_lossless_decompress)
 from .api import batch_2d_config, compress_2d_batch_dev, decompress_2d_batch_dev  # noqa: F401
 from .api import batch_3d_config, compress_3d_batch_dev, decompress_3d_batch_dev  # noqa: F401

@@ -35,6 +35,12 @@ _SIGS = {
     "cpsz_lossless_compress": [C.c_int, _P, _SZ, C.POINTER(_P), C.POINTER(_SZ)],
     "cpsz_lossless_decompress": [_P, _SZ, _P, _SZ],
     "cpsz_critical_points_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_compress_2d_log_f32": [_P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_2d_log_f32": [_P, _SZ, _SZ, _SZ, _P, _P],
+    "cpsz_compress_2d_log_f32_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_2d_log_f32_dev": [C.c_int, _P, _P, _SZ, _SZ, _SZ, _P, _P],
+    "cpsz_stream_size_2d_log": [_P, _SZ, _SZ, C.POINTER(_SZ)],
+    "cpsz_eb_table_2d_log": [_P, C.c_int, C.POINTER(C.c_int)],
     "cpsz_critical_points_2d_noalloc_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
     "cpsz_critical_points_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ,
                                     C.POINTER(_SZ)],
@@ -155,6 +161,25 @@ def sz_decompress_cp_preserve_2d_offline(stream, r1: int, r2: int):
     U = np.empty((r1, r2), np.float32)
     V = np.empty((r1, r2), np.float32)
     _check(lib().cpsz_decompress_2d_f32(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)))
+    return U, V
+
+
+def sz_compress_cp_preserve_2d_offline_log(U, V, max_pwr_eb: float = 0.1) -> np.ndarray:
+    """The 2D offline_log compressor (bounds in the log domain); returns the stream."""
+    U, V = _f32(U), _f32(V)
+    if U.shape != V.shape or U.ndim != 2:
+        raise ValueError("U and V must be 2D arrays of the same shape")
+    r1, r2 = U.shape
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_compress_2d_log_f32(_ptr(U), _ptr(V), r1, r2, max_pwr_eb, C.byref(p), C.byref(s)))
+    return _take(p, s.value)
+
+
+def sz_decompress_cp_preserve_2d_offline_log(stream, r1: int, r2: int):
+    stream = np.ascontiguousarray(np.frombuffer(bytes(stream), np.uint8) if isinstance(stream, (bytes, bytearray)) else stream, np.uint8)
+    U = np.empty((r1, r2), np.float32)
+    V = np.empty((r1, r2), np.float32)
+    _check(lib().cpsz_decompress_2d_log_f32(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)))
     return U, V
 
 

@@ -143,6 +143,75 @@ int cpsz_decompress_2d_f32(const unsigned char* in, size_t in_size, size_t r1, s
   });
 }
 
+int cpsz_compress_2d_log_f32(const float* U, const float* V, size_t r1, size_t r2, double max_pwr_eb,
+                             unsigned char** out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(U && V && out && out_size && r1 && r2, kErrArg, "null argument or empty field");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    float* d = c.buf<float>("api.in", 2 * n);
+    CPSZ_HIP(hipMemcpyAsync(d, U, n * 4, hipMemcpyHostToDevice, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(d + n, V, n * 4, hipMemcpyHostToDevice, c.stream()));
+    DeviceStream s;
+    compress_2d_log_device(c, d, d + n, r1, r2, max_pwr_eb, s);
+    unsigned char* h = (unsigned char*)std::malloc(s.size ? s.size : 1);
+    CPSZ_REQUIRE(h, kErrAlloc, "host allocation failed");
+    CPSZ_HIP(hipMemcpyAsync(h, s.data, s.size, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+    *out = h;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_2d_log_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2, float* U,
+                               float* V) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && U && V && r1 && r2 && in_size >= 36, kErrArg, "bad arguments");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    uint8_t* din = c.buf<uint8_t>("api.stream", in_size + 16);
+    CPSZ_HIP(hipMemcpyAsync(din, in, in_size, hipMemcpyHostToDevice, c.stream()));
+    float* d = c.buf<float>("api.out", 2 * n);
+    decompress_2d_log_device(c, din, in, in_size, r1, r2, d, d + n);
+    CPSZ_HIP(hipMemcpyAsync(U, d, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(V, d + n, n * 4, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_compress_2d_log_f32_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                 size_t r2, double max_pwr_eb, const unsigned char** d_out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_out && out_size, kErrArg, "null output argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    DeviceStream s;
+    compress_2d_log_device(c, dU, dV, r1, r2, max_pwr_eb, s);
+    *d_out = s.data;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_2d_log_f32_dev(int device, void* stream, const unsigned char* d_in, size_t in_size,
+                                   size_t r1, size_t r2, float* dU, float* dV) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    decompress_2d_log_device(c, d_in, nullptr, in_size, r1, r2, dU, dV);
+  });
+}
+
+int cpsz_stream_size_2d_log(const unsigned char* in, size_t r1, size_t r2, size_t* size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && size && r1 && r2, kErrArg, "bad arguments");
+    size_t eb, u, v;
+    std::memcpy(&eb, in + 12, 8);
+    std::memcpy(&u, in + 20, 8);
+    std::memcpy(&v, in + 28, 8);
+    *size = 36 + eb + 2 * ((r1 * r2 - 1) / 8 + 1) + u + v;
+  });
+}
+
 int cpsz_compress_3d_f32(const float* U, const float* V, const float* W, size_t r1, size_t r2,
                          size_t r3, double max_pwr_eb, unsigned char** out, size_t* out_size) {
   return guarded([&] {
@@ -350,6 +419,15 @@ int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int
     for (int i = 0; i < (int)t.q3d.size() && i < cap3d; i++) q3d[i] = t.q3d[i];
     *n2d = (int)t.q2d.size();
     *n3d = (int)t.q3d.size();
+  });
+}
+
+int cpsz_eb_table_2d_log(double* q, int cap, int* n) {
+  return guarded([&] {
+    CPSZ_REQUIRE(q && n, kErrArg, "null argument");
+    const EbTables& t = eb_tables();
+    for (int i = 0; i < (int)t.q2l.size() && i < cap; i++) q[i] = t.q2l[i];
+    *n = (int)t.q2l.size();
   });
 }
 

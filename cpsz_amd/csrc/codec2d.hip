@@ -30,6 +30,7 @@
 #include "pool.hpp"
 #include "stream_reader.hpp"
 #include "huff_gpu.hpp"
+#include "libm_math.hpp"
 #include "lorenzo2d_step.hpp"
 
 namespace cpsz {
@@ -155,6 +156,19 @@ __device__ __forceinline__ uint32_t eb_code_2d(double x, const double* __restric
   return (uint32_t)k + 1;
 }
 
+// Eb code from a threshold table q[1..nq-2] (q[nq-1] = +inf): the largest k
+// with x >= q[k], 0 below q[1].
+__device__ __forceinline__ uint32_t code_from_table(double x, const double* __restrict__ q, int nq) {
+  if (!(x >= q[1])) return 0;
+  int lo = 1, hi = nq - 2;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (x >= q[mid]) lo = mid;
+    else hi = mid - 1;
+  }
+  return (uint32_t)lo;
+}
+
 constexpr int kTH = 8, kTW = 64;  // vertex tile
 
 __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const float* __restrict__ V,
@@ -162,7 +176,7 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
                                               const double* __restrict__ q2d, int nq,
                                               uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
                                               uint32_t* __restrict__ hist, int* __restrict__ flags,
-                                              EbOverride ov) {
+                                              EbOverride ov, const double* __restrict__ qlog, int nqlog) {
   __shared__ float su[kTH + 2][kTW + 2];
   __shared__ float sv[kTH + 2][kTW + 2];
   __shared__ double tb[kTH + 1][kTW + 1][2];
@@ -216,6 +230,14 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
       eb = dminf(max_pwr_eb, ldexp(eb, (int)ov.gain[vix]));
     const float u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     if (!isfinite(u) || !isfinite(v) || !(eb == eb)) bad = 1;
+    if (qlog) {
+      // offline_log: one code per vertex for both components, from the bound
+      // itself (src/sz_compress_cp_preserve_2d.cpp:327-331)
+      const uint32_t a = code_from_table(eb, qlog, nqlog);
+      cu[(size_t)i * r2 + j] = (uint16_t)a;
+      atomicAdd(&sh_hist[a ? a - 1 : 0], 1u);
+      continue;
+    }
     // the reference's eb/DBL_EPSILON overflows from 2^972 on (undefined id)
     if ((double)fabsf(u) * eb >= 0x1p972 || (double)fabsf(v) * eb >= 0x1p972) bad = 1;
     const uint32_t a = eb_code_2d((double)fabsf(u) * eb, q2d, nq);
@@ -250,7 +272,8 @@ struct Strip2d {
   int* prog;             // [2][nstrips] unused; [2 * nstrips] the item ticket
   int* ticket;
   int r1, r2, nstrips;
-  int wide;  // ids above 81: double-precision step
+  int wide;   // ids above 81: double-precision step
+  int shift;  // log2 of the eb base: 2 (offline, 4^id eps) or 1 (offline_log, 2^id eps)
 };
 
 // One item (component, 64-row strip) of a field's wavefront. The strip's last
@@ -340,10 +363,10 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       float R;
       if (COMPRESS) {
         uint16_t code;
-        R = l2d_compress_step<WIDE>(pred, B.xs[k], B.cs[k], code);
+        R = l2d_compress_step<WIDE>(pred, B.xs[k], B.cs[k], code, a.shift);
         if (act) Q[j] = code;
       } else {
-        R = l2d_decompress_step<WIDE>(pred, B.qs[k], B.cs[k], B.raws[k]);
+        R = l2d_decompress_step<WIDE>(pred, B.qs[k], B.cs[k], B.raws[k], a.shift);
         if (act) OUT[j] = R;
       }
       if (publish && lane == 63 && act) st_coh64(bnd_out + j, kBndTag | __float_as_uint(R));
@@ -413,17 +436,19 @@ __global__ void k_zero_i32(int* p, int n) {
 }
 
 // ------------------------------------------------------- huffman sources ----
+// m ids: the u codes then the v codes (offline, m = 2n), or the one code per
+// vertex of offline_log (m = n)
 struct SrcIds2d {
   const uint16_t* cu;
   const uint16_t* cv;
-  size_t n;
+  size_t n, m;
   static constexpr int kChunk = 256;
   static constexpr int kMaxItems = kChunk, kPer = 1;
   __device__ int get(size_t c, int idx, uint32_t* sym) const {
-    const size_t p0 = c * kChunk + idx, p = p0 < 2 * n ? p0 : 2 * n - 1;
+    const size_t p0 = c * kChunk + idx, p = p0 < m ? p0 : m - 1;
     const uint32_t x = p < n ? cu[p] : cv[p - n];
     sym[0] = x ? x - 1 : 0;
-    return p0 < 2 * n;
+    return p0 < m;
   }
 };
 
@@ -561,6 +586,43 @@ __global__ void __launch_bounds__(256) k_scatter_unpred(const uint16_t* __restri
   }
 }
 
+// offline_log transform (include/sz_cp_preserve_utils.hpp:205-221): host-libm
+// log2f of |x| (-100 for 0) and the MSB-first sign bitmaps (1 = positive):
+// one wave per 64 values; reversing the ballot's 64 bits and then its byte
+// order puts value 8k+j at bit 7-j of byte k. Block y = component.
+__global__ void __launch_bounds__(256) k_log2d(const float* __restrict__ U, const float* __restrict__ V, size_t n,
+                                               size_t nq, LibmView lv, float* __restrict__ LU,
+                                               float* __restrict__ LV, uint64_t* __restrict__ smap) {
+  const int c = blockIdx.y, lane = threadIdx.x & 63;
+  const float* X = c ? V : U;
+  float* L = c ? LV : LU;
+  const size_t q = (size_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (q >= nq) return;  // uniform across the wave
+  const size_t e = q * 64 + lane;
+  const float x = e < n ? X[e] : 0.f;
+  if (e < n) L[e] = x != 0 ? host_log2f(lv, fabsf(x)) : -100.f;
+  const uint64_t m = __ballot(x > 0);
+  if (lane == 0) smap[c * nq + q] = __builtin_bswap64(__builtin_bitreverse64(m));
+}
+
+// offline_log decode, in place (src/sz_decompress_cp_preserve_2d.cpp:80-92):
+// 0 below -99, else +-exp2f (host libm) by the sign bitmap.
+__global__ void __launch_bounds__(256) k_finish2d(float* __restrict__ U, float* __restrict__ V, size_t n,
+                                                  const uint8_t* __restrict__ signs, size_t smap, LibmView ev) {
+  const int c = blockIdx.y;
+  float* X = c ? V : U;
+  const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const float lx = X[e];
+  const bool pos = (signs[c * smap + e / 8] >> (7 - (e % 8))) & 1;
+  float y = 0.f;
+  if (!(lx < -99)) {
+    const float t = host_exp2f(ev, lx);
+    y = pos ? t : -t;
+  }
+  X[e] = y;
+}
+
 struct U16Hist {
   const uint16_t* p;
   size_t n;
@@ -672,7 +734,8 @@ void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
   CPSZ_HIP(hipMemsetAsync(w.flags, 0, 4 * sizeof(int), st));
   dim3 grid((unsigned)div_up(w.r2, kTW), (unsigned)div_up(w.r1, kTH));
   k_eb2d<<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
-                               w.cu, w.cv, w.hist, w.flags, ov);
+                               w.cu, w.cv, w.hist, w.flags, ov, w.log ? ctx.d_q2l() : nullptr,
+                               ctx.q2l_size());
   CPSZ_HIP(hipGetLastError());
 }
 
@@ -694,10 +757,12 @@ void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap) {
   for (size_t k = 0; k < nf; k++) {
     Strip2d& sa = f[k];
     sa = Strip2d{};
-    sa.x[0] = w[k]->dU;
-    sa.x[1] = w[k]->dV;
+    const bool lg = w[k]->log;
+    sa.x[0] = lg ? w[k]->lu : w[k]->dU;
+    sa.x[1] = lg ? w[k]->lv : w[k]->dV;
     sa.c[0] = w[k]->cu;
-    sa.c[1] = w[k]->cv;
+    sa.c[1] = lg ? w[k]->cu : w[k]->cv;
+    sa.shift = lg ? 1 : 2;
     sa.q[0] = w[k]->qu;
     sa.q[1] = w[k]->qv;
     sa.nstrips = (int)div_up(w[k]->r1, 64);
@@ -714,7 +779,8 @@ void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap) {
 void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   hipStream_t st = ctx.stream();
   const size_t r1 = w.r1, r2 = w.r2, n = r1 * r2;
-  const float *dU = w.dU, *dV = w.dV;
+  // the values the codec ran on (unpredictable ones are stored as they are)
+  const float *dU = w.log ? w.lu : w.dU, *dV = w.log ? w.lv : w.dV;
   const uint16_t *cu = w.cu, *cv = w.cv, *qu = w.qu, *qv = w.qv;
   uint32_t* hist = w.hist;
   // ---- histograms of the quant codes
@@ -737,8 +803,9 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   // while the host builds the trees
   const int nby = (int)div_up(r2, kBlock2d);
   const size_t nb = div_up(r1, kBlock2d) * nby;
-  const size_t nch_ids = div_up(2 * n, SrcIds2d::kChunk);
-  SrcIds2d s_ids{cu, cv, n};
+  const size_t m_ids = w.log ? n : 2 * n;
+  const size_t nch_ids = div_up(m_ids, SrcIds2d::kChunk);
+  SrcIds2d s_ids{cu, cv, n, m_ids};
   SrcCodes2d s_u{qu, (int)r1, (int)r2, nby}, s_v{qv, (int)r1, (int)r2, nby};
   uint64_t* bits = ctx.buf<uint64_t>("c2.bits", nch_ids + 2 * nb);
   uint64_t* offs = ctx.buf<uint64_t>("c2.offs", nch_ids + 2 * nb + 8);
@@ -839,12 +906,15 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   const size_t u_size = hu1.size() + 4 * unp_u + hu2.size() + bytes_u;
   const size_t v_size = hv1.size() + 4 * unp_v + hv2.size() + bytes_v;
   std::vector<uint8_t> hdr;
-  put(hdr, (int)4);
+  put(hdr, (int)(w.log ? 2 : 4));
   put(hdr, (double)DBL_EPSILON);
   put(hdr, eb_size);
   put(hdr, u_size);
   put(hdr, v_size);
-  const size_t total = hdr.size() + eb_size + u_size + v_size;
+  // offline_log: the two sign bitmaps follow the id section
+  // (src/sz_compress_cp_preserve_2d.cpp:358-366)
+  const size_t smap = w.log ? (n - 1) / 8 + 1 : 0;
+  const size_t total = hdr.size() + eb_size + 2 * smap + u_size + v_size;
   uint8_t* dout = ctx.buf<uint8_t>("c2.out", total + 16);
   // host pieces go through one pinned staging buffer
   std::vector<const std::vector<uint8_t>*> hp = {&hdr, &h_eb, &hu1, &hu2, &hv1, &hv2};
@@ -866,6 +936,12 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   host_piece(hdr);
   host_piece(h_eb);
   dev_piece(words, bytes_eb);
+  if (w.log) {
+    const size_t nq = div_up(n, 64);
+    if (nq * 8 == smap) dev_piece(w.smap, 2 * smap);  // the two bitmaps are already contiguous
+    else
+      for (int c = 0; c < 2; c++) dev_piece(w.smap + c * nq, smap);
+  }
   host_piece(hu1);
   dev_piece(unp, 4 * unp_u);
   host_piece(hu2);
@@ -921,6 +997,38 @@ void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, s
   ctx.finish_marks();
 }
 
+void compress_2d_log_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
+                            double max_pwr_eb, DeviceStream& out) {
+  ctx.stages.clear();
+  ctx.mark("start");
+  CPSZ_REQUIRE(dU && dV && r1 >= 1 && r2 >= 1 && r1 < (1u << 30) && r2 < (1u << 30), kErrArg,
+               "compress_2d_log: bad arguments");
+  hipStream_t st = ctx.stream();
+  C2Work w;
+  w.dU = dU;
+  w.dV = dV;
+  w.r1 = r1;
+  w.r2 = r2;
+  w.log = true;
+  const size_t n = r1 * r2, nq = div_up(n, 64);
+  c2_bind(w, ctx.buf<uint8_t>("c2.scratch", c2_scratch_bytes(r1, r2)));
+  float* L = ctx.buf<float>("c2l.log", 2 * n);
+  uint64_t* sm = ctx.buf<uint64_t>("c2l.smap", 2 * nq);
+  w.lu = L;
+  w.lv = L + n;
+  w.smap = sm;
+  c2_eb(ctx, w, max_pwr_eb, EbOverride());
+  k_log2d<<<dim3((unsigned)div_up(nq, 4), 2), 256, 0, st>>>(dU, dV, n, nq, ctx.libm_dev().log2f, L, L + n, sm);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("eb");
+  C2Work* wp = &w;
+  c2_check(ctx, &wp, 1);
+  c2_wavefront(ctx, &wp, 1, 4096);
+  ctx.mark("lorenzo");
+  c2_entropy(ctx, w, out);
+  ctx.finish_marks();
+}
+
 namespace {
 // Reads stream bytes either from a host copy or from device memory.
 struct HuffSection {
@@ -956,20 +1064,31 @@ struct D2Plan {
   } comp[2];
   int nby = 0;
   size_t nb = 0;
+  size_t sign_off = 0;  // offline_log: the two sign bitmaps
 };
 
-void d2_parse(StreamReader& rd, size_t r1, size_t r2, D2Plan& p) {
+// log: the offline_log layout (base 2, the sign bitmaps between the id
+// section and the u section, src/sz_decompress_cp_preserve_2d.cpp:60-79).
+void d2_parse(StreamReader& rd, size_t r1, size_t r2, D2Plan& p, bool log = false) {
   const size_t n = r1 * r2;
   const int base = rd.get<int>(0);
   const double thr = rd.get<double>(4);
   const size_t u_size = rd.get<size_t>(20);
-  CPSZ_REQUIRE(base == 4 && thr == DBL_EPSILON, kErrUnsupported,
-               "decompress_2d: stream base/threshold differ from the offline 2D format");
+  CPSZ_REQUIRE(base == (log ? 2 : 4) && thr == DBL_EPSILON, kErrUnsupported,
+               log ? "decompress_2d_log: stream base/threshold differ from the offline_log 2D format"
+                   : "decompress_2d: stream base/threshold differ from the offline 2D format");
   p.h_eb = parse_huff(rd, 36);
-  // components follow the id section directly; the u section spans u_size bytes
+  // components follow the id section (and the sign bitmaps) directly; the u
+  // section spans u_size bytes
   p.nby = (int)div_up(r2, kBlock2d);
   p.nb = div_up(r1, kBlock2d) * p.nby;
   size_t off = p.h_eb.end;
+  if (log) {
+    const size_t smap = (n - 1) / 8 + 1;
+    p.sign_off = off;
+    rd.read(off, 2 * smap);  // bounds check
+    off += 2 * smap;
+  }
   for (int c = 0; c < 2; c++) {
     D2Plan::Comp& k = p.comp[c];
     k.off = off;
@@ -1050,9 +1169,11 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   hipStream_t st = ctx.stream();
   const size_t r1 = w.r1, r2 = w.r2, n = r1 * r2;
   uint16_t *sid_u = w.sid_u, *sid_v = w.sid_v, *qu = w.qu, *qv = w.qv;
+  const bool log = w.log;
   StreamReader rd(ctx, h_in, d_in, in_size);
   D2Plan p;
-  d2_parse(rd, r1, r2, p);
+  d2_parse(rd, r1, r2, p, w.log);
+  if (w.log) w.signs = d_in + p.sign_off;
   const HuffSection& h_eb = p.h_eb;
   const int nby = p.nby;
   ctx.mark("parse");
@@ -1080,7 +1201,8 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
     HelperJob u_ready([&] { prep(0); }, !ctx.batch_worker);
     uint32_t* w = stage_bits(h_eb, "d2.w.eb");
     DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
-    decode_stream(ctx, dd, w, h_eb.nbytes, 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb", [&] { prep(1); });
+    decode_stream(ctx, dd, w, h_eb.nbytes, log ? n : 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb",
+                  [&] { prep(1); });
     u_ready.join();  // rethrows a malformed code tree
   }
   {
@@ -1180,7 +1302,8 @@ void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap) {
     sa = Strip2d{};
     const size_t n = w[k]->r1 * w[k]->r2;
     sa.c[0] = w[k]->sid_u;
-    sa.c[1] = w[k]->sid_v;
+    sa.c[1] = w[k]->log ? w[k]->sid_u : w[k]->sid_v;
+    sa.shift = w[k]->log ? 1 : 2;
     sa.q[0] = w[k]->qu;
     sa.q[1] = w[k]->qv;
     sa.raw[0] = w[k]->raw;
@@ -1223,6 +1346,32 @@ void decompress_2d_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, si
   D2Work* wp = &w;
   d2_wavefront(ctx, &wp, 1, 4096);
   ctx.mark("lorenzo");
+  d2_check(ctx, &wp, 1);
+  ctx.finish_marks();
+}
+
+void decompress_2d_log_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, size_t r1,
+                              size_t r2, float* dU, float* dV) {
+  CPSZ_REQUIRE(d_in && dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "decompress_2d_log: bad arguments");
+  ctx.stages.clear();
+  ctx.mark("start");
+  D2Work w;
+  w.r1 = r1;
+  w.r2 = r2;
+  w.dU = dU;
+  w.dV = dV;
+  w.log = true;
+  d2_bind(w, ctx.buf<uint8_t>("d2.scratch", d2_scratch_bytes(r1, r2)));
+  CPSZ_HIP(hipMemsetAsync(w.err, 0, 16, ctx.stream()));
+  d2_entropy(ctx, d_in, h_in, in_size, w);
+  D2Work* wp = &w;
+  d2_wavefront(ctx, &wp, 1, 4096);
+  ctx.mark("lorenzo");
+  const size_t n = r1 * r2;
+  k_finish2d<<<dim3((unsigned)div_up(n, 256), 2), 256, 0, ctx.stream()>>>(dU, dV, n, w.signs, (n - 1) / 8 + 1,
+                                                                         ctx.libm_dev().exp2f);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("finish");
   d2_check(ctx, &wp, 1);
   ctx.finish_marks();
 }

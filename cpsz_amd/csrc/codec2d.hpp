@@ -38,6 +38,13 @@ struct C2Work {
   uint64_t* bnd = nullptr;  // wavefront boundary rows (tagged values)
   int* prog = nullptr;      // wavefront item ticket
   bool wide = false;
+  // offline_log: one eb code per vertex (cu) for both components, the codec
+  // runs on the log values lu/lv, and smap holds the two sign bitmaps as
+  // 64-bit words (ceil(n/64) per component)
+  bool log = false;
+  const float* lu = nullptr;
+  const float* lv = nullptr;
+  const uint64_t* smap = nullptr;
 };
 size_t c2_scratch_bytes(size_t r1, size_t r2);
 void c2_bind(C2Work& w, uint8_t* scratch);  // carves c2_scratch_bytes of device memory
@@ -55,6 +62,8 @@ struct D2Work {
   uint64_t* bnd = nullptr;  // wavefront boundary rows (tagged values)
   int *prog = nullptr, *err = nullptr;
   bool wide = false;
+  bool log = false;               // offline_log stream: ids shared by u and v, log values
+  const uint8_t* signs = nullptr;  // offline_log: the stream's two sign bitmaps (device)
 };
 size_t d2_scratch_bytes(size_t r1, size_t r2);
 void d2_bind(D2Work& w, uint8_t* scratch);
@@ -65,6 +74,14 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
                       D2Work* const* w, size_t nf, int workers);
 void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap);
 void d2_check(Ctx& ctx, D2Work* const* w, size_t nf);
+
+// 2D offline_log pair (sz_compress_cp_preserve_2d_offline_log /
+// sz_decompress_cp_preserve_2d_offline_log): the offline bounds taken in the
+// log domain, the codec run on log2f|x|, sign bitmaps in the stream.
+void compress_2d_log_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
+                            double max_pwr_eb, DeviceStream& out);
+void decompress_2d_log_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, size_t r1,
+                              size_t r2, float* dU, float* dV);
 
 // Analytic per-vertex bound of the plain method (before quantization).
 void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,

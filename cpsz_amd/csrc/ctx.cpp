@@ -27,6 +27,17 @@ int id3d_ref(double e) {
   return (int)(std::log2(a / thr) / std::log2(2.0));
 }
 
+// The reference's 2D offline_log eb code for a vertex bound e >= 0
+// (src/sz_compress_cp_preserve_2d.cpp:327-331): a = log2(1+e) quantized by
+// eb_exponential_quantize(a, 2, log2(2), DBL_EPSILON). Code 0: a <= eps (zero
+// precision); code c >= 1: id c - 1 with precision 2^(c-1) eps.
+int code2l_ref(double e) {
+  const double thr = std::numeric_limits<double>::epsilon();
+  const double a = std::log2(1 + e);
+  if (a <= thr) return 0;
+  return (int)(std::log2(a / thr) / std::log2(2.0)) + 1;
+}
+
 double next_down(double x) { return std::nextafter(x, 0.0); }
 double next_up(double x) { return std::nextafter(x, INFINITY); }
 
@@ -79,6 +90,20 @@ EbTables build_eb_tables() {
     t.q3d.push_back(threshold(id3d_ref, k, lo, hi));
   }
   t.q3d.push_back(INFINITY);
+  t.q2l.push_back(0);
+  // code >= 1 from the first e with 1 + e > 1 (e just above 2^-53)
+  t.q2l.push_back(threshold(code2l_ref, 1, 0x1p-60, 0x1p-50));
+  for (int k = 2; k < 80; k++) {
+    // code >= k once log2(1+e) >= 2^(k-1) eps: e near 2^(2^(k-1) eps) - 1
+    const double a = std::ldexp(std::numeric_limits<double>::epsilon(), k - 1);
+    if (a > 1000) break;
+    const double guess = a < 1e-3 ? a * 0.6931471805599453 : std::exp2(a) - 1;
+    const double lo = guess * 0.5, hi = guess * 2;
+    if (!std::isfinite(hi)) break;
+    if (code2l_ref(lo) >= k || code2l_ref(hi) < k) fail(kErrInternal, "2D log eb threshold bracket failed");
+    t.q2l.push_back(threshold(code2l_ref, k, lo, hi));
+  }
+  t.q2l.push_back(INFINITY);
   return t;
 }
 
@@ -101,6 +126,7 @@ Ctx::~Ctx() {
   for (auto& kv : hbufs_) hipHostFree(kv.second.p);
   if (d_q2d_) hipFree(d_q2d_);
   if (d_q3d_) hipFree(d_q3d_);
+  if (d_q2l_) hipFree(d_q2l_);
   for (void* p : libm_allocs_) hipFree(p);
   for (auto e : event_pool_) hipEventDestroy(e);
   if (point_) hipEventDestroy(point_);
@@ -160,6 +186,15 @@ const double* Ctx::d_q2d() {
     CPSZ_HIP(hipMemcpy(d_q2d_, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   return d_q2d_;
+}
+
+const double* Ctx::d_q2l() {
+  if (!d_q2l_) {
+    const auto& q = eb_tables().q2l;
+    CPSZ_HIP(hipMalloc(&d_q2l_, q.size() * sizeof(double)));
+    CPSZ_HIP(hipMemcpy(d_q2l_, q.data(), q.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  return d_q2l_;
 }
 
 const double* Ctx::d_q3d() {

@@ -23,6 +23,10 @@ namespace cpsz {
 struct EbTables {
   std::vector<double> q2d;  // index k; q2d[0] unused; +inf past the range
   std::vector<double> q3d;
+  // 2D offline_log (src/sz_compress_cp_preserve_2d.cpp:327-331, eb
+  // log2(1+e) base 2 over DBL_EPSILON): q2l[k] = min { e : code(e) >= k },
+  // code 0 = zero precision, code c >= 1 = id c - 1
+  std::vector<double> q2l;
 };
 const EbTables& eb_tables();
 
@@ -57,6 +61,8 @@ class Ctx {
   // Device copies of the eb threshold tables.
   const double* d_q2d();
   const double* d_q3d();
+  const double* d_q2l();
+  int q2l_size() const { return (int)eb_tables().q2l.size(); }
   int q2d_size() const { return (int)eb_tables().q2d.size(); }
   int q3d_size() const { return (int)eb_tables().q3d.size(); }
   // Device copies of the libm exception tables (uploaded on first use).
@@ -83,6 +89,7 @@ class Ctx {
   std::map<std::string, DevBuf> hbufs_;
   double* d_q2d_ = nullptr;
   double* d_q3d_ = nullptr;
+  double* d_q2l_ = nullptr;
   LibmDev libm_{};
   std::vector<void*> libm_allocs_;
   bool libm_ready_ = false;

@@ -15,6 +15,9 @@
 //       sum is exact when the exponents differ by <= 28; otherwise the smaller
 //       term is below 1/32 ulp of the larger and both roundings return it).
 // Eb code c: 0 = zero precision (always unpredictable), c >= 1: id = c - 1.
+// `shift` is log2 of the id base: 2 for the offline codec (E = 4^id eps), 1
+// for offline_log (E = 2^id eps, ids <= 62, never the wide path); the
+// argument above holds for both, since E stays a power of two.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -24,16 +27,17 @@
 namespace cpsz {
 
 template <bool WIDE>
-__host__ __device__ inline float l2d_compress_step(float pred, float x, uint32_t c, uint16_t& code) {
+__host__ __device__ inline float l2d_compress_step(float pred, float x, uint32_t c, uint16_t& code, int shift = 2) {
   if (!WIDE) {
     // branch-free: code 0 is computed with id 0 and then rejected
     const int id = c ? (int)c - 1 : 0;
+    const int pe = shift * id - 52;  // log2 E
     const float diff = x - pred;
-    const float a = ldexpf(fabsf(diff), 52 - 2 * id);
+    const float a = ldexpf(fabsf(diff), -pe);
     const float qf = floorf(fmaf(floorf(a), 0.5f, 0.5f));
     const float qs = copysignf(qf, diff);
-    const float rec = pred + ldexpf(qs, 2 * id - 51);
-    const bool ok = (c != 0) & (a < 65535.0f) & (fabsf(rec - x) <= ldexpf(1.0f, 2 * id - 52));
+    const float rec = pred + ldexpf(qs, pe + 1);
+    const bool ok = (c != 0) & (a < 65535.0f) & (fabsf(rec - x) <= ldexpf(1.0f, pe));
     code = ok ? (uint16_t)((int)qs + 32768) : 0;
     return ok ? rec : x;
   } else {
@@ -64,9 +68,10 @@ __host__ __device__ inline float l2d_compress_step(float pred, float x, uint32_t
 // eb = 0 for stream id 0 and 2^(2 sid - 52) otherwise
 // (src/sz_decompress_cp_preserve_2d.cpp:30-33).
 template <bool WIDE>
-__host__ __device__ inline float l2d_decompress_step(float pred, uint32_t q, uint32_t sid, float raw) {
+__host__ __device__ inline float l2d_decompress_step(float pred, uint32_t q, uint32_t sid, float raw,
+                                                     int shift = 2) {
   if (!WIDE) {
-    const float kE = sid ? ldexpf((float)(2 * ((int)q - 32768)), 2 * (int)sid - 52) : 0.f;
+    const float kE = sid ? ldexpf((float)(2 * ((int)q - 32768)), shift * (int)sid - 52) : 0.f;
     return q == 0 ? raw : pred + kE;
   }
   if (q == 0) return raw;

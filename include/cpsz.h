@@ -56,6 +56,24 @@ int cpsz_compress_2d_f32(const float* U, const float* V, size_t r1, size_t r2, d
 int cpsz_decompress_2d_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2,
                            float* U, float* V);
 
+/* Replaces sz_compress_cp_preserve_2d_offline_log<float>
+ * (reference include/sz_compress_cp_preserve_2d.hpp:12-14; implementation
+ * src/sz_compress_cp_preserve_2d.cpp:285-374): the offline bounds in the log
+ * domain, the codec on log2f|x|, sign bitmaps in the stream. */
+int cpsz_compress_2d_log_f32(const float* U, const float* V, size_t r1, size_t r2, double max_pwr_eb,
+                             unsigned char** out, size_t* out_size);
+/* Replaces sz_decompress_cp_preserve_2d_offline_log<float>
+ * (reference include/sz_decompress_cp_preserve_2d.hpp:12-14; implementation
+ * src/sz_decompress_cp_preserve_2d.cpp:54-99). */
+int cpsz_decompress_2d_log_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2, float* U,
+                               float* V);
+int cpsz_compress_2d_log_f32_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                 size_t r2, double max_pwr_eb, const unsigned char** d_out, size_t* out_size);
+int cpsz_decompress_2d_log_f32_dev(int device, void* stream, const unsigned char* d_in, size_t in_size,
+                                   size_t r1, size_t r2, float* dU, float* dV);
+/* Total byte size of a 2D offline_log stream from its own framing. */
+int cpsz_stream_size_2d_log(const unsigned char* in, size_t r1, size_t r2, size_t* size);
+
 /* Replaces sz_compress_cp_preserve_3d_offline_log<float>
  * (reference include/sz_compress_cp_preserve_3d.hpp:6-8; implementation
  * src/sz_compress_cp_preserve_3d.cpp:360-581). r1 slowest .. r3 fastest. */
@@ -139,6 +157,10 @@ int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const
  * id >= k (both from this host's libm, as the reference computes ids). Returns
  * the table lengths through n2d / n3d. */
 int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int* n3d);
+/* The 2D offline_log eb-code thresholds: q[k] is the smallest vertex bound e
+ * whose code (0 = zero precision, c >= 1 = id c - 1 of
+ * eb_exponential_quantize(log2(1 + e), 2, 1, DBL_EPSILON)) is >= k. */
+int cpsz_eb_table_2d_log(double* q, int cap, int* n);
 /* Builds the Huffman tree the encoder would emit for the given symbol
  * frequencies and writes the serialized tree bytes and node count. */
 int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,

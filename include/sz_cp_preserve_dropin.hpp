@@ -6,6 +6,8 @@
 // library:
 //   sz_compress_cp_preserve_2d_offline     include/sz_compress_cp_preserve_2d.hpp:8-10
 //   sz_decompress_cp_preserve_2d_offline   include/sz_decompress_cp_preserve_2d.hpp:8-11
+//   sz_compress_cp_preserve_2d_offline_log include/sz_compress_cp_preserve_2d.hpp:12-14
+//   sz_decompress_cp_preserve_2d_offline_log include/sz_decompress_cp_preserve_2d.hpp:13-15
 //   sz_compress_cp_preserve_3d_offline_log include/sz_compress_cp_preserve_3d.hpp:6-8
 //   sz_decompress_cp_preserve_3d_online_log include/sz_decompress_cp_preserve_3d.hpp:8-11
 //   ZSTD_COMPRESSOR, is_lossless_compressed_data, sz_lossless_compress,
@@ -36,6 +38,13 @@ unsigned char* sz_compress_cp_preserve_2d_offline(const T* U, const T* V, size_t
 template <typename T>
 void sz_decompress_cp_preserve_2d_offline(const unsigned char* compressed, size_t r1, size_t r2,
                                           T*& U, T*& V);
+template <typename T>
+unsigned char* sz_compress_cp_preserve_2d_offline_log(const T* U, const T* V, size_t r1, size_t r2,
+                                                      size_t& compressed_size, bool transpose = false,
+                                                      double max_pwr_eb = 0.1);
+template <typename T>
+void sz_decompress_cp_preserve_2d_offline_log(const unsigned char* compressed, size_t r1, size_t r2, T*& U,
+                                              T*& V);
 template <typename T>
 unsigned char* sz_compress_cp_preserve_3d_offline_log(const T* U, const T* V, const T* W, size_t r1,
                                                       size_t r2, size_t r3, size_t& compressed_size,
@@ -103,6 +112,39 @@ inline void sz_decompress_cp_preserve_2d_offline<double>(const unsigned char*, s
                                                          double*& V) {
   cpsz_dropin::release(U);
   cpsz_dropin::release(V);
+}
+
+template <>
+inline unsigned char* sz_compress_cp_preserve_2d_offline_log<float>(const float* U, const float* V, size_t r1,
+                                                                    size_t r2, size_t& compressed_size, bool,
+                                                                    double max_pwr_eb) {
+  unsigned char* out = nullptr;
+  size_t size = 0;
+  if (cpsz_compress_2d_log_f32(U, V, r1, r2, max_pwr_eb, &out, &size) != CPSZ_OK) {
+    compressed_size = 0;
+    return nullptr;
+  }
+  compressed_size = size;
+  return out;
+}
+
+// src/sz_decompress_cp_preserve_2d.cpp:57-58 frees incoming outputs; results are malloc'd.
+template <>
+inline void sz_decompress_cp_preserve_2d_offline_log<float>(const unsigned char* compressed, size_t r1,
+                                                            size_t r2, float*& U, float*& V) {
+  cpsz_dropin::release(U);
+  cpsz_dropin::release(V);
+  size_t size = 0;
+  if (cpsz_stream_size_2d_log(compressed, r1, r2, &size) != CPSZ_OK) return;
+  float* u = cpsz_dropin::fresh<float>(r1 * r2);
+  float* v = cpsz_dropin::fresh<float>(r1 * r2);
+  if (!u || !v || cpsz_decompress_2d_log_f32(compressed, size, r1, r2, u, v) != CPSZ_OK) {
+    std::free(u);
+    std::free(v);
+    return;
+  }
+  U = u;
+  V = v;
 }
 
 template <>

@@ -3,7 +3,8 @@
 // test/sz_compress_cp_preserve_3d_test.cpp:26-42), compiled against the
 // drop-in header and linked with libcpsz_hip.so.
 //
-// usage: dropin_roundtrip <2d|3d> <in.bin> <out_prefix> r1 r2 [r3] eb
+// usage: dropin_roundtrip <2d|2dlog|3d> <in.bin> <out_prefix> r1 r2 [r3] eb
+//   (2dlog: the 2D offline_log pair)
 //   in.bin holds the components back to back (float32, row-major); writes
 //   <out_prefix>.stream (pre-ZSTD stream), <out_prefix>.zst and
 //   <out_prefix>.out (decompressed components back to back). Prints
@@ -30,6 +31,7 @@ int main(int argc, char** argv) {
     return 2;
   }
   const bool is3d = std::strcmp(argv[1], "3d") == 0;
+  const bool islog = std::strcmp(argv[1], "2dlog") == 0;
   // int dims and atof, as the reference drivers parse them
   // (test/sz_compress_cp_preserve_2d_test.cpp:14-16)
   const int r1 = std::atoi(argv[4]), r2 = std::atoi(argv[5]);
@@ -48,7 +50,8 @@ int main(int argc, char** argv) {
   unsigned char* stream =
       is3d ? sz_compress_cp_preserve_3d_offline_log(in.data(), in.data() + n, in.data() + 2 * n, r1, r2, r3,
                                                     size, false, max_eb)
-           : sz_compress_cp_preserve_2d_offline(in.data(), in.data() + n, r1, r2, size, false, max_eb);
+           : islog ? sz_compress_cp_preserve_2d_offline_log(in.data(), in.data() + n, r1, r2, size, false, max_eb)
+                   : sz_compress_cp_preserve_2d_offline(in.data(), in.data() + n, r1, r2, size, false, max_eb);
   if (!stream) {
     std::printf("fail compress: %s\n", cpsz_last_error());
     return 1;
@@ -73,6 +76,7 @@ int main(int argc, char** argv) {
   std::free(head);
   float *U = nullptr, *V = nullptr, *W = nullptr;
   if (is3d) sz_decompress_cp_preserve_3d_online_log<float>(back, r1, r2, r3, U, V, W);
+  else if (islog) sz_decompress_cp_preserve_2d_offline_log<float>(back, r1, r2, U, V);
   else sz_decompress_cp_preserve_2d_offline<float>(back, r1, r2, U, V);
   if (!U || !V || (is3d && !W)) {
     std::printf("fail decompress: %s\n", cpsz_last_error());

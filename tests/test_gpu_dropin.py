@@ -44,3 +44,14 @@ def test_dropin_3d(tmp_path):
     assert np.array_equal(stream, ref)
     for a, b in zip(dec, orc.decompress_3d(ref, 20, 24, 28)):
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_dropin_2d_offline_log(tmp_path):
+    orc = load_oracle()
+    U, V = field_2d(90, 140, 5)
+    stream, dec = _run(tmp_path, "2dlog", (U, V), (90, 140), 0.1)
+    from _libs import mask_2d_log_padding
+    ref = orc.compress_2d_log(U, V, 0.1)
+    assert np.array_equal(mask_2d_log_padding(stream, 90, 140), mask_2d_log_padding(ref, 90, 140))
+    for a, b in zip(dec, orc.decompress_2d_log(ref, 90, 140)):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))

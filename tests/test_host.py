@@ -372,3 +372,31 @@ def test_two_level_decode_tables_match_oracle_encoder(seed):
     assert L.cpsz_huffman_decode_host(tree.ctypes.data, tree.size, node_count, bits.ctypes.data, nbytes,
                                       syms.size, out.ctypes.data) == 0
     assert np.array_equal(out, syms.astype(np.uint32))
+
+
+def test_eb_table_2d_log_matches_reference_quantizer():
+    """The device's offline_log code table against log2 of this host's libm and
+    the reference's eb_exponential_quantize (include/sz_cp_preserve_utils.hpp:14-22,
+    base 2 over DBL_EPSILON, src/sz_compress_cp_preserve_2d.cpp:327-331): at every
+    threshold and its predecessor, and on random bounds over 40 decades."""
+    L = cz.lib()
+    q = np.zeros(128)
+    n = C.c_int()
+    assert L.cpsz_eb_table_2d_log(q.ctypes.data, 128, C.byref(n)) == 0
+    q = q[:n.value]
+    assert np.isinf(q[-1]) and np.all(np.diff(q[1:-1]) > 0)
+    libm = C.CDLL("libm.so.6")
+    libm.log2.restype = C.c_double
+    libm.log2.argtypes = [C.c_double]
+    rng = np.random.default_rng(3)
+    e = list(10.0 ** rng.uniform(-17, 3, 4000))
+    for t in q[1:-1]:
+        e += [t, np.nextafter(t, 0)]
+    e = np.array(e)
+    a = np.array([libm.log2(1 + x) for x in e])
+    from _libs import Reference, have_reference
+    quant = Reference() if have_reference() else load_oracle()
+    ids, _ = quant.eb_exp_quantize(a, 2, 1.0, np.finfo(np.float64).eps)
+    want = np.where(a <= np.finfo(np.float64).eps, 0, ids + 1)
+    got = np.searchsorted(q[1:-1], e, side="right")  # number of thresholds <= e
+    assert np.array_equal(got, want)
