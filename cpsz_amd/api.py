@@ -35,6 +35,10 @@ _SIGS = {
     "cpsz_lossless_compress": [C.c_int, _P, _SZ, C.POINTER(_P), C.POINTER(_SZ)],
     "cpsz_lossless_decompress": [_P, _SZ, _P, _SZ],
     "cpsz_critical_points_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_compress_2d_f64": [_P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_2d_f64": [_P, _SZ, _SZ, _SZ, _P, _P],
+    "cpsz_compress_2d_f64_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_decompress_2d_f64_dev": [C.c_int, _P, _P, _SZ, _SZ, _SZ, _P, _P],
     "cpsz_compress_2d_log_f32": [_P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
     "cpsz_decompress_2d_log_f32": [_P, _SZ, _SZ, _SZ, _P, _P],
     "cpsz_compress_2d_log_f32_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
@@ -77,6 +81,8 @@ _SIGS = {
     "cpsz_mlp_grad_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _P, _P],
     "cpsz_learn_train_3d_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, _SZ, C.c_int, C.c_int,
                                 C.c_float, C.c_uint64, _P, _P],
+    "cpsz_learn_infer_gains_dev": [C.c_int, _P, C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, C.c_double, _P, _SZ, _SZ,
+                                   _P],
     "cpsz_compress_2d_learned_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, C.c_double, _P, C.c_int,
                                            C.POINTER(_P), _P, _P, C.POINTER(C.c_int), C.POINTER(_SZ),
                                            C.POINTER(_SZ)],
@@ -146,7 +152,15 @@ def _take(p, n):
 
 # ------------------------------------------------------------- host API ----
 def sz_compress_cp_preserve_2d_offline(U, V, max_pwr_eb: float = 0.1) -> np.ndarray:
-    """Compresses row-major r1 x r2 float32 fields U, V; returns the stream."""
+    """Compresses row-major r1 x r2 fields U, V (float32, or float64 for the
+    reference's double instantiation); returns the stream."""
+    if np.asarray(U).dtype == np.float64:
+        U, V = np.ascontiguousarray(U, np.float64), np.ascontiguousarray(V, np.float64)
+        if U.shape != V.shape or U.ndim != 2:
+            raise ValueError("U and V must be 2D arrays of the same shape")
+        p, s = _P(), _SZ()
+        _check(lib().cpsz_compress_2d_f64(_ptr(U), _ptr(V), *U.shape, max_pwr_eb, C.byref(p), C.byref(s)))
+        return _take(p, s.value)
     U, V = _f32(U), _f32(V)
     if U.shape != V.shape or U.ndim != 2:
         raise ValueError("U and V must be 2D arrays of the same shape")
@@ -156,11 +170,13 @@ def sz_compress_cp_preserve_2d_offline(U, V, max_pwr_eb: float = 0.1) -> np.ndar
     return _take(p, s.value)
 
 
-def sz_decompress_cp_preserve_2d_offline(stream, r1: int, r2: int):
+def sz_decompress_cp_preserve_2d_offline(stream, r1: int, r2: int, dtype=np.float32):
+    """dtype float64 decodes a stream of the double instantiation."""
     stream = np.ascontiguousarray(np.frombuffer(bytes(stream), np.uint8) if isinstance(stream, (bytes, bytearray)) else stream, np.uint8)
-    U = np.empty((r1, r2), np.float32)
-    V = np.empty((r1, r2), np.float32)
-    _check(lib().cpsz_decompress_2d_f32(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)))
+    U = np.empty((r1, r2), dtype)
+    V = np.empty((r1, r2), dtype)
+    f = lib().cpsz_decompress_2d_f64 if np.dtype(dtype) == np.float64 else lib().cpsz_decompress_2d_f32
+    _check(f(_ptr(stream), stream.size, r1, r2, _ptr(U), _ptr(V)))
     return U, V
 
 

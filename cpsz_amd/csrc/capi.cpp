@@ -143,6 +143,63 @@ int cpsz_decompress_2d_f32(const unsigned char* in, size_t in_size, size_t r1, s
   });
 }
 
+int cpsz_compress_2d_f64(const double* U, const double* V, size_t r1, size_t r2, double max_pwr_eb,
+                         unsigned char** out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(U && V && out && out_size && r1 && r2, kErrArg, "null argument or empty field");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    double* d = c.buf<double>("api.in64", 2 * n);
+    CPSZ_HIP(hipMemcpyAsync(d, U, n * 8, hipMemcpyHostToDevice, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(d + n, V, n * 8, hipMemcpyHostToDevice, c.stream()));
+    DeviceStream s;
+    compress_2d_f64_device(c, d, d + n, r1, r2, max_pwr_eb, s);
+    unsigned char* h = (unsigned char*)std::malloc(s.size ? s.size : 1);
+    CPSZ_REQUIRE(h, kErrAlloc, "host allocation failed");
+    CPSZ_HIP(hipMemcpyAsync(h, s.data, s.size, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+    *out = h;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_2d_f64(const unsigned char* in, size_t in_size, size_t r1, size_t r2, double* U, double* V) {
+  return guarded([&] {
+    CPSZ_REQUIRE(in && U && V && r1 && r2 && in_size >= 36, kErrArg, "bad arguments");
+    Ctx& c = ctx_of(current_device());
+    const size_t n = r1 * r2;
+    uint8_t* din = c.buf<uint8_t>("api.stream", in_size + 16);
+    CPSZ_HIP(hipMemcpyAsync(din, in, in_size, hipMemcpyHostToDevice, c.stream()));
+    double* d = c.buf<double>("api.out64", 2 * n);
+    decompress_2d_f64_device(c, din, in, in_size, r1, r2, d, d + n);
+    CPSZ_HIP(hipMemcpyAsync(U, d, n * 8, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipMemcpyAsync(V, d + n, n * 8, hipMemcpyDeviceToHost, c.stream()));
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));
+  });
+}
+
+int cpsz_compress_2d_f64_dev(int device, void* stream, const double* dU, const double* dV, size_t r1, size_t r2,
+                             double max_pwr_eb, const unsigned char** d_out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_out && out_size, kErrArg, "null output argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    DeviceStream s;
+    compress_2d_f64_device(c, dU, dV, r1, r2, max_pwr_eb, s);
+    *d_out = s.data;
+    *out_size = s.size;
+  });
+}
+
+int cpsz_decompress_2d_f64_dev(int device, void* stream, const unsigned char* d_in, size_t in_size, size_t r1,
+                               size_t r2, double* dU, double* dV) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    decompress_2d_f64_device(c, d_in, nullptr, in_size, r1, r2, dU, dV);
+  });
+}
+
 int cpsz_compress_2d_log_f32(const float* U, const float* V, size_t r1, size_t r2, double max_pwr_eb,
                              unsigned char** out, size_t* out_size) {
   return guarded([&] {
@@ -799,6 +856,23 @@ int cpsz_compress_2d_learned_batch_dev(int device, void* stream, size_t nfields,
     if (iterations) *iterations = st.iterations;
     if (boosted) *boosted = st.boosted_vertices;
     if (fallback) *fallback = st.fallback_vertices;
+  });
+}
+
+int cpsz_learn_infer_gains_dev(int device, void* stream, int dim, const float* dU, const float* dV,
+                               const float* dW, const double* dE, size_t r1, size_t r2, size_t r3,
+                               double max_pwr_eb, const float* params, size_t v0, size_t m, uint8_t* d_gain) {
+  return guarded([&] {
+    CPSZ_REQUIRE(params && (dim == 2 || dim == 3), kErrArg, "bad arguments");
+    const size_t n = r1 * r2 * (dim == 3 ? r3 : 1);
+    CPSZ_REQUIRE(v0 <= n && m <= n - v0, kErrArg, "vertex range outside the field");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    const size_t np = dim == 2 ? kNumParams : kNumParams3;
+    float* P = c.buf<float>("api.infer.p", np);
+    CPSZ_HIP(hipMemcpyAsync(P, params, np * 4, hipMemcpyHostToDevice, c.stream()));
+    mlp_infer_gains(c, dim, dU, dV, dW, dE, r1, r2, dim == 3 ? r3 : 1, max_pwr_eb, P, v0, m, d_gain);
+    CPSZ_HIP(hipStreamSynchronize(c.stream()));  // the parameter upload reads the caller's host buffer
   });
 }
 

@@ -73,13 +73,15 @@ __device__ __forceinline__ double dbound4(double a, double b, double c, double d
   return dsign_bound(p, n, 2);
 }
 
-// max_eb_to_keep_sign_eigen_delta_2 (:75-123); rarely taken (CP triangles)
-__device__ __noinline__ double deigen(float u0, float u1, float u2, float v0, float v1, float v2,
-                                      float x0, float x1, float x2, float y0, float y1, float y2) {
+// max_eb_to_keep_sign_eigen_delta_2 (:75-123); rarely taken (CP triangles).
+// T is the field's value type: the reference computes c[], m, C and the
+// coefficients in T.
+template <typename T>
+__device__ __noinline__ double deigen(T u0, T u1, T u2, T v0, T v1, T v2, T x0, T x1, T x2, T y0, T y1, T y2) {
   double eb = 1;
-  float c[4];
+  T c[4];
   {
-    const float det = (x0 - x2) * (y1 - y2) - (x1 - x2) * (y0 - y2);
+    const T det = (x0 - x2) * (y1 - y2) - (x1 - x2) * (y0 - y2);
     c[0] = (y1 - y2) / det;
     c[1] = -(y0 - y2) / det;
     c[2] = -(x1 - x2) / det;
@@ -101,12 +103,11 @@ __device__ __noinline__ double deigen(float u0, float u1, float u2, float v0, fl
     dacc(-u2 * v1, p2, n2);
     eb = dminf(eb, dsign_bound(p2, n2, 2));
   }
-  const float m = c[1] * c[2] - c[0] * c[3];
-  const float C = (-m) * (u0 * v1 - u0 * v2 + u1 * v2 - u1 * v0 + u2 * v0 - u2 * v1);
+  const T m = c[1] * c[2] - c[0] * c[3];
+  const T C = (-m) * (u0 * v1 - u0 * v2 + u1 * v2 - u1 * v0 + u2 * v0 - u2 * v1);
   if (C == 0) return 0;
   if (C < 0) return eb;
-  const float co[6] = {c[0] * u0, c[1] * u1, -(c[1] + c[0]) * u2,
-                       c[2] * v0, c[3] * v1, -(c[3] + c[2]) * v2};
+  const T co[6] = {c[0] * u0, c[1] * u1, -(c[1] + c[0]) * u2, c[2] * v0, c[3] * v1, -(c[3] + c[2]) * v2};
   double p = 0, n = 0;
   for (int i = 0; i < 6; i++)
     for (int j = 0; j < 6; j++) dacc(co[i] * co[j], p, n);
@@ -121,7 +122,8 @@ __device__ __noinline__ double deigen(float u0, float u1, float u2, float v0, fl
 
 // max_eb_to_keep_position_and_type (src/sz_compress_cp_preserve_2d.cpp:125-172)
 // for the two triangle shapes of a cell (coordinates passed "reversed", :187-206).
-__device__ double dtri(float u0, float u1, float u2, float v0, float v1, float v2, int lower) {
+template <typename T>
+__device__ double dtri(T u0, T u1, T u2, T v0, T v1, T v2, int lower) {
   const double u0v1 = u0 * v1, u1v0 = u1 * v0, u0v2 = u0 * v2, u2v0 = u2 * v0, u1v2 = u1 * v2,
                u2v1 = u2 * v1;
   const double det = u0v1 - u1v0 + u1v2 - u2v1 + u2v0 - u0v2;
@@ -134,8 +136,9 @@ __device__ double dtri(float u0, float u1, float u2, float v0, float v1, float v
     const double e2 = dminf(dbound2(u1v2, -u2v1), dbound4(u0v1, -u1v0, u2v0, -u0v2));
     const double e3 = dminf(dbound2(u0v1, -u1v0), dbound4(u1v2, -u2v1, u2v0, -u0v2));
     // X_upper = {{0,0},{1,0},{1,1}}, X_lower = {{0,0},{0,1},{1,1}}; x = X[.][1], y = X[.][0]
-    const double e4 = lower ? dminf(e3, deigen(u0, u1, u2, v0, v1, v2, 0.f, 1.f, 1.f, 0.f, 0.f, 1.f))
-                            : dminf(e3, deigen(u0, u1, u2, v0, v1, v2, 0.f, 0.f, 1.f, 0.f, 1.f, 1.f));
+    const T o = 0, l = 1;
+    const double e4 = lower ? dminf(e3, deigen<T>(u0, u1, u2, v0, v1, v2, o, l, l, o, o, l))
+                            : dminf(e3, deigen<T>(u0, u1, u2, v0, v1, v2, o, o, l, o, l, l));
     double eb = dminf(e1, e2);
     return dminf(eb, e4);
   }
@@ -171,14 +174,15 @@ __device__ __forceinline__ uint32_t code_from_table(double x, const double* __re
 
 constexpr int kTH = 8, kTW = 64;  // vertex tile
 
-__global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const float* __restrict__ V,
+template <typename T>
+__global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* __restrict__ V,
                                               int r1, int r2, double max_pwr_eb,
                                               const double* __restrict__ q2d, int nq,
                                               uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
                                               uint32_t* __restrict__ hist, int* __restrict__ flags,
                                               EbOverride ov, const double* __restrict__ qlog, int nqlog) {
-  __shared__ float su[kTH + 2][kTW + 2];
-  __shared__ float sv[kTH + 2][kTW + 2];
+  __shared__ T su[kTH + 2][kTW + 2];
+  __shared__ T sv[kTH + 2][kTW + 2];
   __shared__ double tb[kTH + 1][kTW + 1][2];
   __shared__ uint32_t sh_hist[kMaxEbCode];
   const int tid = threadIdx.x;
@@ -187,7 +191,7 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
   for (int k = tid; k < (kTH + 2) * (kTW + 2); k += 256) {
     const int r = k / (kTW + 2), c = k % (kTW + 2);
     const int gi = i0 - 1 + r, gj = j0 - 1 + c;
-    float a = 0, b = 0;
+    T a = 0, b = 0;
     if (gi >= 0 && gi < r1 && gj >= 0 && gj < r2) {
       a = U[(size_t)gi * r2 + gj];
       b = V[(size_t)gi * r2 + gj];
@@ -202,10 +206,10 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
     double up = INFINITY, lo = INFINITY;
     if (ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1) {
       // upper: (i,j),(i+1,j),(i+1,j+1); lower: (i,j),(i,j+1),(i+1,j+1)
-      up = dtri(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
-                sv[cr + 1][cc + 1], 0);
-      lo = dtri(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
-                sv[cr + 1][cc + 1], 1);
+      up = dtri<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
+                   sv[cr + 1][cc + 1], 0);
+      lo = dtri<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
+                   sv[cr + 1][cc + 1], 1);
     }
     tb[cr][cc][0] = up;
     tb[cr][cc][1] = lo;
@@ -228,7 +232,7 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
     if (ov.eb_out) ov.eb_out[vix] = eb;
     if (ov.gain && !(ov.fallback && ov.fallback[vix]) && ov.gain[vix])
       eb = dminf(max_pwr_eb, ldexp(eb, (int)ov.gain[vix]));
-    const float u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
+    const T u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     if (!isfinite(u) || !isfinite(v) || !(eb == eb)) bad = 1;
     if (qlog) {
       // offline_log: one code per vertex for both components, from the bound
@@ -239,9 +243,11 @@ __global__ void __launch_bounds__(256) k_eb2d(const float* __restrict__ U, const
       continue;
     }
     // the reference's eb/DBL_EPSILON overflows from 2^972 on (undefined id)
-    if ((double)fabsf(u) * eb >= 0x1p972 || (double)fabsf(v) * eb >= 0x1p972) bad = 1;
-    const uint32_t a = eb_code_2d((double)fabsf(u) * eb, q2d, nq);
-    const uint32_t b = eb_code_2d((double)fabsf(v) * eb, q2d, nq);
+    // eb_u = fabs(U[i]) * eb[i] (src/sz_compress_cp_preserve_2d.cpp:231)
+    const double au = (double)fabs(u) * eb, av = (double)fabs(v) * eb;
+    if (au >= 0x1p972 || av >= 0x1p972) bad = 1;
+    const uint32_t a = eb_code_2d(au, q2d, nq);
+    const uint32_t b = eb_code_2d(av, q2d, nq);
     cu[(size_t)i * r2 + j] = (uint16_t)a;
     cv[(size_t)i * r2 + j] = (uint16_t)b;
     atomicAdd(&sh_hist[a ? a - 1 : 0], 1u);
@@ -430,6 +436,128 @@ __global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
   }
 }
 
+// ---- T = double wavefront ----------------------------------------------------
+// The reference's double instantiation predicts and reconstructs in double
+// (pred = (float)((L + U) - UL), include/sz_compress_block_processing.hpp:127;
+// quantize / recover in double, include/sz_compression_utils.hpp:50-65,
+// include/sz_decompression_utils.hpp:63-73), so this strip wavefront carries
+// doubles: the same one-wave-per-64-row-strip structure as the float kernel,
+// the row above arriving through two 32-bit DPP shifts, the strip's last row
+// handed to the next strip through 64-bit values that replace a NaN pattern
+// no reconstruction takes (every value is finite: non-finite input is rejected).
+constexpr uint64_t kBndEmpty = 0x7FF4DEADBEEF0000ull;
+
+struct Strip2d64 {
+  const double* x[2];
+  const uint16_t* c[2];  // compress: eb codes; decompress: stream ids
+  uint16_t* q[2];
+  const double* raw[2];
+  double* out[2];
+  uint64_t* bnd;  // [2][nstrips][r2], kBndEmpty until published
+  int* ticket;
+  int r1, r2, nstrips;
+};
+
+__device__ __forceinline__ double shr1_d(double src, double lane0) {
+  const uint64_t b = (uint64_t)__double_as_longlong(src), z = (uint64_t)__double_as_longlong(lane0);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)z, (int)(uint32_t)b, 0x138, 0xF, 0xF, false);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(z >> 32), (int)(uint32_t)(b >> 32), 0x138,
+                                                            0xF, 0xF, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+__device__ __forceinline__ double readlane_d(double v, int k) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)b, k);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(b >> 32), k);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+
+// quantize<double> (include/sz_compression_utils.hpp:50-65) for eb code c (0:
+// zero precision, c >= 1: precision 4^(c-1) eps)
+__device__ __forceinline__ double l2d_compress_step64(float pred, double x, uint32_t c, uint16_t& code) {
+  code = 0;
+  if (c == 0) return x;
+  const double prec = ldexp(1.0, 2 * ((int)c - 1) - 52);
+  const double diff = x - (double)pred;
+  double qd = fabs(diff) / prec + 1;
+  if (qd < 65536) {
+    qd = (diff > 0) ? qd : -qd;
+    const int qi = (int)(qd / 2) + 32768;
+    const double dec = (double)pred + 2 * (qi - 32768) * prec;
+    if (fabs(dec - x) <= prec) {
+      code = (uint16_t)qi;
+      return dec;
+    }
+  }
+  return x;
+}
+
+template <bool COMPRESS>
+__device__ void strip_item64(const Strip2d64& a, int item, int lane) {
+  const int r1 = a.r1, r2 = a.r2;
+  const int comp = item & 1, s = item >> 1;
+  const int row = s * 64 + lane;
+  const bool rvalid = row < r1;
+  const size_t roff = (size_t)(rvalid ? row : 0) * r2;
+  const bool has_up = s > 0, publish = s + 1 < a.nstrips;
+  const uint64_t* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
+  uint64_t* bnd_out = a.bnd + ((size_t)comp * a.nstrips + s) * r2;
+  double Rprev = 0, Uprev = 0;
+  const int steps = r2 + 63;
+  for (int t0 = 0; t0 < steps; t0 += kCH) {
+    double inc = 0;
+    if (has_up && t0 < r2) {
+      const bool want = lane < kCH && t0 + lane < r2;
+      uint64_t v = want ? ld_coh64(bnd_in + t0 + lane) : 0;
+      while (!__all(!want || v != kBndEmpty)) {
+        __builtin_amdgcn_s_sleep(1);
+        if (want && v == kBndEmpty) v = ld_coh64(bnd_in + t0 + lane);
+      }
+      inc = __longlong_as_double((long long)v);
+    }
+    for (int k = 0; k < kCH && t0 + k < steps; k++) {
+      const int j = t0 + k - lane;
+      const bool act = rvalid && j >= 0 && j < r2;
+      const double lane0 = readlane_d(inc, k);
+      const double Uv = shr1_d(Rprev, lane0);
+      const float pred = (float)((Rprev + Uv) - Uprev);
+      double R;
+      if (COMPRESS) {
+        uint16_t code;
+        R = l2d_compress_step64(pred, act ? a.x[comp][roff + j] : 0.0, act ? a.c[comp][roff + j] : 0u, code);
+        if (act) a.q[comp][roff + j] = code;
+      } else {
+        const uint32_t q = act ? a.q[comp][roff + j] : 0u, sid = act ? a.c[comp][roff + j] : 0u;
+        const double prec = sid ? ldexp(1.0, 2 * (int)sid - 52) : 0.0;  // pow(4, id) * DBL_EPSILON
+        R = q == 0 ? (act ? a.raw[comp][roff + j] : 0.0) : (double)pred + 2 * ((int)q - 32768) * prec;
+        if (act) a.out[comp][roff + j] = R;
+      }
+      if (publish && lane == 63 && act) st_coh64(bnd_out + j, (uint64_t)__double_as_longlong(R));
+      Uprev = Uv;
+      Rprev = j < 0 ? 0.0 : R;
+    }
+  }
+}
+
+template <bool COMPRESS>
+__global__ void __launch_bounds__(64) k_lorenzo2d_f64(Strip2d64 a) {
+  const int lane = threadIdx.x;
+  const int items = 2 * a.nstrips;
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(a.ticket, 1);
+    item = __shfl(item, 0);
+    if (item >= items) return;
+    strip_item64<COMPRESS>(a, item, lane);
+  }
+}
+
+__global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
 __global__ void k_zero_i32(int* p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0;
@@ -543,9 +671,10 @@ __global__ void __launch_bounds__(256) k_block_zeros(const uint16_t* __restrict_
   if (lane == 0) cnt[b] = z;
 }
 
-__global__ void __launch_bounds__(256) k_gather_unpred(const uint16_t* __restrict__ q, const float* __restrict__ x,
+template <typename T>
+__global__ void __launch_bounds__(256) k_gather_unpred(const uint16_t* __restrict__ q, const T* __restrict__ x,
                                                        int r1, int r2, int nby, size_t nb,
-                                                       const uint64_t* __restrict__ off, float* __restrict__ out) {
+                                                       const uint64_t* __restrict__ off, T* __restrict__ out) {
   const size_t b = (size_t)blockIdx.x * kUnpWaves + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= nb) return;
@@ -562,10 +691,11 @@ __global__ void __launch_bounds__(256) k_gather_unpred(const uint16_t* __restric
   }
 }
 
-__global__ void __launch_bounds__(256) k_scatter_unpred(const uint16_t* __restrict__ q, const float* __restrict__ vals,
+template <typename T>
+__global__ void __launch_bounds__(256) k_scatter_unpred(const uint16_t* __restrict__ q, const T* __restrict__ vals,
                                                         int r1, int r2, int nby, size_t nb,
                                                         const uint64_t* __restrict__ off, uint64_t count,
-                                                        float* __restrict__ raw, int* __restrict__ err) {
+                                                        T* __restrict__ raw, int* __restrict__ err) {
   const size_t b = (size_t)blockIdx.x * kUnpWaves + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (b >= nb) return;
@@ -726,16 +856,21 @@ void c2_bind(C2Work& w, uint8_t* p) {
 }
 
 void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
-  CPSZ_REQUIRE(w.dU && w.dV && w.r1 >= 1 && w.r2 >= 1, kErrArg, "compress_2d: bad arguments");
+  CPSZ_REQUIRE((w.f64 ? w.dU64 && w.dV64 : w.dU && w.dV) && w.r1 >= 1 && w.r2 >= 1, kErrArg,
+               "compress_2d: bad arguments");
   CPSZ_REQUIRE(w.r1 < (1u << 30) && w.r2 < (1u << 30) && w.r1 * w.r2 < (1ull << 40), kErrUnsupported,
                "compress_2d: field too large");
   hipStream_t st = ctx.stream();
   CPSZ_HIP(hipMemsetAsync(w.hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
   CPSZ_HIP(hipMemsetAsync(w.flags, 0, 4 * sizeof(int), st));
   dim3 grid((unsigned)div_up(w.r2, kTW), (unsigned)div_up(w.r1, kTH));
-  k_eb2d<<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
-                               w.cu, w.cv, w.hist, w.flags, ov, w.log ? ctx.d_q2l() : nullptr,
-                               ctx.q2l_size());
+  if (w.f64)
+    k_eb2d<double><<<grid, 256, 0, st>>>(w.dU64, w.dV64, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
+                                         ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
+  else
+    k_eb2d<float><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
+                                        w.cu, w.cv, w.hist, w.flags, ov, w.log ? ctx.d_q2l() : nullptr,
+                                        ctx.q2l_size());
   CPSZ_HIP(hipGetLastError());
 }
 
@@ -752,7 +887,37 @@ void c2_check(Ctx& ctx, C2Work* const* w, size_t nf) {
   }
 }
 
+namespace {
+void launch_wavefront64(Ctx& ctx, Strip2d64& sa, int* prog, int grid_cap) {
+  hipStream_t st = ctx.stream();
+  sa.ticket = prog;
+  k_zero_i32<<<1, 64, 0, st>>>(prog, 1);
+  const size_t nb = (size_t)2 * sa.nstrips * sa.r2;
+  k_fill_u64<<<(int)std::min<size_t>(div_up(nb, 256), 4096), 256, 0, st>>>(sa.bnd, nb, kBndEmpty);
+  const int grid = std::min(2 * sa.nstrips, grid_cap);
+  k_lorenzo2d_f64<true><<<grid, 64, 0, st>>>(sa);
+  CPSZ_HIP(hipGetLastError());
+}
+}  // namespace
+
 void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap) {
+  if (nf == 1 && w[0]->f64) {
+    Strip2d64 sa{};
+    sa.x[0] = w[0]->dU64;
+    sa.x[1] = w[0]->dV64;
+    sa.c[0] = w[0]->cu;
+    sa.c[1] = w[0]->cv;
+    sa.q[0] = w[0]->qu;
+    sa.q[1] = w[0]->qv;
+    sa.bnd = w[0]->bnd;
+    sa.r1 = (int)w[0]->r1;
+    sa.r2 = (int)w[0]->r2;
+    sa.nstrips = (int)div_up(w[0]->r1, 64);
+    launch_wavefront64(ctx, sa, w[0]->prog, grid_cap);
+    return;
+  }
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(!w[k]->f64, kErrUnsupported, "compress_2d: double fields go one at a time");
   std::vector<Strip2d> f(nf);
   for (size_t k = 0; k < nf; k++) {
     Strip2d& sa = f[k];
@@ -868,10 +1033,18 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
                                                    bits + nch_ids, words + w_eb);
   k_encode<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
                                                    offs + nch_ids + nb, bits + nch_ids + nb, words + w_eb + w_u);
-  float* unp = ctx.buf<float>("c2.unp", unp_u + unp_v + 1);
-  k_gather_unpred<<<unp_grid(nb), 256, 0, st>>>(qu, dU, (int)r1, (int)r2, nby, nb, zoff, unp);
-  k_gather_unpred<<<unp_grid(nb), 256, 0, st>>>(qv, dV, (int)r1, (int)r2, nby, nb, zoff + nb,
-                                                        unp + unp_u);
+  // unpredictable values in the field's own type (T = double: 8 bytes each)
+  const size_t vb = w.f64 ? 8 : 4;
+  uint8_t* unp = ctx.buf<uint8_t>("c2.unp", vb * (unp_u + unp_v + 1));
+  if (w.f64) {
+    k_gather_unpred<double><<<unp_grid(nb), 256, 0, st>>>(qu, w.dU64, (int)r1, (int)r2, nby, nb, zoff, (double*)unp);
+    k_gather_unpred<double><<<unp_grid(nb), 256, 0, st>>>(qv, w.dV64, (int)r1, (int)r2, nby, nb, zoff + nb,
+                                                          (double*)unp + unp_u);
+  } else {
+    k_gather_unpred<float><<<unp_grid(nb), 256, 0, st>>>(qu, dU, (int)r1, (int)r2, nby, nb, zoff, (float*)unp);
+    k_gather_unpred<float><<<unp_grid(nb), 256, 0, st>>>(qv, dV, (int)r1, (int)r2, nby, nb, zoff + nb,
+                                                         (float*)unp + unp_u);
+  }
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
   // ---- stream layout (src/sz_compress_cp_preserve_2d.cpp:258-269, src/sz_compress_3d.cpp:219-228)
@@ -889,7 +1062,8 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
     put(h, (int)kBlock2d);
     put(h, (char)0);
     put(h, (int)kRadius);
-    put(h, (uint64_t)0);  // meanInfo<float>{false, 0}: flag byte, 3 padding bytes, float 0
+    put(h, (uint64_t)0);  // meanInfo<T>{false, 0}: flag byte, padding bytes, T 0 (8 or 16 bytes)
+    if (w.f64) put(h, (uint64_t)0);
     put(h, (size_t)0);    // reg_count
     put(h, (size_t)unp_count);
     return h;
@@ -903,8 +1077,8 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
     hv2.insert(hv2.end(), b.begin(), b.end());
   }
   const size_t eb_size = h_eb.size() + bytes_eb;
-  const size_t u_size = hu1.size() + 4 * unp_u + hu2.size() + bytes_u;
-  const size_t v_size = hv1.size() + 4 * unp_v + hv2.size() + bytes_v;
+  const size_t u_size = hu1.size() + vb * unp_u + hu2.size() + bytes_u;
+  const size_t v_size = hv1.size() + vb * unp_v + hv2.size() + bytes_v;
   std::vector<uint8_t> hdr;
   put(hdr, (int)(w.log ? 2 : 4));
   put(hdr, (double)DBL_EPSILON);
@@ -943,11 +1117,11 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
       for (int c = 0; c < 2; c++) dev_piece(w.smap + c * nq, smap);
   }
   host_piece(hu1);
-  dev_piece(unp, 4 * unp_u);
+  dev_piece(unp, vb * unp_u);
   host_piece(hu2);
   dev_piece(words + w_eb, bytes_u);
   host_piece(hv1);
-  dev_piece(unp + unp_u, 4 * unp_v);
+  dev_piece(unp + vb * unp_u, vb * unp_v);
   host_piece(hv2);
   dev_piece(words + w_eb + w_u, bytes_v);
   CPSZ_REQUIRE(pos == total, kErrInternal, "stream layout mismatch");
@@ -1069,7 +1243,7 @@ struct D2Plan {
 
 // log: the offline_log layout (base 2, the sign bitmaps between the id
 // section and the u section, src/sz_decompress_cp_preserve_2d.cpp:60-79).
-void d2_parse(StreamReader& rd, size_t r1, size_t r2, D2Plan& p, bool log = false) {
+void d2_parse(StreamReader& rd, size_t r1, size_t r2, D2Plan& p, bool log = false, size_t vb = 4) {
   const size_t n = r1 * r2;
   const int base = rd.get<int>(0);
   const double thr = rd.get<double>(4);
@@ -1094,14 +1268,15 @@ void d2_parse(StreamReader& rd, size_t r1, size_t r2, D2Plan& p, bool log = fals
     k.off = off;
     const int bs = rd.get<int>(off);
     const int radius = rd.get<int>(off + 5);
+    // meanInfo<T> is 2 * sizeof(T) bytes: bool, padding, T mean
     const uint8_t use_mean = rd.get<uint8_t>(off + 9);
-    const size_t reg = rd.get<size_t>(off + 17);
-    k.unp_count = rd.get<size_t>(off + 25);
+    const size_t reg = rd.get<size_t>(off + 9 + 2 * vb);
+    k.unp_count = rd.get<size_t>(off + 17 + 2 * vb);
     CPSZ_REQUIRE(bs == kBlock2d && radius == kRadius && use_mean == 0 && reg == 0, kErrUnsupported,
                  "decompress_2d: stream uses block settings the offline compressor never writes");
     CPSZ_REQUIRE(k.unp_count <= n, kErrStream, "bad unpredictable count");
-    k.unp_off = off + 33;
-    const size_t ind = k.unp_off + 4 * k.unp_count;
+    k.unp_off = off + 25 + 2 * vb;
+    const size_t ind = k.unp_off + vb * k.unp_count;
     const size_t ind_bytes = (p.nb - 1) / 8 + 1;
     const uint8_t* ib = rd.read(ind, ind_bytes);
     for (size_t b = 0; b < ind_bytes; b++)
@@ -1127,24 +1302,32 @@ void d2_unpred(Ctx& ctx, const uint8_t* d_in, const D2Plan& p, D2Work& w) {
   uint64_t* zoff = ctx.buf<uint64_t>("d2.zoff", 2 * nb + 4);
   CPSZ_HIP(hipMemsetAsync(w.err, 0, 8, st));
   uint16_t* qs[2] = {w.qu, w.qv};
+  const size_t vb = w.f64 ? 8 : 4;
   for (int c = 0; c < 2; c++) {
-    float* vals = ctx.buf<float>(c ? "d2.unpv" : "d2.unpu", p.comp[c].unp_count + 1);
+    uint8_t* vals = ctx.buf<uint8_t>(c ? "d2.unpv" : "d2.unpu", vb * (p.comp[c].unp_count + 1));
     if (p.comp[c].unp_count)
-      CPSZ_HIP(hipMemcpyAsync(vals, d_in + p.comp[c].unp_off, 4 * p.comp[c].unp_count, hipMemcpyDeviceToDevice, st));
+      CPSZ_HIP(hipMemcpyAsync(vals, d_in + p.comp[c].unp_off, vb * p.comp[c].unp_count, hipMemcpyDeviceToDevice,
+                              st));
     k_block_zeros<<<unp_grid(nb), 256, 0, st>>>(qs[c], (int)r1, (int)r2, p.nby, nb, zc + c * nb);
     scan_u32_to_u64(ctx, zc + c * nb, zoff + c * (nb + 1), nb, zoff + c * (nb + 1) + nb);
-    k_scatter_unpred<<<unp_grid(nb), 256, 0, st>>>(qs[c], vals, (int)r1, (int)r2, p.nby, nb,
-                                                           zoff + c * (nb + 1), p.comp[c].unp_count,
-                                                           w.raw + c * n, w.err);
+    if (w.f64)
+      k_scatter_unpred<double><<<unp_grid(nb), 256, 0, st>>>(qs[c], (const double*)vals, (int)r1, (int)r2, p.nby,
+                                                             nb, zoff + c * (nb + 1), p.comp[c].unp_count,
+                                                             (double*)w.raw + c * n, w.err);
+    else
+      k_scatter_unpred<float><<<unp_grid(nb), 256, 0, st>>>(qs[c], (const float*)vals, (int)r1, (int)r2, p.nby, nb,
+                                                            zoff + c * (nb + 1), p.comp[c].unp_count,
+                                                            w.raw + c * n, w.err);
     CPSZ_HIP(hipGetLastError());
   }
 }
 
 }  // namespace
 
-size_t d2_scratch_bytes(size_t r1, size_t r2) {
+size_t d2_scratch_bytes(size_t r1, size_t r2, bool f64) {
   const size_t n = r1 * r2, ns = div_up(r1, 64);
-  return 4 * al256(2 * n) + al256(8 * n) + al256(2 * ns * r2 * 8) + al256((2 * ns + 1) * 4) + al256(16);
+  return 4 * al256(2 * n) + al256((f64 ? 16 : 8) * n) + al256(2 * ns * r2 * 8) + al256((2 * ns + 1) * 4) +
+         al256(16);
 }
 
 void d2_bind(D2Work& w, uint8_t* p) {
@@ -1158,7 +1341,7 @@ void d2_bind(D2Work& w, uint8_t* p) {
   w.sid_v = (uint16_t*)take(2 * n);
   w.qu = (uint16_t*)take(2 * n);
   w.qv = (uint16_t*)take(2 * n);
-  w.raw = (float*)take(8 * n);
+  w.raw = (float*)take((w.f64 ? 16 : 8) * n);  // T = double: 2n doubles
   w.bnd = (uint64_t*)take(2 * ns * w.r2 * 8);
   w.prog = (int*)take((2 * ns + 1) * 4);
   w.err = (int*)take(16);
@@ -1172,7 +1355,7 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   const bool log = w.log;
   StreamReader rd(ctx, h_in, d_in, in_size);
   D2Plan p;
-  d2_parse(rd, r1, r2, p, w.log);
+  d2_parse(rd, r1, r2, p, w.log, w.f64 ? 8 : 4);
   if (w.log) w.signs = d_in + p.sign_off;
   const HuffSection& h_eb = p.h_eb;
   const int nby = p.nby;
@@ -1296,6 +1479,32 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
 }
 
 void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap) {
+  if (nf == 1 && w[0]->f64) {
+    hipStream_t st = ctx.stream();
+    const size_t n = w[0]->r1 * w[0]->r2;
+    Strip2d64 sa{};
+    sa.c[0] = w[0]->sid_u;
+    sa.c[1] = w[0]->sid_v;
+    sa.q[0] = w[0]->qu;
+    sa.q[1] = w[0]->qv;
+    sa.raw[0] = (const double*)w[0]->raw;
+    sa.raw[1] = (const double*)w[0]->raw + n;
+    sa.out[0] = w[0]->dU64;
+    sa.out[1] = w[0]->dV64;
+    sa.bnd = w[0]->bnd;
+    sa.r1 = (int)w[0]->r1;
+    sa.r2 = (int)w[0]->r2;
+    sa.nstrips = (int)div_up(w[0]->r1, 64);
+    sa.ticket = w[0]->prog;
+    k_zero_i32<<<1, 64, 0, st>>>(w[0]->prog, 1);
+    const size_t nb = (size_t)2 * sa.nstrips * sa.r2;
+    k_fill_u64<<<(int)std::min<size_t>(div_up(nb, 256), 4096), 256, 0, st>>>(sa.bnd, nb, kBndEmpty);
+    k_lorenzo2d_f64<false><<<std::min(2 * sa.nstrips, grid_cap), 64, 0, st>>>(sa);
+    CPSZ_HIP(hipGetLastError());
+    return;
+  }
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(!w[k]->f64, kErrUnsupported, "decompress_2d: double fields go one at a time");
   std::vector<Strip2d> f(nf);
   for (size_t k = 0; k < nf; k++) {
     Strip2d& sa = f[k];
@@ -1372,6 +1581,50 @@ void decompress_2d_log_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in
                                                                          ctx.libm_dev().exp2f);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("finish");
+  d2_check(ctx, &wp, 1);
+  ctx.finish_marks();
+}
+
+void compress_2d_f64_device(Ctx& ctx, const double* dU, const double* dV, size_t r1, size_t r2,
+                            double max_pwr_eb, DeviceStream& out) {
+  ctx.stages.clear();
+  ctx.mark("start");
+  CPSZ_REQUIRE(dU && dV && r1 >= 1 && r2 >= 1 && r1 < (1u << 30) && r2 < (1u << 30), kErrArg,
+               "compress_2d (double): bad arguments");
+  C2Work w;
+  w.f64 = true;
+  w.dU64 = dU;
+  w.dV64 = dV;
+  w.r1 = r1;
+  w.r2 = r2;
+  c2_bind(w, ctx.buf<uint8_t>("c2.scratch", c2_scratch_bytes(r1, r2)));
+  c2_eb(ctx, w, max_pwr_eb, EbOverride());
+  ctx.mark("eb");
+  C2Work* wp = &w;
+  c2_check(ctx, &wp, 1);
+  c2_wavefront(ctx, &wp, 1, 4096);
+  ctx.mark("lorenzo");
+  c2_entropy(ctx, w, out);
+  ctx.finish_marks();
+}
+
+void decompress_2d_f64_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, size_t r1,
+                              size_t r2, double* dU, double* dV) {
+  CPSZ_REQUIRE(d_in && dU && dV && r1 >= 1 && r2 >= 1, kErrArg, "decompress_2d (double): bad arguments");
+  ctx.stages.clear();
+  ctx.mark("start");
+  D2Work w;
+  w.r1 = r1;
+  w.r2 = r2;
+  w.f64 = true;
+  w.dU64 = dU;
+  w.dV64 = dV;
+  d2_bind(w, ctx.buf<uint8_t>("d2.scratch", d2_scratch_bytes(r1, r2, true)));
+  CPSZ_HIP(hipMemsetAsync(w.err, 0, 16, ctx.stream()));
+  d2_entropy(ctx, d_in, h_in, in_size, w);
+  D2Work* wp = &w;
+  d2_wavefront(ctx, &wp, 1, 4096);
+  ctx.mark("lorenzo");
   d2_check(ctx, &wp, 1);
   ctx.finish_marks();
 }

@@ -45,6 +45,10 @@ struct C2Work {
   const float* lu = nullptr;
   const float* lv = nullptr;
   const uint64_t* smap = nullptr;
+  // T = double (the reference's double instantiation): the fields are dU64/dV64
+  bool f64 = false;
+  const double* dU64 = nullptr;
+  const double* dV64 = nullptr;
 };
 size_t c2_scratch_bytes(size_t r1, size_t r2);
 void c2_bind(C2Work& w, uint8_t* scratch);  // carves c2_scratch_bytes of device memory
@@ -64,9 +68,12 @@ struct D2Work {
   bool wide = false;
   bool log = false;               // offline_log stream: ids shared by u and v, log values
   const uint8_t* signs = nullptr;  // offline_log: the stream's two sign bitmaps (device)
+  // T = double: outputs dU64/dV64, raw holds 2n doubles (d2_scratch_bytes(.., true))
+  bool f64 = false;
+  double *dU64 = nullptr, *dV64 = nullptr;
 };
-size_t d2_scratch_bytes(size_t r1, size_t r2);
-void d2_bind(D2Work& w, uint8_t* scratch);
+size_t d2_scratch_bytes(size_t r1, size_t r2, bool f64 = false);
+void d2_bind(D2Work& w, uint8_t* scratch);  // carves d2_scratch_bytes(r1, r2, w.f64)
 void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, D2Work& w);
 // The entropy stage of many fields at once (headers and trees on `workers`
 // pool threads, every field's bitstreams decoded together on ctx's stream).
@@ -82,6 +89,15 @@ void compress_2d_log_device(Ctx& ctx, const float* dU, const float* dV, size_t r
                             double max_pwr_eb, DeviceStream& out);
 void decompress_2d_log_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, size_t r1,
                               size_t r2, float* dU, float* dV);
+
+// The T = double instantiation of the offline pair (the reference's
+// src/sz_compress_cp_preserve_2d.cpp:280-282, src/sz_decompress_cp_preserve_2d.cpp:46-52):
+// products, predictions and reconstructions in double as the reference's
+// double code computes them, double unpredictable values in the stream.
+void compress_2d_f64_device(Ctx& ctx, const double* dU, const double* dV, size_t r1, size_t r2,
+                            double max_pwr_eb, DeviceStream& out);
+void decompress_2d_f64_device(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_size, size_t r1,
+                              size_t r2, double* dU, double* dV);
 
 // Analytic per-vertex bound of the plain method (before quantization).
 void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,

@@ -177,6 +177,181 @@ __global__ void __launch_bounds__(256) k_gemm_bf16(int M, int N, int K, const ui
   }
 }
 
+// ------------------------------------------------------ fused inference --
+// Feature build + both hidden layers + output in one kernel, for compression
+// (training keeps the GEMM path, whose activations the backward pass needs).
+// A wave takes 32 vertices at a time: its lanes write the BF16 features of the
+// 5x5 (2D) / 3x3x3 (3D) patch into the wave's LDS tile (lane t: vertex t%32,
+// features of half t/32), then layer 1 and layer 2 run on
+// v_mfma_f32_32x32x16_bf16 with the transposed BF16 weights resident in LDS,
+// bias + ReLU between them in FP32, and the output layer reduces the 32 lanes
+// that share a row. The hidden rows overwrite the feature tile in place (a
+// wave's LDS reads complete before its later writes). Operand layout: lane t
+// supplies A(t%32, 8*(t/32)+q) and B(8*(t/32)+q, t%32), q = 0..7, and
+// receives D(8*(r/4) + 4*(t/32) + r%4, t%32) in accumulator r.
+typedef __bf16 bf16x8m __attribute__((ext_vector_type(8)));
+
+template <int DIM>
+struct InferCfg;
+template <>
+struct InferCfg<2> {
+  static constexpr int F = kFeatDim, P = kFeatUsed - 1, WPB = 8;  // 50 patch values, then the bound
+};
+template <>
+struct InferCfg<3> {
+  static constexpr int F = kFeatDim3, P = kFeatUsed3 - 1, WPB = 8;  // 81 patch values, then the bound
+};
+
+__device__ __forceinline__ bf16x8m lds_bf16x8(const uint16_t* p) {
+  bf16x8m v;
+  __builtin_memcpy(&v, p, 16);
+  return v;
+}
+
+// W1t[n][k] = bf16(W1[k][n]), W2t[n][k] = bf16(W2[k][n])
+__global__ void k_weights_t(const float* __restrict__ P, int fdim, size_t w1, size_t w2, uint16_t* __restrict__ W1t,
+                            uint16_t* __restrict__ W2t) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  const size_t n1 = (size_t)kHidden * fdim;
+  if (i < n1) {
+    const size_t n = i / fdim, k = i - n * fdim;
+    W1t[i] = f2bf(P[w1 + k * kHidden + n]);
+  } else if (i < n1 + (size_t)kHidden * kHidden) {
+    const size_t j = i - n1, n = j / kHidden, k = j - n * kHidden;
+    W2t[j] = f2bf(P[w2 + k * kHidden + n]);
+  }
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(64 * InferCfg<DIM>::WPB) k_mlp_infer(
+    const float* __restrict__ U, const float* __restrict__ V, const float* __restrict__ W,
+    const double* __restrict__ E, int r1, int r2, int r3, double max_pwr_eb, const uint16_t* __restrict__ W1t,
+    const uint16_t* __restrict__ W2t, const float* __restrict__ P, NetLayout L, size_t v0, size_t m,
+    uint8_t* __restrict__ gain) {
+  constexpr int F = InferCfg<DIM>::F, NP = InferCfg<DIM>::P, WPB = InferCfg<DIM>::WPB, HALF = F / 2;
+  __shared__ __align__(16) uint16_t sW1[kHidden * F];
+  __shared__ __align__(16) uint16_t sW2[kHidden * kHidden];
+  __shared__ float sb1[kHidden], sb2[kHidden], sw3[kHidden];
+  __shared__ __align__(16) uint16_t sbuf[WPB][32 * kHidden];  // features, then the hidden rows
+  for (int i = threadIdx.x; i < kHidden * F; i += blockDim.x) sW1[i] = W1t[i];
+  for (int i = threadIdx.x; i < kHidden * kHidden; i += blockDim.x) sW2[i] = W2t[i];
+  for (int i = threadIdx.x; i < kHidden; i += blockDim.x) {
+    sb1[i] = P[L.b1 + i];
+    sb2[i] = P[L.b2 + i];
+    sw3[i] = P[L.w3 + i];
+  }
+  __syncthreads();
+  const float b3 = P[L.b3];
+  const int wave = threadIdx.x >> 6, t = threadIdx.x & 63, col = t & 31, hf = t >> 5;
+  uint16_t* buf = sbuf[wave];
+  const float* src[3] = {U, V, W};
+  for (size_t tile = (size_t)blockIdx.x * WPB + wave; tile * 32 < m; tile += (size_t)gridDim.x * WPB) {
+    // ---- features (as k_feat2d / k_feat3d write them)
+    const size_t q = tile * 32 + col;
+    const bool valid = q < m;
+    const size_t v = v0 + (valid ? q : 0);
+    int i, j, k = 0;
+    if (DIM == 2) {
+      i = (int)(v / (size_t)r2);
+      j = (int)(v - (size_t)i * r2);
+    } else {
+      const size_t d0 = (size_t)r2 * r3;
+      i = (int)(v / d0);
+      const size_t rem = v - (size_t)i * d0;
+      j = (int)(rem / (size_t)r3);
+      k = (int)(rem - (size_t)j * r3);
+    }
+    // patch value of feature f (0 outside the field); the largest magnitude
+    // first, then the values again (cached) for the normalized BF16 features
+    auto patch = [&](int f) -> float {
+      int c;
+      bool in;
+      size_t o;
+      if (DIM == 2) {
+        c = f / 25;
+        const int a = (f % 25) / 5, b = f % 5, ii = i + a - 2, jj = j + b - 2;
+        in = ii >= 0 && ii < r1 && jj >= 0 && jj < r2;
+        o = in ? (size_t)ii * r2 + jj : 0;
+      } else {
+        c = f / 27;
+        const int oo = f % 27, ii = i + oo / 9 - 1, jj = j + (oo / 3) % 3 - 1, kk = k + oo % 3 - 1;
+        in = ii >= 0 && ii < r1 && jj >= 0 && jj < r2 && kk >= 0 && kk < r3;
+        o = in ? ((size_t)ii * r2 + jj) * r3 + kk : 0;
+      }
+      return in ? src[c][o] : 0.f;
+    };
+    float smax = 0.f;
+    for (int ff = 0; ff < HALF; ff++) {
+      const int f = hf * HALF + ff;
+      if (f < NP) smax = fmaxf(smax, fabsf(patch(f)));
+    }
+    smax = fmaxf(smax, __shfl_xor(smax, 32));
+    if (!(smax > 0.f)) smax = 1.f;
+    const double e = E[v];
+    const float le = e > 0 ? fmaxf((float)log2(e / max_pwr_eb), -64.f) / 16.f : -4.f;
+    for (int ff = 0; ff < HALF; ff++) {
+      const int f = hf * HALF + ff;
+      const uint16_t h = f < NP ? f2bf(patch(f) / smax) : (f == NP ? f2bf(le) : (uint16_t)0);
+      buf[col * F + f] = valid ? h : (uint16_t)0;
+    }
+    // ---- layer 1: Z1 = X W1 (K = F)
+    floatx16 acc[4];
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+      for (int r = 0; r < 16; r++) acc[nt][r] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < F / 16; ks++) {
+      const bf16x8m a = lds_bf16x8(buf + col * F + ks * 16 + hf * 8);
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const bf16x8m b = lds_bf16x8(sW1 + (nt * 32 + col) * F + ks * 16 + hf * 8);
+        acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nt], 0, 0, 0);
+      }
+    }
+    // ---- H1 = relu(Z1 + b1) as BF16, over the feature tile
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) {
+        const int row = 8 * (r >> 2) + 4 * hf + (r & 3), c = nt * 32 + col;
+        const float z = acc[nt][r] + sb1[c];
+        buf[row * kHidden + c] = f2bf(z > 0.f ? z : 0.f);
+      }
+    // ---- layer 2: Z2 = H1 W2 (K = 128)
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++)
+      for (int r = 0; r < 16; r++) acc[nt][r] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < kHidden / 16; ks++) {
+      const bf16x8m a = lds_bf16x8(buf + col * kHidden + ks * 16 + hf * 8);
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const bf16x8m b = lds_bf16x8(sW2 + (nt * 32 + col) * kHidden + ks * 16 + hf * 8);
+        acc[nt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[nt], 0, 0, 0);
+      }
+    }
+    // ---- y = b3 + sum_c relu(Z2 + b2)[c] w3[c]; the 32 lanes of a half share
+    // rows, so each row's partial sums are reduced across them; lane col = r
+    // writes row 8*(r/4) + 4*hf + r%4 of its half
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+      float sacc = 0.f;
+#pragma unroll
+      for (int nt = 0; nt < 4; nt++) {
+        const int c = nt * 32 + col;
+        const float z = acc[nt][r] + sb2[c];
+        sacc = fmaf(z > 0.f ? z : 0.f, sw3[c], sacc);
+      }
+      for (int o = 1; o < 32; o <<= 1) sacc += __shfl_xor(sacc, o);
+      const size_t row = tile * 32 + 8 * (r >> 2) + 4 * hf + (r & 3);
+      if (col == r && row < m) {  // the proposal gain = clamp(floor(y), 0, kMaxGain), as k_gain
+        const float f = floorf(sacc + b3);
+        gain[row] = (uint8_t)(f > 0.f ? (f < (float)kMaxGain ? f : (float)kMaxGain) : 0.f);
+      }
+    }
+  }
+}
+
 // ----------------------------------------------------------- elementwise --
 __global__ void k_to_bf16(const float* __restrict__ x, size_t n, uint16_t* __restrict__ y) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
@@ -395,6 +570,32 @@ void features_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, s
   CPSZ_HIP(hipGetLastError());
 }
 
+void mlp_infer_gains(Ctx& ctx, int dim, const float* dU, const float* dV, const float* dW, const double* dE,
+                     size_t r1, size_t r2, size_t r3, double max_pwr_eb, const float* d_params, size_t v0, size_t m,
+                     uint8_t* d_gain) {
+  CPSZ_REQUIRE(dim == 2 || dim == 3, kErrArg, "mlp_infer_gains: dim must be 2 or 3");
+  CPSZ_REQUIRE(dU && dV && (dim == 2 || dW) && dE && d_params && d_gain, kErrArg, "mlp_infer_gains: bad arguments");
+  if (!m) return;
+  hipStream_t st = ctx.stream();
+  const int fdim = dim == 2 ? kFeatDim : kFeatDim3;
+  const NetLayout L = net_layout(fdim);
+  uint16_t* W1t = ctx.buf<uint16_t>("inf.w1t", (size_t)kHidden * fdim);
+  uint16_t* W2t = ctx.buf<uint16_t>("inf.w2t", (size_t)kHidden * kHidden);
+  const size_t nw = (size_t)kHidden * fdim + (size_t)kHidden * kHidden;
+  k_weights_t<<<blocks_for(nw), 256, 0, st>>>(d_params, fdim, L.w1, L.w2, W1t, W2t);
+  const size_t tiles = div_up(m, (size_t)32);
+  if (dim == 2) {
+    const int grid = (int)std::min<size_t>(div_up(tiles, (size_t)InferCfg<2>::WPB), 2048);
+    k_mlp_infer<2><<<grid, 64 * InferCfg<2>::WPB, 0, st>>>(dU, dV, nullptr, dE, (int)r1, (int)r2, 1, max_pwr_eb,
+                                                          W1t, W2t, d_params, L, v0, m, d_gain);
+  } else {
+    const int grid = (int)std::min<size_t>(div_up(tiles, (size_t)InferCfg<3>::WPB), 2048);
+    k_mlp_infer<3><<<grid, 64 * InferCfg<3>::WPB, 0, st>>>(dU, dV, dW, dE, (int)r1, (int)r2, (int)r3, max_pwr_eb,
+                                                          W1t, W2t, d_params, L, v0, m, d_gain);
+  }
+  CPSZ_HIP(hipGetLastError());
+}
+
 void labels_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, size_t r1, size_t r2,
                double max_pwr_eb, const uint32_t* d_idx, size_t m, uint8_t* dL) {
   CPSZ_REQUIRE(dU && dV && dE && dL && r1 * r2 < (1ull << 32), kErrArg, "labels_2d: bad arguments");
@@ -548,15 +749,10 @@ void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, 
   stats = LearnedStats();
   double* E = ctx.buf<double>("ln.e", n);
   analytic_eb_2d(ctx, dU, dV, r1, r2, max_pwr_eb, E);
-  uint16_t* X = ctx.buf<uint16_t>("ln.x", n * kFeatDim);
-  features_2d(ctx, dU, dV, E, r1, r2, max_pwr_eb, nullptr, n, X);
   float* P = ctx.buf<float>("ln.p", kNumParams);
   CPSZ_HIP(hipMemcpyAsync(P, params, kNumParams * 4, hipMemcpyHostToDevice, st));
-  float* y = ctx.buf<float>("ln.y", n);
-  mlp_forward(ctx, P, X, n, y);
   uint8_t* gain = ctx.buf<uint8_t>("ln.gain", n);
-  k_gain<<<blocks_for(n), 256, 0, st>>>(y, n, gain);
-  CPSZ_HIP(hipGetLastError());
+  mlp_infer_gains(ctx, 2, dU, dV, nullptr, E, r1, r2, 1, max_pwr_eb, P, 0, n, gain);
   std::vector<uint8_t> hg(n), hf(n, 0);
   CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, n, hipMemcpyDeviceToHost, st));
   std::vector<int64_t> id0, id1;
@@ -858,20 +1054,11 @@ void compress_3d_learned_batch(Ctx& ctx, const std::vector<TrainField3>& fields,
   uint8_t* gain = ctx.buf<uint8_t>("l3.gain", ntot);
   uint8_t* fb = ctx.buf<uint8_t>("l3.fb", ntot);
   double* E = ctx.buf<double>("l3.e", maxn);
-  const size_t chunk = std::min<size_t>(maxn, 1u << 21);
-  uint16_t* X = ctx.buf<uint16_t>("l3.x", chunk * kFeatDim3);
-  float* y = ctx.buf<float>("l3.y", chunk);
   for (size_t k = 0; k < nf; k++) {
     const TrainField3& F = fields[k];
     const size_t n = nofs[k + 1] - nofs[k];
     analytic_eb_3d(ctx, F.dU, F.dV, F.dW, F.r1, F.r2, F.r3, max_pwr_eb, E);
-    for (size_t v0 = 0; v0 < n; v0 += chunk) {
-      const size_t mm = std::min(chunk, n - v0);
-      features_3d_range(ctx, F.dU, F.dV, F.dW, E, F.r1, F.r2, F.r3, max_pwr_eb, nullptr, v0, mm, X);
-      mlp_forward(ctx, P, X, mm, y, kFeatDim3);
-      k_gain<<<blocks_for(mm), 256, 0, st>>>(y, mm, gain + nofs[k] + v0);
-      CPSZ_HIP(hipGetLastError());
-    }
+    mlp_infer_gains(ctx, 3, F.dU, F.dV, F.dW, E, F.r1, F.r2, F.r3, max_pwr_eb, P, 0, n, gain + nofs[k]);
   }
   std::vector<uint8_t> hg(ntot), hf(ntot, 0);
   CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, ntot, hipMemcpyDeviceToHost, st));
@@ -991,22 +1178,11 @@ void compress_2d_learned_batch(Ctx& ctx, const std::vector<TrainField>& fields, 
   uint8_t* gain = ctx.buf<uint8_t>("l2b.gain", ntot);
   uint8_t* fb = ctx.buf<uint8_t>("l2b.fb", ntot);
   double* E = ctx.buf<double>("l2b.e", maxn);
-  const size_t chunk = std::min<size_t>(maxn, 1u << 21);
-  uint16_t* X = ctx.buf<uint16_t>("l2b.x", chunk * kFeatDim);
-  float* y = ctx.buf<float>("l2b.y", chunk);
   for (size_t k = 0; k < nf; k++) {
     const TrainField& F = fields[k];
     const size_t n = nofs[k + 1] - nofs[k];
     analytic_eb_2d(ctx, F.dU, F.dV, F.r1, F.r2, max_pwr_eb, E);
-    for (size_t v0 = 0; v0 < n; v0 += chunk) {
-      const size_t mm = std::min(chunk, n - v0);
-      k_feat2d<<<blocks_for(mm), 256, 0, st>>>(F.dU, F.dV, E, (int)F.r1, (int)F.r2, max_pwr_eb, nullptr, v0, mm,
-                                               X);
-      CPSZ_HIP(hipGetLastError());
-      mlp_forward(ctx, P, X, mm, y);
-      k_gain<<<blocks_for(mm), 256, 0, st>>>(y, mm, gain + nofs[k] + v0);
-      CPSZ_HIP(hipGetLastError());
-    }
+    mlp_infer_gains(ctx, 2, F.dU, F.dV, nullptr, E, F.r1, F.r2, 1, max_pwr_eb, P, 0, n, gain + nofs[k]);
   }
   std::vector<uint8_t> hg(ntot), hf(ntot, 0);
   CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, ntot, hipMemcpyDeviceToHost, st));

@@ -67,6 +67,15 @@ void features_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, s
 void labels_2d(Ctx& ctx, const float* dU, const float* dV, const double* dE, size_t r1, size_t r2,
                double max_pwr_eb, const uint32_t* d_idx, size_t m, uint8_t* dL);
 
+// Proposal gains of vertices v0 .. v0+m (row-major) in one fused kernel:
+// features from the field and the analytic bound dE, both hidden layers on
+// MFMA with the weights in LDS, gain = clamp(floor(network output), 0, 24)
+// into d_gain[0..m). dim 2 uses (dU, dV) and the 51-feature net, dim 3
+// (dU, dV, dW) and the 82-feature net.
+void mlp_infer_gains(Ctx& ctx, int dim, const float* dU, const float* dV, const float* dW, const double* dE,
+                     size_t r1, size_t r2, size_t r3, double max_pwr_eb, const float* d_params, size_t v0, size_t m,
+                     uint8_t* d_gain);
+
 // C[M][N] (fp32, row-major ldc) = sum_k A(m,k) B(k,n) with BF16 operands
 // A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn]; M, N multiples of 32,
 // K a multiple of 8. split > 1 sums K slices with atomics into a zeroed C.

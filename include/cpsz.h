@@ -56,6 +56,20 @@ int cpsz_compress_2d_f32(const float* U, const float* V, size_t r1, size_t r2, d
 int cpsz_decompress_2d_f32(const unsigned char* in, size_t in_size, size_t r1, size_t r2,
                            float* U, float* V);
 
+/* Replace sz_compress_cp_preserve_2d_offline<double> and
+ * sz_decompress_cp_preserve_2d_offline<double> (reference explicit
+ * instantiations src/sz_compress_cp_preserve_2d.cpp:280-282,
+ * src/sz_decompress_cp_preserve_2d.cpp:49-52): the double arithmetic of the
+ * reference's double code, 8-byte unpredictable values in the stream. */
+int cpsz_compress_2d_f64(const double* U, const double* V, size_t r1, size_t r2, double max_pwr_eb,
+                         unsigned char** out, size_t* out_size);
+int cpsz_decompress_2d_f64(const unsigned char* in, size_t in_size, size_t r1, size_t r2, double* U,
+                           double* V);
+int cpsz_compress_2d_f64_dev(int device, void* stream, const double* dU, const double* dV, size_t r1,
+                             size_t r2, double max_pwr_eb, const unsigned char** d_out, size_t* out_size);
+int cpsz_decompress_2d_f64_dev(int device, void* stream, const unsigned char* d_in, size_t in_size,
+                               size_t r1, size_t r2, double* dU, double* dV);
+
 /* Replaces sz_compress_cp_preserve_2d_offline_log<float>
  * (reference include/sz_compress_cp_preserve_2d.hpp:12-14; implementation
  * src/sz_compress_cp_preserve_2d.cpp:285-374): the offline bounds in the log
@@ -321,6 +335,15 @@ int cpsz_learn_train_3d_dev(int device, void* stream, size_t nfields, const floa
                             const float* const* dV, const float* const* dW, const size_t* r1, const size_t* r2,
                             const size_t* r3, double max_pwr_eb, size_t samples_per_field, int epochs, int batch,
                             float lr, uint64_t seed, float* params, float* loss_per_epoch);
+/* Proposal gains of vertices v0 .. v0+m-1 from one fused kernel (features,
+ * both hidden layers on BF16 MFMA with FP32 accumulation, output): gain =
+ * clamp(floor(net output), 0, 24) into d_gain[0..m). dim 2: the 51-feature
+ * net on (dU, dV); dim 3: the 82-feature net on (dU, dV, dW). dE is the
+ * analytic bound per vertex (cpsz_learn_analytic_eb_*_dev). No reference
+ * counterpart. */
+int cpsz_learn_infer_gains_dev(int device, void* stream, int dim, const float* dU, const float* dV,
+                               const float* dW, const double* dE, size_t r1, size_t r2, size_t r3,
+                               double max_pwr_eb, const float* params, size_t v0, size_t m, uint8_t* d_gain);
 /* Learned-bound compression of a batch of 2D fields (no reference
  * counterpart): the single-field learned compressor over many fields, each
  * verification round one batched compress and decompress of the fields still

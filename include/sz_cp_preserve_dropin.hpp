@@ -16,9 +16,11 @@
 // Streams are byte-compatible with the reference in both directions.
 //
 // Differences a caller can observe:
-//  * float only: the double instantiations are declared but fail (return
-//    nullptr with compressed_size = 0, or leave the outputs null), because the
-//    GPU kernels compute in float as the reference's float path does;
+//  * the 2D pair has both instantiations (float and double); the 3D pair is
+//    float only: its double instantiations are declared but fail (return
+//    nullptr with compressed_size = 0, or leave the outputs null), because
+//    the reference's double 3D decoder applies the C library's double exp2,
+//    whose rounding the device does not reproduce;
 //  * failures (no GPU, non-finite input, corrupt stream) are reported as
 //    nullptr / compressed_size = 0 / null outputs instead of the reference's
 //    exit(0) or silent garbage; cpsz_last_error() gives the message;
@@ -80,11 +82,17 @@ inline unsigned char* sz_compress_cp_preserve_2d_offline<float>(const float* U, 
 }
 
 template <>
-inline unsigned char* sz_compress_cp_preserve_2d_offline<double>(const double*, const double*, size_t,
-                                                                 size_t, size_t& compressed_size, bool,
-                                                                 double) {
-  compressed_size = 0;
-  return nullptr;
+inline unsigned char* sz_compress_cp_preserve_2d_offline<double>(const double* U, const double* V, size_t r1,
+                                                                 size_t r2, size_t& compressed_size, bool,
+                                                                 double max_pwr_eb) {
+  unsigned char* out = nullptr;
+  size_t size = 0;
+  if (cpsz_compress_2d_f64(U, V, r1, r2, max_pwr_eb, &out, &size) != CPSZ_OK) {
+    compressed_size = 0;
+    return nullptr;
+  }
+  compressed_size = size;
+  return out;
 }
 
 // The reference frees incoming non-null outputs and returns malloc'd ones
@@ -108,10 +116,21 @@ inline void sz_decompress_cp_preserve_2d_offline<float>(const unsigned char* com
 }
 
 template <>
-inline void sz_decompress_cp_preserve_2d_offline<double>(const unsigned char*, size_t, size_t, double*& U,
-                                                         double*& V) {
+inline void sz_decompress_cp_preserve_2d_offline<double>(const unsigned char* compressed, size_t r1, size_t r2,
+                                                         double*& U, double*& V) {
   cpsz_dropin::release(U);
   cpsz_dropin::release(V);
+  size_t size = 0;
+  if (cpsz_stream_size_2d(compressed, &size) != CPSZ_OK) return;
+  double* u = cpsz_dropin::fresh<double>(r1 * r2);
+  double* v = cpsz_dropin::fresh<double>(r1 * r2);
+  if (!u || !v || cpsz_decompress_2d_f64(compressed, size, r1, r2, u, v) != CPSZ_OK) {
+    std::free(u);
+    std::free(v);
+    return;
+  }
+  U = u;
+  V = v;
 }
 
 template <>

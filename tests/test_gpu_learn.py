@@ -168,3 +168,61 @@ def test_learned_batch_equals_single_field_learned(trained):
         one = np.empty(n1, np.uint8)
         hip.hipMemcpy(C.c_void_p(one.ctypes.data), C.c_void_p(ptr), C.c_size_t(n1), 2)
         assert np.array_equal(got, one), k
+
+
+def _infer(dim, fields, dE, shape, eb, params):
+    import ctypes as C
+    n = int(np.prod(shape))
+    g = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    r3 = shape[2] if dim == 3 else 1
+    ptrs = [f.data_ptr() for f in fields] + ([None] if dim == 2 else [])
+    rc = cz.lib().cpsz_learn_infer_gains_dev(0, None, dim, *ptrs, dE.data_ptr(), shape[0], shape[1], r3, eb,
+                                             params.ctypes.data, 0, n, g.data_ptr())
+    assert rc == 0, cz.lib().cpsz_last_error()
+    torch.cuda.synchronize()
+    return g.cpu().numpy().astype(np.int32)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_fused_inference_matches_gemm_path(trained, dim):
+    """The fused MFMA inference kernel gives the gains of the feature kernel +
+    GEMM forward + floor path; summation orders differ (MFMA K-slices of 16
+    vs 8, the output reduction), so a gain may differ by one where the output
+    lies within a few ulps of an integer."""
+    from cpsz_amd.synth import field_3d
+    if dim == 2:
+        shape = (300, 400)
+        p = trained[0]
+        fields = [_dev(x) for x in field_2d(*shape, 77)]
+        n = shape[0] * shape[1]
+        dE = torch.empty(n, dtype=torch.float64, device="cuda")
+        cz.learn_analytic_eb_2d_dev(fields[0].data_ptr(), fields[1].data_ptr(), *shape, 0.1, dE.data_ptr())
+        torch.cuda.synchronize()
+        X = torch.empty(n * 64, dtype=torch.int16, device="cuda")
+        cz.learn_features_2d_dev(fields[0].data_ptr(), fields[1].data_ptr(), dE.data_ptr(), *shape, 0.1, 0, n,
+                                 X.data_ptr())
+        y = torch.empty(n, dtype=torch.float32, device="cuda")
+        cz.mlp_forward_dev(p, X.data_ptr(), n, y.data_ptr())
+        eb = 0.1
+    else:
+        shape = (40, 50, 60)
+        p = cz.learn_init_params_3d(3)
+        p[-1] = 6.0  # the output bias: gains spread over several integers
+        fields = [_dev(x) for x in field_3d(*shape, 5)]
+        n = int(np.prod(shape))
+        dE = torch.empty(n, dtype=torch.float64, device="cuda")
+        cz.learn_analytic_eb_3d_dev(*(f.data_ptr() for f in fields), *shape, 0.05, dE.data_ptr())
+        torch.cuda.synchronize()
+        X = torch.empty(n * 96, dtype=torch.int16, device="cuda")
+        cz.learn_features_3d_dev(*(f.data_ptr() for f in fields), dE.data_ptr(), *shape, 0.05, 0, n, X.data_ptr())
+        y = torch.empty(n, dtype=torch.float32, device="cuda")
+        cz.mlp_forward_3d_dev(p, X.data_ptr(), n, y.data_ptr())
+        eb = 0.05
+    torch.cuda.synchronize()
+    want = np.clip(np.floor(y.cpu().numpy()), 0, 24).astype(np.int32)
+    got = _infer(dim, fields, dE, shape, eb, p)
+    diff = np.abs(got - want)
+    print(f"[fused {dim}D] {n} vertices, {int((diff > 0).sum())} gains differ, gains in [{got.min()}, {got.max()}]",
+          flush=True)
+    assert diff.max() <= 1 and (diff > 0).mean() < 1e-3
+    assert len(np.unique(got)) >= 3
