@@ -19,6 +19,7 @@
 // include/sz_compression_utils.hpp:50-65); the Lorenzo step runs in FP32, which
 // reproduces the reference's double arithmetic exactly (the argument is in the
 // comments of lorenzo2d_step.hpp).
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -304,7 +305,9 @@ struct StripChunk {
   float raws[kCH];
 };
 
-template <bool COMPRESS, bool WIDE>
+// SHIFT (a.shift as a compile-time constant) keeps the quantum's exponent a
+// shift of the id: a runtime factor costs an integer multiply per element.
+template <bool COMPRESS, bool WIDE, int SHIFT>
 __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane) {
   const int r1 = a.r1, r2 = a.r2;
   const int comp = item & 1, s = item >> 1;
@@ -338,8 +341,18 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
         B.xs[k] = act ? X[j] : 0.f;
       } else {
         B.qs[k] = act ? Q[j] : 0;
-        B.raws[k] = act ? RAW[j] : 0.f;
       }
+    }
+  };
+  // decompress: the unpredictable values are read only where the code is 0
+  // (a few percent of elements), once the chunk's codes have arrived
+  auto load_raw = [&](StripChunk& B, int t0, auto inner) {
+    constexpr bool IN = decltype(inner)::value;
+#pragma unroll
+    for (int k = 0; k < kCH; k++) {
+      const int j = t0 + k - lane;
+      const bool act = IN || (rvalid && j >= 0 && j < r2);
+      B.raws[k] = (act && B.qs[k] == 0) ? RAW[j] : 0.f;
     }
   };
   auto chunk = [&](const StripChunk& B, StripChunk& N, int t0, auto inner) {
@@ -354,12 +367,17 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       }
       inc = __uint_as_float((uint32_t)v);
     }
-    if (t0 + kCH < steps) {
-      if (interior(t0 + kCH)) load(N, t0 + kCH, std::true_type{});
+    const bool more = t0 + kCH < steps, next_in = interior(t0 + kCH);
+    if (more) {
+      if (next_in) load(N, t0 + kCH, std::true_type{});
       else load(N, t0 + kCH, std::false_type{});
     }
 #pragma unroll
     for (int k = 0; k < kCH; k++) {
+      if (!COMPRESS && k == kCH / 2 && more) {
+        if (next_in) load_raw(N, t0 + kCH, std::true_type{});
+        else load_raw(N, t0 + kCH, std::false_type{});
+      }
       const int t = t0 + k;
       const int j = t - lane;
       const bool act = IN || (rvalid && j >= 0 && j < r2);
@@ -369,10 +387,10 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       float R;
       if (COMPRESS) {
         uint16_t code;
-        R = l2d_compress_step<WIDE>(pred, B.xs[k], B.cs[k], code, a.shift);
+        R = l2d_compress_step<WIDE>(pred, B.xs[k], B.cs[k], code, SHIFT);
         if (act) Q[j] = code;
       } else {
-        R = l2d_decompress_step<WIDE>(pred, B.qs[k], B.cs[k], B.raws[k], a.shift);
+        R = l2d_decompress_step<WIDE>(pred, B.qs[k], B.cs[k], B.raws[k], SHIFT);
         if (act) OUT[j] = R;
       }
       if (publish && lane == 63 && act) st_coh64(bnd_out + j, kBndTag | __float_as_uint(R));
@@ -386,6 +404,7 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
   };
   StripChunk A, B;
   load(A, 0, std::false_type{});
+  if (!COMPRESS) load_raw(A, 0, std::false_type{});
   for (int t0 = 0; t0 < steps; t0 += 2 * kCH) {
     run(A, B, t0);
     if (t0 + kCH < steps) run(B, A, t0 + kCH);
@@ -401,7 +420,9 @@ __global__ void __launch_bounds__(64) k_lorenzo2d(Strip2d a) {
     if (lane == 0) item = atomicAdd(a.ticket, 1);
     item = __shfl(item, 0);
     if (item >= items) return;
-    strip_item<COMPRESS, WIDE>(a, item, lane);
+    if (WIDE) strip_item<COMPRESS, WIDE, 2>(a, item, lane);
+    else if (a.shift == 1) strip_item<COMPRESS, false, 1>(a, item, lane);
+    else strip_item<COMPRESS, false, 2>(a, item, lane);
   }
 }
 
@@ -415,6 +436,17 @@ struct StripBatch {
   int nf, items;
   int* ticket;
 };
+
+// Clears every field's boundary rows and counters (block row y = field) and
+// the batch ticket, from the field table already on the device.
+__global__ void __launch_bounds__(256) k_zero_fields(const Strip2d* __restrict__ f, int* __restrict__ ticket) {
+  const Strip2d& a = f[blockIdx.y];
+  const size_t nb = (size_t)2 * a.nstrips * a.r2 * 2, nz = 2 * (size_t)a.nstrips + 1;
+  const size_t step = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nb; i += step) ((int*)a.bnd)[i] = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nz; i += step) a.prog[i] = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *ticket = 0;
+}
 
 template <bool COMPRESS>
 __global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
@@ -431,8 +463,9 @@ __global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
       else hi = mid - 1;
     }
     const Strip2d a = b.f[lo];
-    if (a.wide) strip_item<COMPRESS, true>(a, item - b.start[lo], lane);
-    else strip_item<COMPRESS, false>(a, item - b.start[lo], lane);
+    if (a.wide) strip_item<COMPRESS, true, 2>(a, item - b.start[lo], lane);
+    else if (a.shift == 1) strip_item<COMPRESS, false, 1>(a, item - b.start[lo], lane);
+    else strip_item<COMPRESS, false, 2>(a, item - b.start[lo], lane);
   }
 }
 
@@ -802,23 +835,30 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
     CPSZ_HIP(hipGetLastError());
     return;
   }
+  // Tickets go out field by field, the fields with the longest wavefront
+  // (one strip's hand-off lag per strip plus the row length) first, so the
+  // tallest fields are not left waiting for a free wave at the end.
   const size_t nf = f.size();
+  CPSZ_REQUIRE(nf <= 65535, kErrUnsupported, "2D batch: at most 65535 fields per call");
+  std::vector<Strip2d> g(f);
+  std::stable_sort(g.begin(), g.end(), [](const Strip2d& a, const Strip2d& b) {
+    return (int64_t)a.nstrips * 96 + a.r2 > (int64_t)b.nstrips * 96 + b.r2;
+  });
   std::vector<int> start(nf + 1, 0);
+  size_t max_words = 0;
   for (size_t k = 0; k < nf; k++) {
-    const int nz = 2 * f[k].nstrips + 1;
-    k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(f[k].prog, nz);
-    const size_t nb = (size_t)2 * f[k].nstrips * f[k].r2 * 2;
-    k_zero_i32<<<(int)div_up(nb, 256), 256, 0, st>>>((int*)f[k].bnd, (int)nb);
-    start[k + 1] = start[k] + 2 * f[k].nstrips;
+    start[k + 1] = start[k] + 2 * g[k].nstrips;
+    max_words = std::max(max_words, (size_t)2 * g[k].nstrips * g[k].r2 * 2);
   }
   const size_t tbytes = nf * sizeof(Strip2d), sbytes = (nf + 1) * sizeof(int);
   uint8_t* h = (uint8_t*)ctx.host_buf("w2b.tab", tbytes + sbytes);
-  std::memcpy(h, f.data(), tbytes);
+  std::memcpy(h, g.data(), tbytes);
   std::memcpy(h + tbytes, start.data(), sbytes);
   uint8_t* d = ctx.buf<uint8_t>("w2b.tab", tbytes + sbytes + 64);
   int* ticket = (int*)(d + tbytes + sbytes + 16 - ((tbytes + sbytes) & 15));
   CPSZ_HIP(hipMemcpyAsync(d, h, tbytes + sbytes, hipMemcpyHostToDevice, st));
-  k_zero_i32<<<1, 64, 0, st>>>(ticket, 1);
+  k_zero_fields<<<dim3((unsigned)std::min<size_t>(div_up(max_words, 256 * 8), 64), (unsigned)nf), 256, 0, st>>>(
+      (const Strip2d*)d, ticket);
   StripBatch b{(const Strip2d*)d, (const int*)(d + tbytes), (int)nf, start[nf], ticket};
   const int grid = std::min(start[nf], grid_cap);
   k_lorenzo2d_batch<COMPRESS><<<grid, 64, 0, st>>>(b);
@@ -937,6 +977,7 @@ void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap) {
     sa.r1 = (int)w[k]->r1;
     sa.r2 = (int)w[k]->r2;
     sa.wide = w[k]->wide;
+    CPSZ_REQUIRE(!(lg && sa.wide), kErrInternal, "compress_2d_log: eb id beyond the base-2 range");
   }
   launch_wavefront<true>(ctx, f, grid_cap);
 }
@@ -1526,6 +1567,7 @@ void d2_wavefront(Ctx& ctx, D2Work* const* w, size_t nf, int grid_cap) {
     sa.r1 = (int)w[k]->r1;
     sa.r2 = (int)w[k]->r2;
     sa.wide = w[k]->wide;
+    CPSZ_REQUIRE(!(w[k]->log && sa.wide), kErrStream, "decompress_2d_log: eb id beyond the base-2 range");
   }
   launch_wavefront<false>(ctx, f, grid_cap);
 }
