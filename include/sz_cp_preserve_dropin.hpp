@@ -17,10 +17,9 @@
 //
 // Differences a caller can observe:
 //  * the 2D pair has both instantiations (float and double); the 3D pair is
-//    float only: its double instantiations are declared but fail (return
-//    nullptr with compressed_size = 0, or leave the outputs null), because
-//    the reference's double 3D decoder applies the C library's double exp2,
-//    whose rounding the device does not reproduce;
+//    float only: its double instantiations are deleted (a compile error where
+//    used), because the reference's double 3D path applies the C library's
+//    double log2 and exp2, whose rounding the device does not reproduce;
 //  * failures (no GPU, non-finite input, corrupt stream) are reported as
 //    nullptr / compressed_size = 0 / null outputs instead of the reference's
 //    exit(0) or silent garbage; cpsz_last_error() gives the message;
@@ -181,14 +180,12 @@ inline unsigned char* sz_compress_cp_preserve_3d_offline_log<float>(const float*
   return out;
 }
 
+// The double 3D pair is deleted: a caller that instantiates it fails to
+// compile instead of receiving streams whose bits differ from the reference's.
 template <>
-inline unsigned char* sz_compress_cp_preserve_3d_offline_log<double>(const double*, const double*,
-                                                                     const double*, size_t, size_t, size_t,
-                                                                     size_t& compressed_size, bool,
-                                                                     double) {
-  compressed_size = 0;
-  return nullptr;
-}
+unsigned char* sz_compress_cp_preserve_3d_offline_log<double>(const double*, const double*, const double*, size_t,
+                                                              size_t, size_t, size_t& compressed_size, bool,
+                                                              double) = delete;
 
 // src/sz_decompress_cp_preserve_3d.cpp:10-12 frees and reallocates U, V, W.
 template <>
@@ -216,12 +213,8 @@ inline void sz_decompress_cp_preserve_3d_online_log<float>(const unsigned char* 
 }
 
 template <>
-inline void sz_decompress_cp_preserve_3d_online_log<double>(const unsigned char*, size_t, size_t, size_t,
-                                                            double*& U, double*& V, double*& W) {
-  cpsz_dropin::release(U);
-  cpsz_dropin::release(V);
-  cpsz_dropin::release(W);
-}
+void sz_decompress_cp_preserve_3d_online_log<double>(const unsigned char*, size_t, size_t, size_t, double*& U,
+                                                     double*& V, double*& W) = delete;
 
 // Host ZSTD stage with the reference's names and signatures
 // (include/sz_lossless.hpp:9-15, src/sz_lossless.cpp:3-62). The outputs are

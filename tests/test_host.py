@@ -43,6 +43,35 @@ def test_dropin_header_caller_links_and_host_stage(tmp_path):
     assert r.returncode == 1 and r.stdout.startswith("fail compress") and "HIP" in r.stdout
 
 
+def test_dropin_3d_double_fails_to_compile(tmp_path):
+    """The double 3D pair is deleted in the drop-in: using it is a compile error,
+    while the float 3D and double 2D forms compile."""
+    import shutil
+    import subprocess
+    if not shutil.which("g++"):
+        pytest.skip("no g++")
+    inc = os.path.join(ROOT, "include")
+    cases = {
+        "bad": "double* a = nullptr; size_t n; return sz_compress_cp_preserve_3d_offline_log<double>"
+               "(a, a, a, 2, 2, 2, n, false, 0.1) != nullptr;",
+        "bad2": "double *u = nullptr, *v = nullptr, *w = nullptr; "
+                "sz_decompress_cp_preserve_3d_online_log<double>(nullptr, 2, 2, 2, u, v, w); return 0;",
+        "ok3f": "float* a = nullptr; size_t n; return sz_compress_cp_preserve_3d_offline_log<float>"
+                "(a, a, a, 2, 2, 2, n, false, 0.1) != nullptr;",
+        "ok2d": "double* a = nullptr; size_t n; return sz_compress_cp_preserve_2d_offline<double>"
+                "(a, a, 2, 2, n, false, 0.1) != nullptr;",
+    }
+    for name, body in cases.items():
+        src = tmp_path / f"{name}.cpp"
+        src.write_text('#include "sz_cp_preserve_dropin.hpp"\nint main() { ' + body + " }\n")
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I" + inc, str(src)], capture_output=True,
+                           text=True, timeout=120)
+        if name.startswith("bad"):
+            assert r.returncode != 0 and "deleted" in r.stderr, name
+        else:
+            assert r.returncode == 0, (name, r.stderr[:2000])
+
+
 def test_dropin_lossless_matches_reference_zstd_level3():
     """The ZSTD stage is the reference's (level 3 frame, src/sz_lossless.cpp:11-45)."""
     rng = np.random.default_rng(1)
