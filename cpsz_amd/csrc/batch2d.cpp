@@ -1,12 +1,11 @@
 // Batched compression / decompression of many 2D fields of uneven sizes.
 //
-// Compress: the eb kernel runs per field on the caller's stream, then ONE
-// persistent wavefront launch covers every field's strips (a single field
-// cannot fill the GPU: its wavefront has at most 2 * ceil(r1/64) waves), then
-// the entropy stage (histograms, host Huffman trees, encode, stream assembly)
-// of different fields runs concurrently on a pool of worker threads, each with
-// its own HIP stream and context, so host tree building and small kernels of
-// one field overlap with those of others. Decompress: per-field parsing and
+// Compress: ONE bounds launch over every field's tiles, ONE persistent
+// wavefront launch over every field's strips (a single field cannot fill the
+// GPU: its wavefront has at most 2 * ceil(r1/64) waves), then the entropy
+// stage of all fields as a few launches over the whole batch (histograms,
+// encodes, unpredictable gathers, assembly), with the host Huffman trees
+// built in parallel on the worker pool. Decompress: per-field parsing and
 // decoder tables on the workers, then every field's bitstreams decoded
 // together (one grid per decode pass for all fields), the unpredictable
 // scatter on the workers and one persistent wavefront launch. Every field's
@@ -54,7 +53,6 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
   const size_t nf = fields.size();
   CPSZ_REQUIRE(nf > 0, kErrArg, "compress_2d_batch: empty batch");
   CPSZ_REQUIRE(!ov || ov->size() == nf, kErrArg, "compress_2d_batch: one override per field");
-  hipStream_t st = ctx.stream();
   ctx.stages.clear();
   ctx.mark("start");
   std::vector<size_t> soff(nf + 1, 0);
@@ -80,37 +78,15 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
     c2_bind(w[k], scratch + soff[k]);
     wp[k] = &w[k];
   }
-  // inputs written on the caller's stream must be complete before the workers'
-  // streams read them
-  CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
-  // bounds passes on the workers' streams: small fields fill only part of the
-  // device, so several run at once
-  {
-    std::vector<std::function<void(Ctx&)>> eb_jobs(nf);
-    for (size_t k = 0; k < nf; k++)
-      eb_jobs[k] = [&, k](Ctx& wc) {
-        c2_eb(wc, w[k], max_pwr_eb, ov ? (*ov)[k] : EbOverride());
-        CPSZ_HIP(hipStreamSynchronize(wc.stream()));
-      };
-    pool_run(pool_for(ctx.device()), eb_jobs, cfg.workers);
-  }
+  // every field's bounds in one launch on the caller's stream (which also
+  // orders them after the caller's writes of the inputs)
+  c2_eb_batch(ctx, wp.data(), nf, max_pwr_eb, ov);
   ctx.mark("eb");
-  c2_check(ctx, wp.data(), nf);
   c2_wavefront(ctx, wp.data(), nf, cfg.grid_cap);
-  CPSZ_HIP(hipStreamSynchronize(st));
   ctx.mark("lorenzo");
-  std::vector<std::function<void(Ctx&)>> jobs(nf);
-  for (size_t k = 0; k < nf; k++) {
-    jobs[k] = [&, k](Ctx& wc) {
-      DeviceStream s;
-      c2_entropy(wc, w[k], s);
-      CPSZ_REQUIRE(s.size <= stream_bound(w[k].r1 * w[k].r2), kErrInternal, "batch stream bound exceeded");
-      CPSZ_HIP(hipMemcpyAsync(dout + out.off[k], s.data, s.size, hipMemcpyDeviceToDevice, wc.stream()));
-      CPSZ_HIP(hipStreamSynchronize(wc.stream()));
-      out.size[k] = s.size;
-    };
-  }
-  pool_run(pool_for(ctx.device()), jobs, cfg.workers);
+  std::vector<size_t> cap(nf);
+  for (size_t k = 0; k < nf; k++) cap[k] = stream_bound(w[k].r1 * w[k].r2);
+  c2_entropy_batch(ctx, wp.data(), nf, dout, out.off, cap, out.size, cfg.workers);
   ctx.mark("entropy");
   out.base = dout;
   ctx.finish_marks();

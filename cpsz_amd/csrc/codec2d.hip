@@ -176,18 +176,17 @@ __device__ __forceinline__ uint32_t code_from_table(double x, const double* __re
 constexpr int kTH = 8, kTW = 64;  // vertex tile
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* __restrict__ V,
-                                              int r1, int r2, double max_pwr_eb,
-                                              const double* __restrict__ q2d, int nq,
-                                              uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
-                                              uint32_t* __restrict__ hist, int* __restrict__ flags,
-                                              EbOverride ov, const double* __restrict__ qlog, int nqlog) {
+__device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __restrict__ V, int r1, int r2,
+                                          double max_pwr_eb, const double* __restrict__ q2d, int nq,
+                                          uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
+                                          uint32_t* __restrict__ hist, int* __restrict__ flags, const EbOverride& ov,
+                                          const double* __restrict__ qlog, int nqlog, int bx, int by) {
   __shared__ T su[kTH + 2][kTW + 2];
   __shared__ T sv[kTH + 2][kTW + 2];
   __shared__ double tb[kTH + 1][kTW + 1][2];
   __shared__ uint32_t sh_hist[kMaxEbCode];
   const int tid = threadIdx.x;
-  const int i0 = blockIdx.y * kTH, j0 = blockIdx.x * kTW;
+  const int i0 = by * kTH, j0 = bx * kTW;
   for (int k = tid; k < kMaxEbCode; k += 256) sh_hist[k] = 0;
   for (int k = tid; k < (kTH + 2) * (kTW + 2); k += 256) {
     const int r = k / (kTW + 2), c = k % (kTW + 2);
@@ -259,6 +258,44 @@ __global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* 
   __syncthreads();
   for (int k = tid; k < kMaxEbCode; k += 256)
     if (sh_hist[k]) atomicAdd(&hist[k], sh_hist[k]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* __restrict__ V,
+                                              int r1, int r2, double max_pwr_eb,
+                                              const double* __restrict__ q2d, int nq,
+                                              uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
+                                              uint32_t* __restrict__ hist, int* __restrict__ flags,
+                                              EbOverride ov, const double* __restrict__ qlog, int nqlog) {
+  eb2d_tile<T>(U, V, r1, r2, max_pwr_eb, q2d, nq, cu, cv, hist, flags, ov, qlog, nqlog, blockIdx.x, blockIdx.y);
+}
+
+// Every field of a batch in one launch (plain offline, float): a job per
+// field, its tiles numbered from blk0 row by row.
+struct Eb2dJob {
+  const float* U;
+  const float* V;
+  int r1, r2, gx;  // gx: tiles per tile row
+  uint16_t* cu;
+  uint16_t* cv;
+  uint32_t* hist;
+  int* flags;
+  EbOverride ov;
+  uint32_t blk0;
+};
+
+__global__ void __launch_bounds__(256) k_eb2d_multi(const Eb2dJob* __restrict__ jobs, int nj, double max_pwr_eb,
+                                                    const double* __restrict__ q2d, int nq) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const Eb2dJob& J = jobs[lo];
+  const uint32_t t = blockIdx.x - J.blk0;
+  eb2d_tile<float>(J.U, J.V, J.r1, J.r2, max_pwr_eb, q2d, nq, J.cu, J.cv, J.hist, J.flags, J.ov, nullptr, 0,
+                   (int)(t % (uint32_t)J.gx), (int)(t / (uint32_t)J.gx));
 }
 
 // ------------------------------------------------------------ wavefront ----
@@ -798,6 +835,170 @@ struct U16Hist {
   }
 };
 
+// ---- batched entropy stage: many fields' streams per launch -----------------
+// One component of one field for the per-block unpredictable passes
+// (kUnpWaves 12x12 blocks per workgroup, from workgroup blk0).
+struct UnpJob {
+  const uint16_t* q;
+  const float* x;        // values the codec ran on (gather source)
+  int r1, r2, nby;
+  size_t nb;
+  uint32_t* cnt;         // per-block zero counts
+  const uint64_t* off;   // global exclusive scan of the counts, at this component's first block
+  float* out;            // this component's unpredictable values
+  uint32_t blk0;
+};
+
+__global__ void __launch_bounds__(256) k_block_zeros_multi(const UnpJob* __restrict__ jobs, int nj) {
+  const UnpJob& J = jobs[job_of(jobs, nj, blockIdx.x)];
+  const size_t b = (size_t)(blockIdx.x - J.blk0) * kUnpWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= J.nb) return;  // uniform across the wave
+  const BlockView v(b, J.r1, J.r2, J.nby);
+  uint32_t z = 0;
+  for (int k0 = 0; k0 < v.m; k0 += 64) {
+    const int k = k0 + lane;
+    const bool zero = k < v.m && J.q[v.elem(k, J.r2)] == 0;
+    z += (uint32_t)__popcll(__ballot(zero));
+  }
+  if (lane == 0) J.cnt[b] = z;
+}
+
+__global__ void __launch_bounds__(256) k_gather_unpred_multi(const UnpJob* __restrict__ jobs, int nj) {
+  const UnpJob& J = jobs[job_of(jobs, nj, blockIdx.x)];
+  const size_t b = (size_t)(blockIdx.x - J.blk0) * kUnpWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= J.nb) return;
+  const BlockView v(b, J.r1, J.r2, J.nby);
+  uint64_t o = J.off[b] - J.off[0];
+  const uint64_t below = (1ull << lane) - 1;
+  for (int k0 = 0; k0 < v.m; k0 += 64) {
+    const int k = k0 + lane;
+    const size_t e = k < v.m ? v.elem(k, J.r2) : 0;
+    const bool zero = k < v.m && J.q[e] == 0;
+    const uint64_t mask = __ballot(zero);
+    if (zero) J.out[o + __popcll(mask & below)] = J.x[e];
+    o += __popcll(mask);
+  }
+}
+
+// The present symbols of each histogram (one 1024-thread workgroup per
+// histogram), in symbol order, packed into one list: histogram h gets
+// count[h] entries from base[h].
+struct CompactJob {
+  const uint32_t* hist;
+  uint32_t nbins;
+};
+
+__global__ void __launch_bounds__(1024) k_compact_hist(const CompactJob* __restrict__ jobs, uint32_t* __restrict__ count,
+                                                       uint64_t* __restrict__ base,
+                                                       unsigned long long* __restrict__ total,
+                                                       uint32_t* __restrict__ syms, uint32_t* __restrict__ freqs) {
+  __shared__ uint32_t wsum[16];
+  __shared__ uint64_t b0;
+  const CompactJob J = jobs[blockIdx.x];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t c = 0;
+  for (uint32_t i = threadIdx.x; i < J.nbins; i += 1024) c += J.hist[i] != 0;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if (lane == 0) wsum[wv] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t k = 0;
+    for (int x = 0; x < 16; x++) k += wsum[x];
+    const uint64_t at = atomicAdd(total, (unsigned long long)k);
+    count[blockIdx.x] = k;
+    base[blockIdx.x] = at;
+    b0 = at;
+  }
+  __syncthreads();
+  uint64_t pos = b0;
+  const uint64_t below = (1ull << lane) - 1;
+  for (uint32_t i0 = 0; i0 < J.nbins; i0 += 1024) {
+    const uint32_t i = i0 + threadIdx.x;
+    const uint32_t f = i < J.nbins ? J.hist[i] : 0u;
+    const uint64_t m = __ballot(f != 0);
+    if (lane == 0) wsum[wv] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int x = 0; x < 16; x++) {
+      const uint32_t y = wsum[x];
+      before += x < wv ? y : 0u;
+      all += y;
+    }
+    if (f) {
+      const uint64_t at = pos + before + (uint32_t)__popcll(m & below);
+      syms[at] = i;
+      freqs[at] = f;
+    }
+    pos += all;
+    __syncthreads();
+  }
+}
+
+// Codebooks of many trees into their dense tables (one workgroup per tree).
+struct CbJob {
+  const uint32_t* syms;
+  const uint64_t* codes;
+  const uint8_t* lens;
+  uint32_t k;
+  uint64_t* code;
+  uint8_t* len;
+};
+
+__global__ void __launch_bounds__(256) k_scatter_codebooks(const CbJob* __restrict__ jobs) {
+  const CbJob& J = jobs[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < J.k; i += blockDim.x) {
+    J.code[J.syms[i]] = J.codes[i];
+    J.len[J.syms[i]] = J.lens[i];
+  }
+}
+
+// out[j] = a[j].hi - a[j].lo: per-stream sums read off global exclusive scans.
+struct ScanDiff {
+  const uint64_t* lo;
+  const uint64_t* hi;
+};
+
+__global__ void k_scan_diffs(const ScanDiff* __restrict__ d, int n, uint64_t* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) out[j] = *d[j].hi - *d[j].lo;
+}
+
+// Stream assembly: one workgroup per piece (at most 64 KiB) copies len bytes
+// to out + dst. The destination's aligned words are stored whole, each built
+// from the two source words that cover it (alignbyte); the at most 3 bytes
+// before and after them go byte by byte, so neighbouring pieces never share a
+// word store.
+struct CopyPiece {
+  uint64_t dst;
+  const uint8_t* src;
+  uint32_t len;
+  uint32_t pad;
+};
+
+__global__ void __launch_bounds__(256) k_assemble(const CopyPiece* __restrict__ pieces, uint8_t* __restrict__ out) {
+  const CopyPiece P = pieces[blockIdx.x];
+  uint8_t* d = out + P.dst;
+  const uint8_t* s = P.src;
+  const uint32_t n = P.len;
+  const uint32_t head = min(n, (uint32_t)((4u - ((uint32_t)(uintptr_t)d & 3u)) & 3u));
+  const uint32_t nw = (n - head) >> 2, tail0 = head + 4 * nw;
+  if (threadIdx.x < head) d[threadIdx.x] = s[threadIdx.x];
+  if (threadIdx.x < n - tail0) d[tail0 + threadIdx.x] = s[tail0 + threadIdx.x];
+  uint32_t* dw = (uint32_t*)(d + head);
+  const uint8_t* sp = s + head;
+  const uint32_t mis = (uint32_t)(uintptr_t)sp & 3u;
+  const uint32_t* sw = (const uint32_t*)(sp - mis);
+  if (mis == 0) {
+    for (uint32_t k = threadIdx.x; k < nw; k += blockDim.x) dw[k] = sw[k];
+  } else {
+    for (uint32_t k = threadIdx.x; k < nw; k += blockDim.x)
+      dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], mis);
+  }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- host ----
@@ -816,6 +1017,74 @@ struct Piece {  // a contiguous part of the output stream
   size_t len;
   bool device;
 };
+
+// Host-written parts of a 2D stream (src/sz_compress_cp_preserve_2d.cpp:258-269,
+// src/sz_compress_3d.cpp:219-228): the id section's tree head, the two
+// components' compressor heads (block size, radius, meanInfo<T>, unpredictable
+// count) and tree heads (the all-zero block indicator bytes first), and the
+// stream header with the three section sizes. vb: bytes of T.
+// One pinned staging area mirrored on the device: tables are filled on the
+// host and sent with one copy; take() returns the device address of a slot and
+// its host address through `host`.
+struct Arena {
+  uint8_t* h = nullptr;
+  uint8_t* d = nullptr;
+  size_t used = 0, cap = 0;
+  template <class T>
+  T* take(size_t count, T** host) {
+    used = (used + 15) & ~(size_t)15;
+    CPSZ_REQUIRE(used + count * sizeof(T) <= cap, kErrInternal, "staging table overflow");
+    *host = (T*)(h + used);
+    T* dp = (T*)(d + used);
+    used += count * sizeof(T);
+    return dp;
+  }
+};
+
+struct Heads2d {
+  std::vector<uint8_t> hdr, h_eb, hu1, hu2, hv1, hv2;
+  size_t eb_size = 0, u_size = 0, v_size = 0;
+};
+
+void heads_2d(const HuffEncodeTable& t_eb, const HuffEncodeTable& t_u, const HuffEncodeTable& t_v, size_t bytes_eb,
+              size_t bytes_u, size_t bytes_v, uint64_t unp_u, uint64_t unp_v, size_t nb, bool log, size_t vb,
+              Heads2d& H) {
+  const size_t ind_bytes = (nb - 1) / 8 + 1;
+  auto huff_head = [](std::vector<uint8_t>& h, const HuffEncodeTable& t, size_t nbytes) {
+    put(h, (size_t)t.node_count);
+    put(h, (uint32_t)t.tree.size());
+    h.insert(h.end(), t.tree.begin(), t.tree.end());
+    put(h, (size_t)nbytes);
+  };
+  auto comp_head = [&](std::vector<uint8_t>& h, uint64_t unp_count) {
+    put(h, (int)kBlock2d);
+    put(h, (char)0);
+    put(h, (int)kRadius);
+    put(h, (uint64_t)0);  // meanInfo<T>{false, 0}: flag byte, padding bytes, T 0 (8 or 16 bytes)
+    if (vb == 8) put(h, (uint64_t)0);
+    put(h, (size_t)0);    // reg_count
+    put(h, (size_t)unp_count);
+  };
+  H.hdr.clear();
+  H.h_eb.clear();
+  H.hu1.clear();
+  H.hv1.clear();
+  huff_head(H.h_eb, t_eb, bytes_eb);
+  comp_head(H.hu1, unp_u);
+  comp_head(H.hv1, unp_v);
+  H.hu2.assign(ind_bytes, 0);
+  H.hv2.assign(ind_bytes, 0);
+  huff_head(H.hu2, t_u, bytes_u);
+  huff_head(H.hv2, t_v, bytes_v);
+  H.eb_size = H.h_eb.size() + bytes_eb;
+  H.u_size = H.hu1.size() + vb * unp_u + H.hu2.size() + bytes_u;
+  H.v_size = H.hv1.size() + vb * unp_v + H.hv2.size() + bytes_v;
+  put(H.hdr, (int)(log ? 2 : 4));
+  put(H.hdr, (double)DBL_EPSILON);
+  put(H.hdr, H.eb_size);
+  put(H.hdr, H.u_size);
+  put(H.hdr, H.v_size);
+}
 
 // Lorenzo wavefront for fields f[0..nf): one launch; a single field keeps the
 // plain kernel, several share one persistent launch.
@@ -912,6 +1181,57 @@ void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
                                         w.cu, w.cv, w.hist, w.flags, ov, w.log ? ctx.d_q2l() : nullptr,
                                         ctx.q2l_size());
   CPSZ_HIP(hipGetLastError());
+}
+
+// Bounds of every field (plain offline, float) in one launch: the fields'
+// histograms and flags move to two contiguous arrays, cleared at once; then
+// the flags of all fields come back in one copy (as c2_check).
+void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const std::vector<EbOverride>* ov) {
+  hipStream_t st = ctx.stream();
+  constexpr size_t HN = kMaxEbCode + 2 * 65536;
+  uint32_t* hist = ctx.buf<uint32_t>("b2eb.hist", nf * HN);
+  int* flags = ctx.buf<int>("b2eb.flags", 4 * nf);
+  std::vector<Eb2dJob> jobs(nf);
+  uint32_t blk = 0;
+  for (size_t k = 0; k < nf; k++) {
+    C2Work& W = *w[k];
+    CPSZ_REQUIRE(W.dU && W.dV && W.r1 >= 1 && W.r2 >= 1 && !W.log && !W.f64, kErrArg, "compress_2d: bad arguments");
+    CPSZ_REQUIRE(W.r1 < (1u << 30) && W.r2 < (1u << 30) && W.r1 * W.r2 < (1ull << 40), kErrUnsupported,
+                 "compress_2d: field too large");
+    W.hist = hist + k * HN;
+    W.flags = flags + 4 * k;
+    Eb2dJob& J = jobs[k];
+    J.U = W.dU;
+    J.V = W.dV;
+    J.r1 = (int)W.r1;
+    J.r2 = (int)W.r2;
+    J.gx = (int)div_up(W.r2, kTW);
+    J.cu = W.cu;
+    J.cv = W.cv;
+    J.hist = W.hist;
+    J.flags = W.flags;
+    J.ov = ov ? (*ov)[k] : EbOverride();
+    J.blk0 = blk;
+    const size_t tiles = div_up(W.r2, kTW) * div_up(W.r1, kTH);
+    CPSZ_REQUIRE(blk + tiles < (1ull << 31), kErrUnsupported, "2D batch: too many tiles in one call");
+    blk += (uint32_t)tiles;
+  }
+  Eb2dJob* hj = (Eb2dJob*)ctx.host_buf("b2eb.jobs", nf * sizeof(Eb2dJob));
+  std::memcpy(hj, jobs.data(), nf * sizeof(Eb2dJob));
+  Eb2dJob* dj = ctx.buf<Eb2dJob>("b2eb.jobs", nf);
+  CPSZ_HIP(hipMemcpyAsync(dj, hj, nf * sizeof(Eb2dJob), hipMemcpyHostToDevice, st));
+  k_fill_u32<<<1024, 256, 0, st>>>(hist, nf * HN, 0);
+  k_fill_u32<<<(unsigned)div_up(4 * nf, 256), 256, 0, st>>>((uint32_t*)flags, 4 * nf, 0);
+  k_eb2d_multi<<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size());
+  CPSZ_HIP(hipGetLastError());
+  int* hf = (int*)ctx.host_buf("b2eb.hflags", 16 * nf);
+  CPSZ_HIP(hipMemcpyAsync(hf, flags, 16 * nf, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  for (size_t k = 0; k < nf; k++) {
+    CPSZ_REQUIRE(hf[4 * k] == 0, kErrNonFinite,
+                 "compress_2d: non-finite input or bound (the reference cannot encode it)");
+    w[k]->wide = hf[4 * k + 1] != 0;
+  }
 }
 
 void c2_check(Ctx& ctx, C2Work* const* w, size_t nf) {
@@ -1088,44 +1408,11 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   }
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
-  // ---- stream layout (src/sz_compress_cp_preserve_2d.cpp:258-269, src/sz_compress_3d.cpp:219-228)
-  const size_t ind_bytes = (nb - 1) / 8 + 1;
-  auto huff_head = [](const HuffEncodeTable& t, size_t nbytes) {
-    std::vector<uint8_t> h;
-    put(h, (size_t)t.node_count);
-    put(h, (uint32_t)t.tree.size());
-    h.insert(h.end(), t.tree.begin(), t.tree.end());
-    put(h, (size_t)nbytes);
-    return h;
-  };
-  auto comp_head = [&](uint64_t unp_count) {
-    std::vector<uint8_t> h;
-    put(h, (int)kBlock2d);
-    put(h, (char)0);
-    put(h, (int)kRadius);
-    put(h, (uint64_t)0);  // meanInfo<T>{false, 0}: flag byte, padding bytes, T 0 (8 or 16 bytes)
-    if (w.f64) put(h, (uint64_t)0);
-    put(h, (size_t)0);    // reg_count
-    put(h, (size_t)unp_count);
-    return h;
-  };
-  std::vector<uint8_t> h_eb = huff_head(t_eb, bytes_eb);
-  std::vector<uint8_t> hu1 = comp_head(unp_u), hv1 = comp_head(unp_v);
-  std::vector<uint8_t> hu2(ind_bytes, 0), hv2(ind_bytes, 0);
-  {
-    std::vector<uint8_t> a = huff_head(t_u, bytes_u), b = huff_head(t_v, bytes_v);
-    hu2.insert(hu2.end(), a.begin(), a.end());
-    hv2.insert(hv2.end(), b.begin(), b.end());
-  }
-  const size_t eb_size = h_eb.size() + bytes_eb;
-  const size_t u_size = hu1.size() + vb * unp_u + hu2.size() + bytes_u;
-  const size_t v_size = hv1.size() + vb * unp_v + hv2.size() + bytes_v;
-  std::vector<uint8_t> hdr;
-  put(hdr, (int)(w.log ? 2 : 4));
-  put(hdr, (double)DBL_EPSILON);
-  put(hdr, eb_size);
-  put(hdr, u_size);
-  put(hdr, v_size);
+  // ---- stream layout
+  Heads2d H;
+  heads_2d(t_eb, t_u, t_v, bytes_eb, bytes_u, bytes_v, unp_u, unp_v, nb, w.log, vb, H);
+  const std::vector<uint8_t>&hdr = H.hdr, &h_eb = H.h_eb, &hu1 = H.hu1, &hu2 = H.hu2, &hv1 = H.hv1, &hv2 = H.hv2;
+  const size_t eb_size = H.eb_size, u_size = H.u_size, v_size = H.v_size;
   // offline_log: the two sign bitmaps follow the id section
   // (src/sz_compress_cp_preserve_2d.cpp:358-366)
   const size_t smap = w.log ? (n - 1) / 8 + 1 : 0;
@@ -1174,6 +1461,315 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   ctx.mark("assemble");
   out.data = dout;
   out.size = total;
+  CPSZ_HIP(hipStreamSynchronize(st));
+}
+
+// Batched entropy stage (plain offline, float): every field's histograms,
+// encodes, unpredictable gathers and stream assembly run as a few launches
+// over all fields, with three host round trips for the whole batch (the
+// present symbols of every histogram, the per-stream bit and unpredictable
+// totals, the finished layout). The Huffman trees are built on the host, in
+// parallel over the batch. Each field's stream is byte-identical to
+// c2_entropy's.
+void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, const std::vector<size_t>& off,
+                      const std::vector<size_t>& cap, std::vector<size_t>& size, int workers) {
+  hipStream_t st = ctx.stream();
+  constexpr size_t HN = kMaxEbCode + 2 * 65536;  // dense codebook entries per field
+  struct G {
+    size_t n, nb, nby, nch_ids, zb, cb;
+  };
+  std::vector<G> g(nf);
+  size_t Z = 0, CB = 0;
+  for (size_t k = 0; k < nf; k++) {
+    const C2Work& W = *w[k];
+    CPSZ_REQUIRE(!W.log && !W.f64, kErrInternal, "batched entropy: plain float fields only");
+    G& x = g[k];
+    x.n = W.r1 * W.r2;
+    x.nby = div_up(W.r2, kBlock2d);
+    x.nb = div_up(W.r1, kBlock2d) * x.nby;
+    x.nch_ids = div_up(2 * x.n, (size_t)SrcIds2d::kChunk);
+    x.zb = Z;
+    Z += 2 * x.nb;
+    x.cb = CB;
+    CB += x.nch_ids + 2 * x.nb;
+  }
+  auto arena = [&](const char* name, size_t bytes) {
+    Arena a;
+    a.h = (uint8_t*)ctx.host_buf(name, bytes);
+    a.d = ctx.buf<uint8_t>(name, bytes);
+    a.cap = bytes;
+    return a;
+  };
+  auto send = [&](Arena& a) {
+    if (a.used) CPSZ_HIP(hipMemcpyAsync(a.d, a.h, a.used, hipMemcpyHostToDevice, st));
+  };
+  // ---- phase A: histograms of the quant codes, block zero counts and their
+  // scan, present symbols of every histogram (eb ids counted by the eb pass)
+  uint32_t* zc = ctx.buf<uint32_t>("b2e.zc", Z + 1);
+  uint64_t* zoff = ctx.buf<uint64_t>("b2e.zoff", Z + 1);
+  const size_t ntree = 3 * nf;
+  Arena A = arena("b2e.tabA", 2 * nf * (sizeof(HistJob<U16Hist>) + sizeof(UnpJob)) + ntree * sizeof(CompactJob) +
+                                  3 * 64);
+  HistJob<U16Hist>* hj;
+  HistJob<U16Hist>* d_hj = A.take(2 * nf, &hj);
+  UnpJob* uj;
+  UnpJob* d_uj = A.take(2 * nf, &uj);
+  CompactJob* cj;
+  CompactJob* d_cj = A.take(ntree, &cj);
+  uint32_t hblk = 0, ublk = 0;
+  for (size_t k = 0; k < nf; k++) {
+    const C2Work& W = *w[k];
+    const size_t nch = div_up(g[k].n, (size_t)U16Hist::kChunk);
+    for (int c = 0; c < 2; c++) {
+      HistJob<U16Hist>& h = hj[2 * k + c];
+      h.src = U16Hist{c ? W.qv : W.qu, g[k].n};
+      h.nchunks = nch;
+      h.hist = W.hist + kMaxEbCode + c * 65536;
+      h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 8), 512);
+      h.blk0 = hblk;
+      hblk += h.nblk;
+      UnpJob& u = uj[2 * k + c];
+      u = UnpJob{};
+      u.q = c ? W.qv : W.qu;
+      u.x = c ? W.dV : W.dU;
+      u.r1 = (int)W.r1;
+      u.r2 = (int)W.r2;
+      u.nby = (int)g[k].nby;
+      u.nb = g[k].nb;
+      u.cnt = zc + g[k].zb + c * g[k].nb;
+      u.off = zoff + g[k].zb + c * g[k].nb;
+      u.blk0 = ublk;
+      ublk += (uint32_t)unp_grid(g[k].nb);
+    }
+    cj[3 * k + 0] = CompactJob{W.hist, (uint32_t)kMaxEbCode};
+    cj[3 * k + 1] = CompactJob{W.hist + kMaxEbCode, 65536u};
+    cj[3 * k + 2] = CompactJob{W.hist + kMaxEbCode + 65536, 65536u};
+  }
+  send(A);
+  const uint32_t win_lo = 32768 - 8192, win_n = 16384;
+  k_hist_multi<U16Hist><<<hblk, 256, win_n * 4, st>>>(d_hj, (int)(2 * nf), win_lo, win_n);
+  k_block_zeros_multi<<<ublk, 256, 0, st>>>(d_uj, (int)(2 * nf));
+  CPSZ_HIP(hipGetLastError());
+  scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
+  // present symbols: counts, bases and the running total, then the lists
+  uint8_t* cmeta = ctx.buf<uint8_t>("b2e.cmeta", ntree * 12 + 64);
+  unsigned long long* d_total = (unsigned long long*)cmeta;
+  uint64_t* d_base = (uint64_t*)(cmeta + 8);
+  uint32_t* d_count = (uint32_t*)(cmeta + 8 + 8 * ntree);
+  const size_t max_present = ntree * (size_t)65536;
+  uint32_t* d_syms = ctx.buf<uint32_t>("b2e.syms", max_present);
+  uint32_t* d_freqs = ctx.buf<uint32_t>("b2e.freqs", max_present);
+  k_zero_i32<<<1, 64, 0, st>>>((int*)d_total, 2);
+  k_compact_hist<<<(unsigned)ntree, 1024, 0, st>>>(d_cj, d_count, d_base, d_total, d_syms, d_freqs);
+  CPSZ_HIP(hipGetLastError());
+  uint8_t* hmeta = (uint8_t*)ctx.host_buf("b2e.cmeta", ntree * 12 + 64);
+  CPSZ_HIP(hipMemcpyAsync(hmeta, cmeta, ntree * 12 + 8, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  const uint64_t present = *(const uint64_t*)hmeta;
+  const uint64_t* h_base = (const uint64_t*)(hmeta + 8);
+  const uint32_t* h_count = (const uint32_t*)(hmeta + 8 + 8 * ntree);
+  uint32_t* h_sf = (uint32_t*)ctx.host_buf("b2e.sf", 8 * present + 8);
+  CPSZ_HIP(hipMemcpyAsync(h_sf, d_syms, 4 * present, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipMemcpyAsync(h_sf + present, d_freqs, 4 * present, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  ctx.mark("hist");
+  // ---- trees on the host, in parallel over the batch
+  // kept per calling thread so their storage is reused; the pool threads reach
+  // them through a reference (a thread_local named inside a job would be the
+  // worker's own)
+  thread_local std::vector<HuffEncodeTable> tabs_keep;
+  std::vector<HuffEncodeTable>& tabs = tabs_keep;
+  if (tabs.size() < ntree) tabs.resize(ntree);
+  {
+    std::vector<std::function<void(Ctx&)>> jobs(ntree);
+    for (size_t t = 0; t < ntree; t++)
+      jobs[t] = [&, t](Ctx&) {
+        CPSZ_REQUIRE(h_count[t] > 0, kErrInternal, "batched entropy: empty histogram");
+        huff_build_sparse(h_sf + h_base[t], h_sf + present + h_base[t], h_count[t],
+                          t % 3 == 0 ? 2 * 1024 : 2 * kCapacity, tabs[t]);
+      };
+    pool_run(pool_for(ctx.device()), jobs, std::max(workers, 1));
+  }
+  ctx.mark("tree");
+  // ---- phase B: codebooks, chunk bit counts, one scan for every stream,
+  // per-stream totals
+  uint64_t* cb_code = ctx.buf<uint64_t>("b2e.cbcode", nf * HN);
+  uint8_t* cb_len = ctx.buf<uint8_t>("b2e.cblen", nf * HN);
+  uint64_t* bits = ctx.buf<uint64_t>("b2e.bits", CB + 1);
+  uint64_t* offs = ctx.buf<uint64_t>("b2e.offs", CB + 1);
+  size_t cb_entries = 0;
+  for (size_t t = 0; t < ntree; t++) cb_entries += tabs[t].syms.size();
+  Arena B = arena("b2e.tabB", cb_entries * 13 + ntree * (sizeof(CbJob) + 48) + nf * sizeof(EncJob<SrcIds2d>) +
+                                  2 * nf * sizeof(EncJob<SrcCodes2d>) + 5 * nf * sizeof(ScanDiff) + 8 * 64);
+  CbJob* cbj;
+  CbJob* d_cbj = B.take(ntree, &cbj);
+  EncJob<SrcIds2d>* ej_i;
+  EncJob<SrcIds2d>* d_ej_i = B.take(nf, &ej_i);
+  EncJob<SrcCodes2d>* ej_c;
+  EncJob<SrcCodes2d>* d_ej_c = B.take(2 * nf, &ej_c);
+  ScanDiff* sd;
+  ScanDiff* d_sd = B.take(5 * nf, &sd);
+  for (size_t t = 0; t < ntree; t++) {
+    const HuffEncodeTable& T = tabs[t];
+    const size_t kk = T.syms.size();
+    uint64_t* hc;
+    uint32_t* hs;
+    uint8_t* hl;
+    cbj[t].codes = B.take(kk, &hc);
+    cbj[t].syms = B.take(kk, &hs);
+    cbj[t].lens = B.take(kk, &hl);
+    std::memcpy(hc, T.codes.data(), 8 * kk);
+    std::memcpy(hs, T.syms.data(), 4 * kk);
+    std::memcpy(hl, T.lens.data(), kk);
+    cbj[t].k = (uint32_t)kk;
+    const size_t k = t / 3, which = t % 3;
+    const size_t at = k * HN + (which == 0 ? 0 : kMaxEbCode + (which - 1) * 65536);
+    cbj[t].code = cb_code + at;
+    cbj[t].len = cb_len + at;
+  }
+  uint32_t iblk = 0, cblk = 0;
+  for (size_t k = 0; k < nf; k++) {
+    const C2Work& W = *w[k];
+    const G& x = g[k];
+    EncJob<SrcIds2d>& e = ej_i[k];
+    e.src = SrcIds2d{W.cu, W.cv, x.n, 2 * x.n};
+    e.nchunks = x.nch_ids;
+    e.code = cb_code + k * HN;
+    e.len = cb_len + k * HN;
+    e.bits = bits + x.cb;
+    e.off = offs + x.cb;
+    e.words = nullptr;
+    e.blk0 = iblk;
+    iblk += (uint32_t)encode_blocks(x.nch_ids);
+    for (int c = 0; c < 2; c++) {
+      EncJob<SrcCodes2d>& f = ej_c[2 * k + c];
+      f.src = SrcCodes2d{c ? W.qv : W.qu, (int)W.r1, (int)W.r2, (int)x.nby};
+      f.nchunks = x.nb;
+      f.code = cb_code + k * HN + kMaxEbCode + c * 65536;
+      f.len = cb_len + k * HN + kMaxEbCode + c * 65536;
+      f.bits = bits + x.cb + x.nch_ids + c * x.nb;
+      f.off = offs + x.cb + x.nch_ids + c * x.nb;
+      f.words = nullptr;
+      f.blk0 = cblk;
+      cblk += (uint32_t)encode_blocks(x.nb);
+    }
+    const size_t c0 = x.cb, c1 = c0 + x.nch_ids, c2 = c1 + x.nb, c3 = c2 + x.nb;
+    sd[5 * k + 0] = ScanDiff{offs + c0, offs + c1};
+    sd[5 * k + 1] = ScanDiff{offs + c1, offs + c2};
+    sd[5 * k + 2] = ScanDiff{offs + c2, offs + c3};
+    sd[5 * k + 3] = ScanDiff{zoff + x.zb, zoff + x.zb + x.nb};
+    sd[5 * k + 4] = ScanDiff{zoff + x.zb + x.nb, zoff + x.zb + 2 * x.nb};
+  }
+  send(B);
+  k_scatter_codebooks<<<(unsigned)ntree, 256, 0, st>>>(d_cbj);
+  k_chunk_bits_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ej_i, (int)nf);
+  k_chunk_bits_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_ej_c, (int)(2 * nf));
+  CPSZ_HIP(hipGetLastError());
+  scan_u64(ctx, bits, offs, CB, offs + CB);
+  uint64_t* d_tot = ctx.buf<uint64_t>("b2e.tot", 5 * nf);
+  k_scan_diffs<<<(unsigned)div_up(5 * nf, 256), 256, 0, st>>>(d_sd, (int)(5 * nf), d_tot);
+  CPSZ_HIP(hipGetLastError());
+  uint64_t* h_tot = (uint64_t*)ctx.host_buf("b2e.tot", 8 * 5 * nf);
+  CPSZ_HIP(hipMemcpyAsync(h_tot, d_tot, 8 * 5 * nf, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipStreamSynchronize(st));
+  ctx.mark("scan");
+  // ---- phase C: layout, encode, gather, assembly
+  std::vector<Heads2d> H(nf);
+  std::vector<size_t> wofs(nf + 1, 0), uofs(nf + 1, 0);
+  size_t hbytes = 0, npieces = 0;
+  constexpr size_t kPiece = 65536;
+  auto pieces_of = [&](size_t len) { return div_up(len, kPiece); };
+  for (size_t k = 0; k < nf; k++) {
+    const uint64_t* t5 = h_tot + 5 * k;
+    const size_t bytes_eb = div_up(t5[0], 8), bytes_u = div_up(t5[1], 8), bytes_v = div_up(t5[2], 8);
+    heads_2d(tabs[3 * k], tabs[3 * k + 1], tabs[3 * k + 2], bytes_eb, bytes_u, bytes_v, t5[3], t5[4], g[k].nb,
+             false, 4, H[k]);
+    const size_t words = (div_up(bytes_eb, 4) + 2) + (div_up(bytes_u, 4) + 2) + (div_up(bytes_v, 4) + 2);
+    wofs[k + 1] = wofs[k] + ((words + 63) & ~(size_t)63);
+    uofs[k + 1] = uofs[k] + ((t5[3] + t5[4] + 63) & ~(size_t)63);
+    const Heads2d& h = H[k];
+    hbytes += h.hdr.size() + h.h_eb.size() + h.hu1.size() + h.hu2.size() + h.hv1.size() + h.hv2.size() + 6 * 16;
+    npieces += pieces_of(h.hdr.size()) + pieces_of(h.h_eb.size()) + pieces_of(h.hu1.size()) +
+               pieces_of(h.hu2.size()) + pieces_of(h.hv1.size()) + pieces_of(h.hv2.size()) + pieces_of(bytes_eb) +
+               pieces_of(bytes_u) + pieces_of(bytes_v) + pieces_of(4 * t5[3]) + pieces_of(4 * t5[4]);
+    const size_t total = h.hdr.size() + h.eb_size + h.u_size + h.v_size;
+    CPSZ_REQUIRE(total <= cap[k], kErrInternal, "batch stream bound exceeded");
+    size[k] = total;
+  }
+  uint32_t* words = ctx.buf<uint32_t>("b2e.words", wofs[nf] + 1);
+  float* unp = ctx.buf<float>("b2e.unp", uofs[nf] + 1);
+  k_fill_u32<<<1024, 256, 0, st>>>(words, wofs[nf], 0);
+  Arena C = arena("b2e.tabC", hbytes + npieces * sizeof(CopyPiece) + nf * sizeof(EncJob<SrcIds2d>) +
+                                  2 * nf * (sizeof(EncJob<SrcCodes2d>) + sizeof(UnpJob)) + 8 * 64);
+  EncJob<SrcIds2d>* ci;
+  EncJob<SrcIds2d>* d_ci = C.take(nf, &ci);
+  EncJob<SrcCodes2d>* cc;
+  EncJob<SrcCodes2d>* d_cc = C.take(2 * nf, &cc);
+  UnpJob* gj;
+  UnpJob* d_gj = C.take(2 * nf, &gj);
+  CopyPiece* cp;
+  CopyPiece* d_cp = C.take(npieces, &cp);
+  size_t np = 0;
+  auto piece = [&](size_t dst, const uint8_t* src, size_t len) {
+    for (size_t o = 0; o < len; o += kPiece) {
+      CPSZ_REQUIRE(np < npieces, kErrInternal, "batched assembly: piece table overflow");
+      cp[np++] = CopyPiece{dst + o, src + o, (uint32_t)std::min(kPiece, len - o), 0};
+    }
+  };
+  for (size_t k = 0; k < nf; k++) {
+    const uint64_t* t5 = h_tot + 5 * k;
+    const size_t bytes_eb = div_up(t5[0], 8), bytes_u = div_up(t5[1], 8), bytes_v = div_up(t5[2], 8);
+    const size_t w_eb = div_up(bytes_eb, 4) + 2, w_u = div_up(bytes_u, 4) + 2;
+    uint32_t* wk = words + wofs[k];
+    ci[k] = ej_i[k];
+    ci[k].words = wk;
+    cc[2 * k] = ej_c[2 * k];
+    cc[2 * k].words = wk + w_eb;
+    cc[2 * k + 1] = ej_c[2 * k + 1];
+    cc[2 * k + 1].words = wk + w_eb + w_u;
+    float* uk = unp + uofs[k];
+    gj[2 * k] = uj[2 * k];
+    gj[2 * k].out = uk;
+    gj[2 * k + 1] = uj[2 * k + 1];
+    gj[2 * k + 1].out = uk + t5[3];
+    // stream: hdr, eb tree head, eb bits, u head, u unpredictable, u tree
+    // head, u bits, then the same for v (the single-field layout)
+    const Heads2d& h = H[k];
+    size_t pos = off[k];
+    auto host_piece = [&](const std::vector<uint8_t>& v) {
+      uint8_t* hp;
+      const uint8_t* dp = C.take(v.size(), &hp);
+      std::memcpy(hp, v.data(), v.size());
+      piece(pos, dp, v.size());
+      pos += v.size();
+    };
+    auto dev_piece = [&](const void* src, size_t len) {
+      piece(pos, (const uint8_t*)src, len);
+      pos += len;
+    };
+    host_piece(h.hdr);
+    host_piece(h.h_eb);
+    dev_piece(wk, bytes_eb);
+    host_piece(h.hu1);
+    dev_piece(uk, 4 * t5[3]);
+    host_piece(h.hu2);
+    dev_piece(wk + w_eb, bytes_u);
+    host_piece(h.hv1);
+    dev_piece(uk + t5[3], 4 * t5[4]);
+    host_piece(h.hv2);
+    dev_piece(wk + w_eb + w_u, bytes_v);
+    CPSZ_REQUIRE(pos - off[k] == size[k], kErrInternal, "batched stream layout mismatch");
+  }
+  send(C);
+  k_encode_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ci, (int)nf);
+  k_encode_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_cc, (int)(2 * nf));
+  k_gather_unpred_multi<<<ublk, 256, 0, st>>>(d_gj, (int)(2 * nf));
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("encode");
+  if (np) k_assemble<<<(unsigned)np, 256, 0, st>>>(d_cp, dout);
+  CPSZ_HIP(hipGetLastError());
+  ctx.mark("assemble");
+  // the pinned tables are reused by the next call only after this point
   CPSZ_HIP(hipStreamSynchronize(st));
 }
 

@@ -54,8 +54,17 @@ size_t c2_scratch_bytes(size_t r1, size_t r2);
 void c2_bind(C2Work& w, uint8_t* scratch);  // carves c2_scratch_bytes of device memory
 void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov);
 void c2_check(Ctx& ctx, C2Work* const* w, size_t nf);  // synchronizes; sets wide
+// Every field's bounds in one launch (plain offline float fields; ov: one
+// override per field or null); re-points hist/flags to contiguous arrays,
+// synchronizes and sets wide (c2_eb + c2_check for the batch).
+void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const std::vector<EbOverride>* ov);
 void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap);
 void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out);
+// Every field's entropy stage at once (plain offline float fields): streams
+// written to dout + off[k] (at most cap[k] bytes each), sizes into size[k].
+// Host trees are built on up to `workers` pool threads.
+void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, const std::vector<size_t>& off,
+                      const std::vector<size_t>& cap, std::vector<size_t>& size, int workers);
 
 struct D2Work {
   size_t r1 = 0, r2 = 0;

@@ -70,13 +70,10 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 }
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_chunk_bits(Src src, size_t nchunks,
-                                                    const uint8_t* __restrict__ len,
-                                                    uint64_t* __restrict__ bits) {
+__device__ __forceinline__ void chunk_bits_wave(const Src& src, size_t c, const uint8_t* __restrict__ len,
+                                                uint64_t* __restrict__ bits) {
   constexpr int kRounds = (Src::kMaxItems + 63) / 64;
-  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (c >= nchunks) return;  // whole wave
   // every round's symbols are fetched before any is used (overlapped latency)
   uint32_t sym[kRounds][Src::kPer];
   int ns[kRounds];
@@ -94,10 +91,53 @@ __global__ void __launch_bounds__(256) k_chunk_bits(Src src, size_t nchunks,
 }
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t* __restrict__ hist,
-                                              uint32_t win_lo, uint32_t win_n) {
+__global__ void __launch_bounds__(256) k_chunk_bits(Src src, size_t nchunks,
+                                                    const uint8_t* __restrict__ len,
+                                                    uint64_t* __restrict__ bits) {
+  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + (threadIdx.x >> 6);
+  if (c >= nchunks) return;  // whole wave
+  chunk_bits_wave(src, c, len, bits);
+}
+
+// ---- many streams per launch: a job table sorted by first block; block b
+// belongs to the job with the largest blk0 <= b.
+template <class Job>
+__device__ __forceinline__ int job_of(const Job* __restrict__ jobs, int nj, uint32_t b) {
+  int lo = 0, hi = nj - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].blk0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class Src>
+struct EncJob {
+  Src src;
+  size_t nchunks;
+  const uint64_t* code;  // dense codebook of this stream's tree
+  const uint8_t* len;
+  const uint64_t* off;   // global exclusive scan of chunk bits, at this stream's first chunk
+  uint64_t* bits;        // chunk bits, at this stream's first chunk
+  uint32_t* words;       // this stream's zero-filled word array
+  uint32_t blk0;         // first block (kEncWaves chunks per block)
+};
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_chunk_bits_multi(const EncJob<Src>* __restrict__ jobs, int nj) {
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const EncJob<Src>& J = jobs[job_of(jobs, nj, b)];
+  const size_t c = (size_t)(b - J.blk0) * kEncWaves + (threadIdx.x >> 6);
+  if (c >= J.nchunks) return;
+  chunk_bits_wave(J.src, c, J.len, J.bits);
+}
+
+template <class Src>
+__device__ __forceinline__ void hist_block(const Src& src, size_t nchunks, uint32_t* __restrict__ hist,
+                                           uint32_t win_lo, uint32_t win_n, uint32_t blk, uint32_t nblk,
+                                           uint32_t* sh) {
   constexpr int kRounds = (Src::kMaxItems + 63) / 64;
-  extern __shared__ uint32_t sh[];
   for (uint32_t i = threadIdx.x; i < win_n; i += blockDim.x) sh[i] = 0;
   __syncthreads();
   const int lane = threadIdx.x & 63;
@@ -106,8 +146,7 @@ __global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t*
   // per wave instead of one per occurrence on a single address
   const bool zero_out = win_lo > 0;
   uint32_t zeros = 0;
-  for (size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + (threadIdx.x >> 6); c < nchunks;
-       c += (size_t)gridDim.x * kEncWaves) {
+  for (size_t c = (size_t)blk * kEncWaves + (threadIdx.x >> 6); c < nchunks; c += (size_t)nblk * kEncWaves) {
     uint32_t sym[kRounds][Src::kPer];
     int ns[kRounds];
 #pragma unroll
@@ -130,11 +169,34 @@ __global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t*
   __syncthreads();
   // blocks start their flush at different bins so that their global atomics
   // on the popular bins do not all arrive together
-  const uint32_t rot = (uint32_t)(((uint64_t)blockIdx.x * 2654435761u) % win_n);
+  const uint32_t rot = (uint32_t)(((uint64_t)blk * 2654435761u) % win_n);
   for (uint32_t i = threadIdx.x; i < win_n; i += blockDim.x) {
     const uint32_t b = i + rot < win_n ? i + rot : i + rot - win_n;
     if (sh[b]) atomicAdd(&hist[win_lo + b], sh[b]);
   }
+}
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t* __restrict__ hist,
+                                              uint32_t win_lo, uint32_t win_n) {
+  extern __shared__ uint32_t sh[];
+  hist_block(src, nchunks, hist, win_lo, win_n, xcd_block(blockIdx.x, gridDim.x), gridDim.x, sh);
+}
+
+template <class Src>
+struct HistJob {
+  Src src;
+  size_t nchunks;
+  uint32_t* hist;
+  uint32_t blk0, nblk;
+};
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_hist_multi(const HistJob<Src>* __restrict__ jobs, int nj, uint32_t win_lo,
+                                                    uint32_t win_n) {
+  extern __shared__ uint32_t sh[];
+  const HistJob<Src>& J = jobs[job_of(jobs, nj, blockIdx.x)];
+  hist_block(J.src, J.nchunks, J.hist, win_lo, win_n, blockIdx.x - J.blk0, J.nblk, sh);
 }
 
 // Writes the MSB-aligned code cw (L bits) at bit p of an MSB-first word array.
@@ -149,25 +211,13 @@ __device__ __forceinline__ void put_bits(uint32_t* st, uint32_t p, uint64_t cw, 
 // symbols at offsets from a wave prefix sum) and then stores the chunk's
 // words with consecutive lanes on consecutive words. The words a chunk
 // shares with its neighbours (a partial first or last word) are merged with
-// atomicOr into the zero-filled output; the others are plain stores.
+// atomicOr into the zero-filled output; the others are plain stores. b0 is
+// the chunk's first bit within its stream. Called by whole blocks.
 template <class Src>
-__global__ void __launch_bounds__(256) k_encode(Src src, size_t nchunks,
-                                                const uint64_t* __restrict__ code,
-                                                const uint8_t* __restrict__ len,
-                                                const uint64_t* __restrict__ off,
-                                                const uint64_t* __restrict__ bits,
-                                                uint32_t* __restrict__ words) {
-  constexpr int kStage = Src::kMaxItems * Src::kPer * 2 + 2;  // 64-bit codes at most
-  __shared__ uint32_t stage[kEncWaves][kStage];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + wv;
-  const bool live = c < nchunks;  // uniform per wave
-  uint32_t* st = stage[wv];
-  uint64_t b0 = 0, nbits = 0;
-  if (live) {
-    b0 = off[c];
-    nbits = bits[c];
-  }
+__device__ __forceinline__ void encode_wave(const Src& src, size_t c, bool live, uint64_t b0, uint64_t nbits,
+                                            const uint64_t* __restrict__ code, const uint8_t* __restrict__ len,
+                                            uint32_t* __restrict__ words, uint32_t* st) {
+  const int lane = threadIdx.x & 63;
   const uint32_t sh = (uint32_t)(b0 & 31), nw = (uint32_t)((sh + nbits + 31) >> 5);
   for (uint32_t k = lane; k < nw; k += 64) st[k] = 0;
   __syncthreads();
@@ -213,6 +263,35 @@ __global__ void __launch_bounds__(256) k_encode(Src src, size_t nchunks,
       else words[w0 + k] = word;
     }
   }
+}
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_encode(Src src, size_t nchunks,
+                                                const uint64_t* __restrict__ code,
+                                                const uint8_t* __restrict__ len,
+                                                const uint64_t* __restrict__ off,
+                                                const uint64_t* __restrict__ bits,
+                                                uint32_t* __restrict__ words) {
+  constexpr int kStage = Src::kMaxItems * Src::kPer * 2 + 2;  // 64-bit codes at most
+  __shared__ uint32_t stage[kEncWaves][kStage];
+  const int wv = threadIdx.x >> 6;
+  const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + wv;
+  const bool live = c < nchunks;  // uniform per wave
+  encode_wave(src, c, live, live ? off[c] : 0, live ? bits[c] : 0, code, len, words, stage[wv]);
+}
+
+template <class Src>
+__global__ void __launch_bounds__(256) k_encode_multi(const EncJob<Src>* __restrict__ jobs, int nj) {
+  constexpr int kStage = Src::kMaxItems * Src::kPer * 2 + 2;
+  __shared__ uint32_t stage[kEncWaves][kStage];
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const EncJob<Src>& J = jobs[job_of(jobs, nj, b)];
+  const int wv = threadIdx.x >> 6;
+  const size_t c = (size_t)(b - J.blk0) * kEncWaves + wv;
+  const bool live = c < J.nchunks;
+  const uint64_t base = J.off[0];
+  encode_wave(J.src, c, live, live ? J.off[c] - base : 0, live ? J.bits[c] : 0, J.code, J.len, J.words,
+              stage[wv]);
 }
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v);

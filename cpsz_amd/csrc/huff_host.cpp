@@ -164,18 +164,22 @@ void serialize(const BuildWork& wk, uint32_t leaves, int32_t root, size_t N, Huf
 
 }  // namespace
 
-void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEncodeTable& t) {
+namespace {
+
+// The tree from the present symbols in increasing order: sym(i), freq(i) for
+// i in [0, k).
+template <class SymAt, class FreqAt>
+void build_from(size_t k, SymAt sym_at, FreqAt freq_at, uint32_t state_num, HuffEncodeTable& t) {
   // fields are overwritten, not reassigned: a reused table keeps its allocations
   t.state_num = state_num;
   t.max_len = 0;
-  size_t leaves = 0;
+  const size_t leaves = k;
   uint64_t total = 0;
-  for (size_t s = 0; s < nbins; s++)
-    if (freq[s]) {
-      CPSZ_REQUIRE(s < 2 * (size_t)state_num, kErrInternal, "huffman symbol out of range");
-      leaves++;
-      total += freq[s];
-    }
+  for (size_t i = 0; i < k; i++) {
+    CPSZ_REQUIRE(sym_at(i) < 2 * (size_t)state_num, kErrInternal, "huffman symbol out of range");
+    CPSZ_REQUIRE(i == 0 || sym_at(i) > sym_at(i - 1), kErrInternal, "huffman symbols out of order");
+    total += freq_at(i);
+  }
   CPSZ_REQUIRE(leaves > 0, kErrInternal, "huffman: empty input");
   CPSZ_REQUIRE(2 * leaves < (1u << 24), kErrUnsupported, "huffman alphabet too large");
   CPSZ_REQUIRE(total < (1ull << 40), kErrUnsupported, "huffman symbol count too large");
@@ -185,10 +189,13 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
   wk.kids.resize(2 * leaves);
   Queue q(wk.heap.data());
   uint32_t next = 0;
-  for (size_t s = 0; s < nbins; s++) {
-    if (!freq[s]) continue;
-    wk.sym[next] = (uint32_t)s;
-    q.push(freq[s] << 24 | next);
+  size_t counted = 0;  // node_count counts leaves below state_num (src/sz_compression_utils.cpp:66-68)
+  for (size_t i = 0; i < k; i++) {
+    const uint64_t f = freq_at(i);
+    CPSZ_REQUIRE(f > 0, kErrInternal, "huffman: zero frequency listed");
+    wk.sym[next] = (uint32_t)sym_at(i);
+    counted += sym_at(i) < state_num;
+    q.push(f << 24 | next);
     next++;
   }
   // Merge: the reference passes two qremove() calls as arguments to
@@ -204,9 +211,6 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
     next++;
   }
   const int32_t root = (int32_t)(q.top() & 0xFFFFFF);
-  // node_count counts leaves below state_num (src/sz_compression_utils.cpp:66-68).
-  size_t counted = 0;
-  for (size_t s = 0; s < nbins && s < state_num; s++) counted += freq[s] != 0;
   CPSZ_REQUIRE(counted == leaves, kErrUnsupported, "symbol beyond the coder's state count");
   t.node_count = 2 * counted - 1;
   t.single = root < (int32_t)leaves;
@@ -219,6 +223,24 @@ void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEnco
   if (w == 1) serialize<uint8_t>(wk, (uint32_t)leaves, root, N, t);
   else if (w == 2) serialize<uint16_t>(wk, (uint32_t)leaves, root, N, t);
   else serialize<uint32_t>(wk, (uint32_t)leaves, root, N, t);
+}
+
+}  // namespace
+
+void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEncodeTable& t) {
+  thread_local std::vector<uint32_t> present;
+  present.clear();
+  for (size_t s = 0; s < nbins; s++)
+    if (freq[s]) present.push_back((uint32_t)s);
+  build_from(
+      present.size(), [&](size_t i) { return (size_t)present[i]; },
+      [&](size_t i) { return freq[present[i]]; }, state_num, t);
+}
+
+void huff_build_sparse(const uint32_t* syms, const uint32_t* freq, size_t k, uint32_t state_num,
+                       HuffEncodeTable& t) {
+  build_from(
+      k, [&](size_t i) { return (size_t)syms[i]; }, [&](size_t i) { return (uint64_t)freq[i]; }, state_num, t);
 }
 
 namespace {

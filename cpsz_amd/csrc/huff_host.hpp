@@ -27,6 +27,11 @@ struct HuffEncodeTable {
 
 // Build from frequencies over symbols [0, nbins). Symbols must be < 2*state_num.
 void huff_build(const uint64_t* freq, size_t nbins, uint32_t state_num, HuffEncodeTable& t);
+// The same from the present symbols only: syms[0..k) strictly increasing,
+// freq[i] > 0 the count of syms[i] (the tree depends only on these pairs in
+// symbol order, so the result is identical to the dense form's).
+void huff_build_sparse(const uint32_t* syms, const uint32_t* freq, size_t k, uint32_t state_num,
+                       HuffEncodeTable& t);
 
 // Parsed tree for decoding: node arrays in serialized (preorder) numbering.
 struct HuffDecodeTree {
