@@ -684,6 +684,20 @@ struct SinkIds2d {
   }
 };
 
+// The id streams when every id symbol is below 256 (the float codec's ids
+// stay below 90): staged as bytes, so a block's whole output fits its LDS
+// stage and leaves in coalesced stores, and decoded up to 3 codes per step.
+struct SinkIds2dB {
+  static constexpr int kBytes = 1;
+  uint16_t* su;
+  uint16_t* sv;
+  size_t n;
+  __device__ void put(uint64_t p, uint32_t s) const {
+    if (p < n) su[p] = (uint16_t)s;
+    else sv[p - n] = (uint16_t)s;
+  }
+};
+
 // block-major position -> row-major element
 struct SinkCodes2d {
   static constexpr int kBytes = 2;
@@ -840,12 +854,14 @@ struct U16Hist {
 // (kUnpWaves 12x12 blocks per workgroup, from workgroup blk0).
 struct UnpJob {
   const uint16_t* q;
-  const float* x;        // values the codec ran on (gather source)
+  const float* x;        // gather: values the codec ran on; scatter: the stream's unpredictable values
   int r1, r2, nby;
   size_t nb;
   uint32_t* cnt;         // per-block zero counts
   const uint64_t* off;   // global exclusive scan of the counts, at this component's first block
-  float* out;            // this component's unpredictable values
+  float* out;            // gather: this component's unpredictable values; scatter: per-element values
+  uint64_t count;        // scatter: values in the stream
+  int* err;              // scatter: set when the codes ask for more values than the stream holds
   uint32_t blk0;
 };
 
@@ -878,6 +894,28 @@ __global__ void __launch_bounds__(256) k_gather_unpred_multi(const UnpJob* __res
     const bool zero = k < v.m && J.q[e] == 0;
     const uint64_t mask = __ballot(zero);
     if (zero) J.out[o + __popcll(mask & below)] = J.x[e];
+    o += __popcll(mask);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_scatter_unpred_multi(const UnpJob* __restrict__ jobs, int nj) {
+  const UnpJob& J = jobs[job_of(jobs, nj, blockIdx.x)];
+  const size_t b = (size_t)(blockIdx.x - J.blk0) * kUnpWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (b >= J.nb) return;
+  const BlockView v(b, J.r1, J.r2, J.nby);
+  uint64_t o = J.off[b] - J.off[0];
+  const uint64_t below = (1ull << lane) - 1;
+  for (int k0 = 0; k0 < v.m; k0 += 64) {
+    const int k = k0 + lane;
+    const size_t e = k < v.m ? v.elem(k, J.r2) : 0;
+    const bool zero = k < v.m && J.q[e] == 0;
+    const uint64_t mask = __ballot(zero);
+    if (zero) {
+      const uint64_t r = o + __popcll(mask & below);
+      if (r < J.count) J.out[e] = J.x[r];
+      else *J.err = 1;
+    }
     o += __popcll(mask);
   }
 }
@@ -2017,12 +2055,23 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   auto prep = [&](int c) {
     prepare_decoder(p.comp[c].h.tree.data(), p.comp[c].h.tree.size(), p.comp[c].h.node_count, hc[c]);
   };
+  uint32_t id_max = 0;
+  {
+    HuffDecodeTree t;
+    CPSZ_REQUIRE(huff_parse_tree(h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, t), kErrStream,
+                 "malformed id tree");
+    id_max = max_symbol(t);
+  }
   {
     HelperJob u_ready([&] { prep(0); }, !ctx.batch_worker);
     uint32_t* w = stage_bits(h_eb, "d2.w.eb");
     DeviceDecoder dd = upload_decoder(ctx, h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, "d2.eb");
-    decode_stream(ctx, dd, w, h_eb.nbytes, log ? n : 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb",
-                  [&] { prep(1); });
+    if (id_max < 256)
+      decode_stream(ctx, dd, w, h_eb.nbytes, log ? n : 2 * n, SinkIds2dB{sid_u, sid_v, n}, "d2.eb",
+                    [&] { prep(1); });
+    else
+      decode_stream(ctx, dd, w, h_eb.nbytes, log ? n : 2 * n, SinkIds2d{sid_u, sid_v, n}, "d2.eb",
+                    [&] { prep(1); });
     u_ready.join();  // rethrows a malformed code tree
   }
   {
@@ -2037,14 +2086,83 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
   }
   ctx.mark("huffman");
   d2_unpred(ctx, d_in, p, w);
-  {
-    HuffDecodeTree t;
-    CPSZ_REQUIRE(huff_parse_tree(h_eb.tree.data(), h_eb.tree.size(), h_eb.node_count, t), kErrStream,
-                 "malformed id tree");
-    w.wide = max_symbol(t) > 81;
-  }
+  w.wide = id_max > 81;
   ctx.mark("unpred");
 }
+
+// Unpredictable values of every field of a batch (float): the stream's
+// values copied to aligned arrays in one assembly launch, the per-block zero
+// counts of every component, one global scan, one scatter launch.
+void d2_unpred_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<D2Plan>& plan,
+                     D2Work* const* w, size_t nf) {
+  hipStream_t st = ctx.stream();
+  size_t Z = 0, V = 0, npieces = 0;
+  constexpr size_t kPiece = 65536;
+  std::vector<size_t> zb(nf), vb(2 * nf);
+  for (size_t k = 0; k < nf; k++) {
+    CPSZ_REQUIRE(!w[k]->f64 && !w[k]->log, kErrInternal, "batched unpredictable scatter: plain float fields only");
+    zb[k] = Z;
+    Z += 2 * plan[k].nb;
+    for (int c = 0; c < 2; c++) {
+      vb[2 * k + c] = V;
+      V += (plan[k].comp[c].unp_count + 63) & ~(size_t)63;
+      npieces += div_up(4 * plan[k].comp[c].unp_count, kPiece);
+    }
+  }
+  uint32_t* zc = ctx.buf<uint32_t>("b2u.zc", Z + 1);
+  uint64_t* zoff = ctx.buf<uint64_t>("b2u.zoff", Z + 1);
+  float* vals = ctx.buf<float>("b2u.vals", V + 1);
+  int* errs = ctx.buf<int>("b2u.err", 4 * nf);
+  Arena A;
+  const size_t bytes = 2 * nf * sizeof(UnpJob) + npieces * sizeof(CopyPiece) + 64;
+  A.h = (uint8_t*)ctx.host_buf("b2u.tab", bytes);
+  A.d = ctx.buf<uint8_t>("b2u.tab", bytes);
+  A.cap = bytes;
+  UnpJob* uj;
+  UnpJob* d_uj = A.take(2 * nf, &uj);
+  CopyPiece* cp;
+  CopyPiece* d_cp = A.take(npieces, &cp);
+  size_t np = 0;
+  uint32_t blk = 0;
+  for (size_t k = 0; k < nf; k++) {
+    const D2Plan& p = plan[k];
+    D2Work& W = *w[k];
+    W.err = errs + 4 * k;
+    for (int c = 0; c < 2; c++) {
+      UnpJob& u = uj[2 * k + c];
+      u = UnpJob{};
+      u.q = c ? W.qv : W.qu;
+      u.x = vals + vb[2 * k + c];
+      u.r1 = (int)W.r1;
+      u.r2 = (int)W.r2;
+      u.nby = p.nby;
+      u.nb = p.nb;
+      u.cnt = zc + zb[k] + c * p.nb;
+      u.off = zoff + zb[k] + c * p.nb;
+      u.out = W.raw + c * W.r1 * W.r2;
+      u.count = p.comp[c].unp_count;
+      u.err = W.err;
+      u.blk0 = blk;
+      blk += (uint32_t)unp_grid(p.nb);
+      const size_t len = 4 * p.comp[c].unp_count;
+      for (size_t o = 0; o < len; o += kPiece)
+        cp[np++] = CopyPiece{4 * vb[2 * k + c] + o, d_in[k] + p.comp[c].unp_off + o,
+                             (uint32_t)std::min(kPiece, len - o), 0};
+    }
+  }
+  CPSZ_HIP(hipMemcpyAsync(A.d, A.h, A.used, hipMemcpyHostToDevice, st));
+  k_fill_u32<<<1, 256, 0, st>>>((uint32_t*)errs, 4 * nf, 0);
+  if (np) k_assemble<<<(unsigned)np, 256, 0, st>>>(d_cp, (uint8_t*)vals);
+  k_block_zeros_multi<<<blk, 256, 0, st>>>(d_uj, (int)(2 * nf));
+  CPSZ_HIP(hipGetLastError());
+  scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
+  k_scatter_unpred_multi<<<blk, 256, 0, st>>>(d_uj, (int)(2 * nf));
+  CPSZ_HIP(hipGetLastError());
+  // the pinned table is reused by the next call only after the stream passes here
+  CPSZ_HIP(hipStreamSynchronize(st));
+}
+
+constexpr uint32_t kIdWarm = 2048;  // speculative warm-up of the id streams, bits
 
 // Batched form: every field's headers parsed and trees tabulated on the
 // workers; all decoders uploaded in one copy and all bitstreams staged in one
@@ -2088,30 +2206,29 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
   stage_bits_multi(ctx, bc.data(), (int)(3 * nf), "b2d.bits");
   ctx.mark("stage");
   std::vector<DecJob<SinkIds2d>> ji(nf);
+  std::vector<DecJob<SinkIds2dB>> ji8(nf);
   std::vector<DecJob<SinkCodes2d>> jc(2 * nf);
+  bool ids8 = true;
+  for (size_t k = 0; k < nf; k++) ids8 = ids8 && max_symbol(hd[3 * k].d) < 256;
   for (size_t k = 0; k < nf; k++) {
     const size_t r1 = w[k]->r1, r2 = w[k]->r2, n = r1 * r2;
     const D2Plan& p = plan[k];
-    ji[k] = {&dd[3 * k], bc[3 * k].dst, p.h_eb.nbytes, 2 * n, SinkIds2d{w[k]->sid_u, w[k]->sid_v, n}, "b2d"};
+    // eb ids: codes of 1-5 bits over long runs of equal ids resynchronize
+    // slowly from an arbitrary bit, so their speculative warm-up is longer
+    ji[k] = {&dd[3 * k], bc[3 * k].dst, p.h_eb.nbytes, 2 * n, SinkIds2d{w[k]->sid_u, w[k]->sid_v, n}, "b2d",
+             kIdWarm};
+    ji8[k] = {&dd[3 * k], bc[3 * k].dst, p.h_eb.nbytes, 2 * n, SinkIds2dB{w[k]->sid_u, w[k]->sid_v, n}, "b2d",
+              kIdWarm};
     uint16_t* qs[2] = {w[k]->qu, w[k]->qv};
     for (int c = 0; c < 2; c++)
       jc[2 * k + c] = {&dd[3 * k + 1 + c], bc[3 * k + 1 + c].dst, p.comp[c].h.nbytes, n,
                        SinkCodes2d{qs[c], (int)r1, (int)r2, p.nby}, "b2d"};
   }
-  decode_streams_multi(ctx, ji.data(), (int)nf, "b2d.ids");
+  if (ids8) decode_streams_multi(ctx, ji8.data(), (int)nf, "b2d.ids");
+  else decode_streams_multi(ctx, ji.data(), (int)nf, "b2d.ids");
   decode_streams_multi(ctx, jc.data(), (int)(2 * nf), "b2d.codes");
   ctx.mark("huffman");
-  // the workers' streams start after the decode
-  CPSZ_HIP(hipStreamSynchronize(st));
-  {
-    std::vector<std::function<void(Ctx&)>> jobs(nf);
-    for (size_t k = 0; k < nf; k++)
-      jobs[k] = [&, k](Ctx& wc) {
-        d2_unpred(wc, d_in[k], plan[k], *w[k]);
-        CPSZ_HIP(hipStreamSynchronize(wc.stream()));
-      };
-    pool_run(pool_for(ctx.device()), jobs, workers);
-  }
+  d2_unpred_batch(ctx, d_in, plan, w, nf);
   ctx.mark("unpred");
 }
 

@@ -18,6 +18,11 @@ __global__ void k_scatter_codebook(const uint32_t* syms, const uint64_t* codes, 
   len[syms[i]] = lens[i];
 }
 
+__device__ __forceinline__ void dec_resync(const DecTree& t, const uint32_t* slut, const uint16_t* ms,
+                                           const uint32_t* __restrict__ w, const uint32_t* sw, uint64_t lbase,
+                                           uint32_t lcount, uint64_t end, uint64_t s0, uint64_t os, uint64_t oe,
+                                           uint32_t oc, uint64_t& ex, uint32_t& c, int* __restrict__ err);
+
 // Block `blk` of one stream's speculative / fix-up pass (k_dec_spec runs one
 // stream's blocks, k_dec_spec_multi every stream's blocks in one grid).
 __device__ __forceinline__ void dec_spec_block(const DecTree& t, const uint32_t* __restrict__ w, uint64_t nbits,
@@ -25,7 +30,7 @@ __device__ __forceinline__ void dec_spec_block(const DecTree& t, const uint32_t*
                                                const uint64_t* __restrict__ prev_start,
                                                uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
                                                uint32_t* __restrict__ cnt, int* __restrict__ err, int first,
-                                               uint32_t blk, uint32_t* slut, uint32_t* sw) {
+                                               uint32_t blk, uint32_t* slut, uint32_t* sw, uint16_t* ms) {
   const uint32_t i = blk * blockDim.x + threadIdx.x;
   bool work = i < nseg;
   uint64_t s0 = 0;
@@ -46,6 +51,8 @@ __device__ __forceinline__ void dec_spec_block(const DecTree& t, const uint32_t*
   uint64_t lbase;
   const uint32_t lcount = stage_words(sw, w, nbits, S, blk * blockDim.x, lbase);
   __syncthreads();
+  build_mstep(slut, ms);
+  __syncthreads();
   if (!work) return;
   uint64_t end = (uint64_t)(i + 1) * S;
   if (end > nbits) end = nbits;
@@ -58,56 +65,22 @@ __device__ __forceinline__ void dec_spec_block(const DecTree& t, const uint32_t*
     // one there: its start then equals the previous segment's exit, and the
     // fix-up passes skip the segment.
     br.init(w, s0 - (s0 < kDecWarm ? s0 : kDecWarm), sw, lbase, lcount);
-    while (br.pos < s0) {
-      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
-        *err = 1;
-        break;
-      }
-    }
+    uint32_t skipped = 0;
+    if (!dec_until(t, slut, ms, br, s0, skipped)) *err = 1;
     s0 = br.pos;
-    while (br.pos < end) {
-      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
-        *err = 1;
-        break;
-      }
-      c++;
-    }
+    if (!dec_until(t, slut, ms, br, end, c)) *err = 1;
     start[i] = s0;
     exitp[i] = br.pos;
     cnt[i] = c;
     return;
   }
-  br.init(w, s0, sw, lbase, lcount);
   // New start: decode the new path and the previous pass's path of this
   // segment in lockstep (always advancing the one behind) until they reach
   // the same code boundary; from there both paths coincide, so the previous
   // pass's exit and the rest of its count carry over. Prefix codes resync
   // within a few code words, so a fix-up costs a few symbols, not a segment.
-  BitReader old;
-  old.init(w, prev_start[i], sw, lbase, lcount);
-  const uint64_t old_exit = prev_exit[i];
-  const uint32_t old_cnt = cnt[i];
-  uint32_t c_old = 0;
   uint64_t ex = 0;
-  for (;;) {
-    if (br.pos >= end) {
-      ex = br.pos;
-      break;
-    }
-    if (br.pos == old.pos) {
-      ex = old_exit;
-      c += old_cnt - c_old;
-      break;
-    }
-    const bool adv_new = br.pos < old.pos;
-    if (dec_one(t, slut, adv_new ? br : old) == 0xFFFFFFFFu) {
-      *err = 1;
-      ex = br.pos;
-      break;
-    }
-    c += adv_new;
-    c_old += !adv_new;
-  }
+  dec_resync(t, slut, ms, w, sw, lbase, lcount, end, s0, prev_start[i], prev_exit[i], cnt[i], ex, c, err);
   start[i] = s0;
   exitp[i] = ex;
   cnt[i] = c;
@@ -122,8 +95,10 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
                                                   int* __restrict__ err, int first) {
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
+  __shared__ uint16_t ms[1 << kLutBits];
   (void)changed;
-  dec_spec_block(t, w, nbits, S, nseg, prev_exit, prev_start, start, exitp, cnt, err, first, blockIdx.x, slut, sw);
+  dec_spec_block(t, w, nbits, S, nseg, prev_exit, prev_start, start, exitp, cnt, err, first, blockIdx.x, slut, sw,
+                 ms);
 }
 
 // ---- block-settled passes (the multi-stream decoder). A pass leaves every
@@ -134,14 +109,19 @@ __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __r
 // (start os, exit oe, count oc) in lockstep, always advancing the one behind,
 // until the two meet at a code boundary (from there they coincide, so the old
 // exit and the rest of its count carry over) or the new path reaches `end`.
-__device__ __forceinline__ void dec_resync(const DecTree& t, const uint32_t* slut, const uint32_t* __restrict__ w,
-                                           const uint32_t* sw, uint64_t lbase, uint32_t lcount, uint64_t end,
-                                           uint64_t s0, uint64_t os, uint64_t oe, uint32_t oc, uint64_t& ex,
-                                           uint32_t& c, int* __restrict__ err) {
+__device__ __forceinline__ void dec_resync(const DecTree& t, const uint32_t* slut, const uint16_t* ms,
+                                           const uint32_t* __restrict__ w, const uint32_t* sw, uint64_t lbase,
+                                           uint32_t lcount, uint64_t end, uint64_t s0, uint64_t os, uint64_t oe,
+                                           uint32_t oc, uint64_t& ex, uint32_t& c, int* __restrict__ err) {
   BitReader br, old;
   br.init(w, s0, sw, lbase, lcount);
   old.init(w, os, sw, lbase, lcount);
   uint32_t cn = 0, co = 0;
+  // A bounded lockstep: paths that have not met after kLock codes rarely meet
+  // soon (the codes resynchronize slowly there), so the new path then runs
+  // alone to `end` with multi-code steps, and its own count is the count.
+  constexpr int kLock = 48;
+  int steps = 0;
   for (;;) {
     if (br.pos >= end) {
       ex = br.pos;
@@ -150,6 +130,11 @@ __device__ __forceinline__ void dec_resync(const DecTree& t, const uint32_t* slu
     if (br.pos == old.pos) {
       ex = oe;
       cn += oc - co;
+      break;
+    }
+    if (++steps > kLock) {
+      if (!dec_until(t, slut, ms, br, end, cn)) *err = 1;
+      ex = br.pos;
       break;
     }
     const bool adv_new = br.pos < old.pos;
@@ -171,10 +156,10 @@ __device__ __forceinline__ void dec_resync(const DecTree& t, const uint32_t* slu
 // given), so blockDim iterations suffice; a segment that adopts a still-wrong
 // predecessor exit is corrected in a later iteration.
 // lex: blockDim LDS words.
-__device__ __forceinline__ void dec_settle(const DecTree& t, const uint32_t* slut, const uint32_t* __restrict__ w,
-                                           const uint32_t* sw, uint64_t lbase, uint32_t lcount, uint64_t end,
-                                           bool valid, uint64_t& st, uint64_t& ex, uint32_t& c, uint64_t* lex,
-                                           int* __restrict__ err) {
+__device__ __forceinline__ void dec_settle(const DecTree& t, const uint32_t* slut, const uint16_t* ms,
+                                           const uint32_t* __restrict__ w, const uint32_t* sw, uint64_t lbase,
+                                           uint32_t lcount, uint64_t end, bool valid, uint64_t& st, uint64_t& ex,
+                                           uint32_t& c, uint64_t* lex, int* __restrict__ err) {
   for (uint32_t it = 0; it < blockDim.x; it++) {
     __syncthreads();
     lex[threadIdx.x] = ex;
@@ -189,7 +174,7 @@ __device__ __forceinline__ void dec_settle(const DecTree& t, const uint32_t* slu
     if (redo) {
       uint64_t e2;
       uint32_t c2;
-      dec_resync(t, slut, w, sw, lbase, lcount, end, want, st, ex, c, e2, c2, err);
+      dec_resync(t, slut, ms, w, sw, lbase, lcount, end, want, st, ex, c, e2, c2, err);
       st = want;
       ex = e2;
       c = c2;
@@ -202,7 +187,8 @@ __device__ __forceinline__ void dec_pass_block(const DecTree& t, const uint32_t*
                                                const uint64_t* __restrict__ prev_start,
                                                uint64_t* __restrict__ start, uint64_t* __restrict__ exitp,
                                                uint32_t* __restrict__ cnt, int* __restrict__ err, int first,
-                                               uint32_t blk, uint32_t* slut, uint32_t* sw, uint64_t* lex) {
+                                               uint32_t blk, uint32_t* slut, uint32_t* sw, uint64_t* lex,
+                                               uint16_t* ms, uint64_t warm) {
   const uint32_t i = blk * blockDim.x + threadIdx.x;
   const bool valid = i < nseg;
   uint64_t st = 0, ex = 0, s0 = 0;
@@ -231,40 +217,32 @@ __device__ __forceinline__ void dec_pass_block(const DecTree& t, const uint32_t*
   uint64_t lbase;
   const uint32_t lcount = stage_words(sw, w, nbits, S, blk * blockDim.x, lbase);
   __syncthreads();
+  build_mstep(slut, ms);
+  __syncthreads();
   uint64_t end = (uint64_t)(i + 1) * S;
   if (end > nbits) end = nbits;
   if (first && valid) {
-    // Warm-up: start kDecWarm bits before the segment and take as its start
+    // Warm-up: start `warm` bits before the segment and take as its start
     // the first code boundary at or after i * S. A prefix-code path usually
     // resyncs within a few code words, so this path is often already the true
     // one there.
     const uint64_t a = (uint64_t)i * S;
     BitReader br;
-    br.init(w, a - (a < kDecWarm ? a : kDecWarm), sw, lbase, lcount);
-    while (br.pos < a) {
-      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
-        *err = 1;
-        break;
-      }
-    }
+    br.init(w, a - (a < warm ? a : warm), sw, lbase, lcount);
+    uint32_t skipped = 0;
+    if (!dec_until(t, slut, ms, br, a, skipped)) *err = 1;
     st = br.pos;
-    while (br.pos < end) {
-      if (dec_one(t, slut, br) == 0xFFFFFFFFu) {
-        *err = 1;
-        break;
-      }
-      c++;
-    }
+    if (!dec_until(t, slut, ms, br, end, c)) *err = 1;
     ex = br.pos;
   } else if (work) {
     uint64_t e2;
     uint32_t c2;
-    dec_resync(t, slut, w, sw, lbase, lcount, end, s0, st, ex, c, e2, c2, err);
+    dec_resync(t, slut, ms, w, sw, lbase, lcount, end, s0, st, ex, c, e2, c2, err);
     st = s0;
     ex = e2;
     c = c2;
   }
-  dec_settle(t, slut, w, sw, lbase, lcount, end, valid, st, ex, c, lex, err);
+  dec_settle(t, slut, ms, w, sw, lbase, lcount, end, valid, st, ex, c, lex, err);
   if (valid) {
     start[i] = st;
     exitp[i] = ex;
@@ -290,14 +268,15 @@ __global__ void __launch_bounds__(256) k_dec_spec_multi(const MDecJob* __restric
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
   __shared__ uint64_t lex[256];
+  __shared__ uint16_t ms[1 << kLutBits];
   const int j = mdec_job(blk0, nj, blockIdx.x);
   const MDecJob& J = jobs[j];
   if (first)
     dec_pass_block(J.t, J.w, J.nbits, J.S, J.nseg, nullptr, nullptr, J.start[0], J.ex[0], J.cnt, err + j, 1,
-                   blockIdx.x - blk0[j], slut, sw, lex);
+                   blockIdx.x - blk0[j], slut, sw, lex, ms, J.warm);
   else
     dec_pass_block(J.t, J.w, J.nbits, J.S, J.nseg, J.ex[cur], J.start[cur], J.start[cur ^ 1], J.ex[cur ^ 1],
-                   J.cnt, err + j, 0, blockIdx.x - blk0[j], slut, sw, lex);
+                   J.cnt, err + j, 0, blockIdx.x - blk0[j], slut, sw, lex, ms, J.warm);
 }
 
 // changed[j] = 1 when a segment of job j does not start at its predecessor's exit

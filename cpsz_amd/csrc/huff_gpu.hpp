@@ -411,6 +411,65 @@ __device__ __forceinline__ uint32_t dec_one(const DecTree& t, const uint32_t* lu
   return 0xFFFFFFFFu;
 }
 
+// Multi-code steps for the passes that only locate code boundaries: for each
+// kLutBits-bit window, the number n of whole codes (each at most kLutBits
+// bits, decoded greedily from the window's start) and their total length,
+// stored as n << 8 | length (0 when the first code is longer). Built from the
+// level-1 table in LDS by the whole block; callers synchronize before use.
+__device__ __forceinline__ void build_mstep(const uint32_t* slut, uint16_t* ms) {
+  constexpr uint32_t kMask = (1u << kLutBits) - 1;
+  for (uint32_t w = threadIdx.x; w <= kMask; w += blockDim.x) {
+    uint32_t pos = 0, n = 0;
+    for (;;) {
+      const uint32_t e = slut[(w << pos) & kMask];
+      const uint32_t L = e & 0xFF;
+      if (!L || (L & 0x80u) || pos + L > (uint32_t)kLutBits) break;
+      pos += L;
+      n++;
+    }
+    ms[w] = (uint16_t)(n << 8 | pos);
+  }
+}
+
+// Decodes from br's position up to the first code boundary at or after
+// `stop`, adding the number of codes to c: a window's run of short codes at a
+// time while the run ends at or before `stop` (so no boundary at or after
+// `stop` is stepped over), one code at a time otherwise. False on a bad code.
+__device__ __forceinline__ bool dec_until(const DecTree& t, const uint32_t* slut, const uint16_t* ms, BitReader& br,
+                                          uint64_t stop, uint32_t& c) {
+  while (br.pos < stop) {
+    const uint32_t m = ms[br.peek() >> (32 - kLutBits)];
+    const uint32_t len = m & 0xFF;
+    if (len && br.pos + len <= stop) {
+      br.skip((int)len);
+      c += m >> 8;
+      continue;
+    }
+    if (dec_one(t, slut, br) == 0xFFFFFFFFu) return false;
+    c++;
+  }
+  return true;
+}
+
+// Multi-code decode for streams whose symbols fit a byte: for each window,
+// up to 3 whole short codes: length | n << 4 | sym0 << 8 | sym1 << 16 | sym2 << 24
+// (n = 0 when the first code is longer than the window).
+__device__ __forceinline__ void build_mdec8(const uint32_t* slut, uint32_t* md) {
+  constexpr uint32_t kMask = (1u << kLutBits) - 1;
+  for (uint32_t w = threadIdx.x; w <= kMask; w += blockDim.x) {
+    uint32_t pos = 0, n = 0, syms = 0;
+    while (n < 3) {
+      const uint32_t e = slut[(w << pos) & kMask];
+      const uint32_t L = e & 0xFF;
+      if (!L || (L & 0x80u) || pos + L > (uint32_t)kLutBits || (e >> 8) > 0xFFu) break;
+      syms |= (e >> 8) << (8 * n);
+      pos += L;
+      n++;
+    }
+    md[w] = pos | n << 4 | syms << 8;
+  }
+}
+
 // Phase 1 / fix-up: decode from start to the first boundary >= seg_end.
 __global__ void __launch_bounds__(256) k_dec_spec(DecTree t, const uint32_t* __restrict__ w, uint64_t nbits,
                                                   uint32_t S, uint32_t nseg, const uint64_t* __restrict__ prev_exit,
@@ -430,6 +489,7 @@ struct MDecJob {
   const uint32_t* w;
   uint64_t nbits, n;
   uint32_t S, nseg, blk0, nblk;
+  uint32_t warm;  // bits decoded before a segment in the speculative pass
   uint64_t* start[2];
   uint64_t* ex[2];
   uint32_t* cnt;
@@ -466,12 +526,16 @@ __device__ __forceinline__ void dec_write_block(const DecTree& t, const uint32_t
                                                 const uint64_t* __restrict__ exitp,
                                                 const uint64_t* __restrict__ off, uint64_t n, const Sink& sink,
                                                 uint32_t blk, uint32_t* slut, uint32_t* sw,
-                                                typename DecStage<Sink::kBytes>::T* stage) {
+                                                typename DecStage<Sink::kBytes>::T* stage, uint32_t* md) {
   using Stage = DecStage<Sink::kBytes>;
   for (int k = threadIdx.x; k < (1 << kLutBits); k += blockDim.x) slut[k] = t.lut[k];
   uint64_t lbase;
   const uint32_t lcount = stage_words(sw, w, nbits, S, blk * blockDim.x, lbase);
   __syncthreads();
+  if (Sink::kBytes == 1) {
+    build_mdec8(slut, md);
+    __syncthreads();
+  }
   const uint32_t b0 = blk * blockDim.x;
   const uint32_t b1 = min(b0 + blockDim.x, nseg);  // off[] holds nseg + 1 entries
   const uint64_t o_begin = min(off[b0], n), o_end = min(off[b1], n);
@@ -483,7 +547,25 @@ __device__ __forceinline__ void dec_write_block(const DecTree& t, const uint32_t
     const uint64_t end = exitp[i];
     uint64_t o = off[i];
     if (staged) {
-      while (br.pos < end && o < o_end) stage[o++ - o_begin] = (typename Stage::T)dec_one(t, slut, br);
+      if (Sink::kBytes == 1) {
+        // up to 3 codes per step while they all start before `end` and fit
+        while (br.pos < end && o < o_end) {
+          const uint32_t m = md[br.peek() >> (32 - kLutBits)];
+          const uint32_t len = m & 0xF, k = (m >> 4) & 3;
+          if (k && br.pos + len <= end && o + k <= o_end) {
+            typename Stage::T* d = stage + (o - o_begin);
+            d[0] = (typename Stage::T)(m >> 8);
+            if (k > 1) d[1] = (typename Stage::T)(m >> 16);
+            if (k > 2) d[2] = (typename Stage::T)(m >> 24);
+            o += k;
+            br.skip((int)len);
+          } else {
+            stage[o++ - o_begin] = (typename Stage::T)dec_one(t, slut, br);
+          }
+        }
+      } else {
+        while (br.pos < end && o < o_end) stage[o++ - o_begin] = (typename Stage::T)dec_one(t, slut, br);
+      }
     } else {
       while (br.pos < end && o < n) sink.put(o++, dec_one(t, slut, br));
     }
@@ -503,7 +585,8 @@ __global__ void __launch_bounds__(256) k_dec_write(DecTree t, const uint32_t* __
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
   __shared__ typename DecStage<Sink::kBytes>::T stage[DecStage<Sink::kBytes>::kCap];
-  dec_write_block(t, w, nbits, S, nseg, start, exitp, off, n, sink, blockIdx.x, slut, sw, stage);
+  __shared__ uint32_t md[Sink::kBytes == 1 ? (1 << kLutBits) : 1];
+  dec_write_block(t, w, nbits, S, nseg, start, exitp, off, n, sink, blockIdx.x, slut, sw, stage, md);
 }
 
 template <class Sink>
@@ -512,6 +595,7 @@ __global__ void __launch_bounds__(256) k_dec_write_multi(const MDecJobS<Sink>* _
   __shared__ uint32_t slut[1 << kLutBits];
   __shared__ uint32_t sw[kStageWords];
   __shared__ typename DecStage<Sink::kBytes>::T stage[DecStage<Sink::kBytes>::kCap];
+  __shared__ uint32_t md[Sink::kBytes == 1 ? (1 << kLutBits) : 1];
   int lo = 0, hi = nj - 1;  // the job owning this block
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -520,7 +604,7 @@ __global__ void __launch_bounds__(256) k_dec_write_multi(const MDecJobS<Sink>* _
   }
   const MDecJob& J = jobs[lo].j;
   dec_write_block(J.t, J.w, J.nbits, J.S, J.nseg, J.start[cur], J.ex[cur], J.off, J.n, jobs[lo].sink,
-                  blockIdx.x - J.blk0, slut, sw, stage);
+                  blockIdx.x - J.blk0, slut, sw, stage, md);
 }
 
 template <class Sink>
@@ -583,6 +667,9 @@ struct DecJob {
   uint64_t nbytes, n;
   Sink sink;
   const char* tag;
+  // speculative warm-up in bits (0: kDecWarm); codes of a few bits that
+  // resynchronize slowly want a longer one (multi-code steps keep it cheap)
+  uint32_t warm = 0;
 };
 
 // Decodes several independent bitstreams in lockstep on the context's stream:
@@ -731,6 +818,7 @@ void decode_streams_multi(Ctx& ctx, const DecJob<Sink>* jobs, int nj, const char
     m.nbits = j.nbytes * 8;
     m.n = j.n;
     m.S = 256;
+    m.warm = j.warm ? j.warm : (uint32_t)kDecWarm;
     while (div_up(m.nbits, (uint64_t)m.S) > (1u << 21)) m.S *= 2;
     m.nseg = (uint32_t)div_up(m.nbits, (uint64_t)m.S);
     m.nblk = (uint32_t)div_up(m.nseg, 256u);
