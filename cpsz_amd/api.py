@@ -46,6 +46,9 @@ _SIGS = {
     "cpsz_stream_size_2d_log": [_P, _SZ, _SZ, C.POINTER(_SZ)],
     "cpsz_eb_table_2d_log": [_P, C.c_int, C.POINTER(C.c_int)],
     "cpsz_critical_points_2d_noalloc_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_critical_points_2d_sos_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_critical_points_3d_sos_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
+    "cpsz_sos_det": [C.c_int, _P, _P, C.POINTER(C.c_int)],
     "cpsz_critical_points_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ,
                                     C.POINTER(_SZ)],
     "cpsz_set_timing": [C.c_int, C.c_int],
@@ -291,6 +294,43 @@ def critical_points_3d_dev(dU: int, dV: int, dW: int, r1: int, r2: int, r3: int,
         raise CpszError(-3, f"{cnt.value} critical points exceed cap {cap}")
     n = cnt.value
     return dict(zip(ids[:n].tolist(), types[:n].tolist()))
+
+
+def critical_points_2d_sos_dev(dU: int, dV: int, r1: int, r2: int, device: int = 0, stream: int = 0,
+                               cap: int = 1 << 20):
+    """The 2D judge under Simulation of Simplicity: {simplex id: type}."""
+    ids = np.empty(cap, np.int64)
+    types = np.empty(cap, np.int32)
+    cnt = _SZ()
+    _check(lib().cpsz_critical_points_2d_sos_dev(device, stream or None, dU, dV, r1, r2, _ptr(ids), _ptr(types), cap,
+                                                 C.byref(cnt)))
+    if cnt.value > cap:
+        raise CpszError(-3, f"{cnt.value} critical points exceed cap {cap}")
+    n = cnt.value
+    return dict(zip(ids[:n].tolist(), types[:n].tolist()))
+
+
+def critical_points_3d_sos_dev(dU: int, dV: int, dW: int, r1: int, r2: int, r3: int, device: int = 0,
+                               stream: int = 0, cap: int = 1 << 22):
+    """The 3D judge under Simulation of Simplicity: {simplex id: type}."""
+    ids = np.empty(cap, np.int64)
+    types = np.empty(cap, np.int32)
+    cnt = _SZ()
+    _check(lib().cpsz_critical_points_3d_sos_dev(device, stream or None, dU, dV, dW, r1, r2, r3, _ptr(ids),
+                                                 _ptr(types), cap, C.byref(cnt)))
+    if cnt.value > cap:
+        raise CpszError(-3, f"{cnt.value} critical points exceed cap {cap}")
+    n = cnt.value
+    return dict(zip(ids[:n].tolist(), types[:n].tolist()))
+
+
+def sos_det(rows, idx) -> int:
+    """Host evaluation of the device's SoS determinant sign (d = len(idx))."""
+    rows = np.ascontiguousarray(rows, np.float32).ravel()
+    idx = np.ascontiguousarray(idx, np.int64)
+    s = C.c_int()
+    _check(lib().cpsz_sos_det(len(idx), _ptr(rows), _ptr(idx), C.byref(s)))
+    return s.value
 
 
 def set_timing(on: bool = True, device: int = 0):

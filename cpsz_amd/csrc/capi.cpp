@@ -469,6 +469,33 @@ int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const
   });
 }
 
+int cpsz_critical_points_2d_sos_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                    size_t r2, int64_t* ids, int32_t* types, size_t cap, size_t* count) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    critical_points_2d(c, dU, dV, r1, r2, ids, types, cap, count, true);
+  });
+}
+
+int cpsz_critical_points_3d_sos_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                                    size_t r1, size_t r2, size_t r3, int64_t* ids, int32_t* types, size_t cap,
+                                    size_t* count) {
+  return guarded([&] {
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    critical_points_3d(c, dU, dV, dW, r1, r2, r3, ids, types, cap, count, true);
+  });
+}
+
+int cpsz_sos_det(int d, const float* rows, const int64_t* idx, int* sign) {
+  return guarded([&] {
+    CPSZ_REQUIRE(rows && idx && sign && (d == 2 || d == 3), kErrArg, "bad arguments");
+    if (d == 2) *sign = sos_cross(rows, rows + 2, idx[0], idx[1]);
+    else *sign = sos_det3(rows, rows + 3, rows + 6, idx[0], idx[1], idx[2]);
+  });
+}
+
 int cpsz_eb_tables(double* q2d, int cap2d, int* n2d, double* q3d, int cap3d, int* n3d) {
   return guarded([&] {
     const EbTables& t = eb_tables();

@@ -232,6 +232,7 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     if (ov.eb_out) ov.eb_out[vix] = eb;
     if (ov.gain && !(ov.fallback && ov.fallback[vix]) && ov.gain[vix])
       eb = dminf(max_pwr_eb, ldexp(eb, (int)ov.gain[vix]));
+    if (ov.fallback && ov.fallback[vix] == 2) eb = 0;  // stored exactly
     const T u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     if (!isfinite(u) || !isfinite(v) || !(eb == eb)) bad = 1;
     if (qlog) {

@@ -9,7 +9,9 @@
 // TwoProduct, Shewchuk 1997) whenever the bound does not settle the sign.
 // A simplex holds a critical point when every numerator has the sign of
 // their sum (closed simplex) and the sum is non-zero; simplices with a zero
-// vertex vector are skipped as in the reference. Types come from the signs of
+// vertex vector are skipped as in the reference. The SoS form
+// (Simulation of Simplicity, cp_math.hpp) perturbs the vertex vectors
+// symbolically instead, so each zero lies in exactly one simplex. Types come from the signs of
 // the eigenvalues of the simplex Jacobian, read off the characteristic
 // polynomial's coefficients with +, -, *, / only (no sqrt or transcendental
 // functions), so host and device classify identically: 3D uses Cardano's
@@ -42,7 +44,8 @@ struct CpOut {
   }
 };
 
-__global__ void k_cp2d(const float* __restrict__ U, const float* __restrict__ V, int r1, int r2, CpOut out) {
+__global__ void k_cp2d(const float* __restrict__ U, const float* __restrict__ V, int r1, int r2, CpOut out,
+                       int sos) {
   const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long nci = r1 - 3, ncj = r2 - 3;  // i in [1, r1-3], j in [1, r2-3]
   if (nci <= 0 || ncj <= 0 || cell >= nci * ncj) return;
@@ -56,13 +59,14 @@ __global__ void k_cp2d(const float* __restrict__ U, const float* __restrict__ V,
       f[p][0] = U[idx[t][p]];
       f[p][1] = V[idx[t][p]];
     }
-    const int type = tri_cp(f, t);
+    const long long gi[3] = {(long long)idx[t][0], (long long)idx[t][1], (long long)idx[t][2]};
+    const int type = sos ? tri_cp_sos(f, gi, t) : tri_cp(f, t);
     if (type >= 0) out.push(sid + t, type);
   }
 }
 
 __global__ void k_cp3d(const float* __restrict__ U, const float* __restrict__ V,
-                       const float* __restrict__ W, int r1, int r2, int r3, CpOut out) {
+                       const float* __restrict__ W, int r1, int r2, int r3, CpOut out, int sos) {
   const long long cell = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   const long long nk = r3 - 1, nj = r2 - 1, ni = r1 - 1;
   if (ni <= 0 || nj <= 0 || nk <= 0 || cell >= ni * nj * nk) return;
@@ -79,7 +83,8 @@ __global__ void k_cp3d(const float* __restrict__ U, const float* __restrict__ V,
       f[p][1] = V[vi[p]];
       f[p][2] = W[vi[p]];
     }
-    const int type = tet_cp(f, ca, cb);
+    const long long gi[4] = {(long long)vi[0], (long long)vi[1], (long long)vi[2], (long long)vi[3]};
+    const int type = sos ? tet_cp_sos(f, gi, ca, cb) : tet_cp(f, ca, cb);
     if (type >= 0) out.push(6ll * (long long)base + judge_tet_sid(t), type);
   }
 }
@@ -164,7 +169,7 @@ void critical_points_2d_noalloc(int device, hipStream_t st, const float* dU, con
   o.cap = kNoAllocMaxCp;
   k_zero_count<<<1, 1, 0, st>>>(o.count);
   const size_t cells = (r1 > 3 && r2 > 3) ? (r1 - 3) * (r2 - 3) : 0;
-  if (cells) k_cp2d<<<(int)div_up(cells, 256), 256, 0, st>>>(dU, dV, (int)r1, (int)r2, o);
+  if (cells) k_cp2d<<<(int)div_up(cells, 256), 256, 0, st>>>(dU, dV, (int)r1, (int)r2, o, 0);
   CPSZ_HIP(hipGetLastError());
   unsigned long long n = 0;
   CPSZ_HIP(hipMemcpyAsync(&n, o.count, 8, hipMemcpyDeviceToHost, st));
@@ -188,13 +193,13 @@ void critical_points_2d_noalloc(int device, hipStream_t st, const float* dU, con
 }
 
 void critical_points_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
-                        int64_t* ids, int32_t* types, size_t cap, size_t* count) {
+                        int64_t* ids, int32_t* types, size_t cap, size_t* count, bool sos) {
   CPSZ_REQUIRE(dU && dV && ids && types && count, kErrArg, "critical_points_2d: bad arguments");
   const size_t cells = (r1 > 3 && r2 > 3) ? (r1 - 3) * (r2 - 3) : 0;
   unsigned long long* dc;
   CpOut o = make_out(ctx, std::max<size_t>(std::min<size_t>(2 * cells, 1u << 24), 1), dc);
   if (cells) {
-    k_cp2d<<<(int)div_up(cells, 256), 256, 0, ctx.stream()>>>(dU, dV, (int)r1, (int)r2, o);
+    k_cp2d<<<(int)div_up(cells, 256), 256, 0, ctx.stream()>>>(dU, dV, (int)r1, (int)r2, o, sos ? 1 : 0);
     CPSZ_HIP(hipGetLastError());
   }
   collect(ctx, o, dc, ids, types, cap, count);
@@ -202,13 +207,14 @@ void critical_points_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, s
 
 void critical_points_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1,
                         size_t r2, size_t r3, int64_t* ids, int32_t* types, size_t cap,
-                        size_t* count) {
+                        size_t* count, bool sos) {
   CPSZ_REQUIRE(dU && dV && dW && ids && types && count, kErrArg, "critical_points_3d: bad arguments");
   const size_t cells = (r1 > 1 && r2 > 1 && r3 > 1) ? (r1 - 1) * (r2 - 1) * (r3 - 1) : 0;
   unsigned long long* dc;
   CpOut o = make_out(ctx, std::max<size_t>(std::min<size_t>(6 * cells, 1u << 24), 1), dc);
   if (cells) {
-    k_cp3d<<<(int)div_up(cells, 256), 256, 0, ctx.stream()>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, o);
+    k_cp3d<<<(int)div_up(cells, 256), 256, 0, ctx.stream()>>>(dU, dV, dW, (int)r1, (int)r2, (int)r3, o,
+                                                              sos ? 1 : 0);
     CPSZ_HIP(hipGetLastError());
   }
   collect(ctx, o, dc, ids, types, cap, count);

@@ -4,8 +4,10 @@
 
 namespace cpsz {
 
+// sos: decide with Simulation of Simplicity (every zero in exactly one
+// simplex) instead of the reference's closed-simplex test
 void critical_points_2d(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,
-                        int64_t* ids, int32_t* types, size_t cap, size_t* count);
+                        int64_t* ids, int32_t* types, size_t cap, size_t* count, bool sos = false);
 // The 2D judge without any allocation: its scratch is a file-scope device
 // array (per GPU, shared by every call on it; overlapping calls on one GPU
 // race). At most kNoAllocMaxCp critical points; ids/types are caller host
@@ -15,6 +17,6 @@ void critical_points_2d_noalloc(int device, hipStream_t stream, const float* dU,
                                 size_t r2, int64_t* ids, int32_t* types, size_t cap, size_t* count);
 void critical_points_3d(Ctx& ctx, const float* dU, const float* dV, const float* dW, size_t r1,
                         size_t r2, size_t r3, int64_t* ids, int32_t* types, size_t cap,
-                        size_t* count);
+                        size_t* count, bool sos = false);
 
 }  // namespace cpsz

@@ -226,4 +226,119 @@ __host__ __device__ inline int tri_cp(const float f[3][2], int lower) {
   return cp_type_2d(v, lower);
 }
 
+// ------------------------------------------------ Simulation of Simplicity --
+// (Edelsbrunner & Muecke, "Simulation of Simplicity", ACM TOG 9(1), 1990.)
+// Every vertex vector is perturbed symbolically: component c of the vertex of
+// global index g moves by eps^(2^(d*rank + c)), where rank orders the vertices
+// of a determinant by global index. A determinant of perturbed rows is a
+// polynomial in eps whose monomials all have distinct exponents, so its sign
+// is the sign of the first nonzero coefficient in increasing exponent order:
+// the plain determinant, then cofactors, entries and finally +-1, which makes
+// the sign never 0. The zero vector then never lies on a simplex face, and a
+// zero of the piecewise-linear field lies in exactly one simplex of the mesh,
+// decided consistently by the simplices that share a face (they evaluate the
+// same determinant). Zero vertex vectors are perturbed like any other.
+
+// det [a; b] with a of lower global index than b: the sequence det, b_v,
+// -b_u, -a_v, -1 (monomials eps_a0, eps_a1, eps_b0, eps_a1 * eps_b0).
+__host__ __device__ inline int sos_det2(float au, float av, float bu, float bv) {
+  const int s = sign_cross(au, bv, av, bu);
+  if (s) return s;
+  if (bv != 0) return bv > 0 ? 1 : -1;
+  if (bu != 0) return bu > 0 ? -1 : 1;
+  if (av != 0) return av > 0 ? -1 : 1;
+  return -1;
+}
+
+// det [p; q] under the perturbation, for vertices of global indices ip != iq
+__host__ __device__ inline int sos_cross(const float* p, const float* q, long long ip, long long iq) {
+  return ip < iq ? sos_det2(p[0], p[1], q[0], q[1]) : -sos_det2(q[0], q[1], p[0], p[1]);
+}
+
+// det [r0; r1; r2] with rows in increasing global index: the coefficient
+// sequence in increasing exponent order (exponent sum of 2^(3 rank + c) over
+// the perturbed entries): kind 0 the determinant, 1 the cofactor of (r, c),
+// 2 the remaining entry (r, c) times the permutation sign s, 3 the constant s.
+struct SosTerm3 {
+  signed char kind, r, c, s;
+};
+constexpr SosTerm3 kSos3[18] = {{0, 0, 0, 1},  {1, 0, 0, 1},  {1, 0, 1, -1}, {1, 0, 2, 1},  {1, 1, 0, -1},
+                                {2, 2, 2, -1}, {2, 2, 1, 1},  {1, 1, 1, 1},  {2, 2, 2, 1},  {2, 2, 0, -1},
+                                {1, 1, 2, -1}, {2, 2, 1, -1}, {2, 2, 0, 1},  {1, 2, 0, 1},  {2, 1, 2, 1},
+                                {2, 1, 1, -1}, {2, 0, 2, -1}, {3, 0, 0, -1}};
+
+__host__ __device__ inline int sos_det3_sorted(const float* m0, const float* m1, const float* m2) {
+  const float* M[3] = {m0, m1, m2};
+  for (int k = 0; k < 18; k++) {
+    const SosTerm3 t = kSos3[k];
+    int sg = 0;
+    if (t.kind == 0) {
+      const Triple tr[6] = {{m0[0], m1[1], m2[2], 1}, {m0[0], m1[2], m2[1], -1}, {m0[1], m1[0], m2[2], -1},
+                            {m0[1], m1[2], m2[0], 1}, {m0[2], m1[0], m2[1], 1},  {m0[2], m1[1], m2[0], -1}};
+      sg = sign_triples(tr, 6);
+    } else if (t.kind == 1) {
+      const int ra = t.r == 0 ? 1 : 0, rb = t.r == 2 ? 1 : 2;
+      const int ca = t.c == 0 ? 1 : 0, cb = t.c == 2 ? 1 : 2;
+      sg = t.s * sign_cross(M[ra][ca], M[rb][cb], M[ra][cb], M[rb][ca]);
+    } else if (t.kind == 2) {
+      const float x = M[t.r][t.c];
+      sg = x > 0 ? t.s : (x < 0 ? -t.s : 0);
+    } else {
+      sg = t.s;
+    }
+    if (sg) return sg;
+  }
+  return -1;  // not reached: the last term is a constant
+}
+
+// det [a; b; c] under the perturbation, vertices of distinct global indices
+__host__ __device__ inline int sos_det3(const float* a, const float* b, const float* c, long long ia, long long ib,
+                                        long long ic) {
+  // sort by index, tracking the permutation's parity
+  const float* r[3] = {a, b, c};
+  long long g[3] = {ia, ib, ic};
+  int par = 1;
+  for (int i = 0; i < 2; i++)
+    for (int j = 0; j < 2 - i; j++)
+      if (g[j] > g[j + 1]) {
+        const long long tg = g[j];
+        g[j] = g[j + 1];
+        g[j + 1] = tg;
+        const float* tr = r[j];
+        r[j] = r[j + 1];
+        r[j + 1] = tr;
+        par = -par;
+      }
+  return par * sos_det3_sorted(r[0], r[1], r[2]);
+}
+
+// tri_cp under Simulation of Simplicity: the zero of the perturbed field lies
+// inside the triangle (vertex global indices idx) iff its three barycentric
+// numerators have one sign; never on an edge or a vertex.
+__host__ __device__ inline int tri_cp_sos(const float f[3][2], const long long idx[3], int lower) {
+  const int s0 = sos_cross(f[1], f[2], idx[1], idx[2]);
+  const int s1 = sos_cross(f[2], f[0], idx[2], idx[0]);
+  const int s2 = sos_cross(f[0], f[1], idx[0], idx[1]);
+  if (s0 != s1 || s0 != s2) return -1;
+  double v[3][2];
+  for (int p = 0; p < 3; p++) v[p][0] = f[p][0], v[p][1] = f[p][1];
+  return cp_type_2d(v, lower);
+}
+
+// tet_cp under Simulation of Simplicity (vertex global indices idx).
+__host__ __device__ inline int tet_cp_sos(const float f[4][3], const long long idx[4], int ca, int cb) {
+  int sp[4];
+  for (int p = 0; p < 4; p++) {
+    const int a = p == 0 ? 1 : 0, b = p <= 1 ? 2 : 1, c = p <= 2 ? 3 : 2;
+    const int sg = (p & 1) ? 1 : -1;  // D_p = (-1)^(p+1) det(other three)
+    sp[p] = sg * sos_det3(f[a], f[b], f[c], idx[a], idx[b], idx[c]);
+  }
+  if (sp[0] != sp[1] || sp[0] != sp[2] || sp[0] != sp[3]) return -1;
+  const int X[4][3] = {{0, 0, 0}, {ca & 1, (ca >> 1) & 1, ca >> 2}, {cb & 1, (cb >> 1) & 1, cb >> 2}, {1, 1, 1}};
+  double v[4][3];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 3; q++) v[p][q] = f[p][q];
+  return type_3d(v, X);
+}
+
 }  // namespace cpsz

@@ -727,17 +727,76 @@ void train_2d(Ctx& ctx, const std::vector<TrainField>& fields, double max_pwr_eb
 
 namespace {
 void cp_list(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, std::vector<int64_t>& ids,
-             std::vector<int32_t>& types) {
+             std::vector<int32_t>& types, bool sos = false) {
   size_t cap = 1 << 16, count = 0;
   for (;;) {
     ids.resize(cap);
     types.resize(cap);
-    critical_points_2d(ctx, dU, dV, r1, r2, ids.data(), types.data(), cap, &count);
+    critical_points_2d(ctx, dU, dV, r1, r2, ids.data(), types.data(), cap, &count, sos);
     if (count <= cap) break;
     cap = count;
   }
   ids.resize(count);
   types.resize(count);
+}
+
+// simplices whose critical point appeared, vanished or changed type
+std::vector<int64_t> cp_changes(const std::vector<int64_t>& id0, const std::vector<int32_t>& ty0,
+                                const std::vector<int64_t>& id1, const std::vector<int32_t>& ty1) {
+  std::vector<int64_t> bad;
+  size_t a = 0, b = 0;
+  while (a < id0.size() || b < id1.size()) {
+    if (b == id1.size() || (a < id0.size() && id0[a] < id1[b])) bad.push_back(id0[a++]);
+    else if (a == id0.size() || id1[b] < id0[a]) bad.push_back(id1[b++]);
+    else {
+      if (ty0[a] != ty1[b]) bad.push_back(id0[a]);
+      a++;
+      b++;
+    }
+  }
+  return bad;
+}
+
+// Per-vertex fallback of the learned bound: 0 learned, 1 the analytic bound
+// (which keeps every critical point of the reference's closed-simplex test),
+// 2 stored exactly (eb 0). A triangle the closed test sees change sends its
+// vertices to the analytic bound; one only the Simulation-of-Simplicity test
+// sees change (a degenerate zero the analytic bound need not keep) sends
+// them one step further.
+void mark_triangles(const std::vector<int64_t>& bad, size_t r2, uint8_t* hf, bool sos) {
+  for (int64_t sid : bad) {
+    const int64_t cell = sid / 2;
+    const int t = (int)(sid % 2);
+    const size_t ci = (size_t)(cell / (int64_t)(r2 - 1)), cj = (size_t)(cell % (int64_t)(r2 - 1));
+    const size_t v[3] = {ci * r2 + cj, t == 0 ? (ci + 1) * r2 + cj : ci * r2 + cj + 1, (ci + 1) * r2 + cj + 1};
+    for (size_t x : v) hf[x] = sos ? (hf[x] ? 2 : 1) : (hf[x] ? hf[x] : 1);
+  }
+}
+
+// The learned verification rounds: kLearnRounds with proposals, then the
+// analytic bound for every vertex not yet fallen back, then up to two more
+// rounds escalating to exact storage, then everything exact.
+constexpr int kLearnRounds = 16;
+constexpr int kLearnLast = kLearnRounds + 3;
+
+// Falls back as the round requires; returns false when the field is done.
+bool verify_round_2d(int it, const std::vector<int64_t>& id0, const std::vector<int32_t>& ty0,
+                     const std::vector<int64_t>& is0, const std::vector<int32_t>& ts0,
+                     const std::vector<int64_t>& id1, const std::vector<int32_t>& ty1,
+                     const std::vector<int64_t>& is1, const std::vector<int32_t>& ts1, size_t r2, uint8_t* hf,
+                     size_t n) {
+  const std::vector<int64_t> bad = cp_changes(id0, ty0, id1, ty1);
+  const std::vector<int64_t> bad_s = cp_changes(is0, ts0, is1, ts1);
+  if (bad.empty() && bad_s.empty()) return false;
+  CPSZ_REQUIRE(bad.empty() || it < kLearnRounds, kErrInternal,
+               "learned compression: the plain bound changed a critical point");
+  CPSZ_REQUIRE(it < kLearnLast, kErrInternal, "learned compression did not converge");
+  mark_triangles(bad, r2, hf, false);
+  mark_triangles(bad_s, r2, hf, true);
+  if (it + 1 == kLearnRounds)
+    for (size_t v = 0; v < n; v++) hf[v] = hf[v] ? hf[v] : 1;
+  if (it + 1 == kLearnLast) std::fill(hf, hf + n, 2);
+  return true;
 }
 }  // namespace
 
@@ -755,9 +814,10 @@ void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, 
   mlp_infer_gains(ctx, 2, dU, dV, nullptr, E, r1, r2, 1, max_pwr_eb, P, 0, n, gain);
   std::vector<uint8_t> hg(n), hf(n, 0);
   CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, n, hipMemcpyDeviceToHost, st));
-  std::vector<int64_t> id0, id1;
-  std::vector<int32_t> ty0, ty1;
+  std::vector<int64_t> id0, id1, is0, is1;
+  std::vector<int32_t> ty0, ty1, ts0, ts1;
   cp_list(ctx, dU, dV, r1, r2, id0, ty0);  // synchronizes
+  cp_list(ctx, dU, dV, r1, r2, is0, ts0, true);
   for (size_t v = 0; v < n; v++) stats.boosted_vertices += hg[v] != 0;
   uint8_t* fb = ctx.buf<uint8_t>("ln.fb", n);
   float* dU2 = ctx.buf<float>("ln.u2", n);
@@ -765,37 +825,14 @@ void compress_2d_learned(Ctx& ctx, const float* dU, const float* dV, size_t r1, 
   EbOverride ov;
   ov.gain = gain;
   ov.fallback = fb;
-  const int kMaxIter = 16;
-  for (int it = 0; it <= kMaxIter; it++) {
-    if (it == kMaxIter) std::fill(hf.begin(), hf.end(), 1);  // the plain method
+  for (int it = 0;; it++) {
     CPSZ_HIP(hipMemcpyAsync(fb, hf.data(), n, hipMemcpyHostToDevice, st));
     compress_2d_device(ctx, dU, dV, r1, r2, max_pwr_eb, out, ov);
     decompress_2d_device(ctx, out.data, nullptr, out.size, r1, r2, dU2, dV2);
     cp_list(ctx, dU2, dV2, r1, r2, id1, ty1);
+    cp_list(ctx, dU2, dV2, r1, r2, is1, ts1, true);
     stats.iterations = it + 1;
-    // simplices whose critical point appeared, vanished or changed type
-    std::vector<int64_t> bad;
-    size_t a = 0, b = 0;
-    while (a < id0.size() || b < id1.size()) {
-      if (b == id1.size() || (a < id0.size() && id0[a] < id1[b])) bad.push_back(id0[a++]);
-      else if (a == id0.size() || id1[b] < id0[a]) bad.push_back(id1[b++]);
-      else {
-        if (ty0[a] != ty1[b]) bad.push_back(id0[a]);
-        a++;
-        b++;
-      }
-    }
-    if (bad.empty()) break;
-    CPSZ_REQUIRE(it < kMaxIter, kErrInternal, "learned compression: the plain bound changed a critical point");
-    for (int64_t sid : bad) {
-      const int64_t cell = sid / 2;
-      const int t = (int)(sid % 2);
-      const size_t ci = (size_t)(cell / (int64_t)(r2 - 1)), cj = (size_t)(cell % (int64_t)(r2 - 1));
-      const size_t v0 = ci * r2 + cj;
-      const size_t v1 = t == 0 ? (ci + 1) * r2 + cj : ci * r2 + cj + 1;
-      const size_t v2 = (ci + 1) * r2 + cj + 1;
-      hf[v0] = hf[v1] = hf[v2] = 1;
-    }
+    if (!verify_round_2d(it, id0, ty0, is0, ts0, id1, ty1, is1, ts1, r2, hf.data(), n)) break;
   }
   for (size_t v = 0; v < n; v++) stats.fallback_vertices += hf[v] && hg[v];
 }
@@ -1188,10 +1225,13 @@ void compress_2d_learned_batch(Ctx& ctx, const std::vector<TrainField>& fields, 
   CPSZ_HIP(hipMemcpyAsync(hg.data(), gain, ntot, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
   for (size_t v = 0; v < ntot; v++) stats.boosted_vertices += hg[v] != 0;
-  // ---- critical points of the originals
-  std::vector<std::vector<int64_t>> id0(nf);
-  std::vector<std::vector<int32_t>> ty0(nf);
-  for (size_t k = 0; k < nf; k++) cp_list(ctx, fields[k].dU, fields[k].dV, fields[k].r1, fields[k].r2, id0[k], ty0[k]);
+  // ---- critical points of the originals (closed-simplex and SoS tests)
+  std::vector<std::vector<int64_t>> id0(nf), is0(nf);
+  std::vector<std::vector<int32_t>> ty0(nf), ts0(nf);
+  for (size_t k = 0; k < nf; k++) {
+    cp_list(ctx, fields[k].dU, fields[k].dV, fields[k].r1, fields[k].r2, id0[k], ty0[k]);
+    cp_list(ctx, fields[k].dU, fields[k].dV, fields[k].r1, fields[k].r2, is0[k], ts0[k], true);
+  }
   // ---- verify and fall back until every field keeps its critical points
   float* dec = ctx.buf<float>("l2b.dec", 2 * ntot);
   std::vector<size_t> soff(nf + 1, 0);
@@ -1204,10 +1244,7 @@ void compress_2d_learned_batch(Ctx& ctx, const std::vector<TrainField>& fields, 
   Batch2dConfig cfg;
   cfg.workers = workers;
   cfg.grid_cap = grid_cap;
-  const int kMaxIter = 16;
-  for (int it = 0; it <= kMaxIter && !pending.empty(); it++) {
-    if (it == kMaxIter)  // the plain method for whatever is left
-      for (size_t k : pending) std::fill(hf.begin() + nofs[k], hf.begin() + nofs[k + 1], 1);
+  for (int it = 0; !pending.empty(); it++) {
     std::vector<Field2d> sub;
     std::vector<EbOverride> ov;
     for (size_t k : pending) {
@@ -1238,25 +1275,16 @@ void compress_2d_learned_batch(Ctx& ctx, const std::vector<TrainField>& fields, 
     for (size_t q = 0; q < pending.size(); q++) {
       const size_t k = pending[q];
       const TrainField& F = fields[k];
-      std::vector<int64_t> id1;
-      std::vector<int32_t> ty1;
+      std::vector<int64_t> id1, is1;
+      std::vector<int32_t> ty1, ts1;
       cp_list(ctx, outs[q].dU, outs[q].dV, F.r1, F.r2, id1, ty1);
-      const std::vector<int64_t> bad = cp_diff(id0[k], ty0[k], id1, ty1);
-      if (bad.empty()) {
+      cp_list(ctx, outs[q].dU, outs[q].dV, F.r1, F.r2, is1, ts1, true);
+      if (!verify_round_2d(it, id0[k], ty0[k], is0[k], ts0[k], id1, ty1, is1, ts1, F.r2, hf.data() + nofs[k],
+                           nofs[k + 1] - nofs[k])) {
         CPSZ_REQUIRE(bo.size[q] <= soff[k + 1] - soff[k], kErrInternal, "learned stream bound exceeded");
         CPSZ_HIP(hipMemcpyAsync(out + soff[k], bo.base + bo.off[q], bo.size[q], hipMemcpyDeviceToDevice, st));
         size[k] = bo.size[q];
         continue;
-      }
-      CPSZ_REQUIRE(it < kMaxIter, kErrInternal, "learned compression: the plain bound changed a critical point");
-      for (int64_t sid : bad) {
-        const int64_t cell = sid / 2;
-        const int t = (int)(sid % 2);
-        const size_t ci = (size_t)(cell / (int64_t)(F.r2 - 1)), cj = (size_t)(cell % (int64_t)(F.r2 - 1));
-        const size_t v0 = ci * F.r2 + cj;
-        const size_t v1 = t == 0 ? (ci + 1) * F.r2 + cj : ci * F.r2 + cj + 1;
-        const size_t v2 = (ci + 1) * F.r2 + cj + 1;
-        hf[nofs[k] + v0] = hf[nofs[k] + v1] = hf[nofs[k] + v2] = 1;
       }
       still.push_back(k);
     }

@@ -165,6 +165,24 @@ int cpsz_critical_points_3d_dev(int device, void* stream, const float* dU, const
                                 const float* dW, size_t r1, size_t r2, size_t r3, int64_t* ids,
                                 int32_t* types, size_t cap, size_t* count);
 
+/* The two judges above under Simulation of Simplicity (Edelsbrunner & Muecke
+ * 1990): vertex vectors are perturbed symbolically by their global index, so
+ * a zero of the field lies in exactly one simplex, never on a face, and zero
+ * vertex vectors take part like any other. Same simplices, ids and type codes
+ * as the closed-simplex judges, which follow the reference's extractor
+ * (test/cp_extraction_2d.cpp:62-147, test/cp_extraction_3d.cpp:79-209); the
+ * perturbation has no counterpart in the reference (the north-star's exact
+ * sign/SoS cell test). */
+int cpsz_critical_points_2d_sos_dev(int device, void* stream, const float* dU, const float* dV, size_t r1,
+                                    size_t r2, int64_t* ids, int32_t* types, size_t cap, size_t* count);
+int cpsz_critical_points_3d_sos_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
+                                    size_t r1, size_t r2, size_t r3, int64_t* ids, int32_t* types, size_t cap,
+                                    size_t* count);
+/* Sign (+-1, never 0) of the d x d determinant (d = 2 or 3) of the rows
+ * rows[0..d*d) of vertices with distinct global indices idx[0..d) under the
+ * same perturbation (host evaluation of the device predicate, for tests). */
+int cpsz_sos_det(int d, const float* rows, const int64_t* idx, int* sign);
+
 /* ---- host-side diagnostics (no GPU needed) -------------------------------- */
 /* Copies the eb-id threshold tables the device kernels use: q2d[k] is the
  * smallest |U|*eb with 2D id >= k, q3d[k] the smallest vertex bound with 3D

@@ -149,6 +149,14 @@ std::vector<CP> cp_extract_2d(const float* U, const float* V, size_t r1, size_t 
 int tri_cp_2d(const float f[3][2], int lower);
 // -1 or the type of tetrahedron t (0..5, judge order) with vertex vectors f.
 int tet_cp_3d(const float f[4][3], int t);
+// Simulation of Simplicity forms (no reference counterpart): rows[0..d*d) of
+// vertices with distinct global indices idx; +-1, never 0.
+int sos_sign_det(int d, const float* rows, const int64_t* idx);
+int tri_cp_2d_sos(const float f[3][2], const int64_t idx[3], int lower);
+int tet_cp_3d_sos(const float f[4][3], const int64_t idx[4], int t);
+std::vector<CP> cp_extract_2d_sos(const float* U, const float* V, size_t r1, size_t r2);
+std::vector<CP> cp_extract_3d_sos(const float* U, const float* V, const float* W, size_t r1, size_t r2,
+                                  size_t r3);
 
 // Learned-bound proposer checkers (no reference counterpart; see orc_learn.cpp).
 void learn_features_2d(const float* U, const float* V, const double* E, size_t r1, size_t r2,

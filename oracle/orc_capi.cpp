@@ -216,4 +216,18 @@ long orc_cp_3d(const float* U, const float* V, const float* W, size_t r1, size_t
   return (long)cps.size();
 }
 
+int orc_sos_det(int d, const float* rows, const int64_t* idx) { return sos_sign_det(d, rows, idx); }
+
+long orc_cp_2d_sos(const float* U, const float* V, size_t r1, size_t r2, int64_t* simplex, int* type, size_t cap) {
+  auto cps = cp_extract_2d_sos(U, V, r1, r2);
+  for (size_t i = 0; i < cps.size() && i < cap; i++) simplex[i] = cps[i].simplex, type[i] = cps[i].type;
+  return (long)cps.size();
+}
+long orc_cp_3d_sos(const float* U, const float* V, const float* W, size_t r1, size_t r2, size_t r3,
+                   int64_t* simplex, int* type, size_t cap) {
+  auto cps = cp_extract_3d_sos(U, V, W, r1, r2, r3);
+  for (size_t i = 0; i < cps.size() && i < cap; i++) simplex[i] = cps[i].simplex, type[i] = cps[i].type;
+  return (long)cps.size();
+}
+
 }  // extern "C"

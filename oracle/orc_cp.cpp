@@ -8,9 +8,11 @@
 // inverse interpolation (absent here): this restatement decides it with exact
 // signs of the barycentric numerators (error-free products and expansion
 // sums), closed simplex, so it is parity unpinned against FTK by construction.
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <limits>
+#include <vector>
 
 #include "orc.hpp"
 
@@ -217,6 +219,167 @@ bool zero_in_tet(const float v[4][3]) {
   return true;
 }
 
+// ---- Simulation of Simplicity (Edelsbrunner & Muecke 1990): checker
+// restatement, written generically. Entry (r, c) of a d x d determinant, r
+// the row's rank by global vertex index, is perturbed by eps^(2^(d r + c));
+// the perturbed determinant is expanded into monomials (partial matchings of
+// rows to columns), ordered by exponent, and the sign is the first nonzero
+// coefficient's. A coefficient is the signed sum over the permutations that
+// extend the matching of the products of the unperturbed remaining entries,
+// evaluated exactly.
+int perm_parity(const int* p, int d) {
+  int s = 1;
+  for (int i = 0; i < d; i++)
+    for (int j = i + 1; j < d; j++)
+      if (p[i] > p[j]) s = -s;
+  return s;
+}
+
+int sos_sorted(int d, const float* const* M) {
+  struct Mono {
+    uint32_t e;
+    int col[3];  // column of each row, -1 = unperturbed
+  };
+  std::vector<Mono> ms;
+  // every assignment row -> (unperturbed or a column), columns distinct
+  const int choices = d + 1;
+  int total = 1;
+  for (int r = 0; r < d; r++) total *= choices;
+  for (int code = 0; code < total; code++) {
+    Mono m{0, {-1, -1, -1}};
+    int x = code, used = 0;
+    bool ok = true;
+    for (int r = 0; r < d; r++) {
+      const int c = x % choices - 1;
+      x /= choices;
+      m.col[r] = c;
+      if (c >= 0) {
+        if (used >> c & 1) ok = false;
+        used |= 1 << c;
+        m.e += 1u << (d * r + c);
+      }
+    }
+    if (ok) ms.push_back(m);
+  }
+  std::sort(ms.begin(), ms.end(), [](const Mono& a, const Mono& b) { return a.e < b.e; });
+  for (const Mono& m : ms) {
+    Expansion ex;
+    int perm[3] = {0, 1, 2};
+    // every permutation of the columns extending the matching
+    do {
+      bool ext = true;
+      for (int r = 0; r < d; r++)
+        if (m.col[r] >= 0 && perm[r] != m.col[r]) ext = false;
+      if (!ext) continue;
+      const double sg = perm_parity(perm, d);
+      float f[3];
+      int nf = 0;
+      for (int r = 0; r < d; r++)
+        if (m.col[r] < 0) f[nf++] = M[r][perm[r]];
+      if (nf == 0) ex.add(sg);
+      else if (nf == 1) ex.add(sg * (double)f[0]);
+      else if (nf == 2) ex.add(sg * ((double)f[0] * f[1]));  // exact
+      else add_triple(ex, sg, f[0], f[1], f[2]);
+    } while (std::next_permutation(perm, perm + d));
+    const int s = ex.sign();
+    if (s) return s;
+  }
+  return 0;  // not reached: the full matchings have coefficient +-1
+}
+
+}  // namespace
+
+int sos_sign_det(int d, const float* rows, const int64_t* idx) {
+  int ord[3] = {0, 1, 2};
+  std::sort(ord, ord + d, [&](int a, int b) { return idx[a] < idx[b]; });
+  const float* M[3];
+  for (int r = 0; r < d; r++) M[r] = rows + d * ord[r];
+  return perm_parity(ord, d) * sos_sorted(d, M);
+}
+
+int tri_cp_2d_sos(const float f[3][2], const int64_t idx[3], int lower) {
+  const double X1[3][2] = {{0, 0}, {0, 1}, {1, 1}};
+  const double X2[3][2] = {{0, 0}, {1, 0}, {1, 1}};
+  // barycentric numerators det[f1; f2], det[f2; f0], det[f0; f1]
+  int s[3];
+  for (int p = 0; p < 3; p++) {
+    const int a = (p + 1) % 3, b = (p + 2) % 3;
+    const float rows[4] = {f[a][0], f[a][1], f[b][0], f[b][1]};
+    const int64_t ix[2] = {idx[a], idx[b]};
+    s[p] = sos_sign_det(2, rows, ix);
+  }
+  if (s[0] != s[1] || s[0] != s[2]) return -1;
+  double v[3][2];
+  for (int p = 0; p < 3; p++) v[p][0] = f[p][0], v[p][1] = f[p][1];
+  return type_2d(v, lower ? X2 : X1);
+}
+
+int tet_cp_3d_sos(const float f[4][3], const int64_t idx[4], int t) {
+  static const int A[6][3] = {{1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 0, 1}};
+  static const int B[6][3] = {{1, 1, 0}, {1, 1, 0}, {1, 0, 1}, {1, 0, 1}, {0, 1, 1}, {0, 1, 1}};
+  // D_p = (-1)^(p+1) det of the other three vectors in vertex order
+  int s[4];
+  for (int p = 0; p < 4; p++) {
+    float rows[9];
+    int64_t ix[3];
+    int k = 0;
+    for (int q = 0; q < 4; q++) {
+      if (q == p) continue;
+      for (int c = 0; c < 3; c++) rows[3 * k + c] = f[q][c];
+      ix[k++] = idx[q];
+    }
+    s[p] = ((p & 1) ? 1 : -1) * sos_sign_det(3, rows, ix);
+  }
+  if (s[0] != s[1] || s[0] != s[2] || s[0] != s[3]) return -1;
+  const int X[4][3] = {{0, 0, 0}, {A[t][2], A[t][1], A[t][0]}, {B[t][2], B[t][1], B[t][0]}, {1, 1, 1}};
+  double v[4][3];
+  for (int p = 0; p < 4; p++)
+    for (int q = 0; q < 3; q++) v[p][q] = f[p][q];
+  return type_3d(v, X);
+}
+
+std::vector<CP> cp_extract_2d_sos(const float* U, const float* V, size_t r1, size_t r2) {
+  std::vector<CP> out;
+  for (size_t i = 1; i + 2 < r1; i++)
+    for (size_t j = 1; j + 2 < r2; j++) {
+      const int64_t cell = 2 * (int64_t)(i * (r2 - 1) + j);
+      const int64_t idx[2][3] = {{(int64_t)(i * r2 + j), (int64_t)((i + 1) * r2 + j), (int64_t)((i + 1) * r2 + j + 1)},
+                                 {(int64_t)(i * r2 + j), (int64_t)(i * r2 + j + 1), (int64_t)((i + 1) * r2 + j + 1)}};
+      for (int t = 0; t < 2; t++) {
+        float f[3][2];
+        for (int p = 0; p < 3; p++) f[p][0] = U[idx[t][p]], f[p][1] = V[idx[t][p]];
+        const int ty = tri_cp_2d_sos(f, idx[t], t);
+        if (ty >= 0) out.push_back(CP{cell + t, ty});
+      }
+    }
+  return out;
+}
+
+std::vector<CP> cp_extract_3d_sos(const float* U, const float* V, const float* W, size_t r1, size_t r2,
+                                  size_t r3) {
+  std::vector<CP> out;
+  const size_t d0 = r2 * r3, d1 = r3;
+  static const int A[6][3] = {{1, 0, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {0, 1, 0}, {0, 0, 1}};
+  static const int B[6][3] = {{1, 1, 0}, {1, 1, 0}, {1, 0, 1}, {1, 0, 1}, {0, 1, 1}, {0, 1, 1}};
+  static const int SID[6] = {0, 2, 1, 4, 3, 5};
+  for (size_t i = 0; i + 1 < r1; i++)
+    for (size_t j = 0; j + 1 < r2; j++)
+      for (size_t k = 0; k + 1 < r3; k++) {
+        const size_t base = i * d0 + j * d1 + k;
+        for (int t = 0; t < 6; t++) {
+          const int64_t vi[4] = {(int64_t)base, (int64_t)((i + A[t][0]) * d0 + (j + A[t][1]) * d1 + k + A[t][2]),
+                                 (int64_t)((i + B[t][0]) * d0 + (j + B[t][1]) * d1 + k + B[t][2]),
+                                 (int64_t)((i + 1) * d0 + (j + 1) * d1 + k + 1)};
+          float f[4][3];
+          for (int p = 0; p < 4; p++) f[p][0] = U[vi[p]], f[p][1] = V[vi[p]], f[p][2] = W[vi[p]];
+          const int ty = tet_cp_3d_sos(f, vi, t);
+          if (ty >= 0) out.push_back(CP{6 * (int64_t)base + SID[t], ty});
+        }
+      }
+  return out;
+}
+
+namespace {
 }  // namespace
 
 int tri_cp_2d(const float f[3][2], int lower) {

@@ -58,6 +58,12 @@ def load_oracle(path=ORC_PATH):
     lib.orc_cp_2d.restype = C.c_long
     lib.orc_cp_3d.argtypes = [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ]
     lib.orc_cp_3d.restype = C.c_long
+    lib.orc_cp_2d_sos.argtypes = [_P, _P, _SZ, _SZ, _P, _P, _SZ]
+    lib.orc_cp_2d_sos.restype = C.c_long
+    lib.orc_cp_3d_sos.argtypes = [_P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ]
+    lib.orc_cp_3d_sos.restype = C.c_long
+    lib.orc_sos_det.argtypes = [C.c_int, _P, _P]
+    lib.orc_sos_det.restype = C.c_int
     lib.orc_free.argtypes = [_P]
     lib.orc_acbrt1_one.argtypes = [C.c_float]
     lib.orc_acbrt1_one.restype = C.c_float
@@ -260,6 +266,27 @@ class Oracle:
         typ = np.empty(cap, np.int32)
         n = self.lib.orc_cp_3d(_ptr(U), _ptr(V), _ptr(W), r1, r2, r3, _ptr(sid), _ptr(typ), cap)
         return dict(zip(sid[:n].tolist(), typ[:n].tolist()))
+
+    def cp_2d_sos(self, U, V):
+        r1, r2 = U.shape
+        cap = 2 * r1 * r2
+        sid = np.empty(cap, np.int64)
+        typ = np.empty(cap, np.int32)
+        n = self.lib.orc_cp_2d_sos(_ptr(U), _ptr(V), r1, r2, _ptr(sid), _ptr(typ), cap)
+        return dict(zip(sid[:n].tolist(), typ[:n].tolist()))
+
+    def cp_3d_sos(self, U, V, W):
+        r1, r2, r3 = U.shape
+        cap = 6 * r1 * r2 * r3
+        sid = np.empty(cap, np.int64)
+        typ = np.empty(cap, np.int32)
+        n = self.lib.orc_cp_3d_sos(_ptr(U), _ptr(V), _ptr(W), r1, r2, r3, _ptr(sid), _ptr(typ), cap)
+        return dict(zip(sid[:n].tolist(), typ[:n].tolist()))
+
+    def sos_det(self, rows, idx):
+        rows = np.ascontiguousarray(rows, np.float32).ravel()
+        idx = np.ascontiguousarray(idx, np.int64)
+        return self.lib.orc_sos_det(len(idx), _ptr(rows), _ptr(idx))
 
 
 def have_reference():

@@ -94,3 +94,57 @@ def test_zero_cp_error_3d():
     a, b = _cp3(U, V, W), _cp3(*d)
     print(f"[cp3d] {len(a)} critical points", flush=True)
     assert len(a) > 0 and a == b
+
+
+# ---- Simulation of Simplicity judges ----------------------------------------
+def _sos2(U, V):
+    dU, dV = _dev(U, V)
+    out = cz.critical_points_2d_sos_dev(dU.data_ptr(), dV.data_ptr(), *U.shape)
+    torch.cuda.synchronize()
+    return out
+
+
+def _sos3(U, V, W):
+    dU, dV, dW = _dev(U, V, W)
+    out = cz.critical_points_3d_sos_dev(dU.data_ptr(), dV.data_ptr(), dW.data_ptr(), *U.shape)
+    torch.cuda.synchronize()
+    return out
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_sos2d_vs_oracle(orc, seed):
+    """Generic, small-integer (ties, zero vectors, parallel rows) and linear
+    fields with exact zeros on vertices and edges."""
+    print(f"[sos2d] seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    U, V = field_2d(150, 170, seed=seed)
+    assert _sos2(U, V) == orc.cp_2d_sos(U, V)
+    U = rng.integers(-2, 3, (90, 110)).astype(np.float32)
+    V = rng.integers(-2, 3, (90, 110)).astype(np.float32)
+    got = _sos2(U, V)
+    assert got == orc.cp_2d_sos(U, V) and len(got) > 0
+    y, x = np.mgrid[0:40, 0:50].astype(np.float64)
+    for x0, y0 in ((20.0, 17.0), (20.5, 17.0), (20.0, 17.5), (20.5, 17.5)):
+        U = (x - x0 + 2 * (y - y0)).astype(np.float32)
+        V = (3 * (x - x0) - (y - y0)).astype(np.float32)
+        got = _sos2(U, V)
+        assert got == orc.cp_2d_sos(U, V) and len(got) == 1
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_sos3d_vs_oracle(orc, seed):
+    print(f"[sos3d] seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    U, V, W = field_3d(24, 28, 30, seed=seed)
+    assert _sos3(U, V, W) == orc.cp_3d_sos(U, V, W)
+    U, V, W = (rng.integers(-1, 2, (14, 15, 16)).astype(np.float32) for _ in range(3))
+    got = _sos3(U, V, W)
+    assert got == orc.cp_3d_sos(U, V, W) and len(got) > 0
+    z, y, x = np.mgrid[0:12, 0:12, 0:12].astype(np.float64)
+    for x0, y0, z0 in ((5.0, 6.0, 4.0), (5.5, 6.0, 4.0), (5.5, 6.5, 4.5)):
+        d = (x - x0, y - y0, z - z0)
+        U = (d[0] + d[1]).astype(np.float32)
+        V = (d[1] + d[2]).astype(np.float32)
+        W = (d[0] + 2 * d[2]).astype(np.float32)
+        got = _sos3(U, V, W)
+        assert got == orc.cp_3d_sos(U, V, W) and len(got) == 1

@@ -139,8 +139,10 @@ def test_learned_ratio_gain_at_zero_cp_errors(orc, trained):
     oU, oV = orc.decompress_2d(stream, 512, 512)
     assert np.array_equal(dU2.view(np.uint32), oU.view(np.uint32))
     assert np.array_equal(dV2.view(np.uint32), oV.view(np.uint32))
-    # zero critical-point errors, by the oracle's judge as well
+    # zero critical-point errors, by the oracle's judge as well, under the
+    # closed-simplex test and under Simulation of Simplicity
     assert orc.cp_2d(U, V) == orc.cp_2d(dU2, dV2)
+    assert orc.cp_2d_sos(U, V) == orc.cp_2d_sos(dU2, dV2)
     # every vertex within its own (learned or analytic) bound: |x' - x| <= |x| * 0.1
     for a, b in ((U, dU2), (V, dV2)):
         assert np.all(np.abs(b.astype(np.float64) - a) <= np.abs(a.astype(np.float64)) * 0.1 + 1e-30)
