@@ -34,6 +34,8 @@ _SIGS = {
     "cpsz_decompress_3d_f32_dev": [C.c_int, _P, _P, _SZ, _SZ, _SZ, _SZ, _P, _P, _P],
     "cpsz_lossless_compress": [C.c_int, _P, _SZ, C.POINTER(_P), C.POINTER(_SZ)],
     "cpsz_lossless_decompress": [_P, _SZ, _P, _SZ],
+    "cpsz_lossless_compress_dev": [C.c_int, _P, C.c_int, _P, _SZ, C.POINTER(_P), C.POINTER(_SZ)],
+    "cpsz_lossless_compress_batch_dev": [C.c_int, _P, C.c_int, _P, _P, _P, _SZ, _P, _P],
     "cpsz_critical_points_2d_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
     "cpsz_compress_2d_f64": [_P, _P, _SZ, _SZ, C.c_double, C.POINTER(_P), C.POINTER(_SZ)],
     "cpsz_decompress_2d_f64": [_P, _SZ, _SZ, _SZ, _P, _P],
@@ -226,6 +228,25 @@ def sz_lossless_compress(data, level: int = 3) -> np.ndarray:
     p, s = _P(), _SZ()
     _check(lib().cpsz_lossless_compress(level, _ptr(data), data.size, C.byref(p), C.byref(s)))
     return _take(p, s.value)
+
+
+def sz_lossless_compress_dev(d_in: int, n: int, level: int = 3, device: int = 0, stream: int = 0) -> np.ndarray:
+    """sz_lossless_compress of n bytes at device address d_in (pipelined D2H)."""
+    p, s = _P(), _SZ()
+    _check(lib().cpsz_lossless_compress_dev(device, stream or None, level, d_in, n, C.byref(p), C.byref(s)))
+    return _take(p, s.value)
+
+
+def sz_lossless_compress_batch_dev(base: int, off, size, level: int = 3, device: int = 0, stream: int = 0):
+    """One ZSTD frame per device stream base + off[k] (size[k] bytes)."""
+    off = np.ascontiguousarray(off, np.uint64)
+    size = np.ascontiguousarray(size, np.uint64)
+    nf = size.size
+    outs = (_P * nf)()
+    sizes = np.zeros(nf, np.uint64)
+    _check(lib().cpsz_lossless_compress_batch_dev(device, stream or None, level, base, _ptr(off), _ptr(size), nf,
+                                                  C.cast(outs, _P), _ptr(sizes)))
+    return [_take(outs[k], int(sizes[k])) for k in range(nf)]
 
 
 def sz_lossless_decompress(data, size: int) -> np.ndarray:

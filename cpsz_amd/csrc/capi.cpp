@@ -1,11 +1,17 @@
 // extern "C" boundary of libcpsz_hip.so (declared in include/cpsz.h).
+#define ZSTD_STATIC_LINKING_ONLY  // the block-level API (ZSTD_compressBegin_advanced, ZSTD_compressContinue)
 #include <zstd.h>
 
+#include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/cpsz.h"
 #include "batch2d.hpp"
@@ -402,6 +408,146 @@ int cpsz_lossless_compress(int level, const unsigned char* in, size_t n, unsigne
     }
     *out = o;
     *out_size = r;
+  });
+}
+
+// ZSTD of a device-resident stream: the bytes come to one pinned host buffer
+// in 8 MiB chunks on the caller's stream while the host compresses the chunks
+// that have arrived, with zstd's block-level API (ZSTD_compressBegin_advanced
+// with the parameters ZSTD_compress derives from the level and the size, then
+// ZSTD_compressContinue per chunk and ZSTD_compressEnd). The chunks are
+// contiguous and multiples of the 128 KiB block, so the compressor sees the
+// same window and the same block boundaries as the one-shot ZSTD_compress of
+// src/sz_lossless.cpp:11-29: the frame is byte-identical to the reference's.
+int cpsz_lossless_compress_dev(int device, void* stream, int level, const unsigned char* d_in, size_t n,
+                               unsigned char** out, size_t* out_size) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_in && out && out_size, kErrArg, "null argument");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    hipStream_t st = c.stream();
+    const size_t cap = ZSTD_compressBound(n);
+    unsigned char* o = (unsigned char*)std::malloc(cap ? cap : 1);
+    CPSZ_REQUIRE(o, kErrAlloc, "host allocation failed");
+    std::unique_ptr<unsigned char, void (*)(void*)> keep(o, &std::free);
+    constexpr size_t kChunk = 8u << 20;  // a multiple of ZSTD_BLOCKSIZE_MAX
+    static_assert(kChunk % ZSTD_BLOCKSIZE_MAX == 0, "chunks must hold whole blocks");
+    uint8_t* h = (uint8_t*)c.host_buf("zs.all", n + 1);
+    const size_t nch = n ? div_up(n, kChunk) : 1;
+    std::vector<hipEvent_t> ev(nch, nullptr);
+    ZSTD_CCtx* z = ZSTD_createCCtx();
+    auto cleanup = [&] {
+      ZSTD_freeCCtx(z);
+      for (auto e : ev)
+        if (e) hipEventDestroy(e);
+    };
+    try {
+      CPSZ_REQUIRE(z, kErrAlloc, "ZSTD_createCCtx failed");
+      for (size_t k = 0; k < nch; k++) {
+        const size_t at = k * kChunk, len = std::min(kChunk, n - at);
+        CPSZ_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        if (len) CPSZ_HIP(hipMemcpyAsync(h + at, d_in + at, len, hipMemcpyDeviceToHost, st));
+        CPSZ_HIP(hipEventRecord(ev[k], st));
+      }
+      const ZSTD_parameters prm = ZSTD_getParams(level, n, 0);
+      size_t r = ZSTD_compressBegin_advanced(z, nullptr, 0, prm, n);
+      CPSZ_REQUIRE(!ZSTD_isError(r), kErrInternal, std::string("ZSTD: ") + ZSTD_getErrorName(r));
+      size_t produced = 0;
+      for (size_t k = 0; k < nch; k++) {
+        CPSZ_HIP(hipEventSynchronize(ev[k]));
+        const size_t at = k * kChunk, len = std::min(kChunk, n - at);
+        r = k + 1 < nch ? ZSTD_compressContinue(z, o + produced, cap - produced, h + at, len)
+                        : ZSTD_compressEnd(z, o + produced, cap - produced, h + at, len);
+        CPSZ_REQUIRE(!ZSTD_isError(r), kErrInternal, std::string("ZSTD: ") + ZSTD_getErrorName(r));
+        produced += r;
+      }
+      cleanup();
+      *out = keep.release();
+      *out_size = produced;
+    } catch (...) {
+      hipStreamSynchronize(st);
+      cleanup();
+      throw;
+    }
+  });
+}
+
+// Per-field ZSTD frames of a batch of device-resident streams: every field's
+// bytes are queued to pinned host memory in order (one event each), and host
+// threads compress fields as their bytes arrive, each into its own frame with
+// the reference's one-shot call. outs[k] is malloc'd (free with cpsz_free).
+int cpsz_lossless_compress_batch_dev(int device, void* stream, int level, const unsigned char* d_base,
+                                     const size_t* off, const size_t* size, size_t nf, unsigned char** outs,
+                                     size_t* out_sizes) {
+  return guarded([&] {
+    CPSZ_REQUIRE(d_base && off && size && outs && out_sizes && nf > 0, kErrArg, "null argument or empty batch");
+    Ctx& c = ctx_of(device);
+    StreamScope ss(c, stream);
+    hipStream_t st = c.stream();
+    std::vector<size_t> po(nf + 1, 0);
+    for (size_t k = 0; k < nf; k++) po[k + 1] = po[k] + size[k];
+    uint8_t* h = (uint8_t*)c.host_buf("zsb.all", po[nf] + 1);
+    std::vector<hipEvent_t> ev(nf, nullptr);
+    for (size_t k = 0; k < nf; k++) outs[k] = nullptr;
+    std::atomic<size_t> next{0};
+    std::atomic<int> failed{0};
+    std::string first_error;
+    std::mutex em;
+    auto release = [&] {
+      for (auto e : ev)
+        if (e) hipEventDestroy(e);
+    };
+    try {
+      for (size_t k = 0; k < nf; k++) {
+        CPSZ_HIP(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        if (size[k]) CPSZ_HIP(hipMemcpyAsync(h + po[k], d_base + off[k], size[k], hipMemcpyDeviceToHost, st));
+        CPSZ_HIP(hipEventRecord(ev[k], st));
+      }
+      const int nt = (int)std::min<size_t>(nf, 16);
+      std::vector<std::thread> th;
+      for (int t = 0; t < nt; t++)
+        th.emplace_back([&] {
+          ZSTD_CCtx* z = ZSTD_createCCtx();
+          for (size_t k; !failed && (k = next++) < nf;) {
+            const char* err = nullptr;
+            if (!z) err = "ZSTD_createCCtx failed";
+            else if (hipEventSynchronize(ev[k]) != hipSuccess) err = "hipEventSynchronize failed";
+            unsigned char* o = nullptr;
+            size_t r = 0;
+            if (!err) {
+              const size_t cap = ZSTD_compressBound(size[k]);
+              o = (unsigned char*)std::malloc(cap ? cap : 1);
+              if (!o) err = "host allocation failed";
+              else {
+                r = ZSTD_compressCCtx(z, o, cap, h + po[k], size[k], level);
+                if (ZSTD_isError(r)) err = ZSTD_getErrorName(r);
+              }
+            }
+            if (err) {
+              std::free(o);
+              std::lock_guard<std::mutex> g(em);
+              if (!failed.exchange(1)) first_error = err;
+              break;
+            }
+            outs[k] = o;
+            out_sizes[k] = r;
+          }
+          ZSTD_freeCCtx(z);
+        });
+      for (auto& x : th) x.join();
+      release();
+    } catch (...) {
+      CPSZ_HIP(hipStreamSynchronize(st));
+      release();
+      throw;
+    }
+    if (failed) {
+      for (size_t k = 0; k < nf; k++) {
+        std::free(outs[k]);
+        outs[k] = nullptr;
+      }
+      fail(kErrInternal, "batched ZSTD: " + first_error);
+    }
   });
 }
 

@@ -132,6 +132,23 @@ int cpsz_lossless_decompress(const unsigned char* in, size_t n, unsigned char* o
 /* Replaces is_lossless_compressed_data (reference src/sz_lossless.cpp:3-9):
  * 1 when `in` starts with a ZSTD frame whose header parses, else 0. */
 int cpsz_lossless_is_frame(const unsigned char* in, size_t n);
+/* sz_lossless_compress (src/sz_lossless.cpp:11-29) of a stream that is in
+ * device memory (as the _dev compressors return it): the bytes travel to
+ * pinned host memory in 8 MiB chunks on `stream` while the host compresses
+ * the chunks that have arrived (zstd's block-level API over the contiguous
+ * buffer, whole 128 KiB blocks per chunk); the frame is byte-identical to the
+ * reference's one-shot call at `level`.
+ * *out is malloc'd (free with cpsz_free). */
+int cpsz_lossless_compress_dev(int device, void* stream, int level, const unsigned char* d_in, size_t n,
+                               unsigned char** out, size_t* out_size);
+/* The same for a batch of device streams d_base + off[k] (size[k] bytes, as
+ * the batched compressors return them): one frame per field, each identical
+ * to the reference's sz_lossless_compress of that field's stream, compressed
+ * by up to 16 host threads as the fields' bytes arrive. */
+int cpsz_lossless_compress_batch_dev(int device, void* stream, int level, const unsigned char* d_base,
+                                     const size_t* off, const size_t* size, size_t nf, unsigned char** outs,
+                                     size_t* out_sizes);
+
 /* Replaces the decoding in sz_lossless_decompress65536bytes (reference
  * src/sz_lossless.cpp:47-62): decodes the first out_size bytes of the frame
  * (fewer if the frame is shorter) into `out`, zero-filling the rest of it;
