@@ -51,6 +51,7 @@ _SIGS = {
     "cpsz_critical_points_2d_sos_dev": [C.c_int, _P, _P, _P, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
     "cpsz_critical_points_3d_sos_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ, C.POINTER(_SZ)],
     "cpsz_sos_det": [C.c_int, _P, _P, C.POINTER(C.c_int)],
+    "cpsz_huffman_tree_dev": [C.c_int, _P, _SZ, C.c_uint32, _P, _SZ, C.POINTER(_SZ), C.POINTER(_SZ), _P, _P],
     "cpsz_critical_points_3d_dev": [C.c_int, _P, _P, _P, _P, _SZ, _SZ, _SZ, _P, _P, _SZ,
                                     C.POINTER(_SZ)],
     "cpsz_set_timing": [C.c_int, C.c_int],

@@ -22,6 +22,7 @@
 #include "cp_math.hpp"
 #include "ctx.hpp"
 #include "eb3d_math.hpp"
+#include "huff_gpu.hpp"
 #include "huff_host.hpp"
 #include "learn.hpp"
 
@@ -670,6 +671,62 @@ int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, ui
     std::memcpy(tree, t.tree.data(), t.tree.size());
     *tree_size = t.tree.size();
     *node_count = t.node_count;
+  });
+}
+
+int cpsz_huffman_tree_dev(int device, const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
+                          size_t cap, size_t* tree_size, size_t* node_count, uint64_t* codes, uint8_t* lens) {
+  return guarded([&] {
+    CPSZ_REQUIRE(freq && tree && tree_size && node_count && nbins < (1u << 24), kErrArg, "bad arguments");
+    Ctx& c = ctx_of(device);
+    hipStream_t st = c.stream();
+    std::vector<uint32_t> sf;
+    for (size_t s = 0; s < nbins; s++)
+      if (freq[s]) {
+        CPSZ_REQUIRE(freq[s] < (1ull << 32), kErrUnsupported, "counts must stay below 2^32");
+        sf.push_back((uint32_t)s);
+      }
+    const uint32_t k = (uint32_t)sf.size();
+    CPSZ_REQUIRE(k > 0, kErrArg, "empty histogram");
+    for (uint32_t q = 0; q < k; q++) sf.push_back((uint32_t)freq[sf[q]]);
+    const size_t N = 2 * (size_t)k - 1, tb = 13 * N + 1;
+    uint32_t* d_sf = c.buf<uint32_t>("ht.sf", 2 * (size_t)k + 4);
+    uint64_t* d_meta = c.buf<uint64_t>("ht.meta", 8);  // base (0), count, info[4]
+    uint64_t* d_heap = c.buf<uint64_t>("ht.heap", k + 2);
+    int32_t* d_kids = c.buf<int32_t>("ht.kids", 2 * (size_t)k);
+    uint8_t* d_tree = c.buf<uint8_t>("ht.tree", tb);
+    uint64_t* d_code = c.buf<uint64_t>("ht.code", nbins);
+    uint8_t* d_len = c.buf<uint8_t>("ht.len", nbins);
+    uint64_t meta[2] = {0, k};
+    CPSZ_HIP(hipMemcpyAsync(d_sf, sf.data(), 8 * (size_t)k, hipMemcpyHostToDevice, st));
+    CPSZ_HIP(hipMemcpyAsync(d_meta, meta, 16, hipMemcpyHostToDevice, st));
+    TreeJob j{};
+    j.syms = d_sf;
+    j.freqs = d_sf + k;
+    j.count = (const uint32_t*)(d_meta + 1);
+    j.base = d_meta;
+    j.state_num = state_num;
+    j.max_leaves = k;
+    j.heap = d_heap;
+    j.kids = d_kids;
+    j.tree = d_tree;
+    j.code = d_code;
+    j.len = d_len;
+    j.info = d_meta + 2;
+    TreeJob* d_job = c.buf<TreeJob>("ht.job", 1);
+    CPSZ_HIP(hipMemcpyAsync(d_job, &j, sizeof(j), hipMemcpyHostToDevice, st));
+    build_trees_device(c, d_job, 1);
+    uint64_t info[4];
+    CPSZ_HIP(hipMemcpyAsync(info, d_meta + 2, 32, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipStreamSynchronize(st));
+    CPSZ_REQUIRE(info[3] == 0, info[3] == 3 ? kErrUnsupported : kErrInternal,
+                 "device huffman tree failed (code " + std::to_string(info[3]) + ")");
+    CPSZ_REQUIRE(info[1] <= cap, kErrAlloc, "tree buffer too small");
+    CPSZ_HIP(hipMemcpy(tree, d_tree, info[1], hipMemcpyDeviceToHost));
+    if (codes) CPSZ_HIP(hipMemcpy(codes, d_code, 8 * nbins, hipMemcpyDeviceToHost));
+    if (lens) CPSZ_HIP(hipMemcpy(lens, d_len, nbins, hipMemcpyDeviceToHost));
+    *tree_size = info[1];
+    *node_count = info[0];
   });
 }
 

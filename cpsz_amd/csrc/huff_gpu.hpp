@@ -294,6 +294,28 @@ __global__ void __launch_bounds__(256) k_encode_multi(const EncJob<Src>* __restr
               stage[wv]);
 }
 
+// One Huffman tree built on the device (huff_tree.hip) from a histogram's
+// present symbols: syms/freqs + *base hold *count (symbol, count) pairs in
+// increasing symbol order. Outputs: the serialized tree (13 * (2k - 1) + 1
+// bytes at most), the dense codebook code[sym] / len[sym], and info = {node
+// count, tree bytes, longest code, error (0 = none)}. heap (max_leaves + 2
+// words) and kids (2 * max_leaves) are scratch for trees too large for LDS.
+struct TreeJob {
+  const uint32_t* syms;
+  const uint32_t* freqs;
+  const uint32_t* count;
+  const uint64_t* base;
+  uint32_t state_num, max_leaves;
+  uint64_t* heap;
+  int32_t* kids;
+  uint8_t* tree;
+  uint64_t* code;
+  uint8_t* len;
+  uint64_t* info;
+};
+__global__ void k_huff_tree(const TreeJob* jobs);
+void build_trees_device(Ctx& ctx, const TreeJob* d_jobs, int ntrees);
+
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v);
 __global__ void k_scatter_codebook(const uint32_t* syms, const uint64_t* codes, const uint8_t* lens,
                                    int k, uint64_t* code, uint8_t* len);

@@ -214,6 +214,11 @@ int cpsz_eb_table_2d_log(double* q, int cap, int* n);
  * frequencies and writes the serialized tree bytes and node count. */
 int cpsz_huffman_tree(const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
                       size_t cap, size_t* tree_size, size_t* node_count);
+/* The same tree built on the GPU (one wave, the reference heap's discipline)
+ * on `device`; codes/lens (nbins entries each, or NULL) receive the dense
+ * codebook the encoder uses (code left-aligned in 64 bits, length). */
+int cpsz_huffman_tree_dev(int device, const uint64_t* freq, size_t nbins, uint32_t state_num, uint8_t* tree,
+                          size_t cap, size_t* tree_size, size_t* node_count, uint64_t* codes, uint8_t* lens);
 /* Decodes n symbols of an MSB-first Huffman bitstream on the host with the
  * two-level tables the GPU decoder builds from a serialized tree (a check of
  * the table construction). */
