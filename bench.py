@@ -347,7 +347,7 @@ def run_batch2d(a, torch):
     if not a.no_cpu:
         host = [(u.cpu().numpy(), v.cpu().numpy()) for u, v in dev[:40]]
         cpu = cpu_baseline_batch2d(host, shapes[:40], eb)
-    w, g = cz.batch_2d_config(shapes)
+    w, g, dt = cz.batch_2d_config(shapes)
     out = {
         "metric": "compress+decompress GB/s and ratio at 0 CP errors: plain vs learned-eb, 1 GPU",
         "value": field_bytes / t_step / 1e9, "unit": "GB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
@@ -357,7 +357,7 @@ def run_batch2d(a, torch):
                                "rel eb 0.1 (220,143,645 vertices, 1.76 GB)", "method": "plain", "fields": 96,
                    "vertices": nv, "field_bytes": field_bytes, "stream_bytes": sb_plain, "zstd_bytes": zb_plain,
                    "ratio": field_bytes / zb_plain, "cp_errors": cp_plain, "wall_s": t_wall,
-                   "autotuned": {"workers": w, "grid_cap": g},
+                   "autotuned": {"workers": w, "grid_cap": g, "huffman_trees": "device" if dt else "host"},
                    "stages_compress_ms": stages_c, "stages_decompress_ms": stages_d,
                    "learned": learned,
                    "learned_vs_plain": None if learned is None else {

@@ -11,7 +11,7 @@ from .api import (CpszError, compress_2d_dev, compress_3d_dev, critical_points_2
                   sz_compress_cp_preserve_3d_offline_log, sz_decompress_cp_preserve_2d_offline,
                   sz_decompress_cp_preserve_2d_offline_log, sz_decompress_cp_preserve_3d_online_log,
                   sz_lossless_compress, sz_lossless_decompress)
-from .api import batch_2d_config, compress_2d_batch_dev, decompress_2d_batch_dev  # noqa: F401
+from .api import batch_2d_config, batch_2d_set_config, compress_2d_batch_dev, decompress_2d_batch_dev  # noqa: F401
 from .api import batch_3d_config, compress_3d_batch_dev, decompress_3d_batch_dev  # noqa: F401
 from .api import compress_2d_learned_batch_dev  # noqa: F401
 from .api import (compress_2d_learned_dev, gemm_bf16_dev, learn_analytic_eb_2d_dev,  # noqa: F401

@@ -60,7 +60,8 @@ _SIGS = {
     "cpsz_compress_2d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, C.c_double, C.c_int, C.POINTER(_P), _P,
                                    _P],
     "cpsz_decompress_2d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P],
-    "cpsz_batch_2d_config": [C.c_int, _SZ, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "cpsz_batch_2d_config": [C.c_int, _SZ, _P, _P, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "cpsz_batch_2d_set_config": [C.c_int, _SZ, _P, _P, C.c_int, C.c_int, C.c_int],
     "cpsz_compress_3d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, C.c_double, C.c_int,
                                    C.POINTER(_P), _P, _P],
     "cpsz_decompress_3d_batch_dev": [C.c_int, _P, _SZ, _P, _P, _P, _P, _P, _P, _P, _P],
@@ -485,12 +486,21 @@ def decompress_2d_batch_dev(streams, shapes, outs, device: int = 0, stream: int 
 
 
 def batch_2d_config(shapes, device: int = 0):
-    """(workers, grid cap) the batch calls use for batches of these shapes."""
+    """(workers, grid cap, trees on the GPU) the batch calls use for batches of
+    these shapes."""
     n = len(shapes)
-    w, g = C.c_int(), C.c_int()
+    w, g, t = C.c_int(), C.c_int(), C.c_int()
     _check(lib().cpsz_batch_2d_config(device, n, _arr(C.c_size_t, [s[0] for s in shapes]),
-                                      _arr(C.c_size_t, [s[1] for s in shapes]), C.byref(w), C.byref(g)))
-    return w.value, g.value
+                                      _arr(C.c_size_t, [s[1] for s in shapes]), C.byref(w), C.byref(g),
+                                      C.byref(t)))
+    return w.value, g.value, bool(t.value)
+
+
+def batch_2d_set_config(shapes, workers: int, grid_cap: int, device_trees: bool, device: int = 0):
+    """Pin the configuration for batches of these shapes."""
+    _check(lib().cpsz_batch_2d_set_config(device, len(shapes), _arr(C.c_size_t, [s[0] for s in shapes]),
+                                          _arr(C.c_size_t, [s[1] for s in shapes]), workers, grid_cap,
+                                          int(device_trees)))
 
 
 # --------------------------------------------------------------- 3D batches --

@@ -86,7 +86,7 @@ void compress_2d_batch(Ctx& ctx, const std::vector<Field2d>& fields, double max_
   ctx.mark("lorenzo");
   std::vector<size_t> cap(nf);
   for (size_t k = 0; k < nf; k++) cap[k] = stream_bound(w[k].r1 * w[k].r2);
-  c2_entropy_batch(ctx, wp.data(), nf, dout, out.off, cap, out.size, cfg.workers);
+  c2_entropy_batch(ctx, wp.data(), nf, dout, out.off, cap, out.size, cfg.workers, cfg.device_trees);
   ctx.mark("entropy");
   out.base = dout;
   ctx.finish_marks();
@@ -125,6 +125,12 @@ void decompress_2d_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, cons
   ctx.finish_marks();
 }
 
+void batch_2d_set_config(const std::vector<Field2d>& fields, const Batch2dConfig& cfg) {
+  const std::string key = signature(fields);
+  std::lock_guard<std::mutex> g(g_tune_m);
+  g_tuned[key] = cfg;
+}
+
 Batch2dConfig batch_2d_config(Ctx& ctx, const std::vector<Field2d>& fields, double max_pwr_eb, bool tune) {
   const std::string key = signature(fields);
   {
@@ -153,6 +159,19 @@ Batch2dConfig batch_2d_config(Ctx& ctx, const std::vector<Field2d>& fields, doub
         best = c;
       }
     }
+  {  // trees on the GPU instead, at the best worker count and grid
+    Batch2dConfig c = best;
+    c.device_trees = true;
+    CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+    const auto t0 = std::chrono::steady_clock::now();
+    compress_2d_batch(ctx, fields, max_pwr_eb, c, o);
+    CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+    const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (t < best_t) {
+      best_t = t;
+      best = c;
+    }
+  }
   std::lock_guard<std::mutex> g(g_tune_m);
   g_tuned[key] = best;
   return best;

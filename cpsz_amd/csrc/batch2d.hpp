@@ -19,6 +19,7 @@ struct Field2dOut {
 struct Batch2dConfig {
   int workers = 8;     // concurrent entropy-stage workers (own stream each)
   int grid_cap = 4096; // workgroups of the persistent wavefront launch
+  bool device_trees = false;  // Huffman trees on the GPU instead of the host pool
 };
 struct BatchOut {
   const uint8_t* base = nullptr;  // library-owned device buffer
@@ -33,5 +34,7 @@ void decompress_2d_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, cons
 // Cached configuration for this batch signature; with tune, times the
 // candidates on this batch first (results are identical for every candidate).
 Batch2dConfig batch_2d_config(Ctx& ctx, const std::vector<Field2d>& fields, double max_pwr_eb, bool tune);
+// Pin the configuration used for this batch signature (replaces a tuned one).
+void batch_2d_set_config(const std::vector<Field2d>& fields, const Batch2dConfig& cfg);
 
 }  // namespace cpsz

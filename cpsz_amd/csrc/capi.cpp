@@ -693,7 +693,6 @@ int cpsz_huffman_tree_dev(int device, const uint64_t* freq, size_t nbins, uint32
     uint32_t* d_sf = c.buf<uint32_t>("ht.sf", 2 * (size_t)k + 4);
     uint64_t* d_meta = c.buf<uint64_t>("ht.meta", 8);  // base (0), count, info[4]
     uint64_t* d_heap = c.buf<uint64_t>("ht.heap", k + 2);
-    int32_t* d_kids = c.buf<int32_t>("ht.kids", 2 * (size_t)k);
     uint8_t* d_tree = c.buf<uint8_t>("ht.tree", tb);
     uint64_t* d_code = c.buf<uint64_t>("ht.code", nbins);
     uint8_t* d_len = c.buf<uint8_t>("ht.len", nbins);
@@ -708,7 +707,6 @@ int cpsz_huffman_tree_dev(int device, const uint64_t* freq, size_t nbins, uint32
     j.state_num = state_num;
     j.max_leaves = k;
     j.heap = d_heap;
-    j.kids = d_kids;
     j.tree = d_tree;
     j.code = d_code;
     j.len = d_len;
@@ -854,7 +852,7 @@ int cpsz_decompress_2d_batch_dev(int device, void* stream, size_t nfields,
 }
 
 int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const size_t* r2, int* workers,
-                         int* grid_cap) {
+                         int* grid_cap, int* device_trees) {
   return guarded([&] {
     CPSZ_REQUIRE(nfields && r1 && r2 && workers && grid_cap, kErrArg, "null argument");
     Ctx& c = ctx_of(device);
@@ -863,6 +861,25 @@ int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const siz
     const Batch2dConfig cfg = batch_2d_config(c, sig, 0.0, false);
     *workers = cfg.workers;
     *grid_cap = cfg.grid_cap;
+    if (device_trees) *device_trees = cfg.device_trees ? 1 : 0;
+  });
+}
+
+int cpsz_batch_2d_set_config(int device, size_t nfields, const size_t* r1, const size_t* r2, int workers,
+                             int grid_cap, int device_trees) {
+  return guarded([&] {
+    CPSZ_REQUIRE(nfields && r1 && r2 && workers >= 1 && workers <= 64 && grid_cap >= 1, kErrArg, "bad arguments");
+    int cur = 0;
+    CPSZ_HIP(hipGetDevice(&cur));
+    CPSZ_HIP(hipSetDevice(device));
+    std::vector<Field2d> sig(nfields);
+    for (size_t k = 0; k < nfields; k++) sig[k] = Field2d{nullptr, nullptr, r1[k], r2[k]};
+    Batch2dConfig cfg;
+    cfg.workers = workers;
+    cfg.grid_cap = grid_cap;
+    cfg.device_trees = device_trees != 0;
+    batch_2d_set_config(sig, cfg);
+    CPSZ_HIP(hipSetDevice(cur));
   });
 }
 

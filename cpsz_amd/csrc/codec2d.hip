@@ -885,7 +885,7 @@ __global__ void __launch_bounds__(256) k_gather_unpred_multi(const UnpJob* __res
   const UnpJob& J = jobs[job_of(jobs, nj, blockIdx.x)];
   const size_t b = (size_t)(blockIdx.x - J.blk0) * kUnpWaves + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  if (b >= J.nb) return;
+  if (b >= J.nb || !J.out) return;
   const BlockView v(b, J.r1, J.r2, J.nby);
   uint64_t o = J.off[b] - J.off[0];
   const uint64_t below = (1ull << lane) - 1;
@@ -994,6 +994,27 @@ __global__ void __launch_bounds__(256) k_scatter_codebooks(const CbJob* __restri
   }
 }
 
+// A host-built tree moved to where the device tree builder puts its results:
+// codebook scattered into the dense tables, serialized bytes, info words.
+struct HostTreeJob {
+  CbJob cb;
+  const uint8_t* tree_src;
+  uint8_t* tree_dst;
+  uint64_t tree_n;
+  uint64_t* info_dst;
+  uint64_t info[4];
+};
+
+__global__ void __launch_bounds__(256) k_host_trees_in(const HostTreeJob* __restrict__ jobs) {
+  const HostTreeJob& J = jobs[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < J.cb.k; i += blockDim.x) {
+    J.cb.code[J.cb.syms[i]] = J.cb.codes[i];
+    J.cb.len[J.cb.syms[i]] = J.cb.lens[i];
+  }
+  for (uint64_t i = threadIdx.x; i < J.tree_n; i += blockDim.x) J.tree_dst[i] = J.tree_src[i];
+  if (threadIdx.x < 4) J.info_dst[threadIdx.x] = J.info[threadIdx.x];
+}
+
 // out[j] = a[j].hi - a[j].lo: per-stream sums read off global exclusive scans.
 struct ScanDiff {
   const uint64_t* lo;
@@ -1017,11 +1038,7 @@ struct CopyPiece {
   uint32_t pad;
 };
 
-__global__ void __launch_bounds__(256) k_assemble(const CopyPiece* __restrict__ pieces, uint8_t* __restrict__ out) {
-  const CopyPiece P = pieces[blockIdx.x];
-  uint8_t* d = out + P.dst;
-  const uint8_t* s = P.src;
-  const uint32_t n = P.len;
+__device__ __forceinline__ void copy_piece(uint8_t* __restrict__ d, const uint8_t* __restrict__ s, uint32_t n) {
   const uint32_t head = min(n, (uint32_t)((4u - ((uint32_t)(uintptr_t)d & 3u)) & 3u));
   const uint32_t nw = (n - head) >> 2, tail0 = head + 4 * nw;
   if (threadIdx.x < head) d[threadIdx.x] = s[threadIdx.x];
@@ -1035,6 +1052,155 @@ __global__ void __launch_bounds__(256) k_assemble(const CopyPiece* __restrict__ 
   } else {
     for (uint32_t k = threadIdx.x; k < nw; k += blockDim.x)
       dw[k] = __builtin_amdgcn_alignbyte(sw[k + 1], sw[k], mis);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_assemble(const CopyPiece* __restrict__ pieces, uint8_t* __restrict__ out) {
+  const CopyPiece P = pieces[blockIdx.x];
+  copy_piece(out + P.dst, P.src, P.len);
+}
+
+// ---- device-side layout of a batch's streams: every size and offset follows
+// from the per-stream totals and the device-built trees, so no host round trip
+// sits between the scans and the assembled streams.
+struct Layout2d {
+  // set by the host
+  uint64_t off;              // stream offset in the batch output
+  uint64_t cap;              // bytes available there
+  uint64_t nb;               // 12x12 blocks
+  const uint64_t* tot;       // bits of the eb/u/v streams, unpredictable u/v counts
+  const uint64_t* tinfo[3];  // device tree info of the eb/u/v trees (node count, tree bytes, ...)
+  const uint8_t* tree[3];
+  EncJob<SrcIds2d>* ej_ids;  // jobs whose word arrays k_layout2d fills in
+  EncJob<SrcCodes2d>* ej_codes;  // [2]
+  UnpJob* gather;                // [2]
+  // set by k_layout2d
+  uint32_t* words;  // eb, u, v word arrays, consecutive
+  float* unp;       // u then v unpredictable values
+  uint64_t bytes[3], wn[3], unp_n[2], size;
+  uint64_t at[11];  // hdr, h_eb, eb bits, hu1, unp u, hu2, u bits, hv1, unp v, hv2, v bits
+};
+
+// One thread: sizes, offsets, the word and value arrays of every field (in
+// the preallocated pools: *err set when they would overflow), sizes[k].
+__global__ void k_layout2d(Layout2d* __restrict__ L, int nf, uint32_t* words, uint64_t words_cap, float* unp,
+                           uint64_t unp_cap, uint64_t* __restrict__ sizes, uint64_t* __restrict__ words_used,
+                           int* __restrict__ err) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  uint64_t wpos = 0, upos = 0;
+  for (int k = 0; k < nf; k++) {
+    Layout2d& F = L[k];
+    uint64_t tb[3];
+    for (int t = 0; t < 3; t++) {
+      F.bytes[t] = (F.tot[t] + 7) / 8;
+      F.wn[t] = (F.bytes[t] + 3) / 4 + 2;
+      tb[t] = F.tinfo[t][1];
+    }
+    F.unp_n[0] = F.tot[3];
+    F.unp_n[1] = F.tot[4];
+    const uint64_t ind = (F.nb - 1) / 8 + 1;
+    const uint64_t h_eb = 8 + 4 + tb[0] + 8, h1 = 4 + 1 + 4 + 8 + 8 + 8;
+    const uint64_t hu2 = ind + 8 + 4 + tb[1] + 8, hv2 = ind + 8 + 4 + tb[2] + 8;
+    uint64_t p = F.off;
+    const uint64_t sec[11] = {36, h_eb, F.bytes[0], h1, 4 * F.unp_n[0], hu2, F.bytes[1], h1, 4 * F.unp_n[1], hv2,
+                              F.bytes[2]};
+    for (int q = 0; q < 11; q++) {
+      F.at[q] = p;
+      p += sec[q];
+    }
+    F.size = p - F.off;
+    sizes[k] = F.size;
+    const uint64_t wneed = (F.wn[0] + F.wn[1] + F.wn[2] + 63) & ~63ull;
+    const uint64_t uneed = (F.unp_n[0] + F.unp_n[1] + 63) & ~63ull;
+    if (F.size > F.cap || wpos + wneed > words_cap || upos + uneed > unp_cap) {
+      // this and the remaining fields keep null arrays: the passes that
+      // follow skip them, and the host reports the error
+      *err = 1;
+      break;
+    }
+    F.words = words + wpos;
+    F.unp = unp + upos;
+    wpos += wneed;
+    upos += uneed;
+    F.ej_ids->words = F.words;
+    F.ej_codes[0].words = F.words + F.wn[0];
+    F.ej_codes[1].words = F.words + F.wn[0] + F.wn[1];
+    F.gather[0].out = F.unp;
+    F.gather[1].out = F.unp + F.unp_n[0];
+  }
+  *words_used = wpos;
+}
+
+__global__ void k_fill_words(uint32_t* __restrict__ w, const uint64_t* __restrict__ n) {
+  const uint64_t m = *n;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < m; i += (uint64_t)gridDim.x * blockDim.x)
+    w[i] = 0;
+}
+
+__device__ __forceinline__ void put_bytes(uint8_t* p, uint64_t v, int nbytes) {
+  for (int b = 0; b < nbytes; b++) p[b] = (uint8_t)(v >> (8 * b));
+}
+
+// One workgroup per field: the stream header, the id tree head, the two
+// compressor heads and the two code tree heads (all-zero block indicators
+// first), trees copied from the device builds (the host layout of
+// src/sz_compress_cp_preserve_2d.cpp:258-269, src/sz_compress_3d.cpp:219-228).
+__global__ void __launch_bounds__(256) k_write_heads2d(const Layout2d* __restrict__ L, uint8_t* __restrict__ out) {
+  const Layout2d& F = L[blockIdx.x];
+  if (!F.words) return;  // not laid out (an overflow the host reports)
+  const uint64_t ind = (F.nb - 1) / 8 + 1;
+  if (threadIdx.x == 0) {
+    uint8_t* h = out + F.at[0];
+    put_bytes(h, 4, 4);
+    put_bytes(h + 4, 0x3CB0000000000000ull, 8);  // DBL_EPSILON
+    put_bytes(h + 12, F.bytes[0] + 8 + 4 + F.tinfo[0][1] + 8, 8);
+    put_bytes(h + 20, F.at[7] - F.at[3], 8);
+    put_bytes(h + 28, F.off + F.size - F.at[7], 8);
+    for (int c = 0; c < 2; c++) {
+      uint8_t* q = out + F.at[c ? 7 : 3];
+      put_bytes(q, kBlock2d, 4);
+      q[4] = 0;
+      put_bytes(q + 5, kRadius, 4);
+      put_bytes(q + 9, 0, 8);   // meanInfo<float>{false, 0}
+      put_bytes(q + 17, 0, 8);  // reg_count
+      put_bytes(q + 25, F.unp_n[c], 8);
+    }
+  }
+  for (int t = 0; t < 3; t++) {
+    uint8_t* q = out + (t == 0 ? F.at[1] : F.at[t == 1 ? 5 : 9] + ind);
+    const uint64_t tb = F.tinfo[t][1];
+    if (threadIdx.x == 0) {
+      put_bytes(q, F.tinfo[t][0], 8);
+      put_bytes(q + 8, tb, 4);
+      put_bytes(q + 12 + tb, F.bytes[t], 8);
+    }
+    for (uint64_t b = threadIdx.x; b < tb; b += blockDim.x) q[12 + b] = F.tree[t][b];
+  }
+  for (uint64_t b = threadIdx.x; b < ind; b += blockDim.x) {
+    out[F.at[5] + b] = 0;
+    out[F.at[9] + b] = 0;
+  }
+}
+
+// The five device sections of every field (bitstreams and unpredictable
+// values) into place, 64 KiB pieces spread over the workgroups of the
+// field's grid row.
+__global__ void __launch_bounds__(256) k_assemble2d(const Layout2d* __restrict__ L, uint8_t* __restrict__ out) {
+  const Layout2d& F = L[blockIdx.y];
+  if (!F.words) return;
+  constexpr uint64_t kP = 65536;
+  const uint8_t* src[5] = {(const uint8_t*)F.words, (const uint8_t*)F.unp, (const uint8_t*)(F.words + F.wn[0]),
+                           (const uint8_t*)(F.unp + F.unp_n[0]), (const uint8_t*)(F.words + F.wn[0] + F.wn[1])};
+  const uint64_t len[5] = {F.bytes[0], 4 * F.unp_n[0], F.bytes[1], 4 * F.unp_n[1], F.bytes[2]};
+  const uint64_t dst[5] = {F.at[2], F.at[4], F.at[6], F.at[8], F.at[10]};
+  uint64_t p = blockIdx.x;
+  for (int q = 0; q < 5; q++) {
+    const uint64_t np = (len[q] + kP - 1) / kP;
+    for (; p < np; p += gridDim.x) {
+      const uint64_t o = p * kP;
+      copy_piece(out + dst[q] + o, src[q] + o, (uint32_t)min(kP, len[q] - o));
+    }
+    p -= np;
   }
 }
 
@@ -1503,22 +1669,25 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   CPSZ_HIP(hipStreamSynchronize(st));
 }
 
-// Batched entropy stage (plain offline, float): every field's histograms,
-// encodes, unpredictable gathers and stream assembly run as a few launches
-// over all fields, with three host round trips for the whole batch (the
-// present symbols of every histogram, the per-stream bit and unpredictable
-// totals, the finished layout). The Huffman trees are built on the host, in
-// parallel over the batch. Each field's stream is byte-identical to
+// Batched entropy stage (plain offline, float), on the device from the codes
+// to the finished streams: every field's histograms, block zero counts and
+// their scan, the present symbols of every histogram, the Huffman trees (one
+// wave per tree, huff_tree.hip), chunk bit counts and one scan for every
+// stream, then the layout (sizes, offsets, word arrays) computed by a kernel
+// from the per-stream totals, the encodes, the unpredictable gathers, the
+// headers and the assembly. The host only fills the job tables up front and
+// reads the sizes at the end. Each field's stream is byte-identical to
 // c2_entropy's.
 void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, const std::vector<size_t>& off,
-                      const std::vector<size_t>& cap, std::vector<size_t>& size, int workers) {
+                      const std::vector<size_t>& cap, std::vector<size_t>& size, int workers, bool device_trees) {
   hipStream_t st = ctx.stream();
   constexpr size_t HN = kMaxEbCode + 2 * 65536;  // dense codebook entries per field
+  constexpr uint32_t kMaxCodeLeaves = 65536, kMaxIdLeaves = kMaxEbCode;
   struct G {
     size_t n, nb, nby, nch_ids, zb, cb;
   };
   std::vector<G> g(nf);
-  size_t Z = 0, CB = 0;
+  size_t Z = 0, CB = 0, words_cap = 0, unp_cap = 0;
   for (size_t k = 0; k < nf; k++) {
     const C2Work& W = *w[k];
     CPSZ_REQUIRE(!W.log && !W.f64, kErrInternal, "batched entropy: plain float fields only");
@@ -1531,37 +1700,70 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     Z += 2 * x.nb;
     x.cb = CB;
     CB += x.nch_ids + 2 * x.nb;
+    words_cap += cap[k] / 4 + 6 + 64;
+    unp_cap += 2 * x.n + 64;
   }
-  auto arena = [&](const char* name, size_t bytes) {
-    Arena a;
-    a.h = (uint8_t*)ctx.host_buf(name, bytes);
-    a.d = ctx.buf<uint8_t>(name, bytes);
-    a.cap = bytes;
-    return a;
-  };
-  auto send = [&](Arena& a) {
-    if (a.used) CPSZ_HIP(hipMemcpyAsync(a.d, a.h, a.used, hipMemcpyHostToDevice, st));
-  };
-  // ---- phase A: histograms of the quant codes, block zero counts and their
-  // scan, present symbols of every histogram (eb ids counted by the eb pass)
+  const size_t ntree = 3 * nf;
+  // ---- device pools (fixed per batch shape)
   uint32_t* zc = ctx.buf<uint32_t>("b2e.zc", Z + 1);
   uint64_t* zoff = ctx.buf<uint64_t>("b2e.zoff", Z + 1);
-  const size_t ntree = 3 * nf;
-  Arena A = arena("b2e.tabA", 2 * nf * (sizeof(HistJob<U16Hist>) + sizeof(UnpJob)) + ntree * sizeof(CompactJob) +
-                                  3 * 64);
+  uint64_t* bits = ctx.buf<uint64_t>("b2e.bits", CB + 1);
+  uint64_t* offs = ctx.buf<uint64_t>("b2e.offs", CB + 1);
+  uint64_t* cb_code = ctx.buf<uint64_t>("b2e.cbcode", nf * HN);
+  uint8_t* cb_len = ctx.buf<uint8_t>("b2e.cblen", nf * HN);
+  uint32_t* d_syms = ctx.buf<uint32_t>("b2e.syms", nf * (kMaxIdLeaves + 2 * (size_t)kMaxCodeLeaves));
+  uint32_t* d_freqs = ctx.buf<uint32_t>("b2e.freqs", nf * (kMaxIdLeaves + 2 * (size_t)kMaxCodeLeaves));
+  // per tree: heap (code trees only: id trees fit LDS), tree bytes, info
+  const size_t heap_n = kMaxCodeLeaves + 2;
+  const size_t tb_code = 13 * (2 * (size_t)kMaxCodeLeaves - 1) + 1, tb_id = 13 * (2 * (size_t)kMaxIdLeaves - 1) + 1;
+  const size_t tb_field = (tb_id + 2 * tb_code + 3 * 255) & ~(size_t)255;
+  uint64_t* t_heap = ctx.buf<uint64_t>("b2e.theap", 2 * nf * heap_n);
+  uint8_t* t_tree = ctx.buf<uint8_t>("b2e.ttree", nf * tb_field);
+  uint8_t* meta = ctx.buf<uint8_t>("b2e.meta", ntree * (12 + 32) + nf * 8 + 5 * nf * 8 + 64);
+  unsigned long long* d_total = (unsigned long long*)meta;        // present symbols, all trees
+  uint64_t* d_words_used = (uint64_t*)(meta + 8);
+  int* d_err = (int*)(meta + 16);                                  // layout pool overflow
+  uint64_t* d_base = (uint64_t*)(meta + 64);
+  uint64_t* d_info = d_base + ntree;                               // 4 per tree
+  uint64_t* d_sizes = d_info + 4 * ntree;
+  uint64_t* d_tot = d_sizes + nf;
+  uint32_t* d_count = (uint32_t*)(d_tot + 5 * nf);
+  uint32_t* words = ctx.buf<uint32_t>("b2e.words", words_cap);
+  float* unp = ctx.buf<float>("b2e.unp", unp_cap);
+  // ---- job tables, filled on the host and sent in one copy
+  Arena A;
+  const size_t abytes = 2 * nf * (sizeof(HistJob<U16Hist>) + 2 * sizeof(UnpJob)) + ntree * sizeof(CompactJob) +
+                        ntree * sizeof(TreeJob) + nf * sizeof(EncJob<SrcIds2d>) + 2 * nf * sizeof(EncJob<SrcCodes2d>) +
+                        5 * nf * sizeof(ScanDiff) + nf * sizeof(Layout2d) + 16 * 64;
+  A.h = (uint8_t*)ctx.host_buf("b2e.tab", abytes);
+  A.d = ctx.buf<uint8_t>("b2e.tab", abytes);
+  A.cap = abytes;
   HistJob<U16Hist>* hj;
   HistJob<U16Hist>* d_hj = A.take(2 * nf, &hj);
   UnpJob* uj;
   UnpJob* d_uj = A.take(2 * nf, &uj);
+  UnpJob* gj;
+  UnpJob* d_gj = A.take(2 * nf, &gj);
   CompactJob* cj;
   CompactJob* d_cj = A.take(ntree, &cj);
-  uint32_t hblk = 0, ublk = 0;
+  TreeJob* tj;
+  TreeJob* d_tj = A.take(ntree, &tj);
+  EncJob<SrcIds2d>* ej_i;
+  EncJob<SrcIds2d>* d_ej_i = A.take(nf, &ej_i);
+  EncJob<SrcCodes2d>* ej_c;
+  EncJob<SrcCodes2d>* d_ej_c = A.take(2 * nf, &ej_c);
+  ScanDiff* sd;
+  ScanDiff* d_sd = A.take(5 * nf, &sd);
+  Layout2d* lj;
+  Layout2d* d_lj = A.take(nf, &lj);
+  uint32_t hblk = 0, ublk = 0, iblk = 0, cblk = 0;
   for (size_t k = 0; k < nf; k++) {
     const C2Work& W = *w[k];
-    const size_t nch = div_up(g[k].n, (size_t)U16Hist::kChunk);
+    const G& x = g[k];
+    const size_t nch = div_up(x.n, (size_t)U16Hist::kChunk);
     for (int c = 0; c < 2; c++) {
       HistJob<U16Hist>& h = hj[2 * k + c];
-      h.src = U16Hist{c ? W.qv : W.qu, g[k].n};
+      h.src = U16Hist{c ? W.qv : W.qu, x.n};
       h.nchunks = nch;
       h.hist = W.hist + kMaxEbCode + c * 65536;
       h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 8), 512);
@@ -1573,103 +1775,32 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
       u.x = c ? W.dV : W.dU;
       u.r1 = (int)W.r1;
       u.r2 = (int)W.r2;
-      u.nby = (int)g[k].nby;
-      u.nb = g[k].nb;
-      u.cnt = zc + g[k].zb + c * g[k].nb;
-      u.off = zoff + g[k].zb + c * g[k].nb;
+      u.nby = (int)x.nby;
+      u.nb = x.nb;
+      u.cnt = zc + x.zb + c * x.nb;
+      u.off = zoff + x.zb + c * x.nb;
       u.blk0 = ublk;
-      ublk += (uint32_t)unp_grid(g[k].nb);
+      ublk += (uint32_t)unp_grid(x.nb);
+      gj[2 * k + c] = u;  // the gather's output array comes from k_layout2d
     }
-    cj[3 * k + 0] = CompactJob{W.hist, (uint32_t)kMaxEbCode};
-    cj[3 * k + 1] = CompactJob{W.hist + kMaxEbCode, 65536u};
-    cj[3 * k + 2] = CompactJob{W.hist + kMaxEbCode + 65536, 65536u};
-  }
-  send(A);
-  const uint32_t win_lo = 32768 - 8192, win_n = 16384;
-  k_hist_multi<U16Hist><<<hblk, 256, win_n * 4, st>>>(d_hj, (int)(2 * nf), win_lo, win_n);
-  k_block_zeros_multi<<<ublk, 256, 0, st>>>(d_uj, (int)(2 * nf));
-  CPSZ_HIP(hipGetLastError());
-  scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
-  // present symbols: counts, bases and the running total, then the lists
-  uint8_t* cmeta = ctx.buf<uint8_t>("b2e.cmeta", ntree * 12 + 64);
-  unsigned long long* d_total = (unsigned long long*)cmeta;
-  uint64_t* d_base = (uint64_t*)(cmeta + 8);
-  uint32_t* d_count = (uint32_t*)(cmeta + 8 + 8 * ntree);
-  const size_t max_present = ntree * (size_t)65536;
-  uint32_t* d_syms = ctx.buf<uint32_t>("b2e.syms", max_present);
-  uint32_t* d_freqs = ctx.buf<uint32_t>("b2e.freqs", max_present);
-  k_zero_i32<<<1, 64, 0, st>>>((int*)d_total, 2);
-  k_compact_hist<<<(unsigned)ntree, 1024, 0, st>>>(d_cj, d_count, d_base, d_total, d_syms, d_freqs);
-  CPSZ_HIP(hipGetLastError());
-  uint8_t* hmeta = (uint8_t*)ctx.host_buf("b2e.cmeta", ntree * 12 + 64);
-  CPSZ_HIP(hipMemcpyAsync(hmeta, cmeta, ntree * 12 + 8, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
-  const uint64_t present = *(const uint64_t*)hmeta;
-  const uint64_t* h_base = (const uint64_t*)(hmeta + 8);
-  const uint32_t* h_count = (const uint32_t*)(hmeta + 8 + 8 * ntree);
-  uint32_t* h_sf = (uint32_t*)ctx.host_buf("b2e.sf", 8 * present + 8);
-  CPSZ_HIP(hipMemcpyAsync(h_sf, d_syms, 4 * present, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipMemcpyAsync(h_sf + present, d_freqs, 4 * present, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
-  ctx.mark("hist");
-  // ---- trees on the host, in parallel over the batch
-  // kept per calling thread so their storage is reused; the pool threads reach
-  // them through a reference (a thread_local named inside a job would be the
-  // worker's own)
-  thread_local std::vector<HuffEncodeTable> tabs_keep;
-  std::vector<HuffEncodeTable>& tabs = tabs_keep;
-  if (tabs.size() < ntree) tabs.resize(ntree);
-  {
-    std::vector<std::function<void(Ctx&)>> jobs(ntree);
-    for (size_t t = 0; t < ntree; t++)
-      jobs[t] = [&, t](Ctx&) {
-        CPSZ_REQUIRE(h_count[t] > 0, kErrInternal, "batched entropy: empty histogram");
-        huff_build_sparse(h_sf + h_base[t], h_sf + present + h_base[t], h_count[t],
-                          t % 3 == 0 ? 2 * 1024 : 2 * kCapacity, tabs[t]);
-      };
-    pool_run(pool_for(ctx.device()), jobs, std::max(workers, 1));
-  }
-  ctx.mark("tree");
-  // ---- phase B: codebooks, chunk bit counts, one scan for every stream,
-  // per-stream totals
-  uint64_t* cb_code = ctx.buf<uint64_t>("b2e.cbcode", nf * HN);
-  uint8_t* cb_len = ctx.buf<uint8_t>("b2e.cblen", nf * HN);
-  uint64_t* bits = ctx.buf<uint64_t>("b2e.bits", CB + 1);
-  uint64_t* offs = ctx.buf<uint64_t>("b2e.offs", CB + 1);
-  size_t cb_entries = 0;
-  for (size_t t = 0; t < ntree; t++) cb_entries += tabs[t].syms.size();
-  Arena B = arena("b2e.tabB", cb_entries * 13 + ntree * (sizeof(CbJob) + 48) + nf * sizeof(EncJob<SrcIds2d>) +
-                                  2 * nf * sizeof(EncJob<SrcCodes2d>) + 5 * nf * sizeof(ScanDiff) + 8 * 64);
-  CbJob* cbj;
-  CbJob* d_cbj = B.take(ntree, &cbj);
-  EncJob<SrcIds2d>* ej_i;
-  EncJob<SrcIds2d>* d_ej_i = B.take(nf, &ej_i);
-  EncJob<SrcCodes2d>* ej_c;
-  EncJob<SrcCodes2d>* d_ej_c = B.take(2 * nf, &ej_c);
-  ScanDiff* sd;
-  ScanDiff* d_sd = B.take(5 * nf, &sd);
-  for (size_t t = 0; t < ntree; t++) {
-    const HuffEncodeTable& T = tabs[t];
-    const size_t kk = T.syms.size();
-    uint64_t* hc;
-    uint32_t* hs;
-    uint8_t* hl;
-    cbj[t].codes = B.take(kk, &hc);
-    cbj[t].syms = B.take(kk, &hs);
-    cbj[t].lens = B.take(kk, &hl);
-    std::memcpy(hc, T.codes.data(), 8 * kk);
-    std::memcpy(hs, T.syms.data(), 4 * kk);
-    std::memcpy(hl, T.lens.data(), kk);
-    cbj[t].k = (uint32_t)kk;
-    const size_t k = t / 3, which = t % 3;
-    const size_t at = k * HN + (which == 0 ? 0 : kMaxEbCode + (which - 1) * 65536);
-    cbj[t].code = cb_code + at;
-    cbj[t].len = cb_len + at;
-  }
-  uint32_t iblk = 0, cblk = 0;
-  for (size_t k = 0; k < nf; k++) {
-    const C2Work& W = *w[k];
-    const G& x = g[k];
+    for (int t = 0; t < 3; t++) {
+      const size_t tt = 3 * k + t;
+      cj[tt] = CompactJob{W.hist + (t == 0 ? 0 : kMaxEbCode + (t - 1) * 65536), t == 0 ? (uint32_t)kMaxEbCode : 65536u};
+      TreeJob& T = tj[tt];
+      T = TreeJob{};
+      T.syms = d_syms;
+      T.freqs = d_freqs;
+      T.count = d_count + tt;
+      T.base = d_base + tt;
+      T.state_num = t == 0 ? 2 * 1024 : 2 * kCapacity;
+      T.max_leaves = t == 0 ? kMaxIdLeaves : kMaxCodeLeaves;
+      T.heap = t == 0 ? nullptr : t_heap + (2 * k + t - 1) * heap_n;
+      T.tree = t_tree + k * tb_field + (t == 0 ? 0 : ((tb_id + 255) & ~(size_t)255) + (t - 1) * ((tb_code + 255) & ~(size_t)255));
+      const size_t at = k * HN + (t == 0 ? 0 : kMaxEbCode + (t - 1) * 65536);
+      T.code = cb_code + at;
+      T.len = cb_len + at;
+      T.info = d_info + 4 * tt;
+    }
     EncJob<SrcIds2d>& e = ej_i[k];
     e.src = SrcIds2d{W.cu, W.cv, x.n, 2 * x.n};
     e.nchunks = x.nch_ids;
@@ -1698,118 +1829,151 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     sd[5 * k + 2] = ScanDiff{offs + c2, offs + c3};
     sd[5 * k + 3] = ScanDiff{zoff + x.zb, zoff + x.zb + x.nb};
     sd[5 * k + 4] = ScanDiff{zoff + x.zb + x.nb, zoff + x.zb + 2 * x.nb};
+    Layout2d& l = lj[k];
+    l = Layout2d{};
+    l.off = off[k];
+    l.cap = cap[k];
+    l.nb = x.nb;
+    l.tot = d_tot + 5 * k;
+    for (int t = 0; t < 3; t++) {
+      l.tinfo[t] = d_info + 4 * (3 * k + t);
+      l.tree[t] = tj[3 * k + t].tree;
+    }
+    l.ej_ids = d_ej_i + k;
+    l.ej_codes = d_ej_c + 2 * k;
+    l.gather = d_gj + 2 * k;
   }
-  send(B);
-  k_scatter_codebooks<<<(unsigned)ntree, 256, 0, st>>>(d_cbj);
+  CPSZ_HIP(hipMemcpyAsync(A.d, A.h, A.used, hipMemcpyHostToDevice, st));
+  // ---- histograms, zero counts, present symbols, trees
+  const uint32_t win_lo = 32768 - 8192, win_n = 16384;
+  k_hist_multi<U16Hist><<<hblk, 256, win_n * 4, st>>>(d_hj, (int)(2 * nf), win_lo, win_n);
+  k_block_zeros_multi<<<ublk, 256, 0, st>>>(d_uj, (int)(2 * nf));
+  CPSZ_HIP(hipGetLastError());
+  scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
+  k_zero_i32<<<1, 64, 0, st>>>((int*)meta, 16);  // total, words used, error flag
+  k_compact_hist<<<(unsigned)ntree, 1024, 0, st>>>(d_cj, d_count, d_base, d_total, d_syms, d_freqs);
+  CPSZ_HIP(hipGetLastError());
+  if (device_trees) {
+    ctx.mark("hist");
+    build_trees_device(ctx, d_tj, (int)ntree);
+  } else {
+    // present symbols to the host, trees on the worker pool, results moved to
+    // where the device builder would have put them
+    const size_t cm = 12 * ntree;
+    uint8_t* hc = (uint8_t*)ctx.host_buf("b2e.hcnt", cm + 64);
+    CPSZ_HIP(hipMemcpyAsync(hc, d_total, 8, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipMemcpyAsync(hc + 64, d_base, 8 * ntree, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipMemcpyAsync(hc + 64 + 8 * ntree, d_count, 4 * ntree, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipStreamSynchronize(st));
+    const uint64_t present = *(const uint64_t*)hc;
+    const uint64_t* h_base = (const uint64_t*)(hc + 64);
+    const uint32_t* h_count = (const uint32_t*)(hc + 64 + 8 * ntree);
+    uint32_t* h_sf = (uint32_t*)ctx.host_buf("b2e.sf", 8 * present + 8);
+    CPSZ_HIP(hipMemcpyAsync(h_sf, d_syms, 4 * present, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipMemcpyAsync(h_sf + present, d_freqs, 4 * present, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipStreamSynchronize(st));
+    ctx.mark("hist");
+    // kept per calling thread so their storage is reused; the pool threads
+    // reach them through a reference (a thread_local named inside a job would
+    // be the worker's own)
+    thread_local std::vector<HuffEncodeTable> tabs_keep;
+    std::vector<HuffEncodeTable>& tabs = tabs_keep;
+    if (tabs.size() < ntree) tabs.resize(ntree);
+    // the upload table's layout follows from the leaf counts alone, so each
+    // worker writes its tree's results straight into it
+    auto tree_bytes_of = [](size_t k) {
+      const size_t N = 2 * k - 1, w = N <= 256 ? 1 : (N <= 65536 ? 2 : 4);
+      return 1 + 2 * N * w + 5 * N;
+    };
+    size_t tbytes = 0;
+    for (size_t t = 0; t < ntree; t++) tbytes += h_count[t] * 13 + tree_bytes_of(h_count[t] ? h_count[t] : 1) + 64;
+    Arena B;
+    const size_t bbytes = tbytes + ntree * sizeof(HostTreeJob) + 64;
+    B.h = (uint8_t*)ctx.host_buf("b2e.tabB", bbytes);
+    B.d = ctx.buf<uint8_t>("b2e.tabB", bbytes);
+    B.cap = bbytes;
+    HostTreeJob* hj2;
+    HostTreeJob* d_hj2 = B.take(ntree, &hj2);
+    std::vector<uint8_t*> hdst(4 * ntree);
+    for (size_t t = 0; t < ntree; t++) {
+      const size_t kk = h_count[t];
+      HostTreeJob& J = hj2[t];
+      J.cb.codes = B.take(kk, (uint64_t**)&hdst[4 * t]);
+      J.cb.syms = B.take(kk, (uint32_t**)&hdst[4 * t + 1]);
+      J.cb.lens = B.take(kk, &hdst[4 * t + 2]);
+      J.tree_src = B.take(tree_bytes_of(kk ? kk : 1), &hdst[4 * t + 3]);
+      J.cb.k = (uint32_t)kk;
+      J.cb.code = tj[t].code;
+      J.cb.len = tj[t].len;
+      J.tree_dst = tj[t].tree;
+      J.info_dst = tj[t].info;
+    }
+    {
+      std::vector<std::function<void(Ctx&)>> jobs(ntree);
+      for (size_t t = 0; t < ntree; t++)
+        jobs[t] = [&, t](Ctx&) {
+          CPSZ_REQUIRE(h_count[t] > 0, kErrInternal, "batched entropy: empty histogram");
+          HuffEncodeTable& T = tabs[t];
+          huff_build_sparse(h_sf + h_base[t], h_sf + present + h_base[t], h_count[t], tj[t].state_num, T);
+          const size_t kk = T.syms.size();
+          CPSZ_REQUIRE(kk == h_count[t] && T.tree.size() == tree_bytes_of(kk), kErrInternal,
+                       "batched entropy: tree size mismatch");
+          std::memcpy(hdst[4 * t], T.codes.data(), 8 * kk);
+          std::memcpy(hdst[4 * t + 1], T.syms.data(), 4 * kk);
+          std::memcpy(hdst[4 * t + 2], T.lens.data(), kk);
+          std::memcpy(hdst[4 * t + 3], T.tree.data(), T.tree.size());
+          HostTreeJob& J = hj2[t];
+          J.tree_n = T.tree.size();
+          J.info[0] = T.node_count;
+          J.info[1] = T.tree.size();
+          J.info[2] = (uint64_t)T.max_len;
+          J.info[3] = 0;
+        };
+      pool_run(pool_for(ctx.device()), jobs, std::max(workers, 1));
+    }
+    CPSZ_HIP(hipMemcpyAsync(B.d, B.h, B.used, hipMemcpyHostToDevice, st));
+    k_host_trees_in<<<(unsigned)ntree, 256, 0, st>>>(d_hj2);
+    CPSZ_HIP(hipGetLastError());
+  }
+  ctx.mark("tree");
+  // ---- chunk bit counts, one scan for every stream, per-stream totals
   k_chunk_bits_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ej_i, (int)nf);
   k_chunk_bits_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_ej_c, (int)(2 * nf));
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits, offs, CB, offs + CB);
-  uint64_t* d_tot = ctx.buf<uint64_t>("b2e.tot", 5 * nf);
   k_scan_diffs<<<(unsigned)div_up(5 * nf, 256), 256, 0, st>>>(d_sd, (int)(5 * nf), d_tot);
   CPSZ_HIP(hipGetLastError());
-  uint64_t* h_tot = (uint64_t*)ctx.host_buf("b2e.tot", 8 * 5 * nf);
-  CPSZ_HIP(hipMemcpyAsync(h_tot, d_tot, 8 * 5 * nf, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
   ctx.mark("scan");
-  // ---- phase C: layout, encode, gather, assembly
-  std::vector<Heads2d> H(nf);
-  std::vector<size_t> wofs(nf + 1, 0), uofs(nf + 1, 0);
-  size_t hbytes = 0, npieces = 0;
-  constexpr size_t kPiece = 65536;
-  auto pieces_of = [&](size_t len) { return div_up(len, kPiece); };
-  for (size_t k = 0; k < nf; k++) {
-    const uint64_t* t5 = h_tot + 5 * k;
-    const size_t bytes_eb = div_up(t5[0], 8), bytes_u = div_up(t5[1], 8), bytes_v = div_up(t5[2], 8);
-    heads_2d(tabs[3 * k], tabs[3 * k + 1], tabs[3 * k + 2], bytes_eb, bytes_u, bytes_v, t5[3], t5[4], g[k].nb,
-             false, 4, H[k]);
-    const size_t words = (div_up(bytes_eb, 4) + 2) + (div_up(bytes_u, 4) + 2) + (div_up(bytes_v, 4) + 2);
-    wofs[k + 1] = wofs[k] + ((words + 63) & ~(size_t)63);
-    uofs[k + 1] = uofs[k] + ((t5[3] + t5[4] + 63) & ~(size_t)63);
-    const Heads2d& h = H[k];
-    hbytes += h.hdr.size() + h.h_eb.size() + h.hu1.size() + h.hu2.size() + h.hv1.size() + h.hv2.size() + 6 * 16;
-    npieces += pieces_of(h.hdr.size()) + pieces_of(h.h_eb.size()) + pieces_of(h.hu1.size()) +
-               pieces_of(h.hu2.size()) + pieces_of(h.hv1.size()) + pieces_of(h.hv2.size()) + pieces_of(bytes_eb) +
-               pieces_of(bytes_u) + pieces_of(bytes_v) + pieces_of(4 * t5[3]) + pieces_of(4 * t5[4]);
-    const size_t total = h.hdr.size() + h.eb_size + h.u_size + h.v_size;
-    CPSZ_REQUIRE(total <= cap[k], kErrInternal, "batch stream bound exceeded");
-    size[k] = total;
-  }
-  uint32_t* words = ctx.buf<uint32_t>("b2e.words", wofs[nf] + 1);
-  float* unp = ctx.buf<float>("b2e.unp", uofs[nf] + 1);
-  k_fill_u32<<<1024, 256, 0, st>>>(words, wofs[nf], 0);
-  Arena C = arena("b2e.tabC", hbytes + npieces * sizeof(CopyPiece) + nf * sizeof(EncJob<SrcIds2d>) +
-                                  2 * nf * (sizeof(EncJob<SrcCodes2d>) + sizeof(UnpJob)) + 8 * 64);
-  EncJob<SrcIds2d>* ci;
-  EncJob<SrcIds2d>* d_ci = C.take(nf, &ci);
-  EncJob<SrcCodes2d>* cc;
-  EncJob<SrcCodes2d>* d_cc = C.take(2 * nf, &cc);
-  UnpJob* gj;
-  UnpJob* d_gj = C.take(2 * nf, &gj);
-  CopyPiece* cp;
-  CopyPiece* d_cp = C.take(npieces, &cp);
-  size_t np = 0;
-  auto piece = [&](size_t dst, const uint8_t* src, size_t len) {
-    for (size_t o = 0; o < len; o += kPiece) {
-      CPSZ_REQUIRE(np < npieces, kErrInternal, "batched assembly: piece table overflow");
-      cp[np++] = CopyPiece{dst + o, src + o, (uint32_t)std::min(kPiece, len - o), 0};
-    }
-  };
-  for (size_t k = 0; k < nf; k++) {
-    const uint64_t* t5 = h_tot + 5 * k;
-    const size_t bytes_eb = div_up(t5[0], 8), bytes_u = div_up(t5[1], 8), bytes_v = div_up(t5[2], 8);
-    const size_t w_eb = div_up(bytes_eb, 4) + 2, w_u = div_up(bytes_u, 4) + 2;
-    uint32_t* wk = words + wofs[k];
-    ci[k] = ej_i[k];
-    ci[k].words = wk;
-    cc[2 * k] = ej_c[2 * k];
-    cc[2 * k].words = wk + w_eb;
-    cc[2 * k + 1] = ej_c[2 * k + 1];
-    cc[2 * k + 1].words = wk + w_eb + w_u;
-    float* uk = unp + uofs[k];
-    gj[2 * k] = uj[2 * k];
-    gj[2 * k].out = uk;
-    gj[2 * k + 1] = uj[2 * k + 1];
-    gj[2 * k + 1].out = uk + t5[3];
-    // stream: hdr, eb tree head, eb bits, u head, u unpredictable, u tree
-    // head, u bits, then the same for v (the single-field layout)
-    const Heads2d& h = H[k];
-    size_t pos = off[k];
-    auto host_piece = [&](const std::vector<uint8_t>& v) {
-      uint8_t* hp;
-      const uint8_t* dp = C.take(v.size(), &hp);
-      std::memcpy(hp, v.data(), v.size());
-      piece(pos, dp, v.size());
-      pos += v.size();
-    };
-    auto dev_piece = [&](const void* src, size_t len) {
-      piece(pos, (const uint8_t*)src, len);
-      pos += len;
-    };
-    host_piece(h.hdr);
-    host_piece(h.h_eb);
-    dev_piece(wk, bytes_eb);
-    host_piece(h.hu1);
-    dev_piece(uk, 4 * t5[3]);
-    host_piece(h.hu2);
-    dev_piece(wk + w_eb, bytes_u);
-    host_piece(h.hv1);
-    dev_piece(uk + t5[3], 4 * t5[4]);
-    host_piece(h.hv2);
-    dev_piece(wk + w_eb + w_u, bytes_v);
-    CPSZ_REQUIRE(pos - off[k] == size[k], kErrInternal, "batched stream layout mismatch");
-  }
-  send(C);
-  k_encode_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ci, (int)nf);
-  k_encode_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_cc, (int)(2 * nf));
+  // ---- layout, encode, gather, headers, assembly
+  k_layout2d<<<1, 1, 0, st>>>(d_lj, (int)nf, words, words_cap, unp, unp_cap, d_sizes, d_words_used, d_err);
+  k_fill_words<<<2048, 256, 0, st>>>(words, d_words_used);
+  k_encode_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ej_i, (int)nf);
+  k_encode_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_ej_c, (int)(2 * nf));
   k_gather_unpred_multi<<<ublk, 256, 0, st>>>(d_gj, (int)(2 * nf));
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
-  if (np) k_assemble<<<(unsigned)np, 256, 0, st>>>(d_cp, dout);
+  k_write_heads2d<<<(unsigned)nf, 256, 0, st>>>(d_lj, dout);
+  k_assemble2d<<<dim3(64, (unsigned)nf), 256, 0, st>>>(d_lj, dout);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("assemble");
-  // the pinned tables are reused by the next call only after this point
+  // ---- sizes and error flags back to the host
+  const size_t mb = ntree * 32 + nf * 8;
+  uint8_t* hm = (uint8_t*)ctx.host_buf("b2e.hmeta", mb + 64);
+  CPSZ_HIP(hipMemcpyAsync(hm, d_info, mb, hipMemcpyDeviceToHost, st));
+  CPSZ_HIP(hipMemcpyAsync(hm + mb, d_err, 4, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipStreamSynchronize(st));
+  const uint64_t* hinfo = (const uint64_t*)hm;
+  const uint64_t* hsizes = hinfo + 4 * ntree;
+  for (size_t t = 0; t < ntree; t++) {
+    const uint64_t e = hinfo[4 * t + 3];
+    CPSZ_REQUIRE(e != 3, kErrUnsupported, "symbol beyond the coder's state count");
+    CPSZ_REQUIRE(e == 0, kErrInternal, "device huffman tree failed (code " + std::to_string(e) + ")");
+  }
+  CPSZ_REQUIRE(*(const int*)(hm + mb) == 0, kErrInternal, "batched entropy: layout pools overflowed");
+  for (size_t k = 0; k < nf; k++) {
+    CPSZ_REQUIRE(hsizes[k] <= cap[k], kErrInternal, "batch stream bound exceeded");
+    size[k] = hsizes[k];
+  }
 }
 
 void eb_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2, double max_pwr_eb,

@@ -62,9 +62,11 @@ void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap);
 void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out);
 // Every field's entropy stage at once (plain offline float fields): streams
 // written to dout + off[k] (at most cap[k] bytes each), sizes into size[k].
-// Host trees are built on up to `workers` pool threads.
+// Huffman trees are built on the GPU (device_trees: one wave per tree, no
+// host round trip) or on up to `workers` host pool threads; the streams are
+// identical either way.
 void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, const std::vector<size_t>& off,
-                      const std::vector<size_t>& cap, std::vector<size_t>& size, int workers);
+                      const std::vector<size_t>& cap, std::vector<size_t>& size, int workers, bool device_trees);
 
 struct D2Work {
   size_t r1 = 0, r2 = 0;

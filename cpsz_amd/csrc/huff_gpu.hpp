@@ -286,6 +286,7 @@ __global__ void __launch_bounds__(256) k_encode_multi(const EncJob<Src>* __restr
   __shared__ uint32_t stage[kEncWaves][kStage];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const EncJob<Src>& J = jobs[job_of(jobs, nj, b)];
+  if (!J.words) return;  // a stream without an output array is skipped (whole block)
   const int wv = threadIdx.x >> 6;
   const size_t c = (size_t)(b - J.blk0) * kEncWaves + wv;
   const bool live = c < J.nchunks;
@@ -298,8 +299,10 @@ __global__ void __launch_bounds__(256) k_encode_multi(const EncJob<Src>* __restr
 // present symbols: syms/freqs + *base hold *count (symbol, count) pairs in
 // increasing symbol order. Outputs: the serialized tree (13 * (2k - 1) + 1
 // bytes at most), the dense codebook code[sym] / len[sym], and info = {node
-// count, tree bytes, longest code, error (0 = none)}. heap (max_leaves + 2
-// words) and kids (2 * max_leaves) are scratch for trees too large for LDS.
+// count, tree bytes, longest code, error (0 = none)}. heap (max_leaves + 1
+// words) is scratch for trees too large for LDS (more than 8190 leaves).
+// Errors: 1 empty, 2 more leaves than max_leaves (or beyond 2^23), 3 a symbol
+// at or beyond state_num, 4 a code longer than 64 bits, 5 internal.
 struct TreeJob {
   const uint32_t* syms;
   const uint32_t* freqs;
@@ -307,7 +310,6 @@ struct TreeJob {
   const uint64_t* base;
   uint32_t state_num, max_leaves;
   uint64_t* heap;
-  int32_t* kids;
   uint8_t* tree;
   uint64_t* code;
   uint8_t* len;

@@ -270,9 +270,15 @@ int cpsz_decompress_2d_batch_dev(int device, void* stream, size_t nfields,
                                  const unsigned char* const* d_in, const size_t* in_size,
                                  const size_t* r1, const size_t* r2, float* const* dU,
                                  float* const* dV);
-/* Configuration the batch calls use for this batch signature. */
+/* Configuration the batch calls use for this batch signature: host worker
+   threads, persistent-grid workgroups and (device_trees may be null) whether
+   the compressor builds its Huffman trees on the GPU (1) or on the host (0). */
 int cpsz_batch_2d_config(int device, size_t nfields, const size_t* r1, const size_t* r2,
-                         int* workers, int* grid_cap);
+                         int* workers, int* grid_cap, int* device_trees);
+/* Pin that configuration for batches of these shapes on this device (for
+   callers that know their best settings, or want the trees on the GPU). */
+int cpsz_batch_2d_set_config(int device, size_t nfields, const size_t* r1, const size_t* r2,
+                             int workers, int grid_cap, int device_trees);
 
 /* ---- batched 3D codec ----------------------------------------------------
  * Many 3D fields (timesteps, ensemble members; sizes may differ) per call, no

@@ -47,8 +47,24 @@ def test_batch_streams_match_single_field(batch, tune):
         if k % 3 == 0:
             ref = orc.compress_2d(U, V, 0.1)
             assert np.array_equal(mask_2d_padding(s, *shape), mask_2d_padding(ref, *shape)), shape
-    w, g = cz.batch_2d_config(SHAPES)
+    w, g, _ = cz.batch_2d_config(SHAPES)
     assert w >= 1 and g >= 1
+
+
+@pytest.mark.parametrize("device_trees", [True, False])
+def test_batch_tree_placement_gives_identical_streams(batch, device_trees):
+    """Huffman trees built on the GPU (one wave per tree) or on the host pool:
+    the same bytes, equal to the single-field codec's."""
+    print(f"[batch] device_trees={device_trees}", flush=True)
+    fields, dev = batch
+    cz.batch_2d_set_config(SHAPES, 4, 1024, device_trees)
+    assert cz.batch_2d_config(SHAPES) == (4, 1024, device_trees)
+    base, off, size = cz.compress_2d_batch_dev([(u.data_ptr(), v.data_ptr(), *SHAPES[k]) for k, (u, v)
+                                                in enumerate(dev)], 0.1)
+    allb = _host(base, int(off[-1] + size[-1]))
+    for k, ((U, V), shape) in enumerate(zip(fields, SHAPES)):
+        single = cz.sz_compress_cp_preserve_2d_offline(U, V, 0.1)
+        assert np.array_equal(allb[off[k]:off[k] + size[k]], single), shape
 
 
 def test_batch_round_trip_matches_single_decoder(batch):
