@@ -319,6 +319,9 @@ struct Strip2d {
   int r1, r2, nstrips;
   int wide;   // ids above 81: double-precision step
   int shift;  // log2 of the eb base: 2 (offline, 4^id eps) or 1 (offline_log, 2^id eps)
+  // compress of a batch: the eb pass's flags, read in place of `wide`
+  // ([1]); a field with [0] set (non-finite input) is skipped as a whole
+  const int* flags;
 };
 
 // One item (component, 64-row strip) of a field's wavefront. The strip's last
@@ -501,7 +504,12 @@ __global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
       else hi = mid - 1;
     }
     const Strip2d a = b.f[lo];
-    if (a.wide) strip_item<COMPRESS, true, 2>(a, item - b.start[lo], lane);
+    bool wide = a.wide;
+    if (a.flags) {
+      if (a.flags[0]) continue;  // every item of the field skips: no strip waits on another
+      wide = a.flags[1] != 0;
+    }
+    if (wide) strip_item<COMPRESS, true, 2>(a, item - b.start[lo], lane);
     else if (a.shift == 1) strip_item<COMPRESS, false, 1>(a, item - b.start[lo], lane);
     else strip_item<COMPRESS, false, 2>(a, item - b.start[lo], lane);
   }
@@ -1296,7 +1304,7 @@ void heads_2d(const HuffEncodeTable& t_eb, const HuffEncodeTable& t_u, const Huf
 template <bool COMPRESS>
 void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
   hipStream_t st = ctx.stream();
-  if (f.size() == 1) {
+  if (f.size() == 1 && !f[0].flags) {
     Strip2d& sa = f[0];
     const int nz = 2 * sa.nstrips + 1;
     k_zero_i32<<<(int)div_up(nz, 256), 256, 0, st>>>(sa.prog, nz);
@@ -1325,20 +1333,18 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
     max_words = std::max(max_words, (size_t)2 * g[k].nstrips * g[k].r2 * 2);
   }
   const size_t tbytes = nf * sizeof(Strip2d), sbytes = (nf + 1) * sizeof(int);
-  uint8_t* h = (uint8_t*)ctx.host_buf("w2b.tab", tbytes + sbytes);
-  std::memcpy(h, g.data(), tbytes);
-  std::memcpy(h + tbytes, start.data(), sbytes);
   uint8_t* d = ctx.buf<uint8_t>("w2b.tab", tbytes + sbytes + 64);
   int* ticket = (int*)(d + tbytes + sbytes + 16 - ((tbytes + sbytes) & 15));
-  CPSZ_HIP(hipMemcpyAsync(d, h, tbytes + sbytes, hipMemcpyHostToDevice, st));
+  uint8_t* h = (uint8_t*)ctx.host_table("w2b.tab", tbytes + sbytes);
+  std::memcpy(h, g.data(), tbytes);
+  std::memcpy(h + tbytes, start.data(), sbytes);
+  ctx.upload_table("w2b.tab", d, h, tbytes + sbytes);
   k_zero_fields<<<dim3((unsigned)std::min<size_t>(div_up(max_words, 256 * 8), 64), (unsigned)nf), 256, 0, st>>>(
       (const Strip2d*)d, ticket);
   StripBatch b{(const Strip2d*)d, (const int*)(d + tbytes), (int)nf, start[nf], ticket};
   const int grid = std::min(start[nf], grid_cap);
   k_lorenzo2d_batch<COMPRESS><<<grid, 64, 0, st>>>(b);
   CPSZ_HIP(hipGetLastError());
-  // the host table is reused by the next batch only after this stream synchronizes
-  CPSZ_HIP(hipStreamSynchronize(st));
 }
 
 size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -1389,8 +1395,8 @@ void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
 }
 
 // Bounds of every field (plain offline, float) in one launch: the fields'
-// histograms and flags move to two contiguous arrays, cleared at once; then
-// the flags of all fields come back in one copy (as c2_check).
+// histograms and flags move to two contiguous arrays, cleared at once. The
+// flags stay on the device (the wavefront reads them there).
 void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const std::vector<EbOverride>* ov) {
   hipStream_t st = ctx.stream();
   constexpr size_t HN = kMaxEbCode + 2 * 65536;
@@ -1405,6 +1411,7 @@ void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const
                  "compress_2d: field too large");
     W.hist = hist + k * HN;
     W.flags = flags + 4 * k;
+    W.flags_on_device = true;
     Eb2dJob& J = jobs[k];
     J.U = W.dU;
     J.V = W.dV;
@@ -1421,22 +1428,15 @@ void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const
     CPSZ_REQUIRE(blk + tiles < (1ull << 31), kErrUnsupported, "2D batch: too many tiles in one call");
     blk += (uint32_t)tiles;
   }
-  Eb2dJob* hj = (Eb2dJob*)ctx.host_buf("b2eb.jobs", nf * sizeof(Eb2dJob));
-  std::memcpy(hj, jobs.data(), nf * sizeof(Eb2dJob));
   Eb2dJob* dj = ctx.buf<Eb2dJob>("b2eb.jobs", nf);
-  CPSZ_HIP(hipMemcpyAsync(dj, hj, nf * sizeof(Eb2dJob), hipMemcpyHostToDevice, st));
+  const double* q2d = ctx.d_q2d();
+  Eb2dJob* hj = (Eb2dJob*)ctx.host_table("b2eb.jobs", nf * sizeof(Eb2dJob));
+  std::memcpy(hj, jobs.data(), nf * sizeof(Eb2dJob));
+  ctx.upload_table("b2eb.jobs", dj, hj, nf * sizeof(Eb2dJob));
   k_fill_u32<<<1024, 256, 0, st>>>(hist, nf * HN, 0);
   k_fill_u32<<<(unsigned)div_up(4 * nf, 256), 256, 0, st>>>((uint32_t*)flags, 4 * nf, 0);
-  k_eb2d_multi<<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size());
+  k_eb2d_multi<<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size());
   CPSZ_HIP(hipGetLastError());
-  int* hf = (int*)ctx.host_buf("b2eb.hflags", 16 * nf);
-  CPSZ_HIP(hipMemcpyAsync(hf, flags, 16 * nf, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
-  for (size_t k = 0; k < nf; k++) {
-    CPSZ_REQUIRE(hf[4 * k] == 0, kErrNonFinite,
-                 "compress_2d: non-finite input or bound (the reference cannot encode it)");
-    w[k]->wide = hf[4 * k + 1] != 0;
-  }
 }
 
 void c2_check(Ctx& ctx, C2Work* const* w, size_t nf) {
@@ -1502,7 +1502,8 @@ void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap) {
     sa.r1 = (int)w[k]->r1;
     sa.r2 = (int)w[k]->r2;
     sa.wide = w[k]->wide;
-    CPSZ_REQUIRE(!(lg && sa.wide), kErrInternal, "compress_2d_log: eb id beyond the base-2 range");
+    sa.flags = w[k]->flags_on_device ? w[k]->flags : nullptr;
+    CPSZ_REQUIRE(!(lg && (sa.wide || sa.flags)), kErrInternal, "compress_2d_log: eb id beyond the base-2 range");
   }
   launch_wavefront<true>(ctx, f, grid_cap);
 }
@@ -1731,12 +1732,14 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   uint32_t* words = ctx.buf<uint32_t>("b2e.words", words_cap);
   float* unp = ctx.buf<float>("b2e.unp", unp_cap);
   // ---- job tables, filled on the host and sent in one copy
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(w[k]->flags == w[0]->flags + 4 * k, kErrInternal, "batched entropy: eb flags not contiguous");
   Arena A;
   const size_t abytes = 2 * nf * (sizeof(HistJob<U16Hist>) + 2 * sizeof(UnpJob)) + ntree * sizeof(CompactJob) +
                         ntree * sizeof(TreeJob) + nf * sizeof(EncJob<SrcIds2d>) + 2 * nf * sizeof(EncJob<SrcCodes2d>) +
                         5 * nf * sizeof(ScanDiff) + nf * sizeof(Layout2d) + 16 * 64;
-  A.h = (uint8_t*)ctx.host_buf("b2e.tab", abytes);
   A.d = ctx.buf<uint8_t>("b2e.tab", abytes);
+  A.h = (uint8_t*)ctx.host_table("b2e.tab", abytes);
   A.cap = abytes;
   HistJob<U16Hist>* hj;
   HistJob<U16Hist>* d_hj = A.take(2 * nf, &hj);
@@ -1843,7 +1846,7 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     l.ej_codes = d_ej_c + 2 * k;
     l.gather = d_gj + 2 * k;
   }
-  CPSZ_HIP(hipMemcpyAsync(A.d, A.h, A.used, hipMemcpyHostToDevice, st));
+  ctx.upload_table("b2e.tab", A.d, A.h, A.used);
   // ---- histograms, zero counts, present symbols, trees
   const uint32_t win_lo = 32768 - 8192, win_n = 16384;
   k_hist_multi<U16Hist><<<hblk, 256, win_n * 4, st>>>(d_hj, (int)(2 * nf), win_lo, win_n);
@@ -1859,12 +1862,17 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   } else {
     // present symbols to the host, trees on the worker pool, results moved to
     // where the device builder would have put them
+    CPSZ_REQUIRE(!ctx.capturing, kErrInternal, "batched entropy: host trees cannot be captured");
     const size_t cm = 12 * ntree;
-    uint8_t* hc = (uint8_t*)ctx.host_buf("b2e.hcnt", cm + 64);
+    uint8_t* hc = (uint8_t*)ctx.host_buf("b2e.hcnt", cm + 16 * nf + 64);
     CPSZ_HIP(hipMemcpyAsync(hc, d_total, 8, hipMemcpyDeviceToHost, st));
     CPSZ_HIP(hipMemcpyAsync(hc + 64, d_base, 8 * ntree, hipMemcpyDeviceToHost, st));
     CPSZ_HIP(hipMemcpyAsync(hc + 64 + 8 * ntree, d_count, 4 * ntree, hipMemcpyDeviceToHost, st));
+    CPSZ_HIP(hipMemcpyAsync(hc + 64 + cm, w[0]->flags, 16 * nf, hipMemcpyDeviceToHost, st));
     CPSZ_HIP(hipStreamSynchronize(st));
+    for (size_t k = 0; k < nf; k++)  // a skipped field's codes are not the codec's: report it before its trees
+      CPSZ_REQUIRE(((const int*)(hc + 64 + cm))[4 * k] == 0, kErrNonFinite,
+                   "compress_2d: non-finite input or bound (the reference cannot encode it)");
     const uint64_t present = *(const uint64_t*)hc;
     const uint64_t* h_base = (const uint64_t*)(hc + 64);
     const uint32_t* h_count = (const uint32_t*)(hc + 64 + 8 * ntree);
@@ -1889,8 +1897,8 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     for (size_t t = 0; t < ntree; t++) tbytes += h_count[t] * 13 + tree_bytes_of(h_count[t] ? h_count[t] : 1) + 64;
     Arena B;
     const size_t bbytes = tbytes + ntree * sizeof(HostTreeJob) + 64;
-    B.h = (uint8_t*)ctx.host_buf("b2e.tabB", bbytes);
     B.d = ctx.buf<uint8_t>("b2e.tabB", bbytes);
+    B.h = (uint8_t*)ctx.host_table("b2e.tabB", bbytes);
     B.cap = bbytes;
     HostTreeJob* hj2;
     HostTreeJob* d_hj2 = B.take(ntree, &hj2);
@@ -1931,7 +1939,7 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
         };
       pool_run(pool_for(ctx.device()), jobs, std::max(workers, 1));
     }
-    CPSZ_HIP(hipMemcpyAsync(B.d, B.h, B.used, hipMemcpyHostToDevice, st));
+    ctx.upload_table("b2e.tabB", B.d, B.h, B.used);
     k_host_trees_in<<<(unsigned)ntree, 256, 0, st>>>(d_hj2);
     CPSZ_HIP(hipGetLastError());
   }
@@ -1956,12 +1964,23 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   k_assemble2d<<<dim3(64, (unsigned)nf), 256, 0, st>>>(d_lj, dout);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("assemble");
-  // ---- sizes and error flags back to the host
+  // ---- tree infos, sizes and error flags back to the host
   const size_t mb = ntree * 32 + nf * 8;
-  uint8_t* hm = (uint8_t*)ctx.host_buf("b2e.hmeta", mb + 64);
+  uint8_t* hm = (uint8_t*)ctx.host_buf("b2e.hmeta", mb + 16 * nf + 64);
   CPSZ_HIP(hipMemcpyAsync(hm, d_info, mb, hipMemcpyDeviceToHost, st));
   CPSZ_HIP(hipMemcpyAsync(hm + mb, d_err, 4, hipMemcpyDeviceToHost, st));
-  CPSZ_HIP(hipStreamSynchronize(st));
+  CPSZ_HIP(hipMemcpyAsync(hm + mb + 64, w[0]->flags, 16 * nf, hipMemcpyDeviceToHost, st));
+  if (!ctx.capturing) c2_entropy_batch_finish(ctx, nf, cap, size);
+}
+
+void c2_entropy_batch_finish(Ctx& ctx, size_t nf, const std::vector<size_t>& cap, std::vector<size_t>& size) {
+  const size_t ntree = 3 * nf, mb = ntree * 32 + nf * 8;
+  const uint8_t* hm = (const uint8_t*)ctx.host_buf("b2e.hmeta", mb + 16 * nf + 64);
+  CPSZ_HIP(hipStreamSynchronize(ctx.stream()));
+  const int* hflags = (const int*)(hm + mb + 64);
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(hflags[4 * k] == 0, kErrNonFinite,
+                 "compress_2d: non-finite input or bound (the reference cannot encode it)");
   const uint64_t* hinfo = (const uint64_t*)hm;
   const uint64_t* hsizes = hinfo + 4 * ntree;
   for (size_t t = 0; t < ntree; t++) {

@@ -38,6 +38,9 @@ struct C2Work {
   uint64_t* bnd = nullptr;  // wavefront boundary rows (tagged values)
   int* prog = nullptr;      // wavefront item ticket
   bool wide = false;
+  // the flags stay on the device: the wavefront reads `wide` from flags[1]
+  // and skips a field with flags[0] set (the caller reports it afterwards)
+  bool flags_on_device = false;
   // offline_log: one eb code per vertex (cu) for both components, the codec
   // runs on the log values lu/lv, and smap holds the two sign bitmaps as
   // 64-bit words (ceil(n/64) per component)
@@ -55,8 +58,9 @@ void c2_bind(C2Work& w, uint8_t* scratch);  // carves c2_scratch_bytes of device
 void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov);
 void c2_check(Ctx& ctx, C2Work* const* w, size_t nf);  // synchronizes; sets wide
 // Every field's bounds in one launch (plain offline float fields; ov: one
-// override per field or null); re-points hist/flags to contiguous arrays,
-// synchronizes and sets wide (c2_eb + c2_check for the batch).
+// override per field or null); re-points hist/flags to contiguous arrays
+// (w[k]->flags = w[0]->flags + 4 k) and leaves the flags on the device, with
+// no synchronization: the batched entropy stage reports them.
 void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const std::vector<EbOverride>* ov);
 void c2_wavefront(Ctx& ctx, C2Work* const* w, size_t nf, int grid_cap);
 void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out);
@@ -65,8 +69,13 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out);
 // Huffman trees are built on the GPU (device_trees: one wave per tree, no
 // host round trip) or on up to `workers` host pool threads; the streams are
 // identical either way.
+// The eb flags of the batch (c2_eb_batch) are checked here too. While the
+// context captures a graph, the stage only enqueues its work (device trees
+// only); c2_entropy_batch_finish then waits for the stream and reports the
+// sizes and errors.
 void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, const std::vector<size_t>& off,
                       const std::vector<size_t>& cap, std::vector<size_t>& size, int workers, bool device_trees);
+void c2_entropy_batch_finish(Ctx& ctx, size_t nf, const std::vector<size_t>& cap, std::vector<size_t>& size);
 
 struct D2Work {
   size_t r1 = 0, r2 = 0;

@@ -129,8 +129,31 @@ Ctx::~Ctx() {
   if (d_q2l_) hipFree(d_q2l_);
   for (void* p : libm_allocs_) hipFree(p);
   for (auto e : event_pool_) hipEventDestroy(e);
+  for (auto& kv : table_sent_) hipEventDestroy(kv.second);
   if (point_) hipEventDestroy(point_);
   if (own_stream_) hipStreamDestroy(own_stream_);
+}
+
+void* Ctx::host_table(const char* name, size_t bytes) {
+  auto it = table_sent_.find(name);
+  // (a capture copies the table's bytes instead of sending them, and the
+  // stream was idle when it began)
+  if (it != table_sent_.end() && !capturing) CPSZ_HIP(hipEventSynchronize(it->second));
+  return host_buf(name, bytes);
+}
+
+void Ctx::upload_table(const char* name, void* dst, const void* src, size_t bytes) {
+  if (!bytes) return;
+  if (capturing) {
+    const size_t off = upload_bytes.size();
+    upload_bytes.insert(upload_bytes.end(), (const uint8_t*)src, (const uint8_t*)src + bytes);
+    uploads.push_back(Upload{dst, off, bytes});
+    return;
+  }
+  CPSZ_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream_));
+  hipEvent_t& e = table_sent_[name];
+  if (!e) CPSZ_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  CPSZ_HIP(hipEventRecord(e, stream_));
 }
 
 hipEvent_t Ctx::record_point() {
@@ -148,6 +171,7 @@ void* Ctx::raw_buf(const char* name, size_t bytes) {
       CPSZ_HIP(hipFree(b.p));
       b.p = nullptr;
       b.cap = 0;
+      generation_++;
     }
     size_t cap = bytes + bytes / 8 + 4096;
     if (hipMalloc(&b.p, cap) != hipSuccess) {
@@ -168,6 +192,7 @@ void* Ctx::host_buf(const char* name, size_t bytes) {
       CPSZ_HIP(hipHostFree(b.p));
       b.p = nullptr;
       b.cap = 0;
+      generation_++;
     }
     size_t cap = bytes + bytes / 8 + 4096;
     if (hipHostMalloc(&b.p, cap, hipHostMallocDefault) != hipSuccess) {
@@ -235,7 +260,7 @@ const Ctx::LibmDev& Ctx::libm_dev() {
 }
 
 void Ctx::mark(const char* name) {
-  if (!timing) return;
+  if (!timing || capturing) return;
   hipEvent_t e;
   if (!event_pool_.empty()) {
     e = event_pool_.back();

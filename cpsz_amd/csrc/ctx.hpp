@@ -75,6 +75,25 @@ class Ctx {
   // later work queued on the stream overlap the host's own work.
   hipEvent_t record_point();
 
+  // Host-side job tables sent to the device. host_table() returns the named
+  // pinned staging buffer once the previous upload from it has left (so it
+  // can be refilled without a stream synchronization); upload_table() sends
+  // it. While a graph is being captured (capturing), uploads are not put on
+  // the stream but kept in `uploads` with a copy of their bytes, to be
+  // replayed before each launch of the graph.
+  void* host_table(const char* name, size_t bytes);
+  void upload_table(const char* name, void* dst, const void* src, size_t bytes);
+  struct Upload {
+    void* dst;
+    size_t off, bytes;  // into upload_bytes
+  };
+  bool capturing = false;
+  std::vector<Upload> uploads;
+  std::vector<uint8_t> upload_bytes;
+  // Incremented whenever a named buffer moves (is freed to grow): device
+  // addresses baked into a captured graph stay valid while it is unchanged.
+  uint64_t generation() const { return generation_; }
+
   // Stage timings of the last call (ms), filled when timing is enabled.
   bool timing = false;
   std::vector<std::pair<std::string, float>> stages;
@@ -96,6 +115,8 @@ class Ctx {
   std::vector<std::pair<std::string, hipEvent_t>> marks_;
   std::vector<hipEvent_t> event_pool_;
   hipEvent_t point_ = nullptr;
+  std::map<std::string, hipEvent_t> table_sent_;
+  uint64_t generation_ = 0;
 };
 
 // Process-wide context for a device (created on first use).

@@ -67,6 +67,67 @@ def test_batch_tree_placement_gives_identical_streams(batch, device_trees):
         assert np.array_equal(allb[off[k]:off[k] + size[k]], single), shape
 
 
+def _stage_names():
+    return [k for k, _ in cz.last_stages(0)]
+
+
+def test_batch_graph_replays_match_eager(batch):
+    """Trees on the GPU: the first call runs eagerly and captures the batch;
+    later calls with the same inputs replay the graph. Every replay (with
+    other eager work on the context in between, and after the inputs change
+    in place) gives the single-field codec's streams for the current data."""
+    fields, dev = batch
+    own = [(u.clone(), v.clone()) for u, v in dev]
+    torch.cuda.synchronize()
+    fl = [(u.data_ptr(), v.data_ptr(), *SHAPES[k]) for k, (u, v) in enumerate(own)]
+    cz.batch_2d_set_config(SHAPES, 4, 1024, True)
+    cz.set_timing(True, 0)
+    host = [(U.copy(), V.copy()) for U, V in fields]
+    try:
+        for call in range(5):
+            print(f"[batch graph] call {call}", flush=True)
+            if call == 3:  # new data in the same buffers
+                U, V = field_2d(*SHAPES[6], 77)
+                own[6][0].copy_(torch.from_numpy(U))
+                own[6][1].copy_(torch.from_numpy(V))
+                host[6] = (U, V)
+                torch.cuda.synchronize()
+            base, off, size = cz.compress_2d_batch_dev(fl, 0.1)
+            assert ("graph" in _stage_names()) == (call > 0), _stage_names()
+            allb = _host(base, int(off[-1] + size[-1]))
+            for k, ((U, V), shape) in enumerate(zip(host, SHAPES)):
+                single = cz.sz_compress_cp_preserve_2d_offline(U, V, 0.1)
+                assert np.array_equal(allb[off[k]:off[k] + size[k]], single), (call, shape)
+            # other eager work on the same context between the replays
+            s = cz.sz_compress_cp_preserve_2d_offline(*field_2d(90, 110, call), 0.1)
+            cz.sz_decompress_cp_preserve_2d_offline(s, 90, 110)
+    finally:
+        cz.set_timing(False, 0)
+        cz.batch_2d_set_config(SHAPES, 4, 1024, False)
+
+
+@pytest.mark.parametrize("device_trees", [False, True])
+def test_batch_nonfinite_field_reported(batch, device_trees):
+    """A field with a NaN makes the batched compress fail with the non-finite
+    error (the wavefront skips that field); the next batch is unaffected."""
+    fields, dev = batch
+    bad = [(u.clone(), v.clone()) for u, v in dev]
+    bad[4][0][7, 9] = float("nan")
+    torch.cuda.synchronize()
+    cz.batch_2d_set_config(SHAPES, 4, 1024, device_trees)
+    try:
+        with pytest.raises(cz.CpszError) as e:
+            cz.compress_2d_batch_dev([(u.data_ptr(), v.data_ptr(), *SHAPES[k]) for k, (u, v) in enumerate(bad)], 0.1)
+        assert e.value.code == -6
+        base, off, size = cz.compress_2d_batch_dev([(u.data_ptr(), v.data_ptr(), *SHAPES[k]) for k, (u, v)
+                                                    in enumerate(dev)], 0.1)
+        allb = _host(base, int(off[-1] + size[-1]))
+        U, V = fields[4]
+        assert np.array_equal(allb[off[4]:off[4] + size[4]], cz.sz_compress_cp_preserve_2d_offline(U, V, 0.1))
+    finally:
+        cz.batch_2d_set_config(SHAPES, 4, 1024, False)
+
+
 def test_batch_round_trip_matches_single_decoder(batch):
     fields, dev = batch
     base, off, size = cz.compress_2d_batch_dev([(u.data_ptr(), v.data_ptr(), *SHAPES[k]) for k, (u, v)
