@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--workload", choices=["batch2d", "3d", "2d", "learned2d", "learned3d"], default="batch2d")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--no-learned", action="store_true", help="batch2d: skip the learned method")
+    p.add_argument("--no-graph", action="store_true", help="batch2d: skip the tree placement / graph probe")
     p.add_argument("--no-cp", action="store_true", help="skip the critical-point check")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="only run a plain 256 MiB device copy (the rocprofv3 FETCH_SIZE / WRITE_SIZE "
@@ -222,6 +223,37 @@ def cpu_baseline_batch2d(fields_host, shapes, eb):
                                         f"median of 3 wall times"}}
 
 
+def graph_probe(cz, torch, shapes, fl, eb, sh):
+    """Compress-only timings (ms, median of 5, outside the timed region) of
+    the two tree placements: host trees run eagerly, GPU trees replay a
+    captured HIP graph. On the benchmark batch and on 256 small fields
+    (h, w log-uniform in [32, 256]), where launch and host round trips weigh
+    more than the trees."""
+    from cpsz_amd.synth import batch_2d_shapes, field_2d_device
+    small = batch_2d_shapes(256, seed=3, lo=32, hi=256)
+    sdev = [field_2d_device(r1, r2, seed=2000 + k) for k, (r1, r2) in enumerate(small)]
+    sfl = [(u.data_ptr(), v.data_ptr(), r1, r2) for (u, v), (r1, r2) in zip(sdev, small)]
+    torch.cuda.synchronize()
+    out = {}
+    for name, shp, flist in (("batch_96_fields", shapes, fl), ("small_256_fields", small, sfl)):
+        w, g, dt = cz.batch_2d_config(shp)
+        res = {}
+        for trees in ("host", "device"):
+            cz.batch_2d_set_config(shp, w, g, trees == "device")
+            ts = []
+            for r in range(7):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                cz.compress_2d_batch_dev(flist, eb, stream=sh)
+                torch.cuda.synchronize()
+                ts.append((time.perf_counter() - t0) * 1e3)
+            stages = [k for k, _ in cz.last_stages(0)]
+            res[trees] = {"ms": float(np.median(ts[2:])), "replayed_graph": stages == ["graph"]}
+        cz.batch_2d_set_config(shp, w, g, dt)
+        out[name] = res
+    return out
+
+
 def run_batch2d(a, torch):
     """96 fields, h and w log-uniform in [256, 4096] (seeded), rel eb 0.1. A
     step is one batched compress + one batched decompress of all fields. The
@@ -299,6 +331,7 @@ def run_batch2d(a, torch):
     base, off, size = acc[-1][:3]
     zb_plain, sb_plain = zbytes(base, off, size)
     cp_plain = None if a.no_cp else cp_errors()
+    graph = None if a.no_graph else graph_probe(cz, torch, shapes, fl, eb, sh)
     # ---- learned: trained on 64 seeded 1024x1024 fields (bisection labels), then
     # learned batched compress (proposals, verification rounds, fallback) +
     # batched decompress of the same 96 fields
@@ -358,6 +391,7 @@ def run_batch2d(a, torch):
                    "vertices": nv, "field_bytes": field_bytes, "stream_bytes": sb_plain, "zstd_bytes": zb_plain,
                    "ratio": field_bytes / zb_plain, "cp_errors": cp_plain, "wall_s": t_wall,
                    "autotuned": {"workers": w, "grid_cap": g, "huffman_trees": "device" if dt else "host"},
+                   "compress_ms_by_tree_placement": graph,
                    "stages_compress_ms": stages_c, "stages_decompress_ms": stages_d,
                    "learned": learned,
                    "learned_vs_plain": None if learned is None else {

@@ -322,6 +322,9 @@ struct Strip2d {
   // compress of a batch: the eb pass's flags, read in place of `wide`
   // ([1]); a field with [0] set (non-finite input) is skipped as a whole
   const int* flags;
+  // batch: issue priority (0-3) of the field's waves on their SIMD, higher
+  // for longer wavefronts
+  int prio;
 };
 
 // One item (component, 64-row strip) of a field's wavefront. The strip's last
@@ -340,11 +343,113 @@ __device__ __forceinline__ void st_coh64(uint64_t* p, uint64_t v) {
 }
 
 struct StripChunk {
-  float xs[kCH];
-  uint32_t cs[kCH];
-  uint32_t qs[kCH];
-  float raws[kCH];
+  float xs[kCH];         // compress: field values
+  uint32_t cs[kCH / 2];  // eb codes (compress) / stream ids (decompress), two u16 per word
+  uint32_t qs[kCH / 2];  // decompress: quantization codes, two u16 per word
+  float raws[kCH];       // decompress: unpredictable value where the code is 0
 };
+
+__device__ __forceinline__ uint32_t lo16(uint32_t w, int k) { return (k & 1) ? w >> 16 : w & 0xFFFFu; }
+
+// Interior chunks move a lane's kCH consecutive elements of its row with
+// 16-byte loads from the 16-byte-aligned block holding the first element (a
+// 16-byte-aligned load never leaves the pages of the row it starts in), then
+// shift them into place with selects. The rows of a strip are 64 different
+// lines per instruction, so wide loads cut the lines the L1 and L2 serve per
+// element by 4 (8 for the u16 arrays) against element-wise loads.
+__device__ __forceinline__ uint32_t sel4(uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t q) {
+  const uint32_t lo = (q & 1) ? b : a, hi = (q & 1) ? d : c;
+  return (q & 2) ? hi : lo;
+}
+
+// kCH floats from p (4-byte aligned) into v
+__device__ __forceinline__ void ld_row_f32(const float* p, float* v) {
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t s = (uint32_t)(a >> 2) & 3;
+  const uint4* w = (const uint4*)(a & ~(uintptr_t)15);
+  uint32_t d[kCH + 4];
+#pragma unroll
+  for (int i = 0; i < kCH / 4; i++) {
+    const uint4 x = w[i];
+    d[4 * i] = x.x;
+    d[4 * i + 1] = x.y;
+    d[4 * i + 2] = x.z;
+    d[4 * i + 3] = x.w;
+  }
+  uint4 t = make_uint4(0, 0, 0, 0);
+  if (s) t = w[kCH / 4];
+  d[kCH] = t.x;
+  d[kCH + 1] = t.y;
+  d[kCH + 2] = t.z;
+  d[kCH + 3] = t.w;
+#pragma unroll
+  for (int k = 0; k < kCH; k++) v[k] = __uint_as_float(sel4(d[k], d[k + 1], d[k + 2], d[k + 3], s));
+}
+
+// kCH u16 from p (2-byte aligned) into kCH/2 words, element k in word k/2
+// (low half for even k)
+__device__ __forceinline__ void ld_row_u16(const uint16_t* p, uint32_t* e) {
+  const uintptr_t a = (uintptr_t)p;
+  const uint32_t s = (uint32_t)(a >> 1) & 7;  // u16 offset in the 16-byte block
+  const uint4* w = (const uint4*)(a & ~(uintptr_t)15);
+  uint32_t d[kCH / 2 + 4];
+#pragma unroll
+  for (int i = 0; i < kCH / 8; i++) {
+    const uint4 x = w[i];
+    d[4 * i] = x.x;
+    d[4 * i + 1] = x.y;
+    d[4 * i + 2] = x.z;
+    d[4 * i + 3] = x.w;
+  }
+  uint4 t = make_uint4(0, 0, 0, 0);
+  if (s) t = w[kCH / 8];
+  d[kCH / 2] = t.x;
+  d[kCH / 2 + 1] = t.y;
+  d[kCH / 2 + 2] = t.z;
+  d[kCH / 2 + 3] = t.w;
+  const uint32_t q = s >> 1, b = (s & 1) * 2;
+  uint32_t g[kCH / 2 + 1];
+#pragma unroll
+  for (int m = 0; m <= kCH / 2; m++)
+    g[m] = m + 3 < kCH / 2 + 4 ? sel4(d[m], d[m + 1], d[m + 2], d[m + 3], q) : d[m];
+#pragma unroll
+  for (int m = 0; m < kCH / 2; m++) e[m] = __builtin_amdgcn_alignbyte(g[m + 1], g[m], b);
+}
+
+// kCH u16 (two per word, as ld_row_u16) to p (2-byte aligned): 4-byte
+// stores, with a 2-byte store at each end when p is not 4-byte aligned
+__device__ __forceinline__ void st_row_u16(uint16_t* p, const uint32_t* e) {
+  const uintptr_t a = (uintptr_t)p;
+  const bool odd = (a & 2) != 0;
+  uint32_t* w = (uint32_t*)((a + 2) & ~(uintptr_t)3);  // first 4-byte boundary at or after p
+  if (odd) {
+    p[0] = (uint16_t)e[0];
+    p[kCH - 1] = (uint16_t)(e[kCH / 2 - 1] >> 16);
+  }
+#pragma unroll
+  for (int m = 0; m < kCH / 2; m++) {
+    const uint32_t v = odd ? (m + 1 < kCH / 2 ? __builtin_amdgcn_alignbyte(e[m + 1], e[m], 2) : 0u) : e[m];
+    if (!odd || m + 1 < kCH / 2) w[m] = v;
+  }
+}
+
+// kCH floats to p (4-byte aligned): 8-byte stores, with a 4-byte store at
+// each end when p is not 8-byte aligned
+__device__ __forceinline__ void st_row_f32(float* p, const float* v) {
+  const uintptr_t a = (uintptr_t)p;
+  const bool odd = (a & 4) != 0;
+  float2* w = (float2*)((a + 4) & ~(uintptr_t)7);
+  if (odd) {
+    p[0] = v[0];
+    p[kCH - 1] = v[kCH - 1];
+  }
+#pragma unroll
+  for (int m = 0; m < kCH / 2; m++) {
+    const float x = odd ? v[2 * m + 1] : v[2 * m];
+    const float y = odd ? (2 * m + 2 < kCH ? v[2 * m + 2] : 0.f) : v[2 * m + 1];
+    if (!odd || m + 1 < kCH / 2) w[m] = make_float2(x, y);
+  }
+}
 
 // SHIFT (a.shift as a compile-time constant) keeps the quantum's exponent a
 // shift of the id: a runtime factor costs an integer multiply per element.
@@ -355,11 +460,13 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
   const int row = s * 64 + lane;
   const bool rvalid = row < r1;
   const size_t roff = (size_t)(rvalid ? row : 0) * r2;
-  const float* X = COMPRESS ? a.x[comp] + roff : nullptr;
-  const uint16_t* C = a.c[comp] + roff;
-  uint16_t* Q = a.q[comp] + roff;
-  const float* RAW = COMPRESS ? nullptr : a.raw[comp] + roff;
-  float* OUT = COMPRESS ? nullptr : a.out[comp] + roff;
+  // (selects rather than a.x[comp]: an indexed member would put a copied
+  // descriptor in scratch memory)
+  const float* X = COMPRESS ? (comp ? a.x[1] : a.x[0]) + roff : nullptr;
+  const uint16_t* C = (comp ? a.c[1] : a.c[0]) + roff;
+  uint16_t* Q = (comp ? a.q[1] : a.q[0]) + roff;
+  const float* RAW = COMPRESS ? nullptr : (comp ? a.raw[1] : a.raw[0]) + roff;
+  float* OUT = COMPRESS ? nullptr : (comp ? a.out[1] : a.out[0]) + roff;
   const bool has_up = s > 0;
   const bool publish = s + 1 < a.nstrips;
   const uint64_t* bnd_in = a.bnd + ((size_t)comp * a.nstrips + (s > 0 ? s - 1 : 0)) * r2;
@@ -368,21 +475,31 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
   const int steps = r2 + 63;
   // An interior chunk has every lane's kCH columns inside the row and all 64
   // rows present, so none of its steps needs a bounds mask; most chunks are
-  // interior and run the unmasked body.
+  // interior and run the unmasked body, with wide loads and buffered stores.
   const bool full_rows = s * 64 + 64 <= r1;
   auto interior = [&](int t0) { return full_rows && t0 >= 63 && t0 + kCH <= r2; };
   auto load = [&](StripChunk& B, int t0, auto inner) {
     constexpr bool IN = decltype(inner)::value;
+    if constexpr (IN) {
+      const int j0 = t0 - lane;
+      ld_row_u16(C + j0, B.cs);
+      if (COMPRESS) ld_row_f32(X + j0, B.xs);
+      else ld_row_u16(Q + j0, B.qs);
+      return;
+    }
 #pragma unroll
-    for (int k = 0; k < kCH; k++) {
-      const int j = t0 + k - lane;
-      const bool act = IN || (rvalid && j >= 0 && j < r2);
-      B.cs[k] = act ? C[j] : 0;
-      if (COMPRESS) {
-        B.xs[k] = act ? X[j] : 0.f;
-      } else {
-        B.qs[k] = act ? Q[j] : 0;
+    for (int k = 0; k < kCH; k += 2) {
+      uint32_t c2[2], q2[2];
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int j = t0 + k + h - lane;
+        const bool act = rvalid && j >= 0 && j < r2;
+        c2[h] = act ? C[j] : 0u;
+        if (COMPRESS) B.xs[k + h] = act ? X[j] : 0.f;
+        else q2[h] = act ? Q[j] : 0u;
       }
+      B.cs[k >> 1] = c2[0] | c2[1] << 16;
+      if (!COMPRESS) B.qs[k >> 1] = q2[0] | q2[1] << 16;
     }
   };
   // decompress: the unpredictable values are read only where the code is 0
@@ -393,7 +510,7 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
     for (int k = 0; k < kCH; k++) {
       const int j = t0 + k - lane;
       const bool act = IN || (rvalid && j >= 0 && j < r2);
-      B.raws[k] = (act && B.qs[k] == 0) ? RAW[j] : 0.f;
+      B.raws[k] = (act && lo16(B.qs[k >> 1], k) == 0) ? RAW[j] : 0.f;
     }
   };
   auto chunk = [&](const StripChunk& B, StripChunk& N, int t0, auto inner) {
@@ -413,6 +530,13 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       if (next_in) load(N, t0 + kCH, std::true_type{});
       else load(N, t0 + kCH, std::false_type{});
     }
+    uint32_t qo[IN && COMPRESS ? kCH / 2 : 1];
+    float oo[IN && !COMPRESS ? kCH : 1];
+    // the boundary row's chunk (lane 63's values) gathers in lane k and
+    // leaves in one store after the chunk: the next strip waits for whole
+    // chunks anyway (one step later than per-step stores, one store in place
+    // of kCH)
+    uint32_t pubv = 0;
 #pragma unroll
     for (int k = 0; k < kCH; k++) {
       if (!COMPRESS && k == kCH / 2 && more) {
@@ -425,18 +549,36 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
       const float lane0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(inc), k));
       const float Uv = shr1(Rprev, lane0);
       const float pred = (Rprev + Uv) - Uprev;
+      const uint32_t ck = lo16(B.cs[k >> 1], k);
       float R;
       if (COMPRESS) {
         uint16_t code;
-        R = l2d_compress_step<WIDE>(pred, B.xs[k], B.cs[k], code, SHIFT);
-        if (act) Q[j] = code;
+        R = l2d_compress_step<WIDE>(pred, B.xs[k], ck, code, SHIFT);
+        if constexpr (IN) {
+          if (k & 1) qo[k >> 1] |= (uint32_t)code << 16;
+          else qo[k >> 1] = code;
+        } else if (act) {
+          Q[j] = code;
+        }
       } else {
-        R = l2d_decompress_step<WIDE>(pred, B.qs[k], B.cs[k], B.raws[k], SHIFT);
-        if (act) OUT[j] = R;
+        R = l2d_decompress_step<WIDE>(pred, lo16(B.qs[k >> 1], k), ck, B.raws[k], SHIFT);
+        if constexpr (IN) oo[k] = R;
+        else if (act) OUT[j] = R;
       }
-      if (publish && lane == 63 && act) st_coh64(bnd_out + j, kBndTag | __float_as_uint(R));
+      if (publish) {
+        const uint32_t r63 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(R), 63);
+        pubv = lane == k ? r63 : pubv;
+      }
       Uprev = Uv;
       Rprev = (!IN && j < 0) ? 0.f : R;
+    }
+    if (publish) {
+      const int jp = t0 - 63 + lane;
+      if (lane < kCH && jp >= 0 && jp < r2) st_coh64(bnd_out + jp, kBndTag | pubv);
+    }
+    if constexpr (IN) {
+      if constexpr (COMPRESS) st_row_u16(Q + (t0 - lane), qo);
+      else st_row_f32(OUT + (t0 - lane), oo);
     }
   };
   auto run = [&](const StripChunk& B, StripChunk& N, int t0) {
@@ -504,6 +646,16 @@ __global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
       else hi = mid - 1;
     }
     const Strip2d a = b.f[lo];
+    // The batch ends when its longest wavefront does, and the waves of a
+    // SIMD share its issue slots: the longer a field's dependency chain, the
+    // higher its waves' priority, so the critical chains step at nearly their
+    // solo speed while the short fields fill the remaining slots.
+    switch (a.prio) {
+      case 3: __builtin_amdgcn_s_setprio(3); break;
+      case 2: __builtin_amdgcn_s_setprio(2); break;
+      case 1: __builtin_amdgcn_s_setprio(1); break;
+      default: __builtin_amdgcn_s_setprio(0); break;
+    }
     bool wide = a.wide;
     if (a.flags) {
       if (a.flags[0]) continue;  // every item of the field skips: no strip waits on another
@@ -1328,7 +1480,10 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
   });
   std::vector<int> start(nf + 1, 0);
   size_t max_words = 0;
+  const double lmax = (double)g[0].nstrips * 96 + g[0].r2;
   for (size_t k = 0; k < nf; k++) {
+    const double l = ((double)g[k].nstrips * 96 + g[k].r2) / lmax;
+    g[k].prio = l >= 0.75 ? 3 : (l >= 0.5 ? 2 : (l >= 0.25 ? 1 : 0));
     start[k + 1] = start[k] + 2 * g[k].nstrips;
     max_words = std::max(max_words, (size_t)2 * g[k].nstrips * g[k].r2 * 2);
   }
