@@ -814,7 +814,20 @@ struct SrcIds2d {
 struct SrcCodes2d {
   const uint16_t* q;
   int r1, r2, nby;
+  // optional (batched compress): the values the codec ran on, the global
+  // exclusive scan of the blocks' zero counts at this component's first
+  // block, and where the encoder gathers the unpredictable values
+  const float* x = nullptr;
+  const uint64_t* uoff = nullptr;
+  float* unp = nullptr;
+  static constexpr bool kUnpred = true;
   static constexpr int kMaxItems = kBlock2d * kBlock2d, kPer = 1;
+  __device__ size_t elem(size_t b, int idx) const {
+    const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+    const int sy = min(kBlock2d, r2 - bj * kBlock2d);
+    const int ii = idx / sy, jj = idx - ii * sy;
+    return (size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj;
+  }
   __device__ int get(size_t b, int idx, uint32_t* sym) const {
     const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
     const int sx = min(kBlock2d, r1 - bi * kBlock2d), sy = min(kBlock2d, r2 - bj * kBlock2d);
@@ -1041,24 +1054,6 @@ __global__ void __launch_bounds__(256) k_block_zeros_multi(const UnpJob* __restr
   if (lane == 0) J.cnt[b] = z;
 }
 
-__global__ void __launch_bounds__(256) k_gather_unpred_multi(const UnpJob* __restrict__ jobs, int nj) {
-  const UnpJob& J = jobs[job_of(jobs, nj, blockIdx.x)];
-  const size_t b = (size_t)(blockIdx.x - J.blk0) * kUnpWaves + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (b >= J.nb || !J.out) return;
-  const BlockView v(b, J.r1, J.r2, J.nby);
-  uint64_t o = J.off[b] - J.off[0];
-  const uint64_t below = (1ull << lane) - 1;
-  for (int k0 = 0; k0 < v.m; k0 += 64) {
-    const int k = k0 + lane;
-    const size_t e = k < v.m ? v.elem(k, J.r2) : 0;
-    const bool zero = k < v.m && J.q[e] == 0;
-    const uint64_t mask = __ballot(zero);
-    if (zero) J.out[o + __popcll(mask & below)] = J.x[e];
-    o += __popcll(mask);
-  }
-}
-
 __global__ void __launch_bounds__(256) k_scatter_unpred_multi(const UnpJob* __restrict__ jobs, int nj) {
   const UnpJob& J = jobs[job_of(jobs, nj, blockIdx.x)];
   const size_t b = (size_t)(blockIdx.x - J.blk0) * kUnpWaves + (threadIdx.x >> 6);
@@ -1232,8 +1227,7 @@ struct Layout2d {
   const uint64_t* tinfo[3];  // device tree info of the eb/u/v trees (node count, tree bytes, ...)
   const uint8_t* tree[3];
   EncJob<SrcIds2d>* ej_ids;  // jobs whose word arrays k_layout2d fills in
-  EncJob<SrcCodes2d>* ej_codes;  // [2]
-  UnpJob* gather;                // [2]
+  EncJob<SrcCodes2d>* ej_codes;  // [2], also their unpredictable-value arrays
   // set by k_layout2d
   uint32_t* words;  // eb, u, v word arrays, consecutive
   float* unp;       // u then v unpredictable values
@@ -1285,8 +1279,8 @@ __global__ void k_layout2d(Layout2d* __restrict__ L, int nf, uint32_t* words, ui
     F.ej_ids->words = F.words;
     F.ej_codes[0].words = F.words + F.wn[0];
     F.ej_codes[1].words = F.words + F.wn[0] + F.wn[1];
-    F.gather[0].out = F.unp;
-    F.gather[1].out = F.unp + F.unp_n[0];
+    F.ej_codes[0].src.unp = F.unp;
+    F.ej_codes[1].src.unp = F.unp + F.unp_n[0];
   }
   *words_used = wpos;
 }
@@ -1724,7 +1718,7 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
     for (int k = 0; k < kMaxEbCode; k++) f[k] = hh[k];
     huff_build(f.data(), kMaxEbCode, 2 * 1024, t_eb);
     upload_codebook(ctx, t_eb, cb_code, cb_len, "c2.eb");
-    k_chunk_bits<<<(int)encode_blocks(nch_ids), 256, 0, st>>>(s_ids, nch_ids, cb_len, bits);
+    k_chunk_bits<<<(int)encode_blocks(nch_ids), kEncThreads, 0, st>>>(s_ids, nch_ids, cb_len, bits);
     CPSZ_HIP(hipGetLastError());
     scan_u64(ctx, bits, offs, nch_ids, totals + 0);
     for (int k = 0; k < 65536; k++) f[k] = hh[kMaxEbCode + k];
@@ -1735,8 +1729,8 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   // ---- codebooks and encode pass A (bits per chunk)
   upload_codebook(ctx, t_u, cb_code + 65536, cb_len + 65536, "c2.u");
   upload_codebook(ctx, t_v, cb_code + 2 * 65536, cb_len + 2 * 65536, "c2.v");
-  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
-  k_chunk_bits<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
+  k_chunk_bits<<<(int)encode_blocks(nb), kEncThreads, 0, st>>>(s_u, nb, cb_len + 65536, bits + nch_ids);
+  k_chunk_bits<<<(int)encode_blocks(nb), kEncThreads, 0, st>>>(s_v, nb, cb_len + 2 * 65536, bits + nch_ids + nb);
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits + nch_ids, offs + nch_ids, nb, totals + 1);
   scan_u64(ctx, bits + nch_ids + nb, offs + nch_ids + nb, nb, totals + 2);
@@ -1750,10 +1744,10 @@ void c2_entropy(Ctx& ctx, const C2Work& w, DeviceStream& out) {
   const size_t w_eb = div_up(bytes_eb, 4) + 2, w_u = div_up(bytes_u, 4) + 2, w_v = div_up(bytes_v, 4) + 2;
   uint32_t* words = ctx.buf<uint32_t>("c2.words", w_eb + w_u + w_v);
   k_fill_u32<<<1024, 256, 0, st>>>(words, w_eb + w_u + w_v, 0);
-  k_encode<<<(int)encode_blocks(nch_ids), 256, 0, st>>>(s_ids, nch_ids, cb_code, cb_len, offs, bits, words);
-  k_encode<<<(int)encode_blocks(nb), 256, 0, st>>>(s_u, nb, cb_code + 65536, cb_len + 65536, offs + nch_ids,
+  k_encode<<<(int)encode_blocks(nch_ids), kEncThreads, 0, st>>>(s_ids, nch_ids, cb_code, cb_len, offs, bits, words);
+  k_encode<<<(int)encode_blocks(nb), kEncThreads, 0, st>>>(s_u, nb, cb_code + 65536, cb_len + 65536, offs + nch_ids,
                                                    bits + nch_ids, words + w_eb);
-  k_encode<<<(int)encode_blocks(nb), 256, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
+  k_encode<<<(int)encode_blocks(nb), kEncThreads, 0, st>>>(s_v, nb, cb_code + 2 * 65536, cb_len + 2 * 65536,
                                                    offs + nch_ids + nb, bits + nch_ids + nb, words + w_eb + w_u);
   // unpredictable values in the field's own type (T = double: 8 bytes each)
   const size_t vb = w.f64 ? 8 : 4;
@@ -1890,7 +1884,7 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   for (size_t k = 0; k < nf; k++)
     CPSZ_REQUIRE(w[k]->flags == w[0]->flags + 4 * k, kErrInternal, "batched entropy: eb flags not contiguous");
   Arena A;
-  const size_t abytes = 2 * nf * (sizeof(HistJob<U16Hist>) + 2 * sizeof(UnpJob)) + ntree * sizeof(CompactJob) +
+  const size_t abytes = 2 * nf * sizeof(HistJob<U16Hist>) + ntree * sizeof(CompactJob) +
                         ntree * sizeof(TreeJob) + nf * sizeof(EncJob<SrcIds2d>) + 2 * nf * sizeof(EncJob<SrcCodes2d>) +
                         5 * nf * sizeof(ScanDiff) + nf * sizeof(Layout2d) + 16 * 64;
   A.d = ctx.buf<uint8_t>("b2e.tab", abytes);
@@ -1898,10 +1892,6 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   A.cap = abytes;
   HistJob<U16Hist>* hj;
   HistJob<U16Hist>* d_hj = A.take(2 * nf, &hj);
-  UnpJob* uj;
-  UnpJob* d_uj = A.take(2 * nf, &uj);
-  UnpJob* gj;
-  UnpJob* d_gj = A.take(2 * nf, &gj);
   CompactJob* cj;
   CompactJob* d_cj = A.take(ntree, &cj);
   TreeJob* tj;
@@ -1914,7 +1904,7 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   ScanDiff* d_sd = A.take(5 * nf, &sd);
   Layout2d* lj;
   Layout2d* d_lj = A.take(nf, &lj);
-  uint32_t hblk = 0, ublk = 0, iblk = 0, cblk = 0;
+  uint32_t hblk = 0, iblk = 0, cblk = 0;
   for (size_t k = 0; k < nf; k++) {
     const C2Work& W = *w[k];
     const G& x = g[k];
@@ -1927,19 +1917,6 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
       h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 8), 512);
       h.blk0 = hblk;
       hblk += h.nblk;
-      UnpJob& u = uj[2 * k + c];
-      u = UnpJob{};
-      u.q = c ? W.qv : W.qu;
-      u.x = c ? W.dV : W.dU;
-      u.r1 = (int)W.r1;
-      u.r2 = (int)W.r2;
-      u.nby = (int)x.nby;
-      u.nb = x.nb;
-      u.cnt = zc + x.zb + c * x.nb;
-      u.off = zoff + x.zb + c * x.nb;
-      u.blk0 = ublk;
-      ublk += (uint32_t)unp_grid(x.nb);
-      gj[2 * k + c] = u;  // the gather's output array comes from k_layout2d
     }
     for (int t = 0; t < 3; t++) {
       const size_t tt = 3 * k + t;
@@ -1968,10 +1945,15 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     e.off = offs + x.cb;
     e.words = nullptr;
     e.blk0 = iblk;
+    e.zc = nullptr;
     iblk += (uint32_t)encode_blocks(x.nch_ids);
     for (int c = 0; c < 2; c++) {
       EncJob<SrcCodes2d>& f = ej_c[2 * k + c];
-      f.src = SrcCodes2d{c ? W.qv : W.qu, (int)W.r1, (int)W.r2, (int)x.nby};
+      // the blocks' zero counts come with their bit counts; the encoder
+      // gathers the unpredictable values (its array comes from k_layout2d)
+      f.src = SrcCodes2d{c ? W.qv : W.qu, (int)W.r1, (int)W.r2, (int)x.nby, c ? W.dV : W.dU,
+                         zoff + x.zb + c * x.nb, nullptr};
+      f.zc = zc + x.zb + c * x.nb;
       f.nchunks = x.nb;
       f.code = cb_code + k * HN + kMaxEbCode + c * 65536;
       f.len = cb_len + k * HN + kMaxEbCode + c * 65536;
@@ -1999,15 +1981,12 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     }
     l.ej_ids = d_ej_i + k;
     l.ej_codes = d_ej_c + 2 * k;
-    l.gather = d_gj + 2 * k;
   }
   ctx.upload_table("b2e.tab", A.d, A.h, A.used);
-  // ---- histograms, zero counts, present symbols, trees
+  // ---- histograms, present symbols, trees
   const uint32_t win_lo = 32768 - 8192, win_n = 16384;
   k_hist_multi<U16Hist><<<hblk, 256, win_n * 4, st>>>(d_hj, (int)(2 * nf), win_lo, win_n);
-  k_block_zeros_multi<<<ublk, 256, 0, st>>>(d_uj, (int)(2 * nf));
   CPSZ_HIP(hipGetLastError());
-  scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
   k_zero_i32<<<1, 64, 0, st>>>((int*)meta, 16);  // total, words used, error flag
   k_compact_hist<<<(unsigned)ntree, 1024, 0, st>>>(d_cj, d_count, d_base, d_total, d_syms, d_freqs);
   CPSZ_HIP(hipGetLastError());
@@ -2099,20 +2078,21 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
     CPSZ_HIP(hipGetLastError());
   }
   ctx.mark("tree");
-  // ---- chunk bit counts, one scan for every stream, per-stream totals
-  k_chunk_bits_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ej_i, (int)nf);
-  k_chunk_bits_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_ej_c, (int)(2 * nf));
+  // ---- chunk bit counts (and the code blocks' zero counts), one scan for
+  // every stream, per-stream totals
+  k_chunk_bits_multi<SrcIds2d><<<iblk, kEncThreads, 0, st>>>(d_ej_i, (int)nf);
+  k_chunk_bits_multi<SrcCodes2d><<<cblk, kEncThreads, 0, st>>>(d_ej_c, (int)(2 * nf));
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits, offs, CB, offs + CB);
+  scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
   k_scan_diffs<<<(unsigned)div_up(5 * nf, 256), 256, 0, st>>>(d_sd, (int)(5 * nf), d_tot);
   CPSZ_HIP(hipGetLastError());
   ctx.mark("scan");
-  // ---- layout, encode, gather, headers, assembly
+  // ---- layout, encode (with the unpredictable gather), headers, assembly
   k_layout2d<<<1, 1, 0, st>>>(d_lj, (int)nf, words, words_cap, unp, unp_cap, d_sizes, d_words_used, d_err);
   k_fill_words<<<2048, 256, 0, st>>>(words, d_words_used);
-  k_encode_multi<SrcIds2d><<<iblk, 256, 0, st>>>(d_ej_i, (int)nf);
-  k_encode_multi<SrcCodes2d><<<cblk, 256, 0, st>>>(d_ej_c, (int)(2 * nf));
-  k_gather_unpred_multi<<<ublk, 256, 0, st>>>(d_gj, (int)(2 * nf));
+  k_encode_multi<SrcIds2d><<<iblk, kEncThreads, 0, st>>>(d_ej_i, (int)nf);
+  k_encode_multi<SrcCodes2d><<<cblk, kEncThreads, 0, st>>>(d_ej_c, (int)(2 * nf));
   CPSZ_HIP(hipGetLastError());
   ctx.mark("encode");
   k_write_heads2d<<<(unsigned)nf, 256, 0, st>>>(d_lj, dout);

@@ -1066,7 +1066,7 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   // the id stream's bit counts run on the device while the host builds the
   // code tree
   upload_codebook(ctx, t_id, cb_code, cb_len, "c3.id");
-  k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_len, bits);
+  k_chunk_bits<<<(int)encode_blocks(nch), kEncThreads, 0, st>>>(s_ids, nch, cb_len, bits);
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits, offs, nch, totals + 0);
   uint64_t npred_syms = 0;
@@ -1077,7 +1077,7 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   huff_build(f.data(), 65536, 2 * kCapacity, t_q);
   ctx.mark("tree");
   upload_codebook(ctx, t_q, cb_code + 65536, cb_len + 65536, "c3.q");
-  k_chunk_bits<<<(int)encode_blocks(nch), 256, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
+  k_chunk_bits<<<(int)encode_blocks(nch), kEncThreads, 0, st>>>(s_codes, nch, cb_len + 65536, bits + nch);
   CPSZ_HIP(hipGetLastError());
   scan_u64(ctx, bits + nch, offs + nch, nch, totals + 1);
   uint64_t* ht = (uint64_t*)ctx.host_buf("c3.totals", 64);
@@ -1090,8 +1090,8 @@ void c3_entropy(Ctx& ctx, const C3Work& w, DeviceStream& out) {
   const size_t w_id = div_up(by_id, 4) + 2, w_q = div_up(by_q, 4) + 2;
   uint32_t* words = ctx.buf<uint32_t>("c3.words", w_id + w_q);
   k_fill_u32<<<1024, 256, 0, st>>>(words, w_id + w_q, 0);
-  k_encode<<<(int)encode_blocks(nch), 256, 0, st>>>(s_ids, nch, cb_code, cb_len, offs, bits, words);
-  k_encode<<<(int)encode_blocks(nch), 256, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536, offs + nch,
+  k_encode<<<(int)encode_blocks(nch), kEncThreads, 0, st>>>(s_ids, nch, cb_code, cb_len, offs, bits, words);
+  k_encode<<<(int)encode_blocks(nch), kEncThreads, 0, st>>>(s_codes, nch, cb_code + 65536, cb_len + 65536, offs + nch,
                                                     bits + nch, words + w_id);
   float* unp = ctx.buf<float>("c3.unp", 3 * nunp + 1);
   k_gather_unpred3d<<<(int)div_up(nch, 4), 256, 0, st>>>(w.vid, sk, w.dU, w.dV, w.dW, n, nch, coff + nch + 1,

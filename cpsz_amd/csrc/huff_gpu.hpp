@@ -56,6 +56,7 @@ __global__ void __launch_bounds__(256) k_hist(Src src, size_t nchunks, uint32_t*
 //                             past the chunk's end (branch-free, so that a
 //                             wave issues every round's loads together)
 constexpr int kEncWaves = 4;
+constexpr int kEncThreads = kEncWaves * 64;  // block size of the chunk-bits and encode kernels
 
 inline size_t encode_blocks(size_t nchunks) { return (nchunks + kEncWaves - 1) / kEncWaves; }
 
@@ -69,9 +70,18 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
   return x;
 }
 
+// A source may also carry the stream's unpredictable values (kUnpred): the
+// values at its symbol-0 items, gathered in stream order while encoding.
+template <class S, class = void>
+struct HasUnpred : std::false_type {};
+template <class S>
+struct HasUnpred<S, std::void_t<decltype(S::kUnpred)>> : std::bool_constant<S::kUnpred> {};
+
+// zc (optional): the chunk's count of symbol 0 (the 2D codes' unpredictable
+// marker) for the caller's offsets of the gathered values
 template <class Src>
 __device__ __forceinline__ void chunk_bits_wave(const Src& src, size_t c, const uint8_t* __restrict__ len,
-                                                uint64_t* __restrict__ bits) {
+                                                uint64_t* __restrict__ bits, uint32_t* __restrict__ zc = nullptr) {
   constexpr int kRounds = (Src::kMaxItems + 63) / 64;
   const int lane = threadIdx.x & 63;
   // every round's symbols are fetched before any is used (overlapped latency)
@@ -88,10 +98,18 @@ __device__ __forceinline__ void chunk_bits_wave(const Src& src, size_t c, const 
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) b += __shfl_xor(b, o);
   if (lane == 0) bits[c] = b;
+  if (zc) {
+    uint32_t z = 0;
+#pragma unroll
+    for (int r = 0; r < kRounds; r++)
+#pragma unroll
+      for (int q = 0; q < Src::kPer; q++) z += (uint32_t)__popcll(__ballot(q < ns[r] && sym[r][q] == 0));
+    if (lane == 0) zc[c] = z;
+  }
 }
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_chunk_bits(Src src, size_t nchunks,
+__global__ void __launch_bounds__(kEncThreads) k_chunk_bits(Src src, size_t nchunks,
                                                     const uint8_t* __restrict__ len,
                                                     uint64_t* __restrict__ bits) {
   const size_t c = (size_t)xcd_block(blockIdx.x, gridDim.x) * kEncWaves + (threadIdx.x >> 6);
@@ -122,15 +140,16 @@ struct EncJob {
   uint64_t* bits;        // chunk bits, at this stream's first chunk
   uint32_t* words;       // this stream's zero-filled word array
   uint32_t blk0;         // first block (kEncWaves chunks per block)
+  uint32_t* zc;          // optional: per-chunk symbol-0 counts (with the bits)
 };
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_chunk_bits_multi(const EncJob<Src>* __restrict__ jobs, int nj) {
+__global__ void __launch_bounds__(kEncThreads) k_chunk_bits_multi(const EncJob<Src>* __restrict__ jobs, int nj) {
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const EncJob<Src>& J = jobs[job_of(jobs, nj, b)];
   const size_t c = (size_t)(b - J.blk0) * kEncWaves + (threadIdx.x >> 6);
   if (c >= J.nchunks) return;
-  chunk_bits_wave(J.src, c, J.len, J.bits);
+  chunk_bits_wave(J.src, c, J.len, J.bits, J.zc);
 }
 
 template <class Src>
@@ -146,7 +165,8 @@ __device__ __forceinline__ void hist_block(const Src& src, size_t nchunks, uint3
   // per wave instead of one per occurrence on a single address
   const bool zero_out = win_lo > 0;
   uint32_t zeros = 0;
-  for (size_t c = (size_t)blk * kEncWaves + (threadIdx.x >> 6); c < nchunks; c += (size_t)nblk * kEncWaves) {
+  const uint32_t waves = blockDim.x >> 6;
+  for (size_t c = (size_t)blk * waves + (threadIdx.x >> 6); c < nchunks; c += (size_t)nblk * waves) {
     uint32_t sym[kRounds][Src::kPer];
     int ns[kRounds];
 #pragma unroll
@@ -212,7 +232,9 @@ __device__ __forceinline__ void put_bits(uint32_t* st, uint32_t p, uint64_t cw, 
 // words with consecutive lanes on consecutive words. The words a chunk
 // shares with its neighbours (a partial first or last word) are merged with
 // atomicOr into the zero-filled output; the others are plain stores. b0 is
-// the chunk's first bit within its stream. Called by whole blocks.
+// the chunk's first bit within its stream. Called by whole blocks: the
+// block's waves (adjacent chunks, whose rows share cache lines) stay in step
+// at its barriers, which measured faster than letting them drift apart.
 template <class Src>
 __device__ __forceinline__ void encode_wave(const Src& src, size_t c, bool live, uint64_t b0, uint64_t nbits,
                                             const uint64_t* __restrict__ code, const uint8_t* __restrict__ len,
@@ -229,6 +251,20 @@ __device__ __forceinline__ void encode_wave(const Src& src, size_t c, bool live,
     int ns[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; r++) ns[r] = src.get(c, r * 64 + lane, sym[r]);
+    if constexpr (HasUnpred<Src>::value) {
+      if (src.unp) {
+        // the chunk's unpredictable values, at its offset among the stream's
+        uint64_t o = src.uoff[c] - src.uoff[0];
+        const uint64_t below = (1ull << lane) - 1;
+#pragma unroll
+        for (int r = 0; r < kRounds; r++) {
+          const bool z = ns[r] && sym[r][0] == 0;
+          const uint64_t m = __ballot(z);
+          if (z) src.unp[o + __popcll(m & below)] = src.x[src.elem(c, r * 64 + lane)];
+          o += __popcll(m);
+        }
+      }
+    }
 #pragma unroll
     for (int r = 0; r < kRounds; r++)
 #pragma unroll
@@ -266,7 +302,7 @@ __device__ __forceinline__ void encode_wave(const Src& src, size_t c, bool live,
 }
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_encode(Src src, size_t nchunks,
+__global__ void __launch_bounds__(kEncThreads) k_encode(Src src, size_t nchunks,
                                                 const uint64_t* __restrict__ code,
                                                 const uint8_t* __restrict__ len,
                                                 const uint64_t* __restrict__ off,
@@ -281,7 +317,7 @@ __global__ void __launch_bounds__(256) k_encode(Src src, size_t nchunks,
 }
 
 template <class Src>
-__global__ void __launch_bounds__(256) k_encode_multi(const EncJob<Src>* __restrict__ jobs, int nj) {
+__global__ void __launch_bounds__(kEncThreads) k_encode_multi(const EncJob<Src>* __restrict__ jobs, int nj) {
   constexpr int kStage = Src::kMaxItems * Src::kPer * 2 + 2;
   __shared__ uint32_t stage[kEncWaves][kStage];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
