@@ -173,7 +173,7 @@ __device__ __forceinline__ uint32_t code_from_table(double x, const double* __re
   return (uint32_t)lo;
 }
 
-constexpr int kTH = 8, kTW = 64;  // vertex tile
+constexpr int kTH = 16, kTW = 32;  // vertex tile (17x33 cells: 9 wave rounds of triangles per 512 vertices)
 
 template <typename T>
 __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __restrict__ V, int r1, int r2,
@@ -801,7 +801,7 @@ struct SrcIds2d {
   const uint16_t* cu;
   const uint16_t* cv;
   size_t n, m;
-  static constexpr int kChunk = 256;
+  static constexpr int kChunk = 512;
   static constexpr int kMaxItems = kChunk, kPer = 1;
   __device__ int get(size_t c, int idx, uint32_t* sym) const {
     const size_t p0 = c * kChunk + idx, p = p0 < m ? p0 : m - 1;
