@@ -594,7 +594,9 @@ __device__ __forceinline__ void strip_item(const Strip2d& a, int item, int lane)
   }
 }
 
-template <bool COMPRESS, bool WIDE>
+// One strip variant per kernel (the host picks it): the unrolled strip loop
+// is large, and one variant keeps the kernel's hot code and registers small.
+template <bool COMPRESS, bool WIDE, int SHIFT>
 __global__ void __launch_bounds__(64) k_lorenzo2d(Strip2d a) {
   const int lane = threadIdx.x;
   const int items = 2 * a.nstrips;
@@ -603,9 +605,7 @@ __global__ void __launch_bounds__(64) k_lorenzo2d(Strip2d a) {
     if (lane == 0) item = atomicAdd(a.ticket, 1);
     item = __shfl(item, 0);
     if (item >= items) return;
-    if (WIDE) strip_item<COMPRESS, WIDE, 2>(a, item, lane);
-    else if (a.shift == 1) strip_item<COMPRESS, false, 1>(a, item, lane);
-    else strip_item<COMPRESS, false, 2>(a, item, lane);
+    strip_item<COMPRESS, WIDE, SHIFT>(a, item, lane);
   }
 }
 
@@ -662,8 +662,7 @@ __global__ void __launch_bounds__(64) k_lorenzo2d_batch(StripBatch b) {
       wide = a.flags[1] != 0;
     }
     if (wide) strip_item<COMPRESS, true, 2>(a, item - b.start[lo], lane);
-    else if (a.shift == 1) strip_item<COMPRESS, false, 1>(a, item - b.start[lo], lane);
-    else strip_item<COMPRESS, false, 2>(a, item - b.start[lo], lane);
+    else strip_item<COMPRESS, false, 2>(a, item - b.start[lo], lane);  // batches are plain offline fields
   }
 }
 
@@ -1458,8 +1457,9 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
     k_zero_i32<<<(int)div_up(nb, 256), 256, 0, st>>>((int*)sa.bnd, (int)nb);
     const int items = 2 * sa.nstrips;
     const int grid = std::min(items, grid_cap);
-    if (sa.wide) k_lorenzo2d<COMPRESS, true><<<grid, 64, 0, st>>>(sa);
-    else k_lorenzo2d<COMPRESS, false><<<grid, 64, 0, st>>>(sa);
+    if (sa.wide) k_lorenzo2d<COMPRESS, true, 2><<<grid, 64, 0, st>>>(sa);
+    else if (sa.shift == 1) k_lorenzo2d<COMPRESS, false, 1><<<grid, 64, 0, st>>>(sa);
+    else k_lorenzo2d<COMPRESS, false, 2><<<grid, 64, 0, st>>>(sa);
     CPSZ_HIP(hipGetLastError());
     return;
   }
@@ -1468,6 +1468,7 @@ void launch_wavefront(Ctx& ctx, std::vector<Strip2d>& f, int grid_cap) {
   // tallest fields are not left waiting for a free wave at the end.
   const size_t nf = f.size();
   CPSZ_REQUIRE(nf <= 65535, kErrUnsupported, "2D batch: at most 65535 fields per call");
+  for (const Strip2d& x : f) CPSZ_REQUIRE(x.shift == 2, kErrInternal, "2D batch wavefront: plain offline fields only");
   std::vector<Strip2d> g(f);
   std::stable_sort(g.begin(), g.end(), [](const Strip2d& a, const Strip2d& b) {
     return (int64_t)a.nstrips * 96 + a.r2 > (int64_t)b.nstrips * 96 + b.r2;
