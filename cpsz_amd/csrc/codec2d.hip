@@ -876,6 +876,9 @@ struct SinkCodes2d {
   static constexpr int kBytes = 2;
   uint16_t* q;
   int r1, r2, nby;
+  // optional: zero-filled per-block counts of code 0 (the unpredictable
+  // marker), added up while the codes are written
+  uint32_t* zc = nullptr;
   __device__ void put(uint64_t p, uint32_t s) const {
     const uint64_t rowblk = (uint64_t)kBlock2d * r2;
     const int bi = (int)(p / rowblk);
@@ -887,6 +890,20 @@ struct SinkCodes2d {
     const int sy = min(kBlock2d, r2 - bj * kBlock2d);
     const int ii = rem2 / sy, jj = rem2 - ii * sy;
     q[(size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj] = (uint16_t)s;
+    if (zc) {
+      // one atomic per block among the active lanes' zeros: consecutive
+      // positions (the staged writes) fall in one or two blocks
+      const uint32_t b = (uint32_t)bi * (uint32_t)nby + (uint32_t)bj;
+      const bool z = s == 0;
+      uint64_t m = __ballot(z);
+      while (m) {
+        const int leader = __ffsll((long long)m) - 1;
+        const uint32_t bl = (uint32_t)__shfl((int)b, leader);
+        const uint64_t same = __ballot(z && b == bl);
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&zc[bl], (uint32_t)__popcll(same));
+        m &= ~same;
+      }
+    }
   }
 };
 
@@ -2413,8 +2430,9 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
 // Unpredictable values of every field of a batch (float): the stream's
 // values copied to aligned arrays in one assembly launch, the per-block zero
 // counts of every component, one global scan, one scatter launch.
+// zeros_counted: the code decoders already filled b2u.zc (SinkCodes2d::zc)
 void d2_unpred_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<D2Plan>& plan,
-                     D2Work* const* w, size_t nf) {
+                     D2Work* const* w, size_t nf, bool zeros_counted) {
   hipStream_t st = ctx.stream();
   size_t Z = 0, V = 0, npieces = 0;
   constexpr size_t kPiece = 65536;
@@ -2473,7 +2491,7 @@ void d2_unpred_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const st
   CPSZ_HIP(hipMemcpyAsync(A.d, A.h, A.used, hipMemcpyHostToDevice, st));
   k_fill_u32<<<1, 256, 0, st>>>((uint32_t*)errs, 4 * nf, 0);
   if (np) k_assemble<<<(unsigned)np, 256, 0, st>>>(d_cp, (uint8_t*)vals);
-  k_block_zeros_multi<<<blk, 256, 0, st>>>(d_uj, (int)(2 * nf));
+  if (!zeros_counted) k_block_zeros_multi<<<blk, 256, 0, st>>>(d_uj, (int)(2 * nf));
   CPSZ_HIP(hipGetLastError());
   scan_u32_to_u64(ctx, zc, zoff, Z, zoff + Z);
   k_scatter_unpred_multi<<<blk, 256, 0, st>>>(d_uj, (int)(2 * nf));
@@ -2528,6 +2546,14 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
   std::vector<DecJob<SinkIds2d>> ji(nf);
   std::vector<DecJob<SinkIds2dB>> ji8(nf);
   std::vector<DecJob<SinkCodes2d>> jc(2 * nf);
+  // the code decoders count each 12x12 block's zero codes as they write them
+  // (the unpredictable scatter's offsets), in the array d2_unpred_batch scans
+  size_t Z = 0;
+  for (size_t k = 0; k < nf; k++) Z += 2 * plan[k].nb;
+  uint32_t* zc = ctx.buf<uint32_t>("b2u.zc", Z + 1);
+  k_fill_u32<<<(unsigned)std::min<size_t>(div_up(Z, 256), 2048), 256, 0, st>>>(zc, Z, 0);
+  CPSZ_HIP(hipGetLastError());
+  size_t zb = 0;
   bool ids8 = true;
   for (size_t k = 0; k < nf; k++) ids8 = ids8 && max_symbol(hd[3 * k].d) < 256;
   for (size_t k = 0; k < nf; k++) {
@@ -2542,13 +2568,14 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
     uint16_t* qs[2] = {w[k]->qu, w[k]->qv};
     for (int c = 0; c < 2; c++)
       jc[2 * k + c] = {&dd[3 * k + 1 + c], bc[3 * k + 1 + c].dst, p.comp[c].h.nbytes, n,
-                       SinkCodes2d{qs[c], (int)r1, (int)r2, p.nby}, "b2d"};
+                       SinkCodes2d{qs[c], (int)r1, (int)r2, p.nby, zc + zb + c * p.nb}, "b2d"};
+    zb += 2 * p.nb;
   }
   if (ids8) decode_streams_multi(ctx, ji8.data(), (int)nf, "b2d.ids");
   else decode_streams_multi(ctx, ji.data(), (int)nf, "b2d.ids");
   decode_streams_multi(ctx, jc.data(), (int)(2 * nf), "b2d.codes");
   ctx.mark("huffman");
-  d2_unpred_batch(ctx, d_in, plan, w, nf);
+  d2_unpred_batch(ctx, d_in, plan, w, nf, true);
   ctx.mark("unpred");
 }
 
