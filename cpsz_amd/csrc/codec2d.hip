@@ -2512,12 +2512,107 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
   hipStream_t st = ctx.stream();
   std::vector<D2Plan> plan(nf);
   std::vector<HostDecoder> hd(3 * nf);
+  // Every stream's header pieces come to the host in four batched rounds (a
+  // gather of the pieces on the device, one copy, one wait each): the start of
+  // the stream (header, id tree), the u and v section heads, their indicator
+  // bytes and tree heads, then their trees. The offsets follow the layout
+  // d2_parse reads; a piece outside what was fetched is read from the device
+  // by the parser itself.
+  for (size_t k = 0; k < nf; k++)
+    CPSZ_REQUIRE(d_in[k] && w[k]->r1 >= 1 && w[k]->r2 >= 1, kErrArg, "decompress_2d: bad arguments");
+  std::vector<std::vector<StreamReader::Range>> pre(nf);
+  {
+    struct Want {
+      size_t k, off, len;
+    };
+    // one round: gathers the wanted pieces into a device staging buffer, copies
+    // it to pinned host memory and records the ranges; returns their host copies
+    int round = 0;
+    auto fetch = [&](const std::vector<Want>& want) {
+      std::vector<const uint8_t*> where(want.size(), nullptr);
+      size_t total = 0, npc = 0;
+      for (const Want& x : want) {
+        total += (x.len + 15) & ~(size_t)15;
+        npc += div_up(x.len, (size_t)65536);
+      }
+      if (!total) return where;
+      const std::string tag = "b2d.pre" + std::to_string(round++);
+      uint8_t* dst = ctx.buf<uint8_t>(tag.c_str(), total);
+      uint8_t* hst = (uint8_t*)ctx.host_buf(tag.c_str(), total);
+      CopyPiece* hp = (CopyPiece*)ctx.host_table((tag + "t").c_str(), npc * sizeof(CopyPiece));
+      CopyPiece* dp = ctx.buf<CopyPiece>((tag + "t").c_str(), npc);
+      size_t at = 0, np = 0;
+      for (size_t i = 0; i < want.size(); i++) {
+        const Want& x = want[i];
+        for (size_t o = 0; o < x.len; o += 65536)
+          hp[np++] = CopyPiece{at + o, d_in[x.k] + x.off + o, (uint32_t)std::min<size_t>(65536, x.len - o), 0};
+        where[i] = hst + at;
+        pre[x.k].push_back({x.off, x.len, hst + at});
+        at += (x.len + 15) & ~(size_t)15;
+      }
+      ctx.upload_table((tag + "t").c_str(), dp, hp, np * sizeof(CopyPiece));
+      k_assemble<<<(unsigned)np, 256, 0, st>>>(dp, dst);
+      CPSZ_HIP(hipGetLastError());
+      CPSZ_HIP(hipMemcpyAsync(hst, dst, at, hipMemcpyDeviceToHost, st));
+      CPSZ_HIP(hipStreamSynchronize(st));
+      return where;
+    };
+    constexpr size_t kStart = 32u << 10;  // header + id tree (at most 1024 leaves: < 27 KiB)
+    std::vector<Want> w1(nf);
+    for (size_t k = 0; k < nf; k++) w1[k] = {k, 0, std::min(kStart, in_size[k])};
+    const std::vector<const uint8_t*> h1 = fetch(w1);
+    // section starts: u after the id section, v u_size bytes later
+    std::vector<Want> w2;
+    for (size_t k = 0; k < nf; k++) {
+      const uint8_t* h = h1[k];
+      if (!h || w1[k].len < 48) continue;
+      uint64_t u_size, nbytes;
+      uint32_t tsz;
+      std::memcpy(&u_size, h + 20, 8);
+      std::memcpy(&tsz, h + 44, 4);
+      if ((uint64_t)48 + tsz + 8 > w1[k].len) continue;
+      std::memcpy(&nbytes, h + 48 + tsz, 8);
+      const uint64_t u0 = 56 + (uint64_t)tsz + nbytes;
+      if (nbytes > in_size[k] || u0 > in_size[k]) continue;
+      // section head: block size, radius, meanInfo, regression count, then
+      // the unpredictable count (at +25) and values (from +33)
+      if (in_size[k] - u0 >= 33) {
+        w2.push_back({k, u0, 33});
+      }
+      if (u_size <= in_size[k] - u0 && in_size[k] - u0 - u_size >= 33) {
+        w2.push_back({k, u0 + u_size, 33});
+      }
+    }
+    const std::vector<const uint8_t*> h2 = fetch(w2);
+    std::vector<Want> w3;
+    for (size_t i = 0; i < w2.size(); i++) {
+      const size_t k = w2[i].k;
+      uint64_t unp;
+      std::memcpy(&unp, h2[i] + 25, 8);
+      const size_t nb = div_up(w[k]->r1, kBlock2d) * div_up(w[k]->r2, kBlock2d);
+      const uint64_t ind = w2[i].off + 33 + 4 * (uint64_t)std::min<uint64_t>(unp, w[k]->r1 * w[k]->r2);
+      const uint64_t len = (nb - 1) / 8 + 1 + 12;  // indicator bytes, tree head
+      if (ind > in_size[k] || len > in_size[k] - ind) continue;
+      w3.push_back({k, ind, len});
+    }
+    const std::vector<const uint8_t*> h3 = fetch(w3);
+    std::vector<Want> w4;
+    for (size_t i = 0; i < w3.size(); i++) {
+      const size_t k = w3[i].k;
+      uint32_t tsz;
+      std::memcpy(&tsz, h3[i] + w3[i].len - 4, 4);
+      const uint64_t t0 = w3[i].off + w3[i].len;
+      if ((uint64_t)tsz + 8 > in_size[k] - t0) continue;
+      w4.push_back({k, t0, (size_t)tsz + 8});  // tree and bitstream size
+    }
+    fetch(w4);
+  }
   {
     std::vector<std::function<void(Ctx&)>> jobs(nf);
     for (size_t k = 0; k < nf; k++)
       jobs[k] = [&, k](Ctx& wc) {
-        CPSZ_REQUIRE(d_in[k] && w[k]->r1 >= 1 && w[k]->r2 >= 1, kErrArg, "decompress_2d: bad arguments");
         StreamReader rd(wc, nullptr, d_in[k], in_size[k]);
+        rd.pre = pre[k];
         d2_parse(rd, w[k]->r1, w[k]->r2, plan[k]);
         const D2Plan& p = plan[k];
         prepare_decoder(p.h_eb.tree.data(), p.h_eb.tree.size(), p.h_eb.node_count, hd[3 * k]);

@@ -6,6 +6,7 @@
 #pragma once
 #include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "ctx.hpp"
 
@@ -19,12 +20,21 @@ struct StreamReader {
   size_t size;
   uint8_t* win = nullptr;
   size_t win_off = 0, win_len = 0;
+  // host copies of stream ranges fetched beforehand (for many streams in a
+  // few batched copies); reads inside one of them need no copy
+  struct Range {
+    size_t off, len;
+    const uint8_t* p;
+  };
+  std::vector<Range> pre;
 
   StreamReader(Ctx& c, const uint8_t* h, const uint8_t* d, size_t n) : ctx(c), host(h), dev(d), size(n) {}
 
   const uint8_t* read(size_t off, size_t len) {
     CPSZ_REQUIRE(off <= size && len <= size - off, kErrStream, "stream truncated");
     if (host) return host + off;
+    for (const Range& r : pre)
+      if (off >= r.off && off + len <= r.off + r.len) return r.p + (off - r.off);
     if (win && off >= win_off && off + len <= win_off + win_len) return win + (off - win_off);
     if (len <= kWindow) {
       if (!win) win = (uint8_t*)ctx.host_buf("rd.window", kWindow);
