@@ -881,14 +881,28 @@ struct SinkCodes2d {
   uint32_t* zc = nullptr;
   __device__ void put(uint64_t p, uint32_t s) const {
     const uint64_t rowblk = (uint64_t)kBlock2d * r2;
-    const int bi = (int)(p / rowblk);
-    const uint64_t rem = p - (uint64_t)bi * rowblk;
-    const int sx = min(kBlock2d, r1 - bi * kBlock2d);
-    int bj = (int)(rem / ((uint64_t)sx * kBlock2d));
-    if (bj > nby - 1) bj = nby - 1;
-    const uint32_t rem2 = (uint32_t)(rem - (uint64_t)bj * sx * kBlock2d);
-    const int sy = min(kBlock2d, r2 - bj * kBlock2d);
-    const int ii = rem2 / sy, jj = rem2 - ii * sy;
+    int bi, bj, ii, jj;
+    if (p < (1ull << 32) && rowblk < (1ull << 32)) {  // 32-bit divisions (a 64-bit one is a long sequence)
+      const uint32_t p32 = (uint32_t)p, rb = (uint32_t)rowblk;
+      const uint32_t b = p32 / rb, rem = p32 - b * rb;
+      bi = (int)b;
+      const uint32_t sx = (uint32_t)min(kBlock2d, r1 - bi * kBlock2d);
+      bj = min((int)(rem / (sx * kBlock2d)), nby - 1);
+      const uint32_t rem2 = rem - (uint32_t)bj * sx * kBlock2d;
+      const uint32_t sy = (uint32_t)min(kBlock2d, r2 - bj * kBlock2d);
+      ii = (int)(rem2 / sy);
+      jj = (int)(rem2 - (uint32_t)ii * sy);
+    } else {
+      bi = (int)(p / rowblk);
+      const uint64_t rem = p - (uint64_t)bi * rowblk;
+      const int sx = min(kBlock2d, r1 - bi * kBlock2d);
+      bj = (int)(rem / ((uint64_t)sx * kBlock2d));
+      if (bj > nby - 1) bj = nby - 1;
+      const uint32_t rem2 = (uint32_t)(rem - (uint64_t)bj * sx * kBlock2d);
+      const int sy = min(kBlock2d, r2 - bj * kBlock2d);
+      ii = rem2 / sy;
+      jj = rem2 - ii * sy;
+    }
     q[(size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj] = (uint16_t)s;
     if (zc) {
       // one atomic per block among the active lanes' zeros: consecutive
