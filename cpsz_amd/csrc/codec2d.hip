@@ -821,16 +821,28 @@ struct SrcCodes2d {
   float* unp = nullptr;
   static constexpr bool kUnpred = true;
   static constexpr int kMaxItems = kBlock2d * kBlock2d, kPer = 1;
+  // block row and column (32-bit division when the block index allows it)
+  __device__ void block(size_t b, int& bi, int& bj) const {
+    if (b < (1ull << 32)) {
+      bi = (int)((uint32_t)b / (uint32_t)nby);
+      bj = (int)((uint32_t)b - (uint32_t)bi * (uint32_t)nby);
+    } else {
+      bi = (int)(b / nby);
+      bj = (int)(b - (size_t)bi * nby);
+    }
+  }
   __device__ size_t elem(size_t b, int idx) const {
-    const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+    int bi, bj;
+    block(b, bi, bj);
     const int sy = min(kBlock2d, r2 - bj * kBlock2d);
-    const int ii = idx / sy, jj = idx - ii * sy;
+    const int ii = (int)((uint32_t)idx / (uint32_t)sy), jj = idx - ii * sy;
     return (size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj;
   }
   __device__ int get(size_t b, int idx, uint32_t* sym) const {
-    const int bi = (int)(b / nby), bj = (int)(b - (size_t)bi * nby);
+    int bi, bj;
+    block(b, bi, bj);
     const int sx = min(kBlock2d, r1 - bi * kBlock2d), sy = min(kBlock2d, r2 - bj * kBlock2d);
-    const int k = min(idx, sx * sy - 1), ii = k / sy, jj = k - ii * sy;
+    const int k = min(idx, sx * sy - 1), ii = (int)((uint32_t)k / (uint32_t)sy), jj = k - ii * sy;
     sym[0] = q[(size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj];
     return idx < sx * sy;
   }
