@@ -21,5 +21,6 @@ from .api import (compress_3d_learned_batch_dev, learn_analytic_eb_3d_dev, learn
                   learn_init_params_3d, learn_labels_3d_dev, learn_param_count_3d, learn_train_3d_dev,
                   mlp_forward_3d_dev, mlp_grad_3d_dev, tet_cp_emulate)
 from .api import check_fast_bounds  # noqa: F401
+from .api import set_exact_2d_bounds  # noqa: F401
 from .api import critical_points_2d_sos_dev, critical_points_3d_sos_dev, sos_det  # noqa: F401
 from .api import sz_lossless_compress_batch_dev, sz_lossless_compress_dev  # noqa: F401

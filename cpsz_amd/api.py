@@ -98,6 +98,7 @@ _SIGS = {
                                            C.POINTER(_SZ)],
     "cpsz_tet_cp_emulate": [_P, C.c_int, C.POINTER(C.c_int)],
     "cpsz_check_fast_bounds": [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
+    "cpsz_set_exact_2d_bounds": [C.c_int],
 }
 
 
@@ -631,6 +632,13 @@ def check_fast_bounds(device: int = 0):
     u, r = C.c_uint32(), C.c_double()
     _check(lib().cpsz_check_fast_bounds(device, C.byref(u), C.byref(r)))
     return u.value, r.value
+
+
+def set_exact_2d_bounds(on: bool) -> None:
+    """Process-wide choice of the 2D bounds kernel for float fields: False
+    (default) = FP32 approximations of the last FP64 steps with exact fix-ups,
+    True = every step in FP64. Both give identical streams."""
+    _check(lib().cpsz_set_exact_2d_bounds(int(bool(on))))
 
 
 def tet_cp_emulate(f, t: int) -> int:

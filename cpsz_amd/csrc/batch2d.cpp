@@ -76,7 +76,7 @@ void drop_graph(Graph2d& g) {
 }
 
 std::string graph_key(int device, const std::vector<Field2d>& f, double eb, int grid_cap) {
-  std::string k = std::to_string(device) + "|" + std::to_string(grid_cap);
+  std::string k = std::to_string(device) + "|" + std::to_string(grid_cap) + "|" + (exact_2d_bounds() ? "x" : "a");
   uint64_t bits;
   std::memcpy(&bits, &eb, 8);
   k += "|" + std::to_string(bits);

@@ -1141,6 +1141,10 @@ int cpsz_check_fast_bounds(int device, uint32_t* max_ulp, double* max_ratio) {
   });
 }
 
+int cpsz_set_exact_2d_bounds(int on) {
+  return guarded([&] { set_exact_2d_bounds(on != 0); });
+}
+
 int cpsz_learn_analytic_eb_3d_dev(int device, void* stream, const float* dU, const float* dV, const float* dW,
                                   size_t r1, size_t r2, size_t r3, double max_pwr_eb, double* dE) {
   return guarded([&] {

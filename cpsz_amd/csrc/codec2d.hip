@@ -20,6 +20,7 @@
 // reproduces the reference's double arithmetic exactly (the argument is in the
 // comments of lorenzo2d_step.hpp).
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstring>
@@ -150,6 +151,94 @@ __device__ double dtri(T u0, T u1, T u2, T v0, T v1, T v2, int lower) {
   return eb;
 }
 
+template <typename T>
+__device__ __noinline__ double dtri_exact(T u0, T u1, T u2, T v0, T v1, T v2, int lower) {
+  return dtri<T>(u0, u1, u2, v0, v1, v2, lower);
+}
+
+// ---- the same bound for float fields with the expensive steps approximated --
+// The reference's bound is a function of the six float products: it sums
+// them in double (det, the p/n sums of the sign bounds), divides det by three
+// of those sums, and takes |sqrt p - sqrt n| / (sqrt p + sqrt n) with
+// correctly rounded double square roots and divisions. Here the sums and the
+// branch decisions stay exact, and only the last step is approximated, in
+// FP32, as t ~ |p - n| / (p + n + 2 sqrt(p n)) from the exact p, n (relative
+// error below 2^-20). The reference's rounded value lies within 5 * 2^-53 of
+// the real t (absolute; t <= 1), so the true bound is enclosed in
+// [a (1 - 2^-16) - 2^-50, a (1 + 2^-16) + 2^-50] for an approximation a; a
+// vertex whose eb codes that enclosure does not fix takes the exact bound.
+constexpr float kExact1 = 2.0f;  // an exact bound of 1 (approximations stay below 1 + 2^-19)
+
+// products in [2^-60, 2^58] (or 0) keep every FP32 step below in the normal range
+__device__ __forceinline__ bool prod_ok(float f) {
+  const uint32_t e = __float_as_uint(f) & 0x7FFFFFFFu;
+  return e == 0 || (e >= 0x21800000u && e <= 0x5C800000u);
+}
+
+// |sqrt p - sqrt n| / (sqrt p + sqrt n) = |p - n| / (p + n + 2 sqrt(p n)); num = |p - n|
+__device__ __forceinline__ float approx_sign_bound(float p, float n, float num) {
+  const float s = __builtin_amdgcn_sqrtf(p * n);
+  return num * __builtin_amdgcn_rcpf((p + n) + 2.0f * s);
+}
+
+// max_eb_to_keep_sign_2d_offline_2(a, b): 1 unless the terms have opposite
+// signs, then p, n are the terms' magnitudes and p - n = a + b
+__device__ __forceinline__ float fbound2(float a, float b) {
+  const bool mixed = (a > 0 && b < 0) || (a < 0 && b > 0);
+  const float r = approx_sign_bound(fabsf(a), fabsf(b), fabsf(a + b));
+  return mixed ? r : kExact1;
+}
+
+// max_eb_to_keep_sign_2d_offline_4: p and n summed in the reference's order
+// (adding 0.0 to a non-negative sum leaves it unchanged)
+__device__ __forceinline__ float fbound4(double t1, double t2, double t3, double t4) {
+  double p = 0, n = 0;
+  p += t1 >= 0 ? t1 : 0.0;
+  n += t1 >= 0 ? 0.0 : -t1;
+  p += t2 >= 0 ? t2 : 0.0;
+  n += t2 >= 0 ? 0.0 : -t2;
+  p += t3 >= 0 ? t3 : 0.0;
+  n += t3 >= 0 ? 0.0 : -t3;
+  p += t4 >= 0 ? t4 : 0.0;
+  n += t4 >= 0 ? 0.0 : -t4;
+  const float r = approx_sign_bound((float)p, (float)n, (float)fabs(p - n));
+  return (p == 0 || n == 0) ? kExact1 : r;
+}
+
+// (det / d >= 1) in double: the rounded quotient reaches 1 exactly when
+// det / d >= 1 - 2^-54, and det - d (exact in sign) tells which side of 1 the
+// quotient is on; only quotients within 2^-50 below 1 (or d == 0) divide
+__device__ __forceinline__ bool ratio_ge1(double det, double d) {
+  const double t = det - d;
+  const bool ge = t == 0 || ((t > 0) == (d > 0));
+  if (d == 0 || (!ge && fabs(t) <= 0x1p-50 * fabs(d))) return det / d >= 1;
+  return ge;
+}
+
+// The cell table entry of one triangle: an exact bound (>= 0, or NaN) or -a
+// for an approximation a > 0 of it. Exact: det == 0, triangles with a
+// critical point (all three ratios >= 1) and products outside prod_ok's range.
+template <int LOWER>
+__device__ __forceinline__ double tri_fast(float u0, float u1, float u2, float v0, float v1, float v2) {
+  const float f01 = u0 * v1, f10 = u1 * v0, f02 = u0 * v2, f20 = u2 * v0, f12 = u1 * v2, f21 = u2 * v1;
+  const double u0v1 = f01, u1v0 = f10, u0v2 = f02, u2v0 = f20, u1v2 = f12, u2v1 = f21;
+  const double det = u0v1 - u1v0 + u1v2 - u2v1 + u2v0 - u0v2;
+  if (!(det != 0)) return 0;
+  const bool ok = prod_ok(f01) && prod_ok(f10) && prod_ok(f02) && prod_ok(f20) && prod_ok(f12) && prod_ok(f21);
+  bool f1 = true, f2 = true, f3 = true;
+  if (ok) {
+    f1 = ratio_ge1(det, u2v0 - u0v2);
+    f2 = ratio_ge1(det, u1v2 - u2v1);
+    f3 = ratio_ge1(det, u0v1 - u1v0);
+  }
+  if (f1 && f2 && f3) return dtri_exact<float>(u0, u1, u2, v0, v1, v2, LOWER);
+  const float m1 = fminf(fbound2(f20, -f02), fbound4(u0v1, -u1v0, u1v2, -u2v1));
+  const float m2 = fminf(fbound2(f12, -f21), fbound4(u0v1, -u1v0, u2v0, -u0v2));
+  const float m3 = fminf(fbound2(f01, -f10), fbound4(u1v2, -u2v1, u2v0, -u0v2));
+  const float r = fmaxf(fmaxf(f1 ? 0.0f : m1, f2 ? 0.0f : m2), f3 ? 0.0f : m3);
+  return r == kExact1 ? 1.0 : (r == 0 ? 0.0 : -(double)r);
+}
+
 // eb code: 0 = zero precision; c >= 1: reference id c-1 with precision
 // 4^(c-1) * DBL_EPSILON (id 0 keeps precision eps in the compressor).
 __device__ __forceinline__ uint32_t eb_code_2d(double x, const double* __restrict__ q2d, int nq) {
@@ -175,7 +264,9 @@ __device__ __forceinline__ uint32_t code_from_table(double x, const double* __re
 
 constexpr int kTH = 16, kTW = 32;  // vertex tile (17x33 cells: 9 wave rounds of triangles per 512 vertices)
 
-template <typename T>
+// APPROX (float fields without overrides or log bounds): the cell table holds
+// tri_fast entries and each vertex takes its codes from their enclosures.
+template <typename T, bool APPROX>
 __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __restrict__ V, int r1, int r2,
                                           double max_pwr_eb, const double* __restrict__ q2d, int nq,
                                           uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
@@ -206,10 +297,17 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     double up = INFINITY, lo = INFINITY;
     if (ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1) {
       // upper: (i,j),(i+1,j),(i+1,j+1); lower: (i,j),(i,j+1),(i+1,j+1)
-      up = dtri<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
-                   sv[cr + 1][cc + 1], 0);
-      lo = dtri<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
-                   sv[cr + 1][cc + 1], 1);
+      if constexpr (APPROX) {
+        up = tri_fast<0>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
+                         sv[cr + 1][cc + 1]);
+        lo = tri_fast<1>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
+                         sv[cr + 1][cc + 1]);
+      } else {
+        up = dtri<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
+                     sv[cr + 1][cc + 1], 0);
+        lo = dtri<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
+                     sv[cr + 1][cc + 1], 1);
+      }
     }
     tb[cr][cc][0] = up;
     tb[cr][cc][1] = lo;
@@ -220,20 +318,61 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     const int vr = k / kTW, vc = k % kTW;
     const int i = i0 + vr, j = j0 + vc;
     if (i >= r1 || j >= r2) continue;
+    const T u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     // incident triangles in the reference's visiting order (cells row-major, upper first)
     double eb = max_pwr_eb;
-    eb = dminf(eb, tb[vr][vc][0]);
-    eb = dminf(eb, tb[vr][vc][1]);
-    eb = dminf(eb, tb[vr][vc + 1][0]);
-    eb = dminf(eb, tb[vr + 1][vc][1]);
-    eb = dminf(eb, tb[vr + 1][vc + 1][0]);
-    eb = dminf(eb, tb[vr + 1][vc + 1][1]);
+    if constexpr (APPROX) {
+      // the bound's enclosure (a fold of dminf, monotone in every argument)
+      double lo = max_pwr_eb, hi = max_pwr_eb;
+      auto take = [&](double t) {
+        double l = t, h = t;
+        if (t < 0) {
+          l = -t * (1 - 0x1p-16) - 0x1p-50;
+          h = -t * (1 + 0x1p-16) + 0x1p-50;
+        }
+        lo = dminf(lo, l);
+        hi = dminf(hi, h);
+      };
+      take(tb[vr][vc][0]);
+      take(tb[vr][vc][1]);
+      take(tb[vr][vc + 1][0]);
+      take(tb[vr + 1][vc][1]);
+      take(tb[vr + 1][vc + 1][0]);
+      take(tb[vr + 1][vc + 1][1]);
+      const double ul = (double)fabs(u) * lo, uh = (double)fabs(u) * hi;
+      const double vl = (double)fabs(v) * lo, vh = (double)fabs(v) * hi;
+      if (lo == lo && hi == hi && uh < 0x1p972 && vh < 0x1p972 && eb_code_2d(ul, q2d, nq) == eb_code_2d(uh, q2d, nq) &&
+          eb_code_2d(vl, q2d, nq) == eb_code_2d(vh, q2d, nq)) {
+        eb = lo;  // every value of the enclosure gives these codes
+      } else {
+        auto ex = [&](int cr, int cc, int lower) -> double {
+          const double t = tb[cr][cc][lower];
+          if (!(t < 0)) return t;
+          return lower ? dtri_exact<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc],
+                                       sv[cr][cc + 1], sv[cr + 1][cc + 1], 1)
+                       : dtri_exact<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc],
+                                       sv[cr + 1][cc], sv[cr + 1][cc + 1], 0);
+        };
+        eb = dminf(eb, ex(vr, vc, 0));
+        eb = dminf(eb, ex(vr, vc, 1));
+        eb = dminf(eb, ex(vr, vc + 1, 0));
+        eb = dminf(eb, ex(vr + 1, vc, 1));
+        eb = dminf(eb, ex(vr + 1, vc + 1, 0));
+        eb = dminf(eb, ex(vr + 1, vc + 1, 1));
+      }
+    } else {
+      eb = dminf(eb, tb[vr][vc][0]);
+      eb = dminf(eb, tb[vr][vc][1]);
+      eb = dminf(eb, tb[vr][vc + 1][0]);
+      eb = dminf(eb, tb[vr + 1][vc][1]);
+      eb = dminf(eb, tb[vr + 1][vc + 1][0]);
+      eb = dminf(eb, tb[vr + 1][vc + 1][1]);
+    }
     const size_t vix = (size_t)i * r2 + j;
     if (ov.eb_out) ov.eb_out[vix] = eb;
     if (ov.gain && !(ov.fallback && ov.fallback[vix]) && ov.gain[vix])
       eb = dminf(max_pwr_eb, ldexp(eb, (int)ov.gain[vix]));
     if (ov.fallback && ov.fallback[vix] == 2) eb = 0;  // stored exactly
-    const T u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     if (!isfinite(u) || !isfinite(v) || !(eb == eb)) bad = 1;
     if (qlog) {
       // offline_log: one code per vertex for both components, from the bound
@@ -261,14 +400,15 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     if (sh_hist[k]) atomicAdd(&hist[k], sh_hist[k]);
 }
 
-template <typename T>
+template <typename T, bool APPROX>
 __global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* __restrict__ V,
                                               int r1, int r2, double max_pwr_eb,
                                               const double* __restrict__ q2d, int nq,
                                               uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
                                               uint32_t* __restrict__ hist, int* __restrict__ flags,
                                               EbOverride ov, const double* __restrict__ qlog, int nqlog) {
-  eb2d_tile<T>(U, V, r1, r2, max_pwr_eb, q2d, nq, cu, cv, hist, flags, ov, qlog, nqlog, blockIdx.x, blockIdx.y);
+  eb2d_tile<T, APPROX>(U, V, r1, r2, max_pwr_eb, q2d, nq, cu, cv, hist, flags, ov, qlog, nqlog, blockIdx.x,
+                       blockIdx.y);
 }
 
 // Every field of a batch in one launch (plain offline, float): a job per
@@ -285,6 +425,7 @@ struct Eb2dJob {
   uint32_t blk0;
 };
 
+template <bool APPROX>
 __global__ void __launch_bounds__(256) k_eb2d_multi(const Eb2dJob* __restrict__ jobs, int nj, double max_pwr_eb,
                                                     const double* __restrict__ q2d, int nq) {
   int lo = 0, hi = nj - 1;
@@ -295,8 +436,8 @@ __global__ void __launch_bounds__(256) k_eb2d_multi(const Eb2dJob* __restrict__ 
   }
   const Eb2dJob& J = jobs[lo];
   const uint32_t t = blockIdx.x - J.blk0;
-  eb2d_tile<float>(J.U, J.V, J.r1, J.r2, max_pwr_eb, q2d, nq, J.cu, J.cv, J.hist, J.flags, J.ov, nullptr, 0,
-                   (int)(t % (uint32_t)J.gx), (int)(t / (uint32_t)J.gx));
+  eb2d_tile<float, APPROX>(J.U, J.V, J.r1, J.r2, max_pwr_eb, q2d, nq, J.cu, J.cv, J.hist, J.flags, J.ov, nullptr,
+                           0, (int)(t % (uint32_t)J.gx), (int)(t / (uint32_t)J.gx));
 }
 
 // ------------------------------------------------------------ wavefront ----
@@ -1568,6 +1709,12 @@ void c2_bind(C2Work& w, uint8_t* p) {
   w.prog = (int*)take((2 * ns + 1) * 4);
 }
 
+namespace {
+std::atomic<bool> g_exact_bounds{false};
+}
+void set_exact_2d_bounds(bool on) { g_exact_bounds.store(on); }
+bool exact_2d_bounds() { return g_exact_bounds.load(); }
+
 void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
   CPSZ_REQUIRE((w.f64 ? w.dU64 && w.dV64 : w.dU && w.dV) && w.r1 >= 1 && w.r2 >= 1, kErrArg,
                "compress_2d: bad arguments");
@@ -1578,12 +1725,15 @@ void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
   CPSZ_HIP(hipMemsetAsync(w.flags, 0, 4 * sizeof(int), st));
   dim3 grid((unsigned)div_up(w.r2, kTW), (unsigned)div_up(w.r1, kTH));
   if (w.f64)
-    k_eb2d<double><<<grid, 256, 0, st>>>(w.dU64, w.dV64, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
-                                         ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
+    k_eb2d<double, false><<<grid, 256, 0, st>>>(w.dU64, w.dV64, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
+                                                ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
+  else if (!w.log && !has_override(ov) && !exact_2d_bounds())
+    k_eb2d<float, true><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
+                                              ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
   else
-    k_eb2d<float><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
-                                        w.cu, w.cv, w.hist, w.flags, ov, w.log ? ctx.d_q2l() : nullptr,
-                                        ctx.q2l_size());
+    k_eb2d<float, false><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
+                                               ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov,
+                                               w.log ? ctx.d_q2l() : nullptr, ctx.q2l_size());
   CPSZ_HIP(hipGetLastError());
 }
 
@@ -1597,6 +1747,7 @@ void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const
   int* flags = ctx.buf<int>("b2eb.flags", 4 * nf);
   std::vector<Eb2dJob> jobs(nf);
   uint32_t blk = 0;
+  bool approx = !exact_2d_bounds();
   for (size_t k = 0; k < nf; k++) {
     C2Work& W = *w[k];
     CPSZ_REQUIRE(W.dU && W.dV && W.r1 >= 1 && W.r2 >= 1 && !W.log && !W.f64, kErrArg, "compress_2d: bad arguments");
@@ -1616,6 +1767,7 @@ void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const
     J.hist = W.hist;
     J.flags = W.flags;
     J.ov = ov ? (*ov)[k] : EbOverride();
+    approx = approx && !has_override(J.ov);
     J.blk0 = blk;
     const size_t tiles = div_up(W.r2, kTW) * div_up(W.r1, kTH);
     CPSZ_REQUIRE(blk + tiles < (1ull << 31), kErrUnsupported, "2D batch: too many tiles in one call");
@@ -1628,7 +1780,10 @@ void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const
   ctx.upload_table("b2eb.jobs", dj, hj, nf * sizeof(Eb2dJob));
   k_fill_u32<<<1024, 256, 0, st>>>(hist, nf * HN, 0);
   k_fill_u32<<<(unsigned)div_up(4 * nf, 256), 256, 0, st>>>((uint32_t*)flags, 4 * nf, 0);
-  k_eb2d_multi<<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size());
+  if (approx)
+    k_eb2d_multi<true><<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size());
+  else
+    k_eb2d_multi<false><<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size());
   CPSZ_HIP(hipGetLastError());
 }
 

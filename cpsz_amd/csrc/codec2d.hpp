@@ -20,6 +20,14 @@ struct EbOverride {
   const uint8_t* fallback = nullptr;
   double* eb_out = nullptr;
 };
+inline bool has_override(const EbOverride& o) { return o.gain || o.fallback || o.eb_out; }
+
+// Bounds of float fields without overrides are computed with the last FP64
+// steps approximated in FP32 and exact fix-ups where the approximation does
+// not fix a vertex's codes; the switch selects the all-FP64 kernel instead
+// (process-wide; both give the same codes).
+void set_exact_2d_bounds(bool on);
+bool exact_2d_bounds();
 
 // U, V: device pointers to r1 x r2 row-major float fields.
 void compress_2d_device(Ctx& ctx, const float* dU, const float* dV, size_t r1, size_t r2,

@@ -419,6 +419,11 @@ int cpsz_compress_3d_learned_batch_dev(int device, void* stream, size_t nfields,
  * (the pass assumes <= 2); *max_ratio = largest |fast - exact| sign bound
  * divided by its error envelope over hashed argument pairs (<= 1 when sound). */
 int cpsz_check_fast_bounds(int device, uint32_t* max_ulp, double* max_ratio);
+/* Process-wide choice of the 2D bounds kernel for float fields (no reference
+ * counterpart; for tests and diagnostics): 0 (default) = FP32 approximations of
+ * the last FP64 steps with exact fix-ups, 1 = every step in FP64. Both give
+ * identical streams. */
+int cpsz_set_exact_2d_bounds(int on);
 /* Host evaluation of the exact tetrahedron predicate the device judge uses:
  * f = 4 vertex vectors (u, v, w) of tetrahedron t (0..5, the judge's order);
  * *type = -1 (no critical point) or the type code. */

@@ -49,6 +49,7 @@ def test_config2_batched_96_fields():
     assert nv == 220143645
     dev = [field_2d_device(r1, r2, seed=1000 + k) for k, (r1, r2) in enumerate(shapes)]
     outs = [(torch.empty_like(u), torch.empty_like(v)) for u, v in dev]
+    torch.cuda.synchronize()  # the library runs on its own stream
     base, off, size = cz.compress_2d_batch_dev([(u.data_ptr(), v.data_ptr(), r1, r2)
                                                 for (u, v), (r1, r2) in zip(dev, shapes)], 0.1)
     allb = _host(base, int(off[-1] + size[-1]))
@@ -95,6 +96,7 @@ def test_config2_batched_96_fields():
 @pytest.fixture(scope="module")
 def learned2d_params():
     train = [field_2d_device(1024, 1024, seed=5000 + k) for k in range(64)]
+    torch.cuda.synchronize()  # the library runs on its own stream
     p, losses = cz.learn_train_2d_dev([(u.data_ptr(), v.data_ptr(), 1024, 1024) for u, v in train], 0.1,
                                       cz.learn_init_params(1), samples_per_field=16384, epochs=8, batch=4096,
                                       lr=2e-3, seed=11)
@@ -132,6 +134,7 @@ def test_config4_learned_batched_3d_16x192():
     dev = [field_3d_device(*shape, seed=0, step=t) for t in range(nt)]
     fl = [(u.data_ptr(), v.data_ptr(), w.data_ptr(), *shape) for u, v, w in dev]
     outs = [tuple(torch.empty_like(x) for x in f) for f in dev]
+    torch.cuda.synchronize()  # the library runs on its own stream
     base, off, size = cz.compress_3d_batch_dev(fl, eb)
     allb = _host(base, int(off[-1] + size[-1]))
     orc = load_oracle()
@@ -142,6 +145,7 @@ def test_config4_learned_batched_3d_16x192():
     print("[config 4] plain batched streams equal the oracle's on timesteps 0 and 15", flush=True)
     z_plain = _zstd_bytes(allb, off, size)
     train = [field_3d_device(128, 128, 128, seed=7000 + k) for k in range(8)]
+    torch.cuda.synchronize()
     p, losses = cz.learn_train_3d_dev([(u.data_ptr(), v.data_ptr(), w.data_ptr(), 128, 128, 128)
                                        for u, v, w in train], eb, cz.learn_init_params_3d(1),
                                       samples_per_field=16384, epochs=8, batch=4096, lr=2e-3, seed=11)

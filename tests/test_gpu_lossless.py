@@ -50,6 +50,7 @@ def test_lossless_batch_on_compressed_streams():
     shapes = batch_2d_shapes(12, seed=4)
     dev = [field_2d_device(min(r1, 700), min(r2, 700), seed=30 + k) for k, (r1, r2) in enumerate(shapes)]
     fl = [(u.data_ptr(), v.data_ptr(), u.shape[0], u.shape[1]) for u, v in dev]
+    torch.cuda.synchronize()  # the library runs on its own stream
     base, off, size = cz.compress_2d_batch_dev(fl, 0.1)
     torch.cuda.synchronize()
     frames = cz.sz_lossless_compress_batch_dev(base, off, size, 3)
