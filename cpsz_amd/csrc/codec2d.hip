@@ -151,11 +151,6 @@ __device__ double dtri(T u0, T u1, T u2, T v0, T v1, T v2, int lower) {
   return eb;
 }
 
-template <typename T>
-__device__ __noinline__ double dtri_exact(T u0, T u1, T u2, T v0, T v1, T v2, int lower) {
-  return dtri<T>(u0, u1, u2, v0, v1, v2, lower);
-}
-
 // ---- the same bound for float fields with the expensive steps approximated --
 // The reference's bound is a function of the six float products: it sums
 // them in double (det, the p/n sums of the sign bounds), divides det by three
@@ -167,12 +162,26 @@ __device__ __noinline__ double dtri_exact(T u0, T u1, T u2, T v0, T v1, T v2, in
 // the real t (absolute; t <= 1), so the true bound is enclosed in
 // [a (1 - 2^-16) - 2^-50, a (1 + 2^-16) + 2^-50] for an approximation a; a
 // vertex whose eb codes that enclosure does not fix takes the exact bound.
-constexpr float kExact1 = 2.0f;  // an exact bound of 1 (approximations stay below 1 + 2^-19)
+//
+// A triangle's entry in the cell table (FP32):
+//   0            the exact bound 0
+//   kExact1      the exact bound 1
+//   +inf         no triangle (cell outside the field)
+//   kNeedExact   only the FP64 bound will do (critical-point triangles,
+//                products outside [2^-60, 2^58])
+//   otherwise    an approximation a in (2^-56, 1 + 2^-19]
+constexpr float kExact1 = 2.0f;
+constexpr float kNeedExact = -1.0f;
 
 // products in [2^-60, 2^58] (or 0) keep every FP32 step below in the normal range
 __device__ __forceinline__ bool prod_ok(float f) {
   const uint32_t e = __float_as_uint(f) & 0x7FFFFFFFu;
   return e == 0 || (e >= 0x21800000u && e <= 0x5C800000u);
+}
+// a vertex value in [2^-30, 2^29] (or 0): products of two such values pass prod_ok
+__device__ __forceinline__ bool value_ok(float f) {
+  const uint32_t e = __float_as_uint(f) & 0x7FFFFFFFu;
+  return e == 0 || (e >= 0x30800000u && e <= 0x4E000000u);
 }
 
 // |sqrt p - sqrt n| / (sqrt p + sqrt n) = |p - n| / (p + n + 2 sqrt(p n)); num = |p - n|
@@ -189,18 +198,9 @@ __device__ __forceinline__ float fbound2(float a, float b) {
   return mixed ? r : kExact1;
 }
 
-// max_eb_to_keep_sign_2d_offline_4: p and n summed in the reference's order
-// (adding 0.0 to a non-negative sum leaves it unchanged)
-__device__ __forceinline__ float fbound4(double t1, double t2, double t3, double t4) {
-  double p = 0, n = 0;
-  p += t1 >= 0 ? t1 : 0.0;
-  n += t1 >= 0 ? 0.0 : -t1;
-  p += t2 >= 0 ? t2 : 0.0;
-  n += t2 >= 0 ? 0.0 : -t2;
-  p += t3 >= 0 ? t3 : 0.0;
-  n += t3 >= 0 ? 0.0 : -t3;
-  p += t4 >= 0 ? t4 : 0.0;
-  n += t4 >= 0 ? 0.0 : -t4;
+// max_eb_to_keep_sign_2d_offline_4 from its p and n, summed in the reference's
+// order by the caller (a zero part adds nothing to a non-negative sum)
+__device__ __forceinline__ float fbound4(double p, double n) {
   const float r = approx_sign_bound((float)p, (float)n, (float)fabs(p - n));
   return (p == 0 || n == 0) ? kExact1 : r;
 }
@@ -215,28 +215,28 @@ __device__ __forceinline__ bool ratio_ge1(double det, double d) {
   return ge;
 }
 
-// The cell table entry of one triangle: an exact bound (>= 0, or NaN) or -a
-// for an approximation a > 0 of it. Exact: det == 0, triangles with a
-// critical point (all three ratios >= 1) and products outside prod_ok's range.
-template <int LOWER>
-__device__ __forceinline__ double tri_fast(float u0, float u1, float u2, float v0, float v1, float v2) {
+// The cell table entry of one triangle (vertex order as dtri's). checked: the
+// tile's values all pass value_ok, so the products need no check.
+__device__ __forceinline__ float tri_fast(float u0, float u1, float u2, float v0, float v1, float v2, bool checked) {
   const float f01 = u0 * v1, f10 = u1 * v0, f02 = u0 * v2, f20 = u2 * v0, f12 = u1 * v2, f21 = u2 * v1;
   const double u0v1 = f01, u1v0 = f10, u0v2 = f02, u2v0 = f20, u1v2 = f12, u2v1 = f21;
   const double det = u0v1 - u1v0 + u1v2 - u2v1 + u2v0 - u0v2;
-  if (!(det != 0)) return 0;
-  const bool ok = prod_ok(f01) && prod_ok(f10) && prod_ok(f02) && prod_ok(f20) && prod_ok(f12) && prod_ok(f21);
-  bool f1 = true, f2 = true, f3 = true;
-  if (ok) {
-    f1 = ratio_ge1(det, u2v0 - u0v2);
-    f2 = ratio_ge1(det, u1v2 - u2v1);
-    f3 = ratio_ge1(det, u0v1 - u1v0);
-  }
-  if (f1 && f2 && f3) return dtri_exact<float>(u0, u1, u2, v0, v1, v2, LOWER);
-  const float m1 = fminf(fbound2(f20, -f02), fbound4(u0v1, -u1v0, u1v2, -u2v1));
-  const float m2 = fminf(fbound2(f12, -f21), fbound4(u0v1, -u1v0, u2v0, -u0v2));
-  const float m3 = fminf(fbound2(f01, -f10), fbound4(u1v2, -u2v1, u2v0, -u0v2));
-  const float r = fmaxf(fmaxf(f1 ? 0.0f : m1, f2 ? 0.0f : m2), f3 ? 0.0f : m3);
-  return r == kExact1 ? 1.0 : (r == 0 ? 0.0 : -(double)r);
+  if (!(det != 0)) return 0.0f;
+  if (!checked && !(prod_ok(f01) && prod_ok(f10) && prod_ok(f02) && prod_ok(f20) && prod_ok(f12) && prod_ok(f21)))
+    return kNeedExact;
+  const bool f1 = ratio_ge1(det, u2v0 - u0v2);
+  const bool f2 = ratio_ge1(det, u1v2 - u2v1);
+  const bool f3 = ratio_ge1(det, u0v1 - u1v0);
+  if (f1 && f2 && f3) return kNeedExact;
+  // positive and negative parts of the signed terms u0v1, -u1v0 | u1v2, -u2v1 | u2v0, -u0v2
+  const double p0 = fmax(u0v1, 0.0), n0 = fmax(-u0v1, 0.0), p1 = fmax(-u1v0, 0.0), n1 = fmax(u1v0, 0.0);
+  const double p2 = fmax(u1v2, 0.0), n2 = fmax(-u1v2, 0.0), p3 = fmax(-u2v1, 0.0), n3 = fmax(u2v1, 0.0);
+  const double p4 = fmax(u2v0, 0.0), n4 = fmax(-u2v0, 0.0), p5 = fmax(-u0v2, 0.0), n5 = fmax(u0v2, 0.0);
+  const double pa = p0 + p1, na = n0 + n1, pb = p2 + p3, nb = n2 + n3;
+  const float m1 = fminf(fbound2(f20, -f02), fbound4((pa + p2) + p3, (na + n2) + n3));
+  const float m2 = fminf(fbound2(f12, -f21), fbound4((pa + p4) + p5, (na + n4) + n5));
+  const float m3 = fminf(fbound2(f01, -f10), fbound4((pb + p4) + p5, (nb + n4) + n5));
+  return fmaxf(fmaxf(f1 ? 0.0f : m1, f2 ? 0.0f : m2), f3 ? 0.0f : m3);
 }
 
 // eb code: 0 = zero precision; c >= 1: reference id c-1 with precision
@@ -264,9 +264,7 @@ __device__ __forceinline__ uint32_t code_from_table(double x, const double* __re
 
 constexpr int kTH = 16, kTW = 32;  // vertex tile (17x33 cells: 9 wave rounds of triangles per 512 vertices)
 
-// APPROX (float fields without overrides or log bounds): the cell table holds
-// tri_fast entries and each vertex takes its codes from their enclosures.
-template <typename T, bool APPROX>
+template <typename T>
 __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __restrict__ V, int r1, int r2,
                                           double max_pwr_eb, const double* __restrict__ q2d, int nq,
                                           uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
@@ -297,17 +295,10 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     double up = INFINITY, lo = INFINITY;
     if (ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1) {
       // upper: (i,j),(i+1,j),(i+1,j+1); lower: (i,j),(i,j+1),(i+1,j+1)
-      if constexpr (APPROX) {
-        up = tri_fast<0>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
-                         sv[cr + 1][cc + 1]);
-        lo = tri_fast<1>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
-                         sv[cr + 1][cc + 1]);
-      } else {
-        up = dtri<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
-                     sv[cr + 1][cc + 1], 0);
-        lo = dtri<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
-                     sv[cr + 1][cc + 1], 1);
-      }
+      up = dtri<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc], sv[cr + 1][cc],
+                   sv[cr + 1][cc + 1], 0);
+      lo = dtri<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc], sv[cr][cc + 1],
+                   sv[cr + 1][cc + 1], 1);
     }
     tb[cr][cc][0] = up;
     tb[cr][cc][1] = lo;
@@ -321,53 +312,12 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     const T u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
     // incident triangles in the reference's visiting order (cells row-major, upper first)
     double eb = max_pwr_eb;
-    if constexpr (APPROX) {
-      // the bound's enclosure (a fold of dminf, monotone in every argument)
-      double lo = max_pwr_eb, hi = max_pwr_eb;
-      auto take = [&](double t) {
-        double l = t, h = t;
-        if (t < 0) {
-          l = -t * (1 - 0x1p-16) - 0x1p-50;
-          h = -t * (1 + 0x1p-16) + 0x1p-50;
-        }
-        lo = dminf(lo, l);
-        hi = dminf(hi, h);
-      };
-      take(tb[vr][vc][0]);
-      take(tb[vr][vc][1]);
-      take(tb[vr][vc + 1][0]);
-      take(tb[vr + 1][vc][1]);
-      take(tb[vr + 1][vc + 1][0]);
-      take(tb[vr + 1][vc + 1][1]);
-      const double ul = (double)fabs(u) * lo, uh = (double)fabs(u) * hi;
-      const double vl = (double)fabs(v) * lo, vh = (double)fabs(v) * hi;
-      if (lo == lo && hi == hi && uh < 0x1p972 && vh < 0x1p972 && eb_code_2d(ul, q2d, nq) == eb_code_2d(uh, q2d, nq) &&
-          eb_code_2d(vl, q2d, nq) == eb_code_2d(vh, q2d, nq)) {
-        eb = lo;  // every value of the enclosure gives these codes
-      } else {
-        auto ex = [&](int cr, int cc, int lower) -> double {
-          const double t = tb[cr][cc][lower];
-          if (!(t < 0)) return t;
-          return lower ? dtri_exact<T>(su[cr][cc], su[cr][cc + 1], su[cr + 1][cc + 1], sv[cr][cc],
-                                       sv[cr][cc + 1], sv[cr + 1][cc + 1], 1)
-                       : dtri_exact<T>(su[cr][cc], su[cr + 1][cc], su[cr + 1][cc + 1], sv[cr][cc],
-                                       sv[cr + 1][cc], sv[cr + 1][cc + 1], 0);
-        };
-        eb = dminf(eb, ex(vr, vc, 0));
-        eb = dminf(eb, ex(vr, vc, 1));
-        eb = dminf(eb, ex(vr, vc + 1, 0));
-        eb = dminf(eb, ex(vr + 1, vc, 1));
-        eb = dminf(eb, ex(vr + 1, vc + 1, 0));
-        eb = dminf(eb, ex(vr + 1, vc + 1, 1));
-      }
-    } else {
-      eb = dminf(eb, tb[vr][vc][0]);
-      eb = dminf(eb, tb[vr][vc][1]);
-      eb = dminf(eb, tb[vr][vc + 1][0]);
-      eb = dminf(eb, tb[vr + 1][vc][1]);
-      eb = dminf(eb, tb[vr + 1][vc + 1][0]);
-      eb = dminf(eb, tb[vr + 1][vc + 1][1]);
-    }
+    eb = dminf(eb, tb[vr][vc][0]);
+    eb = dminf(eb, tb[vr][vc][1]);
+    eb = dminf(eb, tb[vr][vc + 1][0]);
+    eb = dminf(eb, tb[vr + 1][vc][1]);
+    eb = dminf(eb, tb[vr + 1][vc + 1][0]);
+    eb = dminf(eb, tb[vr + 1][vc + 1][1]);
     const size_t vix = (size_t)i * r2 + j;
     if (ov.eb_out) ov.eb_out[vix] = eb;
     if (ov.gain && !(ov.fallback && ov.fallback[vix]) && ov.gain[vix])
@@ -400,19 +350,20 @@ __device__ __forceinline__ void eb2d_tile(const T* __restrict__ U, const T* __re
     if (sh_hist[k]) atomicAdd(&hist[k], sh_hist[k]);
 }
 
-template <typename T, bool APPROX>
-__global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* __restrict__ V,
-                                              int r1, int r2, double max_pwr_eb,
-                                              const double* __restrict__ q2d, int nq,
-                                              uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
-                                              uint32_t* __restrict__ hist, int* __restrict__ flags,
-                                              EbOverride ov, const double* __restrict__ qlog, int nqlog) {
-  eb2d_tile<T, APPROX>(U, V, r1, r2, max_pwr_eb, q2d, nq, cu, cv, hist, flags, ov, qlog, nqlog, blockIdx.x,
-                       blockIdx.y);
+// code(x) >= c for the eb code function above (c = code(y) for some y >= x):
+// the codes step at eps and at the thresholds q2d[1..]
+__device__ __forceinline__ bool code_reaches(uint32_t c, double x, const double* __restrict__ q2d) {
+  if (c == 0) return true;
+  if (c == 1) return x > 2.220446049250313080847263336181640625e-16;
+  return x >= q2d[c - 1];
 }
 
-// Every field of a batch in one launch (plain offline, float): a job per
-// field, its tiles numbered from blk0 row by row.
+// Marks, in cu, a vertex whose bound the fix-up pass computes in FP64 (eb
+// codes stay far below).
+constexpr uint16_t kFixCode = 0xFFFF;
+
+// Every field of a batch in one launch (float fields): a job per field, its
+// tiles numbered from blk0 row by row.
 struct Eb2dJob {
   const float* U;
   const float* V;
@@ -425,19 +376,187 @@ struct Eb2dJob {
   uint32_t blk0;
 };
 
-template <bool APPROX>
-__global__ void __launch_bounds__(256) k_eb2d_multi(const Eb2dJob* __restrict__ jobs, int nj, double max_pwr_eb,
-                                                    const double* __restrict__ q2d, int nq) {
+__device__ __forceinline__ const Eb2dJob& eb_job_of(const Eb2dJob* __restrict__ jobs, int nj, uint32_t tile) {
   int lo = 0, hi = nj - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].blk0 <= blockIdx.x) lo = mid;
+    if (jobs[mid].blk0 <= tile) lo = mid;
     else hi = mid - 1;
   }
-  const Eb2dJob& J = jobs[lo];
+  return jobs[lo];
+}
+
+// eb2d_tile for float fields without overrides or log bounds: the cell table
+// holds tri_fast entries and each vertex takes its eb codes from the
+// enclosure of its bound. Vertices that need the FP64 bound get kFixCode, and
+// the tile goes on the fix-up list (fix_tiles, counted by fix_n) for
+// k_eb2d_fix, which also counts them in the histogram.
+__device__ __forceinline__ void eb2d_tile_fast(const float* __restrict__ U, const float* __restrict__ V, int r1,
+                                               int r2, double max_pwr_eb, const double* __restrict__ q2d, int nq,
+                                               uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
+                                               uint32_t* __restrict__ hist, int* __restrict__ flags, int bx, int by,
+                                               uint32_t tile, uint32_t* __restrict__ fix_tiles,
+                                               uint32_t* __restrict__ fix_n) {
+  __shared__ float su[kTH + 2][kTW + 2];
+  __shared__ float sv[kTH + 2][kTW + 2];
+  __shared__ float tb[kTH + 1][kTW + 1][2];
+  __shared__ uint32_t sh_hist[kMaxEbCode];
+  __shared__ int sh_fix;
+  const int tid = threadIdx.x;
+  const int i0 = by * kTH, j0 = bx * kTW;
+  for (int k = tid; k < kMaxEbCode; k += 256) sh_hist[k] = 0;
+  if (tid == 0) sh_fix = 0;
+  int in_range = 1;
+  for (int k = tid; k < (kTH + 2) * (kTW + 2); k += 256) {
+    const int r = k / (kTW + 2), c = k % (kTW + 2);
+    const int gi = i0 - 1 + r, gj = j0 - 1 + c;
+    float a = 0, b = 0;
+    if (gi >= 0 && gi < r1 && gj >= 0 && gj < r2) {
+      a = U[(size_t)gi * r2 + gj];
+      b = V[(size_t)gi * r2 + gj];
+    }
+    su[r][c] = a;
+    sv[r][c] = b;
+    in_range &= (int)(value_ok(a) && value_ok(b));
+  }
+  const bool checked = __syncthreads_and(in_range) != 0;
+  // one triangle per item: upper (i,j),(i+1,j),(i+1,j+1); lower (i,j),(i,j+1),(i+1,j+1)
+  for (int k = tid; k < 2 * (kTH + 1) * (kTW + 1); k += 256) {
+    const int cell = k >> 1, lower = k & 1;
+    const int cr = cell / (kTW + 1), cc = cell % (kTW + 1);
+    const int ci = i0 - 1 + cr, cj = j0 - 1 + cc;
+    float t = INFINITY;
+    if (ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1) {
+      const int mr = lower ? cr : cr + 1, mc = lower ? cc + 1 : cc;
+      t = tri_fast(su[cr][cc], su[mr][mc], su[cr + 1][cc + 1], sv[cr][cc], sv[mr][mc], sv[cr + 1][cc + 1], checked);
+    }
+    tb[cr][cc][lower] = t;
+  }
+  __syncthreads();
+  int bad = 0, fix = 0;
+  for (int k = tid; k < kTH * kTW; k += 256) {
+    const int vr = k / kTW, vc = k % kTW;
+    const int i = i0 + vr, j = j0 + vc;
+    if (i >= r1 || j >= r2) continue;
+    const float u = su[vr + 1][vc + 1], v = sv[vr + 1][vc + 1];
+    // the six incident triangles (for the reference, in the order cells
+    // row-major, upper first; the order matters only through NaN bounds,
+    // which go to the fix-up pass)
+    const float m = fminf(fminf(fminf(tb[vr][vc][0], tb[vr][vc][1]), fminf(tb[vr][vc + 1][0], tb[vr + 1][vc][1])),
+                          fminf(tb[vr + 1][vc + 1][0], tb[vr + 1][vc + 1][1]));
+    uint32_t a, b;
+    if (m > 0 && m < kExact1) {
+      // the smallest bound is an approximation: exact 1s and missing cells do not matter
+      const double am = m;
+      const double lo = dminf(max_pwr_eb, am * (1 - 0x1p-16) - 0x1p-50);
+      const double hi = dminf(max_pwr_eb, am * (1 + 0x1p-16) + 0x1p-50);
+      const double uh = (double)fabsf(u) * hi, vh = (double)fabsf(v) * hi;
+      a = eb_code_2d(uh, q2d, nq);
+      b = eb_code_2d(vh, q2d, nq);
+      if (!(uh < 0x1p972 && vh < 0x1p972 && code_reaches(a, (double)fabsf(u) * lo, q2d) &&
+            code_reaches(b, (double)fabsf(v) * lo, q2d))) {
+        cu[(size_t)i * r2 + j] = kFixCode;
+        fix = 1;
+        continue;
+      }
+    } else if (m >= 0) {
+      // exact: 0, 1 or no triangle
+      const double eb = m == 0 ? 0.0 : (m == kExact1 ? dminf(max_pwr_eb, 1.0) : max_pwr_eb);
+      // eb_u = fabs(U[i]) * eb[i] (src/sz_compress_cp_preserve_2d.cpp:231); the
+      // reference's eb/DBL_EPSILON overflows from 2^972 on (undefined id)
+      const double au = (double)fabsf(u) * eb, av = (double)fabsf(v) * eb;
+      if (au >= 0x1p972 || av >= 0x1p972 || !isfinite(u) || !isfinite(v)) bad = 1;
+      a = eb_code_2d(au, q2d, nq);
+      b = eb_code_2d(av, q2d, nq);
+    } else {
+      cu[(size_t)i * r2 + j] = kFixCode;
+      fix = 1;
+      continue;
+    }
+    cu[(size_t)i * r2 + j] = (uint16_t)a;
+    cv[(size_t)i * r2 + j] = (uint16_t)b;
+    atomicAdd(&sh_hist[a ? a - 1 : 0], 1u);
+    atomicAdd(&sh_hist[b ? b - 1 : 0], 1u);
+    if (a > 82 || b > 82) atomicOr(&flags[1], 1);  // beyond the FP32 Lorenzo path
+  }
+  if (bad) atomicOr(&flags[0], 1);
+  if (fix) sh_fix = 1;
+  __syncthreads();
+  for (int k = tid; k < kMaxEbCode; k += 256)
+    if (sh_hist[k]) atomicAdd(&hist[k], sh_hist[k]);
+  if (tid == 0 && sh_fix) fix_tiles[atomicAdd(fix_n, 1u)] = tile;
+}
+
+// The FP64 bounds of the vertices eb2d_tile_fast marked, tile by tile from the
+// fix-up list: the six incident triangles in the reference's order, from
+// global memory (missing cells contribute +inf, as in eb2d_tile).
+__global__ void __launch_bounds__(256) k_eb2d_fix(const Eb2dJob* __restrict__ jobs, int nj,
+                                                  const uint32_t* __restrict__ fix_tiles,
+                                                  const uint32_t* __restrict__ fix_n, double max_pwr_eb,
+                                                  const double* __restrict__ q2d, int nq) {
+  const uint32_t n = *fix_n;
+  for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+    const uint32_t tile = fix_tiles[e];
+    const Eb2dJob& J = eb_job_of(jobs, nj, tile);
+    const uint32_t t = tile - J.blk0;
+    const int i0 = (int)(t / (uint32_t)J.gx) * kTH, j0 = (int)(t % (uint32_t)J.gx) * kTW;
+    const int r1 = J.r1, r2 = J.r2;
+    int bad = 0;
+    for (int k = threadIdx.x; k < kTH * kTW; k += blockDim.x) {
+      const int i = i0 + k / kTW, j = j0 + k % kTW;
+      if (i >= r1 || j >= r2) continue;
+      const size_t vix = (size_t)i * r2 + j;
+      if (J.cu[vix] != kFixCode) continue;
+      auto tri = [&](int ci, int cj, int lower) -> double {
+        if (ci < 0 || ci >= r1 - 1 || cj < 0 || cj >= r2 - 1) return INFINITY;
+        const size_t c0 = (size_t)ci * r2 + cj, c2 = c0 + r2 + 1, c1 = lower ? c0 + 1 : c0 + r2;
+        return dtri<float>(J.U[c0], J.U[c1], J.U[c2], J.V[c0], J.V[c1], J.V[c2], lower);
+      };
+      double eb = max_pwr_eb;
+      eb = dminf(eb, tri(i - 1, j - 1, 0));
+      eb = dminf(eb, tri(i - 1, j - 1, 1));
+      eb = dminf(eb, tri(i - 1, j, 0));
+      eb = dminf(eb, tri(i, j - 1, 1));
+      eb = dminf(eb, tri(i, j, 0));
+      eb = dminf(eb, tri(i, j, 1));
+      const float u = J.U[vix], v = J.V[vix];
+      const double au = (double)fabsf(u) * eb, av = (double)fabsf(v) * eb;
+      if (au >= 0x1p972 || av >= 0x1p972 || !(eb == eb) || !isfinite(u) || !isfinite(v)) bad = 1;
+      const uint32_t a = eb_code_2d(au, q2d, nq), b = eb_code_2d(av, q2d, nq);
+      J.cu[vix] = (uint16_t)a;
+      J.cv[vix] = (uint16_t)b;
+      atomicAdd(&J.hist[a ? a - 1 : 0], 1u);
+      atomicAdd(&J.hist[b ? b - 1 : 0], 1u);
+      if (a > 82 || b > 82) atomicOr(&J.flags[1], 1);
+    }
+    if (bad) atomicOr(&J.flags[0], 1);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_eb2d(const T* __restrict__ U, const T* __restrict__ V,
+                                              int r1, int r2, double max_pwr_eb,
+                                              const double* __restrict__ q2d, int nq,
+                                              uint16_t* __restrict__ cu, uint16_t* __restrict__ cv,
+                                              uint32_t* __restrict__ hist, int* __restrict__ flags,
+                                              EbOverride ov, const double* __restrict__ qlog, int nqlog) {
+  eb2d_tile<T>(U, V, r1, r2, max_pwr_eb, q2d, nq, cu, cv, hist, flags, ov, qlog, nqlog, blockIdx.x, blockIdx.y);
+}
+
+// APPROX: eb2d_tile_fast (jobs without overrides), with its fix-up list
+template <bool APPROX>
+__global__ void __launch_bounds__(256) k_eb2d_multi(const Eb2dJob* __restrict__ jobs, int nj, double max_pwr_eb,
+                                                    const double* __restrict__ q2d, int nq,
+                                                    uint32_t* __restrict__ fix_tiles, uint32_t* __restrict__ fix_n) {
+  const Eb2dJob& J = eb_job_of(jobs, nj, blockIdx.x);
   const uint32_t t = blockIdx.x - J.blk0;
-  eb2d_tile<float, APPROX>(J.U, J.V, J.r1, J.r2, max_pwr_eb, q2d, nq, J.cu, J.cv, J.hist, J.flags, J.ov, nullptr,
-                           0, (int)(t % (uint32_t)J.gx), (int)(t / (uint32_t)J.gx));
+  const int bx = (int)(t % (uint32_t)J.gx), by = (int)(t / (uint32_t)J.gx);
+  if constexpr (APPROX)
+    eb2d_tile_fast(J.U, J.V, J.r1, J.r2, max_pwr_eb, q2d, nq, J.cu, J.cv, J.hist, J.flags, bx, by, blockIdx.x,
+                   fix_tiles, fix_n);
+  else
+    eb2d_tile<float>(J.U, J.V, J.r1, J.r2, max_pwr_eb, q2d, nq, J.cu, J.cv, J.hist, J.flags, J.ov, nullptr, 0, bx,
+                     by);
 }
 
 // ------------------------------------------------------------ wavefront ----
@@ -1715,6 +1834,20 @@ std::atomic<bool> g_exact_bounds{false};
 void set_exact_2d_bounds(bool on) { g_exact_bounds.store(on); }
 bool exact_2d_bounds() { return g_exact_bounds.load(); }
 
+namespace {
+// The fast bounds pass over a job table already on its way to the device
+// (blk tiles), then the FP64 fix-up of the vertices it marked. Graph-capture
+// safe: the list counter is cleared by a kernel and nothing waits.
+void launch_eb_fast(Ctx& ctx, const Eb2dJob* dj, int nj, uint32_t blk, double max_pwr_eb) {
+  hipStream_t st = ctx.stream();
+  uint32_t* fx = ctx.buf<uint32_t>("eb.fix", (size_t)blk + 1);  // [0] count, [1..] tiles
+  k_fill_u32<<<1, 64, 0, st>>>(fx, 1, 0);
+  k_eb2d_multi<true><<<blk, 256, 0, st>>>(dj, nj, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(), fx + 1, fx);
+  k_eb2d_fix<<<std::min<uint32_t>(blk, 1024), 256, 0, st>>>(dj, nj, fx + 1, fx, max_pwr_eb, ctx.d_q2d(),
+                                                            ctx.q2d_size());
+}
+}  // namespace
+
 void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
   CPSZ_REQUIRE((w.f64 ? w.dU64 && w.dV64 : w.dU && w.dV) && w.r1 >= 1 && w.r2 >= 1, kErrArg,
                "compress_2d: bad arguments");
@@ -1724,16 +1857,33 @@ void c2_eb(Ctx& ctx, C2Work& w, double max_pwr_eb, const EbOverride& ov) {
   CPSZ_HIP(hipMemsetAsync(w.hist, 0, (kMaxEbCode + 2 * 65536) * sizeof(uint32_t), st));
   CPSZ_HIP(hipMemsetAsync(w.flags, 0, 4 * sizeof(int), st));
   dim3 grid((unsigned)div_up(w.r2, kTW), (unsigned)div_up(w.r1, kTH));
-  if (w.f64)
-    k_eb2d<double, false><<<grid, 256, 0, st>>>(w.dU64, w.dV64, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
-                                                ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
-  else if (!w.log && !has_override(ov) && !exact_2d_bounds())
-    k_eb2d<float, true><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
-                                              ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
-  else
-    k_eb2d<float, false><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
-                                               ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov,
-                                               w.log ? ctx.d_q2l() : nullptr, ctx.q2l_size());
+  if (w.f64) {
+    k_eb2d<double><<<grid, 256, 0, st>>>(w.dU64, w.dV64, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(),
+                                         ctx.q2d_size(), w.cu, w.cv, w.hist, w.flags, ov, nullptr, 0);
+  } else if (!w.log && !has_override(ov) && !exact_2d_bounds()) {
+    Eb2dJob J{};
+    J.U = w.dU;
+    J.V = w.dV;
+    J.r1 = (int)w.r1;
+    J.r2 = (int)w.r2;
+    J.gx = (int)grid.x;
+    J.cu = w.cu;
+    J.cv = w.cv;
+    J.hist = w.hist;
+    J.flags = w.flags;
+    J.blk0 = 0;
+    const size_t tiles = (size_t)grid.x * grid.y;
+    CPSZ_REQUIRE(tiles < (1ull << 31), kErrUnsupported, "compress_2d: field too large");
+    Eb2dJob* dj = ctx.buf<Eb2dJob>("c2eb.job", 1);
+    Eb2dJob* hj = (Eb2dJob*)ctx.host_table("c2eb.job", sizeof(Eb2dJob));
+    *hj = J;
+    ctx.upload_table("c2eb.job", dj, hj, sizeof(Eb2dJob));
+    launch_eb_fast(ctx, dj, 1, (uint32_t)tiles, max_pwr_eb);
+  } else {
+    k_eb2d<float><<<grid, 256, 0, st>>>(w.dU, w.dV, (int)w.r1, (int)w.r2, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(),
+                                        w.cu, w.cv, w.hist, w.flags, ov, w.log ? ctx.d_q2l() : nullptr,
+                                        ctx.q2l_size());
+  }
   CPSZ_HIP(hipGetLastError());
 }
 
@@ -1781,9 +1931,9 @@ void c2_eb_batch(Ctx& ctx, C2Work* const* w, size_t nf, double max_pwr_eb, const
   k_fill_u32<<<1024, 256, 0, st>>>(hist, nf * HN, 0);
   k_fill_u32<<<(unsigned)div_up(4 * nf, 256), 256, 0, st>>>((uint32_t*)flags, 4 * nf, 0);
   if (approx)
-    k_eb2d_multi<true><<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size());
+    launch_eb_fast(ctx, dj, (int)nf, blk, max_pwr_eb);
   else
-    k_eb2d_multi<false><<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size());
+    k_eb2d_multi<false><<<blk, 256, 0, st>>>(dj, (int)nf, max_pwr_eb, q2d, ctx.q2d_size(), nullptr, nullptr);
   CPSZ_HIP(hipGetLastError());
 }
 
