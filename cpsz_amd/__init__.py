@@ -22,5 +22,6 @@ from .api import (compress_3d_learned_batch_dev, learn_analytic_eb_3d_dev, learn
                   mlp_forward_3d_dev, mlp_grad_3d_dev, tet_cp_emulate)
 from .api import check_fast_bounds  # noqa: F401
 from .api import set_exact_2d_bounds  # noqa: F401
+from .api import ratio_ge1_host, tri_bound_2d_fast_host  # noqa: F401
 from .api import critical_points_2d_sos_dev, critical_points_3d_sos_dev, sos_det  # noqa: F401
 from .api import sz_lossless_compress_batch_dev, sz_lossless_compress_dev  # noqa: F401

@@ -99,6 +99,8 @@ _SIGS = {
     "cpsz_tet_cp_emulate": [_P, C.c_int, C.POINTER(C.c_int)],
     "cpsz_check_fast_bounds": [C.c_int, C.POINTER(C.c_uint32), C.POINTER(C.c_double)],
     "cpsz_set_exact_2d_bounds": [C.c_int],
+    "cpsz_tri_bound_2d_fast_host": [_P, _SZ, C.c_int, _P],
+    "cpsz_ratio_ge1_host": [_P, _P, _SZ, _P],
 }
 
 
@@ -639,6 +641,26 @@ def set_exact_2d_bounds(on: bool) -> None:
     (default) = FP32 approximations of the last FP64 steps with exact fix-ups,
     True = every step in FP64. Both give identical streams."""
     _check(lib().cpsz_set_exact_2d_bounds(int(bool(on))))
+
+
+def tri_bound_2d_fast_host(uv6, ulps: int = 0):
+    """Host evaluation of the approximate 2D bound's cell-table entries for
+    triangles uv6 (n x 6: u0, u1, u2, v0, v1, v2), square roots and reciprocals
+    moved by up to `ulps` ulp: 0 / 2.0 = exact 0 / 1, -1 = exact bound needed,
+    else an approximation."""
+    uv6 = np.ascontiguousarray(uv6, np.float32).reshape(-1, 6)
+    out = np.empty(uv6.shape[0], np.float32)
+    _check(lib().cpsz_tri_bound_2d_fast_host(_ptr(uv6), uv6.shape[0], int(ulps), _ptr(out)))
+    return out
+
+
+def ratio_ge1_host(det, d):
+    """Host evaluation of the bounds pass's division-free (det / d >= 1)."""
+    det = np.ascontiguousarray(det, np.float64)
+    d = np.ascontiguousarray(d, np.float64)
+    out = np.empty(det.size, np.uint8)
+    _check(lib().cpsz_ratio_ge1_host(_ptr(det), _ptr(d), det.size, _ptr(out)))
+    return out.astype(bool)
 
 
 def tet_cp_emulate(f, t: int) -> int:

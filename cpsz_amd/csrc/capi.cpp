@@ -21,6 +21,7 @@
 #include "cp.hpp"
 #include "cp_math.hpp"
 #include "ctx.hpp"
+#include "bounds2d_fast.hpp"
 #include "eb3d_math.hpp"
 #include "huff_gpu.hpp"
 #include "huff_host.hpp"
@@ -1138,6 +1139,48 @@ int cpsz_check_fast_bounds(int device, uint32_t* max_ulp, double* max_ratio) {
     CPSZ_REQUIRE(max_ulp && max_ratio, kErrArg, "null argument");
     Ctx& c = ctx_of(device);
     check_fast_bounds(c, max_ulp, max_ratio);
+  });
+}
+
+namespace {
+// correctly rounded square root and reciprocal moved by up to g_ulps units in
+// the last place, from a hash of a running count
+thread_local uint64_t g_perturb_n = 0;
+thread_local int g_ulps = 0;
+float perturb(float x) {
+  if (!g_ulps || !(x > 0) || !std::isfinite(x)) return x;
+  uint64_t h = (g_perturb_n++ + 1) * 0x9E3779B97F4A7C15ull;
+  h ^= h >> 31;
+  const int d = (int)(h % (uint64_t)(2 * g_ulps + 1)) - g_ulps;
+  uint32_t u;
+  std::memcpy(&u, &x, 4);
+  u += (uint32_t)d;
+  std::memcpy(&x, &u, 4);
+  return x;
+}
+struct PerturbedRoots {
+  static float sqrt(float x) { return perturb(std::sqrt(x)); }
+  static float rcp(float x) { return perturb(1.0f / x); }
+};
+}  // namespace
+
+int cpsz_tri_bound_2d_fast_host(const float* uv6, size_t n, int ulps, float* entry) {
+  return guarded([&] {
+    CPSZ_REQUIRE(uv6 && entry && ulps >= 0 && ulps < 64, kErrArg, "bad argument");
+    g_ulps = ulps;
+    g_perturb_n = 0;
+    for (size_t i = 0; i < n; i++) {
+      const float* t = uv6 + 6 * i;
+      entry[i] = tri_fast<PerturbedRoots>(t[0], t[1], t[2], t[3], t[4], t[5], false);
+    }
+    g_ulps = 0;
+  });
+}
+
+int cpsz_ratio_ge1_host(const double* det, const double* d, size_t n, unsigned char* out) {
+  return guarded([&] {
+    CPSZ_REQUIRE(det && d && out, kErrArg, "null argument");
+    for (size_t i = 0; i < n; i++) out[i] = ratio_ge1(det[i], d[i]) ? 1 : 0;
   });
 }
 

@@ -34,6 +34,7 @@
 #include "huff_gpu.hpp"
 #include "libm_math.hpp"
 #include "lorenzo2d_step.hpp"
+#include "bounds2d_fast.hpp"
 
 namespace cpsz {
 
@@ -151,93 +152,12 @@ __device__ double dtri(T u0, T u1, T u2, T v0, T v1, T v2, int lower) {
   return eb;
 }
 
-// ---- the same bound for float fields with the expensive steps approximated --
-// The reference's bound is a function of the six float products: it sums
-// them in double (det, the p/n sums of the sign bounds), divides det by three
-// of those sums, and takes |sqrt p - sqrt n| / (sqrt p + sqrt n) with
-// correctly rounded double square roots and divisions. Here the sums and the
-// branch decisions stay exact, and only the last step is approximated, in
-// FP32, as t ~ |p - n| / (p + n + 2 sqrt(p n)) from the exact p, n (relative
-// error below 2^-20). The reference's rounded value lies within 5 * 2^-53 of
-// the real t (absolute; t <= 1), so the true bound is enclosed in
-// [a (1 - 2^-16) - 2^-50, a (1 + 2^-16) + 2^-50] for an approximation a; a
-// vertex whose eb codes that enclosure does not fix takes the exact bound.
-//
-// A triangle's entry in the cell table (FP32):
-//   0            the exact bound 0
-//   kExact1      the exact bound 1
-//   +inf         no triangle (cell outside the field)
-//   kNeedExact   only the FP64 bound will do (critical-point triangles,
-//                products outside [2^-60, 2^58])
-//   otherwise    an approximation a in (2^-56, 1 + 2^-19]
-constexpr float kExact1 = 2.0f;
-constexpr float kNeedExact = -1.0f;
-
-// products in [2^-60, 2^58] (or 0) keep every FP32 step below in the normal range
-__device__ __forceinline__ bool prod_ok(float f) {
-  const uint32_t e = __float_as_uint(f) & 0x7FFFFFFFu;
-  return e == 0 || (e >= 0x21800000u && e <= 0x5C800000u);
-}
-// a vertex value in [2^-30, 2^29] (or 0): products of two such values pass prod_ok
-__device__ __forceinline__ bool value_ok(float f) {
-  const uint32_t e = __float_as_uint(f) & 0x7FFFFFFFu;
-  return e == 0 || (e >= 0x30800000u && e <= 0x4E000000u);
-}
-
-// |sqrt p - sqrt n| / (sqrt p + sqrt n) = |p - n| / (p + n + 2 sqrt(p n)); num = |p - n|
-__device__ __forceinline__ float approx_sign_bound(float p, float n, float num) {
-  const float s = __builtin_amdgcn_sqrtf(p * n);
-  return num * __builtin_amdgcn_rcpf((p + n) + 2.0f * s);
-}
-
-// max_eb_to_keep_sign_2d_offline_2(a, b): 1 unless the terms have opposite
-// signs, then p, n are the terms' magnitudes and p - n = a + b
-__device__ __forceinline__ float fbound2(float a, float b) {
-  const bool mixed = (a > 0 && b < 0) || (a < 0 && b > 0);
-  const float r = approx_sign_bound(fabsf(a), fabsf(b), fabsf(a + b));
-  return mixed ? r : kExact1;
-}
-
-// max_eb_to_keep_sign_2d_offline_4 from its p and n, summed in the reference's
-// order by the caller (a zero part adds nothing to a non-negative sum)
-__device__ __forceinline__ float fbound4(double p, double n) {
-  const float r = approx_sign_bound((float)p, (float)n, (float)fabs(p - n));
-  return (p == 0 || n == 0) ? kExact1 : r;
-}
-
-// (det / d >= 1) in double: the rounded quotient reaches 1 exactly when
-// det / d >= 1 - 2^-54, and det - d (exact in sign) tells which side of 1 the
-// quotient is on; only quotients within 2^-50 below 1 (or d == 0) divide
-__device__ __forceinline__ bool ratio_ge1(double det, double d) {
-  const double t = det - d;
-  const bool ge = t == 0 || ((t > 0) == (d > 0));
-  if (d == 0 || (!ge && fabs(t) <= 0x1p-50 * fabs(d))) return det / d >= 1;
-  return ge;
-}
-
-// The cell table entry of one triangle (vertex order as dtri's). checked: the
-// tile's values all pass value_ok, so the products need no check.
-__device__ __forceinline__ float tri_fast(float u0, float u1, float u2, float v0, float v1, float v2, bool checked) {
-  const float f01 = u0 * v1, f10 = u1 * v0, f02 = u0 * v2, f20 = u2 * v0, f12 = u1 * v2, f21 = u2 * v1;
-  const double u0v1 = f01, u1v0 = f10, u0v2 = f02, u2v0 = f20, u1v2 = f12, u2v1 = f21;
-  const double det = u0v1 - u1v0 + u1v2 - u2v1 + u2v0 - u0v2;
-  if (!(det != 0)) return 0.0f;
-  if (!checked && !(prod_ok(f01) && prod_ok(f10) && prod_ok(f02) && prod_ok(f20) && prod_ok(f12) && prod_ok(f21)))
-    return kNeedExact;
-  const bool f1 = ratio_ge1(det, u2v0 - u0v2);
-  const bool f2 = ratio_ge1(det, u1v2 - u2v1);
-  const bool f3 = ratio_ge1(det, u0v1 - u1v0);
-  if (f1 && f2 && f3) return kNeedExact;
-  // positive and negative parts of the signed terms u0v1, -u1v0 | u1v2, -u2v1 | u2v0, -u0v2
-  const double p0 = fmax(u0v1, 0.0), n0 = fmax(-u0v1, 0.0), p1 = fmax(-u1v0, 0.0), n1 = fmax(u1v0, 0.0);
-  const double p2 = fmax(u1v2, 0.0), n2 = fmax(-u1v2, 0.0), p3 = fmax(-u2v1, 0.0), n3 = fmax(u2v1, 0.0);
-  const double p4 = fmax(u2v0, 0.0), n4 = fmax(-u2v0, 0.0), p5 = fmax(-u0v2, 0.0), n5 = fmax(u0v2, 0.0);
-  const double pa = p0 + p1, na = n0 + n1, pb = p2 + p3, nb = n2 + n3;
-  const float m1 = fminf(fbound2(f20, -f02), fbound4((pa + p2) + p3, (na + n2) + n3));
-  const float m2 = fminf(fbound2(f12, -f21), fbound4((pa + p4) + p5, (na + n4) + n5));
-  const float m3 = fminf(fbound2(f01, -f10), fbound4((pb + p4) + p5, (nb + n4) + n5));
-  return fmaxf(fmaxf(f1 ? 0.0f : m1, f2 ? 0.0f : m2), f3 ? 0.0f : m3);
-}
+// The FP32-approximated triangle bound (tri_fast) and its constants are in
+// bounds2d_fast.hpp, shared with the host check.
+struct DeviceRoots {
+  __device__ static float sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+  __device__ static float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+};
 
 // eb code: 0 = zero precision; c >= 1: reference id c-1 with precision
 // 4^(c-1) * DBL_EPSILON (id 0 keeps precision eps in the compressor).
@@ -428,7 +348,8 @@ __device__ __forceinline__ void eb2d_tile_fast(const float* __restrict__ U, cons
     float t = INFINITY;
     if (ci >= 0 && ci < r1 - 1 && cj >= 0 && cj < r2 - 1) {
       const int mr = lower ? cr : cr + 1, mc = lower ? cc + 1 : cc;
-      t = tri_fast(su[cr][cc], su[mr][mc], su[cr + 1][cc + 1], sv[cr][cc], sv[mr][mc], sv[cr + 1][cc + 1], checked);
+      t = tri_fast<DeviceRoots>(su[cr][cc], su[mr][mc], su[cr + 1][cc + 1], sv[cr][cc], sv[mr][mc], sv[cr + 1][cc + 1],
+                                   checked);
     }
     tb[cr][cc][lower] = t;
   }
@@ -448,8 +369,8 @@ __device__ __forceinline__ void eb2d_tile_fast(const float* __restrict__ U, cons
     if (m > 0 && m < kExact1) {
       // the smallest bound is an approximation: exact 1s and missing cells do not matter
       const double am = m;
-      const double lo = dminf(max_pwr_eb, am * (1 - 0x1p-16) - 0x1p-50);
-      const double hi = dminf(max_pwr_eb, am * (1 + 0x1p-16) + 0x1p-50);
+      const double lo = dminf(max_pwr_eb, enclosure_lo(am));
+      const double hi = dminf(max_pwr_eb, enclosure_hi(am));
       const double uh = (double)fabsf(u) * hi, vh = (double)fabsf(v) * hi;
       a = eb_code_2d(uh, q2d, nq);
       b = eb_code_2d(vh, q2d, nq);

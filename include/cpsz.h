@@ -424,6 +424,13 @@ int cpsz_check_fast_bounds(int device, uint32_t* max_ulp, double* max_ratio);
  * the last FP64 steps with exact fix-ups, 1 = every step in FP64. Both give
  * identical streams. */
 int cpsz_set_exact_2d_bounds(int on);
+/* Host evaluations of the approximate 2D bound's pieces (no reference
+ * counterpart; for tests, no GPU needed): the cell-table entry of each
+ * triangle uv6[6i..6i+5] = (u0,u1,u2,v0,v1,v2) with square roots and
+ * reciprocals moved by up to `ulps` units in the last place, and the
+ * division-free test of (det / d >= 1). */
+int cpsz_tri_bound_2d_fast_host(const float* uv6, size_t n, int ulps, float* entry);
+int cpsz_ratio_ge1_host(const double* det, const double* d, size_t n, unsigned char* out);
 /* Host evaluation of the exact tetrahedron predicate the device judge uses:
  * f = 4 vertex vectors (u, v, w) of tetrahedron t (0..5, the judge's order);
  * *type = -1 (no critical point) or the type code. */
