@@ -21,6 +21,7 @@ from __future__ import annotations
 
 import argparse
 import ctypes as C
+import gc
 import json
 import os
 import sys
@@ -287,19 +288,27 @@ def run_batch2d(a, torch):
         st_d = cz.last_stages(0)
         return base, off, size, st_c, st_d
 
+    per_step = []  # ms of each timed step (events between steps)
+
     def timed(fn, steps, warmup):
         for _ in range(max(warmup, 1)):
             r = fn()
         torch.cuda.synchronize()
-        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
         acc = []
-        ev0.record(stream)
-        for _ in range(steps):
+        # as timeit does: no garbage-collector pass inside the timed region (a
+        # full pass stalls the host thread that drives the device for tens of ms)
+        gc.collect()
+        gc.disable()
+        ev[0].record(stream)
+        for k in range(steps):
             r = fn()
             acc.append(r)
-        ev1.record(stream)
+            ev[k + 1].record(stream)
         torch.cuda.synchronize()
-        return acc, ev0.elapsed_time(ev1) / 1e3 / steps
+        gc.enable()
+        per_step.extend(ev[k].elapsed_time(ev[k + 1]) for k in range(steps))
+        return acc, ev[0].elapsed_time(ev[steps]) / 1e3 / steps
 
     def zbytes(base, off, size):
         # one ZSTD level-3 frame per field (the reference's sz_lossless_compress
@@ -322,6 +331,10 @@ def run_batch2d(a, torch):
     t_wall = time.perf_counter()
     acc, t_step = timed(step, a.steps, a.warmup)
     t_wall = time.perf_counter() - t_wall
+    plain_steps = [round(x, 2) for x in per_step]
+    slow = int(np.argmax(per_step))  # the slowest step's stages, to see where a stall sits
+    slowest = {"step": slow, "compress": {k: round(v, 2) for k, v in acc[slow][3]},
+               "decompress": {k: round(v, 2) for k, v in acc[slow][4]}}
     stages_c, stages_d = {}, {}
     for _, _, _, st_c, st_d in acc:
         for k, v in st_c:
@@ -390,6 +403,7 @@ def run_batch2d(a, torch):
                                "rel eb 0.1 (220,143,645 vertices, 1.76 GB)", "method": "plain", "fields": 96,
                    "vertices": nv, "field_bytes": field_bytes, "stream_bytes": sb_plain, "zstd_bytes": zb_plain,
                    "ratio": field_bytes / zb_plain, "cp_errors": cp_plain, "wall_s": t_wall,
+                   "ms_each_step": plain_steps, "slowest_step_stages_ms": slowest,
                    "autotuned": {"workers": w, "grid_cap": g, "huffman_trees": "device" if dt else "host"},
                    "compress_ms_by_tree_placement": graph,
                    "stages_compress_ms": stages_c, "stages_decompress_ms": stages_d,
