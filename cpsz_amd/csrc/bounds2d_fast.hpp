@@ -65,8 +65,9 @@ __host__ __device__ inline float fbound2(float a, float b) {
   return mixed ? r : kExact1;
 }
 
-// max_eb_to_keep_sign_2d_offline_4 from its p and n, summed in the reference's
-// order by the caller (a zero part adds nothing to a non-negative sum)
+// max_eb_to_keep_sign_2d_offline_4 from its p and n (or both doubled), summed
+// in the reference's order by the caller (a zero part adds nothing to a
+// non-negative sum); products stay below 2^58, so doubled sums below 2^61
 template <class R>
 __host__ __device__ inline float fbound4(double p, double n) {
   const float r = approx_sign_bound<R>((float)p, (float)n, (float)fabs(p - n));
@@ -97,10 +98,12 @@ __host__ __device__ inline float tri_fast(float u0, float u1, float u2, float v0
   const bool f2 = ratio_ge1(det, u1v2 - u2v1);
   const bool f3 = ratio_ge1(det, u0v1 - u1v0);
   if (f1 && f2 && f3) return kNeedExact;
-  // positive and negative parts of the signed terms u0v1, -u1v0 | u1v2, -u2v1 | u2v0, -u0v2
-  const double p0 = fmax(u0v1, 0.0), n0 = fmax(-u0v1, 0.0), p1 = fmax(-u1v0, 0.0), n1 = fmax(u1v0, 0.0);
-  const double p2 = fmax(u1v2, 0.0), n2 = fmax(-u1v2, 0.0), p3 = fmax(-u2v1, 0.0), n3 = fmax(u2v1, 0.0);
-  const double p4 = fmax(u2v0, 0.0), n4 = fmax(-u2v0, 0.0), p5 = fmax(-u0v2, 0.0), n5 = fmax(u0v2, 0.0);
+  // twice the positive and negative parts of the signed terms u0v1, -u1v0 |
+  // u1v2, -u2v1 | u2v0, -u0v2: t + |t| and |t| - t are exact, and sums of
+  // doubled terms are the doubled sums bit for bit; fbound4 is scale-free
+  const double p0 = u0v1 + fabs(u0v1), n0 = fabs(u0v1) - u0v1, p1 = fabs(u1v0) - u1v0, n1 = u1v0 + fabs(u1v0);
+  const double p2 = u1v2 + fabs(u1v2), n2 = fabs(u1v2) - u1v2, p3 = fabs(u2v1) - u2v1, n3 = u2v1 + fabs(u2v1);
+  const double p4 = u2v0 + fabs(u2v0), n4 = fabs(u2v0) - u2v0, p5 = fabs(u0v2) - u0v2, n5 = u0v2 + fabs(u0v2);
   const double pa = p0 + p1, na = n0 + n1, pb = p2 + p3, nb = n2 + n3;
   const float m1 = fminf(fbound2<R>(f20, -f02), fbound4<R>((pa + p2) + p3, (na + n2) + n3));
   const float m2 = fminf(fbound2<R>(f12, -f21), fbound4<R>((pa + p4) + p5, (na + n4) + n5));
