@@ -1235,14 +1235,25 @@ __global__ void __launch_bounds__(256) k_finish2d(float* __restrict__ U, float* 
 }
 
 struct U16Hist {
-  const uint16_t* p;
+  const uint16_t* p;  // 8-byte aligned (the code arrays are 256-byte aligned)
   size_t n;
   static constexpr int kChunk = 1024;
-  static constexpr int kMaxItems = kChunk, kPer = 1;
+  // four codes per item, one 8-byte load
+  static constexpr int kMaxItems = kChunk / 4, kPer = 4;
   __device__ int get(size_t c, int idx, uint32_t* sym) const {
-    const size_t i = c * kChunk + idx;
-    sym[0] = p[i < n ? i : n - 1];
-    return i < n;
+    const size_t i = c * kChunk + 4 * (size_t)idx;
+    if (i + 4 <= n) {
+      const uint2 w = *reinterpret_cast<const uint2*>(p + i);
+      sym[0] = w.x & 0xFFFFu;
+      sym[1] = w.x >> 16;
+      sym[2] = w.y & 0xFFFFu;
+      sym[3] = w.y >> 16;
+      return 4;
+    }
+    int k = 0;
+    for (; k < 4 && i + k < n; k++) sym[k] = p[i + k];
+    for (int q = k; q < 4; q++) sym[q] = 0;
+    return k;
   }
 };
 
@@ -2184,7 +2195,8 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
       h.src = U16Hist{c ? W.qv : W.qu, x.n};
       h.nchunks = nch;
       h.hist = W.hist + kMaxEbCode + c * 65536;
-      h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 8), 512);
+      // 32 chunks per block: a block clears and flushes its 16384-bin window once per 32768 codes
+      h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 32), 512);
       h.blk0 = hblk;
       hblk += h.nblk;
     }
