@@ -1072,9 +1072,13 @@ struct SinkCodes2d {
   // optional: zero-filled per-block counts of code 0 (the unpredictable
   // marker), added up while the codes are written
   uint32_t* zc = nullptr;
+  // optional: zero-filled per-block masks of the code-0 positions (three
+  // words per block, bit k for the block's k-th element in row-major order)
+  uint64_t* zm = nullptr;
   __device__ void put(uint64_t p, uint32_t s) const {
     const uint64_t rowblk = (uint64_t)kBlock2d * r2;
     int bi, bj, ii, jj;
+    uint32_t k;  // position within the block
     if (p < (1ull << 32) && rowblk < (1ull << 32)) {  // 32-bit divisions (a 64-bit one is a long sequence)
       const uint32_t p32 = (uint32_t)p, rb = (uint32_t)rowblk;
       const uint32_t b = p32 / rb, rem = p32 - b * rb;
@@ -1085,6 +1089,7 @@ struct SinkCodes2d {
       const uint32_t sy = (uint32_t)min(kBlock2d, r2 - bj * kBlock2d);
       ii = (int)(rem2 / sy);
       jj = (int)(rem2 - (uint32_t)ii * sy);
+      k = rem2;
     } else {
       bi = (int)(p / rowblk);
       const uint64_t rem = p - (uint64_t)bi * rowblk;
@@ -1095,8 +1100,11 @@ struct SinkCodes2d {
       const int sy = min(kBlock2d, r2 - bj * kBlock2d);
       ii = rem2 / sy;
       jj = rem2 - ii * sy;
+      k = rem2;
     }
     q[(size_t)(bi * kBlock2d + ii) * r2 + bj * kBlock2d + jj] = (uint16_t)s;
+    if (zm && s == 0)
+      atomicOr((unsigned long long*)&zm[3 * ((size_t)bi * nby + bj) + (k >> 6)], 1ull << (k & 63));
     if (zc) {
       // one atomic per block among the active lanes' zeros: consecutive
       // positions (the staged writes) fall in one or two blocks
@@ -1271,6 +1279,7 @@ struct UnpJob {
   uint64_t count;        // scatter: values in the stream
   int* err;              // scatter: set when the codes ask for more values than the stream holds
   uint32_t blk0;
+  const uint64_t* zm;    // scatter: per-block code-0 masks (SinkCodes2d::zm), else the codes are read
 };
 
 __global__ void __launch_bounds__(256) k_block_zeros_multi(const UnpJob* __restrict__ jobs, int nj) {
@@ -1299,7 +1308,7 @@ __global__ void __launch_bounds__(256) k_scatter_unpred_multi(const UnpJob* __re
   for (int k0 = 0; k0 < v.m; k0 += 64) {
     const int k = k0 + lane;
     const size_t e = k < v.m ? v.elem(k, J.r2) : 0;
-    const bool zero = k < v.m && J.q[e] == 0;
+    const bool zero = J.zm ? ((J.zm[3 * b + (k0 >> 6)] >> lane) & 1) != 0 : (k < v.m && J.q[e] == 0);
     const uint64_t mask = __ballot(zero);
     if (zero) {
       const uint64_t r = o + __popcll(mask & below);
@@ -2696,7 +2705,7 @@ void d2_entropy(Ctx& ctx, const uint8_t* d_in, const uint8_t* h_in, size_t in_si
 // counts of every component, one global scan, one scatter launch.
 // zeros_counted: the code decoders already filled b2u.zc (SinkCodes2d::zc)
 void d2_unpred_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const std::vector<D2Plan>& plan,
-                     D2Work* const* w, size_t nf, bool zeros_counted) {
+                     D2Work* const* w, size_t nf, bool zeros_counted, const uint64_t* zmask) {
   hipStream_t st = ctx.stream();
   size_t Z = 0, V = 0, npieces = 0;
   constexpr size_t kPiece = 65536;
@@ -2745,6 +2754,7 @@ void d2_unpred_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const st
       u.count = p.comp[c].unp_count;
       u.err = W.err;
       u.blk0 = blk;
+      u.zm = zmask ? zmask + 3 * (zb[k] + c * p.nb) : nullptr;
       blk += (uint32_t)unp_grid(p.nb);
       const size_t len = 4 * p.comp[c].unp_count;
       for (size_t o = 0; o < len; o += kPiece)
@@ -2911,6 +2921,10 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
   for (size_t k = 0; k < nf; k++) Z += 2 * plan[k].nb;
   uint32_t* zc = ctx.buf<uint32_t>("b2u.zc", Z + 1);
   k_fill_u32<<<(unsigned)std::min<size_t>(div_up(Z, 256), 2048), 256, 0, st>>>(zc, Z, 0);
+  // and record where the zeros are, so that the scatter reads 24 bytes of masks per
+  // block instead of the block's codes
+  uint64_t* zm = ctx.buf<uint64_t>("b2u.zm", 3 * Z + 1);
+  k_fill_u32<<<(unsigned)std::min<size_t>(div_up(6 * Z, 256), 2048), 256, 0, st>>>((uint32_t*)zm, 6 * Z, 0);
   CPSZ_HIP(hipGetLastError());
   size_t zb = 0;
   bool ids8 = true;
@@ -2927,14 +2941,15 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
     uint16_t* qs[2] = {w[k]->qu, w[k]->qv};
     for (int c = 0; c < 2; c++)
       jc[2 * k + c] = {&dd[3 * k + 1 + c], bc[3 * k + 1 + c].dst, p.comp[c].h.nbytes, n,
-                       SinkCodes2d{qs[c], (int)r1, (int)r2, p.nby, zc + zb + c * p.nb}, "b2d"};
+                       SinkCodes2d{qs[c], (int)r1, (int)r2, p.nby, zc + zb + c * p.nb, zm + 3 * (zb + c * p.nb)},
+                       "b2d"};
     zb += 2 * p.nb;
   }
   if (ids8) decode_streams_multi(ctx, ji8.data(), (int)nf, "b2d.ids");
   else decode_streams_multi(ctx, ji.data(), (int)nf, "b2d.ids");
   decode_streams_multi(ctx, jc.data(), (int)(2 * nf), "b2d.codes");
   ctx.mark("huffman");
-  d2_unpred_batch(ctx, d_in, plan, w, nf, true);
+  d2_unpred_batch(ctx, d_in, plan, w, nf, true, zm);
   ctx.mark("unpred");
 }
 
