@@ -1784,7 +1784,9 @@ void launch_eb_fast(Ctx& ctx, const Eb2dJob* dj, int nj, uint32_t blk, double ma
   uint32_t* fx = ctx.buf<uint32_t>("eb.fix", (size_t)blk + 1);  // [0] count, [1..] tiles
   k_fill_u32<<<1, 64, 0, st>>>(fx, 1, 0);
   k_eb2d_multi<true><<<blk, 256, 0, st>>>(dj, nj, max_pwr_eb, ctx.d_q2d(), ctx.q2d_size(), fx + 1, fx);
-  k_eb2d_fix<<<std::min<uint32_t>(blk, 1024), 256, 0, st>>>(dj, nj, fx + 1, fx, max_pwr_eb, ctx.d_q2d(),
+  // a listed tile costs a few dependent global loads per marked vertex: enough
+  // blocks to keep every CU's slots filled (70 VGPRs: 7 blocks per CU)
+  k_eb2d_fix<<<std::min<uint32_t>(blk, 2048), 256, 0, st>>>(dj, nj, fx + 1, fx, max_pwr_eb, ctx.d_q2d(),
                                                             ctx.q2d_size());
 }
 }  // namespace
