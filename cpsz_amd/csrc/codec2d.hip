@@ -978,16 +978,31 @@ __global__ void k_zero_i32(int* p, int n) {
 // m ids: the u codes then the v codes (offline, m = 2n), or the one code per
 // vertex of offline_log (m = n)
 struct SrcIds2d {
-  const uint16_t* cu;
+  const uint16_t* cu;  // 256-byte aligned
   const uint16_t* cv;
   size_t n, m;
   static constexpr int kChunk = 512;
-  static constexpr int kMaxItems = kChunk, kPer = 1;
+  // four consecutive ids per item: one 8-byte load where the four lie in one
+  // array at an 8-byte boundary, element loads otherwise
+  static constexpr int kMaxItems = kChunk / 4, kPer = 4;
   __device__ int get(size_t c, int idx, uint32_t* sym) const {
-    const size_t p0 = c * kChunk + idx, p = p0 < m ? p0 : m - 1;
-    const uint32_t x = p < n ? cu[p] : cv[p - n];
-    sym[0] = x ? x - 1 : 0;
-    return p0 < m;
+    const size_t p0 = c * kChunk + 4 * (size_t)idx;
+    uint32_t x[4] = {0, 0, 0, 0};
+    int k = 0;
+    const uint16_t* src = p0 + 4 <= n ? cu + p0 : (p0 >= n && p0 + 4 <= m ? cv + (p0 - n) : nullptr);
+    if (src && ((uintptr_t)src & 7) == 0) {
+      const uint2 w = *reinterpret_cast<const uint2*>(src);
+      x[0] = w.x & 0xFFFFu;
+      x[1] = w.x >> 16;
+      x[2] = w.y & 0xFFFFu;
+      x[3] = w.y >> 16;
+      k = 4;
+    } else {
+      for (; k < 4 && p0 + k < m; k++) x[k] = p0 + k < n ? cu[p0 + k] : cv[p0 + k - n];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) sym[q] = x[q] ? x[q] - 1 : 0;
+    return k;
   }
 };
 
