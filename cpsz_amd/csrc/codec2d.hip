@@ -2221,8 +2221,8 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
       h.src = U16Hist{c ? W.qv : W.qu, x.n};
       h.nchunks = nch;
       h.hist = W.hist + kMaxEbCode + c * 65536;
-      // 32 chunks per block: a block clears and flushes its 16384-bin window once per 32768 codes
-      h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 32), 512);
+      // 128 chunks per block: a block clears and flushes its 16384-bin window once per 131072 codes
+      h.nblk = (uint32_t)std::min<size_t>(div_up(nch, 128), 512);
       h.blk0 = hblk;
       hblk += h.nblk;
     }
