@@ -2292,7 +2292,7 @@ void c2_entropy_batch(Ctx& ctx, C2Work* const* w, size_t nf, uint8_t* dout, cons
   }
   ctx.upload_table("b2e.tab", A.d, A.h, A.used);
   // ---- histograms, present symbols, trees
-  const uint32_t win_lo = 32768 - 8192, win_n = 16384;
+  const uint32_t win_lo = 32768 - 4096, win_n = 8192;
   k_hist_multi<U16Hist><<<hblk, 256, win_n * 4, st>>>(d_hj, (int)(2 * nf), win_lo, win_n);
   CPSZ_HIP(hipGetLastError());
   k_zero_i32<<<1, 64, 0, st>>>((int*)meta, 16);  // total, words used, error flag
