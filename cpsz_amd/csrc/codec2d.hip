@@ -2918,7 +2918,7 @@ void d2_entropy_batch(Ctx& ctx, const std::vector<const uint8_t*>& d_in, const s
   std::vector<const HostDecoder*> hp(3 * nf);
   for (size_t k = 0; k < 3 * nf; k++) hp[k] = &hd[k];
   std::vector<DeviceDecoder> dd(3 * nf);
-  upload_decoders(ctx, hp.data(), (int)(3 * nf), dd.data(), "b2d.dec");
+  upload_decoders(ctx, hp.data(), (int)(3 * nf), dd.data(), "b2d.dec", &pool_for(ctx.device()), workers);
   ctx.mark("upload");
   std::vector<BitsCopy> bc(3 * nf);
   for (size_t k = 0; k < nf; k++) {

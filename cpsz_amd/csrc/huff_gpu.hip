@@ -1,6 +1,8 @@
 #include "huff_gpu.hpp"
+#include "pool.hpp"
 
 #include <cstring>
+#include <functional>
 
 namespace cpsz {
 
@@ -619,7 +621,8 @@ __global__ void k_stage_bits_multi(const BitsCopy* __restrict__ c, const uint32_
 }
 }  // namespace
 
-void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecoder* out, const char* tag) {
+void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecoder* out, const char* tag,
+                     WorkerPool* pool, int workers) {
   hipStream_t st = ctx.stream();
   auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
   std::vector<size_t> base(n + 1, 0), l2base(n + 1, 0);
@@ -635,6 +638,30 @@ void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecode
   uint8_t* dbuf = ctx.buf<uint8_t>((t + ".d").c_str(), base[n] + nfill);
   uint32_t* dl2 = ctx.buf<uint32_t>((t + ".lut2").c_str(), l2base[n] / 4 + 1);
   Lut2Fill* hf = (Lut2Fill*)(hbuf + base[n]);
+  // a decoder's tables into its slot of the pinned staging buffer
+  auto copy_one = [&](int k) {
+    const HostDecoder& h = *hd[k];
+    if (h.d.root_leaf) return;
+    const size_t N = h.d.node_count, lut_b = h.lut.size() * 4, pfx_b = h.pfx.size() * 4, arr_b = N * 4;
+    uint8_t* q = hbuf + base[k];
+    std::memcpy(q, h.lut.data(), lut_b);
+    if (pfx_b) std::memcpy(q + lut_b, h.pfx.data(), pfx_b);
+    uint8_t* a = q + lut_b + pfx_b;
+    std::memcpy(a, h.d.left.data(), arr_b);
+    std::memcpy(a + arr_b, h.d.right.data(), arr_b);
+    std::memcpy(a + 2 * arr_b, h.d.sym.data(), arr_b);
+    std::memcpy(a + 3 * arr_b, h.d.leaf.data(), N);
+  };
+  if (pool) {
+    // the copies of many decoders (megabytes) split over the host workers
+    const int parts = std::max(1, std::min(workers, n));
+    std::vector<std::function<void(Ctx&)>> jobs(parts);
+    for (int j = 0; j < parts; j++)
+      jobs[j] = [&, j](Ctx&) {
+        for (int k = j; k < n; k += parts) copy_one(k);
+      };
+    pool_run(*pool, jobs, parts);
+  }
   int nf = 0;
   uint64_t total2 = 0;
   for (int k = 0; k < n; k++) {
@@ -647,14 +674,7 @@ void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecode
       continue;
     }
     const size_t N = h.d.node_count, lut_b = h.lut.size() * 4, pfx_b = h.pfx.size() * 4, arr_b = N * 4;
-    uint8_t* q = hbuf + base[k];
-    std::memcpy(q, h.lut.data(), lut_b);
-    if (pfx_b) std::memcpy(q + lut_b, h.pfx.data(), pfx_b);
-    uint8_t* a = q + lut_b + pfx_b;
-    std::memcpy(a, h.d.left.data(), arr_b);
-    std::memcpy(a + arr_b, h.d.right.data(), arr_b);
-    std::memcpy(a + 2 * arr_b, h.d.sym.data(), arr_b);
-    std::memcpy(a + 3 * arr_b, h.d.leaf.data(), N);
+    if (!pool) copy_one(k);
     const uint8_t* dq = dbuf + base[k];
     const uint8_t* da = dq + lut_b + pfx_b;
     o.tree.lut = (const uint32_t*)dq;

@@ -704,7 +704,11 @@ DeviceDecoder upload_decoder(Ctx& ctx, const uint8_t* tree, size_t tree_size, si
 
 // Many decoders in one packed copy and one level-2 fill launch (out[k] for
 // hd[k]); `tag` names the buffers.
-void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecoder* out, const char* tag);
+// pool (optional): host workers that share the table copies (the caller must
+// not itself be running on that pool)
+class WorkerPool;
+void upload_decoders(Ctx& ctx, const HostDecoder* const* hd, int n, DeviceDecoder* out, const char* tag,
+                     WorkerPool* pool = nullptr, int workers = 0);
 
 // Bitstreams at arbitrary byte offsets copied into aligned word arrays with
 // zero slack (the decoder's input form), all in one launch. Fills dst/nwords.
